@@ -1,0 +1,821 @@
+// ilh_api.hip -- the C ABI of libimmersedhip.so (include/immersedhip.h):
+// context and plan lifetime, host/device staging, the per-op entry points and
+// the Dirichlet / Neumann solve composites.
+#include "ilh_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+using namespace ilh;
+
+// --------------------------------------------------------------- errors
+static thread_local std::string g_err;
+
+namespace ilh {
+void set_error(const std::string& msg) { g_err = msg; }
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+void check_hip(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Error{ILH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+void DevBuf::ensure(size_t b)
+{
+    if (b <= bytes && p) return;
+    release();
+    if (b == 0) b = 8;
+    if (hipMalloc(&p, b) != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        throw Error{ILH_ERR_OOM, "device allocation of " + std::to_string(b) + " bytes failed"};
+    }
+    bytes = b;
+}
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+}  // namespace ilh
+
+template <class F>
+static int guarded(F&& f)
+{
+    try {
+        f();
+        return ILH_OK;
+    } catch (const Error& e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return ILH_ERR_OOM;
+    } catch (...) {
+        g_err = "unknown error";
+        return ILH_ERR_ARG;
+    }
+}
+
+extern "C" const char* ilh_last_error(void) { return g_err.c_str(); }
+extern "C" const char* ilh_version(void) { return "immersedhip 0.1 (gfx950, FP64)"; }
+
+// -------------------------------------------------------- solve caches
+struct SolveCache {
+    DevBuf LU, piv, info;
+    DevBuf M[4];              // row-major N x N solve matrices
+    DevBuf terms1, terms2;    // merged source terms
+    SourceSet src1, src2;
+    DevBuf amp1, amp2;
+    ~SolveCache()
+    {
+        free_bins(src1.bins);
+        free_bins(src2.bins);
+        if (src1.pairs) (void)hipFree(src1.pairs);
+        if (src2.pairs) (void)hipFree(src2.pairs);
+    }
+};
+
+static void drop_caches(ilh_plan* pl)
+{
+    delete pl->dir;
+    delete pl->neu;
+    pl->dir = pl->neu = nullptr;
+    for (auto& f : pl->fac) {
+        delete f;
+        f = nullptr;
+    }
+    for (int op = 0; op < OP_COUNT; ++op) {
+        pl->terms_ok[op] = false;
+        if (pl->bins_ok[op]) {
+            free_bins(pl->bins[op][0]);
+            free_bins(pl->bins[op][1]);
+            pl->bins_ok[op] = false;
+        }
+    }
+}
+
+// ------------------------------------------------------------- context
+extern "C" int ilh_ctx_create(int device, ilh_ctx** out)
+{
+    return guarded([&] {
+        if (!out) fail(ILH_ERR_ARG, "null output pointer");
+        int n = 0;
+        ILH_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail(ILH_ERR_ARG, "no such HIP device");
+        ILH_HIP(hipSetDevice(device));
+        ilh_ctx* c = new ilh_ctx();
+        c->device = device;
+        ILH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        ILH_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount;
+        *out = c;
+    });
+}
+
+extern "C" int ilh_ctx_destroy(ilh_ctx* c)
+{
+    return guarded([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        c->twiddle.release();
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+
+extern "C" int ilh_ctx_synchronize(ilh_ctx* c)
+{
+    return guarded([&] {
+        if (!c) fail(ILH_ERR_ARG, "null ctx");
+        ILH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int ilh_malloc(ilh_ctx* c, size_t bytes, void** p)
+{
+    return guarded([&] {
+        if (!c || !p) fail(ILH_ERR_ARG, "null argument");
+        ILH_HIP(hipSetDevice(c->device));
+        if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(ILH_ERR_OOM, "hipMalloc failed");
+        }
+    });
+}
+
+extern "C" int ilh_free(ilh_ctx* c, void* p)
+{
+    return guarded([&] {
+        if (!c) fail(ILH_ERR_ARG, "null ctx");
+        if (p) ILH_HIP(hipFree(p));
+    });
+}
+
+extern "C" int ilh_memcpy_h2d(ilh_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        ILH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        ILH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int ilh_memcpy_d2h(ilh_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        ILH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        ILH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ---------------------------------------------------------------- plan
+static void load_points(ilh_plan* pl, const double* X, const double* Y, const double* nx, const double* ny,
+                        const double* ds)
+{
+    const int N = pl->N;
+    if (N > 0 && (!X || !Y || !nx || !ny || !ds)) fail(ILH_ERR_ARG, "null geometry array");
+    double mm = 1e300;
+    for (int p = 0; p < N; ++p) {
+        if (!std::isfinite(X[p]) || !std::isfinite(Y[p]) || !std::isfinite(nx[p]) || !std::isfinite(ny[p]) ||
+            !std::isfinite(ds[p]))
+            fail(ILH_ERR_NONFINITE, "non-finite surface geometry");
+        const double xp = X[p] / pl->g.dx - (1.0 - pl->g.I0x), yp = Y[p] / pl->g.dx - (1.0 - pl->g.I0y);
+        mm = std::min({mm, xp, (pl->g.NX - 2) - xp, yp, (pl->g.NY - 2) - yp});
+    }
+    pl->min_margin = N > 0 ? (int)std::floor(mm) : 1 << 20;
+    const size_t b = sizeof(double) * (size_t)std::max(1, N);
+    pl->X.ensure(b);
+    pl->Y.ensure(b);
+    pl->nx.ensure(b);
+    pl->ny.ensure(b);
+    pl->ds.ensure(b);
+    hipStream_t st = pl->ctx->stream;
+    if (N > 0) {
+        ILH_HIP(hipMemcpyAsync(pl->X.p, X, b, hipMemcpyHostToDevice, st));
+        ILH_HIP(hipMemcpyAsync(pl->Y.p, Y, b, hipMemcpyHostToDevice, st));
+        ILH_HIP(hipMemcpyAsync(pl->nx.p, nx, b, hipMemcpyHostToDevice, st));
+        ILH_HIP(hipMemcpyAsync(pl->ny.p, ny, b, hipMemcpyHostToDevice, st));
+        ILH_HIP(hipMemcpyAsync(pl->ds.p, ds, b, hipMemcpyHostToDevice, st));
+    }
+    ILH_HIP(hipStreamSynchronize(st));
+}
+
+extern "C" int ilh_plan_create(ilh_ctx* ctx, const ilh_grid* grid, int npts, const double* X, const double* Y,
+                               const double* nx, const double* ny, const double* ds, int ddf, int scaling,
+                               ilh_plan** out)
+{
+    ilh_plan* pl = nullptr;
+    int rc = guarded([&] {
+        if (!ctx || !grid || !out) fail(ILH_ERR_ARG, "null argument");
+        if (grid->NX < 4 || grid->NY < 4) fail(ILH_ERR_DIM, "grid must have NX, NY >= 4");
+        if (!(grid->dx > 0.0) || !std::isfinite(grid->dx)) fail(ILH_ERR_ARG, "cell size must be positive");
+        if (npts < 0) fail(ILH_ERR_DIM, "negative point count");
+        if (ddf < ILH_DDF_YANG3 || ddf > ILH_DDF_WITCHHAT) fail(ILH_ERR_ARG, "unsupported DDF kind");
+        if (scaling != ILH_SCALING_GRID) fail(ILH_ERR_ARG, "only GridScaling is supported");
+        ILH_HIP(hipSetDevice(ctx->device));
+        pl = new ilh_plan();
+        pl->ctx = ctx;
+        pl->g = Grid{grid->NX, grid->NY, grid->I0x, grid->I0y, grid->dx};
+        pl->N = npts;
+        pl->ddf = ddf;
+        load_points(pl, X, Y, nx, ny, ds);
+        *out = pl;
+    });
+    if (rc != ILH_OK && pl) {
+        drop_caches(pl);
+        delete pl;
+    }
+    return rc;
+}
+
+extern "C" int ilh_plan_update_points(ilh_plan* pl, const double* X, const double* Y, const double* nx,
+                                      const double* ny, const double* ds)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        ILH_HIP(hipSetDevice(pl->ctx->device));
+        ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+        drop_caches(pl);
+        load_points(pl, X, Y, nx, ny, ds);
+    });
+}
+
+extern "C" int ilh_plan_destroy(ilh_plan* pl)
+{
+    return guarded([&] {
+        if (!pl) return;
+        (void)hipSetDevice(pl->ctx->device);
+        (void)hipStreamSynchronize(pl->ctx->stream);
+        drop_caches(pl);
+        delete pl;
+    });
+}
+
+extern "C" int ilh_plan_npts(const ilh_plan* pl) { return pl ? pl->N : -1; }
+
+// ------------------------------------------------------------- staging
+namespace {
+struct Io {
+    ilh_plan* pl;
+    int mem;
+    Io(ilh_plan* p, int m) : pl(p), mem(m)
+    {
+        if (!p) fail(ILH_ERR_ARG, "null plan");
+        if (m != ILH_MEM_HOST && m != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
+        ILH_HIP(hipSetDevice(p->ctx->device));
+    }
+    const double* in(const double* h, size_t n, DevBuf& buf)
+    {
+        if (n > 0 && !h) fail(ILH_ERR_ARG, "null input array");
+        if (mem == ILH_MEM_DEVICE) return h;
+        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
+        if (n) ILH_HIP(hipMemcpyAsync(buf.p, h, sizeof(double) * n, hipMemcpyHostToDevice, pl->ctx->stream));
+        return buf.as<double>();
+    }
+    double* out(double* h, size_t n, DevBuf& buf)
+    {
+        if (n > 0 && !h) fail(ILH_ERR_ARG, "null output array");
+        if (mem == ILH_MEM_DEVICE) return h;
+        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
+        return buf.as<double>();
+    }
+    void back(double* h, const double* d, size_t n)
+    {
+        if (mem == ILH_MEM_DEVICE || n == 0) return;
+        ILH_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, pl->ctx->stream));
+    }
+    void done()
+    {
+        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+    }
+};
+
+size_t grid_len(const Grid& g, int kind)
+{
+    switch (kind) {
+    case ILH_NODES_PRIMAL: return g.n(GK_PRIMAL);
+    case ILH_NODES_DUAL: return g.n(GK_DUAL);
+    case ILH_EDGES_PRIMAL: return g.n_edges();
+    }
+    fail(ILH_ERR_ARG, "unknown grid kind");
+}
+
+void need_margin(ilh_plan* pl, const char* what)
+{
+    if (pl->N > 0 && pl->min_margin < 5)
+        fail(ILH_ERR_RANGE, std::string(what) + ": surface points must lie at least 5 cells inside the grid");
+}
+
+// component grids touched by a scatter op
+int op_components(int op, int* gks)
+{
+    switch (op) {
+    case OP_R_PRIMAL: case OP_DS: gks[0] = GK_PRIMAL; return 1;
+    case OP_R_DUAL: case OP_CST: case OP_CXT: gks[0] = GK_DUAL; return 1;
+    default: gks[0] = GK_EDGE_U; gks[1] = GK_EDGE_V; return 2;
+    }
+}
+
+void scatter_op(ilh_plan* pl, int op, const double* amp, double* out)
+{
+    const Term* t = get_terms(pl, op);
+    int gks[2];
+    const int nc = op_components(op, gks);
+    if (!pl->bins_ok[op]) {
+        for (int c = 0; c < nc; ++c) build_bins(pl, t, pl->N * op_terms_per_point(op), gks[c], pl->bins[op][c]);
+        pl->bins_ok[op] = true;
+    }
+    size_t off = 0;
+    for (int c = 0; c < nc; ++c) {
+        scatter(pl, t, pl->bins[op][c], amp, out + off);
+        off += pl->g.n(gks[c]);
+    }
+}
+
+void gather_op(ilh_plan* pl, int op, const double* field, double* out)
+{
+    const Term* t = get_terms(pl, op);
+    const double* fu = field;
+    const double* fv = field + pl->g.n(GK_EDGE_U);
+    gather(pl, t, op_terms_per_point(op), field, fu, fv, out);
+}
+
+int kind_to_gk(int kind)
+{
+    if (kind == ILH_NODES_PRIMAL) return GK_PRIMAL;
+    if (kind == ILH_NODES_DUAL) return GK_DUAL;
+    fail(ILH_ERR_ARG, "grid kind must be Nodes{Primal} or Nodes{Dual}");
+}
+
+// generic surface -> grid and grid -> surface entry points
+int surf_to_grid(ilh_plan* pl, int op, size_t nin, size_t nout, double* out, const double* f, int mem,
+                 bool derivative, const char* name)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        if (derivative) need_margin(pl, name);
+        const double* d_in = io.in(f, nin, pl->io_a);
+        double* d_out = io.out(out, nout, pl->io_b);
+        if (pl->N == 0) fill_zero(pl->ctx->stream, d_out, nout);
+        else scatter_op(pl, op, d_in, d_out);
+        io.back(out, d_out, nout);
+        io.done();
+    });
+}
+
+int grid_to_surf(ilh_plan* pl, int op, size_t nin, size_t nout, double* out, const double* gfield, int mem,
+                 bool derivative, const char* name)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        if (derivative) need_margin(pl, name);
+        const double* d_in = io.in(gfield, nin, pl->io_a);
+        double* d_out = io.out(out, nout, pl->io_b);
+        if (pl->N > 0) gather_op(pl, op, d_in, d_out);
+        io.back(out, d_out, nout);
+        io.done();
+    });
+}
+}  // namespace
+
+// ------------------------------------------------- surface <-> grid ops
+extern "C" int ilh_regularize(ilh_plan* pl, int kind, double* out, const double* f, int mem)
+{
+    if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+    if (kind == ILH_NODES_PRIMAL)
+        return surf_to_grid(pl, OP_R_PRIMAL, pl->N, pl->g.n(GK_PRIMAL), out, f, mem, false, "regularize!");
+    if (kind == ILH_NODES_DUAL)
+        return surf_to_grid(pl, OP_R_DUAL, pl->N, pl->g.n(GK_DUAL), out, f, mem, false, "regularize!");
+    if (kind == ILH_EDGES_PRIMAL)
+        return surf_to_grid(pl, OP_R_EDGES, 2 * (size_t)pl->N, pl->g.n_edges(), out, f, mem, false, "regularize!");
+    g_err = "unknown grid kind";
+    return ILH_ERR_ARG;
+}
+
+extern "C" int ilh_interpolate(ilh_plan* pl, int kind, double* out, const double* gf, int mem)
+{
+    if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+    if (kind == ILH_NODES_PRIMAL)
+        return grid_to_surf(pl, OP_E_PRIMAL, pl->g.n(GK_PRIMAL), pl->N, out, gf, mem, false, "interpolate!");
+    if (kind == ILH_NODES_DUAL)
+        return grid_to_surf(pl, OP_E_DUAL, pl->g.n(GK_DUAL), pl->N, out, gf, mem, false, "interpolate!");
+    if (kind == ILH_EDGES_PRIMAL)
+        return grid_to_surf(pl, OP_E_EDGES, pl->g.n_edges(), 2 * (size_t)pl->N, out, gf, mem, false, "interpolate!");
+    g_err = "unknown grid kind";
+    return ILH_ERR_ARG;
+}
+
+#define PL_CHECK if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+
+extern "C" int ilh_regularize_normal(ilh_plan* pl, double* q, const double* f, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_RN, pl->N, pl->g.n_edges(), q, f, mem, false, "regularize_normal!");
+}
+extern "C" int ilh_normal_interpolate(ilh_plan* pl, double* vn, const double* q, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_NI, pl->g.n_edges(), pl->N, vn, q, mem, false, "normal_interpolate!");
+}
+extern "C" int ilh_regularize_normal_cross(ilh_plan* pl, double* q, const double* f, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_RNX, pl->N, pl->g.n_edges(), q, f, mem, false, "regularize_normal_cross!");
+}
+extern "C" int ilh_normal_cross_interpolate(ilh_plan* pl, double* vn, const double* q, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_NXI, pl->g.n_edges(), pl->N, vn, q, mem, false, "normal_cross_interpolate!");
+}
+extern "C" int ilh_surface_divergence(ilh_plan* pl, double* th, const double* f, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_DS, pl->N, pl->g.n(GK_PRIMAL), th, f, mem, true, "surface_divergence!");
+}
+extern "C" int ilh_surface_grad(ilh_plan* pl, double* vn, const double* phi, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_GS, pl->g.n(GK_PRIMAL), pl->N, vn, phi, mem, true, "surface_grad!");
+}
+extern "C" int ilh_surface_curl(ilh_plan* pl, double* w, const double* f, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_CST, pl->N, pl->g.n(GK_DUAL), w, f, mem, true, "surface_curl!");
+}
+extern "C" int ilh_surface_curl_adj(ilh_plan* pl, double* vn, const double* s, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_CS, pl->g.n(GK_DUAL), pl->N, vn, s, mem, true, "surface_curl!");
+}
+extern "C" int ilh_surface_curl_cross(ilh_plan* pl, double* w, const double* f, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_CXT, pl->N, pl->g.n(GK_DUAL), w, f, mem, true, "surface_curl_cross!");
+}
+extern "C" int ilh_surface_curl_cross_adj(ilh_plan* pl, double* vn, const double* s, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_CXS, pl->g.n(GK_DUAL), pl->N, vn, s, mem, true, "surface_curl_cross!");
+}
+extern "C" int ilh_surface_grad_cross(ilh_plan* pl, double* vn, const double* phi, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_GXS, pl->g.n(GK_PRIMAL), pl->N, vn, phi, mem, true, "surface_grad_cross!");
+}
+
+// -------------------------------------------------------------- stencils
+template <class K>
+static int grid_op(ilh_plan* pl, size_t nin, size_t nout, double* out, const double* in, int mem, K&& kern)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        const double* d_in = io.in(in, nin, pl->io_a);
+        double* d_out = io.out(out, nout, pl->io_b);
+        if (mem == ILH_MEM_DEVICE && (const void*)d_in == (const void*)d_out)
+            fail(ILH_ERR_ARG, "stencil output must not alias its input");
+        kern(d_out, d_in);
+        io.back(out, d_out, nout);
+        io.done();
+    });
+}
+
+extern "C" int ilh_divergence(ilh_plan* pl, double* p, const double* q, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n_edges(), pl->g.n(GK_PRIMAL), p, q, mem,
+                   [&](double* o, const double* i) { k_divergence(pl, o, i); });
+}
+extern "C" int ilh_grad(ilh_plan* pl, double* q, const double* p, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n(GK_PRIMAL), pl->g.n_edges(), q, p, mem,
+                   [&](double* o, const double* i) { k_grad(pl, o, i); });
+}
+extern "C" int ilh_curl_nodes_to_edges(ilh_plan* pl, double* q, const double* s, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n(GK_DUAL), pl->g.n_edges(), q, s, mem,
+                   [&](double* o, const double* i) { k_curl_n2e(pl, o, i); });
+}
+extern "C" int ilh_curl_edges_to_nodes(ilh_plan* pl, double* w, const double* q, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n_edges(), pl->g.n(GK_DUAL), w, q, mem,
+                   [&](double* o, const double* i) { k_curl_e2n(pl, o, i); });
+}
+extern "C" int ilh_laplacian(ilh_plan* pl, int kind, double* out, const double* in, int mem)
+{
+    PL_CHECK
+    int gk;
+    if (int rc = guarded([&] { gk = kind_to_gk(kind); })) return rc;
+    return grid_op(pl, pl->g.n(gk), pl->g.n(gk), out, in, mem,
+                   [&](double* o, const double* i) { k_laplacian(pl, gk, o, i); });
+}
+
+extern "C" int ilh_inverse_laplacian(ilh_plan* pl, int kind, double* out, const double* in, int mem)
+{
+    PL_CHECK
+    return guarded([&] {
+        const int gk = kind_to_gk(kind);
+        Io io(pl, mem);
+        const size_t n = pl->g.n(gk);
+        const double* d_in = io.in(in, n, pl->io_a);
+        double* d_out = io.out(out, n, pl->io_b);
+        conv_dense(pl, gk, d_in, d_out);
+        io.back(out, d_out, n);
+        io.done();
+    });
+}
+
+// --------------------------------------------------------------- Schur
+namespace {
+__global__ void k_rhs_dir(double* B, int n)   // B = [K - I/2, -K - I/2], K in B[:, 0:n]
+{
+    const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= (size_t)n * n) return;
+    const int r = (int)(k % n), c = (int)(k / n);
+    const double v = B[k];
+    const double h = (r == c) ? 0.5 : 0.0;
+    B[k] = v - h;
+    B[k + (size_t)n * n] = -v - h;
+}
+
+__global__ void k_neg(const double* a, double* b, size_t n)
+{
+    const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k < n) b[k] = -a[k];
+}
+
+__global__ void k_term_shift(const Term* src, int n, int ai_off, Term* dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    Term t = src[k];
+    t.ai += ai_off;
+    dst[k] = t;
+}
+
+void build_schur(ilh_plan* pl, int kind, double scale, double* K)
+{
+    const double dx2 = pl->g.dx * pl->g.dx;
+    switch (kind) {
+    case ILH_SCHUR_RTLINVR: assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -scale * dx2, false, K); break;
+    case ILH_SCHUR_CLINVCT: assemble(pl, OP_CS, OP_CST, GK_DUAL, -scale * dx2, false, K); break;
+    case ILH_SCHUR_GLINVD: assemble(pl, OP_GS, OP_DS, GK_PRIMAL, -scale * dx2, false, K); break;
+    case ILH_SCHUR_NRTRN: assemble(pl, OP_NI, OP_RN, GK_EDGE_U, scale, true, K); break;
+    default: fail(ILH_ERR_ARG, "unknown Schur kind");
+    }
+}
+
+void factor_into(ilh_plan* pl, SolveCache* sc, int kind)
+{
+    const int N = pl->N;
+    sc->LU.ensure(sizeof(double) * (size_t)N * N);
+    sc->piv.ensure(sizeof(int) * (size_t)std::max(1, N));
+    sc->info.ensure(sizeof(int));
+    build_schur(pl, kind, 1.0, sc->LU.as<double>());
+    const int info = lu_factor(pl, sc->LU.as<double>(), N, sc->piv.as<int>(), sc->info.as<int>());
+    if (info) fail(ILH_ERR_SINGULAR, "Schur complement is singular (zero pivot at column " + std::to_string(info) + ")");
+}
+
+// merged source terms: first op with amplitude slot 0, second with slot 1
+void merge_sources(ilh_plan* pl, int op0, int op1, int gk, DevBuf& tbuf, SourceSet& src)
+{
+    const int N = pl->N;
+    const int n0 = N * op_terms_per_point(op0), n1 = N * op_terms_per_point(op1);
+    tbuf.ensure(sizeof(Term) * (size_t)std::max(1, n0 + n1));
+    hipStream_t st = pl->ctx->stream;
+    Term* t = tbuf.as<Term>();
+    if (n0) k_term_shift<<<(n0 + 255) / 256, 256, 0, st>>>(get_terms(pl, op0), n0, 0, t);
+    if (n1) k_term_shift<<<(n1 + 255) / 256, 256, 0, st>>>(get_terms(pl, op1), n1, N, t + n0);
+    ILH_HIP(hipGetLastError());
+    src.terms = t;
+    build_bins(pl, t, n0 + n1, gk, src.bins);
+    build_source_pairs(pl, src);
+}
+
+SolveCache* prepare_dirichlet(ilh_plan* pl)
+{
+    if (pl->dir) return pl->dir;
+    need_margin(pl, "Dirichlet solve");
+    const int N = pl->N;
+    SolveCache* sc = new SolveCache();
+    try {
+        hipStream_t st = pl->ctx->stream;
+        const double dx2 = pl->g.dx * pl->g.dx;
+        if (N > 0) {
+            factor_into(pl, sc, ILH_SCHUR_RTLINVR);
+            // RHS [K_D - I/2, -K_D - I/2] with K_D = E L^-1 D_s
+            DevBuf B;
+            B.ensure(sizeof(double) * (size_t)N * 2 * N);
+            assemble(pl, OP_E_PRIMAL, OP_DS, GK_PRIMAL, dx2, false, B.as<double>());
+            k_rhs_dir<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, st>>>(B.as<double>(), N);
+            ILH_HIP(hipGetLastError());
+            lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 2 * N);
+            for (int k = 0; k < 2; ++k) {
+                sc->M[k].ensure(sizeof(double) * (size_t)N * N);
+                transpose(st, B.as<double>() + (size_t)k * N * N, sc->M[k].as<double>(), N, N);
+            }
+            merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, sc->terms1, sc->src1);
+            ILH_HIP(hipStreamSynchronize(st));
+        }
+        sc->amp1.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));
+    } catch (...) {
+        delete sc;
+        throw;
+    }
+    pl->dir = sc;
+    return sc;
+}
+
+SolveCache* prepare_neumann(ilh_plan* pl)
+{
+    if (pl->neu) return pl->neu;
+    need_margin(pl, "Neumann solve");
+    const int N = pl->N;
+    SolveCache* sc = new SolveCache();
+    try {
+        hipStream_t st = pl->ctx->stream;
+        const double dx2 = pl->g.dx * pl->g.dx;
+        if (N > 0) {
+            factor_into(pl, sc, ILH_SCHUR_CLINVCT);
+            DevBuf B;
+            const size_t NN = (size_t)N * N;
+            B.ensure(sizeof(double) * NN * 3);
+            assemble(pl, OP_GS, OP_R_PRIMAL, GK_PRIMAL, dx2, false, B.as<double>());        // K_GR
+            k_rhs_dir<<<(unsigned)((NN + 255) / 256), 256, 0, st>>>(B.as<double>(), N);
+            assemble(pl, OP_GXS, OP_R_PRIMAL, GK_PRIMAL, dx2, false, B.as<double>() + 2 * NN);  // K_XR
+            ILH_HIP(hipGetLastError());
+            lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 3 * N);
+            for (int k = 0; k < 3; ++k) {
+                sc->M[k].ensure(sizeof(double) * NN);
+                transpose(st, B.as<double>() + (size_t)k * NN, sc->M[k].as<double>(), N, N);
+            }
+            sc->M[3].ensure(sizeof(double) * NN);
+            k_neg<<<(unsigned)((NN + 255) / 256), 256, 0, st>>>(sc->M[2].as<double>(), sc->M[3].as<double>(), NN);
+            ILH_HIP(hipGetLastError());
+            merge_sources(pl, OP_DS, OP_R_PRIMAL, GK_PRIMAL, sc->terms1, sc->src1);
+            merge_sources(pl, OP_CST, OP_CXT, GK_DUAL, sc->terms2, sc->src2);
+            ILH_HIP(hipStreamSynchronize(st));
+        }
+        sc->amp1.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));
+        sc->amp2.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));
+    } catch (...) {
+        delete sc;
+        throw;
+    }
+    pl->neu = sc;
+    return sc;
+}
+}  // namespace
+
+extern "C" int ilh_schur_build(ilh_plan* pl, int kind, double scale, double* out, int mem)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        if (kind != ILH_SCHUR_NRTRN) need_margin(pl, "Schur build");
+        const int N = pl->N;
+        const size_t NN = (size_t)N * N;
+        double* d_out = io.out(out, NN, pl->io_b);
+        if (N > 0) build_schur(pl, kind, scale, d_out);
+        io.back(out, d_out, NN);
+        io.done();
+    });
+}
+
+extern "C" int ilh_schur_factor(ilh_plan* pl, int kind)
+{
+    return guarded([&] {
+        Io io(pl, ILH_MEM_DEVICE);
+        if (kind < 0 || kind > 3) fail(ILH_ERR_ARG, "unknown Schur kind");
+        if (kind != ILH_SCHUR_NRTRN) need_margin(pl, "Schur factor");
+        delete pl->fac[kind];
+        pl->fac[kind] = nullptr;
+        SolveCache* sc = new SolveCache();
+        try {
+            if (pl->N > 0) factor_into(pl, sc, kind);
+        } catch (...) {
+            delete sc;
+            throw;
+        }
+        pl->fac[kind] = sc;
+    });
+}
+
+extern "C" int ilh_schur_solve(ilh_plan* pl, int kind, double* x, const double* b, int mem)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        if (kind < 0 || kind > 3 || !pl->fac[kind]) fail(ILH_ERR_STATE, "ilh_schur_factor must be called first");
+        const int N = pl->N;
+        const double* d_b = io.in(b, N, pl->io_a);
+        double* d_x = io.out(x, N, pl->io_b);
+        if (N > 0) {
+            if (d_x != d_b)
+                ILH_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * N, hipMemcpyDeviceToDevice, pl->ctx->stream));
+            SolveCache* sc = pl->fac[kind];
+            lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, d_x, 1);
+        }
+        io.back(x, d_x, N);
+        io.done();
+    });
+}
+
+// ---------------------------------------------------------- composites
+extern "C" int ilh_dirichlet_solve(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s, int mem)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        SolveCache* sc = prepare_dirichlet(pl);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        hipStream_t st = pl->ctx->stream;
+        const double* d_fp = io.in(fp, N, pl->io_a);
+        const double* d_fm = fm ? io.in(fm, N, pl->io_c) : nullptr;
+        double* d_f = io.out(f, ng, pl->io_b);
+        double* d_s = io.out(s, N, pl->io_d);
+        if (N == 0) {
+            fill_zero(st, d_f, ng);
+        } else {
+            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), d_fp, d_fm, sc->amp1.as<double>(), N);
+            ILH_HIP(hipMemcpyAsync(d_s, sc->amp1.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), d_f);
+        }
+        io.back(f, d_f, ng);
+        io.back(s, d_s, N);
+        io.done();
+    });
+}
+
+extern "C" int ilh_neumann_solve(ilh_plan* pl, const double* vp, const double* vm, double* f, double* df, double* s,
+                                 double* dsv, int mem)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        SolveCache* sc = prepare_neumann(pl);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL), nd = pl->g.n(GK_DUAL);
+        hipStream_t st = pl->ctx->stream;
+        const double* d_vp = io.in(vp, N, pl->io_a);
+        const double* d_vm = vm ? io.in(vm, N, pl->io_c) : nullptr;
+        double* d_f = io.out(f, ng, pl->io_b);
+        double* d_s = io.out(s, nd, pl->scratch_a);
+        double* d_df = io.out(df, N, pl->io_d);
+        double* d_ds = io.out(dsv, N, pl->scratch_b);
+        if (N == 0) {
+            fill_zero(st, d_f, ng);
+            fill_zero(st, d_s, nd);
+        } else {
+            neumann_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), sc->M[2].as<double>(), sc->M[3].as<double>(),
+                        d_vp, d_vm, sc->amp1.as<double>(), sc->amp2.as<double>(), d_df, d_ds, N);
+            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), d_f);
+            conv_sources(pl, GK_DUAL, sc->src2, sc->amp2.as<double>(), d_s);
+        }
+        io.back(f, d_f, ng);
+        io.back(df, d_df, N);
+        io.back(s, d_s, nd);
+        io.back(dsv, d_ds, N);
+        io.done();
+    });
+}
+
+extern "C" int ilh_profile_dirichlet(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s, int reps,
+                                     double* stage_ms, double* total_ms)
+{
+    return guarded([&] {
+        Io io(pl, ILH_MEM_DEVICE);
+        if (reps < 1) fail(ILH_ERR_ARG, "reps must be >= 1");
+        SolveCache* sc = prepare_dirichlet(pl);
+        const int N = pl->N;
+        hipStream_t st = pl->ctx->stream;
+        hipEvent_t ev[5];
+        for (auto& e : ev) ILH_HIP(hipEventCreate(&e));
+        double acc[4] = {0, 0, 0, 0};
+        double tot = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            ILH_HIP(hipEventRecord(ev[0], st));
+            dirichlet_amp(st, sc->M[0].as<double>(), fm ? sc->M[1].as<double>() : nullptr, fp, fm,
+                          sc->amp1.as<double>(), N);
+            ILH_HIP(hipMemcpyAsync(s, sc->amp1.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            ILH_HIP(hipEventRecord(ev[1], st));
+            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f, ev + 2);
+            ILH_HIP(hipEventRecord(ev[4], st));
+            ILH_HIP(hipEventSynchronize(ev[4]));
+            float ms;
+            for (int k = 0; k < 4; ++k) {
+                ILH_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                acc[k] += ms;
+            }
+            ILH_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
+            tot += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < 4; ++k)
+            if (stage_ms) stage_ms[k] = acc[k] / reps;
+        if (total_ms) *total_ms = tot / reps;
+    });
+}

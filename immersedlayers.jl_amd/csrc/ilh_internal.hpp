@@ -1,0 +1,193 @@
+// ilh_internal.hpp -- shared types of libimmersedhip.so (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/immersedhip.h"
+
+namespace ilh {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string& msg);
+void check_hip(hipError_t e, const char* what);
+#define ILH_HIP(call) ::ilh::check_hip((call), #call)
+
+// -------------------------------------------------------------- grid kinds
+// Component grids: Nodes{Primal}, Nodes{Dual}, Edges{Primal}.u, Edges{Primal}.v
+enum GK { GK_PRIMAL = 0, GK_DUAL = 1, GK_EDGE_U = 2, GK_EDGE_V = 3 };
+
+struct Grid {
+    int NX, NY, I0x, I0y;
+    double dx;
+    int nx(int gk) const { return (gk == GK_PRIMAL || gk == GK_EDGE_V) ? NX - 1 : NX; }
+    int ny(int gk) const { return (gk == GK_PRIMAL || gk == GK_EDGE_U) ? NY - 1 : NY; }
+    size_t n(int gk) const { return (size_t)nx(gk) * ny(gk); }
+    size_t n_edges() const { return n(GK_EDGE_U) + n(GK_EDGE_V); }
+};
+
+// -------------------------------------------------------------------- terms
+// One separable piece of a point's stencil on one component grid:
+//   contribution(i, j) = coef * wx[i - bx] * wy[j - by],  0 <= i-bx, j-by < 5.
+// Surface->grid operators scatter amp[ai] * contribution; grid->surface
+// operators gather sum_ij contribution(i, j) * field(i, j) into out[ai].
+struct Term {
+    int ai;        // amplitude / output index
+    int gk;        // component grid
+    int bx, by;    // window base
+    double coef;
+    double wx[5];
+    double wy[5];
+};
+
+// Surface operators, each a list of terms per point (T terms per point,
+// stored point-major: term index = p * T + t).
+enum OpKind {
+    OP_R_PRIMAL = 0,   // regularize!  ScalarData -> Nodes{Primal}
+    OP_R_DUAL,         // regularize!  ScalarData -> Nodes{Dual}
+    OP_R_EDGES,        // regularize!  VectorData -> Edges{Primal}
+    OP_E_PRIMAL,       // interpolate! Nodes{Primal} -> ScalarData
+    OP_E_DUAL,         // interpolate! Nodes{Dual} -> ScalarData
+    OP_E_EDGES,        // interpolate! Edges{Primal} -> VectorData
+    OP_RN,             // regularize_normal!
+    OP_NI,             // normal_interpolate!
+    OP_RNX,            // regularize_normal_cross!
+    OP_NXI,            // normal_cross_interpolate!
+    OP_DS,             // surface_divergence! ScalarData -> Nodes{Primal}
+    OP_GS,             // surface_grad!       Nodes{Primal} -> ScalarData
+    OP_CST,            // surface_curl!       ScalarData -> Nodes{Dual}
+    OP_CS,             // surface_curl!       Nodes{Dual} -> ScalarData
+    OP_CXT,            // surface_curl_cross! ScalarData -> Nodes{Dual}
+    OP_CXS,            // surface_curl_cross! Nodes{Dual} -> ScalarData
+    OP_GXS,            // surface_grad_cross! Nodes{Primal} -> ScalarData
+    OP_COUNT
+};
+int op_terms_per_point(int op);
+bool op_is_scatter(int op);
+
+// Row-binned index of scatter terms on one component grid:
+// entries[row_ptr[j] .. row_ptr[j+1]) are term indices whose window touches
+// row j, sorted by (bx, term index).
+struct RowBins {
+    int gk = 0;
+    int ny = 0;
+    int* row_ptr = nullptr;     // device, ny + 1
+    int* entries = nullptr;     // device
+    int n_entries = 0;
+    std::vector<int> h_row_ptr; // host copy (row counts drive launch shapes)
+};
+
+// ---------------------------------------------------------- device buffers
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b);
+    void release();
+    template <class T> T* as() const { return (T*)p; }
+    ~DevBuf() { release(); }
+};
+
+// Zero-padded FFT convolution with the LGF on one grid kind.
+struct ConvPlan {
+    int gk = 0;
+    int nx = 0, ny = 0;       // field dims
+    int Px = 0, Py = 0;       // padded periods (powers of two, >= 2n-2)
+    int H = 0;                // Px/2 + 1 (stored spectrum columns)
+    DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], H x Py
+    DevBuf T1, T2;            // intermediate spectra, rows x H complex
+};
+
+}  // namespace ilh
+
+struct ilh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    ilh::DevBuf twiddle;      // exp(-2 pi i k / 8192) table
+    int tw_len = 0;
+};
+
+struct SolveCache;   // Schur factor and solve matrices of one composite
+
+struct ilh_plan {
+    ilh_ctx* ctx = nullptr;
+    ilh::Grid g{};
+    int N = 0;
+    int ddf = 0;
+    int min_margin = 0;       // cells between the point supports and the grid edge
+    ilh::DevBuf X, Y, nx, ny, ds;            // device geometry
+    ilh::DevBuf terms[ilh::OP_COUNT];        // per-op term arrays (built lazily)
+    bool terms_ok[ilh::OP_COUNT] = {};
+    ilh::RowBins bins[ilh::OP_COUNT][2];     // row bins of scatter ops per component grid
+    bool bins_ok[ilh::OP_COUNT] = {};
+    ilh::ConvPlan conv[2];                    // primal, dual
+    bool conv_ok[2] = {};
+    ilh::DevBuf lgf;                          // LGF table tmx x tmy (covers both kinds)
+    int lgf_mx = 0, lgf_my = 0;
+    ilh::DevBuf io_a, io_b, io_c, io_d;       // staging for host-memory calls
+    ilh::DevBuf scratch_a, scratch_b;         // grid scratch
+    SolveCache* dir = nullptr;                // Dirichlet
+    SolveCache* neu = nullptr;                // Neumann
+    SolveCache* fac[4] = {};                  // ilh_schur_factor caches per kind
+};
+
+namespace ilh {
+// ---------------------------------------------------------------- launchers
+// terms / binning (ilh_terms.hip)
+void build_terms(ilh_plan* pl, int op);
+const Term* get_terms(ilh_plan* pl, int op);
+void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& bins);
+void free_bins(RowBins& b);
+void scatter(ilh_plan* pl, const Term* terms, const RowBins& bins, const double* amp, double* out);
+void gather(ilh_plan* pl, const Term* terms, int nterms_per_point, const double* field_primal_or_dual,
+            const double* field_u, const double* field_v, double* out);
+
+// stencils (ilh_stencil.hip)
+void k_divergence(ilh_plan* pl, double* p, const double* q);
+void k_grad(ilh_plan* pl, double* q, const double* p);
+void k_curl_n2e(ilh_plan* pl, double* q, const double* s);
+void k_curl_e2n(ilh_plan* pl, double* w, const double* q);
+void k_laplacian(ilh_plan* pl, int gk, double* w, const double* s);
+void fill_zero(hipStream_t st, double* a, size_t n);
+void axpby(hipStream_t st, double* y, const double* x, double a, double b, size_t n);  // y = a x + b y
+
+// FFT convolution (ilh_fft.hip)
+void ensure_twiddles(ilh_ctx* ctx);
+void ensure_lgf(ilh_plan* pl);
+ConvPlan& get_conv(ilh_plan* pl, int gk);
+// field -> field:  out = dx^2 (G * in)
+void conv_dense(ilh_plan* pl, int gk, const double* in, double* out);
+// point sources -> field: out = dx^2 (G * sum_terms amp*term), rows listed in pairs
+struct SourceSet {
+    const Term* terms = nullptr;
+    RowBins bins;
+    int* pairs = nullptr;      // device list of nonzero row pairs
+    int npairs = 0;
+};
+void build_source_pairs(ilh_plan* pl, SourceSet& s);
+void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
+                  hipEvent_t* ev = nullptr);
+
+// dense linear algebra (ilh_dense.hip)
+void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel,
+              double* K /* N x N col-major */);
+int  lu_factor(ilh_plan* pl, double* A, int n, int* piv_dev, int* info_dev);
+void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, double* B, int nrhs);
+void transpose(hipStream_t st, const double* A, double* At, int rows, int cols);
+void matvec_rows(hipStream_t st, const double* Mrow, const double* x, double* y, int n, int m,
+                 double alpha, int accumulate);
+void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, const double* fp,
+                   const double* fm, double* amp, int n);
+void neumann_amp(hipStream_t st, const double* Mdf_p, const double* Mdf_m, const double* Mds_p,
+                 const double* Mds_m, const double* vp, const double* vm, double* amp_f, double* amp_s,
+                 double* df, double* dsv, int n);
+void check_finite(hipStream_t st, const double* a, size_t n, int* flag_dev);
+
+}  // namespace ilh

@@ -1,0 +1,446 @@
+// ilh_terms.hip -- DDF stencil terms of the surface operators, their row
+// binning, and the deterministic scatter (surface -> grid) / gather
+// (grid -> surface) kernels.
+//
+// Every surface operator of the scalar cache (src/surface_operators.jl) is a
+// sum over points of at most two separable 5x5 windows ("terms") on one of the
+// four component grids.  The DDF weights and the MAC stencil of the composed
+// operator are folded into the window weights once per geometry, so e.g.
+// surface_divergence! = D Rf (n o f) / dx becomes a single scatter pass.
+//
+// Scatter is a gather over output cells: terms are binned by output row and
+// sorted by window base, each output cell sums the terms covering it in that
+// fixed order.  No float atomics; results are bitwise reproducible.
+#include "ilh_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ilh {
+
+// ----------------------------------------------------------------- DDFs
+// Yang3 (Yang, Zhang, Li & He 2009, the default Regularize kernel,
+// src/ImmersedLayers.jl:83), Roma, M4', linear hat.  r in grid units.
+__device__ __forceinline__ double ddf_eval(int kind, double r)
+{
+    const double a = fabs(r);
+    if (kind == ILH_DDF_YANG3) {
+        const double s3 = 1.7320508075688772935274463;
+        const double pi = 3.14159265358979323846264338;
+        if (a <= 1.0)
+            return 17.0 / 48.0 + s3 * pi / 108.0 + a / 4.0 - a * a / 4.0 +
+                   (1.0 - 2.0 * a) / 16.0 * sqrt(-12.0 * a * a + 12.0 * a + 1.0) -
+                   s3 / 12.0 * asin(s3 / 2.0 * (2.0 * a - 1.0));
+        if (a <= 2.0)
+            return 55.0 / 48.0 - s3 * pi / 108.0 - 13.0 * a / 12.0 + a * a / 4.0 +
+                   (2.0 * a - 3.0) / 48.0 * sqrt(-12.0 * a * a + 36.0 * a - 23.0) +
+                   s3 / 36.0 * asin(s3 / 2.0 * (2.0 * a - 3.0));
+        return 0.0;
+    }
+    if (kind == ILH_DDF_ROMA) {
+        if (a <= 0.5) return (1.0 + sqrt(1.0 - 3.0 * a * a)) / 3.0;
+        if (a <= 1.5) return (5.0 - 3.0 * a - sqrt(1.0 - 3.0 * (1.0 - a) * (1.0 - a))) / 6.0;
+        return 0.0;
+    }
+    if (kind == ILH_DDF_M4PRIME) {
+        if (a <= 1.0) return 1.0 - 2.5 * a * a + 1.5 * a * a * a;
+        if (a <= 2.0) return 0.5 * (2.0 - a) * (2.0 - a) * (1.0 - a);
+        return 0.0;
+    }
+    return a <= 1.0 ? 1.0 - a : 0.0;   // witch hat
+}
+
+// 4-tap window of a DDF centred at fractional index xi: base floor(xi)-1.
+struct Win {
+    int b;
+    double w[5];
+};
+
+__device__ __forceinline__ Win ddf_win(int kind, double xi)
+{
+    Win r;
+    r.b = (int)floor(xi) - 1;
+    for (int e = 0; e < 4; ++e) r.w[e] = ddf_eval(kind, (double)(r.b + e) - xi);
+    r.w[4] = 0.0;
+    return r;
+}
+
+// first difference w'[e] = w[e] - w[e-1] (5 taps); the caller picks the base:
+// same base for curl (out_i = a_i - a_{i-1}), base-1 for divergence
+// (out_i = a_{i+1} - a_i).
+__device__ __forceinline__ Win diff_win(const Win& a, int base_shift)
+{
+    Win r;
+    r.b = a.b + base_shift;
+    r.w[0] = a.w[0];
+    for (int e = 1; e < 4; ++e) r.w[e] = a.w[e] - a.w[e - 1];
+    r.w[4] = -a.w[3];
+    return r;
+}
+
+__device__ __forceinline__ void put(Term& t, int ai, int gk, const Win& x, const Win& y, double coef)
+{
+    t.ai = ai;
+    t.gk = gk;
+    t.bx = x.b;
+    t.by = y.b;
+    t.coef = coef;
+    for (int e = 0; e < 5; ++e) {
+        t.wx[e] = x.w[e];
+        t.wy[e] = y.w[e];
+    }
+}
+
+__global__ void k_build_terms(int op, int N, int ddf, Grid g, const double* __restrict__ X,
+                              const double* __restrict__ Y, const double* __restrict__ NXv,
+                              const double* __restrict__ NYv, const double* __restrict__ DS,
+                              Term* __restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const double dx = g.dx;
+    // fractional indices of the point on the primal (x_i = (i+1-I0) dx) and
+    // dual (x_i = (i+1/2-I0) dx) axes, 0-based
+    const double xp = X[p] / dx - (1.0 - g.I0x), yp = Y[p] / dx - (1.0 - g.I0y);
+    const Win Px = ddf_win(ddf, xp), Py = ddf_win(ddf, yp);
+    const Win Dx = ddf_win(ddf, xp + 0.5), Dy = ddf_win(ddf, yp + 0.5);
+    const double n_x = NXv[p], n_y = NYv[p], ds = DS[p];
+    const double r2 = ds / (dx * dx), r3 = ds / (dx * dx * dx), i1 = 1.0 / dx;
+    Term a, b;
+    switch (op) {
+    case OP_R_PRIMAL: put(a, p, GK_PRIMAL, Px, Py, r2); out[p] = a; return;
+    case OP_R_DUAL: put(a, p, GK_DUAL, Dx, Dy, r2); out[p] = a; return;
+    case OP_E_PRIMAL: put(a, p, GK_PRIMAL, Px, Py, 1.0); out[p] = a; return;
+    case OP_E_DUAL: put(a, p, GK_DUAL, Dx, Dy, 1.0); out[p] = a; return;
+    case OP_R_EDGES:
+        put(a, p, GK_EDGE_U, Dx, Py, r2);
+        put(b, N + p, GK_EDGE_V, Px, Dy, r2);
+        break;
+    case OP_E_EDGES:
+        put(a, p, GK_EDGE_U, Dx, Py, 1.0);
+        put(b, N + p, GK_EDGE_V, Px, Dy, 1.0);
+        break;
+    case OP_RN:
+        put(a, p, GK_EDGE_U, Dx, Py, n_x * r2);
+        put(b, p, GK_EDGE_V, Px, Dy, n_y * r2);
+        break;
+    case OP_NI:
+        put(a, p, GK_EDGE_U, Dx, Py, n_x);
+        put(b, p, GK_EDGE_V, Px, Dy, n_y);
+        break;
+    case OP_RNX:   // n x (f e_z) = (n_y f, -n_x f)
+        put(a, p, GK_EDGE_U, Dx, Py, n_y * r2);
+        put(b, p, GK_EDGE_V, Px, Dy, -n_x * r2);
+        break;
+    case OP_NXI:   // e_z . (n x v) = n_x v_y - n_y v_x
+        put(a, p, GK_EDGE_U, Dx, Py, -n_y);
+        put(b, p, GK_EDGE_V, Px, Dy, n_x);
+        break;
+    case OP_DS:    // D Rf (n o f) / dx
+        put(a, p, GK_PRIMAL, diff_win(Dx, -1), Py, n_x * r3);
+        put(b, p, GK_PRIMAL, Px, diff_win(Dy, -1), n_y * r3);
+        break;
+    case OP_GS:    // n . Rf^T G phi / dx   (G = -D^T)
+        put(a, p, GK_PRIMAL, diff_win(Dx, -1), Py, -n_x * i1);
+        put(b, p, GK_PRIMAL, Px, diff_win(Dy, -1), -n_y * i1);
+        break;
+    case OP_GXS:   // e_z . (n x Rf^T G phi) / dx
+        put(a, p, GK_PRIMAL, diff_win(Dx, -1), Py, n_y * i1);
+        put(b, p, GK_PRIMAL, Px, diff_win(Dy, -1), -n_x * i1);
+        break;
+    case OP_CST:   // C^T Rf (n o f) / dx,  w = (v_i - v_{i-1}) - (u_j - u_{j-1})
+        put(a, p, GK_DUAL, diff_win(Px, 0), Dy, n_y * r3);
+        put(b, p, GK_DUAL, Dx, diff_win(Py, 0), -n_x * r3);
+        break;
+    case OP_CS:    // n . Rf^T C s / dx,  u = s_{j+1} - s_j, v = s_i - s_{i+1}
+        put(a, p, GK_DUAL, diff_win(Px, 0), Dy, n_y * i1);
+        put(b, p, GK_DUAL, Dx, diff_win(Py, 0), -n_x * i1);
+        break;
+    case OP_CXT:   // C^T Rf (n x f e_z) / dx
+        put(a, p, GK_DUAL, diff_win(Px, 0), Dy, -n_x * r3);
+        put(b, p, GK_DUAL, Dx, diff_win(Py, 0), -n_y * r3);
+        break;
+    case OP_CXS:   // e_z . (n x Rf^T C s) / dx
+        put(a, p, GK_DUAL, diff_win(Px, 0), Dy, n_x * i1);
+        put(b, p, GK_DUAL, Dx, diff_win(Py, 0), n_y * i1);
+        break;
+    default: return;
+    }
+    out[2 * p] = a;
+    out[2 * p + 1] = b;
+}
+
+int op_terms_per_point(int op)
+{
+    switch (op) {
+    case OP_R_PRIMAL: case OP_R_DUAL: case OP_E_PRIMAL: case OP_E_DUAL: return 1;
+    default: return 2;
+    }
+}
+
+bool op_is_scatter(int op)
+{
+    switch (op) {
+    case OP_R_PRIMAL: case OP_R_DUAL: case OP_R_EDGES: case OP_RN: case OP_RNX:
+    case OP_DS: case OP_CST: case OP_CXT: return true;
+    default: return false;
+    }
+}
+
+void build_terms(ilh_plan* pl, int op)
+{
+    const int T = op_terms_per_point(op);
+    pl->terms[op].ensure(sizeof(Term) * (size_t)std::max(1, pl->N * T));
+    if (pl->N > 0) {
+        k_build_terms<<<(pl->N + 127) / 128, 128, 0, pl->ctx->stream>>>(
+            op, pl->N, pl->ddf, pl->g, pl->X.as<double>(), pl->Y.as<double>(), pl->nx.as<double>(),
+            pl->ny.as<double>(), pl->ds.as<double>(), pl->terms[op].as<Term>());
+        ILH_HIP(hipGetLastError());
+    }
+    pl->terms_ok[op] = true;
+}
+
+const Term* get_terms(ilh_plan* pl, int op)
+{
+    if (!pl->terms_ok[op]) build_terms(pl, op);
+    return pl->terms[op].as<Term>();
+}
+
+// ------------------------------------------------------------ row binning
+__global__ void k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny, int* __restrict__ cnt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt || t[k].gk != gk) return;
+    for (int e = 0; e < 5; ++e) {
+        const int row = t[k].by + e;
+        if (row >= 0 && row < ny && t[k].wy[e] != 0.0) atomicAdd(&cnt[row], 1);
+    }
+}
+
+// exclusive scan of cnt[0..n) into ptr[0..n], single work-group
+__global__ void k_scan(const int* __restrict__ cnt, int n, int* __restrict__ ptr)
+{
+    __shared__ int part[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        int v = i < n ? cnt[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) ptr[i] = carry + part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += part[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ptr[n] = carry;
+}
+
+__global__ void k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny, const int* __restrict__ ptr,
+                           int* __restrict__ cursor, int* __restrict__ ent)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt || t[k].gk != gk) return;
+    for (int e = 0; e < 5; ++e) {
+        const int row = t[k].by + e;
+        if (row >= 0 && row < ny && t[k].wy[e] != 0.0) {
+            const int pos = atomicAdd(&cursor[row], 1);
+            ent[ptr[row] + pos] = k;
+        }
+    }
+}
+
+// Sort each row's entries by (bx, term index): the order in which a cell sums
+// its contributions.  Bitonic sort in LDS for rows up to 4096 entries, an
+// in-place insertion sort by one thread beyond that.
+__global__ void k_bin_sort(const Term* __restrict__ t, const int* __restrict__ ptr, int* __restrict__ ent)
+{
+    const int row = blockIdx.x;
+    const int lo = ptr[row], n = ptr[row + 1] - lo;
+    if (n <= 1) return;
+    __shared__ unsigned long long key[4096];
+    auto mk = [&](int k) {
+        return ((unsigned long long)(unsigned)(t[k].bx + (1 << 30)) << 32) | (unsigned)k;
+    };
+    if (n > 4096) {
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < n; ++i) {
+                const int v = ent[lo + i];
+                const unsigned long long kv = mk(v);
+                int j = i - 1;
+                while (j >= 0 && mk(ent[lo + j]) > kv) {
+                    ent[lo + j + 1] = ent[lo + j];
+                    --j;
+                }
+                ent[lo + j + 1] = v;
+            }
+        }
+        return;
+    }
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) key[i] = i < n ? mk(ent[lo + i]) : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const bool up = (i & size) == 0;
+                    const unsigned long long a = key[i], b = key[j];
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ent[lo + i] = (int)(key[i] & 0xffffffffu);
+}
+
+__global__ void k_bin_bx(const Term* __restrict__ t, const int* __restrict__ ent, int n, int* __restrict__ ebx)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) ebx[k] = t[ent[k]].bx;
+}
+
+void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
+{
+    hipStream_t st = pl->ctx->stream;
+    free_bins(b);
+    b.gk = gk;
+    b.ny = pl->g.ny(gk);
+    int* cnt = nullptr;
+    ILH_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(2 * b.ny + 2)));
+    int* cursor = cnt + b.ny + 1;
+    ILH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)(2 * b.ny + 2), st));
+    ILH_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (size_t)(b.ny + 1)));
+    if (nterms > 0)
+        k_bin_count<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, cnt);
+    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.row_ptr);
+    ILH_HIP(hipGetLastError());
+    b.h_row_ptr.resize(b.ny + 1);
+    ILH_HIP(hipMemcpyAsync(b.h_row_ptr.data(), b.row_ptr, sizeof(int) * (size_t)(b.ny + 1),
+                           hipMemcpyDeviceToHost, st));
+    ILH_HIP(hipStreamSynchronize(st));
+    b.n_entries = b.h_row_ptr[b.ny];
+    ILH_HIP(hipMalloc(&b.entries, sizeof(int) * (size_t)std::max(1, 2 * b.n_entries)));
+    if (nterms > 0 && b.n_entries > 0) {
+        k_bin_fill<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, b.row_ptr, cursor, b.entries);
+        k_bin_sort<<<b.ny, 256, 0, st>>>(terms, b.row_ptr, b.entries);
+        k_bin_bx<<<(b.n_entries + 255) / 256, 256, 0, st>>>(terms, b.entries, b.n_entries, b.entries + b.n_entries);
+        ILH_HIP(hipGetLastError());
+    }
+    ILH_HIP(hipStreamSynchronize(st));
+    ILH_HIP(hipFree(cnt));
+}
+
+void free_bins(RowBins& b)
+{
+    if (b.row_ptr) (void)hipFree(b.row_ptr);
+    if (b.entries) (void)hipFree(b.entries);
+    b.row_ptr = nullptr;
+    b.entries = nullptr;
+    b.n_entries = 0;
+    b.h_row_ptr.clear();
+}
+
+// --------------------------------------------------------------- scatter
+// out(x, j) = sum over the row's terms with bx <= x < bx+5, in (bx, term) order.
+__device__ __forceinline__ double scatter_cell(const Term* __restrict__ t, const int* __restrict__ ent,
+                                               const int* __restrict__ ebx, int lo, int hi, int x, int j,
+                                               const double* __restrict__ amp)
+{
+    // first entry with bx >= x - 4
+    int a = lo, b = hi;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (ebx[m] < x - 4) a = m + 1; else b = m;
+    }
+    double acc = 0.0;
+    for (int k = a; k < hi; ++k) {
+        const int bx = ebx[k];
+        if (bx > x) break;
+        const Term& tt = t[ent[k]];
+        acc += amp[tt.ai] * tt.coef * tt.wy[j - tt.by] * tt.wx[x - bx];
+    }
+    return acc;
+}
+
+__global__ void k_scatter(const Term* __restrict__ t, const int* __restrict__ ptr, const int* __restrict__ ent,
+                          const int* __restrict__ ebx, int nx, const double* __restrict__ amp,
+                          double* __restrict__ out)
+{
+    const int j = blockIdx.y;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    const int lo = ptr[j], hi = ptr[j + 1];
+    out[x + (size_t)nx * j] = (lo == hi) ? 0.0 : scatter_cell(t, ent, ebx, lo, hi, x, j, amp);
+}
+
+void scatter(ilh_plan* pl, const Term* terms, const RowBins& bins, const double* amp, double* out)
+{
+    const int nx = pl->g.nx(bins.gk);
+    dim3 grid((nx + 255) / 256, bins.ny);
+    k_scatter<<<grid, 256, 0, pl->ctx->stream>>>(terms, bins.row_ptr, bins.entries, bins.entries + bins.n_entries,
+                                                 nx, amp, out);
+    ILH_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- gather
+struct GatherFields {
+    const double* f[4];   // by component grid
+    int nx[4], ny[4];
+};
+
+__global__ void k_gather(const Term* __restrict__ t, int T, int N, GatherFields F, double* __restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    double acc0 = 0.0, acc1 = 0.0;
+    int ai0 = -1, ai1 = -1;
+    for (int k = 0; k < T; ++k) {
+        const Term& tt = t[p * T + k];
+        const double* fld = F.f[tt.gk];
+        const int nx = F.nx[tt.gk], ny = F.ny[tt.gk];
+        double s = 0.0;
+        for (int b = 0; b < 5; ++b) {
+            const int row = tt.by + b;
+            if (row < 0 || row >= ny || tt.wy[b] == 0.0) continue;
+            double r = 0.0;
+            for (int a = 0; a < 5; ++a) {
+                const int col = tt.bx + a;
+                if (col >= 0 && col < nx) r += tt.wx[a] * fld[col + (size_t)nx * row];
+            }
+            s += tt.wy[b] * r;
+        }
+        s *= tt.coef;
+        if (ai0 < 0 || tt.ai == ai0) { ai0 = tt.ai; acc0 += s; }
+        else { ai1 = tt.ai; acc1 += s; }
+    }
+    out[ai0] = acc0;
+    if (ai1 >= 0) out[ai1] = acc1;
+}
+
+void gather(ilh_plan* pl, const Term* terms, int T, const double* fpd, const double* fu, const double* fv,
+            double* out)
+{
+    if (pl->N == 0) return;
+    GatherFields F;
+    const Grid& g = pl->g;
+    F.f[GK_PRIMAL] = fpd; F.f[GK_DUAL] = fpd; F.f[GK_EDGE_U] = fu; F.f[GK_EDGE_V] = fv;
+    for (int k = 0; k < 4; ++k) { F.nx[k] = g.nx(k); F.ny[k] = g.ny(k); }
+    k_gather<<<(pl->N + 127) / 128, 128, 0, pl->ctx->stream>>>(terms, T, pl->N, F, out);
+    ILH_HIP(hipGetLastError());
+}
+
+}  // namespace ilh

@@ -1,0 +1,449 @@
+"""Host-side mirror of ImmersedLayers.jl's scalar-cache API over libimmersedhip.so.
+
+Same names, argument order (output first) and meaning as the reference's
+operators; Julia's trailing ``!`` is dropped.  Grid data live in numpy arrays
+with the reference's column-major layouts (Nodes are ``(nx, ny)`` Fortran-ordered
+arrays, Edges{Primal} is one buffer ``[u; v]``, VectorData is ``[u; v]``), or in
+device buffers (:class:`DeviceArray`) for chained calls without PCIe traffic.
+
+Reference entry points mirrored here (paths in the reference checkout):
+  SurfaceScalarCache          src/cache.jl:164-182, 205-224
+  regularize / interpolate    src/surface_operators.jl:13-88
+  regularize_normal(_cross)   src/surface_operators.jl:98-103, 150-162
+  normal_(cross_)interpolate  src/surface_operators.jl:228-233, 277-290
+  surface_divergence/grad/curl/curl_cross/grad_cross
+                              src/surface_operators.jl:357-725
+  divergence/grad/curl/laplacian/inverse_laplacian
+                              src/grid_operators.jl:25-179
+  create_RTLinvR/CLinvCT/GLinvD/nRTRn
+                              src/matrix_operators.jl:9-61, 135-155, 225-244
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .geometry import GridSpec
+
+Primal, Dual = "Primal", "Dual"
+GridScaling = "GridScaling"
+Yang3, Roma, M4prime, Witchhat = L.DDF_YANG3, L.DDF_ROMA, L.DDF_M4PRIME, L.DDF_WITCHHAT
+
+
+# ----------------------------------------------------------------- context
+class Context:
+    """One HIP device and one stream (``ilh_ctx``)."""
+
+    def __init__(self, device=0):
+        L.load()
+        h = C.c_void_p()
+        L.call("ilh_ctx_create", device, C.byref(h))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        L.call("ilh_ctx_synchronize", self.h)
+
+    def close(self):
+        if self.h:
+            L.load().ilh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DeviceArray:
+    """FP64 device buffer owned by a context (``ilh_malloc``)."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        p = C.c_void_p()
+        L.call("ilh_malloc", ctx.h, C.c_size_t(8 * max(self.n, 1)), C.byref(p))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+        d = cls(ctx, a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+        if a.size != self.n:
+            raise L.DimensionMismatch("upload of %d values into %d" % (a.size, self.n))
+        L.call("ilh_memcpy_h2d", self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(8 * self.n))
+
+    def download(self):
+        out = np.empty(self.n)
+        L.call("ilh_memcpy_d2h", self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(8 * self.n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            L.load().ilh_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ grid data
+class Nodes:
+    """Nodes{Primal|Dual} grid data: ``data`` is an (nx, ny) Fortran-ordered array."""
+
+    def __init__(self, kind, g, data=None):
+        self.kind = kind
+        shape = g.primal_shape() if kind == Primal else g.dual_shape()
+        self.data = np.zeros(shape, order="F") if data is None else np.asfortranarray(data, dtype=np.float64)
+        if self.data.shape != shape:
+            raise L.DimensionMismatch("Nodes{%s} of size %s, expected %s" % (kind, self.data.shape, shape))
+
+    @property
+    def ilh_kind(self):
+        return L.NODES_PRIMAL if self.kind == Primal else L.NODES_DUAL
+
+    def flat(self):
+        return self.data.reshape(-1, order="F")
+
+
+class Edges:
+    """Edges{Primal} grid data: one buffer [u; v], u (NX, NY-1), v (NX-1, NY)."""
+
+    def __init__(self, g, data=None):
+        nu, nv = g.NX * (g.NY - 1), (g.NX - 1) * g.NY
+        self.buf = np.zeros(nu + nv) if data is None else np.ascontiguousarray(data, dtype=np.float64)
+        if self.buf.size != nu + nv:
+            raise L.DimensionMismatch("Edges{Primal} of length %d, expected %d" % (self.buf.size, nu + nv))
+        self.u = self.buf[:nu].reshape((g.NX, g.NY - 1), order="F")
+        self.v = self.buf[nu:].reshape((g.NX - 1, g.NY), order="F")
+
+    def flat(self):
+        return self.buf
+
+
+class ScalarData:
+    def __init__(self, n, data=None):
+        self.data = np.zeros(n) if data is None else np.ascontiguousarray(data, dtype=np.float64).copy()
+        if self.data.size != n:
+            raise L.DimensionMismatch("ScalarData of length %d, expected %d" % (self.data.size, n))
+
+    def flat(self):
+        return self.data
+
+
+class VectorData:
+    def __init__(self, n, data=None):
+        self.buf = np.zeros(2 * n) if data is None else np.ascontiguousarray(data, dtype=np.float64).copy()
+        if self.buf.size != 2 * n:
+            raise L.DimensionMismatch("VectorData of length %d, expected %d" % (self.buf.size, 2 * n))
+        self.u = self.buf[:n]
+        self.v = self.buf[n:]
+
+    def flat(self):
+        return self.buf
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# -------------------------------------------------------------- the cache
+class SurfaceScalarCache:
+    """The reference's SurfaceScalarCache (src/cache.jl:164-182) backed by an
+    ``ilh_plan``: grid, surface points, normals, areas, DDF and scaling."""
+
+    def __init__(self, surface, g, ddftype=Yang3, scaling=GridScaling, ctx=None):
+        if scaling != GridScaling:
+            raise L.ArgumentError("only GridScaling is supported")
+        self.ctx = ctx or default_context()
+        self.g = g if isinstance(g, GridSpec) else GridSpec(*g)
+        self.ddftype = ddftype
+        self._set_points(surface)
+        grid = L.Grid(self.g.NX, self.g.NY, self.g.I0x, self.g.I0y, self.g.dx)
+        h = C.c_void_p()
+        L.call("ilh_plan_create", self.ctx.h, C.byref(grid), self.N, _ptr(self._X), _ptr(self._Y),
+               _ptr(self._nx), _ptr(self._ny), _ptr(self._ds), ddftype, 0, C.byref(h))
+        self.h = h
+
+    def _set_points(self, s):
+        self._X = np.ascontiguousarray(s.X, dtype=np.float64)
+        self._Y = np.ascontiguousarray(s.Y, dtype=np.float64)
+        self._nx = np.ascontiguousarray(s.nx, dtype=np.float64)
+        self._ny = np.ascontiguousarray(s.ny, dtype=np.float64)
+        self._ds = np.ascontiguousarray(s.ds, dtype=np.float64)
+        self.N = len(self._X)
+        for a in (self._Y, self._nx, self._ny, self._ds):
+            if len(a) != self.N:
+                raise L.DimensionMismatch("geometry arrays of unequal length")
+
+    def update_points(self, surface):
+        """update_system! for moving bodies (src/system.jl:26-50): same N, new geometry."""
+        n_old = self.N
+        self._set_points(surface)
+        if self.N != n_old:
+            raise L.DimensionMismatch("point count changed from %d to %d" % (n_old, self.N))
+        L.call("ilh_plan_update_points", self.h, _ptr(self._X), _ptr(self._Y), _ptr(self._nx), _ptr(self._ny),
+               _ptr(self._ds))
+
+    # reference accessors (src/tools.jl:16-38, src/cache.jl:399-595)
+    def points(self):
+        return VectorData(self.N, np.concatenate([self._X, self._Y]))
+
+    def normals(self):
+        return VectorData(self.N, np.concatenate([self._nx, self._ny]))
+
+    def areas(self):
+        return ScalarData(self.N, self._ds)
+
+    def zeros_grid(self):
+        return Nodes(Primal, self.g)
+
+    def zeros_gridcurl(self):
+        return Nodes(Dual, self.g)
+
+    def zeros_gridgrad(self):
+        return Edges(self.g)
+
+    def zeros_surface(self):
+        return ScalarData(self.N)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().ilh_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -------------------------------------------------------------- dispatch
+def _check_len(a, n, what):
+    if a.flat().size != n:
+        raise L.DimensionMismatch("%s has %d values, expected %d" % (what, a.flat().size, n))
+
+
+def _run(name, cache, out, inp, *lead):
+    o, i = out.flat(), inp.flat()
+    if not o.flags.c_contiguous:
+        raise L.ArgumentError("output must be contiguous")
+    i = np.ascontiguousarray(i)
+    L.call(name, cache.h, *lead, _ptr(o), _ptr(i), L.MEM_HOST)
+    return out
+
+
+def regularize(out, f, cache):
+    """s = Rc f (Nodes) or v = Rf vb (Edges{Primal} from VectorData)."""
+    if isinstance(out, Nodes) and isinstance(f, ScalarData):
+        _check_len(f, cache.N, "ScalarData")
+        return _run("ilh_regularize", cache, out, f, out.ilh_kind)
+    if isinstance(out, Edges) and isinstance(f, VectorData):
+        _check_len(f, 2 * cache.N, "VectorData")
+        return _run("ilh_regularize", cache, out, f, L.EDGES_PRIMAL)
+    raise TypeError("regularize: no method for (%s, %s)" % (type(out).__name__, type(f).__name__))
+
+
+def interpolate(out, field, cache):
+    """f = Rc^T s (ScalarData from Nodes) or vb = Rf^T v (VectorData from Edges{Primal})."""
+    if isinstance(out, ScalarData) and isinstance(field, Nodes):
+        _check_len(out, cache.N, "ScalarData")
+        return _run("ilh_interpolate", cache, out, field, field.ilh_kind)
+    if isinstance(out, VectorData) and isinstance(field, Edges):
+        _check_len(out, 2 * cache.N, "VectorData")
+        return _run("ilh_interpolate", cache, out, field, L.EDGES_PRIMAL)
+    raise TypeError("interpolate: no method for (%s, %s)" % (type(out).__name__, type(field).__name__))
+
+
+def regularize_normal(q, f, cache):
+    _check_len(f, cache.N, "ScalarData")
+    return _run("ilh_regularize_normal", cache, q, f)
+
+
+def normal_interpolate(vn, q, cache):
+    _check_len(vn, cache.N, "ScalarData")
+    return _run("ilh_normal_interpolate", cache, vn, q)
+
+
+def regularize_normal_cross(q, f, cache):
+    _check_len(f, cache.N, "ScalarData")
+    return _run("ilh_regularize_normal_cross", cache, q, f)
+
+
+def normal_cross_interpolate(vn, q, cache):
+    _check_len(vn, cache.N, "ScalarData")
+    return _run("ilh_normal_cross_interpolate", cache, vn, q)
+
+
+def surface_divergence(theta, f, cache):
+    _check_len(f, cache.N, "ScalarData")
+    return _run("ilh_surface_divergence", cache, theta, f)
+
+
+def surface_grad(vn, phi, cache):
+    _check_len(vn, cache.N, "ScalarData")
+    return _run("ilh_surface_grad", cache, vn, phi)
+
+
+def surface_curl(out, inp, cache):
+    """w = C_s^T f (Nodes{Dual} from ScalarData) or vn = C_s s (ScalarData from Nodes{Dual})."""
+    if isinstance(out, Nodes) and out.kind == Dual:
+        return _run("ilh_surface_curl", cache, out, inp)
+    if isinstance(out, ScalarData) and isinstance(inp, Nodes) and inp.kind == Dual:
+        return _run("ilh_surface_curl_adj", cache, out, inp)
+    raise TypeError("surface_curl: no method for (%s, %s)" % (type(out).__name__, type(inp).__name__))
+
+
+def surface_curl_cross(out, inp, cache):
+    if isinstance(out, Nodes) and out.kind == Dual:
+        return _run("ilh_surface_curl_cross", cache, out, inp)
+    if isinstance(out, ScalarData) and isinstance(inp, Nodes) and inp.kind == Dual:
+        return _run("ilh_surface_curl_cross_adj", cache, out, inp)
+    raise TypeError("surface_curl_cross: no method for (%s, %s)" % (type(out).__name__, type(inp).__name__))
+
+
+def surface_grad_cross(vn, phi, cache):
+    _check_len(vn, cache.N, "ScalarData")
+    return _run("ilh_surface_grad_cross", cache, vn, phi)
+
+
+def divergence(p, q, cache):
+    return _run("ilh_divergence", cache, p, q)
+
+
+def grad(q, p, cache):
+    return _run("ilh_grad", cache, q, p)
+
+
+def curl(out, inp, cache):
+    if isinstance(out, Edges) and isinstance(inp, Nodes) and inp.kind == Dual:
+        return _run("ilh_curl_nodes_to_edges", cache, out, inp)
+    if isinstance(out, Nodes) and out.kind == Dual and isinstance(inp, Edges):
+        return _run("ilh_curl_edges_to_nodes", cache, out, inp)
+    raise TypeError("curl: no method for (%s, %s)" % (type(out).__name__, type(inp).__name__))
+
+
+def laplacian(w, s, cache):
+    if w.kind != s.kind:
+        raise TypeError("laplacian: grid types differ")
+    return _run("ilh_laplacian", cache, w, s, w.ilh_kind)
+
+
+def inverse_laplacian(w, cache, rhs=None):
+    """inverse_laplacian!(w, cache) in place, or inverse_laplacian!(sol, rhs, cache) when rhs is given."""
+    src = w if rhs is None else rhs
+    return _run("ilh_inverse_laplacian", cache, w, src, w.ilh_kind)
+
+
+def _schur(cache, kind, scale):
+    A = np.zeros(cache.N * cache.N)
+    L.call("ilh_schur_build", cache.h, kind, C.c_double(scale), _ptr(A), L.MEM_HOST)
+    return A.reshape((cache.N, cache.N), order="F")
+
+
+def create_RTLinvR(cache, scale=1.0):
+    return _schur(cache, L.SCHUR_RTLINVR, scale)
+
+
+def create_CLinvCT(cache, scale=1.0):
+    return _schur(cache, L.SCHUR_CLINVCT, scale)
+
+
+def create_GLinvD(cache, scale=1.0):
+    return _schur(cache, L.SCHUR_GLINVD, scale)
+
+
+def create_nRTRn(cache, scale=1.0):
+    return _schur(cache, L.SCHUR_NRTRN, scale)
+
+
+# -------------------------------------------------------------- problems
+def _vec(a, n):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
+    if a.size != n:
+        raise L.DimensionMismatch("surface data of length %d, expected %d" % (a.size, n))
+    return a
+
+
+class DirichletPoisson:
+    """The Dirichlet Poisson problem of test/literate/problems.jl:243-298:
+    S = create_RTLinvR is built and factored once per geometry; ``solve``
+    returns the grid field f and the (unfiltered) surface strength s."""
+
+    def __init__(self, cache):
+        self.cache = cache
+
+    def solve(self, fbplus, fbminus=None):
+        c = self.cache
+        fp = _vec(fbplus, c.N)
+        fm = None if fbminus is None else _vec(fbminus, c.N)
+        f = Nodes(Primal, c.g)
+        s = ScalarData(c.N)
+        L.call("ilh_dirichlet_solve", c.h, _ptr(fp), None if fm is None else _ptr(fm), _ptr(f.flat()),
+               _ptr(s.flat()), L.MEM_HOST)
+        return f, s
+
+    def solve_device(self, fp_dev, fm_dev, f_dev, s_dev):
+        """Device-resident form: all four arguments are DeviceArrays (fm may be None); asynchronous."""
+        L.call("ilh_dirichlet_solve", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,
+               f_dev.ptr, s_dev.ptr, L.MEM_DEVICE)
+
+    def profile(self, fp_dev, fm_dev, f_dev, s_dev, reps):
+        st = (C.c_double * 4)()
+        tot = C.c_double()
+        L.call("ilh_profile_dirichlet", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,
+               f_dev.ptr, s_dev.ptr, int(reps), st, C.byref(tot))
+        return list(st), tot.value
+
+
+class NeumannPoisson:
+    """The Neumann Poisson problem of test/literate/neumann.jl:80-142: returns
+    (f, [ϕ], s, [ψ]) with S = create_CLinvCT factored once per geometry."""
+
+    def __init__(self, cache):
+        self.cache = cache
+
+    def solve(self, vnplus, vnminus=None):
+        c = self.cache
+        vp = _vec(vnplus, c.N)
+        vm = None if vnminus is None else _vec(vnminus, c.N)
+        f = Nodes(Primal, c.g)
+        s = Nodes(Dual, c.g)
+        df = ScalarData(c.N)
+        ds = ScalarData(c.N)
+        L.call("ilh_neumann_solve", c.h, _ptr(vp), None if vm is None else _ptr(vm), _ptr(f.flat()),
+               _ptr(df.flat()), _ptr(s.flat()), _ptr(ds.flat()), L.MEM_HOST)
+        return f, df, s, ds
+
+
+class SchurFactor:
+    """Cached LU factor of one Schur complement kind; ``solve(b)`` = S \\ b."""
+
+    def __init__(self, cache, kind=L.SCHUR_RTLINVR):
+        self.cache, self.kind = cache, kind
+        L.call("ilh_schur_factor", cache.h, kind)
+
+    def solve(self, b):
+        b = _vec(b, self.cache.N)
+        x = np.zeros_like(b)
+        L.call("ilh_schur_solve", self.cache.h, self.kind, _ptr(x), _ptr(b), L.MEM_HOST)
+        return x

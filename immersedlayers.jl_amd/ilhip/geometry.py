@@ -1,0 +1,172 @@
+"""Seeded synthetic geometry and grids for the immersed-layer workloads.
+
+Surface geometry is an explicit input of both the HIP library and its checker:
+point coordinates, unit normals and panel lengths ``ds`` (the reference takes
+them from RigidBodyTools' ``points``/``normals``/``areas``, src/tools.jl:16-38).
+Circles follow RigidBodyTools' ``Circle(a, Δs)`` convention: ``N = ⌊2πa/Δs⌋``
+points at ``θ_p = 2π(p-1)/N``, radial normals and ``ds = 2a·tan(π/N)``.
+Other curves are sampled at equal arclength from their analytic form.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class GridSpec:
+    """CartesianGrids ``PhysicalGrid`` essentials: ``size(g) = (NX, NY)`` dual-node
+    counts, ``origin(g) = (I0x, I0y)`` (1-based) and the cell size ``dx``."""
+    NX: int
+    NY: int
+    I0x: int
+    I0y: int
+    dx: float
+
+    def primal_shape(self):
+        return (self.NX - 1, self.NY - 1)
+
+    def dual_shape(self):
+        return (self.NX, self.NY)
+
+    def x_primal(self):
+        return (np.arange(self.NX - 1) + 1 - self.I0x) * self.dx
+
+    def y_primal(self):
+        return (np.arange(self.NY - 1) + 1 - self.I0y) * self.dx
+
+    def x_dual(self):
+        return (np.arange(self.NX) + 0.5 - self.I0x) * self.dx
+
+    def y_dual(self):
+        return (np.arange(self.NY) + 0.5 - self.I0y) * self.dx
+
+
+@dataclass
+class Surface:
+    X: np.ndarray
+    Y: np.ndarray
+    nx: np.ndarray
+    ny: np.ndarray
+    ds: np.ndarray
+    body: np.ndarray          # body index of each point
+
+    @property
+    def N(self):
+        return len(self.X)
+
+    def theta(self, cx=0.0, cy=0.0):
+        return np.arctan2(self.Y - cy, self.X - cx)
+
+
+def concat(surfs):
+    cat = lambda k: np.concatenate([getattr(s, k) for s in surfs])
+    body = np.concatenate([np.full(s.N, i) for i, s in enumerate(surfs)])
+    return Surface(cat("X"), cat("Y"), cat("nx"), cat("ny"), cat("ds"), body)
+
+
+def circle(a, ds_target, cx=0.0, cy=0.0):
+    N = int(np.floor(2 * np.pi * a / ds_target))
+    th = 2 * np.pi * np.arange(N) / N
+    ds = np.full(N, 2 * a * np.tan(np.pi / N))
+    return Surface(cx + a * np.cos(th), cy + a * np.sin(th), np.cos(th), np.sin(th), ds, np.zeros(N, int))
+
+
+def _sample_curve(rfun, drfun, N, cx, cy, fine=200000):
+    """Equal-arclength sampling of a closed curve given as r(t) = (x(t), y(t))."""
+    t = np.linspace(0, 2 * np.pi, fine + 1)
+    dx, dy = drfun(t)
+    sp = np.hypot(dx, dy)
+    s = np.concatenate([[0], np.cumsum(0.5 * (sp[1:] + sp[:-1]) * np.diff(t))])
+    L = s[-1]
+    s_pts = L * np.arange(N) / N
+    s_mid = L * (np.arange(N) + 0.5) / N
+    tp = np.interp(s_pts, s, t)
+    x, y = rfun(tp)
+    dxp, dyp = drfun(tp)
+    nrm = np.hypot(dxp, dyp)
+    # outward normal for a counter-clockwise curve: (dy, -dx)/|r'|
+    nx, ny = dyp / nrm, -dxp / nrm
+    ds = np.full(N, L / N)
+    del s_mid
+    return Surface(cx + x, cy + y, nx, ny, ds, np.zeros(N, int)), L
+
+
+def ellipse(a, b, ds_target, cx=0.0, cy=0.0):
+    rf = lambda t: (a * np.cos(t), b * np.sin(t))
+    drf = lambda t: (-a * np.sin(t), b * np.cos(t))
+    _, L = _sample_curve(rf, drf, 16, 0, 0)
+    N = max(8, int(np.floor(L / ds_target)))
+    return _sample_curve(rf, drf, N, cx, cy)[0]
+
+
+def star(r0, bk, ds_target, cx=0.0, cy=0.0, N=None):
+    """r(θ) = r0 (1 + Σ_{k=2..} b_k cos kθ)."""
+    ks = np.arange(2, 2 + len(bk))
+    bk = np.asarray(bk)
+
+    def R(t):
+        return r0 * (1 + np.sum(bk[:, None] * np.cos(np.outer(ks, t)), axis=0))
+
+    def dR(t):
+        return r0 * (-np.sum((bk * ks)[:, None] * np.sin(np.outer(ks, t)), axis=0))
+
+    rf = lambda t: (R(t) * np.cos(t), R(t) * np.sin(t))
+    drf = lambda t: (dR(t) * np.cos(t) - R(t) * np.sin(t), dR(t) * np.sin(t) + R(t) * np.cos(t))
+    if N is None:
+        _, L = _sample_curve(rf, drf, 16, 0, 0)
+        N = max(8, int(np.floor(L / ds_target)))
+    return _sample_curve(rf, drf, N, cx, cy)[0]
+
+
+def fourier_data(theta, rng, K=8):
+    """f_b(θ) = Σ_{k=1..K} a_k cos(kθ + φ_k), a_k ~ N(0, 1/k²), φ_k ~ U(0, 2π)."""
+    k = np.arange(1, K + 1)
+    a = rng.standard_normal(K) / k
+    ph = rng.uniform(0, 2 * np.pi, K)
+    return np.sum(a[:, None] * np.cos(np.outer(k, theta) + ph[:, None]), axis=0), a, ph
+
+
+# ----------------------------------------------------------------- workloads
+def config_plain(seed=1):
+    """Plain FP64 CPU-runnable case: Δx=0.02 on [-2,2]² (NX=NY=206, I0=103),
+    circle r=0.5 at the origin, points at 1.4Δx, Dirichlet f_b on the exterior."""
+    g = GridSpec(206, 206, 103, 103, 0.02)
+    s = circle(0.5, 1.4 * g.dx)
+    rng = np.random.default_rng(seed)
+    fb, a, ph = fourier_data(s.theta(), rng)
+    return g, s, fb
+
+
+def config_microbench(seed=2, npts_per_curve=2048):
+    """2048² Nodes{Primal} on [-2,2]² with 4 random star curves of 2048 points."""
+    dx = 4.0 / 2048
+    g = GridSpec(2049, 2049, 1025, 1025, dx)
+    rng = np.random.default_rng(seed)
+    surfs = []
+    for _ in range(4):
+        r0 = rng.uniform(0.2, 0.4)
+        bk = rng.uniform(-0.1, 0.1, 5)
+        c = rng.uniform(-1.3, 1.3, 2)
+        surfs.append(star(r0, bk, None, c[0], c[1], N=npts_per_curve))
+    return g, concat(surfs)
+
+
+def config_headline(seed=5, spacing=1.0):
+    """1024² Nodes{Primal} on [-2,2]² (Δx=4/1024): an ellipse of aspect 2 and a
+    random star curve, about 1000 points each at ``spacing``·Δx."""
+    dx = 4.0 / 1024
+    g = GridSpec(1025, 1025, 513, 513, dx)
+    rng = np.random.default_rng(seed)
+    e = ellipse(0.8, 0.4, spacing * dx, -0.95, 0.0)
+    bk = rng.uniform(-0.1, 0.1, 5)
+    st = star(0.55, bk, spacing * dx, 0.95, 0.05)
+    s = concat([e, st])
+    fb, _, _ = fourier_data(s.theta(), rng)
+    return g, s, fb
+
+
+def square_grid(n_primal, L=4.0, margin=0):
+    """Grid whose Nodes{Primal} is n_primal² covering about [-L/2, L/2]²."""
+    dx = L / n_primal
+    I0 = n_primal // 2 + 1
+    return GridSpec(n_primal + 1, n_primal + 1, I0, I0, dx)

@@ -1,0 +1,173 @@
+/*
+ * immersedhip.h -- C ABI of libimmersedhip.so, the MI355X (gfx950) immersed-layer
+ * hot path: DDF regularization/interpolation, staggered-grid stencils, the
+ * lattice-Green's-function inverse Laplacian and the constrained (Schur
+ * complement) Dirichlet/Neumann Poisson solves of ImmersedLayers.jl.
+ *
+ * Every reference citation below is a path inside JuliaIBPM/ImmersedLayers.jl.
+ *
+ * Conventions
+ *  - Grid: (NX, NY) = size(g) (dual-node counts), (I0x, I0y) = origin(g) (1-based,
+ *    as CartesianGrids reports it), dx = cellsize(g).
+ *  - Layouts are the reference's, column-major (x fastest):
+ *      Nodes{Primal} (NX-1)x(NY-1), Nodes{Dual} NX x NY,
+ *      Edges{Primal} = [u (NX x (NY-1)); v ((NX-1) x NY)] in one buffer,
+ *      ScalarData N, VectorData [u(N); v(N)].
+ *  - In-place, output first, as the reference's `op!(out, in, cache)`.
+ *  - `mem` says where array pointers live: ILH_MEM_HOST (synchronous copy in and
+ *    out on the plan's stream) or ILH_MEM_DEVICE (device pointers, the call is
+ *    asynchronous on the ctx stream; synchronize with ilh_ctx_synchronize).
+ *  - Scaling is GridScaling (the reference default, src/ImmersedLayers.jl:81).
+ *  - Calls on one plan are not re-entrant (the reference's cache owns scratch,
+ *    src/cache.jl:170-178); distinct ctxs are independent.
+ *  - Return value: 0 = ILH_OK, negative = error; ilh_last_error() gives the
+ *    message of the last failing call on this thread.
+ */
+#ifndef IMMERSEDHIP_H
+#define IMMERSEDHIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ilh_ctx ilh_ctx;
+typedef struct ilh_plan ilh_plan;
+
+typedef struct {
+    int NX, NY;     /* size(g) */
+    int I0x, I0y;   /* origin(g) */
+    double dx;      /* cellsize(g) */
+} ilh_grid;
+
+enum {
+    ILH_OK = 0,
+    ILH_ERR_ARG = -1,          /* bad argument / unsupported option */
+    ILH_ERR_DIM = -2,          /* DimensionMismatch (test/surface_ops.jl:524) */
+    ILH_ERR_NONFINITE = -3,    /* NaN/Inf in an input */
+    ILH_ERR_SINGULAR = -4,     /* zero pivot in the Schur factorisation */
+    ILH_ERR_HIP = -5,          /* HIP runtime error */
+    ILH_ERR_OOM = -6,          /* device allocation failed */
+    ILH_ERR_RANGE = -7,        /* surface point support leaves the grid interior */
+    ILH_ERR_STATE = -8         /* call order (e.g. solve before factor) */
+};
+
+enum { ILH_MEM_HOST = 0, ILH_MEM_DEVICE = 1 };
+/* DDF kinds (CartesianGrids DDF types; default Yang3, src/ImmersedLayers.jl:83) */
+enum { ILH_DDF_YANG3 = 0, ILH_DDF_ROMA = 1, ILH_DDF_M4PRIME = 2, ILH_DDF_WITCHHAT = 3 };
+enum { ILH_SCALING_GRID = 0 };
+/* grid data kinds */
+enum { ILH_NODES_PRIMAL = 0, ILH_NODES_DUAL = 1, ILH_EDGES_PRIMAL = 2 };
+/* Schur-complement kinds (src/matrix_operators.jl) */
+enum { ILH_SCHUR_RTLINVR = 0, ILH_SCHUR_CLINVCT = 1, ILH_SCHUR_GLINVD = 2, ILH_SCHUR_NRTRN = 3 };
+
+const char* ilh_last_error(void);
+const char* ilh_version(void);
+
+/* ---- context: one HIP device + one stream ---- */
+int ilh_ctx_create(int device, ilh_ctx** out);
+int ilh_ctx_destroy(ilh_ctx* ctx);
+int ilh_ctx_synchronize(ilh_ctx* ctx);
+/* device memory helpers for callers without their own allocator */
+int ilh_malloc(ilh_ctx* ctx, size_t bytes, void** dptr);
+int ilh_free(ilh_ctx* ctx, void* dptr);
+int ilh_memcpy_h2d(ilh_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ilh_memcpy_d2h(ilh_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- plan = the reference's BasicILMCache of a scalar problem ----
+ * Replaces SurfaceScalarCache(bl, ds, nrm, g; ddftype, scaling)
+ * (src/cache.jl:164-182, _surfacecache src/cache.jl:232-261, the LGF plan of
+ * _get_laplacian src/cache.jl:321-324).  X, Y, nx, ny, ds are host arrays of
+ * length npts (points(cache), normals(cache), areas(cache), src/tools.jl:16-38). */
+int ilh_plan_create(ilh_ctx* ctx, const ilh_grid* grid, int npts,
+                    const double* X, const double* Y, const double* nx, const double* ny,
+                    const double* ds, int ddf, int scaling, ilh_plan** out);
+/* New geometry, same grid and point count: update_system! (src/system.jl:26-50).
+ * Drops every cached Schur factor. */
+int ilh_plan_update_points(ilh_plan* plan, const double* X, const double* Y,
+                           const double* nx, const double* ny, const double* ds);
+int ilh_plan_destroy(ilh_plan* plan);
+int ilh_plan_npts(const ilh_plan* plan);
+
+/* ---- surface <-> grid (A4-A7) ---- */
+/* regularize!(s::Nodes{Primal|Dual}, f::ScalarData) = Rc f, or
+   regularize!(v::Edges{Primal}, vb::VectorData) = Rf vb
+   (src/surface_operators.jl:13,25,36,38-46) */
+int ilh_regularize(ilh_plan* plan, int grid_kind, double* out, const double* f, int mem);
+/* interpolate!(f::ScalarData, s::Nodes{...}) = Rc^T s / interpolate!(vb::VectorData, v::Edges{Primal})
+   (src/surface_operators.jl:56,67,78,80-88) */
+int ilh_interpolate(ilh_plan* plan, int grid_kind, double* out, const double* grid, int mem);
+/* regularize_normal!(q::Edges{Primal}, f::ScalarData) = Rf (n o f)  (src/surface_operators.jl:98-103) */
+int ilh_regularize_normal(ilh_plan* plan, double* q_edges, const double* f, int mem);
+/* normal_interpolate!(vn::ScalarData, q::Edges{Primal}) = n . Rf^T q  (src/surface_operators.jl:228-233) */
+int ilh_normal_interpolate(ilh_plan* plan, double* vn, const double* q_edges, int mem);
+/* regularize_normal_cross!(q, f) = Rf (n x f e_z)  (src/surface_operators.jl:150-162) */
+int ilh_regularize_normal_cross(ilh_plan* plan, double* q_edges, const double* f, int mem);
+/* normal_cross_interpolate!(wn, q) = e_z . (n x Rf^T q)  (src/surface_operators.jl:277-290) */
+int ilh_normal_cross_interpolate(ilh_plan* plan, double* vn, const double* q_edges, int mem);
+
+/* ---- surface calculus (A11) ---- */
+/* surface_divergence!(θ::Nodes{Primal}, f)  (src/surface_operators.jl:530-547) */
+int ilh_surface_divergence(ilh_plan* plan, double* theta, const double* f, int mem);
+/* surface_grad!(vn, ϕ::Nodes{Primal})  (src/surface_operators.jl:604-620) */
+int ilh_surface_grad(ilh_plan* plan, double* vn, const double* phi, int mem);
+/* surface_curl!(w::Nodes{Dual}, f)  (src/surface_operators.jl:357-375) */
+int ilh_surface_curl(ilh_plan* plan, double* w, const double* f, int mem);
+/* surface_curl!(vn, s::Nodes{Dual})  (src/surface_operators.jl:412-429) */
+int ilh_surface_curl_adj(ilh_plan* plan, double* vn, const double* s, int mem);
+/* surface_curl_cross!(w::Nodes{Dual}, f)  (src/surface_operators.jl:467-484) */
+int ilh_surface_curl_cross(ilh_plan* plan, double* w, const double* f, int mem);
+/* surface_curl_cross!(γ, s::Nodes{Dual})  (src/surface_operators.jl:498-515) */
+int ilh_surface_curl_cross_adj(ilh_plan* plan, double* vn, const double* s, int mem);
+/* surface_grad_cross!(γ, ϕ::Nodes{Primal})  (src/surface_operators.jl:708-725) */
+int ilh_surface_grad_cross(ilh_plan* plan, double* vn, const double* phi, int mem);
+
+/* ---- grid stencils (A8), GridScaling ---- */
+/* divergence!(p::Nodes{Primal}, q::Edges{Primal})  (src/grid_operators.jl:25-31) */
+int ilh_divergence(ilh_plan* plan, double* p, const double* q_edges, int mem);
+/* grad!(q::Edges{Primal}, p::Nodes{Primal})  (src/grid_operators.jl:45-51) */
+int ilh_grad(ilh_plan* plan, double* q_edges, const double* p, int mem);
+/* curl!(q::Edges{Primal}, s::Nodes{Dual})  (src/grid_operators.jl:79-87) */
+int ilh_curl_nodes_to_edges(ilh_plan* plan, double* q_edges, const double* s, int mem);
+/* curl!(w::Nodes{Dual}, q::Edges{Primal})  (src/grid_operators.jl:97-105) */
+int ilh_curl_edges_to_nodes(ilh_plan* plan, double* w, const double* q_edges, int mem);
+/* laplacian!(w, s) on Nodes{Primal|Dual}  (src/grid_operators.jl:114-134) */
+int ilh_laplacian(ilh_plan* plan, int grid_kind, double* out, const double* in, int mem);
+/* inverse_laplacian!(sol, rhs) / in place when out == in  (src/grid_operators.jl:144-179) */
+int ilh_inverse_laplacian(ilh_plan* plan, int grid_kind, double* out, const double* in, int mem);
+
+/* ---- Schur complements (A10-A12) ---- */
+/* create_RTLinvR / create_CLinvCT / create_GLinvD / create_nRTRn with `scale`
+   (src/matrix_operators.jl:9-30,40-61,135-155,225-244).  `out` is N x N
+   column-major. */
+int ilh_schur_build(ilh_plan* plan, int kind, double scale, double* out, int mem);
+/* Factor the plan's own S of `kind` (LU with partial pivoting, kept on the device).
+   The reference re-factorises inside every `S\s` (test/literate/problems.jl:291). */
+int ilh_schur_factor(ilh_plan* plan, int kind);
+/* x = S \ b with the cached factor of `kind` */
+int ilh_schur_solve(ilh_plan* plan, int kind, double* x, const double* b, int mem);
+
+/* ---- constrained Poisson composites (A13, A14) ---- */
+/* Dirichlet solve of test/literate/problems.jl:259-298 (saddle system of
+   test/literate/dirichlet.jl:28-44): fb_plus/fb_minus = exterior/interior data
+   (prescribed_surface_jump!/average!, src/bc.jl:20-24,47-51); fb_minus may be
+   NULL (zero).  Outputs f (Nodes{Primal}) and the unfiltered strength s.
+   The first call on a geometry builds and factors S. */
+int ilh_dirichlet_solve(ilh_plan* plan, const double* fb_plus, const double* fb_minus,
+                        double* f, double* s, int mem);
+/* Neumann solve of test/literate/neumann.jl:101-142: outputs f (Nodes{Primal}),
+   df = [ϕ], s (Nodes{Dual} streamfunction) and ds = [ψ]. vn_minus may be NULL. */
+int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_minus,
+                      double* f, double* df, double* s, double* ds, int mem);
+
+/* ---- measurement helpers (no reference counterpart) ---- */
+/* Average device time per call, in ms, of `reps` repeated Dirichlet solves on
+   device buffers, split by stage: stage_ms[0]=surface matvec, [1]=row FFT,
+   [2]=column FFT+multiply, [3]=inverse row FFT (each from HIP events on the
+   plan's stream).  Returns the total per solve in *total_ms. */
+int ilh_profile_dirichlet(ilh_plan* plan, const double* fb_plus_dev, const double* fb_minus_dev,
+                          double* f_dev, double* s_dev, int reps, double* stage_ms, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
