@@ -73,8 +73,10 @@ struct SolveCache {
     {
         free_bins(src1.bins);
         free_bins(src2.bins);
-        if (src1.pairs) (void)hipFree(src1.pairs);
-        if (src2.pairs) (void)hipFree(src2.pairs);
+        for (SourceSet* ss : {&src1, &src2}) {
+            if (ss->pairs) (void)hipFree(ss->pairs);
+            if (ss->slot_of_row) (void)hipFree(ss->slot_of_row);
+        }
     }
 };
 
@@ -616,9 +618,10 @@ SolveCache* prepare_dirichlet(ilh_plan* pl)
             k_rhs_dir<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, st>>>(B.as<double>(), N);
             ILH_HIP(hipGetLastError());
             lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 2 * N);
+            const int ld = solve_ld(N);
             for (int k = 0; k < 2; ++k) {
-                sc->M[k].ensure(sizeof(double) * (size_t)N * N);
-                transpose(st, B.as<double>() + (size_t)k * N * N, sc->M[k].as<double>(), N, N);
+                sc->M[k].ensure(sizeof(double) * (size_t)N * ld);
+                transpose(st, B.as<double>() + (size_t)k * N * N, sc->M[k].as<double>(), N, N, ld);
             }
             merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, sc->terms1, sc->src1);
             ILH_HIP(hipStreamSynchronize(st));
@@ -651,12 +654,14 @@ SolveCache* prepare_neumann(ilh_plan* pl)
             assemble(pl, OP_GXS, OP_R_PRIMAL, GK_PRIMAL, dx2, false, B.as<double>() + 2 * NN);  // K_XR
             ILH_HIP(hipGetLastError());
             lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 3 * N);
+            const int ld = solve_ld(N);
+            const size_t NL = (size_t)N * ld;
             for (int k = 0; k < 3; ++k) {
-                sc->M[k].ensure(sizeof(double) * NN);
-                transpose(st, B.as<double>() + (size_t)k * NN, sc->M[k].as<double>(), N, N);
+                sc->M[k].ensure(sizeof(double) * NL);
+                transpose(st, B.as<double>() + (size_t)k * NN, sc->M[k].as<double>(), N, N, ld);
             }
-            sc->M[3].ensure(sizeof(double) * NN);
-            k_neg<<<(unsigned)((NN + 255) / 256), 256, 0, st>>>(sc->M[2].as<double>(), sc->M[3].as<double>(), NN);
+            sc->M[3].ensure(sizeof(double) * NL);
+            k_neg<<<(unsigned)((NL + 255) / 256), 256, 0, st>>>(sc->M[2].as<double>(), sc->M[3].as<double>(), NL);
             ILH_HIP(hipGetLastError());
             merge_sources(pl, OP_DS, OP_R_PRIMAL, GK_PRIMAL, sc->terms1, sc->src1);
             merge_sources(pl, OP_CST, OP_CXT, GK_DUAL, sc->terms2, sc->src2);
@@ -741,8 +746,8 @@ extern "C" int ilh_dirichlet_solve(ilh_plan* pl, const double* fp, const double*
         if (N == 0) {
             fill_zero(st, d_f, ng);
         } else {
-            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), d_fp, d_fm, sc->amp1.as<double>(), N);
-            ILH_HIP(hipMemcpyAsync(d_s, sc->amp1.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), solve_ld(N), d_fp, d_fm,
+                          sc->amp1.as<double>(), d_s, N);
             conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), d_f);
         }
         io.back(f, d_f, ng);
@@ -771,7 +776,7 @@ extern "C" int ilh_neumann_solve(ilh_plan* pl, const double* vp, const double* v
             fill_zero(st, d_s, nd);
         } else {
             neumann_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), sc->M[2].as<double>(), sc->M[3].as<double>(),
-                        d_vp, d_vm, sc->amp1.as<double>(), sc->amp2.as<double>(), d_df, d_ds, N);
+                        solve_ld(N), d_vp, d_vm, sc->amp1.as<double>(), sc->amp2.as<double>(), d_df, d_ds, N);
             conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), d_f);
             conv_sources(pl, GK_DUAL, sc->src2, sc->amp2.as<double>(), d_s);
         }
@@ -788,31 +793,34 @@ extern "C" int ilh_profile_dirichlet(ilh_plan* pl, const double* fp, const doubl
 {
     return guarded([&] {
         Io io(pl, ILH_MEM_DEVICE);
-        if (reps < 1) fail(ILH_ERR_ARG, "reps must be >= 1");
+        if (reps < 1 || reps > 100000) fail(ILH_ERR_ARG, "reps must be in [1, 100000]");
         SolveCache* sc = prepare_dirichlet(pl);
         const int N = pl->N;
         hipStream_t st = pl->ctx->stream;
-        hipEvent_t ev[5];
+        // five events per solve, recorded back to back; one synchronisation at the end
+        std::vector<hipEvent_t> ev((size_t)5 * reps);
         for (auto& e : ev) ILH_HIP(hipEventCreate(&e));
-        double acc[4] = {0, 0, 0, 0};
-        double tot = 0.0;
         for (int r = 0; r < reps; ++r) {
-            ILH_HIP(hipEventRecord(ev[0], st));
-            dirichlet_amp(st, sc->M[0].as<double>(), fm ? sc->M[1].as<double>() : nullptr, fp, fm,
-                          sc->amp1.as<double>(), N);
-            ILH_HIP(hipMemcpyAsync(s, sc->amp1.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-            ILH_HIP(hipEventRecord(ev[1], st));
-            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f, ev + 2);
-            ILH_HIP(hipEventRecord(ev[4], st));
-            ILH_HIP(hipEventSynchronize(ev[4]));
-            float ms;
+            hipEvent_t* e = ev.data() + (size_t)5 * r;
+            ILH_HIP(hipEventRecord(e[0], st));
+            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), solve_ld(N), fp, fm,
+                          sc->amp1.as<double>(), s, N);
+            ILH_HIP(hipEventRecord(e[1], st));
+            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f, e + 2);
+            ILH_HIP(hipEventRecord(e[4], st));
+        }
+        ILH_HIP(hipEventSynchronize(ev[(size_t)5 * reps - 1]));
+        double acc[4] = {0, 0, 0, 0};
+        float ms = 0.f;
+        for (int r = 0; r < reps; ++r) {
+            hipEvent_t* e = ev.data() + (size_t)5 * r;
             for (int k = 0; k < 4; ++k) {
-                ILH_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                ILH_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
                 acc[k] += ms;
             }
-            ILH_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
-            tot += ms;
         }
+        ILH_HIP(hipEventElapsedTime(&ms, ev[0], ev[(size_t)5 * reps - 1]));
+        const double tot = ms;
         for (auto& e : ev) (void)hipEventDestroy(e);
         for (int k = 0; k < 4; ++k)
             if (stage_ms) stage_ms[k] = acc[k] / reps;

@@ -367,7 +367,7 @@ void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, doubl
 }
 
 // ------------------------------------------------------------- helpers
-__global__ void k_transpose(const double* __restrict__ A, double* __restrict__ At, int rows, int cols)
+__global__ void k_transpose(const double* __restrict__ A, double* __restrict__ At, int rows, int cols, int ldt)
 {
     __shared__ double t[32][33];
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
@@ -378,78 +378,107 @@ __global__ void k_transpose(const double* __restrict__ A, double* __restrict__ A
     __syncthreads();
     for (int k = threadIdx.y; k < 32; k += 8) {
         const int c = by + threadIdx.x, r = bx + k;
-        if (r < rows && c < cols) At[c + (size_t)cols * r] = t[threadIdx.x][k];
+        if (r < rows && c < ldt) At[c + (size_t)ldt * r] = c < cols ? t[threadIdx.x][k] : 0.0;
     }
 }
 
-// A: rows x cols column-major -> At: cols x rows column-major (= A row-major)
-void transpose(hipStream_t st, const double* A, double* At, int rows, int cols)
+// A: rows x cols column-major -> At: row-major rows x cols with leading
+// dimension ldt >= cols (padding columns zeroed)
+void transpose(hipStream_t st, const double* A, double* At, int rows, int cols, int ldt)
 {
-    k_transpose<<<dim3((rows + 31) / 32, (cols + 31) / 32), dim3(32, 8), 0, st>>>(A, At, rows, cols);
+    k_transpose<<<dim3((rows + 31) / 32, (ldt + 31) / 32), dim3(32, 8), 0, st>>>(A, At, rows, cols, ldt);
     ILH_HIP(hipGetLastError());
 }
 
-// One wave per row: y[i] = alpha * dot(M[i, :], x) (+ y[i]).  M row-major n x m.
+// One wave per row: y[i] = alpha * dot(M[i, :], x) (+ y[i]).  M row-major n x m
+// with leading dimension ld (a multiple of 2, rows 16-byte aligned).
 __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const double* __restrict__ x, int m, int lane)
 {
-    double s = 0.0;
-    for (int k = lane; k < m; k += 64) s += row[k] * x[k];
+    if (((uintptr_t)x & 15) != 0) {   // caller's vector not 16-byte aligned: plain loads
+        double s = 0.0;
+        for (int k = lane; k < m; k += 64) s += row[k] * x[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        return s;
+    }
+    // 16-byte loads, four independent partial sums per lane
+    const double2* r2 = (const double2*)row;
+    const double2* x2 = (const double2*)x;
+    const int m2 = m >> 1;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = lane;
+    for (; k + 192 < m2; k += 256) {
+        const double2 a0 = r2[k], a1 = r2[k + 64], a2 = r2[k + 128], a3 = r2[k + 192];
+        const double2 b0 = x2[k], b1 = x2[k + 64], b2 = x2[k + 128], b3 = x2[k + 192];
+        s0 += a0.x * b0.x + a0.y * b0.y;
+        s1 += a1.x * b1.x + a1.y * b1.y;
+        s2 += a2.x * b2.x + a2.y * b2.y;
+        s3 += a3.x * b3.x + a3.y * b3.y;
+    }
+    for (; k < m2; k += 64) {
+        const double2 a0 = r2[k], b0 = x2[k];
+        s0 += a0.x * b0.x + a0.y * b0.y;
+    }
+    if ((m & 1) && lane == 0) s1 += row[m - 1] * x[m - 1];
+    double s = (s0 + s1) + (s2 + s3);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
 }
 
-__global__ void k_matvec_rows(const double* __restrict__ M, const double* __restrict__ x, double* __restrict__ y,
+__global__ void k_matvec_rows(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
                               int n, int m, double alpha, int accumulate)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
-    const double s = wave_dot(M + (size_t)m * i, x, m, lane);
+    const double s = wave_dot(M + (size_t)ld * i, x, m, lane);
     if (lane == 0) y[i] = alpha * s + (accumulate ? y[i] : 0.0);
 }
 
-void matvec_rows(hipStream_t st, const double* M, const double* x, double* y, int n, int m, double alpha, int acc)
+void matvec_rows(hipStream_t st, const double* M, int ld, const double* x, double* y, int n, int m, double alpha,
+                 int acc)
 {
     if (n <= 0) return;
-    k_matvec_rows<<<(n + 3) / 4, 256, 0, st>>>(M, x, y, n, m, alpha, acc);
+    k_matvec_rows<<<(n + 3) / 4, 256, 0, st>>>(M, ld, x, y, n, m, alpha, acc);
     ILH_HIP(hipGetLastError());
 }
 
-// Dirichlet surface step: s = M+ f+ (+ M- f-),  amp = [s; f+ - f-]
-__global__ void k_dir_amp(const double* __restrict__ Mp, const double* __restrict__ Mm, const double* __restrict__ fp,
-                          const double* __restrict__ fm, double* __restrict__ amp, int n)
+// Dirichlet surface step: s = M+ f+ (+ M- f-),  amp = [s; f+ - f-], s_out = s
+__global__ void k_dir_amp(const double* __restrict__ Mp, const double* __restrict__ Mm, int ld,
+                          const double* __restrict__ fp, const double* __restrict__ fm, double* __restrict__ amp,
+                          double* __restrict__ s_out, int n)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
-    double s = wave_dot(Mp + (size_t)n * i, fp, n, lane);
-    if (fm) s += wave_dot(Mm + (size_t)n * i, fm, n, lane);
+    double s = wave_dot(Mp + (size_t)ld * i, fp, n, lane);
+    if (fm) s += wave_dot(Mm + (size_t)ld * i, fm, n, lane);
     if (lane == 0) {
         amp[i] = s;
         amp[n + i] = fp[i] - (fm ? fm[i] : 0.0);
+        if (s_out) s_out[i] = s;
     }
 }
 
-void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, const double* fp, const double* fm,
-                   double* amp, int n)
+void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, int ld, const double* fp, const double* fm,
+                   double* amp, double* s_out, int n)
 {
     if (n <= 0) return;
-    k_dir_amp<<<(n + 3) / 4, 256, 0, st>>>(Mp, Mm, fp, fm, amp, n);
+    k_dir_amp<<<(n + 3) / 4, 256, 0, st>>>(Mp, Mm, ld, fp, fm, amp, s_out, n);
     ILH_HIP(hipGetLastError());
 }
 
 // Neumann surface step: df = A+ v+ + A- v-, ds = B+ v+ + B- v-;
 // primal amplitudes [df; v+ - v-], dual amplitudes [df; -ds]
 __global__ void k_neu_amp(const double* __restrict__ Ap, const double* __restrict__ Am, const double* __restrict__ Bp,
-                          const double* __restrict__ Bm, const double* __restrict__ vp, const double* __restrict__ vm,
-                          double* __restrict__ af, double* __restrict__ as, double* __restrict__ df,
-                          double* __restrict__ dsv, int n)
+                          const double* __restrict__ Bm, int ld, const double* __restrict__ vp,
+                          const double* __restrict__ vm, double* __restrict__ af, double* __restrict__ as,
+                          double* __restrict__ df, double* __restrict__ dsv, int n)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
-    double a = wave_dot(Ap + (size_t)n * i, vp, n, lane);
-    double b = wave_dot(Bp + (size_t)n * i, vp, n, lane);
+    double a = wave_dot(Ap + (size_t)ld * i, vp, n, lane);
+    double b = wave_dot(Bp + (size_t)ld * i, vp, n, lane);
     if (vm) {
-        a += wave_dot(Am + (size_t)n * i, vm, n, lane);
-        b += wave_dot(Bm + (size_t)n * i, vm, n, lane);
+        a += wave_dot(Am + (size_t)ld * i, vm, n, lane);
+        b += wave_dot(Bm + (size_t)ld * i, vm, n, lane);
     }
     if (lane == 0) {
         af[i] = a;
@@ -461,11 +490,11 @@ __global__ void k_neu_amp(const double* __restrict__ Ap, const double* __restric
     }
 }
 
-void neumann_amp(hipStream_t st, const double* Ap, const double* Am, const double* Bp, const double* Bm,
+void neumann_amp(hipStream_t st, const double* Ap, const double* Am, const double* Bp, const double* Bm, int ld,
                  const double* vp, const double* vm, double* af, double* as, double* df, double* dsv, int n)
 {
     if (n <= 0) return;
-    k_neu_amp<<<(n + 3) / 4, 256, 0, st>>>(Ap, Am, Bp, Bm, vp, vm, af, as, df, dsv, n);
+    k_neu_amp<<<(n + 3) / 4, 256, 0, st>>>(Ap, Am, Bp, Bm, ld, vp, vm, af, as, df, dsv, n);
     ILH_HIP(hipGetLastError());
 }
 

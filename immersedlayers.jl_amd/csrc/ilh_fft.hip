@@ -4,422 +4,610 @@
 // inverse_laplacian!(w, cache) = L \ w = dx^2 (G * w) over the field's own
 // index set (src/grid_operators.jl:144-179, LGF plan of src/cache.jl:321-324),
 // G(0,0) = 0, L G = delta.  G is even, so a circular period P >= 2n-2 per axis
-// reproduces the linear convolution exactly; P is the next power of two.
+// reproduces the linear convolution exactly; P is a power of two >= 16.
 //
-// Three passes, all with hand-written Stockham FFTs staged in LDS:
+// Transforms are Stockham radix-V stages (plus one smaller radix stage when
+// needed) with each thread holding V complex values in registers; LDS is only
+// the exchange between stages.  A transform of length P uses P/V threads.
+// Three passes per convolution:
 //   1. row pass:    two real rows packed as one complex P_x-point FFT, split
-//                   into the two half spectra (P_x/2+1 columns).  The rows are
-//                   read from a field, generated from the LGF table, or built
-//                   on the fly from surface-point stencil terms (no grid-sized
-//                   source array is ever written).  Only row pairs that hold
-//                   sources are transformed.
-//   2. column pass: W adjacent spectrum columns per work-group; forward
-//                   P_y-point FFT, multiply by the real LGF spectrum (dx^2 and
-//                   1/(P_x P_y) folded in), inverse FFT, keep the n_y rows.
-//                   Columns 0 and P_x/2 are real sequences and share one slot.
-//   3. row pass:    two half spectra packed into one complex inverse FFT,
-//                   real/imaginary parts are the two output rows.
-// The LGF table itself is built here by Gauss-Legendre quadrature of
+//                   into the two half spectra (P_x/2+1 columns).  Rows come
+//                   from a field, from the LGF table, or are generated on the
+//                   fly from the surface-point stencil terms staged in LDS (no
+//                   grid-sized source array is written).  Only row pairs that
+//                   hold sources are transformed.
+//   2. column pass: one spectrum column per transform: forward P_y-point FFT
+//                   whose first stage reads global memory, multiply by the
+//                   real LGF spectrum in registers (dx^2/(P_x P_y) folded in),
+//                   inverse FFT whose first stage consumes those registers and
+//                   whose last stage writes the n_y output rows.  Column
+//                   blocks are ordered so that neighbouring columns run on the
+//                   same XCD and share its L2.
+//   3. row pass:    the two half spectra of a row pair packed into one complex
+//                   inverse FFT; real/imaginary parts are the two output rows.
+// The LGF table is built here by Gauss-Legendre quadrature of
 //   G(m,n) = (1/2pi) int_0^pi [1 - cos(mt) e^{-n s(t)}] / sinh s(t) dt,
 //   cosh s = 2 - cos t,  n = max(|m|,|n|).
 #include "ilh_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 namespace ilh {
 
-static constexpr int TWN = 8192;         // twiddle table length = largest period
-static constexpr int NT = 256;           // threads per FFT work-group
+static constexpr int MAXP = 8192;
+// complex values held per thread (= largest radix); a length-P transform uses P/V threads
+static constexpr int V = 8;
 
-// ------------------------------------------------------------- twiddles
-__global__ void k_twiddle(double2* tw)
+// --------------------------------------------------------------- plans
+// Stage list of a length-P transform: radix-V stages then the remainder
+// ("fwd" order), or the remainder first ("rev" order, used by the inverse of
+// the column pass so that its first stage reads the forward's last-stage
+// registers).
+struct StagePlan {
+    int n;
+    int R[6], Ns[6], off[6];
+};
+
+struct FftDev {
+    int P, T;              // T = P/V threads per transform
+    StagePlan fwd, rev;
+    const double2* tw;     // per stage: W_{Ns R}^{r k} = exp(-2 pi i r k/(Ns R)), k < Ns, 1 <= r < R
+};
+
+static StagePlan make_plan(int P, bool rev, int& off)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= TWN) return;
+    StagePlan s{};
+    int rem = P, nsx = 0;
+    int rs[6];
+    while (rem >= V) { rs[nsx++] = V; rem /= V; }
+    if (rem > 1) rs[nsx++] = rem;
+    if (rev && nsx > 1 && rs[nsx - 1] != V) {
+        const int last = rs[nsx - 1];
+        for (int k = nsx - 1; k > 0; --k) rs[k] = rs[k - 1];
+        rs[0] = last;
+    }
+    int Ns = 1;
+    for (int k = 0; k < nsx; ++k) {
+        s.R[k] = rs[k];
+        s.Ns[k] = Ns;
+        s.off[k] = off;
+        off += Ns * (rs[k] - 1);
+        Ns *= rs[k];
+    }
+    s.n = nsx;
+    return s;
+}
+
+__global__ void k_stage_tw(double2* tw, int off, int Ns, int R)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Ns * (R - 1)) return;
+    const int k = e / (R - 1), r = e % (R - 1) + 1;
     double s, c;
-    sincospi(2.0 * (double)k / TWN, &s, &c);
-    tw[k] = make_double2(c, -s);         // exp(-2 pi i k / TWN)
+    sincospi(2.0 * (double)((long long)r * k) / (double)(Ns * R), &s, &c);
+    tw[off + e] = make_double2(c, -s);
 }
 
-void ensure_twiddles(ilh_ctx* ctx)
+struct FftHost {
+    FftDev d;
+    DevBuf buf;
+};
+
+static FftDev get_fft(ilh_ctx* ctx, int P)
 {
-    if (ctx->tw_len == TWN) return;
-    ctx->twiddle.ensure(sizeof(double2) * TWN);
-    k_twiddle<<<TWN / 256, 256, 0, ctx->stream>>>(ctx->twiddle.as<double2>());
+    // one table set per (device, P); immutable once built
+    static std::map<std::pair<int, int>, FftHost*> cache;
+    auto key = std::make_pair(ctx->device, P);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second->d;
+    if (P < 16 || P > MAXP || (P & (P - 1))) fail(ILH_ERR_ARG, "FFT length must be a power of two in [16, 8192]");
+    FftHost* h = new FftHost();
+    int off = 0;
+    h->d.P = P;
+    h->d.T = P / V;
+    h->d.fwd = make_plan(P, false, off);
+    h->d.rev = make_plan(P, true, off);
+    h->buf.ensure(sizeof(double2) * (size_t)std::max(off, 1));
+    for (const StagePlan* sp : {&h->d.fwd, &h->d.rev})
+        for (int k = 0; k < sp->n; ++k) {
+            const int n = sp->Ns[k] * (sp->R[k] - 1);
+            k_stage_tw<<<(n + 255) / 256, 256, 0, ctx->stream>>>(h->buf.as<double2>(), sp->off[k], sp->Ns[k], sp->R[k]);
+        }
     ILH_HIP(hipGetLastError());
-    ctx->tw_len = TWN;
+    ILH_HIP(hipStreamSynchronize(ctx->stream));
+    h->d.tw = h->buf.as<double2>();
+    cache[key] = h;
+    return h->d;
 }
 
-// ------------------------------------------------------------ LDS FFT
+void ensure_twiddles(ilh_ctx* ctx) { (void)ctx; }
+
+// ---------------------------------------------------------- butterflies
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-// multiply by s*i (s = +-1)
-__device__ __forceinline__ double2 mul_si(double2 a, int s) { return s > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x); }
+template <int S>   // multiply by S*i
+__device__ __forceinline__ double2 mul_si(double2 a) { return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x); }
 
-template <int R>
-__device__ __forceinline__ void dft(double2* a, int s);
-
-template <>
-__device__ __forceinline__ void dft<2>(double2* a, int)
+template <int S>
+__device__ __forceinline__ void dft2(double2& a, double2& b)
 {
-    double2 t = a[0];
-    a[0] = cadd(t, a[1]);
-    a[1] = csub(t, a[1]);
+    const double2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
 }
 
-template <>
-__device__ __forceinline__ void dft<4>(double2* a, int s)
+template <int S>
+__device__ __forceinline__ void dft4(double2& a0, double2& a1, double2& a2, double2& a3)
 {
-    double2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
-    double2 t2 = cadd(a[1], a[3]), t3 = mul_si(csub(a[1], a[3]), s);
-    a[0] = cadd(t0, t2);
-    a[2] = csub(t0, t2);
-    a[1] = cadd(t1, t3);
-    a[3] = csub(t1, t3);
+    const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    const double2 t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
 }
 
-template <>
-__device__ __forceinline__ void dft<8>(double2* a, int s)
+template <int S>
+__device__ __forceinline__ void dft8(double2* a)
 {
     const double c = 0.70710678118654752440084436210485;
-    double2 e[4] = {a[0], a[2], a[4], a[6]};
-    double2 o[4] = {a[1], a[3], a[5], a[7]};
-    dft<4>(e, s);
-    dft<4>(o, s);
-    // W8^k, W8 = exp(s 2 pi i / 8)
-    double2 o1 = cmul(o[1], make_double2(c, s * c));
-    double2 o2 = mul_si(o[2], s);
-    double2 o3 = cmul(o[3], make_double2(-c, s * c));
-    a[0] = cadd(e[0], o[0]);
-    a[4] = csub(e[0], o[0]);
-    a[1] = cadd(e[1], o1);
-    a[5] = csub(e[1], o1);
-    a[2] = cadd(e[2], o2);
-    a[6] = csub(e[2], o2);
-    a[3] = cadd(e[3], o3);
-    a[7] = csub(e[3], o3);
+    double2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    double2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    dft4<S>(e0, e1, e2, e3);
+    dft4<S>(o0, o1, o2, o3);
+    o1 = cmul(o1, make_double2(c, S * c));
+    o2 = mul_si<S>(o2);
+    o3 = cmul(o3, make_double2(-c, S * c));
+    a[0] = cadd(e0, o0);
+    a[4] = csub(e0, o0);
+    a[1] = cadd(e1, o1);
+    a[5] = csub(e1, o1);
+    a[2] = cadd(e2, o2);
+    a[6] = csub(e2, o2);
+    a[3] = cadd(e3, o3);
+    a[7] = csub(e3, o3);
 }
 
-// One Stockham radix-R stage over `nb` contiguous transforms of length P held
-// in LDS (in place: all inputs are read into registers before any output is
-// written).  VALS = complex values per thread = nb*P/NT.
-template <int R, int VALS>
-__device__ __forceinline__ void stage(double2* x, int P, int nb, int Ns, const double2* __restrict__ tw, int s)
+// 16 = 4 x 4:  X[k1 + 4 k2] = sum_n2 W4^{n2 k2} W16^{n2 k1} sum_n1 W4^{n1 k1} a[4 n1 + n2]
+template <int S>
+__device__ __forceinline__ void dft16(double2* a)
 {
-    constexpr int IT = VALS / R;
-    const int PR = P / R, nbf = nb * PR;
-    double2 v[VALS];
+    const double c1 = 0.92387953251128675612818318939679;   // cos(pi/8)
+    const double s1 = 0.38268343236508977172845998403040;   // sin(pi/8)
+    const double c2 = 0.70710678118654752440084436210485;
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int b = threadIdx.x + it * NT;
-        if (b < nbf) {
-            const int c = b / PR, j = b - c * PR;
-            const double2* xc = x + (size_t)c * P;
+    for (int n2 = 0; n2 < 4; ++n2) dft4<S>(a[n2], a[4 + n2], a[8 + n2], a[12 + n2]);
+    // a[4 k1 + n2] holds the inner sum of (k1, n2); twiddle by W16^{n2 k1}
+    a[5] = cmul(a[5], make_double2(c1, S * s1));     // W^1
+    a[6] = cmul(a[6], make_double2(c2, S * c2));     // W^2
+    a[7] = cmul(a[7], make_double2(s1, S * c1));     // W^3
+    a[9] = cmul(a[9], make_double2(c2, S * c2));     // W^2
+    a[10] = mul_si<S>(a[10]);                        // W^4
+    a[11] = cmul(a[11], make_double2(-c2, S * c2));  // W^6
+    a[13] = cmul(a[13], make_double2(s1, S * c1));   // W^3
+    a[14] = cmul(a[14], make_double2(-c2, S * c2));  // W^6
+    a[15] = cmul(a[15], make_double2(-c1, -S * s1)); // W^9
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[it * R + r] = xc[j + r * PR];
-        }
-    }
-    __syncthreads();
-    const int tstep = TWN / (Ns * R);
+    for (int k1 = 0; k1 < 4; ++k1) dft4<S>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
+    // a[4 k1 + k2] holds X[k1 + 4 k2]: transpose into natural order
+    double2 t[16];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int b = threadIdx.x + it * NT;
-        if (b < nbf) {
-            const int c = b / PR, j = b - c * PR;
-            const int k = j % Ns;
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = a[4 * k1 + k2];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = t[k];
+}
+
+template <int R, int S>
+__device__ __forceinline__ void dft(double2* a)
+{
+    if (R == 16) { if constexpr (V >= 16) dft16<S>(a); }
+    else if (R == 8) dft8<S>(a);
+    else if (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
+    else dft2<S>(a[0], a[1]);
+}
+
+// ------------------------------------------------------- stage helpers
+// LDS exchange buffers hold P complex values with one pad slot after every 8,
+// so the stride-R stores of a first stage fall in distinct banks.
+__device__ __forceinline__ int lx(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int lds_len(int P) { return P + P / 8; }
+
+// Thread t of a transform owns butterflies j = t + q T (q < V/R) of every
+// stage; v[q R + r] is element r of butterfly q.
+template <int R, int S>
+__device__ __forceinline__ void st_compute(double2 (&v)[V], int t, int T, int Ns, const double2* __restrict__ tw)
+{
+#pragma unroll
+    for (int q = 0; q < V / R; ++q) {
+        const int j = t + q * T;
+        const int k = j & (Ns - 1);
+        if (Ns > 1) {
+            const double2* w = tw + (size_t)k * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                double2 w = tw[r * k * tstep];
-                if (s > 0) w.y = -w.y;
-                v[it * R + r] = cmul(v[it * R + r], w);
+                double2 ww = w[r - 1];
+                if (S > 0) ww.y = -ww.y;
+                v[q * R + r] = cmul(v[q * R + r], ww);
             }
-            dft<R>(v + it * R, s);
-            double2* xc = x + (size_t)c * P;
-            const int base = (j / Ns) * Ns * R + k;
+        }
+        dft<R, S>(v + q * R);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void st_load(double2 (&v)[V], const double2* x, int t, int T, int P)
+{
 #pragma unroll
-            for (int r = 0; r < R; ++r) xc[base + r * Ns] = v[it * R + r];
+    for (int q = 0; q < V / R; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q * R + r] = x[lx(t + q * T + r * (P / R))];
+}
+
+template <int R>
+__device__ __forceinline__ void st_store(const double2 (&v)[V], double2* x, int t, int T, int Ns)
+{
+#pragma unroll
+    for (int q = 0; q < V / R; ++q) {
+        const int j = t + q * T;
+        const int base = (j & ~(Ns - 1)) * R + (j & (Ns - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[lx(base + r * Ns)] = v[q * R + r];
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void compute_any(int R, double2 (&v)[V], int t, int T, int Ns, const double2* tw)
+{
+    switch (R) {
+    case 16: if constexpr (V >= 16) st_compute<16, S>(v, t, T, Ns, tw); break;
+    case 8: st_compute<8, S>(v, t, T, Ns, tw); break;
+    case 4: st_compute<4, S>(v, t, T, Ns, tw); break;
+    default: st_compute<2, S>(v, t, T, Ns, tw); break;
+    }
+}
+
+__device__ __forceinline__ void load_any(int R, double2 (&v)[V], const double2* x, int t, int T, int P)
+{
+    switch (R) {
+    case 16: if constexpr (V >= 16) st_load<16>(v, x, t, T, P); break;
+    case 8: st_load<8>(v, x, t, T, P); break;
+    case 4: st_load<4>(v, x, t, T, P); break;
+    default: st_load<2>(v, x, t, T, P); break;
+    }
+}
+
+__device__ __forceinline__ void store_any(int R, const double2 (&v)[V], double2* x, int t, int T, int Ns)
+{
+    switch (R) {
+    case 16: if constexpr (V >= 16) st_store<16>(v, x, t, T, Ns); break;
+    case 8: st_store<8>(v, x, t, T, Ns); break;
+    case 4: st_store<4>(v, x, t, T, Ns); break;
+    default: st_store<2>(v, x, t, T, Ns); break;
+    }
+}
+
+// Run all stages of `sp`.  On entry v holds the inputs of stage 0 (element r
+// of butterfly q = x[t + q T + r P/R0]); on exit v holds the outputs of the
+// last stage (element r of butterfly q at t + q T + r P/R_last).  `x` is the
+// transform's LDS exchange buffer; the caller has synchronised the work-group
+// if the buffer was in use before.
+template <int S>
+__device__ __forceinline__ void run_stages(const StagePlan& sp, double2 (&v)[V], double2* x, int t, int T, int P,
+                                           const double2* __restrict__ tw)
+{
+    for (int s = 0; s < sp.n; ++s) {
+        if (s > 0) {
+            __syncthreads();
+            load_any(sp.R[s], v, x, t, T, P);
+        }
+        compute_any<S>(sp.R[s], v, t, T, sp.Ns[s], tw + sp.off[s]);
+        if (s + 1 < sp.n) {
+            __syncthreads();
+            store_any(sp.R[s], v, x, t, T, sp.Ns[s]);
         }
     }
-    __syncthreads();
 }
 
-// nb transforms of length P (power of two, 2 <= P <= TWN) in LDS, sign s.
-template <int VALS>
-__device__ void lds_fft(double2* x, int P, int nb, const double2* __restrict__ tw, int s)
+// element (q, r) of the first stage's inputs or the last stage's outputs
+__device__ __forceinline__ int io_index(int R, int q, int r, int t, int T, int P)
 {
-    int Ns = 1;
-    while (Ns * 8 <= P) {
-        stage<8, VALS>(x, P, nb, Ns, tw, s);
-        Ns *= 8;
-    }
-    if (Ns * 4 == P) stage<4, VALS>(x, P, nb, Ns, tw, s);
-    else if (Ns * 2 == P) stage<2, VALS>(x, P, nb, Ns, tw, s);
+    return t + q * T + r * (P / R);
 }
 
-// ------------------------------------------------------------ row sources
+// --------------------------------------------------------- row sources
 enum RowMode { ROW_DENSE = 0, ROW_LGF = 1, ROW_TERMS = 2 };
 
 struct RowSrc {
     int mode;
-    // dense field
-    const double* field;
+    const double* field;   // dense field (nx x ny)
     int nx, ny;
-    // LGF circular rows
-    const double* tab;
+    const double* tab;     // LGF table (circular rows of the Px x Py kernel)
     int tmx, Px, Py;
-    // point terms
-    const Term* terms;
-    const int* ptr;
-    const int* ent;
-    const int* ebx;
+    const int* ebx;        // point terms, row-binned: sorted bx, amplitude
+    const int* eai;        // index and 5 weights per entry, and the nonzero
+    const double* erw;     // cells of each row (x, entry range)
+    const int* cell_ptr;
+    const int* cells;
     const double* amp;
-    const int* pairs;      // compacted row pairs (ROW_TERMS), else null
+    const int* pairs;      // compacted row pairs, or null
 };
 
-__device__ __forceinline__ double row_value(const RowSrc& S, int x, int j)
+// value of one cell: its entries in (bx, term) order
+__device__ __forceinline__ double cell_value(const RowSrc& S, const int* cl)
 {
-    if (S.mode == ROW_LGF) {
-        const int a = x <= S.Px / 2 ? x : S.Px - x;
-        const int b = j <= S.Py / 2 ? j : S.Py - j;
-        return S.tab[a + (size_t)S.tmx * b];
-    }
-    if (j >= S.ny || x >= S.nx) return 0.0;
-    if (S.mode == ROW_DENSE) return S.field[x + (size_t)S.nx * j];
-    const int lo = S.ptr[j], hi = S.ptr[j + 1];
-    if (lo == hi) return 0.0;
-    int a = lo, b = hi;
-    while (a < b) {
-        const int m = (a + b) >> 1;
-        if (S.ebx[m] < x - 4) a = m + 1; else b = m;
-    }
+    const int x = cl[0];
     double acc = 0.0;
-    for (int k = a; k < hi; ++k) {
-        const int bx = S.ebx[k];
-        if (bx > x) break;
-        const Term& tt = S.terms[S.ent[k]];
-        acc += S.amp[tt.ai] * tt.coef * tt.wy[j - tt.by] * tt.wx[x - bx];
-    }
+    for (int k = cl[1]; k < cl[2]; ++k) acc += S.amp[S.eai[k]] * S.erw[5 * (size_t)k + (x - S.ebx[k])];
     return acc;
 }
 
-// Pass 1: work-group b transforms rows (j0, j0+1) of the source, writes the two
-// half spectra to T rows 2b, 2b+1 (row stride H = Px/2+1).
-template <int VALS>
-__global__ void __launch_bounds__(NT) k_rows_fwd(RowSrc S, int Px, int nrows_limit, double2* __restrict__ T,
-                                                 const double2* __restrict__ tw)
+// Spectrum intermediates use an 8-row tiled layout: element (row, c) at
+// (row/8)*8*H + c*8 + row%8, so a column's 8-row run is one 128-byte segment
+// (column pass) and a row pair's value at c is 32 contiguous bytes (row passes).
+__device__ __forceinline__ size_t tidx(int row, int c, int H)
 {
-    extern __shared__ double2 lds[];
-    const int b = blockIdx.x;
-    const int j0 = S.pairs ? 2 * S.pairs[b] : 2 * b;
-    const int H = Px / 2 + 1;
-    const int nx_src = S.mode == ROW_LGF ? Px : S.nx;
-    for (int x = threadIdx.x; x < Px; x += NT) {
-        double a = 0.0, c = 0.0;
-        if (x < nx_src) {
-            a = row_value(S, x, j0);
-            if (j0 + 1 < nrows_limit) c = row_value(S, x, j0 + 1);
-        }
-        lds[x] = make_double2(a, c);
-    }
-    __syncthreads();
-    lds_fft<VALS>(lds, Px, 1, tw, -1);
-    double2* ra = T + (size_t)(2 * b) * H;
-    double2* rb = ra + H;
-    for (int k = threadIdx.x; k < H; k += NT) {
-        const double2 z = lds[k];
-        const double2 zc = lds[(Px - k) & (Px - 1)];
-        // X_a = (Z + conj Zr)/2,  X_b = (Z - conj Zr)/(2i)
-        ra[k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
-        rb[k] = make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
-    }
+    return (size_t)(row >> 3) * 8 * H + (size_t)c * 8 + (row & 7);
 }
 
-// Pass 2: columns.  Work-group handles column slots [c0, c0+W); slot 0 packs
-// the real columns kx = 0 and kx = Px/2.  Input rows: T1 row r holds field row
-// rowmap(r); absent rows are zero.  Output: T2 rows 0..ny-1 (full).
-template <int VALS>
-__global__ void __launch_bounds__(NT) k_cols(const double2* __restrict__ T1, const int* __restrict__ pairs,
-                                             int nin_rows, int Px, int Py, int ny, int W,
-                                             const double* __restrict__ spec, double2* __restrict__ T2,
-                                             const double2* __restrict__ tw)
+// ------------------------------------------------------------- pass 1
+// Work-group = nb transforms of length Px (T = Px/16 threads each); transform
+// c of block b handles row pair p = b nb + c (field rows 2p, 2p+1, or
+// 2 pairs[p], +1) and writes the two half spectra to rows 2p, 2p+1 of the
+// tiled Tout (H = Px/2 + 1 columns).
+__global__ void __launch_bounds__(512) k_rows_fwd(RowSrc S, FftDev F, int npairs, int nrows_limit,
+                                                  double2* __restrict__ Tout)
 {
     extern __shared__ double2 lds[];
-    const int H = Px / 2 + 1;
-    const int c0 = blockIdx.x * W;
-    for (int k = threadIdx.x; k < W * Py; k += NT) lds[k] = make_double2(0.0, 0.0);
-    __syncthreads();
-    for (int k = threadIdx.x; k < W * nin_rows; k += NT) {
-        const int w = k % W, r = k / W;
-        const int row = pairs ? 2 * pairs[r >> 1] + (r & 1) : r;
-        if (row >= ny) continue;
-        const int c = c0 + w;
-        if (c >= Px / 2) continue;
-        const double2* src = T1 + (size_t)r * H;
-        double2 v;
-        if (c == 0) v = make_double2(src[0].x, src[Px / 2].x);
-        else v = src[c];
-        lds[w * Py + row] = v;
-    }
-    __syncthreads();
-    lds_fft<VALS>(lds, Py, W, tw, -1);
-    // multiply by the real LGF spectrum
-    if (c0 == 0) {
-        // packed slot: C' = ((G0+GN)/2) C(k) + ((G0-GN)/2) conj(C(Py-k))
-        double2 nv[VALS];
-        const int per = (Py + NT - 1) / NT;
-        for (int q = 0; q < per && q < VALS; ++q) {
-            const int ky = threadIdx.x + q * NT;
-            if (ky < Py) {
-                const double g0 = spec[ky], gn = spec[(size_t)(Px / 2) * Py + ky];
-                const double2 cz = lds[ky], cr = lds[(Py - ky) & (Py - 1)];
-                const double a = 0.5 * (g0 + gn), d = 0.5 * (g0 - gn);
-                nv[q] = make_double2(a * cz.x + d * cr.x, a * cz.y - d * cr.y);
+    const int P = F.P, T = F.T, H = P / 2 + 1;
+    const int nb = blockDim.x / T;
+    const int c = threadIdx.x / T, t = threadIdx.x - c * T;
+    const int p = blockIdx.x * nb + c;
+    const bool live = p < npairs;
+    const int j0 = live ? (S.pairs ? 2 * S.pairs[p] : 2 * p) : 0;
+    double2* x = lds + (size_t)c * lds_len(P);
+    const int R0 = F.fwd.R[0];
+    double2 v[V];
+    if (S.mode == ROW_TERMS) {
+        // the packed pair (row j0 -> real part, row j0+1 -> imaginary part) in
+        // LDS: zeros, then the rows' nonzero cells
+        for (int i = t; i < P; i += T) x[lx(i)] = make_double2(0.0, 0.0);
+        __syncthreads();
+        if (live) {
+            for (int ci = S.cell_ptr[j0] + t; ci < S.cell_ptr[j0 + 1]; ci += T) {
+                const int* cl = S.cells + 3 * (size_t)ci;
+                x[lx(cl[0])].x = cell_value(S, cl);
             }
+            if (j0 + 1 < S.ny)
+                for (int ci = S.cell_ptr[j0 + 1] + t; ci < S.cell_ptr[j0 + 2]; ci += T) {
+                    const int* cl = S.cells + 3 * (size_t)ci;
+                    x[lx(cl[0])].y = cell_value(S, cl);
+                }
         }
         __syncthreads();
-        for (int q = 0; q < per && q < VALS; ++q) {
-            const int ky = threadIdx.x + q * NT;
-            if (ky < Py) lds[ky] = nv[q];
-        }
-        for (int k = threadIdx.x; k < (W - 1) * Py; k += NT) {
-            const int w = 1 + k / Py, ky = k % Py;
-            if (c0 + w >= Px / 2) continue;
-            const double g = spec[(size_t)(c0 + w) * Py + ky];
-            lds[w * Py + ky].x *= g;
-            lds[w * Py + ky].y *= g;
-        }
+#pragma unroll
+        for (int q = 0; q < V; ++q) v[q] = x[lx(io_index(R0, q / R0, q % R0, t, T, P))];
     } else {
-        for (int k = threadIdx.x; k < W * Py; k += NT) {
-            const int w = k / Py, ky = k - w * Py;
-            if (c0 + w >= Px / 2) continue;
-            const double g = spec[(size_t)(c0 + w) * Py + ky];
-            lds[k].x *= g;
-            lds[k].y *= g;
+        const int nx_src = S.mode == ROW_LGF ? P : S.nx;
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int xpos = io_index(R0, q / R0, q % R0, t, T, P);
+            double a = 0.0, b = 0.0;
+            if (live && xpos < nx_src) {
+                if (S.mode == ROW_DENSE) {
+                    a = S.field[xpos + (size_t)S.nx * j0];
+                    if (j0 + 1 < nrows_limit) b = S.field[xpos + (size_t)S.nx * (j0 + 1)];
+                } else {
+                    const int ia = xpos <= S.Px / 2 ? xpos : S.Px - xpos;
+                    const int ja = j0 <= S.Py / 2 ? j0 : S.Py - j0;
+                    const int jb = (j0 + 1) <= S.Py / 2 ? j0 + 1 : S.Py - (j0 + 1);
+                    a = S.tab[ia + (size_t)S.tmx * ja];
+                    if (j0 + 1 < nrows_limit) b = S.tab[ia + (size_t)S.tmx * jb];
+                }
+            }
+            v[q] = make_double2(a, b);
         }
     }
     __syncthreads();
-    lds_fft<VALS>(lds, Py, W, tw, +1);
-    for (int k = threadIdx.x; k < W * ny; k += NT) {
-        const int w = k % W, r = k / W;
-        const int c = c0 + w;
-        if (c >= Px / 2) continue;
-        const double2 v = lds[w * Py + r];
-        double2* dst = T2 + (size_t)r * H;
-        if (c == 0) {
-            dst[0] = make_double2(v.x, 0.0);
-            dst[Px / 2] = make_double2(v.y, 0.0);
-        } else {
-            dst[c] = v;
+    run_stages<-1>(F.fwd, v, x, t, T, P, F.tw);
+    // natural-order spectrum to LDS, then split into the two half spectra
+    const int RL = F.fwd.R[F.fwd.n - 1];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < V; ++q) x[lx(io_index(RL, q / RL, q % RL, t, T, P))] = v[q];
+    __syncthreads();
+    if (!live) return;
+    for (int k = t; k < H; k += T) {
+        const double2 z = x[lx(k)];
+        const double2 zc = x[lx((P - k) & (P - 1))];
+        const double4 ab = make_double4(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y), 0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+        *(double4*)(Tout + tidx(2 * p, k, H)) = ab;
+    }
+}
+
+// ------------------------------------------------------------- pass 2
+// Work-group = nb column transforms of length Py = P.  Tiled T1 row `slot`
+// holds field row r where slot = slot_of_row[r] (-1: the row is zero), or
+// slot = r when slot_of_row is null and r < nrows_in.  The LGF spectrum is
+// even in ky: only ky <= P/2 is stored (spec[c][ky], P/2+1 per column).
+// MODE 0: convolve, write tiled T2 rows 0..ny-1.  MODE 1: forward only, write
+// Re * scale to spec_out[c][ky] for ky <= P/2 (the LGF spectrum build).
+template <int MODE>
+__global__ void __launch_bounds__(512) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
+                                              FftDev F, int H, int ny, int nrows_in, const double* __restrict__ spec,
+                                              double scale, double2* __restrict__ T2, double* __restrict__ spec_out)
+{
+    extern __shared__ double2 lds[];
+    const int P = F.P, T = F.T, SH = P / 2 + 1;
+    const int nb = blockDim.x / T;
+    const int cl = threadIdx.x / T, t = threadIdx.x - cl * T;
+    // blocks are dispatched round-robin over the 8 XCDs: give each XCD a run of
+    // consecutive columns so that their 128-byte row segments meet in its L2
+    const int nblk = gridDim.x, full = (nblk / 8) * 8;
+    const int b = (int)blockIdx.x < full ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
+    const int c = b * nb + cl;
+    const bool live = c < H;
+    double2* x = lds + (size_t)cl * lds_len(P);
+    const int R0 = F.fwd.R[0];
+    double2 v[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const int row = io_index(R0, q / R0, q % R0, t, T, P);
+        double2 val = make_double2(0.0, 0.0);
+        if (live && row < ny) {
+            const int slot = slot_of_row ? slot_of_row[row] : (row < nrows_in ? row : -1);
+            if (slot >= 0) val = T1[tidx(slot, c, H)];
+        }
+        v[q] = val;
+    }
+    run_stages<-1>(F.fwd, v, x, t, T, P, F.tw);
+    const int RL = F.fwd.R[F.fwd.n - 1];
+    if (MODE == 1) {
+        if (!live) return;
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int ky = io_index(RL, q / RL, q % RL, t, T, P);
+            if (ky <= P / 2) spec_out[(size_t)c * SH + ky] = v[q].x * scale;
+        }
+        return;
+    }
+    // multiply by the real spectrum; the reverse plan's first stage has radix
+    // RL and reads exactly these indices from registers
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int ky = io_index(RL, q / RL, q % RL, t, T, P);
+            const double g = spec[(size_t)c * SH + (ky <= P / 2 ? ky : P - ky)];
+            v[q].x *= g;
+            v[q].y *= g;
         }
     }
-}
-
-// Spectrum build: forward column FFT of T (all Py rows), keep the real part.
-template <int VALS>
-__global__ void __launch_bounds__(NT) k_cols_spec(const double2* __restrict__ T, int Px, int Py, int W, double scale,
-                                                  double* __restrict__ spec, const double2* __restrict__ tw)
-{
-    extern __shared__ double2 lds[];
-    const int H = Px / 2 + 1;
-    const int c0 = blockIdx.x * W;
-    for (int k = threadIdx.x; k < W * Py; k += NT) {
-        const int w = k % W, r = k / W;
-        const int c = c0 + w;
-        lds[w * Py + r] = c < H ? T[(size_t)r * H + c] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    lds_fft<VALS>(lds, Py, W, tw, -1);
-    for (int k = threadIdx.x; k < W * Py; k += NT) {
-        const int w = k / Py, ky = k - w * Py;
-        const int c = c0 + w;
-        if (c < H) spec[(size_t)c * Py + ky] = lds[k].x * scale;
+    __syncthreads();   // all forward LDS loads done before the reverse plan stores
+    run_stages<1>(F.rev, v, x, t, T, P, F.tw);
+    if (!live) return;
+    const int RR = F.rev.R[F.rev.n - 1];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const int row = io_index(RR, q / RR, q % RR, t, T, P);
+        if (row < ny) T2[tidx(row, c, H)] = v[q];
     }
 }
 
-// Pass 3: inverse rows.  Work-group b makes output rows 2b, 2b+1.
-template <int VALS>
-__global__ void __launch_bounds__(NT) k_rows_inv(const double2* __restrict__ T2, int Px, int nx, int ny,
-                                                 double* __restrict__ out, const double2* __restrict__ tw)
+// ------------------------------------------------------------- pass 3
+// Work-group = nb row-pair transforms; transform c makes output rows 2p, 2p+1.
+__global__ void __launch_bounds__(512) k_rows_inv(const double2* __restrict__ T2, FftDev F, int nx, int ny,
+                                                  double* __restrict__ out)
 {
     extern __shared__ double2 lds[];
-    const int j0 = 2 * blockIdx.x;
-    const int H = Px / 2 + 1;
+    const int P = F.P, T = F.T, H = P / 2 + 1;
+    const int nb = blockDim.x / T;
+    const int c = threadIdx.x / T, t = threadIdx.x - c * T;
+    const int p = blockIdx.x * nb + c;
+    const int j0 = 2 * p;
+    const bool live = j0 < ny;
     const bool two = j0 + 1 < ny;
-    const double2* ra = T2 + (size_t)j0 * H;
-    const double2* rb = ra + H;
-    for (int k = threadIdx.x; k < H; k += NT) {
-        const double2 A = ra[k];
-        const double2 B = two ? rb[k] : make_double2(0.0, 0.0);
-        // Z(k) = A + iB, Z(P-k) = conj(A) + i conj(B)
-        lds[k] = make_double2(A.x - B.y, A.y + B.x);
-        if (k > 0 && k < Px / 2) lds[Px - k] = make_double2(A.x + B.y, B.x - A.y);
+    double2* x = lds + (size_t)c * lds_len(P);
+    const int R0 = F.fwd.R[0];
+    double2 v[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const int m = io_index(R0, q / R0, q % R0, t, T, P);
+        double2 z = make_double2(0.0, 0.0);
+        if (live) {
+            // Z(m) = A(m) + i B(m),  Z(P-m) = conj(A(m)) + i conj(B(m))
+            const int k = m <= P / 2 ? m : P - m;
+            const double4 ab = *(const double4*)(T2 + tidx(j0, k, H));
+            const double Ax = ab.x, Ay = ab.y;
+            const double Bx = two ? ab.z : 0.0, By = two ? ab.w : 0.0;
+            z = m <= P / 2 ? make_double2(Ax - By, Ay + Bx) : make_double2(Ax + By, Bx - Ay);
+        }
+        v[q] = z;
     }
-    __syncthreads();
-    lds_fft<VALS>(lds, Px, 1, tw, +1);
-    for (int x = threadIdx.x; x < nx; x += NT) {
-        const double2 z = lds[x];
-        out[x + (size_t)nx * j0] = z.x;
-        if (two) out[x + (size_t)nx * (j0 + 1)] = z.y;
+    run_stages<1>(F.fwd, v, x, t, T, P, F.tw);
+    if (!live) return;
+    const int RL = F.fwd.R[F.fwd.n - 1];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const int xo = io_index(RL, q / RL, q % RL, t, T, P);
+        if (xo < nx) {
+            out[xo + (size_t)nx * j0] = v[q].x;
+            if (two) out[xo + (size_t)nx * (j0 + 1)] = v[q].y;
+        }
     }
 }
 
 // ------------------------------------------------------------- launches
-static int vals_for(int total)   // complex values per thread
-{
-    const int v = (total + NT - 1) / NT;
-    if (v <= 8) return 8;
-    if (v <= 16) return 16;
-    return 32;
-}
-
-static int cols_per_wg(int Py) { return std::max(1, std::min(8, 8192 / Py)); }
-
 static void set_lds(const void* fn, size_t bytes)
 {
+    static const void* fns[32];
+    static size_t lim[32];
+    static int n = 0;
+    for (int k = 0; k < n; ++k)
+        if (fns[k] == fn) {
+            if (lim[k] >= bytes) return;
+            ILH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+            lim[k] = bytes;
+            return;
+        }
     ILH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (n < 32) {
+        fns[n] = fn;
+        lim[n] = bytes;
+        ++n;
+    }
 }
 
-static void launch_rows_fwd(hipStream_t st, const RowSrc& S, int Px, int nrows_limit, int nblocks, double2* T,
-                            const double2* tw)
+// threads per work-group and transforms per work-group for length P
+static void shape(int P, int& tpb, int& nb)
 {
-    const size_t lds = sizeof(double2) * Px;
-    switch (vals_for(Px)) {
-    case 8: set_lds((const void*)k_rows_fwd<8>, lds); k_rows_fwd<8><<<nblocks, NT, lds, st>>>(S, Px, nrows_limit, T, tw); break;
-    case 16: set_lds((const void*)k_rows_fwd<16>, lds); k_rows_fwd<16><<<nblocks, NT, lds, st>>>(S, Px, nrows_limit, T, tw); break;
-    default: set_lds((const void*)k_rows_fwd<32>, lds); k_rows_fwd<32><<<nblocks, NT, lds, st>>>(S, Px, nrows_limit, T, tw); break;
-    }
+    const int T = P / V;
+    tpb = std::max(64, T);
+    nb = tpb / T;
+}
+
+static void launch_rows_fwd(ilh_ctx* ctx, const RowSrc& S, int P, int npairs, int nrows_limit, double2* Tout)
+{
+    if (npairs <= 0) return;
+    const FftDev F = get_fft(ctx, P);
+    int tpb, nb;
+    shape(P, tpb, nb);
+    const size_t lds = sizeof(double2) * (size_t)nb * lds_len(P);
+    set_lds((const void*)k_rows_fwd, lds);
+    k_rows_fwd<<<(npairs + nb - 1) / nb, tpb, lds, ctx->stream>>>(S, F, npairs, nrows_limit, Tout);
     ILH_HIP(hipGetLastError());
 }
 
-static void launch_cols(hipStream_t st, const double2* T1, const int* pairs, int nin_rows, int Px, int Py, int ny,
-                        const double* spec, double2* T2, const double2* tw)
+template <int MODE>
+static void launch_cols(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int Px, int Py, int ny,
+                        int nrows_in, const double* spec, double scale, double2* T2, double* spec_out)
 {
-    const int W = cols_per_wg(Py);
-    const int nslots = Px / 2;
-    const int nblocks = (nslots + W - 1) / W;
-    const size_t lds = sizeof(double2) * (size_t)W * Py;
-    switch (vals_for(W * Py)) {
-    case 8: set_lds((const void*)k_cols<8>, lds); k_cols<8><<<nblocks, NT, lds, st>>>(T1, pairs, nin_rows, Px, Py, ny, W, spec, T2, tw); break;
-    case 16: set_lds((const void*)k_cols<16>, lds); k_cols<16><<<nblocks, NT, lds, st>>>(T1, pairs, nin_rows, Px, Py, ny, W, spec, T2, tw); break;
-    default: set_lds((const void*)k_cols<32>, lds); k_cols<32><<<nblocks, NT, lds, st>>>(T1, pairs, nin_rows, Px, Py, ny, W, spec, T2, tw); break;
-    }
+    const FftDev F = get_fft(ctx, Py);
+    const int H = Px / 2 + 1;
+    int tpb, nb;
+    shape(Py, tpb, nb);
+    const size_t lds = sizeof(double2) * (size_t)nb * lds_len(Py);
+    set_lds((const void*)k_cols<MODE>, lds);
+    k_cols<MODE><<<(H + nb - 1) / nb, tpb, lds, ctx->stream>>>(T1, slot_of_row, F, H, ny, nrows_in, spec, scale, T2,
+                                                               spec_out);
     ILH_HIP(hipGetLastError());
 }
 
-static void launch_rows_inv(hipStream_t st, const double2* T2, int Px, int nx, int ny, double* out, const double2* tw)
+static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, double* out)
 {
-    const size_t lds = sizeof(double2) * Px;
-    const int nblocks = (ny + 1) / 2;
-    switch (vals_for(Px)) {
-    case 8: set_lds((const void*)k_rows_inv<8>, lds); k_rows_inv<8><<<nblocks, NT, lds, st>>>(T2, Px, nx, ny, out, tw); break;
-    case 16: set_lds((const void*)k_rows_inv<16>, lds); k_rows_inv<16><<<nblocks, NT, lds, st>>>(T2, Px, nx, ny, out, tw); break;
-    default: set_lds((const void*)k_rows_inv<32>, lds); k_rows_inv<32><<<nblocks, NT, lds, st>>>(T2, Px, nx, ny, out, tw); break;
-    }
+    const FftDev F = get_fft(ctx, Px);
+    int tpb, nb;
+    shape(Px, tpb, nb);
+    const int npairs = (ny + 1) / 2;
+    const size_t lds = sizeof(double2) * (size_t)nb * lds_len(Px);
+    set_lds((const void*)k_rows_inv, lds);
+    k_rows_inv<<<(npairs + nb - 1) / nb, tpb, lds, ctx->stream>>>(T2, F, nx, ny, out);
     ILH_HIP(hipGetLastError());
 }
 
@@ -526,7 +714,7 @@ void ensure_lgf(ilh_plan* pl)
     int need = std::max(g.NX, g.NY) + 8;
     for (int gk = 0; gk < 2; ++gk) {
         int n = std::max(g.nx(gk), g.ny(gk));
-        int P = 2;
+        int P = 16;
         while (P < 2 * n - 2) P <<= 1;
         need = std::max(need, P / 2 + 1);
     }
@@ -545,28 +733,33 @@ void ensure_lgf(ilh_plan* pl)
     pl->lgf_mx = pl->lgf_my = need;
 }
 
+
 // -------------------------------------------------------------- plans
+static int period(int n)
+{
+    int P = 16;
+    while (P < 2 * n - 2) P <<= 1;
+    return P;
+}
+
 ConvPlan& get_conv(ilh_plan* pl, int gk)
 {
     ConvPlan& C = pl->conv[gk];
     if (pl->conv_ok[gk]) return C;
     ilh_ctx* ctx = pl->ctx;
-    ensure_twiddles(ctx);
     ensure_lgf(pl);
     const Grid& g = pl->g;
     C.gk = gk;
     C.nx = g.nx(gk);
     C.ny = g.ny(gk);
-    C.Px = 2;
-    while (C.Px < 2 * C.nx - 2) C.Px <<= 1;
-    C.Py = 2;
-    while (C.Py < 2 * C.ny - 2) C.Py <<= 1;
-    if (C.Px > TWN || C.Py > TWN) fail(ILH_ERR_ARG, "grid too large for the FFT convolution (period > 8192)");
+    C.Px = period(C.nx);
+    C.Py = period(C.ny);
+    if (C.Px > MAXP || C.Py > MAXP) fail(ILH_ERR_ARG, "grid too large for the FFT convolution (period > 8192)");
     C.H = C.Px / 2 + 1;
-    const int rows = std::max(C.Py, C.ny + 1);
+    const int rows = (std::max(C.Py, C.ny + 1) + 7) / 8 * 8;   // tiled layout: whole 8-row tiles
     C.T1.ensure(sizeof(double2) * (size_t)rows * C.H);
-    C.T2.ensure(sizeof(double2) * (size_t)(C.ny + 1) * C.H);
-    C.spec.ensure(sizeof(double) * (size_t)C.H * C.Py);
+    C.T2.ensure(sizeof(double2) * (size_t)((C.ny + 1 + 7) / 8 * 8) * C.H);
+    C.spec.ensure(sizeof(double) * (size_t)C.H * (C.Py / 2 + 1));
     // spectrum of the circular kernel: rows of G_circ, then forward columns
     RowSrc S{};
     S.mode = ROW_LGF;
@@ -574,19 +767,10 @@ ConvPlan& get_conv(ilh_plan* pl, int gk)
     S.tmx = pl->lgf_mx;
     S.Px = C.Px;
     S.Py = C.Py;
-    hipStream_t st = ctx->stream;
-    const double2* tw = ctx->twiddle.as<double2>();
-    launch_rows_fwd(st, S, C.Px, C.Py, C.Py / 2, C.T1.as<double2>(), tw);
-    const int W = cols_per_wg(C.Py);
-    const size_t lds = sizeof(double2) * (size_t)W * C.Py;
-    const int nblocks = (C.H + W - 1) / W;
+    launch_rows_fwd(ctx, S, C.Px, C.Py / 2, C.Py, C.T1.as<double2>());
     const double scale = g.dx * g.dx / ((double)C.Px * C.Py);
-    switch (vals_for(W * C.Py)) {
-    case 8: set_lds((const void*)k_cols_spec<8>, lds); k_cols_spec<8><<<nblocks, NT, lds, st>>>(C.T1.as<double2>(), C.Px, C.Py, W, scale, C.spec.as<double>(), tw); break;
-    case 16: set_lds((const void*)k_cols_spec<16>, lds); k_cols_spec<16><<<nblocks, NT, lds, st>>>(C.T1.as<double2>(), C.Px, C.Py, W, scale, C.spec.as<double>(), tw); break;
-    default: set_lds((const void*)k_cols_spec<32>, lds); k_cols_spec<32><<<nblocks, NT, lds, st>>>(C.T1.as<double2>(), C.Px, C.Py, W, scale, C.spec.as<double>(), tw); break;
-    }
-    ILH_HIP(hipGetLastError());
+    launch_cols<1>(ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.Py, C.Py, nullptr, scale, nullptr,
+                   C.spec.as<double>());
     pl->conv_ok[gk] = true;
     return C;
 }
@@ -594,59 +778,65 @@ ConvPlan& get_conv(ilh_plan* pl, int gk)
 void conv_dense(ilh_plan* pl, int gk, const double* in, double* out)
 {
     ConvPlan& C = get_conv(pl, gk);
-    hipStream_t st = pl->ctx->stream;
-    const double2* tw = pl->ctx->twiddle.as<double2>();
     RowSrc S{};
     S.mode = ROW_DENSE;
     S.field = in;
     S.nx = C.nx;
     S.ny = C.ny;
-    const int npairs = (C.ny + 1) / 2;
-    launch_rows_fwd(st, S, C.Px, C.ny, npairs, C.T1.as<double2>(), tw);
-    launch_cols(st, C.T1.as<double2>(), nullptr, C.ny, C.Px, C.Py, C.ny, C.spec.as<double>(), C.T2.as<double2>(), tw);
-    launch_rows_inv(st, C.T2.as<double2>(), C.Px, C.nx, C.ny, out, tw);
+    launch_rows_fwd(pl->ctx, S, C.Px, (C.ny + 1) / 2, C.ny, C.T1.as<double2>());
+    launch_cols<0>(pl->ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.ny, C.ny, C.spec.as<double>(), 1.0,
+                   C.T2.as<double2>(), nullptr);
+    launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
 }
 
 void build_source_pairs(ilh_plan* pl, SourceSet& s)
 {
     const std::vector<int>& rp = s.bins.h_row_ptr;
-    std::vector<int> pairs;
     const int ny = s.bins.ny;
+    std::vector<int> pairs, slot(ny, -1);
     for (int j = 0; j < ny; j += 2) {
         const bool a = rp[j + 1] > rp[j];
         const bool b = (j + 1 < ny) && (rp[j + 2] > rp[j + 1]);
-        if (a || b) pairs.push_back(j / 2);
+        if (a || b) {
+            const int k = (int)pairs.size();
+            pairs.push_back(j / 2);
+            slot[j] = 2 * k;
+            if (j + 1 < ny) slot[j + 1] = 2 * k + 1;
+        }
     }
     s.npairs = (int)pairs.size();
     if (s.pairs) (void)hipFree(s.pairs);
+    if (s.slot_of_row) (void)hipFree(s.slot_of_row);
     ILH_HIP(hipMalloc(&s.pairs, sizeof(int) * (size_t)std::max(1, s.npairs)));
+    ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * (size_t)std::max(1, ny)));
+    hipStream_t st = pl->ctx->stream;
     if (s.npairs)
-        ILH_HIP(hipMemcpyAsync(s.pairs, pairs.data(), sizeof(int) * (size_t)s.npairs, hipMemcpyHostToDevice,
-                               pl->ctx->stream));
-    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+        ILH_HIP(hipMemcpyAsync(s.pairs, pairs.data(), sizeof(int) * (size_t)s.npairs, hipMemcpyHostToDevice, st));
+    ILH_HIP(hipMemcpyAsync(s.slot_of_row, slot.data(), sizeof(int) * (size_t)ny, hipMemcpyHostToDevice, st));
+    ILH_HIP(hipStreamSynchronize(st));
 }
 
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, hipEvent_t* ev)
 {
     ConvPlan& C = get_conv(pl, gk);
     hipStream_t st = pl->ctx->stream;
-    const double2* tw = pl->ctx->twiddle.as<double2>();
     RowSrc S{};
     S.mode = ROW_TERMS;
     S.nx = C.nx;
     S.ny = C.ny;
-    S.terms = src.terms;
-    S.ptr = src.bins.row_ptr;
-    S.ent = src.bins.entries;
     S.ebx = src.bins.entries + src.bins.n_entries;
+    S.eai = src.bins.eai;
+    S.erw = src.bins.erw;
+    S.cell_ptr = src.bins.cell_ptr;
+    S.cells = src.bins.cells;
     S.amp = amp;
     S.pairs = src.pairs;
-    if (src.npairs > 0) launch_rows_fwd(st, S, C.Px, C.ny, src.npairs, C.T1.as<double2>(), tw);
+    launch_rows_fwd(pl->ctx, S, C.Px, src.npairs, C.ny, C.T1.as<double2>());
     if (ev) ILH_HIP(hipEventRecord(ev[0], st));
-    launch_cols(st, C.T1.as<double2>(), src.pairs, 2 * src.npairs, C.Px, C.Py, C.ny, C.spec.as<double>(),
-                C.T2.as<double2>(), tw);
+    launch_cols<0>(pl->ctx, C.T1.as<double2>(), src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0,
+                   C.T2.as<double2>(), nullptr);
     if (ev) ILH_HIP(hipEventRecord(ev[1], st));
-    launch_rows_inv(st, C.T2.as<double2>(), C.Px, C.nx, C.ny, out, tw);
+    launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
 }
 
 }  // namespace ilh

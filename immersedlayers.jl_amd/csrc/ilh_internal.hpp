@@ -75,12 +75,21 @@ bool op_is_scatter(int op);
 // Row-binned index of scatter terms on one component grid:
 // entries[row_ptr[j] .. row_ptr[j+1]) are term indices whose window touches
 // row j, sorted by (bx, term index).
+// Each entry also carries its record: amplitude index and the five x-weights
+// with the term's coefficient and its row weight folded in (erw[5k + e]).
 struct RowBins {
     int gk = 0;
     int ny = 0;
     int* row_ptr = nullptr;     // device, ny + 1
-    int* entries = nullptr;     // device
+    int* entries = nullptr;     // device: term indices, then their bx
+    int* eai = nullptr;         // device: amplitude index per entry
+    double* erw = nullptr;      // device: 5 weights per entry
     int n_entries = 0;
+    // nonzero cells of each row, ascending x: cells[3c] = x, [3c+1, 3c+2) = the
+    // entries k (bx in [x-4, x]) that contribute erw[5k + x - bx] to it
+    int* cell_ptr = nullptr;    // device, ny + 1
+    int* cells = nullptr;       // device, 3 per cell
+    int n_cells = 0;
     std::vector<int> h_row_ptr; // host copy (row counts drive launch shapes)
 };
 
@@ -100,8 +109,8 @@ struct ConvPlan {
     int nx = 0, ny = 0;       // field dims
     int Px = 0, Py = 0;       // padded periods (powers of two, >= 2n-2)
     int H = 0;                // Px/2 + 1 (stored spectrum columns)
-    DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], H x Py
-    DevBuf T1, T2;            // intermediate spectra, rows x H complex
+    DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], ky <= Py/2 (even in ky)
+    DevBuf T1, T2;            // intermediate spectra, rows x H complex, 8-row tiles
 };
 
 }  // namespace ilh
@@ -169,6 +178,7 @@ struct SourceSet {
     const Term* terms = nullptr;
     RowBins bins;
     int* pairs = nullptr;      // device list of nonzero row pairs
+    int* slot_of_row = nullptr;  // device: field row -> row of the compacted spectrum, or -1
     int npairs = 0;
 };
 void build_source_pairs(ilh_plan* pl, SourceSet& s);
@@ -180,13 +190,15 @@ void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_k
               double* K /* N x N col-major */);
 int  lu_factor(ilh_plan* pl, double* A, int n, int* piv_dev, int* info_dev);
 void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, double* B, int nrhs);
-void transpose(hipStream_t st, const double* A, double* At, int rows, int cols);
-void matvec_rows(hipStream_t st, const double* Mrow, const double* x, double* y, int n, int m,
+// row-major solve matrices use a leading dimension padded to a multiple of 32
+inline int solve_ld(int n) { return (n + 31) / 32 * 32; }
+void transpose(hipStream_t st, const double* A, double* At, int rows, int cols, int ldt);
+void matvec_rows(hipStream_t st, const double* Mrow, int ld, const double* x, double* y, int n, int m,
                  double alpha, int accumulate);
-void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, const double* fp,
-                   const double* fm, double* amp, int n);
+void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, int ld, const double* fp,
+                   const double* fm, double* amp, double* s_out, int n);
 void neumann_amp(hipStream_t st, const double* Mdf_p, const double* Mdf_m, const double* Mds_p,
-                 const double* Mds_m, const double* vp, const double* vm, double* amp_f, double* amp_s,
+                 const double* Mds_m, int ld, const double* vp, const double* vm, double* amp_f, double* amp_s,
                  double* df, double* dsv, int n);
 void check_finite(hipStream_t st, const double* a, size_t n, int* flag_dev);
 

@@ -313,6 +313,95 @@ __global__ void k_bin_bx(const Term* __restrict__ t, const int* __restrict__ ent
     if (k < n) ebx[k] = t[ent[k]].bx;
 }
 
+// entry records of row j: amplitude index and coef * wy[j - by] * wx[e]
+__global__ void k_bin_rec(const Term* __restrict__ t, const int* __restrict__ ptr, const int* __restrict__ ent,
+                          int* __restrict__ eai, double* __restrict__ erw)
+{
+    const int j = blockIdx.x;
+    for (int k = ptr[j] + threadIdx.x; k < ptr[j + 1]; k += blockDim.x) {
+        const Term& tt = t[ent[k]];
+        const double cw = tt.coef * tt.wy[j - tt.by];
+        eai[k] = tt.ai;
+#pragma unroll
+        for (int e = 0; e < 5; ++e) erw[5 * (size_t)k + e] = cw * tt.wx[e];
+    }
+}
+
+// Cells of a row: the x positions some entry contributes to (nonzero weight).
+// A work-group per row marks them in an LDS bitmap (x < 8192).
+static constexpr int CELL_WORDS = 8192 / 32 + 1;
+
+__device__ void mark_cells(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
+                           int j, int nx, unsigned* bm)
+{
+    for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) bm[w] = 0u;
+    __syncthreads();
+    for (int k = ptr[j] + threadIdx.x; k < ptr[j + 1]; k += blockDim.x)
+        for (int e = 0; e < 5; ++e) {
+            const int x = ebx[k] + e;
+            if (x >= 0 && x < nx && erw[5 * (size_t)k + e] != 0.0) atomicOr(&bm[x >> 5], 1u << (x & 31));
+        }
+    __syncthreads();
+}
+
+__global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
+                             int nx, int* __restrict__ cnt)
+{
+    __shared__ unsigned bm[CELL_WORDS];
+    __shared__ int tot[256];
+    const int j = blockIdx.x;
+    mark_cells(ptr, ebx, erw, j, nx, bm);
+    int c = 0;
+    for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) c += __popc(bm[w]);
+    tot[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < (int)blockDim.x; ++i) s += tot[i];
+        cnt[j] = s;
+    }
+}
+
+__global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
+                            int nx, const int* __restrict__ cptr, int* __restrict__ cells)
+{
+    __shared__ unsigned bm[CELL_WORDS];
+    __shared__ int woff[CELL_WORDS];
+    const int j = blockIdx.x;
+    mark_cells(ptr, ebx, erw, j, nx, bm);
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int w = 0; w < CELL_WORDS; ++w) {
+            woff[w] = s;
+            s += __popc(bm[w]);
+        }
+    }
+    __syncthreads();
+    const int lo = ptr[j], hi = ptr[j + 1];
+    for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) {
+        unsigned bits = bm[w];
+        int rank = 0;
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int x = w * 32 + b;
+            // entries with bx in [x-4, x]: a contiguous run of the sorted row
+            int a = lo, c = hi;
+            while (a < c) {
+                const int m = (a + c) >> 1;
+                if (ebx[m] < x - 4) a = m + 1; else c = m;
+            }
+            int e = a;
+            while (e < hi && ebx[e] <= x) ++e;
+            int* cl = cells + 3 * (size_t)(cptr[j] + woff[w] + rank);
+            cl[0] = x;
+            cl[1] = a;
+            cl[2] = e;
+            ++rank;
+        }
+    }
+}
+
 void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
 {
     hipStream_t st = pl->ctx->stream;
@@ -334,10 +423,29 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
     ILH_HIP(hipStreamSynchronize(st));
     b.n_entries = b.h_row_ptr[b.ny];
     ILH_HIP(hipMalloc(&b.entries, sizeof(int) * (size_t)std::max(1, 2 * b.n_entries)));
+    ILH_HIP(hipMalloc(&b.eai, sizeof(int) * (size_t)std::max(1, b.n_entries)));
+    ILH_HIP(hipMalloc(&b.erw, sizeof(double) * 5 * (size_t)std::max(1, b.n_entries)));
     if (nterms > 0 && b.n_entries > 0) {
         k_bin_fill<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, b.row_ptr, cursor, b.entries);
         k_bin_sort<<<b.ny, 256, 0, st>>>(terms, b.row_ptr, b.entries);
         k_bin_bx<<<(b.n_entries + 255) / 256, 256, 0, st>>>(terms, b.entries, b.n_entries, b.entries + b.n_entries);
+        k_bin_rec<<<b.ny, 64, 0, st>>>(terms, b.row_ptr, b.entries, b.eai, b.erw);
+        ILH_HIP(hipGetLastError());
+    }
+    // cells
+    const int nx = pl->g.nx(gk);
+    if (nx > 8192) fail(ILH_ERR_ARG, "grid too wide for the cell index (nx > 8192)");
+    ILH_HIP(hipMalloc(&b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1)));
+    ILH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)(b.ny + 1), st));
+    if (b.n_entries > 0)
+        k_cell_count<<<b.ny, 256, 0, st>>>(b.row_ptr, b.entries + b.n_entries, b.erw, nx, cnt);
+    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.cell_ptr);
+    ILH_HIP(hipGetLastError());
+    ILH_HIP(hipMemcpyAsync(&b.n_cells, b.cell_ptr + b.ny, sizeof(int), hipMemcpyDeviceToHost, st));
+    ILH_HIP(hipStreamSynchronize(st));
+    ILH_HIP(hipMalloc(&b.cells, sizeof(int) * 3 * (size_t)std::max(1, b.n_cells)));
+    if (b.n_cells > 0) {
+        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, b.entries + b.n_entries, b.erw, nx, b.cell_ptr, b.cells);
         ILH_HIP(hipGetLastError());
     }
     ILH_HIP(hipStreamSynchronize(st));
@@ -348,51 +456,55 @@ void free_bins(RowBins& b)
 {
     if (b.row_ptr) (void)hipFree(b.row_ptr);
     if (b.entries) (void)hipFree(b.entries);
+    if (b.eai) (void)hipFree(b.eai);
+    if (b.erw) (void)hipFree(b.erw);
+    if (b.cell_ptr) (void)hipFree(b.cell_ptr);
+    if (b.cells) (void)hipFree(b.cells);
+    b.cell_ptr = nullptr;
+    b.cells = nullptr;
+    b.n_cells = 0;
     b.row_ptr = nullptr;
     b.entries = nullptr;
+    b.eai = nullptr;
+    b.erw = nullptr;
     b.n_entries = 0;
     b.h_row_ptr.clear();
 }
 
 // --------------------------------------------------------------- scatter
-// out(x, j) = sum over the row's terms with bx <= x < bx+5, in (bx, term) order.
-__device__ __forceinline__ double scatter_cell(const Term* __restrict__ t, const int* __restrict__ ent,
-                                               const int* __restrict__ ebx, int lo, int hi, int x, int j,
-                                               const double* __restrict__ amp)
-{
-    // first entry with bx >= x - 4
-    int a = lo, b = hi;
-    while (a < b) {
-        const int m = (a + b) >> 1;
-        if (ebx[m] < x - 4) a = m + 1; else b = m;
-    }
-    double acc = 0.0;
-    for (int k = a; k < hi; ++k) {
-        const int bx = ebx[k];
-        if (bx > x) break;
-        const Term& tt = t[ent[k]];
-        acc += amp[tt.ai] * tt.coef * tt.wy[j - tt.by] * tt.wx[x - bx];
-    }
-    return acc;
-}
-
-__global__ void k_scatter(const Term* __restrict__ t, const int* __restrict__ ptr, const int* __restrict__ ent,
-                          const int* __restrict__ ebx, int nx, const double* __restrict__ amp,
+// out(x, j) = sum over the row's entries with bx <= x < bx+5, in (bx, term)
+// order, of amp[ai] * w[x - bx].
+__global__ void k_scatter(const int* __restrict__ ptr, const int* __restrict__ ebx, const int* __restrict__ eai,
+                          const double* __restrict__ erw, int nx, const double* __restrict__ amp,
                           double* __restrict__ out)
 {
     const int j = blockIdx.y;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nx) return;
     const int lo = ptr[j], hi = ptr[j + 1];
-    out[x + (size_t)nx * j] = (lo == hi) ? 0.0 : scatter_cell(t, ent, ebx, lo, hi, x, j, amp);
+    double acc = 0.0;
+    if (lo < hi) {
+        int a = lo, b = hi;
+        while (a < b) {
+            const int m = (a + b) >> 1;
+            if (ebx[m] < x - 4) a = m + 1; else b = m;
+        }
+        for (int k = a; k < hi; ++k) {
+            const int bx = ebx[k];
+            if (bx > x) break;
+            acc += amp[eai[k]] * erw[5 * (size_t)k + (x - bx)];
+        }
+    }
+    out[x + (size_t)nx * j] = acc;
 }
 
 void scatter(ilh_plan* pl, const Term* terms, const RowBins& bins, const double* amp, double* out)
 {
+    (void)terms;
     const int nx = pl->g.nx(bins.gk);
     dim3 grid((nx + 255) / 256, bins.ny);
-    k_scatter<<<grid, 256, 0, pl->ctx->stream>>>(terms, bins.row_ptr, bins.entries, bins.entries + bins.n_entries,
-                                                 nx, amp, out);
+    k_scatter<<<grid, 256, 0, pl->ctx->stream>>>(bins.row_ptr, bins.entries + bins.n_entries, bins.eai, bins.erw, nx,
+                                                 amp, out);
     ILH_HIP(hipGetLastError());
 }
 

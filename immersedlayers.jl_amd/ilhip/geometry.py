@@ -151,15 +151,15 @@ def config_microbench(seed=2, npts_per_curve=2048):
     return g, concat(surfs)
 
 
-def config_headline(seed=5, spacing=1.0):
+def config_headline(seed=5, spacing=1.2):
     """1024² Nodes{Primal} on [-2,2]² (Δx=4/1024): an ellipse of aspect 2 and a
     random star curve, about 1000 points each at ``spacing``·Δx."""
     dx = 4.0 / 1024
     g = GridSpec(1025, 1025, 513, 513, dx)
     rng = np.random.default_rng(seed)
-    e = ellipse(0.8, 0.4, spacing * dx, -0.95, 0.0)
+    e = ellipse(0.95, 0.475, spacing * dx, -1.0, 0.0)
     bk = rng.uniform(-0.1, 0.1, 5)
-    st = star(0.55, bk, spacing * dx, 0.95, 0.05)
+    st = star(0.62, bk, spacing * dx, 1.0, 0.0)
     s = concat([e, st])
     fb, _, _ = fourier_data(s.theta(), rng)
     return g, s, fb

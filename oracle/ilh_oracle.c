@@ -830,6 +830,22 @@ static int dense_solve(int n, const double* S, double* b)
     return info;
 }
 
+/* The reference keeps the Laplacian plan (LGF table + FFT plans) in the cache
+   (src/cache.jl:170-178); the composites reuse the last plans built here. */
+static orc_lap* cached_lap(const orc_grid* g, int kind)
+{
+    static orc_grid keys[2];
+    static orc_lap* plans[2];
+    static int used[2];
+    for (int k = 0; k < 2; ++k)
+        if (used[k] && plans[k] && memcmp(&keys[k], g, sizeof(*g)) == 0 && k == kind) return plans[k];
+    if (used[kind]) orc_lap_destroy(plans[kind]);
+    plans[kind] = orc_lap_create(g, kind);
+    keys[kind] = *g;
+    used[kind] = 1;
+    return plans[kind];
+}
+
 /* ---------------------------------------------------------- composites */
 /* test/literate/problems.jl:259-298 (no forcing). */
 int orc_dirichlet_solve(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
@@ -843,7 +859,7 @@ int orc_dirichlet_solve(const orc_grid* g, int ddf, int N, const double* X, cons
         orc_schur_direct(g, ddf, N, X, Y, nx, ny, ds, ORC_S_RTLINVR, 1.0, Sown);
         S = Sown;
     }
-    orc_lap* L = orc_lap_create(g, ORC_PRIMAL);
+    orc_lap* L = cached_lap(g, ORC_PRIMAL);
     double* fb = dalloc(N);
     double* fstar = dalloc(ng);
     for (int p = 0; p < N; ++p) fb[p] = fplus[p] - (fminus ? fminus[p] : 0.0);   /* jump */
@@ -858,7 +874,6 @@ int orc_dirichlet_solve(const orc_grid* g, int ddf, int N, const double* X, cons
     orc_inverse_laplacian(L, f, f);
     for (size_t k = 0; k < ng; ++k) f[k] += fstar[k];
     free(fb); free(fstar); free(Sown);
-    orc_lap_destroy(L);
     return info;
 }
 
@@ -875,8 +890,8 @@ int orc_neumann_solve(const orc_grid* g, int ddf, int N, const double* X, const 
         orc_schur_direct(g, ddf, N, X, Y, nx, ny, ds, ORC_S_CLINVCT, 1.0, Sown);
         S = Sown;
     }
-    orc_lap* Lp = orc_lap_create(g, ORC_PRIMAL);
-    orc_lap* Ld = orc_lap_create(g, ORC_DUAL);
+    orc_lap* Lp = cached_lap(g, ORC_PRIMAL);
+    orc_lap* Ld = cached_lap(g, ORC_DUAL);
     double* dvn = dalloc(N); double* vn = dalloc(N);
     double* fstar = dalloc(ng); double* sstar = dalloc(nd);
     for (int p = 0; p < N; ++p) {
@@ -900,6 +915,5 @@ int orc_neumann_solve(const orc_grid* g, int ddf, int N, const double* X, const 
     for (size_t k = 0; k < nd; ++k) { s[k] -= sstar[k]; s[k] *= -1.0; }
     orc_inverse_laplacian(Ld, s, s);
     free(dvn); free(vn); free(fstar); free(sstar); free(Sown);
-    orc_lap_destroy(Lp); orc_lap_destroy(Ld);
     return info;
 }
