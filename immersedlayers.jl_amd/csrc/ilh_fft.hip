@@ -316,6 +316,16 @@ __device__ __forceinline__ void run_stages(const StagePlan& sp, double2 (&v)[V],
     }
 }
 
+// Work-groups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
+// Renumber blocks so that each XCD gets a run of consecutive logical blocks:
+// neighbours that share 128-byte lines of the tiled spectra then meet in one L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int nblk = gridDim.x, full = (nblk / 8) * 8;
+    const int b = blockIdx.x;
+    return b < full ? (b % 8) * (nblk / 8) + b / 8 : b;
+}
+
 // element (q, r) of the first stage's inputs or the last stage's outputs
 __device__ __forceinline__ int io_index(int R, int q, int r, int t, int T, int P)
 {
@@ -369,7 +379,7 @@ __global__ void __launch_bounds__(512) k_rows_fwd(RowSrc S, FftDev F, int npairs
     const int P = F.P, T = F.T, H = P / 2 + 1;
     const int nb = blockDim.x / T;
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-    const int p = blockIdx.x * nb + c;
+    const int p = xcd_block() * nb + c;
     const bool live = p < npairs;
     const int j0 = live ? (S.pairs ? 2 * S.pairs[p] : 2 * p) : 0;
     double2* x = lds + (size_t)c * lds_len(P);
@@ -450,9 +460,7 @@ __global__ void __launch_bounds__(512) k_cols(const double2* __restrict__ T1, co
     const int cl = threadIdx.x / T, t = threadIdx.x - cl * T;
     // blocks are dispatched round-robin over the 8 XCDs: give each XCD a run of
     // consecutive columns so that their 128-byte row segments meet in its L2
-    const int nblk = gridDim.x, full = (nblk / 8) * 8;
-    const int b = (int)blockIdx.x < full ? (blockIdx.x % 8) * (nblk / 8) + blockIdx.x / 8 : blockIdx.x;
-    const int c = b * nb + cl;
+    const int c = xcd_block() * nb + cl;
     const bool live = c < H;
     double2* x = lds + (size_t)cl * lds_len(P);
     const int R0 = F.fwd.R[0];
@@ -509,7 +517,7 @@ __global__ void __launch_bounds__(512) k_rows_inv(const double2* __restrict__ T2
     const int P = F.P, T = F.T, H = P / 2 + 1;
     const int nb = blockDim.x / T;
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-    const int p = blockIdx.x * nb + c;
+    const int p = xcd_block() * nb + c;
     const int j0 = 2 * p;
     const bool live = j0 < ny;
     const bool two = j0 + 1 < ny;

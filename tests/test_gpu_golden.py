@@ -1,0 +1,129 @@
+"""GPU tests against the golden fixtures, the reference's published Neumann
+added mass, headline-size parity, determinism and error paths.
+
+Tolerances: FP64 solves agree with the oracle to relative L2 <= 1e-10.  The
+surface strength s of the 1024^2 workload is the solution of a Schur system
+with condition number ~3e5, so two independent FP64 computations (different
+summation orders) agree to ~kappa*eps; its bar is 1e-9.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _load(name):
+    import ilhip
+    d = np.load(os.path.join(ROOT, "tests", "golden", name))
+    NX, NY, I0x, I0y = (int(v) for v in d["grid"])
+    g = ilhip.PhysicalGrid(NX, NY, I0x, I0y, float(d["dx"]))
+    from ilhip.geometry import Surface
+    s = Surface(d["X"], d["Y"], d["nx"], d["ny"], d["ds"], d["body"] if "body" in d else np.zeros(len(d["X"]), int))
+    return d, g, s
+
+
+def test_dirichlet_golden(ctx):
+    import ilhip
+    d, g, s = _load("dirichlet_plain.npz")
+    print("dirichlet golden N", s.N, flush=True)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    f, sv = ilhip.DirichletPoisson(cache).solve(d["fb"])
+    assert rel_err(f.flat(), d["f"]) < 1e-10
+    assert rel_err(sv.flat(), d["s"]) < 1e-10
+
+
+def test_neumann_added_mass(ctx):
+    """examples/neumann.ipynb cell 37: M = [0.22649277527914002, ~0]."""
+    import ilhip
+    d, g, s = _load("neumann_square_circle.npz")
+    print("neumann added mass N", s.N, flush=True)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    f, df, st, ds = ilhip.NeumannPoisson(cache).solve(d["vp"])
+    m = s.body == 1
+    Mx = -np.sum(df.flat()[m] * s.nx[m] * s.ds[m])
+    My = -np.sum(df.flat()[m] * s.ny[m] * s.ds[m])
+    assert abs(Mx - 0.22649277527914002) < 1e-11
+    assert abs(My) < 1e-9
+    assert rel_err(df.flat(), d["df"]) < 1e-10
+    assert rel_err(ds.flat(), d["dsv"]) < 1e-10
+    assert abs(np.linalg.norm(f.flat()) - d["f_norm"]) < 1e-10 * d["f_norm"]
+    assert abs(np.linalg.norm(st.flat()) - d["s_norm"]) < 1e-10 * d["s_norm"]
+    assert np.abs(f.flat()[d["probe"]] - d["f_probe"]).max() < 1e-10 * np.abs(d["f_probe"]).max()
+
+
+def test_headline_parity_and_determinism(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    g, s, fb = G.config_headline()
+    print("headline parity N", s.N, flush=True)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    prob = ilhip.DirichletPoisson(cache)
+    f1, s1 = prob.solve(fb)
+    f2, s2 = prob.solve(fb)
+    assert np.array_equal(f1.flat(), f2.flat()) and np.array_equal(s1.flat(), s2.flat())   # bitwise
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+    fr, sr = O.dirichlet_solve(og, geo, fb)
+    assert rel_err(f1.flat(), fr) < 1e-10
+    assert rel_err(s1.flat(), sr) < 1e-9
+
+
+def test_moving_geometry_update(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    g, s, fb = G.config_plain()
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    prob = ilhip.DirichletPoisson(cache)
+    prob.solve(fb)
+    s2 = G.circle(0.5, 1.4 * g.dx, cx=0.13, cy=-0.07)
+    assert s2.N == s.N
+    cache.update_points(s2)
+    f, sv = prob.solve(fb)
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    fr, sr = O.dirichlet_solve(og, O.Geometry(s2.X, s2.Y, s2.nx, s2.ny, s2.ds), fb)
+    assert rel_err(f.flat(), fr) < 1e-10 and rel_err(sv.flat(), sr) < 1e-10
+
+
+def test_error_paths(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    g, s, fb = G.config_plain()
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    with pytest.raises(ilhip.DimensionMismatch):
+        ilhip.DirichletPoisson(cache).solve(fb[:-1])
+    with pytest.raises(TypeError):
+        ilhip.regularize(ilhip.ScalarData(s.N), ilhip.ScalarData(s.N), cache)
+    bad = G.circle(0.5, 1.4 * g.dx, cx=1.98)          # support leaves the grid interior
+    edge = ilhip.SurfaceScalarCache(bad, g, ctx=ctx)
+    out = ilhip.Nodes(ilhip.Primal, g)
+    ilhip.regularize(out, ilhip.ScalarData(bad.N, np.ones(bad.N)), edge)   # clipped, allowed
+    with pytest.raises(ilhip.RangeError):
+        ilhip.DirichletPoisson(edge).solve(np.ones(bad.N))
+    nan = G.circle(0.5, 1.4 * g.dx)
+    nan.X[3] = np.nan
+    with pytest.raises(ilhip.NonFiniteError):
+        ilhip.SurfaceScalarCache(nan, g, ctx=ctx)
+    with pytest.raises(ilhip.StateError):
+        from ilhip import _lib
+        _lib.call("ilh_schur_solve", cache.h, 0, np.zeros(s.N).ctypes.data_as(__import__("ctypes").c_void_p),
+                  np.zeros(s.N).ctypes.data_as(__import__("ctypes").c_void_p), 0)
+
+
+def test_empty_body_list(ctx):
+    """No surface points: operators give exact zeros (test/surface_ops.jl:355-371)."""
+    import ilhip
+    from ilhip.geometry import GridSpec, Surface
+    g = GridSpec(40, 40, 20, 20, 0.1)
+    e = np.zeros(0)
+    cache = ilhip.SurfaceScalarCache(Surface(e, e, e, e, e, np.zeros(0, int)), g, ctx=ctx)
+    out = ilhip.Nodes(ilhip.Primal, g, np.ones(g.primal_shape()))
+    ilhip.regularize(out, ilhip.ScalarData(0), cache)
+    assert np.all(out.data == 0.0)
+    f, s = ilhip.DirichletPoisson(cache).solve(np.zeros(0))
+    assert np.all(f.data == 0.0) and s.flat().size == 0

@@ -1,0 +1,289 @@
+"""CPU tests: the FP64 oracle pinned against the reference's own checks,
+its published numbers, closed forms and brute force.
+
+Pins (reference paths relative to the reference checkout):
+  * Neumann added mass of Circle(0.25) in Square(1), dx = 0.01 on [-2,2]^2:
+    M_x = 0.22649277527914002 (examples/neumann.ipynb cell 37, from
+    test/literate/neumann.jl:286-287).
+  * sum(ds) of Circle(1, 0.014) = 6.283288300933757 (examples/caches.ipynb cell 28);
+    integral of x.n over a Circle(0.5, 0.014) = 1.5708993266236526
+    (examples/multbodies.ipynb cell 16).
+  * partition of unity and adjointness of Rc/Ec, Rf integral (test/tools.jl:100-124).
+  * RnTRn, create_CLinvCT, create_GLinvD, create_nRTRn magnitudes and the mask
+    area (test/surface_ops.jl:60-77, 184-188).
+  * Yang3 moments, LGF closed values G(1,0)=1/4, G(1,1)=1/pi, G(2,0)=1-2/pi,
+    G(n,n) = (1/pi) sum_{k<=n} 1/(2k-1), L G = delta, far field.
+  * FFT convolution against brute-force convolution; L(L^-1 w) = w.
+  * MAC identities D G = L5, D C = 0, C G = 0, D = -G^T.
+  * Schur direct assembly against the reference's column loop.
+  * Dirichlet constraint E f = fbar and convergence to the exact exterior field.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import max_rel, rel_err
+
+
+def _geo(s):
+    return O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+
+
+def _G():
+    from ilhip import geometry as G
+    return G
+
+
+# ------------------------------------------------------------------- DDFs
+def test_yang3_values():
+    assert abs(O.ddf(O.YANG3, 0.0) - 0.618199929359) < 1e-11
+    assert abs(O.ddf(O.YANG3, 1.0) - 0.190900035320) < 1e-11
+    assert abs(O.ddf(O.YANG3, 2.0)) < 1e-15
+    assert O.ddf(O.YANG3, 2.5) == 0.0
+
+
+@pytest.mark.parametrize("kind", [O.YANG3, O.ROMA, O.M4PRIME, O.WITCHHAT])
+def test_ddf_moments(kind):
+    for r in np.linspace(-0.5, 0.5, 11):
+        w = np.array([O.ddf(kind, r - j) for j in range(-4, 5)])
+        d = np.array([r - j for j in range(-4, 5)])
+        assert abs(w.sum() - 1.0) < 1e-14
+        assert abs((d * w).sum()) < 1e-14
+
+
+# -------------------------------------------------------------------- LGF
+def test_lgf_closed_values():
+    assert O.lgf(0, 0) == 0.0
+    assert abs(O.lgf(1, 0) - 0.25) < 1e-15
+    assert abs(O.lgf(0, 1) - 0.25) < 1e-15
+    assert abs(O.lgf(1, 1) - 1 / np.pi) < 1e-15
+    assert abs(O.lgf(2, 0) - (1 - 2 / np.pi)) < 1e-15
+    for n in (3, 17, 100, 1000, 2047):
+        exact = sum(1.0 / (2 * k - 1) for k in range(1, n + 1)) / np.pi
+        assert abs(O.lgf(n, n) - exact) < 4e-15 * exact
+
+
+def test_lgf_is_discrete_green_function():
+    T = O.lgf_table(160, 160)
+    a = np.abs(np.arange(-159, 160))
+    G = T[a[:, None], a[None, :]]
+    LG = G[:-2, 1:-1] + G[2:, 1:-1] + G[1:-1, :-2] + G[1:-1, 2:] - 4 * G[1:-1, 1:-1]
+    c = LG.shape[0] // 2
+    assert abs(LG[c, c] - 1.0) < 1e-13
+    LG[c, c] = 0.0
+    assert np.abs(LG).max() < 1e-13
+
+
+def test_lgf_far_field():
+    # G = (1/2pi)(ln r + gamma + 3/2 ln 2) - cos(4 theta)/(24 pi r^2) + O(r^-4)
+    for m, n in [(300, 0), (200, 200), (310, 170), (1000, 37)]:
+        r2 = m * m + n * n
+        th = np.arctan2(n, m)
+        ff = (0.5 * np.log(r2) + np.euler_gamma + 1.5 * np.log(2)) / (2 * np.pi) - np.cos(4 * th) / (24 * np.pi * r2)
+        assert abs(O.lgf(m, n) - ff) < 5.0 / r2 ** 2
+
+
+@pytest.mark.parametrize("kind", [O.PRIMAL, O.DUAL])
+def test_inverse_laplacian_vs_direct(kind):
+    g = O.grid(23, 18, 11, 9, 0.1)
+    nx, ny = O.dims(g, kind)
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal((nx, ny))
+    out = O.InverseLaplacian(g, kind)(w.ravel(order="F"))
+    ref = O.conv_direct(O.lgf_table(40, 40), w).ravel(order="F") * g.dx ** 2
+    assert max_rel(out, ref) < 2e-15 * 10
+    lap = O.laplacian(g, kind, out).reshape((nx, ny), order="F")
+    assert np.abs(lap[1:-1, 1:-1] - w[1:-1, 1:-1]).max() < 1e-12
+
+
+# ---------------------------------------------------- regularize/interpolate
+def _tools_setup():
+    # test/tools.jl: dx = 0.02 on +-5.98, Circle(1, 1.5 dx), Nodes{Dual}
+    G = _G()
+    g = O.grid(602, 602, 301, 301, 0.02)
+    s = G.circle(1.0, 1.5 * 0.02)
+    return g, s
+
+
+def test_partition_of_unity_and_adjointness():
+    g, s = _tools_setup()
+    geo = _geo(s)
+    ones = np.ones(s.N)
+    Rc1 = O.regularize(g, O.DUAL, geo, ones)
+    assert abs(g.dx ** 2 * Rc1.sum() - s.ds.sum()) < 1e-14            # tools.jl:107
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(O.nel(g, O.DUAL))
+    phi = rng.standard_normal(s.N)
+    lhs = g.dx ** 2 * np.dot(u, O.regularize(g, O.DUAL, geo, phi))
+    rhs = np.dot(O.interpolate(g, O.DUAL, geo, u) * phi, s.ds)
+    assert abs(lhs - rhs) < 1e-14 * max(1.0, abs(lhs)) * 10          # tools.jl:112
+    # Rf integral (tools.jl:119-120): u-component of Rf with f.u = 1
+    Rfu = O.regularize(g, O.EDGE_U, geo, ones)
+    assert abs(g.dx ** 2 * Rfu.sum() - s.ds.sum()) < 1e-12
+    assert abs(g.dx ** 2 * Rfu.sum() - 2 * np.pi) < 1e-3
+
+
+def test_interpolation_reproduces_linear_fields():
+    G = _G()
+    g = O.grid(120, 110, 60, 55, 0.03)
+    s = G.star(0.8, [0.05, -0.03, 0.02, 0.0, 0.01], 1.4 * 0.03)
+    geo = _geo(s)
+    gs = G.GridSpec(120, 110, 60, 55, 0.03)
+    for kind, xs, ys in ((O.PRIMAL, gs.x_primal(), gs.y_primal()), (O.DUAL, gs.x_dual(), gs.y_dual())):
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        for fld, exp in ((np.ones_like(X), np.ones(s.N)), (X, s.X), (Y, s.Y), (2 * X - 3 * Y, 2 * s.X - 3 * s.Y)):
+            out = O.interpolate(g, kind, geo, fld.ravel(order="F"))
+            assert np.abs(out - exp).max() < 1e-13
+
+
+def test_regularize_brute_force():
+    # dense R from the DDF on a tiny grid (R_ij,p = ds_p/dx^2 phi(.)phi(.))
+    G = _G()
+    gs = G.GridSpec(24, 22, 12, 11, 0.1)
+    g = O.grid(gs.NX, gs.NY, gs.I0x, gs.I0y, gs.dx)
+    s = G.circle(0.5, 0.13)
+    f = np.random.default_rng(2).standard_normal(s.N)
+    xs, ys = gs.x_primal(), gs.y_primal()
+    ref = np.zeros((len(xs), len(ys)))
+    for p in range(s.N):
+        wx = np.array([O.ddf(O.YANG3, (x - s.X[p]) / gs.dx) for x in xs])
+        wy = np.array([O.ddf(O.YANG3, (y - s.Y[p]) / gs.dx) for y in ys])
+        ref += s.ds[p] / gs.dx ** 2 * f[p] * np.outer(wx, wy)
+    out = O.regularize(g, O.PRIMAL, _geo(s), f).reshape(ref.shape, order="F")
+    assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+# ------------------------------------------------------------------ stencils
+def test_mac_identities():
+    g = O.grid(31, 27, 15, 13, 0.05)
+    rng = np.random.default_rng(9)
+    npr, nd, ne = O.nel(g, O.PRIMAL), O.nel(g, O.DUAL), O.edge_len(g)
+    p = rng.standard_normal(npr)
+    s = rng.standard_normal(nd)
+    # D G = 5-point Laplacian on interior primal nodes
+    DG = O.divergence(g, O.grad(g, p)).reshape((30, 26), order="F")
+    L5 = O.laplacian(g, O.PRIMAL, p).reshape((30, 26), order="F")
+    assert np.abs(DG[1:-1, 1:-1] - L5[1:-1, 1:-1]).max() < 1e-10
+    # D C = 0 and C G = 0 (interior)
+    assert np.abs(O.divergence(g, O.curl_n2e(g, s))).max() < 1e-10
+    CG = O.curl_e2n(g, O.grad(g, p)).reshape((31, 27), order="F")
+    assert np.abs(CG).max() < 1e-10
+    # D = -G^T for edge data vanishing on boundary faces
+    q = rng.standard_normal(ne)
+    Gp = O.grad(g, p)
+    q[Gp == 0.0] = 0.0
+    assert abs(np.dot(O.divergence(g, q), p) + np.dot(q, Gp)) < 1e-9
+
+
+# --------------------------------------------------------------------- Schur
+def _circle_setup(dx=0.04, a=1.0):
+    # test/surface_ops.jl: dx = 0.04 on [-2,2]^2, Circle(1, 1.4 dx)
+    G = _G()
+    n = int(round(2.0 / dx)) + 2
+    g = O.grid(2 * n, 2 * n, n, n, dx)
+    s = G.circle(a, 1.4 * dx)
+    return g, s
+
+
+@pytest.mark.parametrize("kind", [O.S_RTLINVR, O.S_CLINVCT, O.S_GLINVD, O.S_NRTRN])
+def test_schur_direct_matches_column_loop(kind):
+    G = _G()
+    g = O.grid(60, 56, 30, 28, 0.05)
+    s = G.star(0.6, [0.08, -0.05, 0.03, 0.0, 0.02], 1.4 * 0.05)
+    A1 = O.schur(g, _geo(s), kind, method="columns")
+    A2 = O.schur(g, _geo(s), kind, method="direct")
+    assert max_rel(A2, A1) < 1e-13
+
+
+def test_surface_ops_magnitudes():
+    g, s = _circle_setup()
+    geo = _geo(s)
+    th = 2 * np.pi * np.arange(s.N) / s.N
+    f = np.sin(th - np.pi / 4)
+    v = O.normal_interpolate(g, geo, O.regularize_normal(g, geo, f))
+    assert abs(v.max() - 11) < 1 and abs(v.min() + 11) < 1                      # surface_ops.jl:66-67
+    A = O.schur(g, geo, O.S_CLINVCT, scale=g.dx)
+    assert abs(np.abs(np.linalg.eigvals(A)).max() - 0.2) < 0.1                    # :71
+    A = O.schur(g, geo, O.S_GLINVD, scale=g.dx)
+    assert abs(np.abs(np.linalg.eigvals(A)).max() - 0.45) < 0.1                   # :74
+    A = O.schur(g, geo, O.S_NRTRN)
+    assert abs(np.linalg.svd(A, compute_uv=False).max() - 11) < 1                 # :77
+
+
+def test_mask_area():
+    # mask = -L^-1 D_s 1 (src/surface_operators.jl:865-871), area of the unit circle
+    g, s = _circle_setup()
+    msk = -O.InverseLaplacian(g, O.PRIMAL)(O.surface_divergence(g, _geo(s), np.ones(s.N)))
+    assert abs(g.dx ** 2 * msk.sum() - np.pi) < 2e-3                              # surface_ops.jl:188
+
+
+# ------------------------------------------------------------ geometry anchors
+def test_circle_geometry_anchors():
+    G = _G()
+    c = G.circle(1.0, 0.014)
+    assert c.N == 448
+    # RigidBodyTools measures the panels numerically: 5.7e-13 relative off the closed form 2aN tan(pi/N)
+    assert abs(c.ds.sum() - 6.283288300933757) < 1e-11                           # caches.ipynb cell 28
+    c = G.circle(0.5, 0.014)
+    assert c.N == 224
+    xdotn = np.sum((c.X * c.nx + c.Y * c.ny) * c.ds)
+    assert abs(xdotn - 1.5708993266236526) < 1e-11                               # multbodies.ipynb cell 16
+
+
+# ----------------------------------------------------------------- composites
+def test_dirichlet_constraint_and_exact_field():
+    G = _G()
+    gs, s, fb = G.config_plain()
+    g = O.grid(gs.NX, gs.NY, gs.I0x, gs.I0y, gs.dx)
+    geo = _geo(s)
+    f, sv = O.dirichlet_solve(g, geo, fb)
+    assert np.abs(O.interpolate(g, O.PRIMAL, geo, f) - 0.5 * fb).max() < 1e-12   # E f = fbar
+    # exterior data f_b(theta) = sum a_k cos(k theta + phi_k): harmonic continuation
+    rng = np.random.default_rng(1)
+    k = np.arange(1, 9)
+    a = rng.standard_normal(8) / k
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    X, Y = np.meshgrid(gs.x_primal(), gs.y_primal(), indexing="ij")
+    R, TH = np.hypot(X, Y), np.arctan2(Y, X)
+    F = f.reshape(gs.primal_shape(), order="F")
+    out = (R > 0.65) & (R < 1.8)
+    exact = sum(a[i] * (0.5 / R[out]) ** k[i] * np.cos(k[i] * TH[out] + ph[i]) for i in range(8))
+    assert np.abs(F[out] - exact).max() < 0.02 * np.abs(fb).max()
+    assert np.abs(F[R < 0.35]).max() < 0.05 * np.abs(fb).max()                   # interior ~ 0
+
+
+def _square(a, dst):
+    """Square of half side a: points at equal spacing along each side."""
+    G = _G()
+    n = int(round(2 * a / dst))
+    h = 2 * a / n
+    s = -a + h * (np.arange(n) + 0.5)
+    X = np.concatenate([s, np.full(n, a), s[::-1], np.full(n, -a)])
+    Y = np.concatenate([np.full(n, -a), s, np.full(n, a), s[::-1]])
+    nx = np.concatenate([np.zeros(n), np.ones(n), np.zeros(n), -np.ones(n)])
+    ny = np.concatenate([-np.ones(n), np.zeros(n), np.ones(n), np.zeros(n)])
+    return G.Surface(X, Y, nx, ny, np.full(4 * n, h), np.zeros(4 * n, int))
+
+
+def test_neumann_added_mass_matches_reference():
+    """The reference's published added mass (examples/neumann.ipynb cell 37)."""
+    G = _G()
+    dx = 0.01
+    g = O.grid(406, 406, 203, 203, dx)
+    s = G.concat([_square(1.0, 1.4 * dx), G.circle(0.25, 1.4 * dx)])
+    vp = np.where(s.body == 1, s.nx, 0.0)
+    f, df, st, dsv = O.neumann_solve(g, _geo(s), vp)
+    m = s.body == 1
+    Mx = -np.sum(df[m] * s.nx[m] * s.ds[m])
+    My = -np.sum(df[m] * s.ny[m] * s.ds[m])
+    assert abs(Mx - 0.22649277527914002) < 1e-12
+    assert abs(My) < 1e-9
+
+
+def test_lu_solve():
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((50, 50))
+    b = rng.standard_normal(50)
+    info, x = O.lu_solve(A, b)
+    assert info == 0
+    assert rel_err(x, np.linalg.solve(A, b)) < 1e-12
