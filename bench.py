@@ -9,9 +9,10 @@ FP64.  One step = one solve (exterior data f+ -> grid field f and surface
 strength s) with the geometry fixed and the Schur factor cached; inputs and
 outputs stay in HBM.  Synthetic seeded data (numpy default_rng).
 
-The JSON line carries the whole-job solves/s, the roofline of the dominant
-kernel (HIP events around every kernel of every timed solve), the relative
-error against the CPU FP64 oracle, and the oracle's own solves/s on the host.
+The JSON line carries the whole-job solves/s (wall clock around K solves),
+the roofline of the dominant kernel (its stage launched alone between HIP
+events), the relative error against the CPU FP64 oracle, and the oracle's own
+solves/s on the host.
 """
 import argparse
 import json
@@ -59,6 +60,18 @@ def cpu_baseline(g, s, fb, budget_s):
                       % (n, s.N, t_build)}, f_ref, s_ref
 
 
+def measured_traffic(stage):
+    """HBM bytes per launch of a solve stage from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "dirichlet_1024_n1885_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes[%d]" % stage):
+            return float(line.split("=")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +79,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle CPU time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run 5 device copies of 256 MiB (torch copy_) as the reference "
+                         "kernel for scaling the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -73,6 +89,15 @@ def main():
         sys.exit(2)
     if os.environ.get("RANK", "0") != "0":
         return
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 5 x %d bytes read + written" % (a.numel() * 8))
 
     import ilhip
     from ilhip import geometry as G
@@ -93,14 +118,18 @@ def main():
     for _ in range(args.warmup):
         prob.solve_device(fp, None, fd, sd)
     ctx.synchronize()
-    # timed region: K solves, HIP events around every kernel, one sync at the end
+    # timed region: exactly K solves launched back to back from one host call
+    # (HIP events only around the whole loop), synchronized on both sides
     t0 = time.perf_counter()
-    stage_ms, tot_ms = prob.profile(fp, None, fd, sd, args.steps)
+    _, tot_ms = prob.profile(fp, None, fd, sd, args.steps, stages=False)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     sps = args.steps / wall
     f_dev = fd.download()
     assert np.allclose(f_dev, f_gpu.flat(), rtol=0, atol=0)
+    # per-kernel launch time: each stage launched alone, back to back, between
+    # two HIP events on the plan's stream (outside the timed region)
+    stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
 
     # algorithmic bytes per launch (FP64) of each kernel of the solve
     N, b = s.N, 8
@@ -136,7 +165,7 @@ def main():
                    "grid_primal": [nx, ny], "dx": g.dx, "npts": N, "ddf": "Yang3", "scaling": "GridScaling",
                    "fft_period": [Px, Py]},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                      "algorithmic_bytes_per_launch": abytes, "avg_launch_ms": stage_ms[dom]},
         "stage_ms": dict(zip(["matvec", "row_fft", "col_fft", "inv_row_fft"], stage_ms)),
         "event_ms_per_solve": tot_ms,

@@ -124,7 +124,6 @@ extern "C" int ilh_ctx_destroy(ilh_ctx* c)
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
-        c->twiddle.release();
         (void)hipStreamDestroy(c->stream);
         delete c;
     });
@@ -793,37 +792,39 @@ extern "C" int ilh_profile_dirichlet(ilh_plan* pl, const double* fp, const doubl
 {
     return guarded([&] {
         Io io(pl, ILH_MEM_DEVICE);
-        if (reps < 1 || reps > 100000) fail(ILH_ERR_ARG, "reps must be in [1, 100000]");
+        if (reps < 1 || reps > 1000000) fail(ILH_ERR_ARG, "reps must be in [1, 1000000]");
         SolveCache* sc = prepare_dirichlet(pl);
         const int N = pl->N;
+        if (N == 0) fail(ILH_ERR_ARG, "no surface points");
         hipStream_t st = pl->ctx->stream;
-        // five events per solve, recorded back to back; one synchronisation at the end
-        std::vector<hipEvent_t> ev((size_t)5 * reps);
-        for (auto& e : ev) ILH_HIP(hipEventCreate(&e));
-        for (int r = 0; r < reps; ++r) {
-            hipEvent_t* e = ev.data() + (size_t)5 * r;
-            ILH_HIP(hipEventRecord(e[0], st));
-            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), solve_ld(N), fp, fm,
-                          sc->amp1.as<double>(), s, N);
-            ILH_HIP(hipEventRecord(e[1], st));
-            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f, e + 2);
-            ILH_HIP(hipEventRecord(e[4], st));
-        }
-        ILH_HIP(hipEventSynchronize(ev[(size_t)5 * reps - 1]));
-        double acc[4] = {0, 0, 0, 0};
+        hipEvent_t e0, e1;
+        ILH_HIP(hipEventCreate(&e0));
+        ILH_HIP(hipEventCreate(&e1));
         float ms = 0.f;
-        for (int r = 0; r < reps; ++r) {
-            hipEvent_t* e = ev.data() + (size_t)5 * r;
-            for (int k = 0; k < 4; ++k) {
-                ILH_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
-                acc[k] += ms;
-            }
+        auto timed = [&](auto&& body) {
+            ILH_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; ++r) body();
+            ILH_HIP(hipEventRecord(e1, st));
+            ILH_HIP(hipEventSynchronize(e1));
+            ILH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        const int ld = solve_ld(N);
+        auto amp = [&] {
+            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), ld, fp, fm, sc->amp1.as<double>(), s, N);
+        };
+        // whole solves back to back, then each stage alone (stages are idempotent)
+        const double tot = timed([&] {
+            amp();
+            conv_sources(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f);
+        });
+        if (stage_ms) {
+            stage_ms[0] = timed(amp);
+            for (int k = 1; k < 4; ++k)
+                stage_ms[k] = timed([&] { conv_stage(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f, k - 1); });
         }
-        ILH_HIP(hipEventElapsedTime(&ms, ev[0], ev[(size_t)5 * reps - 1]));
-        const double tot = ms;
-        for (auto& e : ev) (void)hipEventDestroy(e);
-        for (int k = 0; k < 4; ++k)
-            if (stage_ms) stage_ms[k] = acc[k] / reps;
-        if (total_ms) *total_ms = tot / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (total_ms) *total_ms = tot;
     });
 }

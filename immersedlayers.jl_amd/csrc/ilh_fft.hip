@@ -6,23 +6,26 @@
 // G(0,0) = 0, L G = delta.  G is even, so a circular period P >= 2n-2 per axis
 // reproduces the linear convolution exactly; P is a power of two >= 16.
 //
-// Transforms are Stockham radix-V stages (plus one smaller radix stage when
-// needed) with each thread holding V complex values in registers; LDS is only
-// the exchange between stages.  A transform of length P uses P/V threads.
-// Three passes per convolution:
+// Transforms are Stockham radix-8 stages (plus one radix-2 or -4 stage when P
+// is not a power of 8), with each thread holding V = 8 complex values in
+// registers; LDS is only the exchange between stages.  Every kernel is
+// instantiated per P, so stage radices, strides and twiddle offsets are
+// compile-time constants; each stage's twiddles are loaded before the LDS
+// exchange that precedes it.  Three passes per convolution:
 //   1. row pass:    two real rows packed as one complex P_x-point FFT, split
-//                   into the two half spectra (P_x/2+1 columns).  Rows come
-//                   from a field, from the LGF table, or are generated on the
-//                   fly from the surface-point stencil terms staged in LDS (no
+//                   into the two half spectra, P_x/2 columns each: column 0
+//                   holds (DC, Nyquist) as (re, im) -- both are real.  Rows
+//                   come from a field, from the LGF table, or are generated
+//                   on the fly from the surface-point stencil terms (no
 //                   grid-sized source array is written).  Only row pairs that
 //                   hold sources are transformed.
-//   2. column pass: one spectrum column per transform: forward P_y-point FFT
-//                   whose first stage reads global memory, multiply by the
-//                   real LGF spectrum in registers (dx^2/(P_x P_y) folded in),
-//                   inverse FFT whose first stage consumes those registers and
-//                   whose last stage writes the n_y output rows.  Column
-//                   blocks are ordered so that neighbouring columns run on the
-//                   same XCD and share its L2.
+//   2. column pass: one spectrum column per transform: forward P_y-point FFT,
+//                   multiply by the real LGF spectrum in registers
+//                   (dx^2/(P_x P_y) folded in; the packed column 0 applies the
+//                   DC and Nyquist spectra to its Hermitian parts), inverse
+//                   FFT whose first stage consumes those registers and whose
+//                   last stage writes the n_y output rows.  Consecutive
+//                   columns run on the same XCD and share its L2.
 //   3. row pass:    the two half spectra of a row pair packed into one complex
 //                   inverse FFT; real/imaginary parts are the two output rows.
 // The LGF table is built here by Gauss-Legendre quadrature of
@@ -37,48 +40,54 @@
 namespace ilh {
 
 static constexpr int MAXP = 8192;
-// complex values held per thread (= largest radix); a length-P transform uses P/V threads
-static constexpr int V = 8;
+static constexpr int V = 8;   // complex values per thread; a length-P transform uses P/V threads
 
 // --------------------------------------------------------------- plans
-// Stage list of a length-P transform: radix-V stages then the remainder
-// ("fwd" order), or the remainder first ("rev" order, used by the inverse of
-// the column pass so that its first stage reads the forward's last-stage
-// registers).
-struct StagePlan {
-    int n;
-    int R[6], Ns[6], off[6];
-};
-
-struct FftDev {
-    int P, T;              // T = P/V threads per transform
-    StagePlan fwd, rev;
-    const double2* tw;     // per stage: W_{Ns R}^{r k} = exp(-2 pi i r k/(Ns R)), k < Ns, 1 <= r < R
-};
-
-static StagePlan make_plan(int P, bool rev, int& off)
+// P = 8^m * rem (rem in {1, 2, 4}).  "fwd" order: m radix-8 stages then rem;
+// "rev" order: rem first, so that the inverse of the column pass starts from
+// the registers the forward's last stage leaves.
+__host__ __device__ constexpr int n_full(int P)
 {
-    StagePlan s{};
-    int rem = P, nsx = 0;
-    int rs[6];
-    while (rem >= V) { rs[nsx++] = V; rem /= V; }
-    if (rem > 1) rs[nsx++] = rem;
-    if (rev && nsx > 1 && rs[nsx - 1] != V) {
-        const int last = rs[nsx - 1];
-        for (int k = nsx - 1; k > 0; --k) rs[k] = rs[k - 1];
-        rs[0] = last;
-    }
-    int Ns = 1;
-    for (int k = 0; k < nsx; ++k) {
-        s.R[k] = rs[k];
-        s.Ns[k] = Ns;
-        s.off[k] = off;
-        off += Ns * (rs[k] - 1);
-        Ns *= rs[k];
-    }
-    s.n = nsx;
-    return s;
+    int n = 0;
+    while (P >= V) { P /= V; ++n; }
+    return n;
 }
+__host__ __device__ constexpr int rem_of(int P)
+{
+    while (P >= V) P /= V;
+    return P;
+}
+__host__ __device__ constexpr int n_stages(int P) { return n_full(P) + (rem_of(P) > 1 ? 1 : 0); }
+__host__ __device__ constexpr int radix(int P, bool rev, int s)
+{
+    const int r = rem_of(P);
+    if (r == 1) return V;
+    if (!rev) return s < n_full(P) ? V : r;
+    return s == 0 ? r : V;
+}
+// stride of stage s: the product of the radices before it
+__host__ __device__ constexpr int nsub(int P, bool rev, int s)
+{
+    int n = 1;
+    for (int k = 0; k < s; ++k) n *= radix(P, rev, k);
+    return n;
+}
+// twiddle table: fwd stages then rev stages; stage s holds
+// W_{Ns R}^{r k} = exp(-2 pi i r k / (Ns R)) at [k (R-1) + r - 1], k < Ns, 1 <= r < R
+__host__ __device__ constexpr int tw_count(int P, bool rev)
+{
+    int c = 0;
+    for (int k = 0; k < n_stages(P); ++k) c += nsub(P, rev, k) * (radix(P, rev, k) - 1);
+    return c;
+}
+__host__ __device__ constexpr int tw_off(int P, bool rev, int s)
+{
+    int c = rev ? tw_count(P, false) : 0;
+    for (int k = 0; k < s; ++k) c += nsub(P, rev, k) * (radix(P, rev, k) - 1);
+    return c;
+}
+// threads per work-group: one transform, or several short ones
+__host__ __device__ constexpr int tpb_of(int P) { return P / V > 64 ? P / V : 64; }
 
 __global__ void k_stage_tw(double2* tw, int off, int Ns, int R)
 {
@@ -90,39 +99,27 @@ __global__ void k_stage_tw(double2* tw, int off, int Ns, int R)
     tw[off + e] = make_double2(c, -s);
 }
 
-struct FftHost {
-    FftDev d;
-    DevBuf buf;
-};
-
-static FftDev get_fft(ilh_ctx* ctx, int P)
+static const double2* get_twiddles(ilh_ctx* ctx, int P)
 {
-    // one table set per (device, P); immutable once built
-    static std::map<std::pair<int, int>, FftHost*> cache;
+    // one table per (device, P); immutable once built
+    static std::map<std::pair<int, int>, DevBuf*> cache;
     auto key = std::make_pair(ctx->device, P);
     auto it = cache.find(key);
-    if (it != cache.end()) return it->second->d;
+    if (it != cache.end()) return it->second->as<double2>();
     if (P < 16 || P > MAXP || (P & (P - 1))) fail(ILH_ERR_ARG, "FFT length must be a power of two in [16, 8192]");
-    FftHost* h = new FftHost();
-    int off = 0;
-    h->d.P = P;
-    h->d.T = P / V;
-    h->d.fwd = make_plan(P, false, off);
-    h->d.rev = make_plan(P, true, off);
-    h->buf.ensure(sizeof(double2) * (size_t)std::max(off, 1));
-    for (const StagePlan* sp : {&h->d.fwd, &h->d.rev})
-        for (int k = 0; k < sp->n; ++k) {
-            const int n = sp->Ns[k] * (sp->R[k] - 1);
-            k_stage_tw<<<(n + 255) / 256, 256, 0, ctx->stream>>>(h->buf.as<double2>(), sp->off[k], sp->Ns[k], sp->R[k]);
+    DevBuf* b = new DevBuf();
+    b->ensure(sizeof(double2) * (size_t)(tw_count(P, false) + tw_count(P, true)));
+    for (bool rev : {false, true})
+        for (int s = 0; s < n_stages(P); ++s) {
+            const int Ns = nsub(P, rev, s), R = radix(P, rev, s);
+            const int n = Ns * (R - 1);
+            k_stage_tw<<<(n + 255) / 256, 256, 0, ctx->stream>>>(b->as<double2>(), tw_off(P, rev, s), Ns, R);
         }
     ILH_HIP(hipGetLastError());
     ILH_HIP(hipStreamSynchronize(ctx->stream));
-    h->d.tw = h->buf.as<double2>();
-    cache[key] = h;
-    return h->d;
+    cache[key] = b;
+    return b->as<double2>();
 }
-
-void ensure_twiddles(ilh_ctx* ctx) { (void)ctx; }
 
 // ---------------------------------------------------------- butterflies
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
@@ -174,43 +171,11 @@ __device__ __forceinline__ void dft8(double2* a)
     a[7] = csub(e3, o3);
 }
 
-// 16 = 4 x 4:  X[k1 + 4 k2] = sum_n2 W4^{n2 k2} W16^{n2 k1} sum_n1 W4^{n1 k1} a[4 n1 + n2]
-template <int S>
-__device__ __forceinline__ void dft16(double2* a)
-{
-    const double c1 = 0.92387953251128675612818318939679;   // cos(pi/8)
-    const double s1 = 0.38268343236508977172845998403040;   // sin(pi/8)
-    const double c2 = 0.70710678118654752440084436210485;
-#pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2) dft4<S>(a[n2], a[4 + n2], a[8 + n2], a[12 + n2]);
-    // a[4 k1 + n2] holds the inner sum of (k1, n2); twiddle by W16^{n2 k1}
-    a[5] = cmul(a[5], make_double2(c1, S * s1));     // W^1
-    a[6] = cmul(a[6], make_double2(c2, S * c2));     // W^2
-    a[7] = cmul(a[7], make_double2(s1, S * c1));     // W^3
-    a[9] = cmul(a[9], make_double2(c2, S * c2));     // W^2
-    a[10] = mul_si<S>(a[10]);                        // W^4
-    a[11] = cmul(a[11], make_double2(-c2, S * c2));  // W^6
-    a[13] = cmul(a[13], make_double2(s1, S * c1));   // W^3
-    a[14] = cmul(a[14], make_double2(-c2, S * c2));  // W^6
-    a[15] = cmul(a[15], make_double2(-c1, -S * s1)); // W^9
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) dft4<S>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
-    // a[4 k1 + k2] holds X[k1 + 4 k2]: transpose into natural order
-    double2 t[16];
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1)
-#pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = a[4 * k1 + k2];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) a[k] = t[k];
-}
-
 template <int R, int S>
 __device__ __forceinline__ void dft(double2* a)
 {
-    if (R == 16) { if constexpr (V >= 16) dft16<S>(a); }
-    else if (R == 8) dft8<S>(a);
-    else if (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
+    if constexpr (R == 8) dft8<S>(a);
+    else if constexpr (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
     else dft2<S>(a[0], a[1]);
 }
 
@@ -221,19 +186,38 @@ __device__ __forceinline__ int lx(int i) { return i + (i >> 3); }
 __host__ __device__ constexpr int lds_len(int P) { return P + P / 8; }
 
 // Thread t of a transform owns butterflies j = t + q T (q < V/R) of every
-// stage; v[q R + r] is element r of butterfly q.
-template <int R, int S>
-__device__ __forceinline__ void st_compute(double2 (&v)[V], int t, int T, int Ns, const double2* __restrict__ tw)
+// stage; v[q R + r] is element r of butterfly q, read from / written to
+// index j + r P/R (first / last stage: the natural order of input / output).
+__device__ __forceinline__ int io_index(int R, int q, int r, int t, int T, int P)
 {
+    return t + q * T + r * (P / R);
+}
+
+template <int P, bool REV, int s>
+__device__ __forceinline__ void tw_load(double2 (&w)[V - 1], int t, const double2* __restrict__ tw)
+{
+    constexpr int R = radix(P, REV, s), Ns = nsub(P, REV, s), T = P / V;
+    if constexpr (Ns > 1) {
+        const double2* base = tw + tw_off(P, REV, s);
+#pragma unroll
+        for (int q = 0; q < V / R; ++q) {
+            const int k = (t + q * T) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[q * (R - 1) + r - 1] = base[k * (R - 1) + r - 1];
+        }
+    }
+}
+
+template <int P, int S, bool REV, int s>
+__device__ __forceinline__ void st_compute(double2 (&v)[V], const double2 (&w)[V - 1])
+{
+    constexpr int R = radix(P, REV, s), Ns = nsub(P, REV, s);
 #pragma unroll
     for (int q = 0; q < V / R; ++q) {
-        const int j = t + q * T;
-        const int k = j & (Ns - 1);
-        if (Ns > 1) {
-            const double2* w = tw + (size_t)k * (R - 1);
+        if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                double2 ww = w[r - 1];
+                double2 ww = w[q * (R - 1) + r - 1];
                 if (S > 0) ww.y = -ww.y;
                 v[q * R + r] = cmul(v[q * R + r], ww);
             }
@@ -242,18 +226,10 @@ __device__ __forceinline__ void st_compute(double2 (&v)[V], int t, int T, int Ns
     }
 }
 
-template <int R>
-__device__ __forceinline__ void st_load(double2 (&v)[V], const double2* x, int t, int T, int P)
+template <int P, bool REV, int s>
+__device__ __forceinline__ void st_store(const double2 (&v)[V], double2* x, int t)
 {
-#pragma unroll
-    for (int q = 0; q < V / R; ++q)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[q * R + r] = x[lx(t + q * T + r * (P / R))];
-}
-
-template <int R>
-__device__ __forceinline__ void st_store(const double2 (&v)[V], double2* x, int t, int T, int Ns)
-{
+    constexpr int R = radix(P, REV, s), Ns = nsub(P, REV, s), T = P / V;
 #pragma unroll
     for (int q = 0; q < V / R; ++q) {
         const int j = t + q * T;
@@ -263,58 +239,45 @@ __device__ __forceinline__ void st_store(const double2 (&v)[V], double2* x, int 
     }
 }
 
-template <int S>
-__device__ __forceinline__ void compute_any(int R, double2 (&v)[V], int t, int T, int Ns, const double2* tw)
+template <int P, int R>
+__device__ __forceinline__ void st_load(double2 (&v)[V], const double2* x, int t)
 {
-    switch (R) {
-    case 16: if constexpr (V >= 16) st_compute<16, S>(v, t, T, Ns, tw); break;
-    case 8: st_compute<8, S>(v, t, T, Ns, tw); break;
-    case 4: st_compute<4, S>(v, t, T, Ns, tw); break;
-    default: st_compute<2, S>(v, t, T, Ns, tw); break;
+    constexpr int T = P / V;
+#pragma unroll
+    for (int q = 0; q < V / R; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q * R + r] = x[lx(io_index(R, q, r, t, T, P))];
+}
+
+template <int P, int S, bool REV, int s>
+__device__ __forceinline__ void fft_from(double2 (&v)[V], double2 (&w)[V - 1], double2* x, int t,
+                                         const double2* __restrict__ tw)
+{
+    st_compute<P, S, REV, s>(v, w);
+    if constexpr (s + 1 < n_stages(P)) {
+        tw_load<P, REV, s + 1>(w, t, tw);   // in flight across the exchange
+        __syncthreads();
+        st_store<P, REV, s>(v, x, t);
+        __syncthreads();
+        st_load<P, radix(P, REV, s + 1)>(v, x, t);
+        fft_from<P, S, REV, s + 1>(v, w, x, t, tw);
     }
 }
 
-__device__ __forceinline__ void load_any(int R, double2 (&v)[V], const double2* x, int t, int T, int P)
+// Length-P transform, sign S (-1 forward, +1 inverse, unnormalised), stage
+// order REV.  On entry v holds the first stage's inputs (element (q, r) =
+// x[io_index(R0, q, r)]), on exit the last stage's outputs (X[io_index(RL, q, r)]).
+// `x` is the transform's LDS exchange buffer; the caller synchronises the
+// work-group before the first exchange if it used the buffer.
+template <int P, int S, bool REV>
+__device__ __forceinline__ void fft(double2 (&v)[V], double2* x, int t, const double2* __restrict__ tw)
 {
-    switch (R) {
-    case 16: if constexpr (V >= 16) st_load<16>(v, x, t, T, P); break;
-    case 8: st_load<8>(v, x, t, T, P); break;
-    case 4: st_load<4>(v, x, t, T, P); break;
-    default: st_load<2>(v, x, t, T, P); break;
-    }
+    double2 w[V - 1];
+    fft_from<P, S, REV, 0>(v, w, x, t, tw);
 }
 
-__device__ __forceinline__ void store_any(int R, const double2 (&v)[V], double2* x, int t, int T, int Ns)
-{
-    switch (R) {
-    case 16: if constexpr (V >= 16) st_store<16>(v, x, t, T, Ns); break;
-    case 8: st_store<8>(v, x, t, T, Ns); break;
-    case 4: st_store<4>(v, x, t, T, Ns); break;
-    default: st_store<2>(v, x, t, T, Ns); break;
-    }
-}
-
-// Run all stages of `sp`.  On entry v holds the inputs of stage 0 (element r
-// of butterfly q = x[t + q T + r P/R0]); on exit v holds the outputs of the
-// last stage (element r of butterfly q at t + q T + r P/R_last).  `x` is the
-// transform's LDS exchange buffer; the caller has synchronised the work-group
-// if the buffer was in use before.
-template <int S>
-__device__ __forceinline__ void run_stages(const StagePlan& sp, double2 (&v)[V], double2* x, int t, int T, int P,
-                                           const double2* __restrict__ tw)
-{
-    for (int s = 0; s < sp.n; ++s) {
-        if (s > 0) {
-            __syncthreads();
-            load_any(sp.R[s], v, x, t, T, P);
-        }
-        compute_any<S>(sp.R[s], v, t, T, sp.Ns[s], tw + sp.off[s]);
-        if (s + 1 < sp.n) {
-            __syncthreads();
-            store_any(sp.R[s], v, x, t, T, sp.Ns[s]);
-        }
-    }
-}
+template <int P> constexpr int R_first(bool rev) { return radix(P, rev, 0); }
+template <int P> constexpr int R_last(bool rev) { return radix(P, rev, n_stages(P) - 1); }
 
 // Work-groups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
 // Renumber blocks so that each XCD gets a run of consecutive logical blocks:
@@ -324,12 +287,6 @@ __device__ __forceinline__ int xcd_block()
     const int nblk = gridDim.x, full = (nblk / 8) * 8;
     const int b = blockIdx.x;
     return b < full ? (b % 8) * (nblk / 8) + b / 8 : b;
-}
-
-// element (q, r) of the first stage's inputs or the last stage's outputs
-__device__ __forceinline__ int io_index(int R, int q, int r, int t, int T, int P)
-{
-    return t + q * T + r * (P / R);
 }
 
 // --------------------------------------------------------- row sources
@@ -368,22 +325,21 @@ __device__ __forceinline__ size_t tidx(int row, int c, int H)
 }
 
 // ------------------------------------------------------------- pass 1
-// Work-group = nb transforms of length Px (T = Px/16 threads each); transform
-// c of block b handles row pair p = b nb + c (field rows 2p, 2p+1, or
-// 2 pairs[p], +1) and writes the two half spectra to rows 2p, 2p+1 of the
-// tiled Tout (H = Px/2 + 1 columns).
-__global__ void __launch_bounds__(512) k_rows_fwd(RowSrc S, FftDev F, int npairs, int nrows_limit,
-                                                  double2* __restrict__ Tout)
+// Transform c of block b handles row pair p = b NB + c (field rows 2p, 2p+1,
+// or 2 pairs[p], +1) and writes the two half spectra to rows 2p, 2p+1 of the
+// tiled Tout (H = P/2 columns, column 0 = (DC, Nyquist)).
+template <int P>
+__global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2* __restrict__ tw, int npairs,
+                                                        int nrows_limit, double2* __restrict__ Tout)
 {
     extern __shared__ double2 lds[];
-    const int P = F.P, T = F.T, H = P / 2 + 1;
-    const int nb = blockDim.x / T;
+    constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
+    constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-    const int p = xcd_block() * nb + c;
+    const int p = xcd_block() * NB + c;
     const bool live = p < npairs;
     const int j0 = live ? (S.pairs ? 2 * S.pairs[p] : 2 * p) : 0;
     double2* x = lds + (size_t)c * lds_len(P);
-    const int R0 = F.fwd.R[0];
     double2 v[V];
     if (S.mode == ROW_TERMS) {
         // the packed pair (row j0 -> real part, row j0+1 -> imaginary part) in
@@ -425,45 +381,52 @@ __global__ void __launch_bounds__(512) k_rows_fwd(RowSrc S, FftDev F, int npairs
             v[q] = make_double2(a, b);
         }
     }
-    __syncthreads();
-    run_stages<-1>(F.fwd, v, x, t, T, P, F.tw);
+    fft<P, -1, false>(v, x, t, tw);
     // natural-order spectrum to LDS, then split into the two half spectra
-    const int RL = F.fwd.R[F.fwd.n - 1];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < V; ++q) x[lx(io_index(RL, q / RL, q % RL, t, T, P))] = v[q];
     __syncthreads();
     if (!live) return;
-    for (int k = t; k < H; k += T) {
+#pragma unroll
+    for (int i = 0; i < H / T; ++i) {
+        const int k = t + i * T;
         const double2 z = x[lx(k)];
-        const double2 zc = x[lx((P - k) & (P - 1))];
-        const double4 ab = make_double4(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y), 0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+        double4 ab;
+        if (k == 0) {
+            const double2 zn = x[lx(P / 2)];
+            ab = make_double4(z.x, zn.x, z.y, zn.y);
+        } else {
+            const double2 zc = x[lx(P - k)];
+            ab = make_double4(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y), 0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+        }
         *(double4*)(Tout + tidx(2 * p, k, H)) = ab;
     }
 }
 
 // ------------------------------------------------------------- pass 2
-// Work-group = nb column transforms of length Py = P.  Tiled T1 row `slot`
-// holds field row r where slot = slot_of_row[r] (-1: the row is zero), or
-// slot = r when slot_of_row is null and r < nrows_in.  The LGF spectrum is
-// even in ky: only ky <= P/2 is stored (spec[c][ky], P/2+1 per column).
+// Transform = one spectrum column c < H of length P = P_y.  Tiled T1 row
+// `slot` holds field row r where slot = slot_of_row[r] (-1: the row is zero),
+// or slot = r when slot_of_row is null and r < nrows_in.  The LGF spectrum is
+// stored for ky <= P/2 (even in ky) as spec[c][ky], (P/2+1) per column, for
+// kx = c < H and kx = H (the Nyquist column, at c = H).
 // MODE 0: convolve, write tiled T2 rows 0..ny-1.  MODE 1: forward only, write
-// Re * scale to spec_out[c][ky] for ky <= P/2 (the LGF spectrum build).
-template <int MODE>
-__global__ void __launch_bounds__(512) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
-                                              FftDev F, int H, int ny, int nrows_in, const double* __restrict__ spec,
-                                              double scale, double2* __restrict__ T2, double* __restrict__ spec_out)
+// the spectrum * scale (the LGF spectrum build).
+template <int P, int MODE>
+__global__ void __launch_bounds__(tpb_of(P)) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
+                                                    const double2* __restrict__ tw, int H, int ny, int nrows_in,
+                                                    const double* __restrict__ spec, double scale,
+                                                    double2* __restrict__ T2, double* __restrict__ spec_out)
 {
     extern __shared__ double2 lds[];
-    const int P = F.P, T = F.T, SH = P / 2 + 1;
-    const int nb = blockDim.x / T;
+    constexpr int T = P / V, SH = P / 2 + 1, NB = tpb_of(P) / T;
+    constexpr int R0 = R_first<P>(false), RL = R_last<P>(false), RO = R_last<P>(true);
+    static_assert(R_first<P>(true) == RL, "the inverse starts from the forward's registers");
     const int cl = threadIdx.x / T, t = threadIdx.x - cl * T;
-    // blocks are dispatched round-robin over the 8 XCDs: give each XCD a run of
-    // consecutive columns so that their 128-byte row segments meet in its L2
-    const int c = xcd_block() * nb + cl;
+    const int c0 = xcd_block() * NB;
+    const int c = c0 + cl;
     const bool live = c < H;
     double2* x = lds + (size_t)cl * lds_len(P);
-    const int R0 = F.fwd.R[0];
     double2 v[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
@@ -475,72 +438,91 @@ __global__ void __launch_bounds__(512) k_cols(const double2* __restrict__ T1, co
         }
         v[q] = val;
     }
-    run_stages<-1>(F.fwd, v, x, t, T, P, F.tw);
-    const int RL = F.fwd.R[F.fwd.n - 1];
-    if (MODE == 1) {
+    fft<P, -1, false>(v, x, t, tw);
+    if constexpr (MODE == 1) {
         if (!live) return;
 #pragma unroll
         for (int q = 0; q < V; ++q) {
             const int ky = io_index(RL, q / RL, q % RL, t, T, P);
-            if (ky <= P / 2) spec_out[(size_t)c * SH + ky] = v[q].x * scale;
+            if (ky <= P / 2) {
+                spec_out[(size_t)c * SH + ky] = v[q].x * scale;
+                if (c == 0) spec_out[(size_t)H * SH + ky] = v[q].y * scale;
+            }
         }
         return;
-    }
-    // multiply by the real spectrum; the reverse plan's first stage has radix
-    // RL and reads exactly these indices from registers
-    if (live) {
+    } else {
+        // multiply by the real spectrum; the inverse's first stage reads exactly
+        // these registers.  Column 0 packs the DC column a (real part) and the
+        // Nyquist column b (imaginary part): with Z = A + iB (A, B Hermitian),
+        // the product g_a A + i g_b B = (g_a+g_b)/2 Z + (g_a-g_b)/2 conj(Z(-ky)).
+        if (c0 == 0) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < V; ++q) x[lx(io_index(RL, q / RL, q % RL, t, T, P))] = v[q];
+            __syncthreads();
+        }
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const int ky = io_index(RL, q / RL, q % RL, t, T, P);
+                const int kk = ky <= P / 2 ? ky : P - ky;
+                const double g = spec[(size_t)c * SH + kk];
+                if (c == 0) {
+                    const double gn = spec[(size_t)H * SH + kk];
+                    const double2 zc = x[lx((P - ky) & (P - 1))];
+                    const double a = 0.5 * (g + gn), b = 0.5 * (g - gn);
+                    v[q] = make_double2(a * v[q].x + b * zc.x, a * v[q].y - b * zc.y);
+                } else {
+                    v[q].x *= g;
+                    v[q].y *= g;
+                }
+            }
+        }
+        fft<P, 1, true>(v, x, t, tw);
+        if (!live) return;
 #pragma unroll
         for (int q = 0; q < V; ++q) {
-            const int ky = io_index(RL, q / RL, q % RL, t, T, P);
-            const double g = spec[(size_t)c * SH + (ky <= P / 2 ? ky : P - ky)];
-            v[q].x *= g;
-            v[q].y *= g;
+            const int row = io_index(RO, q / RO, q % RO, t, T, P);
+            if (row < ny) T2[tidx(row, c, H)] = v[q];
         }
-    }
-    __syncthreads();   // all forward LDS loads done before the reverse plan stores
-    run_stages<1>(F.rev, v, x, t, T, P, F.tw);
-    if (!live) return;
-    const int RR = F.rev.R[F.rev.n - 1];
-#pragma unroll
-    for (int q = 0; q < V; ++q) {
-        const int row = io_index(RR, q / RR, q % RR, t, T, P);
-        if (row < ny) T2[tidx(row, c, H)] = v[q];
     }
 }
 
 // ------------------------------------------------------------- pass 3
-// Work-group = nb row-pair transforms; transform c makes output rows 2p, 2p+1.
-__global__ void __launch_bounds__(512) k_rows_inv(const double2* __restrict__ T2, FftDev F, int nx, int ny,
-                                                  double* __restrict__ out)
+// Transform c of block b makes output rows 2p, 2p+1 (p = b NB + c).
+template <int P>
+__global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restrict__ T2, const double2* __restrict__ tw,
+                                                        int nx, int ny, double* __restrict__ out)
 {
     extern __shared__ double2 lds[];
-    const int P = F.P, T = F.T, H = P / 2 + 1;
-    const int nb = blockDim.x / T;
+    constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
+    constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-    const int p = xcd_block() * nb + c;
+    const int p = xcd_block() * NB + c;
     const int j0 = 2 * p;
     const bool live = j0 < ny;
     const bool two = j0 + 1 < ny;
     double2* x = lds + (size_t)c * lds_len(P);
-    const int R0 = F.fwd.R[0];
     double2 v[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int m = io_index(R0, q / R0, q % R0, t, T, P);
         double2 z = make_double2(0.0, 0.0);
         if (live) {
-            // Z(m) = A(m) + i B(m),  Z(P-m) = conj(A(m)) + i conj(B(m))
+            // Z(m) = A(m) + i B(m),  Z(P-m) = conj(A(m)) + i conj(B(m)); column 0
+            // holds the real DC (x) and Nyquist (y) values of rows j0, j0+1
             const int k = m <= P / 2 ? m : P - m;
-            const double4 ab = *(const double4*)(T2 + tidx(j0, k, H));
-            const double Ax = ab.x, Ay = ab.y;
-            const double Bx = two ? ab.z : 0.0, By = two ? ab.w : 0.0;
+            const double4 ab = *(const double4*)(T2 + tidx(j0, k == H ? 0 : k, H));
+            double Ax = ab.x, Ay = ab.y, Bx = ab.z, By = ab.w;
+            if (k == 0) { Ay = 0.0; By = 0.0; }
+            if (k == H) { Ax = ab.y; Bx = ab.w; Ay = 0.0; By = 0.0; }
+            if (!two) { Bx = 0.0; By = 0.0; }
             z = m <= P / 2 ? make_double2(Ax - By, Ay + Bx) : make_double2(Ax + By, Bx - Ay);
         }
         v[q] = z;
     }
-    run_stages<1>(F.fwd, v, x, t, T, P, F.tw);
+    fft<P, 1, false>(v, x, t, tw);
     if (!live) return;
-    const int RL = F.fwd.R[F.fwd.n - 1];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int xo = io_index(RL, q / RL, q % RL, t, T, P);
@@ -554,41 +536,54 @@ __global__ void __launch_bounds__(512) k_rows_inv(const double2* __restrict__ T2
 // ------------------------------------------------------------- launches
 static void set_lds(const void* fn, size_t bytes)
 {
-    static const void* fns[32];
-    static size_t lim[32];
-    static int n = 0;
-    for (int k = 0; k < n; ++k)
-        if (fns[k] == fn) {
-            if (lim[k] >= bytes) return;
-            ILH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-            lim[k] = bytes;
-            return;
-        }
+    static std::map<const void*, size_t> lim;
+    auto it = lim.find(fn);
+    if (it != lim.end() && it->second >= bytes) return;
     ILH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    if (n < 32) {
-        fns[n] = fn;
-        lim[n] = bytes;
-        ++n;
-    }
+    lim[fn] = bytes;
 }
 
-// threads per work-group and transforms per work-group for length P
-static void shape(int P, int& tpb, int& nb)
+#define ILH_FOR_P(P, BODY)                                                                               \
+    switch (P) {                                                                                         \
+    case 16: { constexpr int PP = 16; BODY; } break;                                                    \
+    case 32: { constexpr int PP = 32; BODY; } break;                                                    \
+    case 64: { constexpr int PP = 64; BODY; } break;                                                    \
+    case 128: { constexpr int PP = 128; BODY; } break;                                                  \
+    case 256: { constexpr int PP = 256; BODY; } break;                                                  \
+    case 512: { constexpr int PP = 512; BODY; } break;                                                  \
+    case 1024: { constexpr int PP = 1024; BODY; } break;                                                \
+    case 2048: { constexpr int PP = 2048; BODY; } break;                                                \
+    case 4096: { constexpr int PP = 4096; BODY; } break;                                                \
+    case 8192: { constexpr int PP = 8192; BODY; } break;                                                \
+    default: fail(ILH_ERR_ARG, "FFT length must be a power of two in [16, 8192]");                       \
+    }
+
+template <int P>
+static void launch_rows_fwd_p(ilh_ctx* ctx, const RowSrc& S, int npairs, int nrows_limit, double2* Tout)
 {
-    const int T = P / V;
-    tpb = std::max(64, T);
-    nb = tpb / T;
+    constexpr int NB = tpb_of(P) / (P / V);
+    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
+    set_lds((const void*)k_rows_fwd<P>, lds);
+    k_rows_fwd<P><<<(npairs + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(S, get_twiddles(ctx, P), npairs,
+                                                                          nrows_limit, Tout);
+    ILH_HIP(hipGetLastError());
 }
 
 static void launch_rows_fwd(ilh_ctx* ctx, const RowSrc& S, int P, int npairs, int nrows_limit, double2* Tout)
 {
     if (npairs <= 0) return;
-    const FftDev F = get_fft(ctx, P);
-    int tpb, nb;
-    shape(P, tpb, nb);
-    const size_t lds = sizeof(double2) * (size_t)nb * lds_len(P);
-    set_lds((const void*)k_rows_fwd, lds);
-    k_rows_fwd<<<(npairs + nb - 1) / nb, tpb, lds, ctx->stream>>>(S, F, npairs, nrows_limit, Tout);
+    ILH_FOR_P(P, launch_rows_fwd_p<PP>(ctx, S, npairs, nrows_limit, Tout));
+}
+
+template <int P, int MODE>
+static void launch_cols_p(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int H, int ny, int nrows_in,
+                          const double* spec, double scale, double2* T2, double* spec_out)
+{
+    constexpr int NB = tpb_of(P) / (P / V);
+    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
+    set_lds((const void*)k_cols<P, MODE>, lds);
+    k_cols<P, MODE><<<(H + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(T1, slot_of_row, get_twiddles(ctx, P), H, ny,
+                                                                        nrows_in, spec, scale, T2, spec_out);
     ILH_HIP(hipGetLastError());
 }
 
@@ -596,27 +591,24 @@ template <int MODE>
 static void launch_cols(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int Px, int Py, int ny,
                         int nrows_in, const double* spec, double scale, double2* T2, double* spec_out)
 {
-    const FftDev F = get_fft(ctx, Py);
-    const int H = Px / 2 + 1;
-    int tpb, nb;
-    shape(Py, tpb, nb);
-    const size_t lds = sizeof(double2) * (size_t)nb * lds_len(Py);
-    set_lds((const void*)k_cols<MODE>, lds);
-    k_cols<MODE><<<(H + nb - 1) / nb, tpb, lds, ctx->stream>>>(T1, slot_of_row, F, H, ny, nrows_in, spec, scale, T2,
-                                                               spec_out);
+    const int H = Px / 2;
+    ILH_FOR_P(Py, (launch_cols_p<PP, MODE>(ctx, T1, slot_of_row, H, ny, nrows_in, spec, scale, T2, spec_out)));
+}
+
+template <int P>
+static void launch_rows_inv_p(ilh_ctx* ctx, const double2* T2, int nx, int ny, double* out)
+{
+    constexpr int NB = tpb_of(P) / (P / V);
+    const int npairs = (ny + 1) / 2;
+    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
+    set_lds((const void*)k_rows_inv<P>, lds);
+    k_rows_inv<P><<<(npairs + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(T2, get_twiddles(ctx, P), nx, ny, out);
     ILH_HIP(hipGetLastError());
 }
 
 static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, double* out)
 {
-    const FftDev F = get_fft(ctx, Px);
-    int tpb, nb;
-    shape(Px, tpb, nb);
-    const int npairs = (ny + 1) / 2;
-    const size_t lds = sizeof(double2) * (size_t)nb * lds_len(Px);
-    set_lds((const void*)k_rows_inv, lds);
-    k_rows_inv<<<(npairs + nb - 1) / nb, tpb, lds, ctx->stream>>>(T2, F, nx, ny, out);
-    ILH_HIP(hipGetLastError());
+    ILH_FOR_P(Px, launch_rows_inv_p<PP>(ctx, T2, nx, ny, out));
 }
 
 // -------------------------------------------------------------- LGF table
@@ -763,11 +755,11 @@ ConvPlan& get_conv(ilh_plan* pl, int gk)
     C.Px = period(C.nx);
     C.Py = period(C.ny);
     if (C.Px > MAXP || C.Py > MAXP) fail(ILH_ERR_ARG, "grid too large for the FFT convolution (period > 8192)");
-    C.H = C.Px / 2 + 1;
+    C.H = C.Px / 2;
     const int rows = (std::max(C.Py, C.ny + 1) + 7) / 8 * 8;   // tiled layout: whole 8-row tiles
     C.T1.ensure(sizeof(double2) * (size_t)rows * C.H);
     C.T2.ensure(sizeof(double2) * (size_t)((C.ny + 1 + 7) / 8 * 8) * C.H);
-    C.spec.ensure(sizeof(double) * (size_t)C.H * (C.Py / 2 + 1));
+    C.spec.ensure(sizeof(double) * (size_t)(C.H + 1) * (C.Py / 2 + 1));
     // spectrum of the circular kernel: rows of G_circ, then forward columns
     RowSrc S{};
     S.mode = ROW_LGF;
@@ -822,6 +814,31 @@ void build_source_pairs(ilh_plan* pl, SourceSet& s)
         ILH_HIP(hipMemcpyAsync(s.pairs, pairs.data(), sizeof(int) * (size_t)s.npairs, hipMemcpyHostToDevice, st));
     ILH_HIP(hipMemcpyAsync(s.slot_of_row, slot.data(), sizeof(int) * (size_t)ny, hipMemcpyHostToDevice, st));
     ILH_HIP(hipStreamSynchronize(st));
+}
+
+// one pass of conv_sources alone (0 = rows, 1 = columns, 2 = inverse rows), for timing
+void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage)
+{
+    ConvPlan& C = get_conv(pl, gk);
+    if (stage == 0) {
+        RowSrc S{};
+        S.mode = ROW_TERMS;
+        S.nx = C.nx;
+        S.ny = C.ny;
+        S.ebx = src.bins.entries + src.bins.n_entries;
+        S.eai = src.bins.eai;
+        S.erw = src.bins.erw;
+        S.cell_ptr = src.bins.cell_ptr;
+        S.cells = src.bins.cells;
+        S.amp = amp;
+        S.pairs = src.pairs;
+        launch_rows_fwd(pl->ctx, S, C.Px, src.npairs, C.ny, C.T1.as<double2>());
+    } else if (stage == 1) {
+        launch_cols<0>(pl->ctx, C.T1.as<double2>(), src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0,
+                       C.T2.as<double2>(), nullptr);
+    } else {
+        launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
+    }
 }
 
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, hipEvent_t* ev)

@@ -108,8 +108,8 @@ struct ConvPlan {
     int gk = 0;
     int nx = 0, ny = 0;       // field dims
     int Px = 0, Py = 0;       // padded periods (powers of two, >= 2n-2)
-    int H = 0;                // Px/2 + 1 (stored spectrum columns)
-    DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], ky <= Py/2 (even in ky)
+    int H = 0;                // Px/2: half-spectrum columns (column 0 packs DC and Nyquist)
+    DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], kx <= H, ky <= Py/2 (even in ky)
     DevBuf T1, T2;            // intermediate spectra, rows x H complex, 8-row tiles
 };
 
@@ -119,8 +119,6 @@ struct ilh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 0;
-    ilh::DevBuf twiddle;      // exp(-2 pi i k / 8192) table
-    int tw_len = 0;
 };
 
 struct SolveCache;   // Schur factor and solve matrices of one composite
@@ -168,7 +166,6 @@ void fill_zero(hipStream_t st, double* a, size_t n);
 void axpby(hipStream_t st, double* y, const double* x, double a, double b, size_t n);  // y = a x + b y
 
 // FFT convolution (ilh_fft.hip)
-void ensure_twiddles(ilh_ctx* ctx);
 void ensure_lgf(ilh_plan* pl);
 ConvPlan& get_conv(ilh_plan* pl, int gk);
 // field -> field:  out = dx^2 (G * in)
@@ -184,6 +181,7 @@ struct SourceSet {
 void build_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);
+void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage);
 
 // dense linear algebra (ilh_dense.hip)
 void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel,

@@ -407,12 +407,14 @@ class DirichletPoisson:
         L.call("ilh_dirichlet_solve", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,
                f_dev.ptr, s_dev.ptr, L.MEM_DEVICE)
 
-    def profile(self, fp_dev, fm_dev, f_dev, s_dev, reps):
-        st = (C.c_double * 4)()
+    def profile(self, fp_dev, fm_dev, f_dev, s_dev, reps, stages=True):
+        """Device-buffer repeat solves timed with HIP events: returns
+        (per-stage ms of each stage launched alone, or None; ms per whole solve)."""
+        st = (C.c_double * 4)() if stages else None
         tot = C.c_double()
         L.call("ilh_profile_dirichlet", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,
                f_dev.ptr, s_dev.ptr, int(reps), st, C.byref(tot))
-        return list(st), tot.value
+        return (list(st) if stages else None), tot.value
 
 
 class NeumannPoisson:

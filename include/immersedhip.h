@@ -160,10 +160,12 @@ int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_mi
                       double* f, double* df, double* s, double* ds, int mem);
 
 /* ---- measurement helpers (no reference counterpart) ---- */
-/* Average device time per call, in ms, of `reps` repeated Dirichlet solves on
-   device buffers, split by stage: stage_ms[0]=surface matvec, [1]=row FFT,
-   [2]=column FFT+multiply, [3]=inverse row FFT (each from HIP events on the
-   plan's stream).  Returns the total per solve in *total_ms. */
+/* HIP-event timings on the plan's stream, in ms per call, of the Dirichlet
+   repeat solve on device buffers: *total_ms = `reps` whole solves back to back
+   (two events around the loop); if stage_ms is not NULL, stage_ms[k] = `reps`
+   launches of stage k alone (0 = surface matvec, 1 = row FFT of the sources,
+   2 = column FFT x LGF spectrum x inverse column FFT, 3 = inverse row FFT
+   writing f). */
 int ilh_profile_dirichlet(ilh_plan* plan, const double* fb_plus_dev, const double* fb_minus_dev,
                           double* f_dev, double* s_dev, int reps, double* stage_ms, double* total_ms);
 

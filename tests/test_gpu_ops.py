@@ -189,3 +189,22 @@ def test_neumann_solve(setup):
         assert rel_err(ds.flat(), dsr) < TOL_SOLVE
         assert rel_err(f.flat(), fr) < TOL_SOLVE
         assert rel_err(s.flat(), sr) < TOL_SOLVE
+
+
+@pytest.mark.parametrize("NX,NY", [(12, 9), (9, 40), (40, 200), (300, 70), (1100, 20), (2100, 12)])
+def test_inverse_laplacian_periods(ctx, NX, NY):
+    """Every FFT length 16..8192 and unequal row/column periods (no surface points)."""
+    import ilhip as il
+    from ilhip.geometry import GridSpec, Surface
+    gs = GridSpec(NX, NY, NX // 2, NY // 2, 0.05)
+    print("inverse_laplacian grid", NX, NY, flush=True)
+    e = np.zeros(0)
+    cache = il.SurfaceScalarCache(Surface(e, e, e, e, e, np.zeros(0, int)), gs, ctx=ctx)
+    og = O.grid(NX, NY, NX // 2, NY // 2, 0.05)
+    rng = np.random.default_rng(NX * 1000 + NY)
+    for kind, okind, shape in ((il.Primal, O.PRIMAL, gs.primal_shape()), (il.Dual, O.DUAL, gs.dual_shape())):
+        w = rng.standard_normal(shape)
+        out = il.Nodes(kind, gs)
+        il.inverse_laplacian(out, cache, rhs=il.Nodes(kind, gs, w))
+        ref = O.InverseLaplacian(og, okind)(w.ravel(order="F"))
+        assert rel_err(out.flat(), ref) < 1e-13, (kind, NX, NY)
