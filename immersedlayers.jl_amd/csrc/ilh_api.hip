@@ -74,7 +74,7 @@ struct SolveCache {
         free_bins(src1.bins);
         free_bins(src2.bins);
         for (SourceSet* ss : {&src1, &src2}) {
-            if (ss->pairs) (void)hipFree(ss->pairs);
+            if (ss->pinfo) (void)hipFree(ss->pinfo);
             if (ss->slot_of_row) (void)hipFree(ss->slot_of_row);
         }
     }
@@ -595,6 +595,7 @@ void merge_sources(ilh_plan* pl, int op0, int op1, int gk, DevBuf& tbuf, SourceS
     if (n1) k_term_shift<<<(n1 + 255) / 256, 256, 0, st>>>(get_terms(pl, op1), n1, N, t + n0);
     ILH_HIP(hipGetLastError());
     src.terms = t;
+    src.amp_len = 2 * N;
     build_bins(pl, t, n0 + n1, gk, src.bins);
     build_source_pairs(pl, src);
 }

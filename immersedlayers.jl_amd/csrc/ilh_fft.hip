@@ -249,9 +249,13 @@ __device__ __forceinline__ void st_load(double2 (&v)[V], const double2* x, int t
         for (int r = 0; r < R; ++r) v[q * R + r] = x[lx(io_index(R, q, r, t, T, P))];
 }
 
-template <int P, int S, bool REV, int s>
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+template <int P, int S, bool REV, int s, class Pre>
 __device__ __forceinline__ void fft_from(double2 (&v)[V], double2 (&w)[V - 1], double2* x, int t,
-                                         const double2* __restrict__ tw)
+                                         const double2* __restrict__ tw, Pre& pre)
 {
     st_compute<P, S, REV, s>(v, w);
     if constexpr (s + 1 < n_stages(P)) {
@@ -260,7 +264,8 @@ __device__ __forceinline__ void fft_from(double2 (&v)[V], double2 (&w)[V - 1], d
         st_store<P, REV, s>(v, x, t);
         __syncthreads();
         st_load<P, radix(P, REV, s + 1)>(v, x, t);
-        fft_from<P, S, REV, s + 1>(v, w, x, t, tw);
+        if constexpr (s + 2 == n_stages(P)) pre();
+        fft_from<P, S, REV, s + 1>(v, w, x, t, tw, pre);
     }
 }
 
@@ -268,12 +273,14 @@ __device__ __forceinline__ void fft_from(double2 (&v)[V], double2 (&w)[V - 1], d
 // order REV.  On entry v holds the first stage's inputs (element (q, r) =
 // x[io_index(R0, q, r)]), on exit the last stage's outputs (X[io_index(RL, q, r)]).
 // `x` is the transform's LDS exchange buffer; the caller synchronises the
-// work-group before the first exchange if it used the buffer.
-template <int P, int S, bool REV>
-__device__ __forceinline__ void fft(double2 (&v)[V], double2* x, int t, const double2* __restrict__ tw)
+// work-group before the first exchange if it used the buffer.  `pre` runs
+// before the last stage (loads issued there overlap its butterflies).
+template <int P, int S, bool REV, class Pre = NoHook>
+__device__ __forceinline__ void fft(double2 (&v)[V], double2* x, int t, const double2* __restrict__ tw,
+                                    Pre pre = Pre())
 {
     double2 w[V - 1];
-    fft_from<P, S, REV, 0>(v, w, x, t, tw);
+    fft_from<P, S, REV, 0>(v, w, x, t, tw, pre);
 }
 
 template <int P> constexpr int R_first(bool rev) { return radix(P, rev, 0); }
@@ -298,22 +305,52 @@ struct RowSrc {
     int nx, ny;
     const double* tab;     // LGF table (circular rows of the Px x Py kernel)
     int tmx, Px, Py;
-    const int* ebx;        // point terms, row-binned: sorted bx, amplitude
-    const int* eai;        // index and 5 weights per entry, and the nonzero
-    const double* erw;     // cells of each row (x, entry range)
-    const int* cell_ptr;
-    const int* cells;
+    const int4* pinfo;     // surface sources: per row pair {j0, cells of row j0, of row j0+1, end},
+    const int4* cells;     // cells {x, c0, c1, 0}, contributions cw[e] * amp[cai[e]]
+    const int* cai;
+    const double* cw;
     const double* amp;
-    const int* pairs;      // compacted row pairs, or null
+    int amp_len;
+    int amp_lds;           // 1: stage amp[0..amp_len) in LDS after the transform buffers
 };
 
-// value of one cell: its entries in (bx, term) order
-__device__ __forceinline__ double cell_value(const RowSrc& S, const int* cl)
+// values of two cells (a second with c0 == c1 is empty): each cell's
+// contributions in order, loaded 8 per cell at a time
+template <bool LDS_AMP>
+__device__ __forceinline__ void cell_values(const RowSrc& S, const double* a_lds, int4 ca, int4 cb, double& va,
+                                            double& vb)
 {
-    const int x = cl[0];
-    double acc = 0.0;
-    for (int k = cl[1]; k < cl[2]; ++k) acc += S.amp[S.eai[k]] * S.erw[5 * (size_t)k + (x - S.ebx[k])];
-    return acc;
+    double acc[2] = {0.0, 0.0};
+    int e[2] = {ca.y, cb.y};
+    const int end[2] = {ca.z, cb.z};
+    while (e[0] < end[0] || e[1] < end[1]) {
+        int a[2][8];
+        double w[2][8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool ok = e[h] + u < end[h];
+                a[h][u] = ok ? S.cai[e[h] + u] : 0;
+                w[h][u] = ok ? S.cw[e[h] + u] : 0.0;
+            }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e[h] + u < end[h]) acc[h] += w[h][u] * (LDS_AMP ? a_lds[a[h][u]] : S.amp[a[h][u]]);
+            e[h] += 8;
+        }
+    }
+    va = acc[0];
+    vb = acc[1];
+}
+
+// max amplitudes staged in LDS beside NB transforms of length P
+__host__ __device__ constexpr int amp_lds_cap(int P)
+{
+    const int room = (160 * 1024 - (tpb_of(P) / (P / V)) * lds_len(P) * 16) / 8;
+    return room < 4096 ? room : 4096;
 }
 
 // Spectrum intermediates use an 8-row tiled layout: element (row, c) at
@@ -323,10 +360,17 @@ __device__ __forceinline__ size_t tidx(int row, int c, int H)
 {
     return (size_t)(row >> 3) * 8 * H + (size_t)c * 8 + (row & 7);
 }
+// The column pass output uses 2-row tiles instead: a row pair's value at c is
+// 32 contiguous bytes and consecutive c are adjacent (the inverse row pass reads
+// whole lines).
+__device__ __forceinline__ size_t pidx(int row, int c, int H)
+{
+    return (size_t)(row >> 1) * 2 * H + (size_t)c * 2 + (row & 1);
+}
 
 // ------------------------------------------------------------- pass 1
 // Transform c of block b handles row pair p = b NB + c (field rows 2p, 2p+1,
-// or 2 pairs[p], +1) and writes the two half spectra to rows 2p, 2p+1 of the
+// or pinfo[p].x, +1) and writes the two half spectra to rows 2p, 2p+1 of the
 // tiled Tout (H = P/2 columns, column 0 = (DC, Nyquist)).
 template <int P>
 __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2* __restrict__ tw, int npairs,
@@ -338,29 +382,49 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
     const int p = xcd_block() * NB + c;
     const bool live = p < npairs;
-    const int j0 = live ? (S.pairs ? 2 * S.pairs[p] : 2 * p) : 0;
     double2* x = lds + (size_t)c * lds_len(P);
     double2 v[V];
     if (S.mode == ROW_TERMS) {
         // the packed pair (row j0 -> real part, row j0+1 -> imaginary part) in
         // LDS: zeros, then the rows' nonzero cells
+        double* a_lds = (double*)(lds + (size_t)NB * lds_len(P));
+        const int4 pi = live ? S.pinfo[p] : make_int4(0, 0, 0, 0);
+        if (S.amp_lds) {
+            // 16 loads in flight per thread, then the LDS stores
+            for (int base = 0; base < S.amp_len; base += 16 * (int)blockDim.x) {
+                double tmp[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int i = base + u * (int)blockDim.x + (int)threadIdx.x;
+                    tmp[u] = i < S.amp_len ? S.amp[i] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int i = base + u * (int)blockDim.x + (int)threadIdx.x;
+                    if (i < S.amp_len) a_lds[i] = tmp[u];
+                }
+            }
+        }
         for (int i = t; i < P; i += T) x[lx(i)] = make_double2(0.0, 0.0);
         __syncthreads();
-        if (live) {
-            for (int ci = S.cell_ptr[j0] + t; ci < S.cell_ptr[j0 + 1]; ci += T) {
-                const int* cl = S.cells + 3 * (size_t)ci;
-                x[lx(cl[0])].x = cell_value(S, cl);
+        for (int ci = pi.y + t; ci < pi.w; ci += 2 * T) {
+            const int4 ca = S.cells[ci];
+            const int4 cb = ci + T < pi.w ? S.cells[ci + T] : make_int4(0, 0, 0, 0);
+            double va, vb;
+            if (S.amp_lds) cell_values<true>(S, a_lds, ca, cb, va, vb);
+            else cell_values<false>(S, a_lds, ca, cb, va, vb);
+            if (ci < pi.z) x[lx(ca.x)].x = va;
+            else x[lx(ca.x)].y = va;
+            if (ci + T < pi.w) {
+                if (ci + T < pi.z) x[lx(cb.x)].x = vb;
+                else x[lx(cb.x)].y = vb;
             }
-            if (j0 + 1 < S.ny)
-                for (int ci = S.cell_ptr[j0 + 1] + t; ci < S.cell_ptr[j0 + 2]; ci += T) {
-                    const int* cl = S.cells + 3 * (size_t)ci;
-                    x[lx(cl[0])].y = cell_value(S, cl);
-                }
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < V; ++q) v[q] = x[lx(io_index(R0, q / R0, q % R0, t, T, P))];
     } else {
+        const int j0 = 2 * p;
         const int nx_src = S.mode == ROW_LGF ? P : S.nx;
 #pragma unroll
         for (int q = 0; q < V; ++q) {
@@ -410,10 +474,10 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
 // or slot = r when slot_of_row is null and r < nrows_in.  The LGF spectrum is
 // stored for ky <= P/2 (even in ky) as spec[c][ky], (P/2+1) per column, for
 // kx = c < H and kx = H (the Nyquist column, at c = H).
-// MODE 0: convolve, write tiled T2 rows 0..ny-1.  MODE 1: forward only, write
+// MODE 0: convolve, write rows 0..ny-1 of T2 (2-row tiles).  MODE 1: forward only, write
 // the spectrum * scale (the LGF spectrum build).
 template <int P, int MODE>
-__global__ void __launch_bounds__(tpb_of(P)) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
+__global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(4))) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
                                                     const double2* __restrict__ tw, int H, int ny, int nrows_in,
                                                     const double* __restrict__ spec, double scale,
                                                     double2* __restrict__ T2, double* __restrict__ spec_out)
@@ -438,7 +502,19 @@ __global__ void __launch_bounds__(tpb_of(P)) k_cols(const double2* __restrict__ 
         }
         v[q] = val;
     }
-    fft<P, -1, false>(v, x, t, tw);
+    // MODE 0: the spectrum values this thread multiplies by, loaded before the
+    // forward's last stage
+    double g[V];
+    auto pre = [&]() {
+        if (MODE == 0 && live) {
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const int ky = io_index(RL, q / RL, q % RL, t, T, P);
+                g[q] = spec[(size_t)c * SH + (ky <= P / 2 ? ky : P - ky)];
+            }
+        }
+    };
+    fft<P, -1, false>(v, x, t, tw, pre);
     if constexpr (MODE == 1) {
         if (!live) return;
 #pragma unroll
@@ -464,17 +540,15 @@ __global__ void __launch_bounds__(tpb_of(P)) k_cols(const double2* __restrict__ 
         if (live) {
 #pragma unroll
             for (int q = 0; q < V; ++q) {
-                const int ky = io_index(RL, q / RL, q % RL, t, T, P);
-                const int kk = ky <= P / 2 ? ky : P - ky;
-                const double g = spec[(size_t)c * SH + kk];
                 if (c == 0) {
-                    const double gn = spec[(size_t)H * SH + kk];
+                    const int ky = io_index(RL, q / RL, q % RL, t, T, P);
+                    const double gn = spec[(size_t)H * SH + (ky <= P / 2 ? ky : P - ky)];
                     const double2 zc = x[lx((P - ky) & (P - 1))];
-                    const double a = 0.5 * (g + gn), b = 0.5 * (g - gn);
+                    const double a = 0.5 * (g[q] + gn), b = 0.5 * (g[q] - gn);
                     v[q] = make_double2(a * v[q].x + b * zc.x, a * v[q].y - b * zc.y);
                 } else {
-                    v[q].x *= g;
-                    v[q].y *= g;
+                    v[q].x *= g[q];
+                    v[q].y *= g[q];
                 }
             }
         }
@@ -483,7 +557,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_cols(const double2* __restrict__ 
 #pragma unroll
         for (int q = 0; q < V; ++q) {
             const int row = io_index(RO, q / RO, q % RO, t, T, P);
-            if (row < ny) T2[tidx(row, c, H)] = v[q];
+            if (row < ny) T2[pidx(row, c, H)] = v[q];
         }
     }
 }
@@ -512,7 +586,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restric
             // Z(m) = A(m) + i B(m),  Z(P-m) = conj(A(m)) + i conj(B(m)); column 0
             // holds the real DC (x) and Nyquist (y) values of rows j0, j0+1
             const int k = m <= P / 2 ? m : P - m;
-            const double4 ab = *(const double4*)(T2 + tidx(j0, k == H ? 0 : k, H));
+            const double4 ab = *(const double4*)(T2 + pidx(j0, k == H ? 0 : k, H));
             double Ax = ab.x, Ay = ab.y, Bx = ab.z, By = ab.w;
             if (k == 0) { Ay = 0.0; By = 0.0; }
             if (k == H) { Ax = ab.y; Bx = ab.w; Ay = 0.0; By = 0.0; }
@@ -559,10 +633,11 @@ static void set_lds(const void* fn, size_t bytes)
     }
 
 template <int P>
-static void launch_rows_fwd_p(ilh_ctx* ctx, const RowSrc& S, int npairs, int nrows_limit, double2* Tout)
+static void launch_rows_fwd_p(ilh_ctx* ctx, RowSrc S, int npairs, int nrows_limit, double2* Tout)
 {
     constexpr int NB = tpb_of(P) / (P / V);
-    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
+    S.amp_lds = S.mode == ROW_TERMS && S.amp_len <= amp_lds_cap(P);
+    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P) + (S.amp_lds ? sizeof(double) * S.amp_len : 0);
     set_lds((const void*)k_rows_fwd<P>, lds);
     k_rows_fwd<P><<<(npairs + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(S, get_twiddles(ctx, P), npairs,
                                                                           nrows_limit, Tout);
@@ -792,28 +867,47 @@ void conv_dense(ilh_plan* pl, int gk, const double* in, double* out)
 void build_source_pairs(ilh_plan* pl, SourceSet& s)
 {
     const std::vector<int>& rp = s.bins.h_row_ptr;
+    const std::vector<int>& cp = s.bins.h_cell_ptr;
     const int ny = s.bins.ny;
-    std::vector<int> pairs, slot(ny, -1);
+    std::vector<int4> info;
+    std::vector<int> slot(ny, -1);
     for (int j = 0; j < ny; j += 2) {
         const bool a = rp[j + 1] > rp[j];
         const bool b = (j + 1 < ny) && (rp[j + 2] > rp[j + 1]);
         if (a || b) {
-            const int k = (int)pairs.size();
-            pairs.push_back(j / 2);
+            const int k = (int)info.size();
+            info.push_back(make_int4(j, cp[j], cp[j + 1], cp[std::min(j + 2, ny)]));
             slot[j] = 2 * k;
             if (j + 1 < ny) slot[j + 1] = 2 * k + 1;
         }
     }
-    s.npairs = (int)pairs.size();
-    if (s.pairs) (void)hipFree(s.pairs);
+    s.npairs = (int)info.size();
+    if (s.pinfo) (void)hipFree(s.pinfo);
     if (s.slot_of_row) (void)hipFree(s.slot_of_row);
-    ILH_HIP(hipMalloc(&s.pairs, sizeof(int) * (size_t)std::max(1, s.npairs)));
+    s.pinfo = nullptr;
+    s.slot_of_row = nullptr;
+    ILH_HIP(hipMalloc(&s.pinfo, sizeof(int4) * (size_t)std::max(1, s.npairs)));
     ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * (size_t)std::max(1, ny)));
     hipStream_t st = pl->ctx->stream;
     if (s.npairs)
-        ILH_HIP(hipMemcpyAsync(s.pairs, pairs.data(), sizeof(int) * (size_t)s.npairs, hipMemcpyHostToDevice, st));
+        ILH_HIP(hipMemcpyAsync(s.pinfo, info.data(), sizeof(int4) * (size_t)s.npairs, hipMemcpyHostToDevice, st));
     ILH_HIP(hipMemcpyAsync(s.slot_of_row, slot.data(), sizeof(int) * (size_t)ny, hipMemcpyHostToDevice, st));
     ILH_HIP(hipStreamSynchronize(st));
+}
+
+static RowSrc terms_source(const ConvPlan& C, const SourceSet& src, const double* amp)
+{
+    RowSrc S{};
+    S.mode = ROW_TERMS;
+    S.nx = C.nx;
+    S.ny = C.ny;
+    S.pinfo = src.pinfo;
+    S.cells = src.bins.cells;
+    S.cai = src.bins.cai;
+    S.cw = src.bins.cw;
+    S.amp = amp;
+    S.amp_len = src.amp_len;
+    return S;
 }
 
 // one pass of conv_sources alone (0 = rows, 1 = columns, 2 = inverse rows), for timing
@@ -821,18 +915,7 @@ void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, d
 {
     ConvPlan& C = get_conv(pl, gk);
     if (stage == 0) {
-        RowSrc S{};
-        S.mode = ROW_TERMS;
-        S.nx = C.nx;
-        S.ny = C.ny;
-        S.ebx = src.bins.entries + src.bins.n_entries;
-        S.eai = src.bins.eai;
-        S.erw = src.bins.erw;
-        S.cell_ptr = src.bins.cell_ptr;
-        S.cells = src.bins.cells;
-        S.amp = amp;
-        S.pairs = src.pairs;
-        launch_rows_fwd(pl->ctx, S, C.Px, src.npairs, C.ny, C.T1.as<double2>());
+        launch_rows_fwd(pl->ctx, terms_source(C, src, amp), C.Px, src.npairs, C.ny, C.T1.as<double2>());
     } else if (stage == 1) {
         launch_cols<0>(pl->ctx, C.T1.as<double2>(), src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0,
                        C.T2.as<double2>(), nullptr);
@@ -845,18 +928,7 @@ void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp,
 {
     ConvPlan& C = get_conv(pl, gk);
     hipStream_t st = pl->ctx->stream;
-    RowSrc S{};
-    S.mode = ROW_TERMS;
-    S.nx = C.nx;
-    S.ny = C.ny;
-    S.ebx = src.bins.entries + src.bins.n_entries;
-    S.eai = src.bins.eai;
-    S.erw = src.bins.erw;
-    S.cell_ptr = src.bins.cell_ptr;
-    S.cells = src.bins.cells;
-    S.amp = amp;
-    S.pairs = src.pairs;
-    launch_rows_fwd(pl->ctx, S, C.Px, src.npairs, C.ny, C.T1.as<double2>());
+    launch_rows_fwd(pl->ctx, terms_source(C, src, amp), C.Px, src.npairs, C.ny, C.T1.as<double2>());
     if (ev) ILH_HIP(hipEventRecord(ev[0], st));
     launch_cols<0>(pl->ctx, C.T1.as<double2>(), src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0,
                    C.T2.as<double2>(), nullptr);

@@ -85,12 +85,17 @@ struct RowBins {
     int* eai = nullptr;         // device: amplitude index per entry
     double* erw = nullptr;      // device: 5 weights per entry
     int n_entries = 0;
-    // nonzero cells of each row, ascending x: cells[3c] = x, [3c+1, 3c+2) = the
-    // entries k (bx in [x-4, x]) that contribute erw[5k + x - bx] to it
+    // nonzero cells of each row, ascending x: cells[c] = {x, c0, c1, 0}; the
+    // cell's value is sum_{c0 <= e < c1} cw[e] * amp[cai[e]] in that order (the
+    // row's entries k with bx in [x-4, x] and a nonzero weight erw[5k + x - bx])
     int* cell_ptr = nullptr;    // device, ny + 1
-    int* cells = nullptr;       // device, 3 per cell
+    int4* cells = nullptr;      // device, one per cell
     int n_cells = 0;
-    std::vector<int> h_row_ptr; // host copy (row counts drive launch shapes)
+    int* cai = nullptr;         // device: amplitude index per contribution
+    double* cw = nullptr;       // device: weight per contribution
+    int n_contrib = 0;
+    std::vector<int> h_row_ptr;  // host copies (row counts drive launch shapes)
+    std::vector<int> h_cell_ptr;
 };
 
 // ---------------------------------------------------------- device buffers
@@ -173,8 +178,9 @@ void conv_dense(ilh_plan* pl, int gk, const double* in, double* out);
 // point sources -> field: out = dx^2 (G * sum_terms amp*term), rows listed in pairs
 struct SourceSet {
     const Term* terms = nullptr;
+    int amp_len = 0;           // amplitude slots the terms index
     RowBins bins;
-    int* pairs = nullptr;      // device list of nonzero row pairs
+    int4* pinfo = nullptr;     // device, per nonzero row pair p: {j0, first cell, first cell of row j0+1, end}
     int* slot_of_row = nullptr;  // device: field row -> row of the compacted spectrum, or -1
     int npairs = 0;
 };

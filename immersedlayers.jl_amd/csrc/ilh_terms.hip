@@ -363,7 +363,7 @@ __global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict_
 }
 
 __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
-                            int nx, const int* __restrict__ cptr, int* __restrict__ cells)
+                            int nx, const int* __restrict__ cptr, int* __restrict__ cells3)
 {
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int woff[CELL_WORDS];
@@ -393,13 +393,45 @@ __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__
             }
             int e = a;
             while (e < hi && ebx[e] <= x) ++e;
-            int* cl = cells + 3 * (size_t)(cptr[j] + woff[w] + rank);
+            int* cl = cells3 + 3 * (size_t)(cptr[j] + woff[w] + rank);
             cl[0] = x;
             cl[1] = a;
             cl[2] = e;
             ++rank;
         }
     }
+}
+
+// contributions of cell c (x, entries [klo, khi)): the entries with a nonzero weight at x
+__global__ void k_contrib_count(const int* __restrict__ cells3, int n, const int* __restrict__ ebx,
+                                const double* __restrict__ erw, int* __restrict__ cnt)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int x = cells3[3 * (size_t)c], klo = cells3[3 * (size_t)c + 1], khi = cells3[3 * (size_t)c + 2];
+    int m = 0;
+    for (int k = klo; k < khi; ++k) m += erw[5 * (size_t)k + (x - ebx[k])] != 0.0;
+    cnt[c] = m;
+}
+
+__global__ void k_contrib_fill(const int* __restrict__ cells3, int n, const int* __restrict__ ebx,
+                               const int* __restrict__ eai, const double* __restrict__ erw,
+                               const int* __restrict__ off, int4* __restrict__ cells, int* __restrict__ cai,
+                               double* __restrict__ cw)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int x = cells3[3 * (size_t)c], klo = cells3[3 * (size_t)c + 1], khi = cells3[3 * (size_t)c + 2];
+    int e = off[c];
+    for (int k = klo; k < khi; ++k) {
+        const double w = erw[5 * (size_t)k + (x - ebx[k])];
+        if (w != 0.0) {
+            cai[e] = eai[k];
+            cw[e] = w;
+            ++e;
+        }
+    }
+    cells[c] = make_int4(x, off[c], e, 0);
 }
 
 void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
@@ -443,10 +475,29 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
     ILH_HIP(hipGetLastError());
     ILH_HIP(hipMemcpyAsync(&b.n_cells, b.cell_ptr + b.ny, sizeof(int), hipMemcpyDeviceToHost, st));
     ILH_HIP(hipStreamSynchronize(st));
-    ILH_HIP(hipMalloc(&b.cells, sizeof(int) * 3 * (size_t)std::max(1, b.n_cells)));
+    b.h_cell_ptr.resize(b.ny + 1);
+    ILH_HIP(hipMemcpyAsync(b.h_cell_ptr.data(), b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1), hipMemcpyDeviceToHost,
+                           st));
+    ILH_HIP(hipMalloc(&b.cells, sizeof(int4) * (size_t)std::max(1, b.n_cells)));
     if (b.n_cells > 0) {
-        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, b.entries + b.n_entries, b.erw, nx, b.cell_ptr, b.cells);
+        DevBuf cells3, ccnt, coff;
+        cells3.ensure(sizeof(int) * 3 * (size_t)b.n_cells);
+        ccnt.ensure(sizeof(int) * (size_t)b.n_cells);
+        coff.ensure(sizeof(int) * (size_t)(b.n_cells + 1));
+        const int* ebx = b.entries + b.n_entries;
+        const unsigned g = (unsigned)((b.n_cells + 255) / 256);
+        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, b.cell_ptr, cells3.as<int>());
+        k_contrib_count<<<g, 256, 0, st>>>(cells3.as<int>(), b.n_cells, ebx, b.erw, ccnt.as<int>());
+        k_scan<<<1, 1024, 0, st>>>(ccnt.as<int>(), b.n_cells, coff.as<int>());
         ILH_HIP(hipGetLastError());
+        ILH_HIP(hipMemcpyAsync(&b.n_contrib, coff.as<int>() + b.n_cells, sizeof(int), hipMemcpyDeviceToHost, st));
+        ILH_HIP(hipStreamSynchronize(st));
+        ILH_HIP(hipMalloc(&b.cai, sizeof(int) * (size_t)std::max(1, b.n_contrib)));
+        ILH_HIP(hipMalloc(&b.cw, sizeof(double) * (size_t)std::max(1, b.n_contrib)));
+        k_contrib_fill<<<g, 256, 0, st>>>(cells3.as<int>(), b.n_cells, ebx, b.eai, b.erw, coff.as<int>(), b.cells,
+                                          b.cai, b.cw);
+        ILH_HIP(hipGetLastError());
+        ILH_HIP(hipStreamSynchronize(st));
     }
     ILH_HIP(hipStreamSynchronize(st));
     ILH_HIP(hipFree(cnt));
@@ -460,9 +511,15 @@ void free_bins(RowBins& b)
     if (b.erw) (void)hipFree(b.erw);
     if (b.cell_ptr) (void)hipFree(b.cell_ptr);
     if (b.cells) (void)hipFree(b.cells);
+    if (b.cai) (void)hipFree(b.cai);
+    if (b.cw) (void)hipFree(b.cw);
     b.cell_ptr = nullptr;
     b.cells = nullptr;
     b.n_cells = 0;
+    b.cai = nullptr;
+    b.cw = nullptr;
+    b.n_contrib = 0;
+    b.h_cell_ptr.clear();
     b.row_ptr = nullptr;
     b.entries = nullptr;
     b.eai = nullptr;
