@@ -60,6 +60,15 @@ def cpu_baseline(g, s, fb, budget_s):
                       % (n, s.N, t_build)}, f_ref, s_ref
 
 
+def moved(s, angle, heave):
+    """The surface rotated by `angle` about the origin and shifted by `heave`
+    in y: same points, panel lengths and count (rigid-body motion)."""
+    from ilhip.geometry import Surface
+    c, n = np.cos(angle), np.sin(angle)
+    return Surface(c * s.X - n * s.Y, n * s.X + c * s.Y + heave, c * s.nx - n * s.ny, n * s.nx + c * s.ny, s.ds,
+                   s.body)
+
+
 def measured_traffic(stage):
     """HBM bytes per launch of a solve stage from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
@@ -131,6 +140,22 @@ def main():
     # two HIP events on the plan's stream (outside the timed region)
     stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
 
+    # the same solve with host arrays in and out (PCIe-inclusive, not the metric)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        prob.solve(fb)
+    host_sps = 20 / (time.perf_counter() - t0)
+    # new-geometry solve (a moving-body step): update_system! with the bodies
+    # pitched and heaved, then Schur build + factorisation + solve matrices + one solve
+    new_geo_ms = []
+    for k in range(1, 4):
+        cache.update_points(moved(s, 0.02 * k, 0.01 * k))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        prob.solve_device(fp, None, fd, sd)
+        ctx.synchronize()
+        new_geo_ms.append(1e3 * (time.perf_counter() - t0))
+
     # algorithmic bytes per launch (FP64) of each kernel of the solve
     N, b = s.N, 8
     Px = Py = 2048
@@ -172,6 +197,8 @@ def main():
         "solve_model_gbs": solve_gbs,
         "solve_model_frac": solve_gbs / HBM_PEAK_GBS,
         "prep_s": t_prep,
+        "host_io_solves_per_s": host_sps,
+        "new_geometry_solve_ms": float(np.median(new_geo_ms)),
     }
     if not args.no_cpu:
         cb, f_ref, s_ref = cpu_baseline(g, s, fb, args.cpu_budget)

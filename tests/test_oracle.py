@@ -210,6 +210,52 @@ def test_surface_ops_magnitudes():
     assert abs(np.linalg.svd(A, compute_uv=False).max() - 11) < 1                 # :77
 
 
+def test_cross_ops_are_rotated_normal_ops():
+    """n x (f e_z) = (n_y f, -n_x f) and e_z . (n x v) = n_x v_y - n_y v_x
+    (src/cartesian_extensions.jl:19-23): every cross operator equals its
+    plain counterpart with the normals rotated by -90 degrees (regularizing
+    side) or +90 degrees (interpolating side)."""
+    g = O.grid(60, 56, 30, 28, 0.05)
+    G = _G()
+    s = G.star(0.6, [0.08, -0.05, 0.03, 0.0, 0.02], 1.4 * 0.05)
+    geo = _geo(s)
+    rm = O.Geometry(s.X, s.Y, s.ny, -s.nx, s.ds)     # (n_y, -n_x)
+    rp = O.Geometry(s.X, s.Y, -s.ny, s.nx, s.ds)     # (-n_y, n_x)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(s.N)
+    q = rng.standard_normal(O.edge_len(g))
+    sd = rng.standard_normal(O.nel(g, O.DUAL))
+    ph = rng.standard_normal(O.nel(g, O.PRIMAL))
+    assert np.array_equal(O.regularize_normal_cross(g, geo, f), O.regularize_normal(g, rm, f))
+    assert np.array_equal(O.normal_cross_interpolate(g, geo, q), O.normal_interpolate(g, rp, q))
+    assert np.array_equal(O.surface_curl_cross(g, geo, f), O.surface_curl(g, rm, f))
+    assert np.array_equal(O.surface_curl_cross_adj(g, geo, sd), O.surface_curl_adj(g, rp, sd))
+    assert np.array_equal(O.surface_grad_cross(g, geo, ph), O.surface_grad(g, rp, ph))
+
+
+def test_surface_operator_adjoints():
+    """GridScaling: R carries ds/dx^2, E does not, so for the surface
+    calculus pairs  dx^2 <C_s^T f, s> = <f, C_s s>_ds  and, from D = -G^T on
+    fields whose support stays inside the grid,  dx^2 <D_s f, phi> = -<f, G_s phi>_ds."""
+    g, s = _circle_setup()
+    geo = _geo(s)
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(s.N)
+    sd = rng.standard_normal(O.nel(g, O.DUAL))
+    ph = rng.standard_normal(O.nel(g, O.PRIMAL))
+    lhs = g.dx ** 2 * np.dot(O.surface_curl(g, geo, f), sd)
+    rhs = np.dot(f * s.ds, O.surface_curl_adj(g, geo, sd))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    lhs = g.dx ** 2 * np.dot(O.surface_divergence(g, geo, f), ph)
+    rhs = -np.dot(f * s.ds, O.surface_grad(g, geo, ph))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # regularize_normal / normal_interpolate: <Rf(n f), q> dx^2 = <f, n.Rf^T q>_ds
+    q = rng.standard_normal(O.edge_len(g))
+    lhs = g.dx ** 2 * np.dot(O.regularize_normal(g, geo, f), q)
+    rhs = np.dot(f * s.ds, O.normal_interpolate(g, geo, q))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
 def test_mask_area():
     # mask = -L^-1 D_s 1 (src/surface_operators.jl:865-871), area of the unit circle
     g, s = _circle_setup()
