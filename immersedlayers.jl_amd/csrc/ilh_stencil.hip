@@ -23,38 +23,37 @@ __global__ void k_div(int NX, int NY, double dx, const double* __restrict__ u, c
     p[i + (size_t)(NX - 1) * j] = d / dx;
 }
 
-// u(i,j) = (p(i,j) - p(i-1,j)) / dx on interior x-faces, v likewise in y; 0 elsewhere
-__global__ void k_grad_u(int NX, int NY, double dx, const double* __restrict__ p, double* __restrict__ u)
+// u(i,j) = (p(i,j) - p(i-1,j)) / dx on interior x-faces, v likewise in y; 0
+// elsewhere.  One pass over the NX x NY index range writes both components
+// (u is NX x (NY-1), v is (NX-1) x NY), so each p value is fetched once.
+__global__ void k_grad_uv(int NX, int NY, double dx, const double* __restrict__ p, double* __restrict__ u,
+                          double* __restrict__ v)
 {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= NX || j >= NY - 1) return;
-    double r = 0.0;
-    if (i >= 1 && i <= NX - 2) r = (p[i + (size_t)(NX - 1) * j] - p[i - 1 + (size_t)(NX - 1) * j]) / dx;
-    u[i + (size_t)NX * j] = r;
+    if (i >= NX || j >= NY) return;
+    const size_t np = NX - 1;
+    const double pc = (i <= NX - 2 && j <= NY - 2) ? p[i + np * j] : 0.0;
+    if (j <= NY - 2) {
+        double r = 0.0;
+        if (i >= 1 && i <= NX - 2) r = (pc - p[i - 1 + np * j]) / dx;
+        u[i + (size_t)NX * j] = r;
+    }
+    if (i <= NX - 2) {
+        double r = 0.0;
+        if (j >= 1 && j <= NY - 2) r = (pc - p[i + np * (j - 1)]) / dx;
+        v[i + np * j] = r;
+    }
 }
 
-__global__ void k_grad_v(int NX, int NY, double dx, const double* __restrict__ p, double* __restrict__ v)
+// u(i,j) = (s(i,j+1) - s(i,j)) / dx,  v(i,j) = (s(i,j) - s(i+1,j)) / dx, in one pass
+__global__ void k_curl_n2e_uv(int NX, int NY, double dx, const double* __restrict__ s, double* __restrict__ u,
+                              double* __restrict__ v)
 {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= NX - 1 || j >= NY) return;
-    double r = 0.0;
-    if (j >= 1 && j <= NY - 2) r = (p[i + (size_t)(NX - 1) * j] - p[i + (size_t)(NX - 1) * (j - 1)]) / dx;
-    v[i + (size_t)(NX - 1) * j] = r;
-}
-
-// u(i,j) = (s(i,j+1) - s(i,j)) / dx,  v(i,j) = (s(i,j) - s(i+1,j)) / dx
-__global__ void k_curl_n2e_u(int NX, int NY, double dx, const double* __restrict__ s, double* __restrict__ u)
-{
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= NX || j >= NY - 1) return;
-    u[i + (size_t)NX * j] = (s[i + (size_t)NX * (j + 1)] - s[i + (size_t)NX * j]) / dx;
-}
-
-__global__ void k_curl_n2e_v(int NX, int NY, double dx, const double* __restrict__ s, double* __restrict__ v)
-{
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= NX - 1 || j >= NY) return;
-    v[i + (size_t)(NX - 1) * j] = (s[i + (size_t)NX * j] - s[i + 1 + (size_t)NX * j]) / dx;
+    if (i >= NX || j >= NY) return;
+    const double sc = s[i + (size_t)NX * j];
+    if (j <= NY - 2) u[i + (size_t)NX * j] = (s[i + (size_t)NX * (j + 1)] - sc) / dx;
+    if (i <= NX - 2) v[i + (size_t)(NX - 1) * j] = (sc - s[i + 1 + (size_t)NX * j]) / dx;
 }
 
 // w(i,j) = ((v(i,j) - v(i-1,j)) - (u(i,j) - u(i,j-1))) / dx on interior dual nodes, ring 0
@@ -94,16 +93,15 @@ void k_divergence(ilh_plan* pl, double* p, const double* q)
 void k_grad(ilh_plan* pl, double* q, const double* p)
 {
     const Grid& g = pl->g;
-    k_grad_u<<<grid2(g.NX, g.NY - 1), dim3(BX, BY), 0, pl->ctx->stream>>>(g.NX, g.NY, g.dx, p, q);
-    k_grad_v<<<grid2(g.NX - 1, g.NY), dim3(BX, BY), 0, pl->ctx->stream>>>(g.NX, g.NY, g.dx, p, q + g.n(GK_EDGE_U));
+    k_grad_uv<<<grid2(g.NX, g.NY), dim3(BX, BY), 0, pl->ctx->stream>>>(g.NX, g.NY, g.dx, p, q, q + g.n(GK_EDGE_U));
     ILH_HIP(hipGetLastError());
 }
 
 void k_curl_n2e(ilh_plan* pl, double* q, const double* s)
 {
     const Grid& g = pl->g;
-    k_curl_n2e_u<<<grid2(g.NX, g.NY - 1), dim3(BX, BY), 0, pl->ctx->stream>>>(g.NX, g.NY, g.dx, s, q);
-    k_curl_n2e_v<<<grid2(g.NX - 1, g.NY), dim3(BX, BY), 0, pl->ctx->stream>>>(g.NX, g.NY, g.dx, s, q + g.n(GK_EDGE_U));
+    k_curl_n2e_uv<<<grid2(g.NX, g.NY), dim3(BX, BY), 0, pl->ctx->stream>>>(g.NX, g.NY, g.dx, s, q,
+                                                                           q + g.n(GK_EDGE_U));
     ILH_HIP(hipGetLastError());
 }
 

@@ -529,39 +529,33 @@ void free_bins(RowBins& b)
 }
 
 // --------------------------------------------------------------- scatter
-// out(x, j) = sum over the row's entries with bx <= x < bx+5, in (bx, term)
-// order, of amp[ai] * w[x - bx].
-__global__ void k_scatter(const int* __restrict__ ptr, const int* __restrict__ ebx, const int* __restrict__ eai,
-                          const double* __restrict__ erw, int nx, const double* __restrict__ amp,
-                          double* __restrict__ out)
+// out(x, j): zero, except at the row's cells, whose value is the sum of their
+// contributions in (bx, term) order.  A work-group per row: a coalesced zero
+// sweep, then one thread per cell.
+__global__ void k_scatter(const int* __restrict__ cell_ptr, const int4* __restrict__ cells,
+                          const int* __restrict__ cai, const double* __restrict__ cw, int nx,
+                          const double* __restrict__ amp, double* __restrict__ out)
 {
-    const int j = blockIdx.y;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nx) return;
-    const int lo = ptr[j], hi = ptr[j + 1];
-    double acc = 0.0;
-    if (lo < hi) {
-        int a = lo, b = hi;
-        while (a < b) {
-            const int m = (a + b) >> 1;
-            if (ebx[m] < x - 4) a = m + 1; else b = m;
-        }
-        for (int k = a; k < hi; ++k) {
-            const int bx = ebx[k];
-            if (bx > x) break;
-            acc += amp[eai[k]] * erw[5 * (size_t)k + (x - bx)];
-        }
+    const int j = blockIdx.x;
+    double* row = out + (size_t)nx * j;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) row[x] = 0.0;
+    const int c0 = cell_ptr[j], c1 = cell_ptr[j + 1];
+    if (c0 == c1) return;   // uniform over the work-group
+    __syncthreads();        // the zeros are ordered before the cell values
+    for (int ci = c0 + threadIdx.x; ci < c1; ci += blockDim.x) {
+        const int4 cr = cells[ci];
+        double acc = 0.0;
+        for (int e = cr.y; e < cr.z; ++e) acc += amp[cai[e]] * cw[e];
+        row[cr.x] = acc;
     }
-    out[x + (size_t)nx * j] = acc;
 }
 
 void scatter(ilh_plan* pl, const Term* terms, const RowBins& bins, const double* amp, double* out)
 {
     (void)terms;
     const int nx = pl->g.nx(bins.gk);
-    dim3 grid((nx + 255) / 256, bins.ny);
-    k_scatter<<<grid, 256, 0, pl->ctx->stream>>>(bins.row_ptr, bins.entries + bins.n_entries, bins.eai, bins.erw, nx,
-                                                 amp, out);
+    if (bins.ny <= 0) return;
+    k_scatter<<<bins.ny, 256, 0, pl->ctx->stream>>>(bins.cell_ptr, bins.cells, bins.cai, bins.cw, nx, amp, out);
     ILH_HIP(hipGetLastError());
 }
 
