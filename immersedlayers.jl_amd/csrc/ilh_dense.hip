@@ -195,7 +195,8 @@ static void gemm_sub(hipStream_t st, int M, int Nc, int K, const double* A, int 
 static constexpr int NB = 32;
 
 // Unblocked LU of the panel A[k0:n, k0:k0+nb] with partial pivoting (one
-// work-group).  Row swaps are applied inside the panel only.
+// work-group, global memory).  Used when the panel's sub-columns do not fit
+// in LDS (n - k0 > 8192).  Row swaps are applied inside the panel only.
 __global__ void __launch_bounds__(1024) k_panel(double* __restrict__ A, int n, int lda, int k0, int nb,
                                                 int* __restrict__ piv, int* __restrict__ info)
 {
@@ -250,6 +251,303 @@ __global__ void __launch_bounds__(1024) k_panel(double* __restrict__ A, int n, i
     }
 }
 
+// Work-group barrier ordering LDS accesses only: outstanding global stores of
+// the calling waves need not complete (a plain __syncthreads waits for them).
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// max |value| over the work-group with the smallest row index among ties
+__device__ __forceinline__ void argmax_block(double& best, int& bi, double* wv, int* wi)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (o > best || (o == best && oi < bi)) { best = o; bi = oi; }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
+    lds_sync();
+    // every thread reduces the per-wave results itself (same order everywhere)
+    const int nw = blockDim.x >> 6;
+    best = wv[0];
+    bi = wi[0];
+    for (int w = 1; w < nw; ++w) {
+        const double o = wv[w];
+        const int oi = wi[w];
+        if (o > best || (o == best && oi < bi)) { best = o; bi = oi; }
+    }
+    lds_sync();   // wv/wi may be overwritten by the next call
+}
+
+static constexpr int PT = 512;   // threads of the panel work-group
+
+// Composite of the row interchanges (r_k <-> p_k, k = 0..cnt-1, in order) on
+// the rows they touch: afterwards row key[i] holds the old row val[i].  Run by
+// one thread with the table in registers (MAXK >= 2 cnt); returns the count.
+template <int MAXK>
+__device__ int swap_composite(const int* r, const int* pv, int cnt, int* key_out, int* val_out)
+{
+    int key[MAXK], val[MAXK];
+    int nk = 0;
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i) { key[i] = -1; val[i] = -1; }
+    for (int k = 0; k < cnt; ++k) {
+        const int a = r[k], b = pv[k];
+        if (a == b) continue;
+        int ia = -1, ib = -1;
+#pragma unroll
+        for (int i = 0; i < MAXK; ++i) {
+            if (key[i] == a) ia = i;
+            if (key[i] == b) ib = i;
+        }
+        if (ia < 0) {
+#pragma unroll
+            for (int i = 0; i < MAXK; ++i)
+                if (i == nk) { key[i] = a; val[i] = a; }
+            ia = nk++;
+        }
+        if (ib < 0) {
+#pragma unroll
+            for (int i = 0; i < MAXK; ++i)
+                if (i == nk) { key[i] = b; val[i] = b; }
+            ib = nk++;
+        }
+        int va = 0, vb = 0;
+#pragma unroll
+        for (int i = 0; i < MAXK; ++i) {
+            if (i == ia) va = val[i];
+            if (i == ib) vb = val[i];
+        }
+#pragma unroll
+        for (int i = 0; i < MAXK; ++i) {
+            if (i == ia) val[i] = vb;
+            if (i == ib) val[i] = va;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXK; ++i)
+        if (i < nk) { key_out[i] = key[i]; val_out[i] = val[i]; }
+    return nk;
+}
+
+// Panel LU with partial pivoting, W columns at a time in LDS (one work-group
+// of PT threads).  Sub-panel s holds A[k0:n, k0+s:k0+s+W] as columns of m =
+// n - k0 rows; after factoring it, its row swaps reach the rest of the panel
+// and the rest of the panel is updated (unit-lower solve of its W rows, then a
+// rank-W update below), one global read-modify-write per sub-panel.  At the
+// end: pmap = {count, (row, source row) pairs} of the panel's interchanges for
+// the columns outside the panel.
+template <int W>
+__global__ void __launch_bounds__(PT) k_panel_lds(double* __restrict__ A, int n, int lda, int k0, int nb,
+                                                    int* __restrict__ piv, int* __restrict__ info,
+                                                    int* __restrict__ pmap)
+{
+    extern __shared__ double sp[];                 // W x m, column-major
+    __shared__ double wv[16];
+    __shared__ int wi[16];
+    __shared__ double Us[W][NB];
+    __shared__ int spiv[NB], srow[NB];
+    __shared__ double sbest[NB];
+    __shared__ int skey[2 * NB], sval[2 * NB];
+    __shared__ int snk;
+    const int m = n - k0, tid = threadIdx.x;
+    constexpr int RB = W >= 8 ? 2 : 4;   // rows per thread per update batch (register budget)
+    for (int s0 = 0; s0 < nb; s0 += W) {
+        const int w = min(W, nb - s0);
+        // 16 loads in flight per thread, then their LDS stores
+        for (int base = 0; base < w * m; base += 16 * PT) {
+            double tmp[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = base + u * PT + tid;
+                const int c = e / m, r = e - c * m;
+                tmp[u] = e < w * m ? A[(k0 + r) + (size_t)lda * (k0 + s0 + c)] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = base + u * PT + tid;
+                if (e < w * m) sp[e] = tmp[u];
+            }
+        }
+        __syncthreads();
+        for (int kk = 0; kk < w; ++kk) {
+            const int col = s0 + kk;                // panel row of the diagonal
+            double* x = sp + (size_t)kk * m;
+            double best = -1.0;
+            int bi = m;
+            for (int r = col + tid; r < m; r += PT) {
+                const double v = fabs(x[r]);
+                if (v > best) { best = v; bi = r; }
+            }
+            argmax_block(best, bi, wv, wi);
+            if (tid == 0) {
+                spiv[col] = bi;
+                srow[col] = col;
+                sbest[col] = best;
+            }
+            if (best != 0.0) {
+                if (bi != col && tid < w) {
+                    double* y = sp + (size_t)tid * m;
+                    const double t = y[col];
+                    y[col] = y[bi];
+                    y[bi] = t;
+                }
+                lds_sync();
+                const double inv = 1.0 / x[col];
+                for (int r = col + 1 + tid; r < m; r += PT) {
+                    const double l = x[r] * inv;
+                    x[r] = l;
+                    for (int c = kk + 1; c < w; ++c) sp[(size_t)c * m + r] -= l * sp[(size_t)c * m + col];
+                }
+            }
+            lds_sync();
+        }
+        if (tid == 0)
+            for (int kk = 0; kk < w; ++kk) {
+                piv[k0 + s0 + kk] = k0 + spiv[s0 + kk];
+                if (sbest[s0 + kk] == 0.0 && *info == 0) *info = k0 + s0 + kk + 1;
+            }
+        for (int e = tid; e < w * m; e += PT) {
+            const int c = e / m, r = e - c * m;
+            A[(k0 + r) + (size_t)lda * (k0 + s0 + c)] = sp[(size_t)c * m + r];
+        }
+        if (tid == 0) snk = swap_composite<2 * W>(srow + s0, spiv + s0, w, skey, sval);
+        __syncthreads();
+        const int j0 = s0 + w, rest = nb - j0;
+        // this sub-panel's interchanges on the panel columns before and after it
+        if (tid < nb && (tid < s0 || tid >= j0)) {
+            double* col = A + k0 + (size_t)lda * (k0 + tid);
+            const int nk = snk;
+            double v[2 * W];
+#pragma unroll
+            for (int i = 0; i < 2 * W; ++i) v[i] = i < nk ? col[sval[i]] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 2 * W; ++i)
+                if (i < nk) col[skey[i]] = v[i];
+        }
+        __syncthreads();
+        if (rest > 0) {
+            // U rows: unit-lower solve of the W x rest block
+            if (tid < rest) {
+                double* col = A + k0 + (size_t)lda * (k0 + j0 + tid);
+                double u[W];
+#pragma unroll
+                for (int c = 0; c < W; ++c) u[c] = c < w ? col[s0 + c] : 0.0;
+#pragma unroll
+                for (int c = 0; c < W; ++c)
+                    if (c < w) {
+#pragma unroll
+                        for (int k = 0; k < c; ++k) u[c] -= sp[(size_t)k * m + s0 + c] * u[k];
+                    }
+#pragma unroll
+                for (int c = 0; c < W; ++c) {
+                    if (c < w) col[s0 + c] = u[c];
+                    Us[c][tid] = c < w ? u[c] : 0.0;
+                }
+            }
+            __syncthreads();
+            // rank-w update of the rows below: per thread RB rows x 8 columns of
+            // loads in flight, then the arithmetic, then the stores
+            for (int r0 = j0 + tid; r0 < m; r0 += RB * PT) {
+#pragma unroll 1
+                for (int jb = 0; jb < rest; jb += 8) {
+                    double acc[RB][8];
+#pragma unroll
+                    for (int q = 0; q < RB; ++q) {
+                        const int r = r0 + q * PT;
+                        const double* a = A + (k0 + r) + (size_t)lda * (k0 + j0 + jb);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) acc[q][j] = (r < m && jb + j < rest) ? a[(size_t)lda * j] : 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < RB; ++q) {
+                        const int r = r0 + q * PT;
+#pragma unroll
+                        for (int c = 0; c < W; ++c) {
+                            const double l = (r < m && c < w) ? sp[(size_t)c * m + r] : 0.0;
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) acc[q][j] -= l * Us[c][jb + j];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < RB; ++q) {
+                        const int r = r0 + q * PT;
+                        double* a = A + (k0 + r) + (size_t)lda * (k0 + j0 + jb);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            if (r < m && jb + j < rest) a[(size_t)lda * j] = acc[q][j];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // the panel's interchange map for the columns outside it: the swaps
+    // replayed on a row-index table in the (now free) LDS, then every moved
+    // row listed
+    int* pl_ = (int*)sp;
+    __shared__ int cnt;
+    for (int r = tid; r < m; r += PT) pl_[r] = r;
+    if (tid == 0) cnt = 0;
+    __syncthreads();
+    if (tid == 0)
+        for (int k = 0; k < nb; ++k) {
+            const int q = spiv[k];
+            if (q != k) {
+                const int t = pl_[k];
+                pl_[k] = pl_[q];
+                pl_[q] = t;
+            }
+        }
+    __syncthreads();
+    for (int r = tid; r < m; r += PT)
+        if (pl_[r] != r) {
+            const int i = atomicAdd(&cnt, 1);
+            pmap[1 + 2 * i] = k0 + r;
+            pmap[2 + 2 * i] = k0 + pl_[r];
+        }
+    __syncthreads();
+    if (tid == 0) pmap[0] = cnt;
+}
+
+// Rows key <- old rows val (the panel's interchange map) on the columns outside
+// [k0, k0+nb): 64 columns per work-group, all reads before any write.
+__global__ void __launch_bounds__(256) k_apply_pmap(double* __restrict__ A, int n, int lda, int k0, int nb,
+                                                    const int* __restrict__ pmap)
+{
+    __shared__ double vals[64][2 * NB + 1];
+    __shared__ int pk[2 * NB], pvv[2 * NB];
+    const int cnt = pmap[0];
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        pk[i] = pmap[1 + 2 * i];
+        pvv[i] = pmap[2 + 2 * i];
+    }
+    __syncthreads();
+    const int nout = n - nb;
+    const int jb = blockIdx.x * 64;
+    for (int e = threadIdx.x; e < 64 * cnt; e += 256) {
+        const int i = e % cnt, jj = e / cnt;
+        int j = jb + jj;
+        if (j < nout) {
+            if (j >= k0) j += nb;
+            vals[jj][i] = A[pvv[i] + (size_t)lda * j];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * cnt; e += 256) {
+        const int i = e % cnt, jj = e / cnt;
+        int j = jb + jj;
+        if (j < nout) {
+            if (j >= k0) j += nb;
+            A[pk[i] + (size_t)lda * j] = vals[jj][i];
+        }
+    }
+}
+
 // apply the panel's row interchanges to columns outside [k0, k0+nb)
 __global__ void k_swap_rows(double* __restrict__ A, int n, int lda, int k0, int nb, const int* __restrict__ piv)
 {
@@ -266,60 +564,118 @@ __global__ void k_swap_rows(double* __restrict__ A, int n, int lda, int k0, int 
     }
 }
 
-// X[k0:k0+nb, j] = L11^{-1} X[k0:k0+nb, j] (unit lower, L11 at LU[k0,k0]) for each column j
-__global__ void k_trsm_lower(const double* __restrict__ L, int ldl, int k0, int nb, double* __restrict__ X, int ldx,
-                             int ncols, int col0)
+// Inverse of the nb x nb diagonal block at (k0, k0) of the LU factors, padded
+// to NB x NB with the identity: UPPER = 0 the unit-lower L11, 1 the upper U11.
+// Thread c forms column c by substitution on e_c.  With n > 0, block b
+// handles the diagonal block at k0 = b NB of an n x n factor (all blocks in
+// one launch) and writes Tinv + b NB^2.
+template <int UPPER>
+__global__ void __launch_bounds__(NB) k_tri_inv(const double* __restrict__ LU, int ld, int k0, int nb,
+                                                double* __restrict__ Tinv, int n = 0)
 {
-    __shared__ double Ls[NB][NB + 1];
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-        const int r = e % nb, c = e / nb;
-        Ls[r][c] = L[(k0 + r) + (size_t)ldl * (k0 + c)];
+    __shared__ double T[NB][NB + 1];
+    if (n > 0) {
+        k0 = blockIdx.x * NB;
+        nb = min(NB, n - k0);
+        Tinv += (size_t)blockIdx.x * NB * NB;
+    }
+    for (int e = threadIdx.x; e < NB * NB; e += NB) {
+        const int r = e % NB, c = e / NB;
+        T[r][c] = (r < nb && c < nb) ? LU[(k0 + r) + (size_t)ld * (k0 + c)] : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncols) return;
-    double* x = X + (size_t)ldx * (col0 + j) + k0;
-    double v[NB];
-    for (int r = 0; r < nb; ++r) v[r] = x[r];
-    for (int r = 0; r < nb; ++r)
-        for (int c = 0; c < r; ++c) v[r] -= Ls[r][c] * v[c];
-    for (int r = 0; r < nb; ++r) x[r] = v[r];
+    const int c = threadIdx.x;
+    double x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) x[r] = r == c ? 1.0 : 0.0;
+    if (!UPPER) {
+#pragma unroll
+        for (int r = 0; r < NB; ++r)
+#pragma unroll
+            for (int k = 0; k < r; ++k) x[r] -= T[r][k] * x[k];
+    } else {
+#pragma unroll
+        for (int r = NB - 1; r >= 0; --r) {
+            double v = x[r];
+#pragma unroll
+            for (int k = r + 1; k < NB; ++k) v -= T[r][k] * x[k];
+            x[r] = v / T[r][r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) Tinv[r + NB * c] = x[r];
 }
 
-// X[k0:k0+nb, j] = U11^{-1} X[k0:k0+nb, j] (upper, non-unit)
-__global__ void k_trsm_upper(const double* __restrict__ U, int ldu, int k0, int nb, double* __restrict__ X, int ldx,
-                             int ncols)
+// X[k0:k0+nb, col0+j] = Tinv X[k0:k0+nb, col0+j] for j < ncols: 64 columns per
+// work-group, the block and the tile staged in LDS.
+__global__ void __launch_bounds__(256) k_tri_apply(const double* __restrict__ Tinv, int k0, int nb,
+                                                   double* __restrict__ X, int ldx, int ncols, int col0)
 {
-    __shared__ double Us[NB][NB + 1];
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-        const int r = e % nb, c = e / nb;
-        Us[r][c] = U[(k0 + r) + (size_t)ldu * (k0 + c)];
+    __shared__ double Ts[NB][NB + 1];
+    __shared__ double Xs[64][NB + 1];
+    const int j0 = blockIdx.x * 64;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) Ts[e % NB][e / NB] = Tinv[e];
+    for (int e = threadIdx.x; e < 64 * NB; e += 256) {
+        const int r = e % NB, j = e / NB;
+        Xs[j][r] = (r < nb && j0 + j < ncols) ? X[(k0 + r) + (size_t)ldx * (col0 + j0 + j)] : 0.0;
     }
     __syncthreads();
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncols) return;
-    double* x = X + (size_t)ldx * j + k0;
-    double v[NB];
-    for (int r = 0; r < nb; ++r) v[r] = x[r];
-    for (int r = nb - 1; r >= 0; --r) {
-        double s = v[r];
-        for (int c = r + 1; c < nb; ++c) s -= Us[r][c] * v[c];
-        v[r] = s / Us[r][r];
+    for (int e = threadIdx.x; e < 64 * NB; e += 256) {
+        const int r = e % NB, j = e / NB;
+        if (r >= nb || j0 + j >= ncols) continue;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < NB; ++k) acc += Ts[r][k] * Xs[j][k];
+        X[(k0 + r) + (size_t)ldx * (col0 + j0 + j)] = acc;
     }
-    for (int r = 0; r < nb; ++r) x[r] = v[r];
+}
+
+// panel + its interchange map; false: the global fallback ran and the caller
+// applies the swaps one by one
+static bool panel_factor(hipStream_t st, double* A, int n, int k0, int nb, int* piv, int* info, int* pmap)
+{
+    const int m = n - k0;
+    const void* fn = nullptr;
+    int W = 0;
+    if (m <= 2048) { fn = (const void*)k_panel_lds<8>; W = 8; }
+    else if (m <= 4096) { fn = (const void*)k_panel_lds<4>; W = 4; }
+    else if (m <= 8192) { fn = (const void*)k_panel_lds<2>; W = 2; }
+    if (!W) {
+        k_panel<<<1, 1024, 0, st>>>(A, n, n, k0, nb, piv, info);
+        return false;
+    }
+    const size_t lds = sizeof(double) * (size_t)W * m;   // W * m <= 16384 doubles
+    static bool set[3] = {false, false, false};
+    const int slot = W == 8 ? 0 : W == 4 ? 1 : 2;
+    if (!set[slot]) {
+        ILH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(16384 * sizeof(double))));
+        set[slot] = true;
+    }
+    if (W == 8) k_panel_lds<8><<<1, PT, lds, st>>>(A, n, n, k0, nb, piv, info, pmap);
+    else if (W == 4) k_panel_lds<4><<<1, PT, lds, st>>>(A, n, n, k0, nb, piv, info, pmap);
+    else k_panel_lds<2><<<1, PT, lds, st>>>(A, n, n, k0, nb, piv, info, pmap);
+    return true;
 }
 
 int lu_factor(ilh_plan* pl, double* A, int n, int* piv, int* info)
 {
     hipStream_t st = pl->ctx->stream;
     ILH_HIP(hipMemsetAsync(info, 0, sizeof(int), st));
+    pl->lu_tinv.ensure(sizeof(double) * NB * NB);
+    pl->lu_pmap.ensure(sizeof(int) * (1 + 4 * NB));
+    double* Tinv = pl->lu_tinv.as<double>();
+    int* pmap = pl->lu_pmap.as<int>();
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = std::min(NB, n - k0);
-        k_panel<<<1, 1024, 0, st>>>(A, n, n, k0, nb, piv, info);
-        if (n - nb > 0) k_swap_rows<<<(n - nb + 255) / 256, 256, 0, st>>>(A, n, n, k0, nb, piv);
+        const bool fused = panel_factor(st, A, n, k0, nb, piv, info, pmap);
+        if (n - nb > 0) {
+            if (fused) k_apply_pmap<<<(n - nb + 63) / 64, 256, 0, st>>>(A, n, n, k0, nb, pmap);
+            else k_swap_rows<<<(n - nb + 255) / 256, 256, 0, st>>>(A, n, n, k0, nb, piv);
+        }
         const int k1 = k0 + nb, rest = n - k1;
         if (rest > 0) {
-            k_trsm_lower<<<(rest + 127) / 128, 128, 0, st>>>(A, n, k0, nb, A, n, rest, k1);
+            k_tri_inv<0><<<1, NB, 0, st>>>(A, n, k0, nb, Tinv);
+            k_tri_apply<<<(rest + 63) / 64, 256, 0, st>>>(Tinv, k0, nb, A, n, rest, k1);
             gemm_sub(st, rest, rest, nb, A + k1 + (size_t)n * k0, n, A + k0 + (size_t)n * k1, n,
                      A + k1 + (size_t)n * k1, n);
         }
@@ -331,36 +687,67 @@ int lu_factor(ilh_plan* pl, double* A, int n, int* piv, int* info)
     return h;
 }
 
-__global__ void k_apply_piv(double* __restrict__ B, int n, int ldb, int ncols, const int* __restrict__ piv)
+// perm[i] = the row of B that lands in row i after the interchanges piv[0..n)
+// are applied in order (LAPACK getrs order); one work-group, LDS when n fits.
+__global__ void __launch_bounds__(256) k_perm_from_piv(const int* __restrict__ piv, int n, int* __restrict__ perm)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncols) return;
-    double* b = B + (size_t)ldb * j;
-    for (int k = 0; k < n; ++k) {
-        const int p = piv[k];
-        if (p != k) {
-            const double t = b[k];
-            b[k] = b[p];
-            b[p] = t;
+    extern __shared__ int pl_[];
+    const bool lds = n <= 16384;
+    int* p = lds ? pl_ : perm;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < n; ++k) {
+            const int q = piv[k];
+            if (q != k) {
+                const int t = p[k];
+                p[k] = p[q];
+                p[q] = t;
+            }
         }
-    }
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = p[i];
 }
 
-// B <- A^{-1} B for the LU factors of A (n x n) and nrhs columns of B.
+// Bp[i, j] = B[perm[i], j]
+__global__ void k_gather_rows(const double* __restrict__ B, int n, int ncols, const int* __restrict__ perm,
+                              double* __restrict__ Bp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n || j >= ncols) return;
+    Bp[i + (size_t)n * j] = B[perm[i] + (size_t)n * j];
+}
+
+// B <- A^{-1} B for the LU factors of A (n x n) and nrhs columns of B:
+// permute, then blocked forward and backward substitution whose diagonal
+// blocks are applied as explicit NB x NB inverses.
 void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, double* B, int nrhs)
 {
     hipStream_t st = pl->ctx->stream;
-    k_apply_piv<<<(nrhs + 255) / 256, 256, 0, st>>>(B, n, n, nrhs, piv);
+    pl->lu_perm.ensure(sizeof(int) * (size_t)n);
+    pl->lu_tmp.ensure(sizeof(double) * (size_t)n * nrhs);
+    const int nblk = (n + NB - 1) / NB;
+    pl->lu_tinv.ensure(sizeof(double) * NB * NB * 2 * (size_t)nblk);
+    int* perm = pl->lu_perm.as<int>();
+    double* Bp = pl->lu_tmp.as<double>();
+    double* Linv = pl->lu_tinv.as<double>();
+    double* Uinv = Linv + (size_t)NB * NB * nblk;
+    k_tri_inv<0><<<nblk, NB, 0, st>>>(LU, n, 0, 0, Linv, n);
+    k_tri_inv<1><<<nblk, NB, 0, st>>>(LU, n, 0, 0, Uinv, n);
+    k_perm_from_piv<<<1, 256, n <= 16384 ? sizeof(int) * (size_t)n : 0, st>>>(piv, n, perm);
+    k_gather_rows<<<dim3((n + 255) / 256, nrhs), 256, 0, st>>>(B, n, nrhs, perm, Bp);
+    ILH_HIP(hipMemcpyAsync(B, Bp, sizeof(double) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, st));
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = std::min(NB, n - k0);
-        k_trsm_lower<<<(nrhs + 127) / 128, 128, 0, st>>>(LU, n, k0, nb, B, n, nrhs, 0);
+        k_tri_apply<<<(nrhs + 63) / 64, 256, 0, st>>>(Linv + (size_t)NB * NB * (k0 / NB), k0, nb, B, n, nrhs, 0);
         const int k1 = k0 + nb;
         gemm_sub(st, n - k1, nrhs, nb, LU + k1 + (size_t)n * k0, n, B + k0, n, B + k1, n);
     }
     const int last = ((n - 1) / NB) * NB;
     for (int k0 = last; k0 >= 0; k0 -= NB) {
         const int nb = std::min(NB, n - k0);
-        k_trsm_upper<<<(nrhs + 127) / 128, 128, 0, st>>>(LU, n, k0, nb, B, n, nrhs);
+        k_tri_apply<<<(nrhs + 63) / 64, 256, 0, st>>>(Uinv + (size_t)NB * NB * (k0 / NB), k0, nb, B, n, nrhs, 0);
         gemm_sub(st, k0, nrhs, nb, LU + (size_t)n * k0, n, B + k0, n, B, n);
     }
     ILH_HIP(hipGetLastError());

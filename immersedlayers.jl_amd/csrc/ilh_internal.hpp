@@ -145,6 +145,7 @@ struct ilh_plan {
     int lgf_mx = 0, lgf_my = 0;
     ilh::DevBuf io_a, io_b, io_c, io_d;       // staging for host-memory calls
     ilh::DevBuf scratch_a, scratch_b;         // grid scratch
+    ilh::DevBuf lu_tinv, lu_perm, lu_tmp, lu_pmap;   // LU scratch (block inverses, row permutations)
     SolveCache* dir = nullptr;                // Dirichlet
     SolveCache* neu = nullptr;                // Neumann
     SolveCache* fac[4] = {};                  // ilh_schur_factor caches per kind
