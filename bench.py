@@ -216,6 +216,22 @@ def main():
         ctx.synchronize()
         new_geo_ms.append(1e3 * (time.perf_counter() - t0))
 
+    # moving bodies (the learned-preconditioner workload's motion, solved by
+    # GMRES preconditioned with a reference geometry's inverse): heave
+    # 0.2 sin(2 pi t) and pitch 0.05 sin(2 pi t) about the origin, dt = 0.004
+    cache.update_points(s)
+    mv_ms, mv_it = [], []
+    for k in range(11):
+        t = 0.004 * k
+        cache.update_points(moved(s, 0.05 * np.sin(2 * np.pi * t), 0.2 * np.sin(2 * np.pi * t)))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        it = prob.solve_moving_device(fp, None, fd, sd, tol=1e-12, max_iter=30)
+        ctx.synchronize()
+        if k:   # step 0 factors the reference geometry
+            mv_ms.append(1e3 * (time.perf_counter() - t0))
+            mv_it.append(it)
+
     # algorithmic bytes per launch (FP64) of each kernel of the solve
     N, b = s.N, 8
     Px = Py = 2048
@@ -259,6 +275,8 @@ def main():
         "prep_s": t_prep,
         "host_io_solves_per_s": host_sps,
         "new_geometry_solve_ms": float(np.median(new_geo_ms)),
+        "moving_step_ms": float(np.mean(mv_ms)),
+        "moving_gmres_iterations": mv_it,
     }
     if not args.no_cpu:
         cb, f_ref, s_ref = cpu_baseline(g, s, fb, args.cpu_budget)

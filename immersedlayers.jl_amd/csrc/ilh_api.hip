@@ -80,8 +80,33 @@ struct SolveCache {
     }
 };
 
+// Moving bodies: the inverse Schur matrix of a reference geometry (row-major,
+// leading dimension solve_ld(N)) preconditions GMRES on the current one; the
+// current geometry's merged sources [R, D_s] are rebuilt after every
+// update_points.
+struct MovingCache {
+    DevBuf Pinv;
+    bool have_ref = false;
+    int ref_age = 0;           // solves since the reference was factored
+    DevBuf terms1, amp, V, w, z, x, b, h, fbar, dvec, grid;
+    SourceSet src1;
+    bool src_ok = false;
+    void drop_sources()
+    {
+        if (src_ok) {
+            free_bins(src1.bins);
+            if (src1.pinfo) (void)hipFree(src1.pinfo);
+            if (src1.slot_of_row) (void)hipFree(src1.slot_of_row);
+            src1 = SourceSet();
+        }
+        src_ok = false;
+    }
+    ~MovingCache() { drop_sources(); }
+};
+
 static void drop_caches(ilh_plan* pl)
 {
+    if (pl->mov) pl->mov->drop_sources();
     delete pl->dir;
     delete pl->neu;
     pl->dir = pl->neu = nullptr;
@@ -252,6 +277,7 @@ extern "C" int ilh_plan_destroy(ilh_plan* pl)
         (void)hipSetDevice(pl->ctx->device);
         (void)hipStreamSynchronize(pl->ctx->stream);
         drop_caches(pl);
+        delete pl->mov;
         delete pl;
     });
 }
@@ -784,6 +810,251 @@ extern "C" int ilh_neumann_solve(ilh_plan* pl, const double* vp, const double* v
         io.back(df, d_df, N);
         io.back(s, d_s, nd);
         io.back(dsv, d_ds, N);
+        io.done();
+    });
+}
+
+// ------------------------------------------------- moving-body Dirichlet
+namespace {
+
+// out[i] = sum_k V[k + n i] w[k], i < nv: one work-group per i, fixed order
+__global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ V, int nv, const double* __restrict__ w,
+                                              int n, double* __restrict__ out)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.x;
+    if (i >= nv) return;
+    const double* v = V + (size_t)n * i;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) acc += v[k] * w[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
+}
+
+// w[k] -= sum_{i < nv} V[k + n i] h[i]
+__global__ void k_mupdate(const double* __restrict__ V, int nv, const double* __restrict__ h, int n,
+                          double* __restrict__ w)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double acc = w[k];
+    for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)n * i] * h[i];
+    w[k] = acc;
+}
+
+// d = f+ - f-,  fbar = (f+ + f-)/2
+__global__ void k_jump_avg(const double* __restrict__ fp, const double* __restrict__ fm, int n, double* __restrict__ d,
+                           double* __restrict__ fbar)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double a = fp[k], b = fm ? fm[k] : 0.0;
+    d[k] = a - b;
+    fbar[k] = 0.5 * (a + b);
+}
+
+__global__ void k_scale(double* __restrict__ y, const double* __restrict__ x, double a, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = a * x[k];
+}
+
+__global__ void k_identity(double* __restrict__ A, int n)   // A (zeroed) += I, column-major n x n
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) A[k + (size_t)n * k] = 1.0;
+}
+
+}  // namespace
+
+// S v = -E L^-1 R v (src/matrix_operators.jl:16-26) applied matrix-free: the
+// R half of the merged sources carries v, the D_s half zero.
+static void schur_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* out)
+{
+    const int N = pl->N;
+    hipStream_t st = pl->ctx->stream;
+    double* amp = mc->amp.as<double>();
+    ILH_HIP(hipMemcpyAsync(amp, v, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    fill_zero(st, amp + N, N);
+    double* g = mc->grid.as<double>();
+    conv_sources(pl, GK_PRIMAL, mc->src1, amp, g);
+    gather_op(pl, OP_E_PRIMAL, g, out);
+    k_scale<<<(N + 255) / 256, 256, 0, st>>>(out, out, -1.0, N);
+}
+
+// reference factorisation at the current geometry: Pinv = S^-1, row-major
+static void moving_reference(ilh_plan* pl, MovingCache* mc)
+{
+    const int N = pl->N;
+    hipStream_t st = pl->ctx->stream;
+    DevBuf LU, piv, info, Sinv;
+    LU.ensure(sizeof(double) * (size_t)N * N);
+    piv.ensure(sizeof(int) * (size_t)N);
+    info.ensure(sizeof(int));
+    build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
+    const int bad = lu_factor(pl, LU.as<double>(), N, piv.as<int>(), info.as<int>());
+    if (bad) fail(ILH_ERR_SINGULAR, "Schur complement is singular (zero pivot at column " + std::to_string(bad) + ")");
+    Sinv.ensure(sizeof(double) * (size_t)N * N);
+    fill_zero(st, Sinv.as<double>(), (size_t)N * N);
+    k_identity<<<(N + 255) / 256, 256, 0, st>>>(Sinv.as<double>(), N);
+    lu_solve_multi(pl, LU.as<double>(), piv.as<int>(), N, Sinv.as<double>(), N);
+    const int ld = solve_ld(N);
+    mc->Pinv.ensure(sizeof(double) * (size_t)N * ld);
+    transpose(st, Sinv.as<double>(), mc->Pinv.as<double>(), N, N, ld);
+    ILH_HIP(hipStreamSynchronize(st));
+    mc->have_ref = true;
+    mc->ref_age = 0;
+}
+
+// GMRES(m) on S x = b, right-preconditioned by Pinv, classical Gram-Schmidt
+// with one re-orthogonalisation; the small Hessenberg problem (Givens
+// rotations) runs on the host.  x0 = Pinv b.  Returns the iteration count, or
+// -1 if the relative residual did not fall below tol within m iterations.
+static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* x, double tol, int m)
+{
+    const int N = pl->N, ld = solve_ld(N);
+    hipStream_t st = pl->ctx->stream;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    double* V = mc->V.as<double>();
+    double* w = mc->w.as<double>();
+    double* z = mc->z.as<double>();
+    double* hd = mc->h.as<double>();
+    // x0 = P b, r0 = b - S x0
+    matvec_rows(st, mc->Pinv.as<double>(), ld, b, x, N, N, 1.0, 0);
+    schur_apply(pl, mc, x, w);
+    axpby(st, w, b, 1.0, -1.0, N);                     // w = b - S x0
+    double hb[2];
+    k_mdot<<<1, 256, 0, st>>>(w, 1, w, N, hd);
+    k_mdot<<<1, 256, 0, st>>>(b, 1, b, N, hd + 1);
+    ILH_HIP(hipMemcpyAsync(hb, hd, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ILH_HIP(hipStreamSynchronize(st));
+    const double beta = std::sqrt(hb[0]), bnorm = std::sqrt(hb[1]);
+    if (bnorm == 0.0 || beta <= tol * bnorm) return 0;
+    k_scale<<<gN, 256, 0, st>>>(V, w, 1.0 / beta, N);
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), hh(3 * (m + 1));
+    g[0] = beta;
+    int j = 0;
+    bool ok = false;
+    for (; j < m; ++j) {
+        matvec_rows(st, mc->Pinv.as<double>(), ld, V + (size_t)N * j, z, N, N, 1.0, 0);
+        schur_apply(pl, mc, z, w);
+        // two passes of classical Gram-Schmidt against v_0..v_j, then |w|^2;
+        // one read-back of all three
+        k_mdot<<<j + 1, 256, 0, st>>>(V, j + 1, w, N, hd);
+        k_mupdate<<<gN, 256, 0, st>>>(V, j + 1, hd, N, w);
+        k_mdot<<<j + 1, 256, 0, st>>>(V, j + 1, w, N, hd + (m + 1));
+        k_mupdate<<<gN, 256, 0, st>>>(V, j + 1, hd + (m + 1), N, w);
+        k_mdot<<<1, 256, 0, st>>>(w, 1, w, N, hd + 2 * (m + 1));
+        ILH_HIP(hipMemcpyAsync(hh.data(), hd, sizeof(double) * 3 * (m + 1), hipMemcpyDeviceToHost, st));
+        ILH_HIP(hipStreamSynchronize(st));
+        const double hn = std::sqrt(hh[2 * (m + 1)]);
+        for (int i = 0; i <= j; ++i) H[i + (size_t)(m + 1) * j] = hh[i] + hh[(m + 1) + i];
+        H[j + 1 + (size_t)(m + 1) * j] = hn;
+        // apply the previous rotations, then a new one zeroing H[j+1, j]
+        for (int i = 0; i < j; ++i) {
+            double& a = H[i + (size_t)(m + 1) * j];
+            double& c = H[i + 1 + (size_t)(m + 1) * j];
+            const double t = cs[i] * a + sn[i] * c;
+            c = -sn[i] * a + cs[i] * c;
+            a = t;
+        }
+        double& a = H[j + (size_t)(m + 1) * j];
+        double& c = H[j + 1 + (size_t)(m + 1) * j];
+        const double r = std::hypot(a, c);
+        cs[j] = r == 0.0 ? 1.0 : a / r;
+        sn[j] = r == 0.0 ? 0.0 : c / r;
+        a = r;
+        c = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        if (hn > 0.0) k_scale<<<gN, 256, 0, st>>>(V + (size_t)N * (j + 1), w, 1.0 / hn, N);
+        if (std::fabs(g[j + 1]) <= tol * bnorm || hn == 0.0) {
+            ok = true;
+            ++j;
+            break;
+        }
+    }
+    if (!ok) return -1;
+    // y = H^-1 g (upper triangular j x j); x += P (V y)
+    std::vector<double> y(j, 0.0);
+    for (int i = j - 1; i >= 0; --i) {
+        double t = g[i];
+        for (int k = i + 1; k < j; ++k) t -= H[i + (size_t)(m + 1) * k] * y[k];
+        y[i] = t / H[i + (size_t)(m + 1) * i];
+    }
+    for (int i = 0; i < j; ++i) hh[i] = -y[i];
+    ILH_HIP(hipMemcpyAsync(hd, hh.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
+    fill_zero(st, w, N);
+    k_mupdate<<<gN, 256, 0, st>>>(V, j, hd, N, w);     // w = V y
+    matvec_rows(st, mc->Pinv.as<double>(), ld, w, x, N, N, 1.0, 1);
+    ILH_HIP(hipStreamSynchronize(st));
+    return j;
+}
+
+extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s,
+                                          int mem, double tol, int max_iter, int* iters)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        if (!(tol > 0.0) || tol >= 1.0) fail(ILH_ERR_ARG, "tol must be in (0, 1)");
+        if (max_iter < 1 || max_iter > 200) fail(ILH_ERR_ARG, "max_iter must be in [1, 200]");
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        hipStream_t st = pl->ctx->stream;
+        const double* d_fp = io.in(fp, N, pl->io_a);
+        const double* d_fm = fm ? io.in(fm, N, pl->io_c) : nullptr;
+        double* d_f = io.out(f, ng, pl->io_b);
+        double* d_s = io.out(s, N, pl->io_d);
+        int it = 0;
+        if (N == 0) {
+            fill_zero(st, d_f, ng);
+        } else {
+            need_margin(pl, "Dirichlet solve");
+            if (!pl->mov) pl->mov = new MovingCache();
+            MovingCache* mc = pl->mov;
+            if (!mc->src_ok) {
+                merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, mc->terms1, mc->src1);
+                mc->src_ok = true;
+            }
+            const int m = max_iter;
+            mc->amp.ensure(sizeof(double) * 2 * (size_t)N);
+            mc->V.ensure(sizeof(double) * (size_t)N * (m + 1));
+            for (DevBuf* bb : {&mc->w, &mc->z, &mc->x, &mc->b, &mc->fbar, &mc->dvec})
+                bb->ensure(sizeof(double) * (size_t)N);
+            mc->h.ensure(sizeof(double) * 3 * (size_t)(m + 1));
+            mc->grid.ensure(sizeof(double) * ng);
+            if (!mc->have_ref) moving_reference(pl, mc);
+            // f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
+            double* amp = mc->amp.as<double>();
+            k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, mc->dvec.as<double>(), mc->fbar.as<double>());
+            fill_zero(st, amp, N);
+            ILH_HIP(hipMemcpyAsync(amp + N, mc->dvec.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            conv_sources(pl, GK_PRIMAL, mc->src1, amp, mc->grid.as<double>());
+            gather_op(pl, OP_E_PRIMAL, mc->grid.as<double>(), mc->b.as<double>());
+            axpby(st, mc->b.as<double>(), mc->fbar.as<double>(), -1.0, 1.0, N);
+            it = moving_gmres(pl, mc, mc->b.as<double>(), mc->x.as<double>(), tol, m);
+            if (it < 0) {
+                // the reference geometry no longer preconditions well: refactor here
+                moving_reference(pl, mc);
+                const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), mc->x.as<double>(), tol, m);
+                if (it2 < 0) fail(ILH_ERR_SINGULAR, "GMRES did not converge with the current geometry's factor");
+                it = m + it2;
+            }
+            ++mc->ref_age;
+            // f = L^-1 (R s + D_s d)
+            ILH_HIP(hipMemcpyAsync(amp, mc->x.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            ILH_HIP(hipMemcpyAsync(amp + N, mc->dvec.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            conv_sources(pl, GK_PRIMAL, mc->src1, amp, d_f);
+            ILH_HIP(hipMemcpyAsync(d_s, mc->x.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+        }
+        if (iters) *iters = it;
+        io.back(f, d_f, ng);
+        io.back(s, d_s, N);
         io.done();
     });
 }

@@ -128,6 +128,7 @@ struct ilh_ctx {
 
 struct SolveCache;   // Schur factor and solve matrices of one composite
 
+struct MovingCache;
 struct ilh_plan {
     ilh_ctx* ctx = nullptr;
     ilh::Grid g{};
@@ -149,6 +150,7 @@ struct ilh_plan {
     SolveCache* dir = nullptr;                // Dirichlet
     SolveCache* neu = nullptr;                // Neumann
     SolveCache* fac[4] = {};                  // ilh_schur_factor caches per kind
+    MovingCache* mov = nullptr;               // moving-body Dirichlet solves (kept across update_points)
 };
 
 namespace ilh {

@@ -402,6 +402,27 @@ class DirichletPoisson:
                _ptr(s.flat()), L.MEM_HOST)
         return f, s
 
+    def solve_moving(self, fbplus, fbminus=None, tol=1e-12, max_iter=30):
+        """The same problem after cache.update_points (moving bodies): GMRES on
+        the current Schur system preconditioned by a reference geometry's
+        inverse.  Returns (f, s, gmres_iterations)."""
+        c = self.cache
+        fp = _vec(fbplus, c.N)
+        fm = None if fbminus is None else _vec(fbminus, c.N)
+        f = Nodes(Primal, c.g)
+        s = ScalarData(c.N)
+        it = C.c_int()
+        L.call("ilh_dirichlet_solve_moving", c.h, _ptr(fp), None if fm is None else _ptr(fm), _ptr(f.flat()),
+               _ptr(s.flat()), L.MEM_HOST, C.c_double(tol), int(max_iter), C.byref(it))
+        return f, s, it.value
+
+    def solve_moving_device(self, fp_dev, fm_dev, f_dev, s_dev, tol=1e-12, max_iter=30):
+        """Device-buffer form of solve_moving; returns the GMRES iterations."""
+        it = C.c_int()
+        L.call("ilh_dirichlet_solve_moving", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,
+               f_dev.ptr, s_dev.ptr, L.MEM_DEVICE, C.c_double(tol), int(max_iter), C.byref(it))
+        return it.value
+
     def solve_device(self, fp_dev, fm_dev, f_dev, s_dev):
         """Device-resident form: all four arguments are DeviceArrays (fm may be None); asynchronous."""
         L.call("ilh_dirichlet_solve", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,

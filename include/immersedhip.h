@@ -160,6 +160,18 @@ int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_mi
                       double* f, double* df, double* s, double* ds, int mem);
 
 /* ---- measurement helpers (no reference counterpart) ---- */
+/* Dirichlet solve for moving bodies (problems.jl:259-298 after every
+   update_system!, src/system.jl:26-50): same outputs as ilh_dirichlet_solve,
+   but the Schur system of the current geometry is solved by GMRES (at most
+   max_iter iterations, relative residual tol), right-preconditioned by the
+   inverse Schur matrix of a reference geometry that the plan keeps across
+   ilh_plan_update_points; S is applied matrix-free (one LGF convolution per
+   iteration).  The first call, and any call whose GMRES does not converge,
+   factors the current geometry and makes it the reference.  *iters (may be
+   NULL) receives the GMRES iterations (max_iter + k after a refactorisation). */
+int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
+                               double* s, int mem, double tol, int max_iter, int* iters);
+
 /* HIP-event timings on the plan's stream, in ms per call, of the Dirichlet
    repeat solve on device buffers: *total_ms = `reps` whole solves back to back
    (two events around the loop); if stage_ms is not NULL, stage_ms[k] = `reps`
