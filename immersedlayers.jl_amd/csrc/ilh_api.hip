@@ -149,6 +149,13 @@ extern "C" int ilh_ctx_destroy(ilh_ctx* c)
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        for (int k = 0; k < ilh_ctx::POOL; ++k)
+            if (c->pool[k]) {
+                (void)hipStreamSynchronize(c->pool[k]);
+                (void)hipStreamDestroy(c->pool[k]);
+            }
+        for (hipEvent_t e : c->ev)
+            if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(c->stream);
         delete c;
     });
@@ -811,6 +818,74 @@ extern "C" int ilh_neumann_solve(ilh_plan* pl, const double* vp, const double* v
         io.back(s, d_s, nd);
         io.back(dsv, d_ds, N);
         io.done();
+    });
+}
+
+// ------------------------------------------------------ batched solves
+// Repeat Dirichlet solves of many independent plans of one context: the
+// plans' caches are prepared first (in order, on the context stream), then
+// their solves are issued round-robin on POOL side streams that wait for the
+// context stream's prior work, and the context stream waits for all of them.
+// Kernels of different problems overlap on the GPU.
+extern "C" int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
+                                           const double* const* fb_minus, double* const* f, double* const* s,
+                                           int mem)
+{
+    return guarded([&] {
+        if (nprob < 0) fail(ILH_ERR_ARG, "negative problem count");
+        if (nprob == 0) return;
+        if (!plans || !fb_plus || !f || !s) fail(ILH_ERR_ARG, "null array of pointers");
+        if (mem != ILH_MEM_HOST && mem != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
+        ilh_ctx* ctx = nullptr;
+        for (int i = 0; i < nprob; ++i) {
+            if (!plans[i]) fail(ILH_ERR_ARG, "null plan in batch");
+            if (!ctx) ctx = plans[i]->ctx;
+            if (plans[i]->ctx != ctx) fail(ILH_ERR_ARG, "all plans of a batch must share one context");
+            for (int k = 0; k < i; ++k)
+                if (plans[k] == plans[i]) fail(ILH_ERR_ARG, "a plan appears twice in one batch");
+        }
+        ILH_HIP(hipSetDevice(ctx->device));
+        for (int k = 0; k < ilh_ctx::POOL; ++k)
+            if (!ctx->pool[k]) ILH_HIP(hipStreamCreateWithFlags(&ctx->pool[k], hipStreamNonBlocking));
+        for (hipEvent_t& e : ctx->ev)
+            if (!e) ILH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        std::vector<SolveCache*> sc(nprob);
+        for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i]);
+        const hipStream_t main = ctx->stream;
+        ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
+        for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamWaitEvent(ctx->pool[k], ctx->ev[ilh_ctx::POOL], 0));
+        struct Restore {
+            ilh_ctx* c;
+            hipStream_t m;
+            ~Restore() { c->stream = m; }
+        } restore{ctx, main};
+        for (int i = 0; i < nprob; ++i) {
+            ilh_plan* pl = plans[i];
+            ctx->stream = ctx->pool[i % ilh_ctx::POOL];
+            Io io(pl, mem);
+            const int N = pl->N;
+            const size_t ng = pl->g.n(GK_PRIMAL);
+            const double* d_fp = io.in(fb_plus[i], N, pl->io_a);
+            const double* fm_i = fb_minus ? fb_minus[i] : nullptr;
+            const double* d_fm = fm_i ? io.in(fm_i, N, pl->io_c) : nullptr;
+            double* d_f = io.out(f[i], ng, pl->io_b);
+            double* d_s = io.out(s[i], N, pl->io_d);
+            if (N == 0) {
+                fill_zero(ctx->stream, d_f, ng);
+            } else {
+                dirichlet_amp(ctx->stream, sc[i]->M[0].as<double>(), sc[i]->M[1].as<double>(), solve_ld(N), d_fp, d_fm,
+                              sc[i]->amp1.as<double>(), d_s, N);
+                conv_sources(pl, GK_PRIMAL, sc[i]->src1, sc[i]->amp1.as<double>(), d_f);
+            }
+            io.back(f[i], d_f, ng);
+            io.back(s[i], d_s, N);
+        }
+        ctx->stream = main;
+        for (int k = 0; k < ilh_ctx::POOL; ++k) {
+            ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
+            ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
+        }
+        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(main));
     });
 }
 

@@ -124,6 +124,9 @@ struct ilh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 0;
+    static constexpr int POOL = 4;            // side streams of batched calls (one per hardware queue)
+    hipStream_t pool[POOL] = {};
+    hipEvent_t ev[POOL + 1] = {};
 };
 
 struct SolveCache;   // Schur factor and solve matrices of one composite

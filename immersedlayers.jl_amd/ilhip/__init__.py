@@ -14,7 +14,8 @@ from .core import (Context, DeviceArray, Nodes, Edges, ScalarData, VectorData, S
                    surface_divergence, surface_grad, surface_curl, surface_curl_cross, surface_grad_cross,
                    divergence, grad, curl, laplacian, inverse_laplacian,
                    create_RTLinvR, create_CLinvCT, create_GLinvD, create_nRTRn,
-                   DirichletPoisson, NeumannPoisson, SchurFactor, default_context)
+                   DirichletPoisson, NeumannPoisson, SchurFactor, default_context,
+                   dirichlet_solve_batched, dirichlet_solve_batched_device)
 from .geometry import GridSpec as PhysicalGrid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -438,6 +438,31 @@ class DirichletPoisson:
         return (list(st) if stages else None), tot.value
 
 
+def dirichlet_solve_batched(problems, fbplus, fbminus=None):
+    """Batched form of DirichletPoisson.solve over independent caches of one
+    context (ilh_dirichlet_solve_batched): returns [(f, s), ...]."""
+    n = len(problems)
+    caches = [p.cache if isinstance(p, DirichletPoisson) else p for p in problems]
+    fps = [_vec(fbplus[k], caches[k].N) for k in range(n)]
+    fms = None if fbminus is None else [None if fbminus[k] is None else _vec(fbminus[k], caches[k].N)
+                                        for k in range(n)]
+    fs = [Nodes(Primal, c.g) for c in caches]
+    ss = [ScalarData(c.N) for c in caches]
+    arr = lambda xs: (C.c_void_p * n)(*[None if x is None else x.ctypes.data for x in xs])
+    plans = (C.c_void_p * n)(*[c.h.value for c in caches])
+    L.call("ilh_dirichlet_solve_batched", n, plans, arr(fps), None if fms is None else arr(fms),
+           arr([f.flat() for f in fs]), arr([s.flat() for s in ss]), L.MEM_HOST)
+    return list(zip(fs, ss))
+
+
+def dirichlet_solve_batched_device(caches, fp_devs, f_devs, s_devs):
+    """Device-buffer form (DeviceArrays, asynchronous on the context stream)."""
+    n = len(caches)
+    ptrs = lambda xs: (C.c_void_p * n)(*[x.ptr.value for x in xs])
+    plans = (C.c_void_p * n)(*[c.h.value for c in caches])
+    L.call("ilh_dirichlet_solve_batched", n, plans, ptrs(fp_devs), None, ptrs(f_devs), ptrs(s_devs), L.MEM_DEVICE)
+
+
 class NeumannPoisson:
     """The Neumann Poisson problem of test/literate/neumann.jl:80-142: returns
     (f, [ϕ], s, [ψ]) with S = create_CLinvCT factored once per geometry."""

@@ -151,6 +151,48 @@ def config_microbench(seed=2, npts_per_curve=2048):
     return g, concat(surfs)
 
 
+def config_batched(seed=3, nprob=64, sizes=(256, 384, 512, 768, 1024), dx=0.01, spacing=1.4):
+    """Independent Dirichlet problems for batched solves: Nodes{Primal} n² with
+    n drawn from `sizes`, dx fixed (the domain is n·dx), 1-3 bodies (circle,
+    ellipse of aspect 1-4, star) placed with >= 10 dx clearance from each other
+    and the grid edge, 200-3000 points in total at `spacing`·dx (fewer where
+    the target does not fit).  Returns a list of (grid, surface, f_b)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nprob):
+        n = int(rng.choice(sizes))
+        g = GridSpec(n + 1, n + 1, n // 2 + 1, n // 2 + 1, dx)
+        half = 0.5 * n * dx
+        clear = 10 * dx
+        nb = int(rng.integers(1, 4))
+        target = rng.uniform(200, 3000)
+        # body radius from its share of the points (perimeter ~ 2 pi r)
+        r_share = target * spacing * dx / (2 * np.pi * nb)
+        bodies, placed = [], []
+        for b in range(nb):
+            kind = int(rng.integers(0, 3))
+            r = min(r_share, 0.45 * (half - clear))
+            for _try in range(200):
+                c = rng.uniform(-half + clear + 1.1 * r, half - clear - 1.1 * r, 2)
+                if all(np.hypot(*(c - pc)) > 1.1 * (r + pr) + clear for pc, pr in placed):
+                    break
+                r *= 0.97
+            else:
+                continue
+            placed.append((c, r))
+            if kind == 0:
+                bodies.append(circle(r, spacing * dx, c[0], c[1]))
+            elif kind == 1:
+                asp = rng.uniform(1, 4)
+                bodies.append(ellipse(r, r / asp, spacing * dx, c[0], c[1]))
+            else:
+                bodies.append(star(r / 1.1, rng.uniform(-0.1, 0.1, 5), spacing * dx, c[0], c[1]))
+        surf = concat(bodies)
+        fb, _, _ = fourier_data(np.arctan2(surf.Y, surf.X), rng)
+        out.append((g, surf, fb))
+    return out
+
+
 def config_headline(seed=5, spacing=1.2):
     """1024² Nodes{Primal} on [-2,2]² (Δx=4/1024): an ellipse of aspect 2 and a
     random star curve, about 1000 points each at ``spacing``·Δx."""

@@ -160,6 +160,16 @@ int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_mi
                       double* f, double* df, double* s, double* ds, int mem);
 
 /* ---- measurement helpers (no reference counterpart) ---- */
+/* Batched form of ilh_dirichlet_solve: nprob independent plans of one context
+   (any mix of grid sizes and point counts), array k of each pointer array
+   belonging to plans[k]; fb_minus may be NULL, or hold NULL entries.  The
+   problems' solves run concurrently on the context's side streams; with
+   ILH_MEM_DEVICE the call is asynchronous on the context stream like the
+   single-problem form.  (The reference solves problems one at a time; this
+   is the batched form of test/literate/problems.jl:259-298.) */
+int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
+                                const double* const* fb_minus, double* const* f, double* const* s, int mem);
+
 /* Dirichlet solve for moving bodies (problems.jl:259-298 after every
    update_system!, src/system.jl:26-50): same outputs as ilh_dirichlet_solve,
    but the Schur system of the current geometry is solved by GMRES (at most

@@ -167,3 +167,24 @@ def test_moving_body_gmres(ctx):
     f, sv, it = prob.solve_moving(fb, fm, tol=1e-13, max_iter=1)
     fr, sr = O.dirichlet_solve(og, O.Geometry(sk.X, sk.Y, sk.nx, sk.ny, sk.ds), fb, fm)
     assert it >= 1 and rel_err(f.flat(), fr) < 1e-10 and rel_err(sv.flat(), sr) < 1e-10
+
+
+def test_batched_solves(ctx):
+    """ilh_dirichlet_solve_batched: every problem of a mixed batch equals its
+    own single solve bit for bit, and the oracle to the FP64 bar."""
+    import ilhip
+    from ilhip import geometry as G
+    probs = G.config_batched(seed=9, nprob=7, sizes=(64, 96, 128), dx=0.02)
+    caches = [ilhip.SurfaceScalarCache(s, g, ctx=ctx) for g, s, _ in probs]
+    print("batched N", [c.N for c in caches], flush=True)
+    rng = np.random.default_rng(5)
+    fms = [None if k % 2 else 0.2 * rng.standard_normal(c.N) for k, c in enumerate(caches)]
+    out = ilhip.dirichlet_solve_batched(caches, [fb for _, _, fb in probs], fms)
+    for k, ((g, s, fb), c) in enumerate(zip(probs, caches)):
+        f1, s1 = ilhip.DirichletPoisson(c).solve(fb, fms[k])
+        assert np.array_equal(out[k][0].flat(), f1.flat()) and np.array_equal(out[k][1].flat(), s1.flat())
+        og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+        fr, sr = O.dirichlet_solve(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), fb, fms[k])
+        assert rel_err(f1.flat(), fr) < 1e-10 and rel_err(s1.flat(), sr) < 1e-10
+    with pytest.raises(ilhip.ArgumentError):
+        ilhip.dirichlet_solve_batched([caches[0], caches[0]], [probs[0][2], probs[0][2]])
