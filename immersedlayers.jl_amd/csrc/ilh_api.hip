@@ -106,6 +106,7 @@ struct MovingCache {
 
 static void drop_caches(ilh_plan* pl)
 {
+    ++pl->gen;
     if (pl->mov) pl->mov->drop_sources();
     delete pl->dir;
     delete pl->neu;
@@ -827,65 +828,168 @@ extern "C" int ilh_neumann_solve(ilh_plan* pl, const double* vp, const double* v
 // their solves are issued round-robin on POOL side streams that wait for the
 // context stream's prior work, and the context stream waits for all of them.
 // Kernels of different problems overlap on the GPU.
+namespace {
+
+ilh_ctx* batch_ctx(int nprob, ilh_plan* const* plans)
+{
+    if (nprob <= 0) fail(ILH_ERR_ARG, "batch needs at least one problem");
+    if (!plans) fail(ILH_ERR_ARG, "null array of plans");
+    ilh_ctx* ctx = nullptr;
+    for (int i = 0; i < nprob; ++i) {
+        if (!plans[i]) fail(ILH_ERR_ARG, "null plan in batch");
+        if (!ctx) ctx = plans[i]->ctx;
+        if (plans[i]->ctx != ctx) fail(ILH_ERR_ARG, "all plans of a batch must share one context");
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == plans[i]) fail(ILH_ERR_ARG, "a plan appears twice in one batch");
+    }
+    ILH_HIP(hipSetDevice(ctx->device));
+    for (int k = 0; k < ilh_ctx::POOL; ++k)
+        if (!ctx->pool[k]) ILH_HIP(hipStreamCreateWithFlags(&ctx->pool[k], hipStreamNonBlocking));
+    for (hipEvent_t& e : ctx->ev)
+        if (!e) ILH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return ctx;
+}
+
+// Issue the batch's solves round-robin on the side streams, forked from and
+// joined back into the context stream.  Caches are prepared by the caller.
+void batch_issue(ilh_ctx* ctx, int nprob, ilh_plan* const* plans, SolveCache* const* sc,
+                 const double* const* fb_plus, const double* const* fb_minus, double* const* f, double* const* s,
+                 int mem)
+{
+    const hipStream_t main = ctx->stream;
+    ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
+    for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamWaitEvent(ctx->pool[k], ctx->ev[ilh_ctx::POOL], 0));
+    struct Restore {
+        ilh_ctx* c;
+        hipStream_t m;
+        ~Restore() { c->stream = m; }
+    } restore{ctx, main};
+    for (int i = 0; i < nprob; ++i) {
+        ilh_plan* pl = plans[i];
+        ctx->stream = ctx->pool[i % ilh_ctx::POOL];
+        Io io(pl, mem);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        const double* d_fp = io.in(fb_plus[i], N, pl->io_a);
+        const double* fm_i = fb_minus ? fb_minus[i] : nullptr;
+        const double* d_fm = fm_i ? io.in(fm_i, N, pl->io_c) : nullptr;
+        double* d_f = io.out(f[i], ng, pl->io_b);
+        double* d_s = io.out(s[i], N, pl->io_d);
+        if (N == 0) {
+            fill_zero(ctx->stream, d_f, ng);
+        } else {
+            dirichlet_amp(ctx->stream, sc[i]->M[0].as<double>(), sc[i]->M[1].as<double>(), solve_ld(N), d_fp, d_fm,
+                          sc[i]->amp1.as<double>(), d_s, N);
+            conv_sources(pl, GK_PRIMAL, sc[i]->src1, sc[i]->amp1.as<double>(), d_f);
+        }
+        io.back(f[i], d_f, ng);
+        io.back(s[i], d_s, N);
+    }
+    ctx->stream = main;
+    for (int k = 0; k < ilh_ctx::POOL; ++k) {
+        ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
+        ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------ batched solves
+// Repeat Dirichlet solves of many independent plans of one context: the
+// plans' caches are prepared first (in order, on the context stream), then
+// their solves are issued round-robin on POOL side streams that wait for the
+// context stream's prior work, and the context stream waits for all of them.
+// Kernels of different problems overlap on the GPU.
 extern "C" int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
                                            const double* const* fb_minus, double* const* f, double* const* s,
                                            int mem)
 {
     return guarded([&] {
-        if (nprob < 0) fail(ILH_ERR_ARG, "negative problem count");
         if (nprob == 0) return;
-        if (!plans || !fb_plus || !f || !s) fail(ILH_ERR_ARG, "null array of pointers");
+        if (nprob < 0) fail(ILH_ERR_ARG, "negative problem count");
+        if (!fb_plus || !f || !s) fail(ILH_ERR_ARG, "null array of pointers");
         if (mem != ILH_MEM_HOST && mem != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
-        ilh_ctx* ctx = nullptr;
-        for (int i = 0; i < nprob; ++i) {
-            if (!plans[i]) fail(ILH_ERR_ARG, "null plan in batch");
-            if (!ctx) ctx = plans[i]->ctx;
-            if (plans[i]->ctx != ctx) fail(ILH_ERR_ARG, "all plans of a batch must share one context");
-            for (int k = 0; k < i; ++k)
-                if (plans[k] == plans[i]) fail(ILH_ERR_ARG, "a plan appears twice in one batch");
-        }
-        ILH_HIP(hipSetDevice(ctx->device));
-        for (int k = 0; k < ilh_ctx::POOL; ++k)
-            if (!ctx->pool[k]) ILH_HIP(hipStreamCreateWithFlags(&ctx->pool[k], hipStreamNonBlocking));
-        for (hipEvent_t& e : ctx->ev)
-            if (!e) ILH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ilh_ctx* ctx = batch_ctx(nprob, plans);
         std::vector<SolveCache*> sc(nprob);
         for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i]);
-        const hipStream_t main = ctx->stream;
-        ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
-        for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamWaitEvent(ctx->pool[k], ctx->ev[ilh_ctx::POOL], 0));
-        struct Restore {
-            ilh_ctx* c;
-            hipStream_t m;
-            ~Restore() { c->stream = m; }
-        } restore{ctx, main};
-        for (int i = 0; i < nprob; ++i) {
-            ilh_plan* pl = plans[i];
-            ctx->stream = ctx->pool[i % ilh_ctx::POOL];
-            Io io(pl, mem);
-            const int N = pl->N;
-            const size_t ng = pl->g.n(GK_PRIMAL);
-            const double* d_fp = io.in(fb_plus[i], N, pl->io_a);
-            const double* fm_i = fb_minus ? fb_minus[i] : nullptr;
-            const double* d_fm = fm_i ? io.in(fm_i, N, pl->io_c) : nullptr;
-            double* d_f = io.out(f[i], ng, pl->io_b);
-            double* d_s = io.out(s[i], N, pl->io_d);
-            if (N == 0) {
-                fill_zero(ctx->stream, d_f, ng);
-            } else {
-                dirichlet_amp(ctx->stream, sc[i]->M[0].as<double>(), sc[i]->M[1].as<double>(), solve_ld(N), d_fp, d_fm,
-                              sc[i]->amp1.as<double>(), d_s, N);
-                conv_sources(pl, GK_PRIMAL, sc[i]->src1, sc[i]->amp1.as<double>(), d_f);
-            }
-            io.back(f[i], d_f, ng);
-            io.back(s[i], d_s, N);
+        batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, mem);
+        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+// A batch captured once as a HIP graph (device buffers fixed at capture) and
+// replayed with one launch per step.
+struct ilh_batch {
+    ilh_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<ilh_plan*> plans;
+    std::vector<unsigned> gens;   // plan generations the graph's buffers belong to
+};
+
+extern "C" int ilh_batch_create(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
+                                const double* const* fb_minus, double* const* f, double* const* s, ilh_batch** out)
+{
+    return guarded([&] {
+        if (!out) fail(ILH_ERR_ARG, "null output pointer");
+        if (!fb_plus || !f || !s) fail(ILH_ERR_ARG, "null array of pointers");
+        ilh_ctx* ctx = batch_ctx(nprob, plans);
+        std::vector<SolveCache*> sc(nprob);
+        for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i]);
+        // one uncaptured pass: every lazily built table and attribute exists
+        // before capture (capture must record kernels only)
+        batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, ILH_MEM_DEVICE);
+        ILH_HIP(hipStreamSynchronize(ctx->stream));
+        ilh_batch* b = new ilh_batch();
+        b->ctx = ctx;
+        b->plans.assign(plans, plans + nprob);
+        for (int i = 0; i < nprob; ++i) b->gens.push_back(plans[i]->gen);
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            delete b;
+            fail(ILH_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
         }
-        ctx->stream = main;
-        for (int k = 0; k < ilh_ctx::POOL; ++k) {
-            ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
-            ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
+        try {
+            batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, ILH_MEM_DEVICE);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            delete b;
+            throw;
         }
-        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(main));
+        e = hipStreamEndCapture(ctx->stream, &b->graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            delete b;
+            fail(ILH_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+        }
+        *out = b;
+    });
+}
+
+extern "C" int ilh_batch_launch(ilh_batch* b)
+{
+    return guarded([&] {
+        if (!b) fail(ILH_ERR_ARG, "null batch");
+        for (size_t i = 0; i < b->plans.size(); ++i)
+            if (b->plans[i]->gen != b->gens[i] || !b->plans[i]->dir)
+                fail(ILH_ERR_STATE, "a plan of the batch changed geometry since the batch was captured");
+        ILH_HIP(hipSetDevice(b->ctx->device));
+        ILH_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+    });
+}
+
+extern "C" int ilh_batch_destroy(ilh_batch* b)
+{
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        delete b;
     });
 }
 

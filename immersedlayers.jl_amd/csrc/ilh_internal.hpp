@@ -154,6 +154,7 @@ struct ilh_plan {
     SolveCache* neu = nullptr;                // Neumann
     SolveCache* fac[4] = {};                  // ilh_schur_factor caches per kind
     MovingCache* mov = nullptr;               // moving-body Dirichlet solves (kept across update_points)
+    unsigned gen = 0;                         // incremented whenever the geometry caches are dropped
 };
 
 namespace ilh {

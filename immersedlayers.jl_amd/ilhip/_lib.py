@@ -103,6 +103,9 @@ SIGNATURES = {
     "ilh_profile_dirichlet": ([_vp, _vp, _vp, _vp, _vp, _i, _dp, _dp], _i),
     "ilh_dirichlet_solve_moving": ([_vp, _vp, _vp, _vp, _vp, _i, C.c_double, _i, _vp], _i),
     "ilh_dirichlet_solve_batched": ([_i, _vp, _vp, _vp, _vp, _vp, _i], _i),
+    "ilh_batch_create": ([_i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "ilh_batch_launch": ([_vp], _i),
+    "ilh_batch_destroy": ([_vp], _i),
 }
 
 _lib = None

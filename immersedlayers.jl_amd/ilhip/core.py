@@ -463,6 +463,34 @@ def dirichlet_solve_batched_device(caches, fp_devs, f_devs, s_devs):
     L.call("ilh_dirichlet_solve_batched", n, plans, ptrs(fp_devs), None, ptrs(f_devs), ptrs(s_devs), L.MEM_DEVICE)
 
 
+class DirichletBatch:
+    """A batch of independent Dirichlet problems on fixed device buffers,
+    captured as one HIP graph and replayed with `launch()`."""
+
+    def __init__(self, caches, fp_devs, f_devs, s_devs, fm_devs=None):
+        n = len(caches)
+        ptrs = lambda xs: (C.c_void_p * n)(*[None if x is None else x.ptr.value for x in xs])
+        plans = (C.c_void_p * n)(*[c.h.value for c in caches])
+        h = C.c_void_p()
+        L.call("ilh_batch_create", n, plans, ptrs(fp_devs), None if fm_devs is None else ptrs(fm_devs),
+               ptrs(f_devs), ptrs(s_devs), C.byref(h))
+        self.h, self._keep = h, (caches, fp_devs, f_devs, s_devs, fm_devs)
+
+    def launch(self):
+        L.call("ilh_batch_launch", self.h)
+
+    def close(self):
+        if self.h:
+            L.load().ilh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class NeumannPoisson:
     """The Neumann Poisson problem of test/literate/neumann.jl:80-142: returns
     (f, [ϕ], s, [ψ]) with S = create_CLinvCT factored once per geometry."""

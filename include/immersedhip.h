@@ -170,6 +170,17 @@ int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_mi
 int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
                                 const double* const* fb_minus, double* const* f, double* const* s, int mem);
 
+/* Persistent batch: the batched solve above on fixed device buffers, captured
+   once as a HIP graph (ilh_batch_create prepares every plan, runs the batch
+   once and captures it) and replayed by ilh_batch_launch, asynchronously on
+   the context stream, one launch per step.  Inputs are read at each launch.
+   A plan whose geometry changes invalidates the batch (ILH_ERR_STATE). */
+typedef struct ilh_batch ilh_batch;
+int ilh_batch_create(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
+                     const double* const* fb_minus, double* const* f, double* const* s, ilh_batch** out);
+int ilh_batch_launch(ilh_batch* batch);
+int ilh_batch_destroy(ilh_batch* batch);
+
 /* Dirichlet solve for moving bodies (problems.jl:259-298 after every
    update_system!, src/system.jl:26-50): same outputs as ilh_dirichlet_solve,
    but the Schur system of the current geometry is solved by GMRES (at most

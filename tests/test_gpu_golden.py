@@ -188,3 +188,34 @@ def test_batched_solves(ctx):
         assert rel_err(f1.flat(), fr) < 1e-10 and rel_err(s1.flat(), sr) < 1e-10
     with pytest.raises(ilhip.ArgumentError):
         ilhip.dirichlet_solve_batched([caches[0], caches[0]], [probs[0][2], probs[0][2]])
+
+
+def test_batch_graph_replays(ctx):
+    """DirichletBatch: a captured batch replayed several times, with new inputs
+    and eager solves in between, matches the eager solves bit for bit; a
+    geometry change invalidates it."""
+    import ilhip
+    from ilhip import geometry as G
+    probs = G.config_batched(seed=4, nprob=5, sizes=(64, 96, 128), dx=0.02)
+    caches = [ilhip.SurfaceScalarCache(s, g, ctx=ctx) for g, s, _ in probs]
+    rng = np.random.default_rng(8)
+    fps = [ilhip.DeviceArray.from_host(ctx, fb) for _, _, fb in probs]
+    fds = [ilhip.DeviceArray(ctx, (g.NX - 1) * (g.NY - 1)) for g, _, _ in probs]
+    sds = [ilhip.DeviceArray(ctx, s.N) for _, s, _ in probs]
+    batch = ilhip.DirichletBatch(caches, fps, fds, sds)
+    for rep in range(3):
+        fbs = [rng.standard_normal(c.N) for c in caches]
+        for d, fb in zip(fps, fbs):
+            d.upload(fb)
+        batch.launch()
+        ctx.synchronize()
+        print("batch replay", rep, flush=True)
+        for k, c in enumerate(caches):
+            f1, s1 = ilhip.DirichletPoisson(c).solve(fbs[k])      # eager work between replays
+            assert np.array_equal(fds[k].download(), f1.flat())
+            assert np.array_equal(sds[k].download(), s1.flat())
+    g, s, fb = probs[0]
+    caches[0].update_points(G.Surface(s.X + 0.3 * g.dx, s.Y, s.nx, s.ny, s.ds, s.body))
+    with pytest.raises(ilhip.StateError):
+        batch.launch()
+    batch.close()
