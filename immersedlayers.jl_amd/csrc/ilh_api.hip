@@ -41,22 +41,12 @@ void DevBuf::release()
 }
 }  // namespace ilh
 
+void ilh::set_last_error(const std::string& msg) { g_err = msg; }
+
 template <class F>
 static int guarded(F&& f)
 {
-    try {
-        f();
-        return ILH_OK;
-    } catch (const Error& e) {
-        g_err = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_err = "host allocation failed";
-        return ILH_ERR_OOM;
-    } catch (...) {
-        g_err = "unknown error";
-        return ILH_ERR_ARG;
-    }
+    return guarded_call(static_cast<F&&>(f));
 }
 
 extern "C" const char* ilh_last_error(void) { return g_err.c_str(); }
@@ -294,41 +284,6 @@ extern "C" int ilh_plan_npts(const ilh_plan* pl) { return pl ? pl->N : -1; }
 
 // ------------------------------------------------------------- staging
 namespace {
-struct Io {
-    ilh_plan* pl;
-    int mem;
-    Io(ilh_plan* p, int m) : pl(p), mem(m)
-    {
-        if (!p) fail(ILH_ERR_ARG, "null plan");
-        if (m != ILH_MEM_HOST && m != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
-        ILH_HIP(hipSetDevice(p->ctx->device));
-    }
-    const double* in(const double* h, size_t n, DevBuf& buf)
-    {
-        if (n > 0 && !h) fail(ILH_ERR_ARG, "null input array");
-        if (mem == ILH_MEM_DEVICE) return h;
-        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
-        if (n) ILH_HIP(hipMemcpyAsync(buf.p, h, sizeof(double) * n, hipMemcpyHostToDevice, pl->ctx->stream));
-        return buf.as<double>();
-    }
-    double* out(double* h, size_t n, DevBuf& buf)
-    {
-        if (n > 0 && !h) fail(ILH_ERR_ARG, "null output array");
-        if (mem == ILH_MEM_DEVICE) return h;
-        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
-        return buf.as<double>();
-    }
-    void back(double* h, const double* d, size_t n)
-    {
-        if (mem == ILH_MEM_DEVICE || n == 0) return;
-        ILH_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, pl->ctx->stream));
-    }
-    void done()
-    {
-        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
-    }
-};
-
 size_t grid_len(const Grid& g, int kind)
 {
     switch (kind) {
@@ -338,6 +293,10 @@ size_t grid_len(const Grid& g, int kind)
     }
     fail(ILH_ERR_ARG, "unknown grid kind");
 }
+
+}  // namespace
+
+namespace ilh {
 
 void need_margin(ilh_plan* pl, const char* what)
 {
@@ -378,6 +337,10 @@ void gather_op(ilh_plan* pl, int op, const double* field, double* out)
     const double* fv = field + pl->g.n(GK_EDGE_U);
     gather(pl, t, op_terms_per_point(op), field, fu, fv, out);
 }
+
+}  // namespace ilh
+
+namespace {
 
 int kind_to_gk(int kind)
 {
@@ -1066,26 +1029,34 @@ static void schur_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* 
     k_scale<<<(N + 255) / 256, 256, 0, st>>>(out, out, -1.0, N);
 }
 
-// reference factorisation at the current geometry: Pinv = S^-1, row-major
-static void moving_reference(ilh_plan* pl, MovingCache* mc)
+// K (N x N column-major, overwritten by its LU factors) -> Minv = K^-1,
+// row-major with leading dimension solve_ld(N) (allocated by the caller)
+void ilh::inverse_rowmajor(ilh_plan* pl, double* K, double* Minv)
 {
     const int N = pl->N;
     hipStream_t st = pl->ctx->stream;
-    DevBuf LU, piv, info, Sinv;
-    LU.ensure(sizeof(double) * (size_t)N * N);
+    DevBuf piv, info, Sinv;
     piv.ensure(sizeof(int) * (size_t)N);
     info.ensure(sizeof(int));
-    build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
-    const int bad = lu_factor(pl, LU.as<double>(), N, piv.as<int>(), info.as<int>());
+    const int bad = lu_factor(pl, K, N, piv.as<int>(), info.as<int>());
     if (bad) fail(ILH_ERR_SINGULAR, "Schur complement is singular (zero pivot at column " + std::to_string(bad) + ")");
     Sinv.ensure(sizeof(double) * (size_t)N * N);
     fill_zero(st, Sinv.as<double>(), (size_t)N * N);
     k_identity<<<(N + 255) / 256, 256, 0, st>>>(Sinv.as<double>(), N);
-    lu_solve_multi(pl, LU.as<double>(), piv.as<int>(), N, Sinv.as<double>(), N);
-    const int ld = solve_ld(N);
-    mc->Pinv.ensure(sizeof(double) * (size_t)N * ld);
-    transpose(st, Sinv.as<double>(), mc->Pinv.as<double>(), N, N, ld);
+    lu_solve_multi(pl, K, piv.as<int>(), N, Sinv.as<double>(), N);
+    transpose(st, Sinv.as<double>(), Minv, N, N, solve_ld(N));
     ILH_HIP(hipStreamSynchronize(st));
+}
+
+// reference factorisation at the current geometry: Pinv = S^-1, row-major
+static void moving_reference(ilh_plan* pl, MovingCache* mc)
+{
+    const int N = pl->N;
+    DevBuf LU;
+    LU.ensure(sizeof(double) * (size_t)N * N);
+    build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
+    mc->Pinv.ensure(sizeof(double) * (size_t)N * solve_ld(N));
+    inverse_rowmajor(pl, LU.as<double>(), mc->Pinv.as<double>());
     mc->have_ref = true;
     mc->ref_age = 0;
 }

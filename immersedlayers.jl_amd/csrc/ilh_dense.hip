@@ -65,12 +65,15 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
                 if (u >= 0 && u < 9 && v >= 0 && v < 9) s = cx[u] * cy[v];
             } else {
                 for (int v = 0; v < 9; ++v) {
-                    if (cy[v] == 0.0) continue;
                     const int jj = abs(dby + v - 4);
+                    if (cy[v] == 0.0 || jj >= Tn) continue;   // kernel zero beyond the table
                     const double* Gr = G + (size_t)Tn * jj;
                     double r = 0.0;
 #pragma unroll
-                    for (int u = 0; u < 9; ++u) r += cx[u] * Gr[abs(dbx + u - 4)];
+                    for (int u = 0; u < 9; ++u) {
+                        const int ii = abs(dbx + u - 4);
+                        if (ii < Tn) r += cx[u] * Gr[ii];
+                    }
                     s += cy[v] * r;
                 }
             }
@@ -92,6 +95,20 @@ void assemble(ilh_plan* pl, int opA, int opB, int gk, double factor, bool delta_
     k_assemble<<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, op_terms_per_point(opA), B, op_terms_per_point(opB), N,
                                                            delta_kernel ? nullptr : pl->lgf.as<double>(), pl->lgf_mx,
                                                            factor, delta_kernel ? 1 : 0, K);
+    ILH_HIP(hipGetLastError());
+}
+
+// K[p,q] = factor * sum over term pairs of the kernel tab (Tn x Tn, even in
+// both axes, zero beyond the table) correlated with the two points' windows
+void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor, double* K)
+{
+    const int N = pl->N;
+    if (N == 0) return;
+    const Term* A = get_terms(pl, opA);
+    const Term* B = get_terms(pl, opB);
+    dim3 grid((N + AT - 1) / AT, (N + AT - 1) / AT);
+    k_assemble<<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, op_terms_per_point(opA), B, op_terms_per_point(opB), N,
+                                                           tab, Tn, factor, 0, K);
     ILH_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,311 @@
+// ilh_heat.hip -- unbounded heat conduction with Dirichlet bodies, advanced by
+// the integrating-factor half-explicit Runge-Kutta method (IF-HERK).
+//
+// The reference poses the problem as the constrained ODE system
+//   dT/dt = kappa L T + r1 + R sigma,   E T = r2
+// (test/literate/heatconduction.jl:88-130: r1 = surface_divergence!(-kappa [T]),
+// constraint force regularize!(-(-sigma)), bc_op interpolate!, bc_rhs the
+// surface average) and marches it with ConstrainedSystems' LiskaIFHERK
+// (src/timemarching.jl:5, integrating factor for lin_op = kappa L).  IF-HERK
+// applies a half-explicit RK scheme to v = exp(-(t - t_n) kappa L) T, so the
+// stiff diffusion is integrated exactly; the 3-stage tableau (Liska & Colonius
+// 2016) is
+//   c = (0, 1/3, 1),  a21 = 1/3,  a31 = -1, a32 = 2,  b = (0, 3/4, 1/4),
+// and the multiplier of each stage solves a Schur system
+//   E exp((c_i - c_{i-1}) dt kappa L) R sigma = E Yhat_i - r2
+// with the factors exp(dt/3 kappa L), exp(2dt/3 kappa L) and the identity.
+//
+// exp(tau kappa L) on the unbounded lattice is separable:
+//   [exp(tau kappa L) u](i,j) = sum_{p,q} K(p) K(q) u(i-p, j-q),
+//   K(p) = e^{-2a} I_p(2a),  a = tau kappa / dx^2,
+// applied here as an x then a y pass of a (2M+1)-tap filter (zero outside the
+// grid).  The three Schur matrices are assembled directly from K(p)K(q) like
+// the LGF ones, inverted once, and applied as dense matvecs.
+#include "ilh_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+struct ilh_heat {
+    ilh_plan* pl = nullptr;
+    unsigned gen = 0;
+    double kappa = 0.0, dt = 0.0;
+    int M13 = 0, M23 = 0;                 // filter half-widths
+    ilh::DevBuf k13, k23;                 // K(0..M) of exp(dt/3 kL), exp(2dt/3 kL)
+    ilh::DevBuf S13, S23, S0;             // row-major inverse Schur matrices
+    ilh::DevBuf F, A, Bt, Y, W1, W2, Rs;  // grid fields
+    ilh::DevBuf e, sg, r2;                // surface vectors
+    bool have_bc = false;
+};
+
+namespace ilh {
+
+// K(p) = e^{-x} I_p(x), x = 2a, p = 0..M, by backward recurrence normalised
+// with e^{x} = I_0(x) + 2 sum_{p>=1} I_p(x) (so sum over all p of K(p) = 1);
+// truncated after the last p with K(p) > 1e-17 K(0).
+static std::vector<double> heat_kernel_1d(double a)
+{
+    if (a <= 0.0) return {1.0};
+    const double x = 2.0 * a;
+    const int top = (int)(2.0 * x + 60.0 + 10.0 * std::sqrt(x));
+    std::vector<double> v(top + 2, 0.0);
+    v[top + 1] = 0.0;
+    v[top] = 1e-280;
+    for (int p = top; p >= 1; --p) {
+        v[p - 1] = (2.0 * p / x) * v[p] + v[p + 1];
+        if (v[p - 1] > 1e250)   // rescale to stay in range
+            for (int q = p - 1; q <= top + 1; ++q) v[q] *= 1e-250;
+    }
+    double sum = v[0];
+    for (int p = 1; p <= top; ++p) sum += 2.0 * v[p];
+    std::vector<double> K;
+    for (int p = 0; p <= top; ++p) {
+        const double k = v[p] / sum;
+        if (p > 0 && k <= 1e-17 * (v[0] / sum)) break;
+        K.push_back(k);
+    }
+    return K;
+}
+
+namespace {
+
+constexpr int HX = 64, HY = 4;
+
+// x pass: out(i,j) = sum_{|p|<=M} K(|p|) in(i-p, j), zero outside [0, nx)
+__global__ void k_heat_x(const double* __restrict__ in, int nx, int ny, const double* __restrict__ K, int M,
+                         double* __restrict__ out)
+{
+    const int i = blockIdx.x * HX + threadIdx.x, j = blockIdx.y * HY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const double* row = in + (size_t)nx * j;
+    double acc = K[0] * row[i];
+    for (int p = 1; p <= M; ++p) {
+        const double a = i - p >= 0 ? row[i - p] : 0.0;
+        const double b = i + p < nx ? row[i + p] : 0.0;
+        acc += K[p] * (a + b);
+    }
+    out[i + (size_t)nx * j] = acc;
+}
+
+// y pass, the same filter along j
+__global__ void k_heat_y(const double* __restrict__ in, int nx, int ny, const double* __restrict__ K, int M,
+                         double* __restrict__ out)
+{
+    const int i = blockIdx.x * HX + threadIdx.x, j = blockIdx.y * HY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    double acc = K[0] * in[i + (size_t)nx * j];
+    for (int p = 1; p <= M; ++p) {
+        const double a = j - p >= 0 ? in[i + (size_t)nx * (j - p)] : 0.0;
+        const double b = j + p < ny ? in[i + (size_t)nx * (j + p)] : 0.0;
+        acc += K[p] * (a + b);
+    }
+    out[i + (size_t)nx * j] = acc;
+}
+
+// out = x + c y
+__global__ void k_lin(double* __restrict__ out, const double* __restrict__ x, double c, const double* __restrict__ y,
+                      size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        out[k] = x[k] + c * y[k];
+}
+
+// d = Tb+ - Tb- (scaled by -kappa), r2 = (Tb+ + Tb-)/2
+__global__ void k_heat_bc(const double* __restrict__ tp, const double* __restrict__ tm, int n, double kappa,
+                          double* __restrict__ dneg, double* __restrict__ r2)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double a = tp[k], b = tm ? tm[k] : 0.0;
+    dneg[k] = -kappa * (a - b);
+    r2[k] = 0.5 * (a + b);
+}
+
+}  // namespace
+
+static void lin(hipStream_t st, double* out, const double* x, double c, const double* y, size_t n)
+{
+    size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
+    k_lin<<<(unsigned)blocks, 256, 0, st>>>(out, x, c, y, n);
+    ILH_HIP(hipGetLastError());
+}
+
+// out = exp(tau kappa L) in  (tmp: one grid field of scratch; out may alias in)
+static void heat_apply(ilh_plan* pl, const double* K, int M, const double* in, double* tmp, double* out)
+{
+    const int nx = pl->g.nx(GK_PRIMAL), ny = pl->g.ny(GK_PRIMAL);
+    dim3 grid((nx + HX - 1) / HX, (ny + HY - 1) / HY);
+    k_heat_x<<<grid, dim3(HX, HY), 0, pl->ctx->stream>>>(in, nx, ny, K, M, tmp);
+    k_heat_y<<<grid, dim3(HX, HY), 0, pl->ctx->stream>>>(tmp, nx, ny, K, M, out);
+    ILH_HIP(hipGetLastError());
+}
+
+// 2-D kernel table K(m)K(n), (M+1)^2, for the Schur assembly
+static void schur_inverse(ilh_plan* pl, const std::vector<double>& K, bool identity, DevBuf& inv)
+{
+    const int N = pl->N;
+    hipStream_t st = pl->ctx->stream;
+    DevBuf S;
+    S.ensure(sizeof(double) * (size_t)N * N);
+    if (identity) {
+        assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, 1.0, true, S.as<double>());
+    } else {
+        const int T = (int)K.size();
+        std::vector<double> tab((size_t)T * T);
+        for (int n = 0; n < T; ++n)
+            for (int m = 0; m < T; ++m) tab[m + (size_t)T * n] = K[m] * K[n];
+        DevBuf d;
+        d.ensure(sizeof(double) * tab.size());
+        ILH_HIP(hipMemcpyAsync(d.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st));
+        assemble_table(pl, OP_E_PRIMAL, OP_R_PRIMAL, d.as<double>(), T, 1.0, S.as<double>());
+        ILH_HIP(hipStreamSynchronize(st));
+    }
+    inv.ensure(sizeof(double) * (size_t)N * solve_ld(N));
+    inverse_rowmajor(pl, S.as<double>(), inv.as<double>());
+}
+
+}  // namespace ilh
+
+using namespace ilh;
+
+namespace {
+constexpr double A21 = 1.0 / 3.0, A31 = -1.0, A32 = 2.0, B2 = 0.75, B3 = 0.25;
+}
+
+extern "C" int ilh_heat_create(ilh_plan* pl, double kappa, double dt, ilh_heat** out)
+{
+    return guarded_call([&] {
+        if (!pl || !out) fail(ILH_ERR_ARG, "null argument");
+        if (!(kappa > 0.0) || !(dt > 0.0) || !std::isfinite(kappa * dt)) fail(ILH_ERR_ARG, "kappa and dt must be positive");
+        ILH_HIP(hipSetDevice(pl->ctx->device));
+        if (pl->N > 0) need_margin(pl, "heat conduction");
+        const double a13 = kappa * dt / 3.0 / (pl->g.dx * pl->g.dx);
+        const std::vector<double> K13 = heat_kernel_1d(a13), K23 = heat_kernel_1d(2.0 * a13);
+        if ((int)K23.size() > 256) fail(ILH_ERR_ARG, "time step too large for the integrating-factor filter (a > ~40)");
+        ilh_heat* h = new ilh_heat();
+        try {
+            h->pl = pl;
+            h->gen = pl->gen;
+            h->kappa = kappa;
+            h->dt = dt;
+            h->M13 = (int)K13.size() - 1;
+            h->M23 = (int)K23.size() - 1;
+            hipStream_t st = pl->ctx->stream;
+            h->k13.ensure(sizeof(double) * K13.size());
+            h->k23.ensure(sizeof(double) * K23.size());
+            ILH_HIP(hipMemcpyAsync(h->k13.p, K13.data(), sizeof(double) * K13.size(), hipMemcpyHostToDevice, st));
+            ILH_HIP(hipMemcpyAsync(h->k23.p, K23.data(), sizeof(double) * K23.size(), hipMemcpyHostToDevice, st));
+            const size_t ng = pl->g.n(GK_PRIMAL);
+            for (DevBuf* b : {&h->F, &h->A, &h->Bt, &h->Y, &h->W1, &h->W2, &h->Rs}) b->ensure(sizeof(double) * ng);
+            const int N = pl->N;
+            for (DevBuf* b : {&h->e, &h->sg, &h->r2}) b->ensure(sizeof(double) * std::max(1, N));
+            if (N > 0) {
+                schur_inverse(pl, K13, false, h->S13);
+                schur_inverse(pl, K23, false, h->S23);
+                schur_inverse(pl, {1.0}, true, h->S0);
+            }
+            ILH_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+extern "C" int ilh_heat_set_boundary(ilh_heat* h, const double* tb_plus, const double* tb_minus, int mem)
+{
+    return guarded_call([&] {
+        if (!h) fail(ILH_ERR_ARG, "null heat plan");
+        ilh_plan* pl = h->pl;
+        if (pl->gen != h->gen) fail(ILH_ERR_STATE, "the plan's geometry changed since ilh_heat_create");
+        Io io(pl, mem);
+        const int N = pl->N;
+        hipStream_t st = pl->ctx->stream;
+        if (N > 0) {
+            const double* tp = io.in(tb_plus, N, pl->io_a);
+            const double* tm = tb_minus ? io.in(tb_minus, N, pl->io_c) : nullptr;
+            // r1 = surface_divergence!(-kappa [T]) (heatconduction.jl:97-98), r2 = surface average
+            k_heat_bc<<<(N + 255) / 256, 256, 0, st>>>(tp, tm, N, h->kappa, h->e.as<double>(), h->r2.as<double>());
+            ILH_HIP(hipGetLastError());
+            scatter_op(pl, OP_DS, h->e.as<double>(), h->F.as<double>());   // includes the 1/dx of GridScaling
+        } else {
+            fill_zero(st, h->F.as<double>(), pl->g.n(GK_PRIMAL));
+        }
+        h->have_bc = true;
+        io.done();
+    });
+}
+
+extern "C" int ilh_heat_advance(ilh_heat* h, double* T, int nsteps, int mem)
+{
+    return guarded_call([&] {
+        if (!h) fail(ILH_ERR_ARG, "null heat plan");
+        if (nsteps < 0) fail(ILH_ERR_ARG, "negative step count");
+        ilh_plan* pl = h->pl;
+        if (pl->gen != h->gen) fail(ILH_ERR_STATE, "the plan's geometry changed since ilh_heat_create");
+        if (!h->have_bc) fail(ILH_ERR_STATE, "ilh_heat_set_boundary must come first");
+        Io io(pl, mem);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        hipStream_t st = pl->ctx->stream;
+        double* d_T = io.out(T, ng, pl->io_b);
+        if (mem == ILH_MEM_HOST)
+            ILH_HIP(hipMemcpyAsync(d_T, T, sizeof(double) * ng, hipMemcpyHostToDevice, st));
+        const double dt = h->dt;
+        const int ld = solve_ld(std::max(1, N));
+        double *F = h->F.as<double>(), *A = h->A.as<double>(), *Bt = h->Bt.as<double>(), *Y = h->Y.as<double>();
+        double *W1 = h->W1.as<double>(), *W2 = h->W2.as<double>(), *Rs = h->Rs.as<double>();
+        double *e = h->e.as<double>(), *sg = h->sg.as<double>(), *r2 = h->r2.as<double>();
+        const double* K13 = h->k13.as<double>();
+        const double* K23 = h->k23.as<double>();
+        // sigma = -(1/(dt c)) Sinv (E Y - r2);  W = F + R sigma
+        auto multiplier = [&](const double* Yh, const double* Sinv, double c, double* W) {
+            gather_op(pl, OP_E_PRIMAL, Yh, e);
+            axpby(st, e, r2, -1.0, 1.0, N);                                  // e = E Y - r2
+            matvec_rows(st, Sinv, ld, e, sg, N, N, -1.0 / (dt * c), 0);
+            scatter_op(pl, OP_R_PRIMAL, sg, Rs);
+            if (W) lin(st, W, F, 1.0, Rs, ng);
+        };
+        for (int n = 0; n < nsteps; ++n) {
+            if (N == 0) {
+                // no bodies: T <- H1 T + dt (b2 H23 F + b3 F)
+                heat_apply(pl, K13, h->M13, d_T, Bt, A);
+                lin(st, A, A, dt * B2, F, ng);
+                heat_apply(pl, K23, h->M23, A, Bt, Y);
+                lin(st, d_T, Y, dt * B3, F, ng);
+                continue;
+            }
+            // stage 2: Yhat2 = H13 (T + dt a21 F)
+            lin(st, A, d_T, dt * A21, F, ng);
+            heat_apply(pl, K13, h->M13, A, Bt, Y);
+            multiplier(Y, h->S13.as<double>(), A21, W1);
+            // stage 3: Yhat3 = H23 [H13 (T + dt a31 W1) + dt a32 F]
+            lin(st, A, d_T, dt * A31, W1, ng);
+            heat_apply(pl, K13, h->M13, A, Bt, A);
+            lin(st, A, A, dt * A32, F, ng);
+            heat_apply(pl, K23, h->M23, A, Bt, Y);
+            multiplier(Y, h->S23.as<double>(), A32, W2);
+            // final: yhat = H23 [H13 T + dt b2 W2] + dt b3 F;  T = yhat + dt b3 R sigma3
+            heat_apply(pl, K13, h->M13, d_T, Bt, A);
+            lin(st, A, A, dt * B2, W2, ng);
+            heat_apply(pl, K23, h->M23, A, Bt, Y);
+            lin(st, Y, Y, dt * B3, F, ng);
+            multiplier(Y, h->S0.as<double>(), B3, nullptr);
+            lin(st, d_T, Y, dt * B3, Rs, ng);
+        }
+        io.back(T, d_T, ng);
+        io.done();
+    });
+}
+
+extern "C" int ilh_heat_destroy(ilh_heat* h)
+{
+    return guarded_call([&] {
+        if (!h) return;
+        (void)hipSetDevice(h->pl->ctx->device);
+        (void)hipStreamSynchronize(h->pl->ctx->stream);
+        delete h;
+    });
+}

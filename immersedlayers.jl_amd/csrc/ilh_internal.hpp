@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -196,9 +197,18 @@ void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp,
                   hipEvent_t* ev = nullptr);
 void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage);
 
+// operators on the plan's geometry (ilh_api.hip): scatter R/D_s-type ops to the
+// grid, gather E-type ops to the surface; inverse of a dense N x N matrix
+// (overwritten) into row-major Minv with leading dimension solve_ld(N)
+void scatter_op(ilh_plan* pl, int op, const double* amp, double* out);
+void gather_op(ilh_plan* pl, int op, const double* field, double* out);
+void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv);
+
 // dense linear algebra (ilh_dense.hip)
 void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel,
               double* K /* N x N col-major */);
+void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor,
+                    double* K /* N x N col-major */);
 int  lu_factor(ilh_plan* pl, double* A, int n, int* piv_dev, int* info_dev);
 void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, double* B, int nrhs);
 // row-major solve matrices use a leading dimension padded to a multiple of 32
@@ -212,5 +222,69 @@ void neumann_amp(hipStream_t st, const double* Mdf_p, const double* Mdf_m, const
                  const double* Mds_m, int ld, const double* vp, const double* vm, double* amp_f, double* amp_s,
                  double* df, double* dsv, int n);
 void check_finite(hipStream_t st, const double* a, size_t n, int* flag_dev);
+
+}  // namespace ilh
+
+// ---------------------------------------------------- C-ABI call helpers
+namespace ilh {
+void set_last_error(const std::string& msg);   // thread-local message of ilh_last_error()
+
+// run f, turning exceptions into the C ABI's status codes
+template <class F>
+int guarded_call(F&& f)
+{
+    try {
+        f();
+        return ILH_OK;
+    } catch (const Error& e) {
+        set_last_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return ILH_ERR_OOM;
+    } catch (...) {
+        set_last_error("unknown error");
+        return ILH_ERR_ARG;
+    }
+}
+
+void need_margin(ilh_plan* pl, const char* what);
+
+// host or device array arguments of one call: host arrays are staged through
+// the plan's buffers on its stream
+struct Io {
+    ilh_plan* pl;
+    int mem;
+    Io(ilh_plan* p, int m) : pl(p), mem(m)
+    {
+        if (!p) fail(ILH_ERR_ARG, "null plan");
+        if (m != ILH_MEM_HOST && m != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
+        ILH_HIP(hipSetDevice(p->ctx->device));
+    }
+    const double* in(const double* h, size_t n, DevBuf& buf)
+    {
+        if (n > 0 && !h) fail(ILH_ERR_ARG, "null input array");
+        if (mem == ILH_MEM_DEVICE) return h;
+        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
+        if (n) ILH_HIP(hipMemcpyAsync(buf.p, h, sizeof(double) * n, hipMemcpyHostToDevice, pl->ctx->stream));
+        return buf.as<double>();
+    }
+    double* out(double* h, size_t n, DevBuf& buf)
+    {
+        if (n > 0 && !h) fail(ILH_ERR_ARG, "null output array");
+        if (mem == ILH_MEM_DEVICE) return h;
+        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
+        return buf.as<double>();
+    }
+    void back(double* h, const double* d, size_t n)
+    {
+        if (mem == ILH_MEM_DEVICE || n == 0) return;
+        ILH_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, pl->ctx->stream));
+    }
+    void done()
+    {
+        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+    }
+};
 
 }  // namespace ilh

@@ -15,7 +15,8 @@ from .core import (Context, DeviceArray, Nodes, Edges, ScalarData, VectorData, S
                    divergence, grad, curl, laplacian, inverse_laplacian,
                    create_RTLinvR, create_CLinvCT, create_GLinvD, create_nRTRn,
                    DirichletPoisson, NeumannPoisson, SchurFactor, default_context,
-                   dirichlet_solve_batched, dirichlet_solve_batched_device, DirichletBatch)
+                   dirichlet_solve_batched, dirichlet_solve_batched_device, DirichletBatch,
+                   DirichletHeatConduction)
 from .geometry import GridSpec as PhysicalGrid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

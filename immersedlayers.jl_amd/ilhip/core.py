@@ -491,6 +491,43 @@ class DirichletBatch:
             pass
 
 
+class DirichletHeatConduction:
+    """Unbounded heat conduction with Dirichlet bodies (test/literate/heatconduction.jl)
+    marched by IF-HERK: ``advance(T, nsteps)`` updates the Nodes{Primal}
+    temperature in place; boundary values are time-independent."""
+
+    def __init__(self, cache, kappa, dt, tb_plus, tb_minus=None):
+        h = C.c_void_p()
+        L.call("ilh_heat_create", cache.h, C.c_double(kappa), C.c_double(dt), C.byref(h))
+        self.h, self.cache, self.kappa, self.dt = h, cache, kappa, dt
+        self.set_boundary(tb_plus, tb_minus)
+
+    def set_boundary(self, tb_plus, tb_minus=None):
+        tp = _vec(tb_plus, self.cache.N)
+        tm = None if tb_minus is None else _vec(tb_minus, self.cache.N)
+        L.call("ilh_heat_set_boundary", self.h, _ptr(tp), None if tm is None else _ptr(tm), L.MEM_HOST)
+
+    def advance(self, T, nsteps):
+        if not isinstance(T, Nodes) or T.kind != Primal:
+            raise TypeError("T must be Nodes{Primal}")
+        L.call("ilh_heat_advance", self.h, _ptr(T.flat()), int(nsteps), L.MEM_HOST)
+        return T
+
+    def advance_device(self, T_dev, nsteps):
+        L.call("ilh_heat_advance", self.h, T_dev.ptr, int(nsteps), L.MEM_DEVICE)
+
+    def close(self):
+        if self.h:
+            L.load().ilh_heat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class NeumannPoisson:
     """The Neumann Poisson problem of test/literate/neumann.jl:80-142: returns
     (f, [ϕ], s, [ψ]) with S = create_CLinvCT factored once per geometry."""

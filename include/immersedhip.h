@@ -193,6 +193,23 @@ int ilh_batch_destroy(ilh_batch* batch);
 int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
                                double* s, int mem, double tol, int max_iter, int* iters);
 
+/* ---- unbounded heat conduction with Dirichlet bodies (config: IF time marching) ----
+ * The constrained ODE system of test/literate/heatconduction.jl:88-180,
+ *   dT/dt = kappa L T - kappa D_s [T] + R sigma,   E T = (T_b+ + T_b-)/2,
+ * T on Nodes{Primal}, marched by the integrating-factor half-explicit RK method
+ * that ConstrainedSystems' LiskaIFHERK implements (src/timemarching.jl:5;
+ * 3 stages, c = (0, 1/3, 1)), the integrating factor exp(tau kappa L) applied
+ * exactly as the separable lattice heat kernel e^{-2a} I_p(2a).
+ * ilh_heat_create fixes kappa and dt for the plan's geometry (it builds and
+ * inverts the three stage Schur matrices); ilh_heat_set_boundary sets
+ * time-independent surface values T_b+ (and T_b-, or NULL for 0);
+ * ilh_heat_advance marches T (in place) by nsteps steps. */
+typedef struct ilh_heat ilh_heat;
+int ilh_heat_create(ilh_plan* plan, double kappa, double dt, ilh_heat** out);
+int ilh_heat_set_boundary(ilh_heat* heat, const double* tb_plus, const double* tb_minus, int mem);
+int ilh_heat_advance(ilh_heat* heat, double* T, int nsteps, int mem);
+int ilh_heat_destroy(ilh_heat* heat);
+
 /* HIP-event timings on the plan's stream, in ms per call, of the Dirichlet
    repeat solve on device buffers: *total_ms = `reps` whole solves back to back
    (two events around the loop); if stage_ms is not NULL, stage_ms[k] = `reps`

@@ -917,3 +917,155 @@ int orc_neumann_solve(const orc_grid* g, int ddf, int N, const double* X, const 
     free(dvn); free(vn); free(fstar); free(sstar); free(Sown);
     return info;
 }
+
+
+/* ------------------------------------------------------------------ heat */
+/* K(p) = e^{-x} I_p(x), x = 2a, from the power series
+   I_p(x) = sum_k (x/2)^{2k+p} / (k! (k+p)!)  (Abramowitz & Stegun 9.6.10),
+   summed in log space for the prefactor; kept while K(p) > 1e-17 K(0). */
+int orc_heat_kernel(double a, double* K, int maxlen)
+{
+    if (a <= 0.0) { K[0] = 1.0; return 1; }
+    const double x = 2.0 * a, h = 0.5 * x;
+    int M = 0;
+    for (int p = 0; p < maxlen; ++p) {
+        /* term_k = (x/2)^{2k+p}/(k!(k+p)!) e^{-x}, first term in logs */
+        double lt = p * log(h) - lgamma(p + 1.0) - x;
+        double t = exp(lt), sum = 0.0;
+        for (int k = 0; k < 100000; ++k) {
+            sum += t;
+            t *= h * h / ((k + 1.0) * (k + 1.0 + p));
+            if (t < 1e-20 * sum && k > h) break;
+        }
+        K[p] = sum;
+        M = p;
+        if (p > 0 && sum <= 1e-17 * K[0]) break;
+    }
+    return M + 1;
+}
+
+void orc_heat_apply(int nx, int ny, const double* K, int M, const double* in, double* out)
+{
+    double* tmp = (double*)malloc(sizeof(double) * (size_t)nx * ny);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double acc = 0.0;
+            for (int p = -M; p <= M; ++p) {
+                const int q = i - p;
+                if (q >= 0 && q < nx) acc += K[abs(p)] * in[q + (size_t)nx * j];
+            }
+            tmp[i + (size_t)nx * j] = acc;
+        }
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double acc = 0.0;
+            for (int p = -M; p <= M; ++p) {
+                const int q = j - p;
+                if (q >= 0 && q < ny) acc += K[abs(p)] * tmp[i + (size_t)nx * q];
+            }
+            out[i + (size_t)nx * j] = acc;
+        }
+    free(tmp);
+}
+
+/* S[:, q] = E H R e_q column by column (H = the filter K, or the identity) */
+static void heat_schur(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
+                       const double* K, int M, double* S)
+{
+    int nx, ny;
+    orc_dims(g, ORC_PRIMAL, &nx, &ny);
+    const size_t ng = (size_t)nx * ny;
+    double* e = (double*)calloc(N, sizeof(double));
+    double* f = (double*)malloc(sizeof(double) * ng);
+    double* h = (double*)malloc(sizeof(double) * ng);
+    for (int q = 0; q < N; ++q) {
+        memset(e, 0, sizeof(double) * N);
+        e[q] = 1.0;
+        orc_regularize(g, ORC_PRIMAL, ddf, N, X, Y, ds, e, f);
+        if (K) orc_heat_apply(nx, ny, K, M, f, h);
+        else memcpy(h, f, sizeof(double) * ng);
+        orc_interpolate(g, ORC_PRIMAL, ddf, N, X, Y, h, S + (size_t)N * q);
+    }
+    free(e);
+    free(f);
+    free(h);
+}
+
+int orc_heat_advance(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* nx_,
+                     const double* ny_, const double* ds, double kappa, double dt, int nsteps, double* T,
+                     const double* tb_plus, const double* tb_minus)
+{
+    /* tableau of the 3-stage HERK scheme: c = (0, 1/3, 1) */
+    const double a21 = 1.0 / 3.0, a31 = -1.0, a32 = 2.0, b2 = 0.75, b3 = 0.25;
+    int nx, ny;
+    orc_dims(g, ORC_PRIMAL, &nx, &ny);
+    const size_t ng = (size_t)nx * ny;
+    const double a13 = kappa * dt / 3.0 / (g->dx * g->dx);
+    double K13[512], K23[512];
+    const int M13 = orc_heat_kernel(a13, K13, 512) - 1, M23 = orc_heat_kernel(2.0 * a13, K23, 512) - 1;
+    double *S13 = NULL, *S23 = NULL, *S0 = NULL;
+    int *p13 = NULL, *p23 = NULL, *p0 = NULL;
+    int info = 0;
+    if (N > 0) {
+        S13 = (double*)malloc(sizeof(double) * (size_t)N * N);
+        S23 = (double*)malloc(sizeof(double) * (size_t)N * N);
+        S0 = (double*)malloc(sizeof(double) * (size_t)N * N);
+        p13 = (int*)malloc(sizeof(int) * N);
+        p23 = (int*)malloc(sizeof(int) * N);
+        p0 = (int*)malloc(sizeof(int) * N);
+        heat_schur(g, ddf, N, X, Y, ds, K13, M13, S13);
+        heat_schur(g, ddf, N, X, Y, ds, K23, M23, S23);
+        heat_schur(g, ddf, N, X, Y, ds, NULL, 0, S0);
+        info = orc_lu_factor(N, S13, p13) | orc_lu_factor(N, S23, p23) | orc_lu_factor(N, S0, p0);
+    }
+    /* r1 = surface_divergence(-kappa [T]), r2 = surface average */
+    double* F = (double*)calloc(ng, sizeof(double));
+    double* r2 = (double*)calloc(N > 0 ? N : 1, sizeof(double));
+    double* dn = (double*)calloc(N > 0 ? N : 1, sizeof(double));
+    for (int k = 0; k < N; ++k) {
+        const double a = tb_plus[k], b = tb_minus ? tb_minus[k] : 0.0;
+        dn[k] = -kappa * (a - b);
+        r2[k] = 0.5 * (a + b);
+    }
+    if (N > 0) orc_surface_divergence(g, ddf, N, X, Y, nx_, ny_, ds, dn, F);
+    double *A = (double*)malloc(sizeof(double) * ng), *Yh = (double*)malloc(sizeof(double) * ng);
+    double *W1 = (double*)malloc(sizeof(double) * ng), *W2 = (double*)malloc(sizeof(double) * ng);
+    double* Rs = (double*)malloc(sizeof(double) * ng);
+    double* sg = (double*)malloc(sizeof(double) * (N > 0 ? N : 1));
+    /* sigma = -(S^-1 (E Y - r2)) / (dt c);  W = F + R sigma (W may be NULL: Rs only) */
+    #define MULT(Yv, S, P, c, W) do { \
+        orc_interpolate(g, ORC_PRIMAL, ddf, N, X, Y, Yv, sg); \
+        for (int k = 0; k < N; ++k) sg[k] = sg[k] - r2[k]; \
+        orc_lu_solve(N, S, P, sg); \
+        for (int k = 0; k < N; ++k) sg[k] *= -1.0 / (dt * (c)); \
+        orc_regularize(g, ORC_PRIMAL, ddf, N, X, Y, ds, sg, Rs); \
+        if (W) for (size_t k = 0; k < ng; ++k) ((double*)(W))[k] = F[k] + Rs[k]; \
+    } while (0)
+    for (int n = 0; n < nsteps; ++n) {
+        if (N == 0) {
+            orc_heat_apply(nx, ny, K13, M13, T, A);
+            for (size_t k = 0; k < ng; ++k) A[k] += dt * b2 * F[k];
+            orc_heat_apply(nx, ny, K23, M23, A, Yh);
+            for (size_t k = 0; k < ng; ++k) T[k] = Yh[k] + dt * b3 * F[k];
+            continue;
+        }
+        for (size_t k = 0; k < ng; ++k) A[k] = T[k] + dt * a21 * F[k];
+        orc_heat_apply(nx, ny, K13, M13, A, Yh);
+        MULT(Yh, S13, p13, a21, W1);
+        for (size_t k = 0; k < ng; ++k) A[k] = T[k] + dt * a31 * W1[k];
+        orc_heat_apply(nx, ny, K13, M13, A, Yh);
+        for (size_t k = 0; k < ng; ++k) Yh[k] += dt * a32 * F[k];
+        orc_heat_apply(nx, ny, K23, M23, Yh, A);
+        MULT(A, S23, p23, a32, W2);
+        orc_heat_apply(nx, ny, K13, M13, T, A);
+        for (size_t k = 0; k < ng; ++k) A[k] += dt * b2 * W2[k];
+        orc_heat_apply(nx, ny, K23, M23, A, Yh);
+        for (size_t k = 0; k < ng; ++k) Yh[k] += dt * b3 * F[k];
+        MULT(Yh, S0, p0, b3, (double*)NULL);
+        for (size_t k = 0; k < ng; ++k) T[k] = Yh[k] + dt * b3 * Rs[k];
+    }
+    #undef MULT
+    free(S13); free(S23); free(S0); free(p13); free(p23); free(p0);
+    free(F); free(r2); free(dn); free(A); free(Yh); free(W1); free(W2); free(Rs); free(sg);
+    return info;
+}

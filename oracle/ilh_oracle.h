@@ -89,6 +89,17 @@ void orc_schur_direct(const orc_grid* g, int ddf, int N, const double* X, const 
                       const double* nx, const double* ny, const double* ds, int kind, double scale, double* A);
 
 /* dense LU with partial pivoting (A12) */
+/* unbounded heat conduction with Dirichlet bodies, IF-HERK (the constrained
+   system of test/literate/heatconduction.jl:88-180 marched by ConstrainedSystems'
+   LiskaIFHERK, src/timemarching.jl:5): lattice heat kernel K(p) = e^{-2a} I_p(2a)
+   (p = 0..M, returns M+1 <= maxlen), its separable application on an nx x ny
+   node field (zero outside), and nsteps steps of T (Nodes{Primal}, in place). */
+int  orc_heat_kernel(double a, double* K, int maxlen);
+void orc_heat_apply(int nx, int ny, const double* K, int M, const double* in, double* out);
+int  orc_heat_advance(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* nx,
+                      const double* ny, const double* ds, double kappa, double dt, int nsteps, double* T,
+                      const double* tb_plus, const double* tb_minus);
+
 int  orc_lu_factor(int n, double* A, int* piv);
 void orc_lu_solve(int n, const double* LU, const int* piv, double* b);
 

@@ -307,3 +307,32 @@ def neumann_solve(g, geo, vplus, vminus=None, S=None, ddf_kind=YANG3):
     if info:
         raise np.linalg.LinAlgError("singular Schur complement (info=%d)" % info)
     return f, df, s, dsv
+
+
+# ------------------------------------------------------------------ heat
+def heat_kernel(a, maxlen=512):
+    """K(p) = e^{-2a} I_p(2a), p = 0..M (the 1-D lattice heat kernel of exp(a L1))."""
+    K = np.zeros(maxlen)
+    n = lib().orc_heat_kernel(C.c_double(a), _p(K), maxlen)
+    return K[:n].copy()
+
+
+def heat_apply(nx, ny, K, field):
+    """Separable application of the kernel K to an (nx, ny) node field (zero outside)."""
+    K = np.ascontiguousarray(K, float)
+    a = np.ascontiguousarray(np.asarray(field, float).ravel(order="F"))
+    out = np.zeros(nx * ny)
+    lib().orc_heat_apply(nx, ny, _p(K), len(K) - 1, _p(a), _p(out))
+    return out
+
+
+def heat_advance(g, geo, kappa, dt, nsteps, T0, tb_plus, tb_minus=None, ddf_kind=YANG3):
+    """nsteps IF-HERK steps of the Dirichlet heat conduction problem (Nodes{Primal} T)."""
+    T = np.ascontiguousarray(np.asarray(T0, float).ravel(order="F")).copy()
+    tp = np.ascontiguousarray(tb_plus, float)
+    tm = None if tb_minus is None else np.ascontiguousarray(tb_minus, float)
+    info = lib().orc_heat_advance(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny), _p(geo.ds),
+                                  C.c_double(kappa), C.c_double(dt), int(nsteps), _p(T), _p(tp), _p(tm))
+    if info:
+        raise np.linalg.LinAlgError("singular stage Schur matrix")
+    return T

@@ -333,3 +333,61 @@ def test_lu_solve():
     info, x = O.lu_solve(A, b)
     assert info == 0
     assert rel_err(x, np.linalg.solve(A, b)) < 1e-12
+
+
+# ------------------------------------------------------------------- heat
+def test_heat_kernel_pins():
+    """K(p) = e^{-2a} I_p(2a): against scipy's scaled Bessel function, unit mass,
+    the semigroup K_a * K_b = K_{a+b}, and dK/da = K(p-1) - 2K(p) + K(p+1)."""
+    from scipy.special import ive
+    for a in (0.01, 0.125, 1.0 / 3.0, 2.0, 9.0):
+        K = O.heat_kernel(a)
+        assert np.abs(K - ive(np.arange(len(K)), 2 * a)).max() < 1e-15
+        assert abs(K[0] + 2 * K[1:].sum() - 1.0) < 1e-14
+    full = lambda K: np.concatenate([K[::-1], K[1:]])
+    c = np.convolve(full(O.heat_kernel(0.2)), full(O.heat_kernel(0.3)))
+    K5 = O.heat_kernel(0.5)
+    m = len(c) // 2
+    assert np.abs(c[m:m + len(K5)] - K5).max() < 1e-15
+    a, h = 0.7, 1e-4
+    Kp, Km, K0 = O.heat_kernel(a + h), O.heat_kernel(a - h), O.heat_kernel(a)
+    n = min(len(Kp), len(Km), len(K0)) - 1
+    dK = (Kp[:n] - Km[:n]) / (2 * h)
+    lap = np.concatenate([[2 * K0[1] - 2 * K0[0]], K0[:n - 1] - 2 * K0[1:n] + K0[2:n + 1]])
+    assert np.abs(dK[:n - 1] - lap[:n - 1]).max() < 1e-8
+
+
+def _heat_setup(n=64, dx=0.04, width=4.0):
+    G = _G()
+    g = O.grid(n + 1, n + 1, n // 2 + 1, n // 2 + 1, dx)
+    s = G.circle(0.5, 1.4 * dx)
+    xs = (np.arange(n) + 1 - (n // 2 + 1)) * dx
+    Xg, Yg = np.meshgrid(xs, xs, indexing="ij")
+    T0 = np.exp(-((Xg - 0.4) ** 2 + (Yg + 0.3) ** 2) / (2 * (width * dx) ** 2)).ravel(order="F")
+    return g, s, T0
+
+
+def test_heat_no_bodies_is_the_exact_semigroup():
+    """Without bodies IF-HERK integrates diffusion exactly: k steps of dt equal
+    one application of exp(k dt kappa L)."""
+    g, s, T0 = _heat_setup(width=2.0)   # negligible at the grid edge: no truncation there
+    e = np.zeros(0)
+    geo0 = O.Geometry(e, e, e, e, e)
+    kappa, dt, k = 1.0, 0.25 * g.dx ** 2, 7
+    T = O.heat_advance(g, geo0, kappa, dt, k, T0, e)
+    K = O.heat_kernel(k * kappa * dt / g.dx ** 2)
+    assert np.abs(T - O.heat_apply(g.NX - 1, g.NY - 1, K, T0)).max() < 1e-13
+
+
+def test_heat_constraint_and_time_convergence():
+    """With a body at T_b = 1: E T = T_b/2 + T_b/2 holds after every step, and
+    halving dt shrinks the error at fixed time at better than second order."""
+    g, s, T0 = _heat_setup()
+    geo = _geo(s)
+    tb = np.ones(s.N)
+    kappa, tend = 1.0, 0.5 * g.dx ** 2 * 8
+    T = O.heat_advance(g, geo, kappa, 0.5 * g.dx ** 2, 3, T0, tb, tb)
+    assert np.abs(O.interpolate(g, O.PRIMAL, geo, T) - 1.0).max() < 1e-11
+    sols = [O.heat_advance(g, geo, kappa, tend / k, k, T0, tb) for k in (2, 4, 8)]
+    ratio = np.linalg.norm(sols[0] - sols[2]) / np.linalg.norm(sols[1] - sols[2])
+    assert ratio > 4.0, ratio
