@@ -189,6 +189,40 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_heat(args):
+    """Unbounded heat conduction with 3 Dirichlet bodies (T_b = 1) on 2048^2,
+    IF-HERK at Fo = 0.25: `steps` time steps (200 by the workload's definition
+    when run with --steps 200) after `warmup` untimed ones."""
+    import ilhip
+    from ilhip import geometry as G
+    g, s, T0 = G.config_heat()
+    ctx = ilhip.Context(0)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    kappa, dt = 1.0, 0.25 * g.dx ** 2
+    t0 = time.perf_counter()
+    heat = ilhip.DirichletHeatConduction(cache, kappa, dt, np.ones(s.N))
+    t_prep = time.perf_counter() - t0
+    Td = ilhip.DeviceArray.from_host(ctx, T0)
+    heat.advance_device(Td, args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    heat.advance_device(Td, args.steps)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    T = ilhip.Nodes(ilhip.Primal, g, Td.download().reshape(g.primal_shape(), order="F"))
+    Eb = ilhip.ScalarData(s.N)
+    ilhip.interpolate(Eb, T, cache)
+    out = {"metric": "IF-HERK heat-conduction steps/s (2048^2, 3 bodies)", "value": args.steps / el, "unit": "steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (3 seeded star bodies at T_b = 1, 4 Gaussian bumps)",
+           "config": {"workload": "Dirichlet heat conduction, Fo = 0.25, kappa = 1",
+                      "grid_primal": list(g.primal_shape()), "npts": s.N, "dt": dt},
+           "prep_s": t_prep, "constraint_residual": float(np.abs(Eb.flat() - 0.5).max())}
+    heat.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +230,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle CPU time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dirichlet", "ops", "batched"], default="dirichlet",
+    ap.add_argument("--workload", choices=["dirichlet", "ops", "batched", "heat"], default="dirichlet",
                     help="dirichlet: the repeat-solve metric; ops: operator sweeps on the 2048^2 "
                          "microbenchmark grid (GB/s against HBM peak)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -223,6 +257,8 @@ def main():
         return bench_ops(args)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "heat":
+        return bench_heat(args)
 
     import ilhip
     from ilhip import geometry as G

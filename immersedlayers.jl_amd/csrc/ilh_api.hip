@@ -147,6 +147,7 @@ extern "C" int ilh_ctx_destroy(ilh_ctx* c)
             }
         for (hipEvent_t e : c->ev)
             if (e) (void)hipEventDestroy(e);
+        release_blas(c);
         (void)hipStreamDestroy(c->stream);
         delete c;
     });

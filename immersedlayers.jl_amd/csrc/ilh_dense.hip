@@ -13,6 +13,8 @@
 // solves run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
 #include "ilh_internal.hpp"
 
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <cstdlib>
 
@@ -174,7 +176,7 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int Nc, int K, const do
             }
 }
 
-// Plain FP64 FMA reference path for the same update (selected by ILH_GEMM_PLAIN=1).
+// Plain FP64 FMA path for the same update (selected by ILH_GEMM=plain).
 __global__ void k_gemm_sub_plain(int M, int Nc, int K, const double* __restrict__ A, int lda,
                                  const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc)
 {
@@ -186,21 +188,49 @@ __global__ void k_gemm_sub_plain(int M, int Nc, int K, const double* __restrict_
     C[r + (size_t)ldc * c] -= s;
 }
 
-static bool gemm_plain()
+// GEMM path: 0 = rocBLAS dgemm (default), 1 = the MFMA kernel above,
+// 2 = plain FMA (ILH_GEMM=hip / ILH_GEMM=plain select the hand-written ones)
+static int gemm_path()
 {
     static int v = -1;
     if (v < 0) {
-        const char* e = getenv("ILH_GEMM_PLAIN");
-        v = (e && e[0] == '1') ? 1 : 0;
+        const char* e = getenv("ILH_GEMM");
+        v = (e && std::string(e) == "hip") ? 1 : (e && std::string(e) == "plain") ? 2 : 0;
     }
-    return v == 1;
+    return v;
 }
 
-static void gemm_sub(hipStream_t st, int M, int Nc, int K, const double* A, int lda, const double* B, int ldb,
-                     double* C, int ldc)
+static rocblas_handle blas_handle(ilh_ctx* ctx)
+{
+    if (!ctx->blas) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) fail(ILH_ERR_HIP, "rocblas_create_handle failed");
+        ctx->blas = h;
+    }
+    return (rocblas_handle)ctx->blas;
+}
+
+void release_blas(ilh_ctx* ctx)
+{
+    if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
+    ctx->blas = nullptr;
+}
+
+// C[M x Nc] -= A[M x K] B[K x Nc] on stream st (the trailing updates of the
+// blocked LU and of its multi-right-hand-side solves: plain GEMMs)
+static void gemm_sub(ilh_ctx* ctx, hipStream_t st, int M, int Nc, int K, const double* A, int lda, const double* B,
+                     int ldb, double* C, int ldc)
 {
     if (M <= 0 || Nc <= 0 || K <= 0) return;
-    if (gemm_plain()) {
+    const int path = gemm_path();
+    if (path == 0) {
+        rocblas_handle h = blas_handle(ctx);
+        if (rocblas_set_stream(h, st) != rocblas_status_success) fail(ILH_ERR_HIP, "rocblas_set_stream failed");
+        const double alpha = -1.0, beta = 1.0;
+        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, M, Nc, K, &alpha, A, lda, B, ldb, &beta,
+                          C, ldc) != rocblas_status_success)
+            fail(ILH_ERR_HIP, "rocblas_dgemm failed");
+    } else if (path == 2) {
         k_gemm_sub_plain<<<dim3((M + 63) / 64, (Nc + 3) / 4), 256, 0, st>>>(M, Nc, K, A, lda, B, ldb, C, ldc);
     } else {
         k_gemm_sub<<<dim3((M + GT - 1) / GT, (Nc + GT - 1) / GT), 256, 0, st>>>(M, Nc, K, A, lda, B, ldb, C, ldc);
@@ -693,7 +723,7 @@ int lu_factor(ilh_plan* pl, double* A, int n, int* piv, int* info)
         if (rest > 0) {
             k_tri_inv<0><<<1, NB, 0, st>>>(A, n, k0, nb, Tinv);
             k_tri_apply<<<(rest + 63) / 64, 256, 0, st>>>(Tinv, k0, nb, A, n, rest, k1);
-            gemm_sub(st, rest, rest, nb, A + k1 + (size_t)n * k0, n, A + k0 + (size_t)n * k1, n,
+            gemm_sub(pl->ctx, st, rest, rest, nb, A + k1 + (size_t)n * k0, n, A + k0 + (size_t)n * k1, n,
                      A + k1 + (size_t)n * k1, n);
         }
         ILH_HIP(hipGetLastError());
@@ -759,13 +789,13 @@ void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, doubl
         const int nb = std::min(NB, n - k0);
         k_tri_apply<<<(nrhs + 63) / 64, 256, 0, st>>>(Linv + (size_t)NB * NB * (k0 / NB), k0, nb, B, n, nrhs, 0);
         const int k1 = k0 + nb;
-        gemm_sub(st, n - k1, nrhs, nb, LU + k1 + (size_t)n * k0, n, B + k0, n, B + k1, n);
+        gemm_sub(pl->ctx, st, n - k1, nrhs, nb, LU + k1 + (size_t)n * k0, n, B + k0, n, B + k1, n);
     }
     const int last = ((n - 1) / NB) * NB;
     for (int k0 = last; k0 >= 0; k0 -= NB) {
         const int nb = std::min(NB, n - k0);
         k_tri_apply<<<(nrhs + 63) / 64, 256, 0, st>>>(Uinv + (size_t)NB * NB * (k0 / NB), k0, nb, B, n, nrhs, 0);
-        gemm_sub(st, k0, nrhs, nb, LU + (size_t)n * k0, n, B + k0, n, B, n);
+        gemm_sub(pl->ctx, st, k0, nrhs, nb, LU + (size_t)n * k0, n, B + k0, n, B, n);
     }
     ILH_HIP(hipGetLastError());
 }

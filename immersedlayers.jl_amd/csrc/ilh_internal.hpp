@@ -125,6 +125,7 @@ struct ilh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 0;
+    void* blas = nullptr;                     // rocBLAS handle (plain GEMMs of the dense factorisations)
     static constexpr int POOL = 4;            // side streams of batched calls (one per hardware queue)
     hipStream_t pool[POOL] = {};
     hipEvent_t ev[POOL + 1] = {};
@@ -207,6 +208,7 @@ void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv);
 // dense linear algebra (ilh_dense.hip)
 void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel,
               double* K /* N x N col-major */);
+void release_blas(ilh_ctx* ctx);
 void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor,
                     double* K /* N x N col-major */);
 int  lu_factor(ilh_plan* pl, double* A, int n, int* piv_dev, int* info_dev);

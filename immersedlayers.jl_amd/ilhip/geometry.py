@@ -193,6 +193,30 @@ def config_batched(seed=3, nprob=64, sizes=(256, 384, 512, 768, 1024), dx=0.01, 
     return out
 
 
+def config_heat(seed=4, n=2048, spacing=1.3):
+    """Unbounded heat conduction: n² Nodes{Primal} on [-2,2]² with 3 seeded star
+    bodies (about 2000 points each at `spacing`·dx) and an initial field of 4
+    Gaussian bumps (widths >= 4 dx) away from the bodies.  Returns (grid,
+    surface, T0 as an (n, n) array)."""
+    dx = 4.0 / n
+    g = GridSpec(n + 1, n + 1, n // 2 + 1, n // 2 + 1, dx)
+    rng = np.random.default_rng(seed)
+    centres = [(-1.0, -0.95), (1.0, -0.95), (0.0, 0.95)]
+    bodies = []
+    for cx, cy in centres:
+        r0 = rng.uniform(0.6, 0.7)
+        bodies.append(star(r0, rng.uniform(-0.1, 0.1, 5), spacing * dx, cx + rng.uniform(-0.05, 0.05),
+                           cy + rng.uniform(-0.05, 0.05)))
+    s = concat(bodies)
+    xs = (np.arange(n) + 1 - g.I0x) * dx
+    Xg, Yg = np.meshgrid(xs, xs, indexing="ij")
+    T0 = np.zeros((n, n))
+    for bx, by in [(-1.0, 0.6), (1.0, 0.6), (0.0, -0.6), (0.0, -1.7)]:
+        w = rng.uniform(4, 12) * dx
+        T0 += np.exp(-((Xg - bx - rng.uniform(-0.1, 0.1)) ** 2 + (Yg - by - rng.uniform(-0.1, 0.1)) ** 2) / (2 * w * w))
+    return g, s, T0
+
+
 def config_headline(seed=5, spacing=1.2):
     """1024² Nodes{Primal} on [-2,2]² (Δx=4/1024): an ellipse of aspect 2 and a
     random star curve, about 1000 points each at ``spacing``·Δx."""
