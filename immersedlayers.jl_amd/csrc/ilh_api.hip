@@ -786,6 +786,130 @@ extern "C" int ilh_neumann_solve(ilh_plan* pl, const double* vp, const double* v
     });
 }
 
+// ----------------------------------------------------------------- masks
+namespace {
+
+// Nodes{Primal} -> other grid kinds by averaging the neighbouring primal cells
+// (CartesianGrids grid_interpolate!): dual node (i,j) from cells (i-1..i,
+// j-1..j), u-edge (i,j) from cells (i-1..i, j), v-edge (i,j) from (i, j-1..j);
+// values on the outer ring (missing cells) are 0.
+__global__ void k_mask_to(const double* __restrict__ m, int NX, int NY, int kind, int comp, double* __restrict__ out,
+                          int multiply)
+{
+    const int np = NX - 1;
+    int nx, ny;
+    if (kind == ILH_NODES_PRIMAL) { nx = NX - 1; ny = NY - 1; }
+    else if (kind == ILH_NODES_DUAL) { nx = NX; ny = NY; }
+    else if (kind == 2) { nx = NX; ny = NY - 1; }     // u edges
+    else { nx = NX - 1; ny = NY; }                    // v edges
+    const int i = blockIdx.x * 64 + threadIdx.x % 64, j = blockIdx.y * 4 + threadIdx.x / 64;
+    if (i >= nx || j >= ny) return;
+    double v = 0.0;
+    if (kind == ILH_NODES_PRIMAL) {
+        v = m[i + (size_t)np * j];
+    } else if (kind == ILH_NODES_DUAL) {
+        if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2)
+            v = 0.25 * (m[i - 1 + (size_t)np * (j - 1)] + m[i + (size_t)np * (j - 1)] + m[i - 1 + (size_t)np * j] +
+                        m[i + (size_t)np * j]);
+    } else if (kind == 2) {
+        if (i >= 1 && i <= NX - 2) v = 0.5 * (m[i - 1 + (size_t)np * j] + m[i + (size_t)np * j]);
+    } else {
+        if (j >= 1 && j <= NY - 2) v = 0.5 * (m[i + (size_t)np * (j - 1)] + m[i + (size_t)np * j]);
+    }
+    if (comp) v = 1.0 - v;
+    double* o = out + i + (size_t)nx * j;
+    *o = multiply ? *o * v : v;
+}
+
+__global__ void k_fill_value(double* a, size_t n, double v)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) a[k] = v;
+}
+
+// mask(cache) = -L^-1 D_s 1 on Nodes{Primal} (src/surface_operators.jl:863-871);
+// 1 everywhere without surface points
+const double* primal_mask(ilh_plan* pl)
+{
+    if (pl->mask_gen == pl->gen) return pl->mask.as<double>();
+    const size_t ng = pl->g.n(GK_PRIMAL);
+    hipStream_t st = pl->ctx->stream;
+    pl->mask.ensure(sizeof(double) * ng);
+    double* m = pl->mask.as<double>();
+    if (pl->N == 0) {
+        k_fill_value<<<(unsigned)std::min<size_t>(4096, (ng + 255) / 256), 256, 0, st>>>(m, ng, 1.0);
+    } else {
+        need_margin(pl, "mask");
+        DevBuf ones, th;
+        ones.ensure(sizeof(double) * pl->N);
+        th.ensure(sizeof(double) * ng);
+        k_fill_value<<<(pl->N + 255) / 256, 256, 0, st>>>(ones.as<double>(), pl->N, 1.0);
+        scatter_op(pl, OP_DS, ones.as<double>(), th.as<double>());
+        conv_dense(pl, GK_PRIMAL, th.as<double>(), m);
+        axpby(st, m, m, 0.0, -1.0, ng);
+        ILH_HIP(hipStreamSynchronize(st));
+    }
+    pl->mask_gen = pl->gen;
+    return m;
+}
+
+size_t kind_len(const Grid& g, int kind)
+{
+    switch (kind) {
+    case ILH_NODES_PRIMAL: return g.n(GK_PRIMAL);
+    case ILH_NODES_DUAL: return g.n(GK_DUAL);
+    case 2: return g.n(GK_EDGE_U);
+    case 3: return g.n(GK_EDGE_V);
+    }
+    fail(ILH_ERR_ARG, "unknown grid kind");
+}
+
+int mask_call(ilh_plan* pl, int grid_kind, int complementary, double* w, int mem, bool multiply)
+{
+    return guarded_call([&] {
+        Io io(pl, mem);
+        if (grid_kind < ILH_NODES_PRIMAL || grid_kind > ILH_EDGES_PRIMAL) fail(ILH_ERR_ARG, "unknown grid kind");
+        const size_t n = grid_len(pl->g, grid_kind);
+        hipStream_t st = pl->ctx->stream;
+        double* d_w = io.out(w, n, pl->io_b);
+        if (multiply && mem == ILH_MEM_HOST)
+            ILH_HIP(hipMemcpyAsync(d_w, w, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        const double* m = primal_mask(pl);
+        const int NX = pl->g.NX, NY = pl->g.NY;
+        auto run = [&](int k, double* out) {
+            int nx, ny;
+            if (k == ILH_NODES_PRIMAL) { nx = NX - 1; ny = NY - 1; }
+            else if (k == ILH_NODES_DUAL) { nx = NX; ny = NY; }
+            else if (k == 2) { nx = NX; ny = NY - 1; }
+            else { nx = NX - 1; ny = NY; }
+            dim3 grid((nx + 63) / 64, (ny + 3) / 4);
+            k_mask_to<<<grid, 256, 0, st>>>(m, NX, NY, k, complementary ? 1 : 0, out, multiply ? 1 : 0);
+            ILH_HIP(hipGetLastError());
+        };
+        if (grid_kind == ILH_EDGES_PRIMAL) {
+            run(2, d_w);
+            run(3, d_w + kind_len(pl->g, 2));
+        } else {
+            run(grid_kind, d_w);
+        }
+        io.back(w, d_w, n);
+        io.done();
+    });
+}
+
+}  // namespace
+
+extern "C" int ilh_mask(ilh_plan* pl, int grid_kind, int complementary, double* out, int mem)
+{
+    if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+    return mask_call(pl, grid_kind, complementary, out, mem, false);
+}
+
+extern "C" int ilh_mask_multiply(ilh_plan* pl, int grid_kind, int complementary, double* w, int mem)
+{
+    if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+    return mask_call(pl, grid_kind, complementary, w, mem, true);
+}
+
 // ------------------------------------------------------ batched solves
 // Repeat Dirichlet solves of many independent plans of one context: the
 // plans' caches are prepared first (in order, on the context stream), then

@@ -157,6 +157,8 @@ struct ilh_plan {
     SolveCache* fac[4] = {};                  // ilh_schur_factor caches per kind
     MovingCache* mov = nullptr;               // moving-body Dirichlet solves (kept across update_points)
     unsigned gen = 0;                         // incremented whenever the geometry caches are dropped
+    ilh::DevBuf mask;                         // mask(cache) on Nodes{Primal} (built lazily)
+    unsigned mask_gen = ~0u;                  // generation the mask belongs to
 };
 
 namespace ilh {

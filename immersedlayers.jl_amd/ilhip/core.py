@@ -354,6 +354,47 @@ def inverse_laplacian(w, cache, rhs=None):
     return _run("ilh_inverse_laplacian", cache, w, src, w.ilh_kind)
 
 
+def _grid_kind_of(w):
+    if isinstance(w, Edges):
+        return L.EDGES_PRIMAL
+    if isinstance(w, Nodes):
+        return w.ilh_kind
+    raise TypeError("masks act on Nodes or Edges{Primal}")
+
+
+def _blank_like(cache, like):
+    if like is None:
+        return Nodes(Primal, cache.g)
+    return Nodes(like.kind, cache.g) if isinstance(like, Nodes) else Edges(cache.g)
+
+
+def mask(cache, like=None):
+    """mask(cache): 1 inside the surfaces (opposite the normals), 0 outside, as
+    Nodes{Primal}, or on the grid kind of `like` (src/surface_operators.jl:737)."""
+    w = _blank_like(cache, like)
+    L.call("ilh_mask", cache.h, _grid_kind_of(w), 0, _ptr(w.flat()), L.MEM_HOST)
+    return w
+
+
+def complementary_mask(cache, like=None):
+    """complementary_mask(cache) = 1 - mask(cache) (src/surface_operators.jl:749)."""
+    w = _blank_like(cache, like)
+    L.call("ilh_mask", cache.h, _grid_kind_of(w), 1, _ptr(w.flat()), L.MEM_HOST)
+    return w
+
+
+def mask_(w, cache):
+    """mask!(w, cache): w times the mask, in place (src/surface_operators.jl:785)."""
+    L.call("ilh_mask_multiply", cache.h, _grid_kind_of(w), 0, _ptr(w.flat()), L.MEM_HOST)
+    return w
+
+
+def complementary_mask_(w, cache):
+    """complementary_mask!(w, cache): w times 1 - mask, in place."""
+    L.call("ilh_mask_multiply", cache.h, _grid_kind_of(w), 1, _ptr(w.flat()), L.MEM_HOST)
+    return w
+
+
 def _schur(cache, kind, scale):
     A = np.zeros(cache.N * cache.N)
     L.call("ilh_schur_build", cache.h, kind, C.c_double(scale), _ptr(A), L.MEM_HOST)

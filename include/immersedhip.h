@@ -193,6 +193,16 @@ int ilh_batch_destroy(ilh_batch* batch);
 int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
                                double* s, int mem, double tol, int max_iter, int* iters);
 
+/* ---- masks (src/surface_operators.jl:728-871) ----
+ * mask(cache) = -L^-1 D_s 1 on Nodes{Primal} (1 inside the surfaces, on the side
+ * opposite the normals; 1 everywhere without points); on Nodes{Dual} and
+ * Edges{Primal} the primal mask averaged onto them (grid_interpolate!).
+ * ilh_mask writes the mask (complementary != 0: 1 - mask, i.e.
+ * complementary_mask) of grid_kind into out; ilh_mask_multiply multiplies w in
+ * place (mask! / complementary_mask!). */
+int ilh_mask(ilh_plan* plan, int grid_kind, int complementary, double* out, int mem);
+int ilh_mask_multiply(ilh_plan* plan, int grid_kind, int complementary, double* w, int mem);
+
 /* ---- unbounded heat conduction with Dirichlet bodies (config: IF time marching) ----
  * The constrained ODE system of test/literate/heatconduction.jl:88-180,
  *   dT/dt = kappa L T - kappa D_s [T] + R sigma,   E T = (T_b+ + T_b-)/2,

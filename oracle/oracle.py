@@ -336,3 +336,36 @@ def heat_advance(g, geo, kappa, dt, nsteps, T0, tb_plus, tb_minus=None, ddf_kind
     if info:
         raise np.linalg.LinAlgError("singular stage Schur matrix")
     return T
+
+
+def mask(g, geo, complementary=False, ddf_kind=YANG3):
+    """mask(cache) on Nodes{Primal}: -L^-1 D_s 1 (src/surface_operators.jl:863-868);
+    ones without points (:870-873); complementary_mask is 1 - mask."""
+    if geo is None or geo.N == 0:
+        m = np.ones(nel(g, PRIMAL))
+    else:
+        th = surface_divergence(g, geo, np.ones(geo.N), ddf_kind)
+        m = -InverseLaplacian(g, PRIMAL)(th)
+    return 1.0 - m if complementary else m
+
+
+def primal_to(g, m, kind):
+    """Nodes{Primal} data averaged onto another grid kind, as CartesianGrids'
+    grid_interpolate! (used by src/surface_operators.jl:882-900): a dual node from
+    its four surrounding primal nodes, an x edge (EDGE_U) from the two primal nodes
+    beside it in x, a y edge (EDGE_V) from the two in y.  The outer ring, which
+    has no surrounding nodes, is 0."""
+    NX, NY = g.NX, g.NY
+    p = np.asarray(m, float).reshape((NX - 1, NY - 1), order="F")
+    if kind == PRIMAL:
+        return p.ravel(order="F").copy()
+    if kind == DUAL:
+        out = np.zeros((NX, NY))
+        out[1:-1, 1:-1] = 0.25 * (p[:-1, :-1] + p[1:, :-1] + p[:-1, 1:] + p[1:, 1:])
+    elif kind == EDGE_U:
+        out = np.zeros((NX, NY - 1))
+        out[1:-1, :] = 0.5 * (p[:-1, :] + p[1:, :])
+    else:
+        out = np.zeros((NX - 1, NY))
+        out[:, 1:-1] = 0.5 * (p[:, :-1] + p[:, 1:])
+    return out.ravel(order="F")

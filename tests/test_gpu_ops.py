@@ -208,3 +208,25 @@ def test_inverse_laplacian_periods(ctx, NX, NY):
         il.inverse_laplacian(out, cache, rhs=il.Nodes(kind, gs, w))
         ref = O.InverseLaplacian(og, okind)(w.ravel(order="F"))
         assert rel_err(out.flat(), ref) < 1e-13, (kind, NX, NY)
+
+
+def test_masks(setup):
+    """mask / complementary_mask / mask! (src/surface_operators.jl:728-900) on all
+    grid kinds against the oracle's -L^-1 D_s 1 and its averaging."""
+    il, cache, gs, og, geo, rng = setup
+    m = O.mask(og, geo)
+    likes = [(None, O.PRIMAL), (il.Nodes(il.Dual, gs), O.DUAL), (il.Edges(gs), None)]
+    for like, okind in likes:
+        ref = (np.concatenate([O.primal_to(og, m, O.EDGE_U), O.primal_to(og, m, O.EDGE_V)]) if okind is None
+               else O.primal_to(og, m, okind))
+        got = il.mask(cache, like)
+        assert max_rel(got.flat(), ref) < 1e-12, okind
+        assert max_rel(il.complementary_mask(cache, like).flat(), 1.0 - ref) < 1e-12, okind
+        w = rng.standard_normal(ref.size)
+        x = il.mask(cache, like)
+        x.flat()[:] = w
+        il.mask_(x, cache)
+        assert max_rel(x.flat(), w * ref) < 1e-12, okind
+        x.flat()[:] = w
+        il.complementary_mask_(x, cache)
+        assert max_rel(x.flat(), w * (1.0 - ref)) < 1e-12, okind

@@ -391,3 +391,48 @@ def test_heat_constraint_and_time_convergence():
     sols = [O.heat_advance(g, geo, kappa, tend / k, k, T0, tb) for k in (2, 4, 8)]
     ratio = np.linalg.norm(sols[0] - sols[2]) / np.linalg.norm(sols[1] - sols[2])
     assert ratio > 4.0, ratio
+
+
+# ------------------------------------------------------------------- masks
+def _mask_setup():
+    from ilhip import geometry as G
+    # test/surface_ops.jl:4-12: dx = 0.04 on [-2,2]^2, unit circle, points at 1.4 dx
+    gs = G.GridSpec(106, 106, 53, 53, 0.04)
+    s = G.circle(1.0, 1.4 * gs.dx)
+    return gs, O.grid(gs.NX, gs.NY, gs.I0x, gs.I0y, gs.dx), O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+
+
+def test_mask_area_pins():
+    """test/surface_ops.jl:180-206: the mask integrates to pi R^2 (2e-3) and the
+    masked ones on primal nodes, dual nodes and both edge families to pi (1e-3);
+    the complementary mask is 1 - mask; no points gives ones."""
+    gs, og, geo = _mask_setup()
+    m = O.mask(og, geo)
+    dA = gs.dx ** 2
+    assert abs(m.sum() * dA - np.pi) < 2e-3
+    for kind in (O.PRIMAL, O.DUAL, O.EDGE_U, O.EDGE_V):
+        assert abs(O.primal_to(og, m, kind).sum() * dA - np.pi) < 1e-3, kind
+    # 1 deep inside, 0 far outside
+    P = m.reshape(gs.primal_shape(), order="F")
+    assert abs(P[52, 52] - 1) < 1e-3 and abs(P[5, 5]) < 1e-3
+    assert np.array_equal(O.mask(og, geo, complementary=True), 1.0 - m)
+    assert np.array_equal(O.mask(og, None), np.ones(O.nel(og, O.PRIMAL)))
+
+
+def test_primal_to_is_the_averaging_of_a_linear_field():
+    """grid_interpolate! of a linear field is that field at the target points."""
+    from ilhip import geometry as G
+    gs = G.GridSpec(23, 17, 11, 8, 0.1)
+    og = O.grid(gs.NX, gs.NY, gs.I0x, gs.I0y, gs.dx)
+    lin = lambda x, y: 0.3 + 2.0 * x - 1.5 * y
+    xp, yp = np.meshgrid(gs.x_primal(), gs.y_primal(), indexing="ij")
+    m = lin(xp, yp).ravel(order="F")
+    xd, yd = np.meshgrid(gs.x_dual(), gs.y_dual(), indexing="ij")
+    d = O.primal_to(og, m, O.DUAL).reshape(gs.dual_shape(), order="F")
+    assert np.allclose(d[1:-1, 1:-1], lin(xd, yd)[1:-1, 1:-1], atol=1e-13)
+    xu, yu = np.meshgrid(gs.x_dual(), gs.y_primal(), indexing="ij")
+    u = O.primal_to(og, m, O.EDGE_U).reshape((gs.NX, gs.NY - 1), order="F")
+    assert np.allclose(u[1:-1], lin(xu, yu)[1:-1], atol=1e-13)
+    xv, yv = np.meshgrid(gs.x_primal(), gs.y_dual(), indexing="ij")
+    v = O.primal_to(og, m, O.EDGE_V).reshape((gs.NX - 1, gs.NY), order="F")
+    assert np.allclose(v[:, 1:-1], lin(xv, yv)[:, 1:-1], atol=1e-13)
