@@ -79,10 +79,28 @@ struct MovingCache {
     bool have_ref = false;
     int ref_age = 0;           // solves since the reference was factored
     DevBuf terms1, amp, V, w, z, x, b, h, fbar, dvec, grid;
+    DevBuf S;                  // create_RTLinvR at the current geometry, row-major
+    bool S_ok = false;
+    DevBuf gm, gmf;            // GMRES state and flags on the device
+    int last_iters = 4;        // steps of the previous solve: the first chunk enqueued
+    double* hpin = nullptr;    // pinned host buffer for the GMRES flags
+    size_t hpin_n = 0;
+    double* pinned(size_t n)
+    {
+        if (n > hpin_n) {
+            if (hpin) (void)hipHostFree(hpin);
+            hpin = nullptr;
+            hpin_n = 0;
+            ILH_HIP(hipHostMalloc((void**)&hpin, sizeof(double) * n, hipHostMallocDefault));
+            hpin_n = n;
+        }
+        return hpin;
+    }
     SourceSet src1;
     bool src_ok = false;
     void drop_sources()
     {
+        S_ok = false;
         if (src_ok) {
             free_bins(src1.bins);
             if (src1.pinfo) (void)hipFree(src1.pinfo);
@@ -91,7 +109,11 @@ struct MovingCache {
         }
         src_ok = false;
     }
-    ~MovingCache() { drop_sources(); }
+    ~MovingCache()
+    {
+        drop_sources();
+        if (hpin) (void)hipHostFree(hpin);
+    }
 };
 
 static void drop_caches(ilh_plan* pl)
@@ -1084,25 +1106,6 @@ extern "C" int ilh_batch_destroy(ilh_batch* b)
 // ------------------------------------------------- moving-body Dirichlet
 namespace {
 
-// out[i] = sum_k V[k + n i] w[k], i < nv: one work-group per i, fixed order
-__global__ void __launch_bounds__(256) k_mdot(const double* __restrict__ V, int nv, const double* __restrict__ w,
-                                              int n, double* __restrict__ out)
-{
-    __shared__ double red[256];
-    const int i = blockIdx.x;
-    if (i >= nv) return;
-    const double* v = V + (size_t)n * i;
-    double acc = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) acc += v[k] * w[k];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[i] = red[0];
-}
-
 // w[k] -= sum_{i < nv} V[k + n i] h[i]
 __global__ void k_mupdate(const double* __restrict__ V, int nv, const double* __restrict__ h, int n,
                           double* __restrict__ w)
@@ -1112,6 +1115,164 @@ __global__ void k_mupdate(const double* __restrict__ V, int nv, const double* __
     double acc = w[k];
     for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)n * i] * h[i];
     w[k] = acc;
+}
+
+// GMRES state on the device (m = restart length):
+//   H (m+1) x m column-major, cs[m], sn[m], g[m+1], then bnorm and tol;
+//   flags[0]: 0 running, 1 converged, 2 not converged within m; flags[1]: steps
+struct GmLayout {
+    int m;
+    __host__ __device__ size_t H() const { return 0; }
+    __host__ __device__ size_t cs() const { return (size_t)(m + 1) * m; }
+    __host__ __device__ size_t sn() const { return cs() + m; }
+    __host__ __device__ size_t g() const { return sn() + m; }
+    __host__ __device__ size_t bnorm() const { return g() + m + 1; }
+    __host__ __device__ size_t tol() const { return bnorm() + 1; }
+    __host__ __device__ size_t size() const { return tol() + 1; }
+};
+
+// block-wide sum over 1024 threads (fixed order)
+__device__ __forceinline__ double block_sum_1024(double a, double* red)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    __syncthreads();
+    if (lane == 0) red[wave] = a;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < 16; ++i) t += red[i];
+    return t;
+}
+
+// r0 = w: beta = |r0|, bnorm = |b|, v_0 = r0 / beta, g = (beta, 0, ...);
+// converged at once when b = 0 or beta <= tol |b|
+__global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, const double* __restrict__ b, int n,
+                                                  double* __restrict__ V, double* __restrict__ st, int m, double tol,
+                                                  int* __restrict__ flags)
+{
+    __shared__ double red[16];
+    const GmLayout L{m};
+    double a = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < n; k += 1024) {
+        a += w[k] * w[k];
+        c += b[k] * b[k];
+    }
+    const double beta = sqrt(block_sum_1024(a, red));
+    const double bn = sqrt(block_sum_1024(c, red));
+    const bool done = bn == 0.0 || beta <= tol * bn;
+    if (threadIdx.x == 0) {
+        st[L.g()] = beta;
+        for (int i = 1; i <= m; ++i) st[L.g() + i] = 0.0;
+        st[L.bnorm()] = bn;
+        st[L.tol()] = tol;
+        flags[0] = done ? 1 : 0;
+        flags[1] = 0;
+    }
+    if (!done) {
+        const double inv = 1.0 / beta;
+        for (int k = threadIdx.x; k < n; k += 1024) V[k] = w[k] * inv;
+    }
+}
+
+// y = M x (M row-major, one wave per row) unless the iteration has stopped
+__global__ void k_gm_matvec(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
+                            int n, const int* __restrict__ flags)
+{
+    if (flags[0]) return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const double s = wave_dot(M + (size_t)ld * i, x, n, lane);
+    if (lane == 0) y[i] = s;
+}
+
+// Arnoldi step j (one work-group, unless the iteration has stopped): w is
+// orthogonalised against v_0..v_j by two passes of classical Gram-Schmidt and
+// normalised into v_{j+1}; column j of H gets the summed coefficients and
+// |w|; the previous Givens rotations are applied, a new one zeroes
+// H[j+1, j], and |g[j+1]| (the residual norm) is tested against tol |b|.
+// Every sum has a fixed order (deterministic).
+__global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int j, double* __restrict__ w, int n,
+                                                     double* __restrict__ st, int m, int* __restrict__ flags)
+{
+    __shared__ double h[2][256];
+    __shared__ double red[16];
+    if (flags[0]) return;
+    const GmLayout L{m};
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nv = j + 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        // dots: wave i % 16 owns basis vector i
+        for (int i = wave; i < nv; i += 16) {
+            const double* v = V + (size_t)n * i;
+            double a0 = 0.0, a1 = 0.0;
+            int k = lane;
+            for (; k + 64 < n; k += 128) {
+                a0 += v[k] * w[k];
+                a1 += v[k + 64] * w[k + 64];
+            }
+            if (k < n) a0 += v[k] * w[k];
+            double acc = a0 + a1;
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (lane == 0) h[pass][i] = acc;
+        }
+        __syncthreads();
+        // w -= V h
+        for (int k = tid; k < n; k += 1024) {
+            double acc = w[k];
+            for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)n * i] * h[pass][i];
+            w[k] = acc;
+        }
+        __syncthreads();
+    }
+    double a = 0.0;
+    for (int k = tid; k < n; k += 1024) a += w[k] * w[k];
+    const double hn = sqrt(block_sum_1024(a, red));
+    if (hn > 0.0) {
+        double* vn = V + (size_t)n * nv;
+        const double inv = 1.0 / hn;
+        for (int k = tid; k < n; k += 1024) vn[k] = w[k] * inv;
+    }
+    if (tid == 0) {
+        double* H = st + L.H() + (size_t)(m + 1) * j;
+        double* cs = st + L.cs();
+        double* sn = st + L.sn();
+        double* g = st + L.g();
+        for (int i = 0; i <= j; ++i) H[i] = h[0][i] + h[1][i];
+        H[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double t = cs[i] * H[i] + sn[i] * H[i + 1];
+            H[i + 1] = -sn[i] * H[i] + cs[i] * H[i + 1];
+            H[i] = t;
+        }
+        const double r = hypot(H[j], H[j + 1]);
+        cs[j] = r == 0.0 ? 1.0 : H[j] / r;
+        sn[j] = r == 0.0 ? 0.0 : H[j + 1] / r;
+        H[j] = r;
+        H[j + 1] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        if (fabs(g[j + 1]) <= st[L.tol()] * st[L.bnorm()] || hn == 0.0) {
+            flags[0] = 1;
+            flags[1] = j + 1;
+        } else if (j + 1 == m) {
+            flags[0] = 2;
+            flags[1] = m;
+        }
+    }
+}
+
+// y = H^-1 g (upper triangular, j x j), written negated into ny[0..j)
+__global__ void k_gm_backsolve(const double* __restrict__ st, int m, int j, double* __restrict__ ny)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const GmLayout L{m};
+    const double* H = st + L.H();
+    const double* g = st + L.g();
+    for (int i = j - 1; i >= 0; --i) {
+        double t = g[i];
+        for (int k = i + 1; k < j; ++k) t += H[i + (size_t)(m + 1) * k] * ny[k];   // ny = -y
+        ny[i] = -t / H[i + (size_t)(m + 1) * i];
+    }
 }
 
 // d = f+ - f-,  fbar = (f+ + f-)/2
@@ -1125,12 +1286,6 @@ __global__ void k_jump_avg(const double* __restrict__ fp, const double* __restri
     fbar[k] = 0.5 * (a + b);
 }
 
-__global__ void k_scale(double* __restrict__ y, const double* __restrict__ x, double a, int n)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) y[k] = a * x[k];
-}
-
 __global__ void k_identity(double* __restrict__ A, int n)   // A (zeroed) += I, column-major n x n
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1139,19 +1294,11 @@ __global__ void k_identity(double* __restrict__ A, int n)   // A (zeroed) += I, 
 
 }  // namespace
 
-// S v = -E L^-1 R v (src/matrix_operators.jl:16-26) applied matrix-free: the
-// R half of the merged sources carries v, the D_s half zero.
+// S v with S = -E L^-1 R (create_RTLinvR, src/matrix_operators.jl:9-30) at
+// the current geometry, assembled row-major from the LGF table
 static void schur_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* out)
 {
-    const int N = pl->N;
-    hipStream_t st = pl->ctx->stream;
-    double* amp = mc->amp.as<double>();
-    ILH_HIP(hipMemcpyAsync(amp, v, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    fill_zero(st, amp + N, N);
-    double* g = mc->grid.as<double>();
-    conv_sources(pl, GK_PRIMAL, mc->src1, amp, g);
-    gather_op(pl, OP_E_PRIMAL, g, out);
-    k_scale<<<(N + 255) / 256, 256, 0, st>>>(out, out, -1.0, N);
+    matvec_rows(pl->ctx->stream, mc->S.as<double>(), solve_ld(pl->N), v, out, pl->N, pl->N, 1.0, 0);
 }
 
 // K (N x N column-major, overwritten by its LU factors) -> Minv = K^-1,
@@ -1187,88 +1334,62 @@ static void moving_reference(ilh_plan* pl, MovingCache* mc)
 }
 
 // GMRES(m) on S x = b, right-preconditioned by Pinv, classical Gram-Schmidt
-// with one re-orthogonalisation; the small Hessenberg problem (Givens
-// rotations) runs on the host.  x0 = Pinv b.  Returns the iteration count, or
-// -1 if the relative residual did not fall below tol within m iterations.
+// with one re-orthogonalisation; x0 = Pinv b.  The Arnoldi steps, Givens
+// rotations and the residual test run on the device; the host enqueues steps
+// in chunks (the previous solve's count first) and reads back two flags after
+// each chunk, steps after convergence returning at once.  Returns the
+// iteration count, or -1 if the relative residual did not fall below tol
+// within m iterations.
 static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* x, double tol, int m)
 {
     const int N = pl->N, ld = solve_ld(N);
     hipStream_t st = pl->ctx->stream;
-    const unsigned gN = (unsigned)((N + 255) / 256);
+    const unsigned gR = (unsigned)((N + 3) / 4);
+    const GmLayout L{m};
+    mc->gm.ensure(sizeof(double) * L.size());
+    mc->gmf.ensure(sizeof(int) * 2);
     double* V = mc->V.as<double>();
     double* w = mc->w.as<double>();
     double* z = mc->z.as<double>();
-    double* hd = mc->h.as<double>();
+    double* gm = mc->gm.as<double>();
+    int* flags = mc->gmf.as<int>();
+    int* hflags = (int*)mc->pinned(1);
     // x0 = P b, r0 = b - S x0
     matvec_rows(st, mc->Pinv.as<double>(), ld, b, x, N, N, 1.0, 0);
     schur_apply(pl, mc, x, w);
-    axpby(st, w, b, 1.0, -1.0, N);                     // w = b - S x0
-    double hb[2];
-    k_mdot<<<1, 256, 0, st>>>(w, 1, w, N, hd);
-    k_mdot<<<1, 256, 0, st>>>(b, 1, b, N, hd + 1);
-    ILH_HIP(hipMemcpyAsync(hb, hd, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    ILH_HIP(hipStreamSynchronize(st));
-    const double beta = std::sqrt(hb[0]), bnorm = std::sqrt(hb[1]);
-    if (bnorm == 0.0 || beta <= tol * bnorm) return 0;
-    k_scale<<<gN, 256, 0, st>>>(V, w, 1.0 / beta, N);
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0), hh(3 * (m + 1));
-    g[0] = beta;
-    int j = 0;
-    bool ok = false;
-    for (; j < m; ++j) {
-        matvec_rows(st, mc->Pinv.as<double>(), ld, V + (size_t)N * j, z, N, N, 1.0, 0);
-        schur_apply(pl, mc, z, w);
-        // two passes of classical Gram-Schmidt against v_0..v_j, then |w|^2;
-        // one read-back of all three
-        k_mdot<<<j + 1, 256, 0, st>>>(V, j + 1, w, N, hd);
-        k_mupdate<<<gN, 256, 0, st>>>(V, j + 1, hd, N, w);
-        k_mdot<<<j + 1, 256, 0, st>>>(V, j + 1, w, N, hd + (m + 1));
-        k_mupdate<<<gN, 256, 0, st>>>(V, j + 1, hd + (m + 1), N, w);
-        k_mdot<<<1, 256, 0, st>>>(w, 1, w, N, hd + 2 * (m + 1));
-        ILH_HIP(hipMemcpyAsync(hh.data(), hd, sizeof(double) * 3 * (m + 1), hipMemcpyDeviceToHost, st));
+    axpby(st, w, b, 1.0, -1.0, N);
+    k_gm_init<<<1, 1024, 0, st>>>(w, b, N, V, gm, m, tol, flags);
+    ILH_HIP(hipGetLastError());
+    int j = 0, state = 0;
+    int chunk = std::max(1, std::min(m, mc->last_iters));
+    while (j < m) {
+        const int j1 = std::min(m, j + chunk);
+        for (; j < j1; ++j) {
+            k_gm_matvec<<<gR, 256, 0, st>>>(mc->Pinv.as<double>(), ld, V + (size_t)N * j, z, N, flags);
+            k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, z, w, N, flags);
+            k_gm_arnoldi<<<1, 1024, 0, st>>>(V, j, w, N, gm, m, flags);
+        }
+        ILH_HIP(hipGetLastError());
+        ILH_HIP(hipMemcpyAsync(hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         ILH_HIP(hipStreamSynchronize(st));
-        const double hn = std::sqrt(hh[2 * (m + 1)]);
-        for (int i = 0; i <= j; ++i) H[i + (size_t)(m + 1) * j] = hh[i] + hh[(m + 1) + i];
-        H[j + 1 + (size_t)(m + 1) * j] = hn;
-        // apply the previous rotations, then a new one zeroing H[j+1, j]
-        for (int i = 0; i < j; ++i) {
-            double& a = H[i + (size_t)(m + 1) * j];
-            double& c = H[i + 1 + (size_t)(m + 1) * j];
-            const double t = cs[i] * a + sn[i] * c;
-            c = -sn[i] * a + cs[i] * c;
-            a = t;
-        }
-        double& a = H[j + (size_t)(m + 1) * j];
-        double& c = H[j + 1 + (size_t)(m + 1) * j];
-        const double r = std::hypot(a, c);
-        cs[j] = r == 0.0 ? 1.0 : a / r;
-        sn[j] = r == 0.0 ? 0.0 : c / r;
-        a = r;
-        c = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        if (hn > 0.0) k_scale<<<gN, 256, 0, st>>>(V + (size_t)N * (j + 1), w, 1.0 / hn, N);
-        if (std::fabs(g[j + 1]) <= tol * bnorm || hn == 0.0) {
-            ok = true;
-            ++j;
-            break;
-        }
+        state = hflags[0];
+        if (state) break;
+        chunk = 2;
     }
-    if (!ok) return -1;
-    // y = H^-1 g (upper triangular j x j); x += P (V y)
-    std::vector<double> y(j, 0.0);
-    for (int i = j - 1; i >= 0; --i) {
-        double t = g[i];
-        for (int k = i + 1; k < j; ++k) t -= H[i + (size_t)(m + 1) * k] * y[k];
-        y[i] = t / H[i + (size_t)(m + 1) * i];
+    if (state != 1) return -1;
+    const int steps = hflags[1];
+    if (steps > 0) {
+        mc->last_iters = steps;
+        // x += P (V y)
+        double* ny = mc->h.as<double>();
+        k_gm_backsolve<<<1, 64, 0, st>>>(gm, m, steps, ny);
+        fill_zero(st, w, N);
+        k_mupdate<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(V, steps, ny, N, w);
+        matvec_rows(st, mc->Pinv.as<double>(), ld, w, x, N, N, 1.0, 1);
+        ILH_HIP(hipGetLastError());
     }
-    for (int i = 0; i < j; ++i) hh[i] = -y[i];
-    ILH_HIP(hipMemcpyAsync(hd, hh.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
-    fill_zero(st, w, N);
-    k_mupdate<<<gN, 256, 0, st>>>(V, j, hd, N, w);     // w = V y
-    matvec_rows(st, mc->Pinv.as<double>(), ld, w, x, N, N, 1.0, 1);
     ILH_HIP(hipStreamSynchronize(st));
-    return j;
+    return steps;
 }
 
 extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s,
@@ -1304,6 +1425,16 @@ extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const 
             mc->h.ensure(sizeof(double) * 3 * (size_t)(m + 1));
             mc->grid.ensure(sizeof(double) * ng);
             if (!mc->have_ref) moving_reference(pl, mc);
+            if (!mc->S_ok) {
+                const int ld = solve_ld(N);
+                mc->S.ensure(sizeof(double) * (size_t)N * ld);
+                DevBuf K;
+                K.ensure(sizeof(double) * (size_t)N * N);
+                build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, K.as<double>());
+                transpose(st, K.as<double>(), mc->S.as<double>(), N, N, ld);
+                ILH_HIP(hipStreamSynchronize(st));
+                mc->S_ok = true;
+            }
             // f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
             double* amp = mc->amp.as<double>();
             k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, mc->dvec.as<double>(), mc->fbar.as<double>());

@@ -218,6 +218,40 @@ void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, doubl
 // row-major solve matrices use a leading dimension padded to a multiple of 32
 inline int solve_ld(int n) { return (n + 31) / 32 * 32; }
 void transpose(hipStream_t st, const double* A, double* At, int rows, int cols, int ldt);
+// dot(row, x) over m entries by one wave of 64 lanes (16-byte loads when x is
+// 16-byte aligned), the same fixed order on every call
+__device__ __forceinline__ double wave_dot(const double* __restrict__ row, const double* __restrict__ x, int m, int lane)
+{
+    if (((uintptr_t)x & 15) != 0) {   // caller's vector not 16-byte aligned: plain loads
+        double s = 0.0;
+        for (int k = lane; k < m; k += 64) s += row[k] * x[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        return s;
+    }
+    // 16-byte loads, four independent partial sums per lane
+    const double2* r2 = (const double2*)row;
+    const double2* x2 = (const double2*)x;
+    const int m2 = m >> 1;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = lane;
+    for (; k + 192 < m2; k += 256) {
+        const double2 a0 = r2[k], a1 = r2[k + 64], a2 = r2[k + 128], a3 = r2[k + 192];
+        const double2 b0 = x2[k], b1 = x2[k + 64], b2 = x2[k + 128], b3 = x2[k + 192];
+        s0 += a0.x * b0.x + a0.y * b0.y;
+        s1 += a1.x * b1.x + a1.y * b1.y;
+        s2 += a2.x * b2.x + a2.y * b2.y;
+        s3 += a3.x * b3.x + a3.y * b3.y;
+    }
+    for (; k < m2; k += 64) {
+        const double2 a0 = r2[k], b0 = x2[k];
+        s0 += a0.x * b0.x + a0.y * b0.y;
+    }
+    if ((m & 1) && lane == 0) s1 += row[m - 1] * x[m - 1];
+    double s = (s0 + s1) + (s2 + s3);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
 void matvec_rows(hipStream_t st, const double* Mrow, int ld, const double* x, double* y, int n, int m,
                  double alpha, int accumulate);
 void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, int ld, const double* fp,

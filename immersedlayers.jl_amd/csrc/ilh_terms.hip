@@ -350,6 +350,7 @@ __global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict_
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int tot[256];
     const int j = blockIdx.x;
+    if (ptr[j] == ptr[j + 1]) return;   // no entries: no cells (cnt[j] stays 0)
     mark_cells(ptr, ebx, erw, j, nx, bm);
     int c = 0;
     for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) c += __popc(bm[w]);
@@ -367,14 +368,26 @@ __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__
 {
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int woff[CELL_WORDS];
+    __shared__ int wsum[4];
     const int j = blockIdx.x;
+    if (ptr[j] == ptr[j + 1]) return;   // no entries: no cells
     mark_cells(ptr, ebx, erw, j, nx, bm);
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int w = 0; w < CELL_WORDS; ++w) {
-            woff[w] = s;
-            s += __popc(bm[w]);
+    // exclusive scan of the popcounts of words 0..255 (256 threads, 4 waves);
+    // word 256 (x >= 8192) follows them
+    {
+        const int t = threadIdx.x, lane = t & 63;
+        const int c = __popc(bm[t]);
+        int v = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(v, off, 64);
+            if (lane >= off) v += u;
         }
+        if (lane == 63) wsum[t >> 6] = v;
+        __syncthreads();
+        int base = 0;
+        for (int i = 0; i < (t >> 6); ++i) base += wsum[i];
+        woff[t] = base + v - c;
+        if (t == 255) woff[256] = base + v;
     }
     __syncthreads();
     const int lo = ptr[j], hi = ptr[j + 1];
