@@ -159,7 +159,7 @@ int ilh_dirichlet_solve(ilh_plan* plan, const double* fb_plus, const double* fb_
 int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_minus,
                       double* f, double* df, double* s, double* ds, int mem);
 
-/* ---- measurement helpers (no reference counterpart) ---- */
+/* ---- batched forms over arrays of plans ---- */
 /* Batched form of ilh_dirichlet_solve: nprob independent plans of one context
    (any mix of grid sizes and point counts), array k of each pointer array
    belonging to plans[k]; fb_minus may be NULL, or hold NULL entries.  The
@@ -186,8 +186,9 @@ int ilh_batch_destroy(ilh_batch* batch);
    but the Schur system of the current geometry is solved by GMRES (at most
    max_iter iterations, relative residual tol), right-preconditioned by the
    inverse Schur matrix of a reference geometry that the plan keeps across
-   ilh_plan_update_points; S is applied matrix-free (one LGF convolution per
-   iteration).  The first call, and any call whose GMRES does not converge,
+   ilh_plan_update_points; the current geometry's S (create_RTLinvR,
+   src/matrix_operators.jl:9-30) is assembled directly and applied as a dense
+   matvec.  The first call, and any call whose GMRES does not converge,
    factors the current geometry and makes it the reference.  *iters (may be
    NULL) receives the GMRES iterations (max_iter + k after a refactorisation). */
 int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
@@ -220,7 +221,8 @@ int ilh_heat_set_boundary(ilh_heat* heat, const double* tb_plus, const double* t
 int ilh_heat_advance(ilh_heat* heat, double* T, int nsteps, int mem);
 int ilh_heat_destroy(ilh_heat* heat);
 
-/* HIP-event timings on the plan's stream, in ms per call, of the Dirichlet
+/* ---- measurement helper (no reference counterpart) ----
+   HIP-event timings on the plan's stream, in ms per call, of the Dirichlet
    repeat solve on device buffers: *total_ms = `reps` whole solves back to back
    (two events around the loop); if stage_ms is not NULL, stage_ms[k] = `reps`
    launches of stage k alone (0 = surface matvec, 1 = row FFT of the sources,
