@@ -1428,11 +1428,9 @@ extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const 
             if (!mc->S_ok) {
                 const int ld = solve_ld(N);
                 mc->S.ensure(sizeof(double) * (size_t)N * ld);
-                DevBuf K;
-                K.ensure(sizeof(double) * (size_t)N * N);
-                build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, K.as<double>());
-                transpose(st, K.as<double>(), mc->S.as<double>(), N, N, ld);
-                ILH_HIP(hipStreamSynchronize(st));
+                // create_RTLinvR (scale 1), stored row-major for the matvecs
+                assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -pl->g.dx * pl->g.dx, false, mc->S.as<double>(), ld,
+                         true);
                 mc->S_ok = true;
             }
             // f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)

@@ -25,29 +25,91 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // ------------------------------------------------------------ assembly
 static constexpr int AT = 16;   // points per assembly tile edge
 
+static constexpr int GBOX = 4096;   // doubles of the kernel table staged per tile (32 KB)
+
+// One 16x16 tile of (p, q) entries per work-group.  The table entries a tile
+// touches lie in a box of |dbx + u|, |dby + v| whose sides are the two tiles'
+// extents plus the 9-wide window (points of a tile are neighbours on a
+// curve), independent of how far apart the two tiles are; the box is staged
+// in LDS when it fits, and the table is read from memory otherwise.
+// ROWMAJOR: K[q + ldk p] (lanes along q: coalesced stores), else K[p + ldk q].
+template <bool ROWMAJOR>
 __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ TA, int TAn, const Term* __restrict__ TB,
                                                       int TBn, int N, const double* __restrict__ G, int Tn,
-                                                      double factor, int delta, double* __restrict__ K)
+                                                      double factor, int delta, double* __restrict__ K, int ldk)
 {
     __shared__ Term sa[AT * 2], sb[AT * 2];
+    __shared__ double sG[GBOX];
+    __shared__ int sbox[4];
     const int p0 = blockIdx.x * AT, q0 = blockIdx.y * AT;
     const int tid = threadIdx.x + AT * threadIdx.y;
-    if (tid < AT * TAn) {
+    const int na = AT * TAn, nb = AT * TBn;
+    if (tid < na) {
         const int p = p0 + tid / TAn;
         if (p < N) sa[tid] = TA[(size_t)p * TAn + tid % TAn];
-    } else if (tid >= 64 && tid < 64 + AT * TBn) {
+    } else if (tid >= 64 && tid < 64 + nb) {
         const int k = tid - 64;
         const int q = q0 + k / TBn;
         if (q < N) sb[k] = TB[(size_t)q * TBn + k % TBn];
     }
     __syncthreads();
-    const int p = p0 + threadIdx.x, q = q0 + threadIdx.y;
+    if (!delta && tid < 64) {
+        // extents of the two tiles' window bases (wave 0: lanes < 32 the A
+        // terms, lanes >= 32 the B terms)
+        const int lane = tid;
+        int ax0 = 1 << 30, ax1 = -(1 << 30), ay0 = 1 << 30, ay1 = -(1 << 30);
+        int bx0 = 1 << 30, bx1 = -(1 << 30), by0 = 1 << 30, by1 = -(1 << 30);
+        if (lane < 32) {
+            if (lane < na && p0 + lane / TAn < N) { ax0 = ax1 = sa[lane].bx; ay0 = ay1 = sa[lane].by; }
+        } else {
+            const int k = lane - 32;
+            if (k < nb && q0 + k / TBn < N) { bx0 = bx1 = sb[k].bx; by0 = by1 = sb[k].by; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            ax0 = min(ax0, __shfl_xor(ax0, off, 64)); ax1 = max(ax1, __shfl_xor(ax1, off, 64));
+            ay0 = min(ay0, __shfl_xor(ay0, off, 64)); ay1 = max(ay1, __shfl_xor(ay1, off, 64));
+            bx0 = min(bx0, __shfl_xor(bx0, off, 64)); bx1 = max(bx1, __shfl_xor(bx1, off, 64));
+            by0 = min(by0, __shfl_xor(by0, off, 64)); by1 = max(by1, __shfl_xor(by1, off, 64));
+        }
+        if (lane == 0) {
+            // |d| over d in [lo, hi] with d = dbx + u - 4
+            auto absrange = [](int lo, int hi, int& r0, int& r1) {
+                if (lo >= 0) { r0 = lo; r1 = hi; }
+                else if (hi <= 0) { r0 = -hi; r1 = -lo; }
+                else { r0 = 0; r1 = max(-lo, hi); }
+            };
+            int i0, i1, j0, j1;
+            absrange(ax0 - bx1 - 4, ax1 - bx0 + 4, i0, i1);
+            absrange(ay0 - by1 - 4, ay1 - by0 + 4, j0, j1);
+            i1 = min(i1, Tn - 1);
+            j1 = min(j1, Tn - 1);
+            const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
+            const bool fits = ni > 0 && nj > 0 && (long)ni * nj <= GBOX;
+            sbox[0] = fits ? i0 : -1;
+            sbox[1] = j0;
+            sbox[2] = ni;
+            sbox[3] = nj;
+        }
+    }
+    __syncthreads();
+    const int bi0 = delta ? -1 : sbox[0];
+    const int bj0 = sbox[1], bni = sbox[2], bnj = sbox[3];
+    if (bi0 >= 0) {
+        for (int e = tid; e < bni * bnj; e += AT * AT) {
+            const int jj = bj0 + e / bni, ii = bi0 + e % bni;
+            sG[e] = G[(size_t)Tn * jj + ii];
+        }
+    }
+    __syncthreads();
+    const int pi = ROWMAJOR ? threadIdx.y : threadIdx.x;
+    const int qi = ROWMAJOR ? threadIdx.x : threadIdx.y;
+    const int p = p0 + pi, q = q0 + qi;
     if (p >= N || q >= N) return;
     double acc = 0.0;
     for (int ia = 0; ia < TAn; ++ia) {
-        const Term& a = sa[threadIdx.x * TAn + ia];
+        const Term& a = sa[pi * TAn + ia];
         for (int ib = 0; ib < TBn; ++ib) {
-            const Term& b = sb[threadIdx.y * TBn + ib];
+            const Term& b = sb[qi * TBn + ib];
             if (a.gk != b.gk) continue;
             double cx[9], cy[9];
 #pragma unroll
@@ -69,12 +131,25 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
                 for (int v = 0; v < 9; ++v) {
                     const int jj = abs(dby + v - 4);
                     if (cy[v] == 0.0 || jj >= Tn) continue;   // kernel zero beyond the table
-                    const double* Gr = G + (size_t)Tn * jj;
                     double r = 0.0;
+                    if (bi0 >= 0) {
+                        // staged box: every |dbx + u - 4| < Tn lies in [bi0, bi0 + bni), so
+                        // the index below is >= 0.  Only the integer index is offset:
+                        // a pointer before an LDS array, used as a generic pointer, can
+                        // land outside the LDS aperture.
+                        const int row = bni * (jj - bj0) - bi0;
 #pragma unroll
-                    for (int u = 0; u < 9; ++u) {
-                        const int ii = abs(dbx + u - 4);
-                        if (ii < Tn) r += cx[u] * Gr[ii];
+                        for (int u = 0; u < 9; ++u) {
+                            const int ii = abs(dbx + u - 4);
+                            if (ii < Tn) r += cx[u] * sG[row + ii];
+                        }
+                    } else {
+                        const double* Gr = G + (size_t)Tn * jj;
+#pragma unroll
+                        for (int u = 0; u < 9; ++u) {
+                            const int ii = abs(dbx + u - 4);
+                            if (ii < Tn) r += cx[u] * Gr[ii];
+                        }
                     }
                     s += cy[v] * r;
                 }
@@ -82,36 +157,46 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
             acc += a.coef * b.coef * s;
         }
     }
-    K[p + (size_t)N * q] = factor * acc;
+    if (ROWMAJOR)
+        K[q + (size_t)ldk * p] = factor * acc;
+    else
+        K[p + (size_t)ldk * q] = factor * acc;
 }
 
-void assemble(ilh_plan* pl, int opA, int opB, int gk, double factor, bool delta_kernel, double* K)
+static void launch_assemble(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor, bool delta,
+                            double* K, int ldk, bool rowmajor)
 {
-    (void)gk;
     const int N = pl->N;
     if (N == 0) return;
     const Term* A = get_terms(pl, opA);
     const Term* B = get_terms(pl, opB);
-    if (!delta_kernel) ensure_lgf(pl);
     dim3 grid((N + AT - 1) / AT, (N + AT - 1) / AT);
-    k_assemble<<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, op_terms_per_point(opA), B, op_terms_per_point(opB), N,
-                                                           delta_kernel ? nullptr : pl->lgf.as<double>(), pl->lgf_mx,
-                                                           factor, delta_kernel ? 1 : 0, K);
+    const int ta = op_terms_per_point(opA), tb = op_terms_per_point(opB);
+    if (ta > 2 || tb > 2) fail(ILH_ERR_ARG, "assembly supports at most two terms per point");
+    if (rowmajor)
+        k_assemble<true><<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, ta, B, tb, N, tab, Tn, factor, delta ? 1 : 0,
+                                                                     K, ldk);
+    else
+        k_assemble<false><<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, ta, B, tb, N, tab, Tn, factor, delta ? 1 : 0,
+                                                                      K, ldk);
     ILH_HIP(hipGetLastError());
+}
+
+void assemble(ilh_plan* pl, int opA, int opB, int gk, double factor, bool delta_kernel, double* K, int ldk,
+              bool rowmajor)
+{
+    (void)gk;
+    if (pl->N == 0) return;
+    if (!delta_kernel) ensure_lgf(pl);
+    launch_assemble(pl, opA, opB, delta_kernel ? nullptr : pl->lgf.as<double>(), delta_kernel ? 1 : pl->lgf_mx,
+                    factor, delta_kernel, K, ldk > 0 ? ldk : pl->N, rowmajor);
 }
 
 // K[p,q] = factor * sum over term pairs of the kernel tab (Tn x Tn, even in
 // both axes, zero beyond the table) correlated with the two points' windows
 void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor, double* K)
 {
-    const int N = pl->N;
-    if (N == 0) return;
-    const Term* A = get_terms(pl, opA);
-    const Term* B = get_terms(pl, opB);
-    dim3 grid((N + AT - 1) / AT, (N + AT - 1) / AT);
-    k_assemble<<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, op_terms_per_point(opA), B, op_terms_per_point(opB), N,
-                                                           tab, Tn, factor, 0, K);
-    ILH_HIP(hipGetLastError());
+    launch_assemble(pl, opA, opB, tab, Tn, factor, false, K, pl->N, false);
 }
 
 // ----------------------------------------------------------------- GEMM

@@ -208,8 +208,10 @@ void gather_op(ilh_plan* pl, int op, const double* field, double* out);
 void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv);
 
 // dense linear algebra (ilh_dense.hip)
-void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel,
-              double* K /* N x N col-major */);
+// K = scale * (opA . kernel . opB): column-major N x N with leading dimension
+// ldk (0: N), or row-major (K[q + ldk p]) when rowmajor
+void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel, double* K, int ldk = 0,
+              bool rowmajor = false);
 void release_blas(ilh_ctx* ctx);
 void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor,
                     double* K /* N x N col-major */);

@@ -217,30 +217,45 @@ __global__ void k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny, 
     }
 }
 
-// exclusive scan of cnt[0..n) into ptr[0..n], single work-group
-__global__ void k_scan(const int* __restrict__ cnt, int n, int* __restrict__ ptr)
+// exclusive scan of cnt[0..n) into ptr[0..n], single work-group of 1024
+// threads, 4096 values per round (4 per thread, wave shuffles, one LDS step).
+// n is *n_dev when n_dev is not null (a count produced on the device, at most
+// the n given).
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int n, int* __restrict__ ptr,
+                                               const int* __restrict__ n_dev)
 {
-    __shared__ int part[1024];
+    __shared__ int wsum[16];
     __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    if (n_dev) n = min(n, *n_dev);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + threadIdx.x;
-        int v = i < n ? cnt[i] : 0;
-        part[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int add = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-            __syncthreads();
-            part[threadIdx.x] += add;
-            __syncthreads();
+    for (int base = 0; base < n; base += 4096) {
+        const int i0 = base + 4 * t;
+        int v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = i0 + e < n ? cnt[i0 + e] : 0;
+        const int own = v[0] + v[1] + v[2] + v[3];
+        int inc = own;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += u;
         }
-        if (i < n) ptr[i] = carry + part[threadIdx.x] - v;
+        if (lane == 63) wsum[wave] = inc;
         __syncthreads();
-        if (threadIdx.x == 1023) carry += part[1023];
+        int before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        int run = before + inc - own;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (i0 + e < n) ptr[i0 + e] = run;
+            run += v[e];
+        }
+        __syncthreads();
+        if (t == 1023) carry = run;
         __syncthreads();
     }
-    if (threadIdx.x == 0) ptr[n] = carry;
+    if (t == 0) ptr[n] = carry;
 }
 
 __global__ void k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny, const int* __restrict__ ptr,
@@ -307,10 +322,11 @@ __global__ void k_bin_sort(const Term* __restrict__ t, const int* __restrict__ p
     for (int i = threadIdx.x; i < n; i += blockDim.x) ent[lo + i] = (int)(key[i] & 0xffffffffu);
 }
 
-__global__ void k_bin_bx(const Term* __restrict__ t, const int* __restrict__ ent, int n, int* __restrict__ ebx)
+__global__ void k_bin_bx(const Term* __restrict__ t, const int* __restrict__ ent, const int* __restrict__ n_dev,
+                         int* __restrict__ ebx)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) ebx[k] = t[ent[k]].bx;
+    if (k < *n_dev) ebx[k] = t[ent[k]].bx;
 }
 
 // entry records of row j: amplitude index and coef * wy[j - by] * wx[e]
@@ -363,15 +379,26 @@ __global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict_
     }
 }
 
-__global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
+static constexpr int CELL_LDS_ENTRIES = 4096;   // row entries whose bx are staged in LDS
+
+__global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__ ebx_g, const double* __restrict__ erw,
                             int nx, const int* __restrict__ cptr, int* __restrict__ cells3)
 {
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int woff[CELL_WORDS];
     __shared__ int wsum[4];
+    __shared__ int sbx[CELL_LDS_ENTRIES];
     const int j = blockIdx.x;
     if (ptr[j] == ptr[j + 1]) return;   // no entries: no cells
-    mark_cells(ptr, ebx, erw, j, nx, bm);
+    const int lo_g = ptr[j], nrow = ptr[j + 1] - lo_g;
+    const bool staged = nrow <= CELL_LDS_ENTRIES;
+    if (staged)
+        for (int k = threadIdx.x; k < nrow; k += blockDim.x) sbx[k] = ebx_g[lo_g + k];
+    // bx of the row's k-th entry (sorted), from LDS when staged.  Only indices
+    // are offset, never the LDS base pointer: a pointer before an LDS array
+    // that is used as a generic pointer can land outside the LDS aperture.
+    auto bx_of = [&](int k) { return staged ? sbx[k] : ebx_g[lo_g + k]; };
+    mark_cells(ptr, ebx_g, erw, j, nx, bm);   // its barriers also publish sbx
     // exclusive scan of the popcounts of words 0..255 (256 threads, 4 waves);
     // word 256 (x >= 8192) follows them
     {
@@ -390,7 +417,6 @@ __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__
         if (t == 255) woff[256] = base + v;
     }
     __syncthreads();
-    const int lo = ptr[j], hi = ptr[j + 1];
     for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) {
         unsigned bits = bm[w];
         int rank = 0;
@@ -399,41 +425,41 @@ __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__
             bits &= bits - 1;
             const int x = w * 32 + b;
             // entries with bx in [x-4, x]: a contiguous run of the sorted row
-            int a = lo, c = hi;
+            int a = 0, c = nrow;
             while (a < c) {
                 const int m = (a + c) >> 1;
-                if (ebx[m] < x - 4) a = m + 1; else c = m;
+                if (bx_of(m) < x - 4) a = m + 1; else c = m;
             }
             int e = a;
-            while (e < hi && ebx[e] <= x) ++e;
+            while (e < nrow && bx_of(e) <= x) ++e;
             int* cl = cells3 + 3 * (size_t)(cptr[j] + woff[w] + rank);
             cl[0] = x;
-            cl[1] = a;
-            cl[2] = e;
+            cl[1] = lo_g + a;
+            cl[2] = lo_g + e;
             ++rank;
         }
     }
 }
 
 // contributions of cell c (x, entries [klo, khi)): the entries with a nonzero weight at x
-__global__ void k_contrib_count(const int* __restrict__ cells3, int n, const int* __restrict__ ebx,
-                                const double* __restrict__ erw, int* __restrict__ cnt)
+__global__ void k_contrib_count(const int* __restrict__ cells3, const int* __restrict__ n_dev,
+                                const int* __restrict__ ebx, const double* __restrict__ erw, int* __restrict__ cnt)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    if (c >= *n_dev) return;
     const int x = cells3[3 * (size_t)c], klo = cells3[3 * (size_t)c + 1], khi = cells3[3 * (size_t)c + 2];
     int m = 0;
     for (int k = klo; k < khi; ++k) m += erw[5 * (size_t)k + (x - ebx[k])] != 0.0;
     cnt[c] = m;
 }
 
-__global__ void k_contrib_fill(const int* __restrict__ cells3, int n, const int* __restrict__ ebx,
-                               const int* __restrict__ eai, const double* __restrict__ erw,
-                               const int* __restrict__ off, int4* __restrict__ cells, int* __restrict__ cai,
-                               double* __restrict__ cw)
+__global__ void k_contrib_fill(const int* __restrict__ cells3, const int* __restrict__ n_dev,
+                               const int* __restrict__ ebx, const int* __restrict__ eai,
+                               const double* __restrict__ erw, const int* __restrict__ off, int4* __restrict__ cells,
+                               int* __restrict__ cai, double* __restrict__ cw)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    if (c >= *n_dev) return;
     const int x = cells3[3 * (size_t)c], klo = cells3[3 * (size_t)c + 1], khi = cells3[3 * (size_t)c + 2];
     int e = off[c];
     for (int k = klo; k < khi; ++k) {
@@ -447,73 +473,101 @@ __global__ void k_contrib_fill(const int* __restrict__ cells3, int n, const int*
     cells[c] = make_int4(x, off[c], e, 0);
 }
 
+__global__ void k_fill0_int(int* __restrict__ a, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = 0;
+}
+
+static void fill_zero_int(hipStream_t st, int* a, int n)
+{
+    if (n > 0) k_fill0_int<<<(n + 255) / 256, 256, 0, st>>>(a, n);
+}
+
+// totals: n_entries, n_cells, n_contrib
+__global__ void k_bin_totals(const int* __restrict__ row_ptr, const int* __restrict__ cell_ptr,
+                             const int* __restrict__ coff, int ny, int* __restrict__ tot)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    tot[0] = row_ptr[ny];
+    tot[1] = cell_ptr[ny];
+    tot[2] = coff[cell_ptr[ny]];
+}
+
+// Row bins of a term list for one output grid kind.  Buffers are sized by
+// upper bounds (a term touches at most 5 rows, an entry at most 5 cells), so
+// the whole build runs on the stream with one read-back at the end.
 void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
 {
     hipStream_t st = pl->ctx->stream;
     free_bins(b);
     b.gk = gk;
     b.ny = pl->g.ny(gk);
-    int* cnt = nullptr;
-    ILH_HIP(hipMalloc(&cnt, sizeof(int) * (size_t)(2 * b.ny + 2)));
-    int* cursor = cnt + b.ny + 1;
-    ILH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)(2 * b.ny + 2), st));
-    ILH_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (size_t)(b.ny + 1)));
-    if (nterms > 0)
-        k_bin_count<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, cnt);
-    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.row_ptr);
-    ILH_HIP(hipGetLastError());
-    b.h_row_ptr.resize(b.ny + 1);
-    ILH_HIP(hipMemcpyAsync(b.h_row_ptr.data(), b.row_ptr, sizeof(int) * (size_t)(b.ny + 1),
-                           hipMemcpyDeviceToHost, st));
-    ILH_HIP(hipStreamSynchronize(st));
-    b.n_entries = b.h_row_ptr[b.ny];
-    ILH_HIP(hipMalloc(&b.entries, sizeof(int) * (size_t)std::max(1, 2 * b.n_entries)));
-    ILH_HIP(hipMalloc(&b.eai, sizeof(int) * (size_t)std::max(1, b.n_entries)));
-    ILH_HIP(hipMalloc(&b.erw, sizeof(double) * 5 * (size_t)std::max(1, b.n_entries)));
-    if (nterms > 0 && b.n_entries > 0) {
-        k_bin_fill<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, b.row_ptr, cursor, b.entries);
-        k_bin_sort<<<b.ny, 256, 0, st>>>(terms, b.row_ptr, b.entries);
-        k_bin_bx<<<(b.n_entries + 255) / 256, 256, 0, st>>>(terms, b.entries, b.n_entries, b.entries + b.n_entries);
-        k_bin_rec<<<b.ny, 64, 0, st>>>(terms, b.row_ptr, b.entries, b.eai, b.erw);
-        ILH_HIP(hipGetLastError());
-    }
-    // cells
     const int nx = pl->g.nx(gk);
     if (nx > 8192) fail(ILH_ERR_ARG, "grid too wide for the cell index (nx > 8192)");
+    const long e_max = 5L * std::max(0, nterms), c_max = 5L * e_max;
+    if (c_max > (1L << 30)) fail(ILH_ERR_ARG, "too many surface terms for the row bins");
+    const size_t E = (size_t)std::max(1L, e_max), Cm = (size_t)std::max(1L, c_max);
+    // count scratch: rows (2 (ny+1)), then the three totals
+    DevBuf scratch, cells3, ccnt, coff;
+    scratch.ensure(sizeof(int) * (size_t)(2 * b.ny + 2 + 4));
+    int* cnt = scratch.as<int>();
+    int* cursor = cnt + b.ny + 1;
+    int* tot = cnt + 2 * b.ny + 2;
+    ILH_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (size_t)(b.ny + 1)));
     ILH_HIP(hipMalloc(&b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1)));
-    ILH_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)(b.ny + 1), st));
-    if (b.n_entries > 0)
-        k_cell_count<<<b.ny, 256, 0, st>>>(b.row_ptr, b.entries + b.n_entries, b.erw, nx, cnt);
-    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.cell_ptr);
-    ILH_HIP(hipGetLastError());
-    ILH_HIP(hipMemcpyAsync(&b.n_cells, b.cell_ptr + b.ny, sizeof(int), hipMemcpyDeviceToHost, st));
-    ILH_HIP(hipStreamSynchronize(st));
-    b.h_cell_ptr.resize(b.ny + 1);
-    ILH_HIP(hipMemcpyAsync(b.h_cell_ptr.data(), b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1), hipMemcpyDeviceToHost,
-                           st));
-    ILH_HIP(hipMalloc(&b.cells, sizeof(int4) * (size_t)std::max(1, b.n_cells)));
-    if (b.n_cells > 0) {
-        DevBuf cells3, ccnt, coff;
-        cells3.ensure(sizeof(int) * 3 * (size_t)b.n_cells);
-        ccnt.ensure(sizeof(int) * (size_t)b.n_cells);
-        coff.ensure(sizeof(int) * (size_t)(b.n_cells + 1));
-        const int* ebx = b.entries + b.n_entries;
-        const unsigned g = (unsigned)((b.n_cells + 255) / 256);
-        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, b.cell_ptr, cells3.as<int>());
-        k_contrib_count<<<g, 256, 0, st>>>(cells3.as<int>(), b.n_cells, ebx, b.erw, ccnt.as<int>());
-        k_scan<<<1, 1024, 0, st>>>(ccnt.as<int>(), b.n_cells, coff.as<int>());
-        ILH_HIP(hipGetLastError());
-        ILH_HIP(hipMemcpyAsync(&b.n_contrib, coff.as<int>() + b.n_cells, sizeof(int), hipMemcpyDeviceToHost, st));
-        ILH_HIP(hipStreamSynchronize(st));
-        ILH_HIP(hipMalloc(&b.cai, sizeof(int) * (size_t)std::max(1, b.n_contrib)));
-        ILH_HIP(hipMalloc(&b.cw, sizeof(double) * (size_t)std::max(1, b.n_contrib)));
-        k_contrib_fill<<<g, 256, 0, st>>>(cells3.as<int>(), b.n_cells, ebx, b.eai, b.erw, coff.as<int>(), b.cells,
-                                          b.cai, b.cw);
-        ILH_HIP(hipGetLastError());
-        ILH_HIP(hipStreamSynchronize(st));
+    ILH_HIP(hipMalloc(&b.entries, sizeof(int) * 2 * E));
+    ILH_HIP(hipMalloc(&b.eai, sizeof(int) * E));
+    ILH_HIP(hipMalloc(&b.erw, sizeof(double) * 5 * E));
+    ILH_HIP(hipMalloc(&b.cells, sizeof(int4) * Cm));
+    ILH_HIP(hipMalloc(&b.cai, sizeof(int) * Cm));
+    ILH_HIP(hipMalloc(&b.cw, sizeof(double) * Cm));
+    cells3.ensure(sizeof(int) * 3 * Cm);
+    ccnt.ensure(sizeof(int) * Cm);
+    coff.ensure(sizeof(int) * (Cm + 1));
+    int* ebx = b.entries + E;
+    fill_zero_int(st, cnt, 2 * b.ny + 2);
+    // entries per row, sorted by (bx, term)
+    if (nterms > 0)
+        k_bin_count<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, cnt);
+    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.row_ptr, nullptr);
+    if (nterms > 0) {
+        k_bin_fill<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, b.row_ptr, cursor, b.entries);
+        k_bin_sort<<<b.ny, 256, 0, st>>>(terms, b.row_ptr, b.entries);
+        k_bin_bx<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(terms, b.entries, b.row_ptr + b.ny, ebx);
+        k_bin_rec<<<b.ny, 64, 0, st>>>(terms, b.row_ptr, b.entries, b.eai, b.erw);
     }
+    ILH_HIP(hipGetLastError());
+    // cells per row
+    fill_zero_int(st, cnt, b.ny + 1);
+    if (nterms > 0) k_cell_count<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, cnt);
+    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.cell_ptr, nullptr);
+    // contributions per cell
+    const unsigned g = (unsigned)((Cm + 255) / 256);
+    if (nterms > 0) {
+        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, b.cell_ptr, cells3.as<int>());
+        k_contrib_count<<<g, 256, 0, st>>>(cells3.as<int>(), b.cell_ptr + b.ny, ebx, b.erw, ccnt.as<int>());
+    }
+    k_scan<<<1, 1024, 0, st>>>(ccnt.as<int>(), (int)c_max, coff.as<int>(), b.cell_ptr + b.ny);
+    if (nterms > 0)
+        k_contrib_fill<<<g, 256, 0, st>>>(cells3.as<int>(), b.cell_ptr + b.ny, ebx, b.eai, b.erw, coff.as<int>(),
+                                          b.cells, b.cai, b.cw);
+    k_bin_totals<<<1, 64, 0, st>>>(b.row_ptr, b.cell_ptr, coff.as<int>(), b.ny, tot);
+    ILH_HIP(hipGetLastError());
+    // one read-back: row and cell offsets and the totals
+    std::vector<int> h(2 * (size_t)(b.ny + 1) + 3);
+    ILH_HIP(hipMemcpyAsync(h.data(), b.row_ptr, sizeof(int) * (size_t)(b.ny + 1), hipMemcpyDeviceToHost, st));
+    ILH_HIP(hipMemcpyAsync(h.data() + b.ny + 1, b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1), hipMemcpyDeviceToHost,
+                           st));
+    ILH_HIP(hipMemcpyAsync(h.data() + 2 * (b.ny + 1), tot, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
     ILH_HIP(hipStreamSynchronize(st));
-    ILH_HIP(hipFree(cnt));
+    b.h_row_ptr.assign(h.begin(), h.begin() + b.ny + 1);
+    b.h_cell_ptr.assign(h.begin() + b.ny + 1, h.begin() + 2 * (b.ny + 1));
+    b.n_entries = h[2 * (b.ny + 1)];
+    b.n_cells = h[2 * (b.ny + 1) + 1];
+    b.n_contrib = h[2 * (b.ny + 1) + 2];
+    if (b.n_entries > e_max || b.n_cells > c_max || b.n_contrib > c_max)
+        fail(ILH_ERR_HIP, "row bins exceeded their bounds");
 }
 
 void free_bins(RowBins& b)
