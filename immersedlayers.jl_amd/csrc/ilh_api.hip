@@ -61,12 +61,8 @@ struct SolveCache {
     DevBuf amp1, amp2;
     ~SolveCache()
     {
-        free_bins(src1.bins);
-        free_bins(src2.bins);
-        for (SourceSet* ss : {&src1, &src2}) {
-            if (ss->pinfo) (void)hipFree(ss->pinfo);
-            if (ss->slot_of_row) (void)hipFree(ss->slot_of_row);
-        }
+        free_sources(src1);
+        free_sources(src2);
     }
 };
 
@@ -78,7 +74,7 @@ struct MovingCache {
     DevBuf Pinv;
     bool have_ref = false;
     int ref_age = 0;           // solves since the reference was factored
-    DevBuf terms1, amp, V, w, z, x, b, h, fbar, dvec, grid;
+    DevBuf terms1, amp, V, w, z, b, fbar, grid;
     DevBuf S;                  // create_RTLinvR at the current geometry, row-major
     bool S_ok = false;
     DevBuf gm, gmf;            // GMRES state and flags on the device
@@ -98,25 +94,23 @@ struct MovingCache {
     }
     SourceSet src1;
     bool src_ok = false;
+    // the current geometry's matrix and sources are stale (their buffers are
+    // kept for the next geometry)
     void drop_sources()
     {
         S_ok = false;
-        if (src_ok) {
-            free_bins(src1.bins);
-            if (src1.pinfo) (void)hipFree(src1.pinfo);
-            if (src1.slot_of_row) (void)hipFree(src1.slot_of_row);
-            src1 = SourceSet();
-        }
         src_ok = false;
     }
     ~MovingCache()
     {
-        drop_sources();
+        free_sources(src1);
         if (hpin) (void)hipHostFree(hpin);
     }
 };
 
-static void drop_caches(ilh_plan* pl)
+// Geometry caches become stale; row-bin buffers are kept for the next
+// geometry unless `release` (plan destruction).
+static void drop_caches(ilh_plan* pl, bool release = false)
 {
     ++pl->gen;
     if (pl->mov) pl->mov->drop_sources();
@@ -129,10 +123,10 @@ static void drop_caches(ilh_plan* pl)
     }
     for (int op = 0; op < OP_COUNT; ++op) {
         pl->terms_ok[op] = false;
-        if (pl->bins_ok[op]) {
+        pl->bins_ok[op] = false;
+        if (release) {
             free_bins(pl->bins[op][0]);
             free_bins(pl->bins[op][1]);
-            pl->bins_ok[op] = false;
         }
     }
 }
@@ -156,22 +150,31 @@ extern "C" int ilh_ctx_create(int device, ilh_ctx** out)
     });
 }
 
+void ilh::ctx_retain(ilh_ctx* c) { c->refs.fetch_add(1); }
+
+void ilh::ctx_release(ilh_ctx* c)
+{
+    if (c->refs.fetch_sub(1) > 1) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < ilh_ctx::POOL; ++k)
+        if (c->pool[k]) {
+            (void)hipStreamSynchronize(c->pool[k]);
+            (void)hipStreamDestroy(c->pool[k]);
+        }
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    release_blas(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// the caller's handle; plans and batches still using the context keep it
+// alive until they are destroyed
 extern "C" int ilh_ctx_destroy(ilh_ctx* c)
 {
     return guarded([&] {
-        if (!c) return;
-        (void)hipSetDevice(c->device);
-        (void)hipStreamSynchronize(c->stream);
-        for (int k = 0; k < ilh_ctx::POOL; ++k)
-            if (c->pool[k]) {
-                (void)hipStreamSynchronize(c->pool[k]);
-                (void)hipStreamDestroy(c->pool[k]);
-            }
-        for (hipEvent_t e : c->ev)
-            if (e) (void)hipEventDestroy(e);
-        release_blas(c);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
+        if (c) ctx_release(c);
     });
 }
 
@@ -270,10 +273,11 @@ extern "C" int ilh_plan_create(ilh_ctx* ctx, const ilh_grid* grid, int npts, con
         pl->N = npts;
         pl->ddf = ddf;
         load_points(pl, X, Y, nx, ny, ds);
+        ctx_retain(ctx);
         *out = pl;
     });
     if (rc != ILH_OK && pl) {
-        drop_caches(pl);
+        drop_caches(pl, true);
         delete pl;
     }
     return rc;
@@ -291,15 +295,26 @@ extern "C" int ilh_plan_update_points(ilh_plan* pl, const double* X, const doubl
     });
 }
 
+void ilh::plan_retain(ilh_plan* pl) { pl->refs.fetch_add(1); }
+
+void ilh::plan_release(ilh_plan* pl)
+{
+    if (pl->refs.fetch_sub(1) > 1) return;
+    ilh_ctx* ctx = pl->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    drop_caches(pl, true);
+    delete pl->mov;
+    delete pl;
+    ctx_release(ctx);
+}
+
+// the caller's handle; batches and heat integrators still using the plan
+// keep it alive until they are destroyed
 extern "C" int ilh_plan_destroy(ilh_plan* pl)
 {
     return guarded([&] {
-        if (!pl) return;
-        (void)hipSetDevice(pl->ctx->device);
-        (void)hipStreamSynchronize(pl->ctx->stream);
-        drop_caches(pl);
-        delete pl->mov;
-        delete pl;
+        if (pl) plan_release(pl);
     });
 }
 
@@ -1075,6 +1090,8 @@ extern "C" int ilh_batch_create(int nprob, ilh_plan* const* plans, const double*
             delete b;
             fail(ILH_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
         }
+        ctx_retain(ctx);
+        for (ilh_plan* p : b->plans) plan_retain(p);
         *out = b;
     });
 }
@@ -1099,23 +1116,14 @@ extern "C" int ilh_batch_destroy(ilh_batch* b)
         (void)hipStreamSynchronize(b->ctx->stream);
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
+        for (ilh_plan* p : b->plans) plan_release(p);
+        ctx_release(b->ctx);
         delete b;
     });
 }
 
 // ------------------------------------------------- moving-body Dirichlet
 namespace {
-
-// w[k] -= sum_{i < nv} V[k + n i] h[i]
-__global__ void k_mupdate(const double* __restrict__ V, int nv, const double* __restrict__ h, int n,
-                          double* __restrict__ w)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    double acc = w[k];
-    for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)n * i] * h[i];
-    w[k] = acc;
-}
 
 // GMRES state on the device (m = restart length):
 //   H (m+1) x m column-major, cs[m], sn[m], g[m+1], then bnorm and tol;
@@ -1148,7 +1156,7 @@ __device__ __forceinline__ double block_sum_1024(double a, double* red)
 // converged at once when b = 0 or beta <= tol |b|
 __global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, const double* __restrict__ b, int n,
                                                   double* __restrict__ V, double* __restrict__ st, int m, double tol,
-                                                  int* __restrict__ flags)
+                                                  int* __restrict__ flags)   // v_0 = V[0, n)
 {
     __shared__ double red[16];
     const GmLayout L{m};
@@ -1186,13 +1194,15 @@ __global__ void k_gm_matvec(const double* __restrict__ M, int ld, const double* 
 }
 
 // Arnoldi step j (one work-group, unless the iteration has stopped): w is
-// orthogonalised against v_0..v_j by two passes of classical Gram-Schmidt and
+// orthogonalised against v_0..v_j (columns of V, stride ldv) by classical
+// Gram-Schmidt, repeated once when the first pass removed more than
+// 1 - 1/sqrt(2) of |w| (the Daniel-Gragg-Kaufman-Stewart test), and
 // normalised into v_{j+1}; column j of H gets the summed coefficients and
 // |w|; the previous Givens rotations are applied, a new one zeroes
 // H[j+1, j], and |g[j+1]| (the residual norm) is tested against tol |b|.
 // Every sum has a fixed order (deterministic).
-__global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int j, double* __restrict__ w, int n,
-                                                     double* __restrict__ st, int m, int* __restrict__ flags)
+__global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int ldv, int j, double* __restrict__ w,
+                                                     int n, double* __restrict__ st, int m, int* __restrict__ flags)
 {
     __shared__ double h[2][256];
     __shared__ double red[16];
@@ -1200,10 +1210,14 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
     const GmLayout L{m};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nv = j + 1;
+    double a = 0.0;
+    for (int k = tid; k < n; k += 1024) a += w[k] * w[k];
+    double wn = sqrt(block_sum_1024(a, red));   // |w| before the pass
+    for (int i = tid; i < nv; i += 1024) h[1][i] = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
         // dots: wave i % 16 owns basis vector i
         for (int i = wave; i < nv; i += 16) {
-            const double* v = V + (size_t)n * i;
+            const double* v = V + (size_t)ldv * i;
             double a0 = 0.0, a1 = 0.0;
             int k = lane;
             for (; k + 64 < n; k += 128) {
@@ -1216,19 +1230,22 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
             if (lane == 0) h[pass][i] = acc;
         }
         __syncthreads();
-        // w -= V h
+        // w -= V h, and |w| after the pass
+        double q = 0.0;
         for (int k = tid; k < n; k += 1024) {
             double acc = w[k];
-            for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)n * i] * h[pass][i];
+            for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)ldv * i] * h[pass][i];
             w[k] = acc;
+            q += acc * acc;
         }
-        __syncthreads();
+        const double wn1 = sqrt(block_sum_1024(q, red));   // its barriers publish w
+        const bool again = pass == 0 && wn1 < 0.70710678118654752 * wn;
+        wn = wn1;
+        if (!again) break;
     }
-    double a = 0.0;
-    for (int k = tid; k < n; k += 1024) a += w[k] * w[k];
-    const double hn = sqrt(block_sum_1024(a, red));
+    const double hn = wn;
     if (hn > 0.0) {
-        double* vn = V + (size_t)n * nv;
+        double* vn = V + (size_t)ldv * nv;
         const double inv = 1.0 / hn;
         for (int k = tid; k < n; k += 1024) vn[k] = w[k] * inv;
     }
@@ -1261,28 +1278,40 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
     }
 }
 
-// y = H^-1 g (upper triangular, j x j), written negated into ny[0..j)
-__global__ void k_gm_backsolve(const double* __restrict__ st, int m, int j, double* __restrict__ ny)
+// y = H^-1 g (upper triangular, j x j, thread 0), then w = V y (one
+// work-group, j <= 256)
+__global__ void __launch_bounds__(1024) k_gm_finish(const double* __restrict__ st, int m, int j,
+                                                    const double* __restrict__ V, int ldv, int n,
+                                                    double* __restrict__ w)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double y[256];
     const GmLayout L{m};
-    const double* H = st + L.H();
-    const double* g = st + L.g();
-    for (int i = j - 1; i >= 0; --i) {
-        double t = g[i];
-        for (int k = i + 1; k < j; ++k) t += H[i + (size_t)(m + 1) * k] * ny[k];   // ny = -y
-        ny[i] = -t / H[i + (size_t)(m + 1) * i];
+    if (threadIdx.x == 0) {
+        const double* H = st + L.H();
+        const double* g = st + L.g();
+        for (int i = j - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int k = i + 1; k < j; ++k) t -= H[i + (size_t)(m + 1) * k] * y[k];
+            y[i] = t / H[i + (size_t)(m + 1) * i];
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += 1024) {
+        double acc = 0.0;
+        for (int i = 0; i < j; ++i) acc += V[k + (size_t)ldv * i] * y[i];
+        w[k] = acc;
     }
 }
 
-// d = f+ - f-,  fbar = (f+ + f-)/2
-__global__ void k_jump_avg(const double* __restrict__ fp, const double* __restrict__ fm, int n, double* __restrict__ d,
-                           double* __restrict__ fbar)
+// amp = [0; f+ - f-] (2n),  fbar = (f+ + f-)/2
+__global__ void k_jump_avg(const double* __restrict__ fp, const double* __restrict__ fm, int n,
+                           double* __restrict__ amp, double* __restrict__ fbar)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double a = fp[k], b = fm ? fm[k] : 0.0;
-    d[k] = a - b;
+    amp[k] = 0.0;
+    amp[n + k] = a - b;
     fbar[k] = 0.5 * (a + b);
 }
 
@@ -1365,9 +1394,9 @@ static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* 
     while (j < m) {
         const int j1 = std::min(m, j + chunk);
         for (; j < j1; ++j) {
-            k_gm_matvec<<<gR, 256, 0, st>>>(mc->Pinv.as<double>(), ld, V + (size_t)N * j, z, N, flags);
+            k_gm_matvec<<<gR, 256, 0, st>>>(mc->Pinv.as<double>(), ld, V + (size_t)ld * j, z, N, flags);
             k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, z, w, N, flags);
-            k_gm_arnoldi<<<1, 1024, 0, st>>>(V, j, w, N, gm, m, flags);
+            k_gm_arnoldi<<<1, 1024, 0, st>>>(V, ld, j, w, N, gm, m, flags);
         }
         ILH_HIP(hipGetLastError());
         ILH_HIP(hipMemcpyAsync(hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1381,10 +1410,7 @@ static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* 
     if (steps > 0) {
         mc->last_iters = steps;
         // x += P (V y)
-        double* ny = mc->h.as<double>();
-        k_gm_backsolve<<<1, 64, 0, st>>>(gm, m, steps, ny);
-        fill_zero(st, w, N);
-        k_mupdate<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(V, steps, ny, N, w);
+        k_gm_finish<<<1, 1024, 0, st>>>(gm, m, steps, V, ld, N, w);
         matvec_rows(st, mc->Pinv.as<double>(), ld, w, x, N, N, 1.0, 1);
         ILH_HIP(hipGetLastError());
     }
@@ -1419,10 +1445,8 @@ extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const 
             }
             const int m = max_iter;
             mc->amp.ensure(sizeof(double) * 2 * (size_t)N);
-            mc->V.ensure(sizeof(double) * (size_t)N * (m + 1));
-            for (DevBuf* bb : {&mc->w, &mc->z, &mc->x, &mc->b, &mc->fbar, &mc->dvec})
-                bb->ensure(sizeof(double) * (size_t)N);
-            mc->h.ensure(sizeof(double) * 3 * (size_t)(m + 1));
+            mc->V.ensure(sizeof(double) * (size_t)solve_ld(N) * (m + 1));   // basis columns, stride solve_ld(N)
+            for (DevBuf* bb : {&mc->w, &mc->z, &mc->b, &mc->fbar}) bb->ensure(sizeof(double) * (size_t)N);
             mc->grid.ensure(sizeof(double) * ng);
             if (!mc->have_ref) moving_reference(pl, mc);
             if (!mc->S_ok) {
@@ -1433,28 +1457,25 @@ extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const 
                          true);
                 mc->S_ok = true;
             }
-            // f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
+            // amp = [0; d], f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
             double* amp = mc->amp.as<double>();
-            k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, mc->dvec.as<double>(), mc->fbar.as<double>());
-            fill_zero(st, amp, N);
-            ILH_HIP(hipMemcpyAsync(amp + N, mc->dvec.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, amp, mc->fbar.as<double>());
             conv_sources(pl, GK_PRIMAL, mc->src1, amp, mc->grid.as<double>());
             gather_op(pl, OP_E_PRIMAL, mc->grid.as<double>(), mc->b.as<double>());
             axpby(st, mc->b.as<double>(), mc->fbar.as<double>(), -1.0, 1.0, N);
-            it = moving_gmres(pl, mc, mc->b.as<double>(), mc->x.as<double>(), tol, m);
+            // s lands in amp[0, N): the sources of the final convolution
+            it = moving_gmres(pl, mc, mc->b.as<double>(), amp, tol, m);
             if (it < 0) {
                 // the reference geometry no longer preconditions well: refactor here
                 moving_reference(pl, mc);
-                const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), mc->x.as<double>(), tol, m);
+                const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), amp, tol, m);
                 if (it2 < 0) fail(ILH_ERR_SINGULAR, "GMRES did not converge with the current geometry's factor");
                 it = m + it2;
             }
             ++mc->ref_age;
             // f = L^-1 (R s + D_s d)
-            ILH_HIP(hipMemcpyAsync(amp, mc->x.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-            ILH_HIP(hipMemcpyAsync(amp + N, mc->dvec.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
             conv_sources(pl, GK_PRIMAL, mc->src1, amp, d_f);
-            ILH_HIP(hipMemcpyAsync(d_s, mc->x.as<double>(), sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            ILH_HIP(hipMemcpyAsync(d_s, amp, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         }
         if (iters) *iters = it;
         io.back(f, d_f, ng);

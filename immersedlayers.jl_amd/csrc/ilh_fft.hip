@@ -882,12 +882,19 @@ void build_source_pairs(ilh_plan* pl, SourceSet& s)
         }
     }
     s.npairs = (int)info.size();
-    if (s.pinfo) (void)hipFree(s.pinfo);
-    if (s.slot_of_row) (void)hipFree(s.slot_of_row);
-    s.pinfo = nullptr;
-    s.slot_of_row = nullptr;
-    ILH_HIP(hipMalloc(&s.pinfo, sizeof(int4) * (size_t)std::max(1, s.npairs)));
-    ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * (size_t)std::max(1, ny)));
+    // tables are kept from the previous geometry when large enough
+    if ((size_t)std::max(1, s.npairs) > s.cap_pairs) {
+        if (s.pinfo) (void)hipFree(s.pinfo);
+        s.pinfo = nullptr;
+        s.cap_pairs = (size_t)ny / 2 + 1;
+        ILH_HIP(hipMalloc(&s.pinfo, sizeof(int4) * s.cap_pairs));
+    }
+    if ((size_t)std::max(1, ny) > s.cap_rows) {
+        if (s.slot_of_row) (void)hipFree(s.slot_of_row);
+        s.slot_of_row = nullptr;
+        s.cap_rows = (size_t)std::max(1, ny);
+        ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * s.cap_rows));
+    }
     hipStream_t st = pl->ctx->stream;
     if (s.npairs)
         ILH_HIP(hipMemcpyAsync(s.pinfo, info.data(), sizeof(int4) * (size_t)s.npairs, hipMemcpyHostToDevice, st));

@@ -210,6 +210,7 @@ extern "C" int ilh_heat_create(ilh_plan* pl, double kappa, double dt, ilh_heat**
             delete h;
             throw;
         }
+        plan_retain(pl);
         *out = h;
     });
 }
@@ -304,8 +305,10 @@ extern "C" int ilh_heat_destroy(ilh_heat* h)
 {
     return guarded_call([&] {
         if (!h) return;
-        (void)hipSetDevice(h->pl->ctx->device);
-        (void)hipStreamSynchronize(h->pl->ctx->stream);
+        ilh_plan* pl = h->pl;
+        (void)hipSetDevice(pl->ctx->device);
+        (void)hipStreamSynchronize(pl->ctx->stream);
         delete h;
+        plan_release(pl);
     });
 }

@@ -1,6 +1,7 @@
 // ilh_internal.hpp -- shared types of libimmersedhip.so (not part of the ABI).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <new>
@@ -97,6 +98,15 @@ struct RowBins {
     int n_contrib = 0;
     std::vector<int> h_row_ptr;  // host copies (row counts drive launch shapes)
     std::vector<int> h_cell_ptr;
+    // build scratch: row counts, cursors and totals; per-cell records, counts
+    // and offsets
+    int* cnt = nullptr;
+    int* cells3 = nullptr;
+    int* ccnt = nullptr;
+    int* coff = nullptr;
+    // allocated capacities (buffers are kept across geometry updates and only
+    // grown): rows, entries, cells
+    size_t cap_rows = 0, cap_e = 0, cap_c = 0;
 };
 
 // ---------------------------------------------------------- device buffers
@@ -129,6 +139,9 @@ struct ilh_ctx {
     static constexpr int POOL = 4;            // side streams of batched calls (one per hardware queue)
     hipStream_t pool[POOL] = {};
     hipEvent_t ev[POOL + 1] = {};
+    // owners: the caller's handle, every plan and batch of the context; the
+    // context is torn down when the last one lets go (any destruction order)
+    std::atomic<int> refs{1};
 };
 
 struct SolveCache;   // Schur factor and solve matrices of one composite
@@ -159,9 +172,16 @@ struct ilh_plan {
     unsigned gen = 0;                         // incremented whenever the geometry caches are dropped
     ilh::DevBuf mask;                         // mask(cache) on Nodes{Primal} (built lazily)
     unsigned mask_gen = ~0u;                  // generation the mask belongs to
+    std::atomic<int> refs{1};                 // the caller's handle, batches and heat integrators using it
 };
 
 namespace ilh {
+// shared ownership of contexts and plans (ilh_api.hip)
+void ctx_retain(ilh_ctx* c);
+void ctx_release(ilh_ctx* c);
+void plan_retain(ilh_plan* pl);
+void plan_release(ilh_plan* pl);
+
 // ---------------------------------------------------------------- launchers
 // terms / binning (ilh_terms.hip)
 void build_terms(ilh_plan* pl, int op);
@@ -194,7 +214,9 @@ struct SourceSet {
     int4* pinfo = nullptr;     // device, per nonzero row pair p: {j0, first cell, first cell of row j0+1, end}
     int* slot_of_row = nullptr;  // device: field row -> row of the compacted spectrum, or -1
     int npairs = 0;
+    size_t cap_pairs = 0, cap_rows = 0;   // allocated sizes of pinfo and slot_of_row
 };
+void free_sources(SourceSet& s);   // releases the bins and pair tables
 void build_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);

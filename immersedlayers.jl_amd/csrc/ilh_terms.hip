@@ -207,14 +207,23 @@ const Term* get_terms(ilh_plan* pl, int op)
 }
 
 // ------------------------------------------------------------ row binning
-__global__ void k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny, int* __restrict__ cnt)
+// entries per row (terms of grid kind gk with a nonzero weight in the row):
+// one work-group, counts in LDS (dynamic, ny ints), written to cnt[0..ny)
+__global__ void __launch_bounds__(1024) k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                    int* __restrict__ cnt)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nt || t[k].gk != gk) return;
-    for (int e = 0; e < 5; ++e) {
-        const int row = t[k].by + e;
-        if (row >= 0 && row < ny && t[k].wy[e] != 0.0) atomicAdd(&cnt[row], 1);
+    extern __shared__ int lc[];
+    for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+        if (t[k].gk != gk) continue;
+        for (int e = 0; e < 5; ++e) {
+            const int row = t[k].by + e;
+            if (row >= 0 && row < ny && t[k].wy[e] != 0.0) atomicAdd(&lc[row], 1);
+        }
     }
+    __syncthreads();
+    for (int r = threadIdx.x; r < ny; r += blockDim.x) cnt[r] = lc[r];
 }
 
 // exclusive scan of cnt[0..n) into ptr[0..n], single work-group of 1024
@@ -258,16 +267,22 @@ __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int 
     if (t == 0) ptr[n] = carry;
 }
 
-__global__ void k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny, const int* __restrict__ ptr,
-                           int* __restrict__ cursor, int* __restrict__ ent)
+// the entries themselves, in arbitrary order within a row (k_bin_sort orders
+// them); one work-group, row cursors in LDS (dynamic, ny ints)
+__global__ void __launch_bounds__(1024) k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                   const int* __restrict__ ptr, int* __restrict__ ent)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nt || t[k].gk != gk) return;
-    for (int e = 0; e < 5; ++e) {
-        const int row = t[k].by + e;
-        if (row >= 0 && row < ny && t[k].wy[e] != 0.0) {
-            const int pos = atomicAdd(&cursor[row], 1);
-            ent[ptr[row] + pos] = k;
+    extern __shared__ int lc[];
+    for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+        if (t[k].gk != gk) continue;
+        for (int e = 0; e < 5; ++e) {
+            const int row = t[k].by + e;
+            if (row >= 0 && row < ny && t[k].wy[e] != 0.0) {
+                const int pos = atomicAdd(&lc[row], 1);
+                ent[ptr[row] + pos] = k;
+            }
         }
     }
 }
@@ -366,7 +381,10 @@ __global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict_
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int tot[256];
     const int j = blockIdx.x;
-    if (ptr[j] == ptr[j + 1]) return;   // no entries: no cells (cnt[j] stays 0)
+    if (ptr[j] == ptr[j + 1]) {   // no entries: no cells
+        if (threadIdx.x == 0) cnt[j] = 0;
+        return;
+    }
     mark_cells(ptr, ebx, erw, j, nx, bm);
     int c = 0;
     for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) c += __popc(bm[w]);
@@ -417,27 +435,31 @@ __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__
         if (t == 255) woff[256] = base + v;
     }
     __syncthreads();
-    for (int w = threadIdx.x; w < CELL_WORDS; w += blockDim.x) {
-        unsigned bits = bm[w];
-        int rank = 0;
-        while (bits) {
-            const int b = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const int x = w * 32 + b;
-            // entries with bx in [x-4, x]: a contiguous run of the sorted row
-            int a = 0, c = nrow;
-            while (a < c) {
-                const int m = (a + c) >> 1;
-                if (bx_of(m) < x - 4) a = m + 1; else c = m;
-            }
-            int e = a;
-            while (e < nrow && bx_of(e) <= x) ++e;
-            int* cl = cells3 + 3 * (size_t)(cptr[j] + woff[w] + rank);
-            cl[0] = x;
-            cl[1] = lo_g + a;
-            cl[2] = lo_g + e;
-            ++rank;
+    // one thread per cell: the r-th set bit of the bitmap (x < nx <= 8192, so
+    // word 256 is empty and woff[256] is the row's cell count)
+    const int ncell = woff[256];
+    for (int r = threadIdx.x; r < ncell; r += blockDim.x) {
+        // the word holding rank r: the last w with woff[w] <= r
+        int wl = 0, wh = 256;
+        while (wh - wl > 1) {
+            const int m = (wl + wh) >> 1;
+            if (woff[m] <= r) wl = m; else wh = m;
         }
+        unsigned bits = bm[wl];
+        for (int k = r - woff[wl]; k > 0; --k) bits &= bits - 1;
+        const int x = wl * 32 + __ffs(bits) - 1;
+        // entries with bx in [x-4, x]: a contiguous run of the sorted row
+        int a = 0, c = nrow;
+        while (a < c) {
+            const int m = (a + c) >> 1;
+            if (bx_of(m) < x - 4) a = m + 1; else c = m;
+        }
+        int e = a;
+        while (e < nrow && bx_of(e) <= x) ++e;
+        int* cl = cells3 + 3 * (size_t)(cptr[j] + r);
+        cl[0] = x;
+        cl[1] = lo_g + a;
+        cl[2] = lo_g + e;
     }
 }
 
@@ -473,17 +495,6 @@ __global__ void k_contrib_fill(const int* __restrict__ cells3, const int* __rest
     cells[c] = make_int4(x, off[c], e, 0);
 }
 
-__global__ void k_fill0_int(int* __restrict__ a, int n)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] = 0;
-}
-
-static void fill_zero_int(hipStream_t st, int* a, int n)
-{
-    if (n > 0) k_fill0_int<<<(n + 255) / 256, 256, 0, st>>>(a, n);
-}
-
 // totals: n_entries, n_cells, n_contrib
 __global__ void k_bin_totals(const int* __restrict__ row_ptr, const int* __restrict__ cell_ptr,
                              const int* __restrict__ coff, int ny, int* __restrict__ tot)
@@ -500,59 +511,72 @@ __global__ void k_bin_totals(const int* __restrict__ row_ptr, const int* __restr
 void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
 {
     hipStream_t st = pl->ctx->stream;
-    free_bins(b);
     b.gk = gk;
     b.ny = pl->g.ny(gk);
     const int nx = pl->g.nx(gk);
     if (nx > 8192) fail(ILH_ERR_ARG, "grid too wide for the cell index (nx > 8192)");
     const long e_max = 5L * std::max(0, nterms), c_max = 5L * e_max;
     if (c_max > (1L << 30)) fail(ILH_ERR_ARG, "too many surface terms for the row bins");
-    const size_t E = (size_t)std::max(1L, e_max), Cm = (size_t)std::max(1L, c_max);
-    // count scratch: rows (2 (ny+1)), then the three totals
-    DevBuf scratch, cells3, ccnt, coff;
-    scratch.ensure(sizeof(int) * (size_t)(2 * b.ny + 2 + 4));
-    int* cnt = scratch.as<int>();
-    int* cursor = cnt + b.ny + 1;
+    // buffers are kept from the previous build when large enough
+    auto grow = [](auto*& p, size_t n) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        ILH_HIP(hipMalloc((void**)&p, sizeof(*p) * n));
+    };
+    if ((size_t)b.ny + 1 > b.cap_rows) {
+        const size_t r = (size_t)b.ny + 1;
+        grow(b.row_ptr, r);
+        grow(b.cell_ptr, r);
+        grow(b.cnt, 2 * r + 4);   // counts, cursors, three totals
+        b.cap_rows = r;
+    }
+    if ((size_t)std::max(1L, e_max) > b.cap_e) {
+        const size_t E = (size_t)std::max(1L, e_max) * 5 / 4;
+        grow(b.entries, 2 * E);
+        grow(b.eai, E);
+        grow(b.erw, 5 * E);
+        b.cap_e = E;
+    }
+    if ((size_t)std::max(1L, c_max) > b.cap_c) {
+        const size_t Cm = (size_t)std::max(1L, c_max) * 5 / 4;
+        grow(b.cells, Cm);
+        grow(b.cai, Cm);
+        grow(b.cw, Cm);
+        grow(b.cells3, 3 * Cm);
+        grow(b.ccnt, Cm);
+        grow(b.coff, Cm + 1);
+        b.cap_c = Cm;
+    }
+    const size_t E = b.cap_e, Cm = b.cap_c;
+    int* cnt = b.cnt;
     int* tot = cnt + 2 * b.ny + 2;
-    ILH_HIP(hipMalloc(&b.row_ptr, sizeof(int) * (size_t)(b.ny + 1)));
-    ILH_HIP(hipMalloc(&b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1)));
-    ILH_HIP(hipMalloc(&b.entries, sizeof(int) * 2 * E));
-    ILH_HIP(hipMalloc(&b.eai, sizeof(int) * E));
-    ILH_HIP(hipMalloc(&b.erw, sizeof(double) * 5 * E));
-    ILH_HIP(hipMalloc(&b.cells, sizeof(int4) * Cm));
-    ILH_HIP(hipMalloc(&b.cai, sizeof(int) * Cm));
-    ILH_HIP(hipMalloc(&b.cw, sizeof(double) * Cm));
-    cells3.ensure(sizeof(int) * 3 * Cm);
-    ccnt.ensure(sizeof(int) * Cm);
-    coff.ensure(sizeof(int) * (Cm + 1));
     int* ebx = b.entries + E;
-    fill_zero_int(st, cnt, 2 * b.ny + 2);
+    if (b.ny > 16384) fail(ILH_ERR_ARG, "grid too tall for the row bins (more than 16384 rows)");
+    const size_t lc_bytes = sizeof(int) * (size_t)b.ny;
     // entries per row, sorted by (bx, term)
-    if (nterms > 0)
-        k_bin_count<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, cnt);
+    k_bin_count<<<1, 1024, lc_bytes, st>>>(terms, nterms, gk, b.ny, cnt);
     k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.row_ptr, nullptr);
     if (nterms > 0) {
-        k_bin_fill<<<(nterms + 255) / 256, 256, 0, st>>>(terms, nterms, gk, b.ny, b.row_ptr, cursor, b.entries);
+        k_bin_fill<<<1, 1024, lc_bytes, st>>>(terms, nterms, gk, b.ny, b.row_ptr, b.entries);
         k_bin_sort<<<b.ny, 256, 0, st>>>(terms, b.row_ptr, b.entries);
         k_bin_bx<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(terms, b.entries, b.row_ptr + b.ny, ebx);
         k_bin_rec<<<b.ny, 64, 0, st>>>(terms, b.row_ptr, b.entries, b.eai, b.erw);
     }
     ILH_HIP(hipGetLastError());
     // cells per row
-    fill_zero_int(st, cnt, b.ny + 1);
-    if (nterms > 0) k_cell_count<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, cnt);
+    k_cell_count<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, cnt);
     k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.cell_ptr, nullptr);
     // contributions per cell
     const unsigned g = (unsigned)((Cm + 255) / 256);
     if (nterms > 0) {
-        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, b.cell_ptr, cells3.as<int>());
-        k_contrib_count<<<g, 256, 0, st>>>(cells3.as<int>(), b.cell_ptr + b.ny, ebx, b.erw, ccnt.as<int>());
+        k_cell_fill<<<b.ny, 256, 0, st>>>(b.row_ptr, ebx, b.erw, nx, b.cell_ptr, b.cells3);
+        k_contrib_count<<<g, 256, 0, st>>>(b.cells3, b.cell_ptr + b.ny, ebx, b.erw, b.ccnt);
     }
-    k_scan<<<1, 1024, 0, st>>>(ccnt.as<int>(), (int)c_max, coff.as<int>(), b.cell_ptr + b.ny);
+    k_scan<<<1, 1024, 0, st>>>(b.ccnt, (int)c_max, b.coff, b.cell_ptr + b.ny);
     if (nterms > 0)
-        k_contrib_fill<<<g, 256, 0, st>>>(cells3.as<int>(), b.cell_ptr + b.ny, ebx, b.eai, b.erw, coff.as<int>(),
-                                          b.cells, b.cai, b.cw);
-    k_bin_totals<<<1, 64, 0, st>>>(b.row_ptr, b.cell_ptr, coff.as<int>(), b.ny, tot);
+        k_contrib_fill<<<g, 256, 0, st>>>(b.cells3, b.cell_ptr + b.ny, ebx, b.eai, b.erw, b.coff, b.cells, b.cai,
+                                          b.cw);
+    k_bin_totals<<<1, 64, 0, st>>>(b.row_ptr, b.cell_ptr, b.coff, b.ny, tot);
     ILH_HIP(hipGetLastError());
     // one read-back: row and cell offsets and the totals
     std::vector<int> h(2 * (size_t)(b.ny + 1) + 3);
@@ -572,27 +596,18 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
 
 void free_bins(RowBins& b)
 {
-    if (b.row_ptr) (void)hipFree(b.row_ptr);
-    if (b.entries) (void)hipFree(b.entries);
-    if (b.eai) (void)hipFree(b.eai);
-    if (b.erw) (void)hipFree(b.erw);
-    if (b.cell_ptr) (void)hipFree(b.cell_ptr);
-    if (b.cells) (void)hipFree(b.cells);
-    if (b.cai) (void)hipFree(b.cai);
-    if (b.cw) (void)hipFree(b.cw);
-    b.cell_ptr = nullptr;
-    b.cells = nullptr;
-    b.n_cells = 0;
-    b.cai = nullptr;
-    b.cw = nullptr;
-    b.n_contrib = 0;
-    b.h_cell_ptr.clear();
-    b.row_ptr = nullptr;
-    b.entries = nullptr;
-    b.eai = nullptr;
-    b.erw = nullptr;
-    b.n_entries = 0;
-    b.h_row_ptr.clear();
+    for (void* p : {(void*)b.row_ptr, (void*)b.entries, (void*)b.eai, (void*)b.erw, (void*)b.cell_ptr, (void*)b.cells,
+                    (void*)b.cai, (void*)b.cw, (void*)b.cnt, (void*)b.cells3, (void*)b.ccnt, (void*)b.coff})
+        if (p) (void)hipFree(p);
+    b = RowBins();
+}
+
+void free_sources(SourceSet& s)
+{
+    free_bins(s.bins);
+    if (s.pinfo) (void)hipFree(s.pinfo);
+    if (s.slot_of_row) (void)hipFree(s.slot_of_row);
+    s = SourceSet();
 }
 
 // --------------------------------------------------------------- scatter

@@ -18,7 +18,9 @@ Reference entry points mirrored here (paths in the reference checkout):
   create_RTLinvR/CLinvCT/GLinvD/nRTRn
                               src/matrix_operators.jl:9-61, 135-155, 225-244
 """
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -28,6 +30,24 @@ from .geometry import GridSpec
 Primal, Dual = "Primal", "Dual"
 GridScaling = "GridScaling"
 Yang3, Roma, M4prime, Witchhat = L.DDF_YANG3, L.DDF_ROMA, L.DDF_M4PRIME, L.DDF_WITCHHAT
+
+
+# Objects that own library handles, closed at interpreter exit in dependency
+# order (buffers, heat integrators, batches, caches, contexts) while the library
+# and the HIP runtime are still fully loaded, instead of in whatever order
+# finalization reaches them.
+_LIVE_ORDER = ("buffer", "heat", "batch", "cache", "ctx")
+_live = {k: weakref.WeakSet() for k in _LIVE_ORDER}
+
+
+@atexit.register
+def _close_all():
+    for kind in _LIVE_ORDER:
+        for obj in list(_live[kind]):
+            try:
+                obj.free() if kind == "buffer" else obj.close()
+            except Exception:
+                pass
 
 
 # ----------------------------------------------------------------- context
@@ -40,6 +60,7 @@ class Context:
         L.call("ilh_ctx_create", device, C.byref(h))
         self.h = h
         self.device = device
+        _live["ctx"].add(self)
 
     def synchronize(self):
         L.call("ilh_ctx_synchronize", self.h)
@@ -74,6 +95,7 @@ class DeviceArray:
         p = C.c_void_p()
         L.call("ilh_malloc", ctx.h, C.c_size_t(8 * max(self.n, 1)), C.byref(p))
         self.ptr = p
+        _live["buffer"].add(self)
 
     @classmethod
     def from_host(cls, ctx, a):
@@ -182,6 +204,7 @@ class SurfaceScalarCache:
         L.call("ilh_plan_create", self.ctx.h, C.byref(grid), self.N, _ptr(self._X), _ptr(self._Y),
                _ptr(self._nx), _ptr(self._ny), _ptr(self._ds), ddftype, 0, C.byref(h))
         self.h = h
+        _live["cache"].add(self)
 
     def _set_points(self, s):
         self._X = np.ascontiguousarray(s.X, dtype=np.float64)
@@ -516,6 +539,7 @@ class DirichletBatch:
         L.call("ilh_batch_create", n, plans, ptrs(fp_devs), None if fm_devs is None else ptrs(fm_devs),
                ptrs(f_devs), ptrs(s_devs), C.byref(h))
         self.h, self._keep = h, (caches, fp_devs, f_devs, s_devs, fm_devs)
+        _live["batch"].add(self)
 
     def launch(self):
         L.call("ilh_batch_launch", self.h)
@@ -541,6 +565,7 @@ class DirichletHeatConduction:
         h = C.c_void_p()
         L.call("ilh_heat_create", cache.h, C.c_double(kappa), C.c_double(dt), C.byref(h))
         self.h, self.cache, self.kappa, self.dt = h, cache, kappa, dt
+        _live["heat"].add(self)
         self.set_boundary(tb_plus, tb_minus)
 
     def set_boundary(self, tb_plus, tb_minus=None):

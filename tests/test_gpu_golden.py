@@ -249,3 +249,28 @@ def test_heat_if_herk_matches_oracle(ctx):
             ilhip.interpolate(Eb, T, cache)
             assert np.abs(Eb.flat() - 0.5).max() < 1e-10      # E T = (T_b+ + T_b-)/2, T_b- = 0
         heat.close()
+
+
+def test_handles_outlive_in_any_order():
+    """A plan keeps its context alive and a heat integrator its plan: closing
+    the caller's handles in any order is safe (the library tears a context
+    down when its last plan lets go)."""
+    import ilhip
+    from ilhip import geometry as G
+    g, s, fb = G.config_plain()
+    ctx = ilhip.Context(0)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    heat = ilhip.DirichletHeatConduction(cache, 1.0, 0.25 * g.dx ** 2, np.ones(s.N))
+    ctx.close()                               # the plan still holds the context
+    f1, s1 = ilhip.DirichletPoisson(cache).solve(fb)
+    cache.close()                             # the heat integrator still holds the plan
+    T = ilhip.Nodes(ilhip.Primal, g)
+    heat.advance(T, 1)
+    heat.close()                              # last owner: plan and context go
+    ctx2 = ilhip.Context(0)
+    cache2 = ilhip.SurfaceScalarCache(s, g, ctx=ctx2)
+    f2, s2 = ilhip.DirichletPoisson(cache2).solve(fb)
+    assert np.array_equal(f1.flat(), f2.flat()) and np.array_equal(s1.flat(), s2.flat())
+    assert np.isfinite(T.flat()).all() and T.flat().max() > 0
+    cache2.close()
+    ctx2.close()
