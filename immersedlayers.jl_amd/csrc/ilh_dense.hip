@@ -33,7 +33,11 @@ static constexpr int GBOX = 4096;   // doubles of the kernel table staged per ti
 // curve), independent of how far apart the two tiles are; the box is staged
 // in LDS when it fits, and the table is read from memory otherwise.
 // ROWMAJOR: K[q + ldk p] (lanes along q: coalesced stores), else K[p + ldk q].
-template <bool ROWMAJOR>
+// SYM: one term per point and identical windows in A and B (E and R of one
+// grid kind): the kernel sum is symmetric in (p, q), so only tiles on or
+// above the diagonal run and also write the mirrored entry
+// K[q, p] = factor coefA_q coefB_p (same sum).
+template <bool ROWMAJOR, bool SYM>
 __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ TA, int TAn, const Term* __restrict__ TB,
                                                       int TBn, int N, const double* __restrict__ G, int Tn,
                                                       double factor, int delta, double* __restrict__ K, int ldk)
@@ -41,6 +45,8 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
     __shared__ Term sa[AT * 2], sb[AT * 2];
     __shared__ double sG[GBOX];
     __shared__ int sbox[4];
+    __shared__ double cAq[AT], cBp[AT];   // SYM: coefficients of the mirrored entries
+    if (SYM && blockIdx.y < blockIdx.x) return;
     const int p0 = blockIdx.x * AT, q0 = blockIdx.y * AT;
     const int tid = threadIdx.x + AT * threadIdx.y;
     const int na = AT * TAn, nb = AT * TBn;
@@ -51,6 +57,12 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
         const int k = tid - 64;
         const int q = q0 + k / TBn;
         if (q < N) sb[k] = TB[(size_t)q * TBn + k % TBn];
+    } else if (SYM && tid >= 128 && tid < 128 + AT) {
+        const int k = tid - 128;
+        if (q0 + k < N) cAq[k] = TA[q0 + k].coef;
+    } else if (SYM && tid >= 192 && tid < 192 + AT) {
+        const int k = tid - 192;
+        if (p0 + k < N) cBp[k] = TB[p0 + k].coef;
     }
     __syncthreads();
     if (!delta && tid < 64) {
@@ -105,7 +117,7 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
     const int qi = ROWMAJOR ? threadIdx.x : threadIdx.y;
     const int p = p0 + pi, q = q0 + qi;
     if (p >= N || q >= N) return;
-    double acc = 0.0;
+    double acc = 0.0, wsym = 0.0;
     for (int ia = 0; ia < TAn; ++ia) {
         const Term& a = sa[pi * TAn + ia];
         for (int ib = 0; ib < TBn; ++ib) {
@@ -155,12 +167,20 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
                 }
             }
             acc += a.coef * b.coef * s;
+            wsym = s;
         }
     }
     if (ROWMAJOR)
         K[q + (size_t)ldk * p] = factor * acc;
     else
         K[p + (size_t)ldk * q] = factor * acc;
+    if (SYM && blockIdx.x != blockIdx.y) {
+        const double mirror = factor * (cAq[qi] * cBp[pi] * wsym);
+        if (ROWMAJOR)
+            K[p + (size_t)ldk * q] = mirror;
+        else
+            K[q + (size_t)ldk * p] = mirror;
+    }
 }
 
 static void launch_assemble(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor, bool delta,
@@ -173,12 +193,18 @@ static void launch_assemble(ilh_plan* pl, int opA, int opB, const double* tab, i
     dim3 grid((N + AT - 1) / AT, (N + AT - 1) / AT);
     const int ta = op_terms_per_point(opA), tb = op_terms_per_point(opB);
     if (ta > 2 || tb > 2) fail(ILH_ERR_ARG, "assembly supports at most two terms per point");
-    if (rowmajor)
-        k_assemble<true><<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, ta, B, tb, N, tab, Tn, factor, delta ? 1 : 0,
-                                                                     K, ldk);
+    // E and R of one grid kind share their windows (one term per point)
+    const bool sym = !delta && ((opA == OP_E_PRIMAL && opB == OP_R_PRIMAL) || (opA == OP_E_DUAL && opB == OP_R_DUAL));
+    const int dl = delta ? 1 : 0;
+    hipStream_t st = pl->ctx->stream;
+    if (rowmajor && sym)
+        k_assemble<true, true><<<grid, dim3(AT, AT), 0, st>>>(A, ta, B, tb, N, tab, Tn, factor, dl, K, ldk);
+    else if (rowmajor)
+        k_assemble<true, false><<<grid, dim3(AT, AT), 0, st>>>(A, ta, B, tb, N, tab, Tn, factor, dl, K, ldk);
+    else if (sym)
+        k_assemble<false, true><<<grid, dim3(AT, AT), 0, st>>>(A, ta, B, tb, N, tab, Tn, factor, dl, K, ldk);
     else
-        k_assemble<false><<<grid, dim3(AT, AT), 0, pl->ctx->stream>>>(A, ta, B, tb, N, tab, Tn, factor, delta ? 1 : 0,
-                                                                      K, ldk);
+        k_assemble<false, false><<<grid, dim3(AT, AT), 0, st>>>(A, ta, B, tb, N, tab, Tn, factor, dl, K, ldk);
     ILH_HIP(hipGetLastError());
 }
 

@@ -289,12 +289,13 @@ template <int P> constexpr int R_last(bool rev) { return radix(P, rev, n_stages(
 // Work-groups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
 // Renumber blocks so that each XCD gets a run of consecutive logical blocks:
 // neighbours that share 128-byte lines of the tiled spectra then meet in one L2.
-__device__ __forceinline__ int xcd_block()
+__device__ __forceinline__ int xcd_block_of(int b, int nblk)
 {
-    const int nblk = gridDim.x, full = (nblk / 8) * 8;
-    const int b = blockIdx.x;
+    const int full = (nblk / 8) * 8;
     return b < full ? (b % 8) * (nblk / 8) + b / 8 : b;
 }
+
+__device__ __forceinline__ int xcd_block() { return xcd_block_of(blockIdx.x, gridDim.x); }
 
 // --------------------------------------------------------- row sources
 enum RowMode { ROW_DENSE = 0, ROW_LGF = 1, ROW_TERMS = 2 };
@@ -312,6 +313,8 @@ struct RowSrc {
     const double* amp;
     int amp_len;
     int amp_lds;           // 1: stage amp[0..amp_len) in LDS after the transform buffers
+    const int* npairs_dev; // surface sources: the live pair count on the device (the
+                           // launch covers an upper bound), or null
 };
 
 // values of two cells (a second with c0 == c1 is empty): each cell's
@@ -380,7 +383,15 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
     constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
     constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-    const int p = xcd_block() * NB + c;
+    // live pairs from the device when the launch covers an upper bound: the
+    // first ceil(npairs / NB) blocks work (spread over the XCDs), the rest exit
+    int nblk = gridDim.x;
+    if (S.npairs_dev) {
+        npairs = *S.npairs_dev;
+        nblk = (npairs + NB - 1) / NB;
+        if ((int)blockIdx.x >= nblk) return;
+    }
+    const int p = xcd_block_of(blockIdx.x, nblk) * NB + c;
     const bool live = p < npairs;
     double2* x = lds + (size_t)c * lds_len(P);
     double2 v[V];
@@ -864,42 +875,77 @@ void conv_dense(ilh_plan* pl, int gk, const double* in, double* out)
     launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
 }
 
+// Row pairs (2p, 2p+1) with at least one entry, compacted in order: pinfo[k]
+// = {2p, first cell of row 2p, first cell of row 2p+1, end of the pair's
+// cells}, slot_of_row[2p] = 2k, slot_of_row[2p+1] = 2k+1 (-1 for rows of idle
+// pairs), *npairs = live pairs.  One work-group of 1024 threads.
+__global__ void __launch_bounds__(1024) k_source_pairs(const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ cell_ptr, int ny,
+                                                       int4* __restrict__ pinfo, int* __restrict__ slot_of_row,
+                                                       int* __restrict__ npairs)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int np = (ny + 1) / 2;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < np; base += 1024) {
+        const int p = base + t;
+        const int j = 2 * p;
+        bool on = false;
+        if (p < np) {
+            on = row_ptr[j + 1] > row_ptr[j];
+            if (j + 1 < ny) on = on || row_ptr[j + 2] > row_ptr[j + 1];
+        }
+        const int f = on ? 1 : 0;
+        int inc = f;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += u;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int k = carry;
+        for (int w = 0; w < wave; ++w) k += wsum[w];
+        k += inc - f;
+        if (p < np) {
+            if (on) pinfo[k] = make_int4(j, cell_ptr[j], cell_ptr[j + 1], cell_ptr[min(j + 2, ny)]);
+            slot_of_row[j] = on ? 2 * k : -1;
+            if (j + 1 < ny) slot_of_row[j + 1] = on ? 2 * k + 1 : -1;
+        }
+        __syncthreads();
+        if (t == 1023) carry = k + f;
+        __syncthreads();
+    }
+    if (t == 0) *npairs = carry;
+}
+
+// The source set's pair tables, built on the device from its row bins (no
+// read-back: the row pass launches over the upper bound (ny + 1) / 2 and
+// reads the live count).
 void build_source_pairs(ilh_plan* pl, SourceSet& s)
 {
-    const std::vector<int>& rp = s.bins.h_row_ptr;
-    const std::vector<int>& cp = s.bins.h_cell_ptr;
     const int ny = s.bins.ny;
-    std::vector<int4> info;
-    std::vector<int> slot(ny, -1);
-    for (int j = 0; j < ny; j += 2) {
-        const bool a = rp[j + 1] > rp[j];
-        const bool b = (j + 1 < ny) && (rp[j + 2] > rp[j + 1]);
-        if (a || b) {
-            const int k = (int)info.size();
-            info.push_back(make_int4(j, cp[j], cp[j + 1], cp[std::min(j + 2, ny)]));
-            slot[j] = 2 * k;
-            if (j + 1 < ny) slot[j + 1] = 2 * k + 1;
-        }
-    }
-    s.npairs = (int)info.size();
+    const size_t np = (size_t)(ny + 1) / 2;
     // tables are kept from the previous geometry when large enough
-    if ((size_t)std::max(1, s.npairs) > s.cap_pairs) {
+    if (np + 1 > s.cap_pairs) {
         if (s.pinfo) (void)hipFree(s.pinfo);
         s.pinfo = nullptr;
-        s.cap_pairs = (size_t)ny / 2 + 1;
+        s.cap_pairs = np + 1;
         ILH_HIP(hipMalloc(&s.pinfo, sizeof(int4) * s.cap_pairs));
     }
-    if ((size_t)std::max(1, ny) > s.cap_rows) {
+    if ((size_t)ny + 1 > s.cap_rows) {
         if (s.slot_of_row) (void)hipFree(s.slot_of_row);
         s.slot_of_row = nullptr;
-        s.cap_rows = (size_t)std::max(1, ny);
-        ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * s.cap_rows));
+        s.cap_rows = (size_t)ny + 1;
+        ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * s.cap_rows));   // rows, then the live count
     }
-    hipStream_t st = pl->ctx->stream;
-    if (s.npairs)
-        ILH_HIP(hipMemcpyAsync(s.pinfo, info.data(), sizeof(int4) * (size_t)s.npairs, hipMemcpyHostToDevice, st));
-    ILH_HIP(hipMemcpyAsync(s.slot_of_row, slot.data(), sizeof(int) * (size_t)ny, hipMemcpyHostToDevice, st));
-    ILH_HIP(hipStreamSynchronize(st));
+    s.npairs = (int)np;
+    s.npairs_dev = s.slot_of_row + ny;
+    k_source_pairs<<<1, 1024, 0, pl->ctx->stream>>>(s.bins.row_ptr, s.bins.cell_ptr, ny, s.pinfo, s.slot_of_row,
+                                                    s.npairs_dev);
+    ILH_HIP(hipGetLastError());
 }
 
 static RowSrc terms_source(const ConvPlan& C, const SourceSet& src, const double* amp)
@@ -914,6 +960,7 @@ static RowSrc terms_source(const ConvPlan& C, const SourceSet& src, const double
     S.cw = src.bins.cw;
     S.amp = amp;
     S.amp_len = src.amp_len;
+    S.npairs_dev = src.npairs_dev;
     return S;
 }
 

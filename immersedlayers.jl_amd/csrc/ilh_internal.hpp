@@ -86,20 +86,14 @@ struct RowBins {
     int* entries = nullptr;     // device: term indices, then their bx
     int* eai = nullptr;         // device: amplitude index per entry
     double* erw = nullptr;      // device: 5 weights per entry
-    int n_entries = 0;
     // nonzero cells of each row, ascending x: cells[c] = {x, c0, c1, 0}; the
     // cell's value is sum_{c0 <= e < c1} cw[e] * amp[cai[e]] in that order (the
     // row's entries k with bx in [x-4, x] and a nonzero weight erw[5k + x - bx])
     int* cell_ptr = nullptr;    // device, ny + 1
     int4* cells = nullptr;      // device, one per cell
-    int n_cells = 0;
     int* cai = nullptr;         // device: amplitude index per contribution
     double* cw = nullptr;       // device: weight per contribution
-    int n_contrib = 0;
-    std::vector<int> h_row_ptr;  // host copies (row counts drive launch shapes)
-    std::vector<int> h_cell_ptr;
-    // build scratch: row counts, cursors and totals; per-cell records, counts
-    // and offsets
+    // build scratch: row counts; per-cell records, counts and offsets
     int* cnt = nullptr;
     int* cells3 = nullptr;
     int* ccnt = nullptr;
@@ -213,7 +207,8 @@ struct SourceSet {
     RowBins bins;
     int4* pinfo = nullptr;     // device, per nonzero row pair p: {j0, first cell, first cell of row j0+1, end}
     int* slot_of_row = nullptr;  // device: field row -> row of the compacted spectrum, or -1
-    int npairs = 0;
+    int npairs = 0;            // upper bound of the live row pairs ((ny + 1) / 2)
+    int* npairs_dev = nullptr; // device: live row pairs (after slot_of_row's rows)
     size_t cap_pairs = 0, cap_rows = 0;   // allocated sizes of pinfo and slot_of_row
 };
 void free_sources(SourceSet& s);   // releases the bins and pair tables

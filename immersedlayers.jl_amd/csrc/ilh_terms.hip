@@ -495,19 +495,9 @@ __global__ void k_contrib_fill(const int* __restrict__ cells3, const int* __rest
     cells[c] = make_int4(x, off[c], e, 0);
 }
 
-// totals: n_entries, n_cells, n_contrib
-__global__ void k_bin_totals(const int* __restrict__ row_ptr, const int* __restrict__ cell_ptr,
-                             const int* __restrict__ coff, int ny, int* __restrict__ tot)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    tot[0] = row_ptr[ny];
-    tot[1] = cell_ptr[ny];
-    tot[2] = coff[cell_ptr[ny]];
-}
-
 // Row bins of a term list for one output grid kind.  Buffers are sized by
 // upper bounds (a term touches at most 5 rows, an entry at most 5 cells), so
-// the whole build runs on the stream with one read-back at the end.
+// the whole build runs asynchronously on the stream.
 void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
 {
     hipStream_t st = pl->ctx->stream;
@@ -549,7 +539,6 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
     }
     const size_t E = b.cap_e, Cm = b.cap_c;
     int* cnt = b.cnt;
-    int* tot = cnt + 2 * b.ny + 2;
     int* ebx = b.entries + E;
     if (b.ny > 16384) fail(ILH_ERR_ARG, "grid too tall for the row bins (more than 16384 rows)");
     const size_t lc_bytes = sizeof(int) * (size_t)b.ny;
@@ -576,22 +565,10 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
     if (nterms > 0)
         k_contrib_fill<<<g, 256, 0, st>>>(b.cells3, b.cell_ptr + b.ny, ebx, b.eai, b.erw, b.coff, b.cells, b.cai,
                                           b.cw);
-    k_bin_totals<<<1, 64, 0, st>>>(b.row_ptr, b.cell_ptr, b.coff, b.ny, tot);
     ILH_HIP(hipGetLastError());
-    // one read-back: row and cell offsets and the totals
-    std::vector<int> h(2 * (size_t)(b.ny + 1) + 3);
-    ILH_HIP(hipMemcpyAsync(h.data(), b.row_ptr, sizeof(int) * (size_t)(b.ny + 1), hipMemcpyDeviceToHost, st));
-    ILH_HIP(hipMemcpyAsync(h.data() + b.ny + 1, b.cell_ptr, sizeof(int) * (size_t)(b.ny + 1), hipMemcpyDeviceToHost,
-                           st));
-    ILH_HIP(hipMemcpyAsync(h.data() + 2 * (b.ny + 1), tot, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
-    ILH_HIP(hipStreamSynchronize(st));
-    b.h_row_ptr.assign(h.begin(), h.begin() + b.ny + 1);
-    b.h_cell_ptr.assign(h.begin() + b.ny + 1, h.begin() + 2 * (b.ny + 1));
-    b.n_entries = h[2 * (b.ny + 1)];
-    b.n_cells = h[2 * (b.ny + 1) + 1];
-    b.n_contrib = h[2 * (b.ny + 1) + 2];
-    if (b.n_entries > e_max || b.n_cells > c_max || b.n_contrib > c_max)
-        fail(ILH_ERR_HIP, "row bins exceeded their bounds");
+    // No read-back: every count stays on the device.  The buffers' bounds
+    // hold by construction (entries <= 5 per term, cells and contributions
+    // <= 5 per entry).
 }
 
 void free_bins(RowBins& b)
