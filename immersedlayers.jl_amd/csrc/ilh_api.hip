@@ -1206,10 +1206,15 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
 {
     __shared__ double h[2][256];
     __shared__ double red[16];
+    __shared__ double rc[256], rs[256];   // the previous rotations, staged for thread 0
     if (flags[0]) return;
     const GmLayout L{m};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nv = j + 1;
+    if (tid < j) {
+        rc[tid] = st[L.cs() + tid];
+        rs[tid] = st[L.sn() + tid];
+    }
     double a = 0.0;
     for (int k = tid; k < n; k += 1024) a += w[k] * w[k];
     double wn = sqrt(block_sum_1024(a, red));   // |w| before the pass
@@ -1250,25 +1255,29 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
         for (int k = tid; k < n; k += 1024) vn[k] = w[k] * inv;
     }
     if (tid == 0) {
-        double* H = st + L.H() + (size_t)(m + 1) * j;
-        double* cs = st + L.cs();
-        double* sn = st + L.sn();
-        double* g = st + L.g();
-        for (int i = 0; i <= j; ++i) H[i] = h[0][i] + h[1][i];
-        H[j + 1] = hn;
+        // the column in LDS (h[0]), rotations from LDS: no dependent global
+        // loads in the serial part
+        double* col = h[0];
+        for (int i = 0; i <= j; ++i) col[i] += h[1][i];
+        col[j + 1] = hn;
         for (int i = 0; i < j; ++i) {
-            const double t = cs[i] * H[i] + sn[i] * H[i + 1];
-            H[i + 1] = -sn[i] * H[i] + cs[i] * H[i + 1];
-            H[i] = t;
+            const double t = rc[i] * col[i] + rs[i] * col[i + 1];
+            col[i + 1] = -rs[i] * col[i] + rc[i] * col[i + 1];
+            col[i] = t;
         }
-        const double r = hypot(H[j], H[j + 1]);
-        cs[j] = r == 0.0 ? 1.0 : H[j] / r;
-        sn[j] = r == 0.0 ? 0.0 : H[j + 1] / r;
-        H[j] = r;
-        H[j + 1] = 0.0;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
-        if (fabs(g[j + 1]) <= st[L.tol()] * st[L.bnorm()] || hn == 0.0) {
+        const double r = hypot(col[j], col[j + 1]);
+        const double cj = r == 0.0 ? 1.0 : col[j] / r, sj = r == 0.0 ? 0.0 : col[j + 1] / r;
+        col[j] = r;
+        col[j + 1] = 0.0;
+        double* H = st + L.H() + (size_t)(m + 1) * j;
+        for (int i = 0; i <= j + 1; ++i) H[i] = col[i];
+        double* g = st + L.g();
+        const double gj = g[j];
+        st[L.cs() + j] = cj;
+        st[L.sn() + j] = sj;
+        g[j + 1] = -sj * gj;
+        g[j] = cj * gj;
+        if (fabs(-sj * gj) <= st[L.tol()] * st[L.bnorm()] || hn == 0.0) {
             flags[0] = 1;
             flags[1] = j + 1;
         } else if (j + 1 == m) {
@@ -1285,14 +1294,20 @@ __global__ void __launch_bounds__(1024) k_gm_finish(const double* __restrict__ s
                                                     double* __restrict__ w)
 {
     __shared__ double y[256];
+    __shared__ double Hs[64 * 64];   // the leading j x j block of H when j <= 64
     const GmLayout L{m};
+    const double* H = st + L.H();
+    const bool staged = j <= 64;
+    if (staged)
+        for (int e = threadIdx.x; e < j * j; e += 1024) Hs[e] = H[e % j + (size_t)(m + 1) * (e / j)];
+    for (int i = threadIdx.x; i < j; i += 1024) y[i] = st[L.g() + i];
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const double* H = st + L.H();
-        const double* g = st + L.g();
         for (int i = j - 1; i >= 0; --i) {
-            double t = g[i];
-            for (int k = i + 1; k < j; ++k) t -= H[i + (size_t)(m + 1) * k] * y[k];
-            y[i] = t / H[i + (size_t)(m + 1) * i];
+            double t = y[i];
+            for (int k = i + 1; k < j; ++k)
+                t -= (staged ? Hs[i + j * k] : H[i + (size_t)(m + 1) * k]) * y[k];
+            y[i] = t / (staged ? Hs[i + j * i] : H[i + (size_t)(m + 1) * i]);
         }
     }
     __syncthreads();
