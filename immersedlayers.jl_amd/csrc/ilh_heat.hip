@@ -70,37 +70,94 @@ static std::vector<double> heat_kernel_1d(double a)
 
 namespace {
 
-constexpr int HX = 64, HY = 4;
+constexpr int HX = 64, HY = 4;          // naive kernels (long filters)
+constexpr int XT = 256;                  // x pass: outputs per work-group (one row segment)
+constexpr int YC = 64, YR = 64;          // y pass: columns x rows of outputs per work-group
+constexpr int MX_LDS = 64, MY_LDS = 32;  // longest filters the tiled passes take
 
-// x pass: out(i,j) = sum_{|p|<=M} K(|p|) in(i-p, j), zero outside [0, nx)
-__global__ void k_heat_x(const double* __restrict__ in, int nx, int ny, const double* __restrict__ K, int M,
-                         double* __restrict__ out)
+// x pass: out(i,j) = sum_{|p|<=M} K(|p|) u(i-p, j), u = a + c b (b may be
+// null), zero outside [0, nx).  The row segment and its halo are staged in LDS.
+__global__ void __launch_bounds__(XT) k_heat_x(const double* __restrict__ a, double c, const double* __restrict__ b,
+                                               int nx, int ny, const double* __restrict__ K, int M,
+                                               double* __restrict__ out)
+{
+    __shared__ double buf[XT + 2 * MX_LDS];
+    const int j = blockIdx.y, x0 = blockIdx.x * XT, t = threadIdx.x;
+    const double* ra = a + (size_t)nx * j;
+    const double* rb = b ? b + (size_t)nx * j : nullptr;
+    for (int e = t; e < XT + 2 * M; e += XT) {
+        const int x = x0 - M + e;
+        double v = 0.0;
+        if (x >= 0 && x < nx) v = rb ? ra[x] + c * rb[x] : ra[x];
+        buf[e] = v;
+    }
+    __syncthreads();
+    const int i = x0 + t;
+    if (i >= nx) return;
+    double acc = K[0] * buf[t + M];
+    for (int p = 1; p <= M; ++p) acc += K[p] * (buf[t + M - p] + buf[t + M + p]);
+    out[i + (size_t)nx * j] = acc;
+}
+
+// y pass: out(i,j) = sum_{|p|<=M} K(|p|) in(i, j-p) + c add(i,j) (add may be
+// null), zero outside [0, ny).  A 64-column strip of YR + 2M rows is staged
+// in LDS (dynamic, (YR + 2M) x 64 doubles).
+__global__ void __launch_bounds__(256) k_heat_y(const double* __restrict__ in, int nx, int ny,
+                                                const double* __restrict__ K, int M, double c,
+                                                const double* __restrict__ add, double* __restrict__ out)
+{
+    extern __shared__ double strip[];   // [YR + 2M][YC]
+    const int tx = threadIdx.x & (YC - 1), ty = threadIdx.x / YC;   // 64 x 4
+    const int i = blockIdx.x * YC + tx, y0 = blockIdx.y * YR;
+    const int rows = YR + 2 * M;
+    for (int r = ty; r < rows; r += 4) {
+        const int y = y0 - M + r;
+        strip[r * YC + tx] = (i < nx && y >= 0 && y < ny) ? in[i + (size_t)nx * y] : 0.0;
+    }
+    __syncthreads();
+    if (i >= nx) return;
+    for (int r = ty; r < YR; r += 4) {
+        const int y = y0 + r;
+        if (y >= ny) break;
+        const double* col = strip + (r + M) * YC + tx;
+        double acc = K[0] * col[0];
+        for (int p = 1; p <= M; ++p) acc += K[p] * (col[-p * YC] + col[p * YC]);
+        const size_t o = i + (size_t)nx * y;
+        out[o] = add ? acc + c * add[o] : acc;
+    }
+}
+
+// the same passes for filters longer than the tiled ones take
+__global__ void k_heat_x_long(const double* __restrict__ a, double c, const double* __restrict__ b, int nx, int ny,
+                              const double* __restrict__ K, int M, double* __restrict__ out)
 {
     const int i = blockIdx.x * HX + threadIdx.x, j = blockIdx.y * HY + threadIdx.y;
     if (i >= nx || j >= ny) return;
-    const double* row = in + (size_t)nx * j;
-    double acc = K[0] * row[i];
+    const double* ra = a + (size_t)nx * j;
+    const double* rb = b ? b + (size_t)nx * j : nullptr;
+    auto u = [&](int x) { return rb ? ra[x] + c * rb[x] : ra[x]; };
+    double acc = K[0] * u(i);
     for (int p = 1; p <= M; ++p) {
-        const double a = i - p >= 0 ? row[i - p] : 0.0;
-        const double b = i + p < nx ? row[i + p] : 0.0;
-        acc += K[p] * (a + b);
+        const double l = i - p >= 0 ? u(i - p) : 0.0;
+        const double r = i + p < nx ? u(i + p) : 0.0;
+        acc += K[p] * (l + r);
     }
     out[i + (size_t)nx * j] = acc;
 }
 
-// y pass, the same filter along j
-__global__ void k_heat_y(const double* __restrict__ in, int nx, int ny, const double* __restrict__ K, int M,
-                         double* __restrict__ out)
+__global__ void k_heat_y_long(const double* __restrict__ in, int nx, int ny, const double* __restrict__ K, int M,
+                              double c, const double* __restrict__ add, double* __restrict__ out)
 {
     const int i = blockIdx.x * HX + threadIdx.x, j = blockIdx.y * HY + threadIdx.y;
     if (i >= nx || j >= ny) return;
     double acc = K[0] * in[i + (size_t)nx * j];
     for (int p = 1; p <= M; ++p) {
-        const double a = j - p >= 0 ? in[i + (size_t)nx * (j - p)] : 0.0;
-        const double b = j + p < ny ? in[i + (size_t)nx * (j + p)] : 0.0;
-        acc += K[p] * (a + b);
+        const double l = j - p >= 0 ? in[i + (size_t)nx * (j - p)] : 0.0;
+        const double r = j + p < ny ? in[i + (size_t)nx * (j + p)] : 0.0;
+        acc += K[p] * (l + r);
     }
-    out[i + (size_t)nx * j] = acc;
+    const size_t o = i + (size_t)nx * j;
+    out[o] = add ? acc + c * add[o] : acc;
 }
 
 // out = x + c y
@@ -131,13 +188,24 @@ static void lin(hipStream_t st, double* out, const double* x, double c, const do
     ILH_HIP(hipGetLastError());
 }
 
-// out = exp(tau kappa L) in  (tmp: one grid field of scratch; out may alias in)
-static void heat_apply(ilh_plan* pl, const double* K, int M, const double* in, double* tmp, double* out)
+// out = exp(tau kappa L) (a + ca b) + cp add   (b, add may be null; tmp: one
+// grid field of scratch; out may alias a, b or add)
+static void heat_apply(ilh_plan* pl, const double* K, int M, const double* a, double ca, const double* b,
+                       double* tmp, double* out, double cp = 0.0, const double* add = nullptr)
 {
     const int nx = pl->g.nx(GK_PRIMAL), ny = pl->g.ny(GK_PRIMAL);
-    dim3 grid((nx + HX - 1) / HX, (ny + HY - 1) / HY);
-    k_heat_x<<<grid, dim3(HX, HY), 0, pl->ctx->stream>>>(in, nx, ny, K, M, tmp);
-    k_heat_y<<<grid, dim3(HX, HY), 0, pl->ctx->stream>>>(tmp, nx, ny, K, M, out);
+    hipStream_t st = pl->ctx->stream;
+    if (M <= MX_LDS)
+        k_heat_x<<<dim3((nx + XT - 1) / XT, ny), XT, 0, st>>>(a, ca, b, nx, ny, K, M, tmp);
+    else
+        k_heat_x_long<<<dim3((nx + HX - 1) / HX, (ny + HY - 1) / HY), dim3(HX, HY), 0, st>>>(a, ca, b, nx, ny, K, M,
+                                                                                          tmp);
+    if (M <= MY_LDS)
+        k_heat_y<<<dim3((nx + YC - 1) / YC, (ny + YR - 1) / YR), 256, sizeof(double) * (YR + 2 * M) * YC, st>>>(
+            tmp, nx, ny, K, M, cp, add, out);
+    else
+        k_heat_y_long<<<dim3((nx + HX - 1) / HX, (ny + HY - 1) / HY), dim3(HX, HY), 0, st>>>(tmp, nx, ny, K, M, cp,
+                                                                                          add, out);
     ILH_HIP(hipGetLastError());
 }
 
@@ -271,28 +339,21 @@ extern "C" int ilh_heat_advance(ilh_heat* h, double* T, int nsteps, int mem)
         };
         for (int n = 0; n < nsteps; ++n) {
             if (N == 0) {
-                // no bodies: T <- H1 T + dt (b2 H23 F + b3 F)
-                heat_apply(pl, K13, h->M13, d_T, Bt, A);
-                lin(st, A, A, dt * B2, F, ng);
-                heat_apply(pl, K23, h->M23, A, Bt, Y);
-                lin(st, d_T, Y, dt * B3, F, ng);
+                // no bodies: T <- H23 [H13 T + dt b2 F] + dt b3 F
+                heat_apply(pl, K13, h->M13, d_T, 0.0, nullptr, Bt, A, dt * B2, F);
+                heat_apply(pl, K23, h->M23, A, 0.0, nullptr, Bt, d_T, dt * B3, F);
                 continue;
             }
             // stage 2: Yhat2 = H13 (T + dt a21 F)
-            lin(st, A, d_T, dt * A21, F, ng);
-            heat_apply(pl, K13, h->M13, A, Bt, Y);
+            heat_apply(pl, K13, h->M13, d_T, dt * A21, F, Bt, Y);
             multiplier(Y, h->S13.as<double>(), A21, W1);
             // stage 3: Yhat3 = H23 [H13 (T + dt a31 W1) + dt a32 F]
-            lin(st, A, d_T, dt * A31, W1, ng);
-            heat_apply(pl, K13, h->M13, A, Bt, A);
-            lin(st, A, A, dt * A32, F, ng);
-            heat_apply(pl, K23, h->M23, A, Bt, Y);
+            heat_apply(pl, K13, h->M13, d_T, dt * A31, W1, Bt, A, dt * A32, F);
+            heat_apply(pl, K23, h->M23, A, 0.0, nullptr, Bt, Y);
             multiplier(Y, h->S23.as<double>(), A32, W2);
             // final: yhat = H23 [H13 T + dt b2 W2] + dt b3 F;  T = yhat + dt b3 R sigma3
-            heat_apply(pl, K13, h->M13, d_T, Bt, A);
-            lin(st, A, A, dt * B2, W2, ng);
-            heat_apply(pl, K23, h->M23, A, Bt, Y);
-            lin(st, Y, Y, dt * B3, F, ng);
+            heat_apply(pl, K13, h->M13, d_T, 0.0, nullptr, Bt, A, dt * B2, W2);
+            heat_apply(pl, K23, h->M23, A, 0.0, nullptr, Bt, Y, dt * B3, F);
             multiplier(Y, h->S0.as<double>(), B3, nullptr);
             lin(st, d_T, Y, dt * B3, Rs, ng);
         }
