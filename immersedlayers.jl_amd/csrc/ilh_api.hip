@@ -947,6 +947,125 @@ extern "C" int ilh_mask_multiply(ilh_plan* pl, int grid_kind, int complementary,
     return mask_call(pl, grid_kind, complementary, w, mem, true);
 }
 
+// ----------------------------------------------------------- surface filter
+namespace {
+
+// C = Et R (create_surface_filter, src/matrix_operators.jl:254-267): Et is the
+// interpolation with each grid value divided by (R 1) there, so that C keeps
+// constants and the surface integral (sum_p ds_p (C s)_p = sum_q ds_q s_q).
+// One thread per point p: its 4 x 4 support cells in (row, x) order, each
+// cell's contributions (points q with R weight cw) in the bins' order.
+// C column-major N x N, zeroed by the caller.
+__global__ void k_filter_build(const Term* __restrict__ TE, int N, const int* __restrict__ cell_ptr,
+                               const int4* __restrict__ cells, const int* __restrict__ cai,
+                               const double* __restrict__ cw, int ny, double* __restrict__ C)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const Term& t = TE[p];
+    for (int b = 0; b < 5; ++b) {
+        const int j = t.by + b;
+        if (t.wy[b] == 0.0 || j < 0 || j >= ny) continue;
+        const int lo = cell_ptr[j], hi = cell_ptr[j + 1];
+        for (int a = 0; a < 5; ++a) {
+            if (t.wx[a] == 0.0) continue;
+            const int x = t.bx + a;
+            int l = lo, h = hi;   // the cell at x (cells of a row ascend in x)
+            while (l < h) {
+                const int m = (l + h) >> 1;
+                if (cells[m].x < x) l = m + 1; else h = m;
+            }
+            if (l >= hi || cells[l].x != x) continue;
+            const int4 cl = cells[l];
+            double r1 = 0.0;
+            for (int e = cl.y; e < cl.z; ++e) r1 += cw[e];
+            if (r1 == 0.0) continue;
+            const double phi = t.coef * t.wx[a] * t.wy[b] / r1;
+            for (int e = cl.y; e < cl.z; ++e) C[p + (size_t)N * cai[e]] += phi * cw[e];
+        }
+    }
+}
+
+// g <- g / (R 1) where R 1 != 0, else 0
+__global__ void k_filter_weight(double* __restrict__ g, const double* __restrict__ r1, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        g[k] = r1[k] != 0.0 ? g[k] / r1[k] : 0.0;
+}
+
+const double* filter_r1(ilh_plan* pl, DevBuf& ones)
+{
+    // R 1 on Nodes{Primal}, cached per geometry in the mask buffer's sibling
+    const size_t ng = pl->g.n(GK_PRIMAL);
+    if (pl->filt_gen != pl->gen) {
+        pl->filt_r1.ensure(sizeof(double) * ng);
+        ones.ensure(sizeof(double) * std::max(1, pl->N));
+        k_fill_value<<<(std::max(1, pl->N) + 255) / 256, 256, 0, pl->ctx->stream>>>(ones.as<double>(), pl->N, 1.0);
+        scatter_op(pl, OP_R_PRIMAL, ones.as<double>(), pl->filt_r1.as<double>());
+        pl->filt_gen = pl->gen;
+    }
+    return pl->filt_r1.as<double>();
+}
+
+}  // namespace
+
+extern "C" int ilh_surface_filter_build(ilh_plan* pl, double* C, int mem)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        Io io(pl, mem);
+        const int N = pl->N;
+        const size_t nn = (size_t)N * N;
+        double* d_C = io.out(C, nn, pl->io_b);
+        hipStream_t st = pl->ctx->stream;
+        if (N > 0) {
+            fill_zero(st, d_C, nn);
+            const Term* te = get_terms(pl, OP_E_PRIMAL);
+            const Term* tr = get_terms(pl, OP_R_PRIMAL);
+            if (!pl->bins_ok[OP_R_PRIMAL]) {
+                build_bins(pl, tr, N, GK_PRIMAL, pl->bins[OP_R_PRIMAL][0]);
+                pl->bins_ok[OP_R_PRIMAL] = true;
+            }
+            const RowBins& b = pl->bins[OP_R_PRIMAL][0];
+            k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, d_C);
+            ILH_HIP(hipGetLastError());
+        }
+        io.back(C, d_C, nn);
+        io.done();
+    });
+}
+
+extern "C" int ilh_surface_filter_apply(ilh_plan* pl, int power, const double* s_in, double* s_out, int mem)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        if (power < 0 || power > 1000) fail(ILH_ERR_ARG, "power must be in [0, 1000]");
+        Io io(pl, mem);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        hipStream_t st = pl->ctx->stream;
+        const double* d_in = io.in(s_in, N, pl->io_a);
+        double* d_out = io.out(s_out, N, pl->io_d);
+        if (N > 0) {
+            if (d_out != d_in) ILH_HIP(hipMemcpyAsync(d_out, d_in, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            DevBuf ones;
+            const double* r1 = filter_r1(pl, ones);
+            pl->scratch_a.ensure(sizeof(double) * ng);
+            double* g = pl->scratch_a.as<double>();
+            const unsigned gb = (unsigned)std::min<size_t>(4096, (ng + 255) / 256);
+            for (int k = 0; k < power; ++k) {
+                scatter_op(pl, OP_R_PRIMAL, d_out, g);
+                k_filter_weight<<<gb, 256, 0, st>>>(g, r1, ng);
+                gather_op(pl, OP_E_PRIMAL, g, d_out);
+            }
+            ILH_HIP(hipGetLastError());
+            ILH_HIP(hipStreamSynchronize(st));
+        }
+        io.back(s_out, d_out, N);
+        io.done();
+    });
+}
+
 // ------------------------------------------------------ batched solves
 // Repeat Dirichlet solves of many independent plans of one context: the
 // plans' caches are prepared first (in order, on the context stream), then

@@ -71,59 +71,91 @@ static std::vector<double> heat_kernel_1d(double a)
 namespace {
 
 constexpr int HX = 64, HY = 4;          // naive kernels (long filters)
-constexpr int XT = 256;                  // x pass: outputs per work-group (one row segment)
+constexpr int XT = 256, XR = 8;          // x pass: 256-wide row segments, 8 rows per work-group
 constexpr int YC = 64, YR = 64;          // y pass: columns x rows of outputs per work-group
 constexpr int MX_LDS = 64, MY_LDS = 32;  // longest filters the tiled passes take
 
 // x pass: out(i,j) = sum_{|p|<=M} K(|p|) u(i-p, j), u = a + c b (b may be
-// null), zero outside [0, nx).  The row segment and its halo are staged in LDS.
+// null), zero outside [0, nx).  Eight row segments and their halos are
+// staged in LDS; every thread issues all of its loads before the first store.
 __global__ void __launch_bounds__(XT) k_heat_x(const double* __restrict__ a, double c, const double* __restrict__ b,
                                                int nx, int ny, const double* __restrict__ K, int M,
                                                double* __restrict__ out)
 {
-    __shared__ double buf[XT + 2 * MX_LDS];
-    const int j = blockIdx.y, x0 = blockIdx.x * XT, t = threadIdx.x;
-    const double* ra = a + (size_t)nx * j;
-    const double* rb = b ? b + (size_t)nx * j : nullptr;
-    for (int e = t; e < XT + 2 * M; e += XT) {
-        const int x = x0 - M + e;
-        double v = 0.0;
-        if (x >= 0 && x < nx) v = rb ? ra[x] + c * rb[x] : ra[x];
-        buf[e] = v;
-    }
+    constexpr int W = XT + 2 * MX_LDS;
+    __shared__ double buf[XR][W];
+    const int j0 = blockIdx.y * XR, x0 = blockIdx.x * XT, t = threadIdx.x;
+    const int w = XT + 2 * M;
+    double va[XR][2], vb[XR][2];
+#pragma unroll
+    for (int r = 0; r < XR; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = t + XT * q, x = x0 - M + e, j = j0 + r;
+            const bool in = e < w && x >= 0 && x < nx && j < ny;
+            va[r][q] = in ? a[x + (size_t)nx * j] : 0.0;
+            vb[r][q] = in && b ? b[x + (size_t)nx * j] : 0.0;
+        }
+#pragma unroll
+    for (int r = 0; r < XR; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = t + XT * q;
+            if (e < w) buf[r][e] = b ? va[r][q] + c * vb[r][q] : va[r][q];
+        }
     __syncthreads();
     const int i = x0 + t;
     if (i >= nx) return;
-    double acc = K[0] * buf[t + M];
-    for (int p = 1; p <= M; ++p) acc += K[p] * (buf[t + M - p] + buf[t + M + p]);
-    out[i + (size_t)nx * j] = acc;
+#pragma unroll
+    for (int r = 0; r < XR; ++r) {
+        const int j = j0 + r;
+        if (j >= ny) break;
+        const double* row = buf[r] + t + M;
+        double acc = K[0] * row[0];
+        for (int p = 1; p <= M; ++p) acc += K[p] * (row[-p] + row[p]);
+        out[i + (size_t)nx * j] = acc;
+    }
 }
 
 // y pass: out(i,j) = sum_{|p|<=M} K(|p|) in(i, j-p) + c add(i,j) (add may be
 // null), zero outside [0, ny).  A 64-column strip of YR + 2M rows is staged
-// in LDS (dynamic, (YR + 2M) x 64 doubles).
+// in LDS (dynamic, (YR + 2M) x 64 doubles); loads, including those of `add`,
+// are all issued before the strip is stored.
 __global__ void __launch_bounds__(256) k_heat_y(const double* __restrict__ in, int nx, int ny,
                                                 const double* __restrict__ K, int M, double c,
                                                 const double* __restrict__ add, double* __restrict__ out)
 {
     extern __shared__ double strip[];   // [YR + 2M][YC]
+    constexpr int QMAX = (YR + 2 * MY_LDS) / 4;
     const int tx = threadIdx.x & (YC - 1), ty = threadIdx.x / YC;   // 64 x 4
     const int i = blockIdx.x * YC + tx, y0 = blockIdx.y * YR;
     const int rows = YR + 2 * M;
-    for (int r = ty; r < rows; r += 4) {
-        const int y = y0 - M + r;
-        strip[r * YC + tx] = (i < nx && y >= 0 && y < ny) ? in[i + (size_t)nx * y] : 0.0;
+    double v[QMAX], ad[YR / 4];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        const int r = ty + 4 * q, y = y0 - M + r;
+        v[q] = (r < rows && i < nx && y >= 0 && y < ny) ? in[i + (size_t)nx * y] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < YR / 4; ++q) {
+        const int y = y0 + ty + 4 * q;
+        ad[q] = (add && i < nx && y < ny) ? add[i + (size_t)nx * y] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        const int r = ty + 4 * q;
+        if (r < rows) strip[r * YC + tx] = v[q];
     }
     __syncthreads();
     if (i >= nx) return;
-    for (int r = ty; r < YR; r += 4) {
-        const int y = y0 + r;
+#pragma unroll
+    for (int q = 0; q < YR / 4; ++q) {
+        const int r = ty + 4 * q, y = y0 + r;
         if (y >= ny) break;
         const double* col = strip + (r + M) * YC + tx;
         double acc = K[0] * col[0];
         for (int p = 1; p <= M; ++p) acc += K[p] * (col[-p * YC] + col[p * YC]);
-        const size_t o = i + (size_t)nx * y;
-        out[o] = add ? acc + c * add[o] : acc;
+        out[i + (size_t)nx * y] = add ? acc + c * ad[q] : acc;
     }
 }
 
@@ -196,7 +228,7 @@ static void heat_apply(ilh_plan* pl, const double* K, int M, const double* a, do
     const int nx = pl->g.nx(GK_PRIMAL), ny = pl->g.ny(GK_PRIMAL);
     hipStream_t st = pl->ctx->stream;
     if (M <= MX_LDS)
-        k_heat_x<<<dim3((nx + XT - 1) / XT, ny), XT, 0, st>>>(a, ca, b, nx, ny, K, M, tmp);
+        k_heat_x<<<dim3((nx + XT - 1) / XT, (ny + XR - 1) / XR), XT, 0, st>>>(a, ca, b, nx, ny, K, M, tmp);
     else
         k_heat_x_long<<<dim3((nx + HX - 1) / HX, (ny + HY - 1) / HY), dim3(HX, HY), 0, st>>>(a, ca, b, nx, ny, K, M,
                                                                                           tmp);

@@ -166,6 +166,8 @@ struct ilh_plan {
     unsigned gen = 0;                         // incremented whenever the geometry caches are dropped
     ilh::DevBuf mask;                         // mask(cache) on Nodes{Primal} (built lazily)
     unsigned mask_gen = ~0u;                  // generation the mask belongs to
+    ilh::DevBuf filt_r1;                      // R 1 on Nodes{Primal} for the surface filter
+    unsigned filt_gen = ~0u;
     std::atomic<int> refs{1};                 // the caller's handle, batches and heat integrators using it
 };
 

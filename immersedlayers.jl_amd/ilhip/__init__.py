@@ -16,7 +16,8 @@ from .core import (Context, DeviceArray, Nodes, Edges, ScalarData, VectorData, S
                    create_RTLinvR, create_CLinvCT, create_GLinvD, create_nRTRn,
                    DirichletPoisson, NeumannPoisson, SchurFactor, default_context,
                    dirichlet_solve_batched, dirichlet_solve_batched_device, DirichletBatch,
-                   DirichletHeatConduction, mask, complementary_mask, mask_, complementary_mask_)
+                   DirichletHeatConduction, mask, complementary_mask, mask_, complementary_mask_,
+                   create_surface_filter, filter_surface)
 from .geometry import GridSpec as PhysicalGrid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -107,6 +107,8 @@ SIGNATURES = {
     "ilh_batch_launch": ([_vp], _i),
     "ilh_batch_destroy": ([_vp], _i),
     "ilh_mask": ([_vp, _i, _i, _vp, _i], _i),
+    "ilh_surface_filter_build": ([_vp, _vp, _i], _i),
+    "ilh_surface_filter_apply": ([_vp, _i, _vp, _vp, _i], _i),
     "ilh_mask_multiply": ([_vp, _i, _i, _vp, _i], _i),
     "ilh_heat_create": ([_vp, C.c_double, C.c_double, _vp], _i),
     "ilh_heat_set_boundary": ([_vp, _vp, _vp, _i], _i),

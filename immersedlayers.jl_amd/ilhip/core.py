@@ -418,6 +418,23 @@ def complementary_mask_(w, cache):
     return w
 
 
+def create_surface_filter(cache):
+    """C = Et R (src/matrix_operators.jl:254-267) as an N x N array: the
+    interpolation divides the regularized field by R 1 first, so C keeps
+    constants and the surface integral."""
+    Cm = np.zeros(cache.N * cache.N)
+    L.call("ilh_surface_filter_build", cache.h, _ptr(Cm), L.MEM_HOST)
+    return Cm.reshape((cache.N, cache.N), order="F")
+
+
+def filter_surface(s, cache, power=1):
+    """C^power s without forming C (the C^5 s / C^6 s of test/literate/dirichlet.jl:122-124)."""
+    x = np.ascontiguousarray(s.flat() if isinstance(s, ScalarData) else s, dtype=np.float64)
+    out = ScalarData(cache.N)
+    L.call("ilh_surface_filter_apply", cache.h, int(power), _ptr(x), _ptr(out.flat()), L.MEM_HOST)
+    return out
+
+
 def _schur(cache, kind, scale):
     A = np.zeros(cache.N * cache.N)
     L.call("ilh_schur_build", cache.h, kind, C.c_double(scale), _ptr(A), L.MEM_HOST)

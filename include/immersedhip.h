@@ -204,6 +204,17 @@ int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const doub
 int ilh_mask(ilh_plan* plan, int grid_kind, int complementary, double* out, int mem);
 int ilh_mask_multiply(ilh_plan* plan, int grid_kind, int complementary, double* w, int mem);
 
+/* ---- surface filter (src/matrix_operators.jl:245-267, create_surface_filter) ----
+ * C = Et R, Et the interpolation whose grid values are first divided by R 1
+ * (the regularized unit strength): C keeps constants and the surface integral
+ * sum_p ds_p (C s)_p = sum_q ds_q s_q, as the reference documents its
+ * filter (test/literate/matrices.jl:49-56).  ilh_surface_filter_build writes C
+ * (N x N, column-major); ilh_surface_filter_apply writes C^power s_in into
+ * s_out (the reference applies C^5 or C^6 to the strength,
+ * test/literate/dirichlet.jl:122-124, problems.jl:298) without forming C. */
+int ilh_surface_filter_build(ilh_plan* plan, double* C, int mem);
+int ilh_surface_filter_apply(ilh_plan* plan, int power, const double* s_in, double* s_out, int mem);
+
 /* ---- unbounded heat conduction with Dirichlet bodies (config: IF time marching) ----
  * The constrained ODE system of test/literate/heatconduction.jl:88-180,
  *   dT/dt = kappa L T - kappa D_s [T] + R sigma,   E T = (T_b+ + T_b-)/2,

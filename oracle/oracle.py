@@ -369,3 +369,22 @@ def primal_to(g, m, kind):
         out = np.zeros((NX - 1, NY))
         out[:, 1:-1] = 0.5 * (p[:, :-1] + p[:, 1:])
     return out.ravel(order="F")
+
+
+def surface_filter(g, geo, ddf_kind=YANG3):
+    """C = Et R (create_surface_filter, src/matrix_operators.jl:254-267), dense
+    [row, col]: Et interpolates the regularized field divided by R 1 (zero where
+    R 1 = 0), so that C keeps constants and sum_p ds_p (C s)_p = sum_q ds_q s_q
+    (the property test/literate/matrices.jl:49-56 states).  E is formed from R
+    by adjointness (E = dx^2 diag(1/ds) R^T)."""
+    N = geo.N
+    R = np.zeros((nel(g, PRIMAL), N))
+    for q in range(N):
+        e = np.zeros(N)
+        e[q] = 1.0
+        R[:, q] = regularize(g, PRIMAL, geo, e, ddf_kind)
+    r1 = R.sum(axis=1)
+    w = np.zeros_like(r1)
+    w[r1 != 0] = 1.0 / r1[r1 != 0]
+    E = (R * (g.dx ** 2 / geo.ds)[None, :]).T
+    return E @ (w[:, None] * R)

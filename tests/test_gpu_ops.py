@@ -230,3 +230,17 @@ def test_masks(setup):
         x.flat()[:] = w
         il.complementary_mask_(x, cache)
         assert max_rel(x.flat(), w * (1.0 - ref)) < 1e-12, okind
+
+
+def test_surface_filter(setup):
+    """create_surface_filter and C^k s (src/matrix_operators.jl:254-267,
+    test/literate/dirichlet.jl:122-124) against the oracle's dense filter."""
+    il, cache, gs, og, geo, rng = setup
+    Cr = O.surface_filter(og, geo)
+    C = il.create_surface_filter(cache)
+    assert max_rel(C, Cr) < 1e-12
+    sv = rng.standard_normal(geo.N)
+    for k in (0, 1, 6):
+        got = il.filter_surface(il.ScalarData(geo.N, sv), cache, k).flat()
+        ref = np.linalg.matrix_power(Cr, k) @ sv
+        assert max_rel(got, ref) < 1e-12, k

@@ -436,3 +436,37 @@ def test_primal_to_is_the_averaging_of_a_linear_field():
     xv, yv = np.meshgrid(gs.x_primal(), gs.y_dual(), indexing="ij")
     v = O.primal_to(og, m, O.EDGE_V).reshape((gs.NX - 1, gs.NY), order="F")
     assert np.allclose(v[:, 1:-1], lin(xv, yv)[:, 1:-1], atol=1e-13)
+
+
+# ---------------------------------------------------------- surface filter
+def test_surface_filter_pins():
+    """create_surface_filter (src/matrix_operators.jl:254-267): C keeps the
+    surface integral and constants (test/literate/matrices.jl:49-56), ds C is
+    symmetric with eigenvalues of C in [0, 1], and a column equals the oracle's
+    own interpolate of the regularized unit strength divided by R 1."""
+    from ilhip import geometry as G
+    g, s, _ = G.config_plain()
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+    C = O.surface_filter(og, geo)
+    ds = geo.ds
+    assert np.abs(ds @ C - ds).max() < 1e-13 * ds.max()
+    assert np.abs(C @ np.ones(geo.N) - 1.0).max() < 1e-13
+    M = ds[:, None] * C
+    assert np.abs(M - M.T).max() < 1e-13 * np.abs(M).max()
+    ev = np.linalg.eigvals(C).real
+    assert ev.min() > -1e-12 and ev.max() < 1.0 + 1e-12
+    r1 = O.regularize(og, O.PRIMAL, geo, np.ones(geo.N))
+    for q in (0, 37, geo.N - 1):
+        e = np.zeros(geo.N)
+        e[q] = 1.0
+        f = O.regularize(og, O.PRIMAL, geo, e)
+        f = np.where(r1 != 0, f / np.where(r1 != 0, r1, 1.0), 0.0)
+        assert np.abs(O.interpolate(og, O.PRIMAL, geo, f) - C[:, q]).max() < 1e-14
+    # filtering smooths: the filtered strength of a noisy signal is closer to its smooth part
+    rng = np.random.default_rng(5)
+    th = s.theta()
+    smooth = np.cos(2 * th)
+    noisy = smooth + 0.3 * rng.standard_normal(geo.N)
+    f6 = np.linalg.matrix_power(C, 6) @ noisy
+    assert np.linalg.norm(f6 - smooth) < 0.5 * np.linalg.norm(noisy - smooth)
