@@ -473,7 +473,9 @@ class DirichletPoisson:
     def __init__(self, cache):
         self.cache = cache
 
-    def solve(self, fbplus, fbminus=None):
+    def solve(self, fbplus, fbminus=None, filter_power=0):
+        """(f, s); with filter_power = k > 0, s is returned filtered as C^k s
+        (the reference's solve returns C^6 s, test/literate/problems.jl:298)."""
         c = self.cache
         fp = _vec(fbplus, c.N)
         fm = None if fbminus is None else _vec(fbminus, c.N)
@@ -481,6 +483,8 @@ class DirichletPoisson:
         s = ScalarData(c.N)
         L.call("ilh_dirichlet_solve", c.h, _ptr(fp), None if fm is None else _ptr(fm), _ptr(f.flat()),
                _ptr(s.flat()), L.MEM_HOST)
+        if filter_power:
+            s = filter_surface(s, c, filter_power)
         return f, s
 
     def solve_moving(self, fbplus, fbminus=None, tol=1e-12, max_iter=30):
