@@ -1497,43 +1497,70 @@ static void moving_reference(ilh_plan* pl, MovingCache* mc)
 }
 
 // GMRES(m) on S x = b, right-preconditioned by Pinv, classical Gram-Schmidt
-// with one re-orthogonalisation; x0 = Pinv b.  The Arnoldi steps, Givens
-// rotations and the residual test run on the device; the host enqueues steps
-// in chunks (the previous solve's count first) and reads back two flags after
-// each chunk, steps after convergence returning at once.  Returns the
-// iteration count, or -1 if the relative residual did not fall below tol
-// within m iterations.
-static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* x, double tol, int m)
+// with a conditional re-orthogonalisation; x0 = Pinv b.  The Arnoldi steps,
+// Givens rotations and the residual test run on the device; the host enqueues
+// steps in chunks (the previous solve's count first) and reads back two
+// flags after each chunk, steps after convergence returning at once.  The
+// phases below issue work on the context's current stream without waiting,
+// so that the batched form can run many plans' solves side by side.
+
+// x0 = P b, r0 = b - S x0, v_0, g
+static void gm_begin(ilh_plan* pl, MovingCache* mc, const double* b, double* x, double tol, int m)
+{
+    const int N = pl->N, ld = solve_ld(N);
+    hipStream_t st = pl->ctx->stream;
+    const GmLayout L{m};
+    mc->gm.ensure(sizeof(double) * L.size());
+    mc->gmf.ensure(sizeof(int) * 2);
+    double* w = mc->w.as<double>();
+    matvec_rows(st, mc->Pinv.as<double>(), ld, b, x, N, N, 1.0, 0);
+    schur_apply(pl, mc, x, w);
+    axpby(st, w, b, 1.0, -1.0, N);
+    k_gm_init<<<1, 1024, 0, st>>>(w, b, N, mc->V.as<double>(), mc->gm.as<double>(), m, tol, mc->gmf.as<int>());
+    ILH_HIP(hipGetLastError());
+}
+
+// Arnoldi steps j0 .. j1-1, then the two flags into pinned host memory
+static void gm_issue(ilh_plan* pl, MovingCache* mc, int j0, int j1, int m)
 {
     const int N = pl->N, ld = solve_ld(N);
     hipStream_t st = pl->ctx->stream;
     const unsigned gR = (unsigned)((N + 3) / 4);
-    const GmLayout L{m};
-    mc->gm.ensure(sizeof(double) * L.size());
-    mc->gmf.ensure(sizeof(int) * 2);
     double* V = mc->V.as<double>();
-    double* w = mc->w.as<double>();
-    double* z = mc->z.as<double>();
-    double* gm = mc->gm.as<double>();
     int* flags = mc->gmf.as<int>();
-    int* hflags = (int*)mc->pinned(1);
-    // x0 = P b, r0 = b - S x0
-    matvec_rows(st, mc->Pinv.as<double>(), ld, b, x, N, N, 1.0, 0);
-    schur_apply(pl, mc, x, w);
-    axpby(st, w, b, 1.0, -1.0, N);
-    k_gm_init<<<1, 1024, 0, st>>>(w, b, N, V, gm, m, tol, flags);
+    for (int j = j0; j < j1; ++j) {
+        k_gm_matvec<<<gR, 256, 0, st>>>(mc->Pinv.as<double>(), ld, V + (size_t)ld * j, mc->z.as<double>(), N, flags);
+        k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, mc->z.as<double>(), mc->w.as<double>(), N, flags);
+        k_gm_arnoldi<<<1, 1024, 0, st>>>(V, ld, j, mc->w.as<double>(), N, mc->gm.as<double>(), m, flags);
+    }
     ILH_HIP(hipGetLastError());
+    ILH_HIP(hipMemcpyAsync(mc->pinned(1), flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+}
+
+// x += P (V y) after `steps` converged steps
+static void gm_end(ilh_plan* pl, MovingCache* mc, double* x, int m, int steps)
+{
+    const int N = pl->N, ld = solve_ld(N);
+    hipStream_t st = pl->ctx->stream;
+    if (steps <= 0) return;
+    mc->last_iters = steps;
+    k_gm_finish<<<1, 1024, 0, st>>>(mc->gm.as<double>(), m, steps, mc->V.as<double>(), ld, N, mc->w.as<double>());
+    matvec_rows(st, mc->Pinv.as<double>(), ld, mc->w.as<double>(), x, N, N, 1.0, 1);
+    ILH_HIP(hipGetLastError());
+}
+
+// the whole solve on one plan: the steps taken, or -1 without convergence
+static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* x, double tol, int m)
+{
+    hipStream_t st = pl->ctx->stream;
+    gm_begin(pl, mc, b, x, tol, m);
     int j = 0, state = 0;
     int chunk = std::max(1, std::min(m, mc->last_iters));
+    const int* hflags = (const int*)mc->pinned(1);
     while (j < m) {
         const int j1 = std::min(m, j + chunk);
-        for (; j < j1; ++j) {
-            k_gm_matvec<<<gR, 256, 0, st>>>(mc->Pinv.as<double>(), ld, V + (size_t)ld * j, z, N, flags);
-            k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, z, w, N, flags);
-            k_gm_arnoldi<<<1, 1024, 0, st>>>(V, ld, j, w, N, gm, m, flags);
-        }
-        ILH_HIP(hipGetLastError());
-        ILH_HIP(hipMemcpyAsync(hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        gm_issue(pl, mc, j, j1, m);
+        j = j1;
         ILH_HIP(hipStreamSynchronize(st));
         state = hflags[0];
         if (state) break;
@@ -1541,15 +1568,81 @@ static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* 
     }
     if (state != 1) return -1;
     const int steps = hflags[1];
-    if (steps > 0) {
-        mc->last_iters = steps;
-        // x += P (V y)
-        k_gm_finish<<<1, 1024, 0, st>>>(gm, m, steps, V, ld, N, w);
-        matvec_rows(st, mc->Pinv.as<double>(), ld, w, x, N, N, 1.0, 1);
-        ILH_HIP(hipGetLastError());
-    }
+    gm_end(pl, mc, x, m, steps);
     ILH_HIP(hipStreamSynchronize(st));
     return steps;
+}
+
+// Buffers and, on the first call, the reference inverse (which waits for the
+// device); nothing else of the current geometry.
+static MovingCache* moving_setup(ilh_plan* pl, int m)
+{
+    const int N = pl->N;
+    const size_t ng = pl->g.n(GK_PRIMAL);
+    need_margin(pl, "Dirichlet solve");
+    if (!pl->mov) pl->mov = new MovingCache();
+    MovingCache* mc = pl->mov;
+    mc->amp.ensure(sizeof(double) * 2 * (size_t)N);
+    mc->V.ensure(sizeof(double) * (size_t)solve_ld(N) * (m + 1));   // basis columns, stride solve_ld(N)
+    for (DevBuf* bb : {&mc->w, &mc->z, &mc->b, &mc->fbar}) bb->ensure(sizeof(double) * (size_t)N);
+    mc->grid.ensure(sizeof(double) * ng);
+    mc->S.ensure(sizeof(double) * (size_t)N * solve_ld(N));
+    (void)mc->pinned(1);
+    if (!mc->have_ref) moving_reference(pl, mc);
+    return mc;
+}
+
+// The current geometry's merged sources [R, D_s] and S, issued on the
+// context's current stream without waiting.
+static void moving_geometry(ilh_plan* pl, MovingCache* mc)
+{
+    if (!mc->src_ok) {
+        merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, mc->terms1, mc->src1);
+        mc->src_ok = true;
+    }
+    if (!mc->S_ok) {
+        // create_RTLinvR (scale 1), stored row-major for the matvecs
+        assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -pl->g.dx * pl->g.dx, false, mc->S.as<double>(),
+                 solve_ld(pl->N), true);
+        mc->S_ok = true;
+    }
+}
+
+// amp = [0; d], f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
+static void moving_rhs(ilh_plan* pl, MovingCache* mc, const double* d_fp, const double* d_fm)
+{
+    const int N = pl->N;
+    hipStream_t st = pl->ctx->stream;
+    double* amp = mc->amp.as<double>();
+    k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, amp, mc->fbar.as<double>());
+    conv_sources(pl, GK_PRIMAL, mc->src1, amp, mc->grid.as<double>());
+    gather_op(pl, OP_E_PRIMAL, mc->grid.as<double>(), mc->b.as<double>());
+    axpby(st, mc->b.as<double>(), mc->fbar.as<double>(), -1.0, 1.0, N);
+}
+
+// f = L^-1 (R s + D_s d) with s in amp[0, N)
+static void moving_final(ilh_plan* pl, MovingCache* mc, double* d_f, double* d_s)
+{
+    const int N = pl->N;
+    double* amp = mc->amp.as<double>();
+    conv_sources(pl, GK_PRIMAL, mc->src1, amp, d_f);
+    ILH_HIP(hipMemcpyAsync(d_s, amp, sizeof(double) * N, hipMemcpyDeviceToDevice, pl->ctx->stream));
+    ++mc->ref_age;
+}
+
+// the solve after a failed GMRES: refactor at the current geometry, solve again
+static int moving_refactor_solve(ilh_plan* pl, MovingCache* mc, double tol, int m)
+{
+    moving_reference(pl, mc);
+    const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), mc->amp.as<double>(), tol, m);
+    if (it2 < 0) fail(ILH_ERR_SINGULAR, "GMRES did not converge with the current geometry's factor");
+    return m + it2;
+}
+
+static void check_moving_args(double tol, int max_iter)
+{
+    if (!(tol > 0.0) || tol >= 1.0) fail(ILH_ERR_ARG, "tol must be in (0, 1)");
+    if (max_iter < 1 || max_iter > 200) fail(ILH_ERR_ARG, "max_iter must be in [1, 200]");
 }
 
 extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s,
@@ -1557,8 +1650,7 @@ extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const 
 {
     return guarded([&] {
         Io io(pl, mem);
-        if (!(tol > 0.0) || tol >= 1.0) fail(ILH_ERR_ARG, "tol must be in (0, 1)");
-        if (max_iter < 1 || max_iter > 200) fail(ILH_ERR_ARG, "max_iter must be in [1, 200]");
+        check_moving_args(tol, max_iter);
         const int N = pl->N;
         const size_t ng = pl->g.n(GK_PRIMAL);
         hipStream_t st = pl->ctx->stream;
@@ -1570,51 +1662,104 @@ extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const 
         if (N == 0) {
             fill_zero(st, d_f, ng);
         } else {
-            need_margin(pl, "Dirichlet solve");
-            if (!pl->mov) pl->mov = new MovingCache();
-            MovingCache* mc = pl->mov;
-            if (!mc->src_ok) {
-                merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, mc->terms1, mc->src1);
-                mc->src_ok = true;
-            }
             const int m = max_iter;
-            mc->amp.ensure(sizeof(double) * 2 * (size_t)N);
-            mc->V.ensure(sizeof(double) * (size_t)solve_ld(N) * (m + 1));   // basis columns, stride solve_ld(N)
-            for (DevBuf* bb : {&mc->w, &mc->z, &mc->b, &mc->fbar}) bb->ensure(sizeof(double) * (size_t)N);
-            mc->grid.ensure(sizeof(double) * ng);
-            if (!mc->have_ref) moving_reference(pl, mc);
-            if (!mc->S_ok) {
-                const int ld = solve_ld(N);
-                mc->S.ensure(sizeof(double) * (size_t)N * ld);
-                // create_RTLinvR (scale 1), stored row-major for the matvecs
-                assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -pl->g.dx * pl->g.dx, false, mc->S.as<double>(), ld,
-                         true);
-                mc->S_ok = true;
-            }
-            // amp = [0; d], f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
-            double* amp = mc->amp.as<double>();
-            k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, amp, mc->fbar.as<double>());
-            conv_sources(pl, GK_PRIMAL, mc->src1, amp, mc->grid.as<double>());
-            gather_op(pl, OP_E_PRIMAL, mc->grid.as<double>(), mc->b.as<double>());
-            axpby(st, mc->b.as<double>(), mc->fbar.as<double>(), -1.0, 1.0, N);
+            MovingCache* mc = moving_setup(pl, m);
+            moving_geometry(pl, mc);
+            moving_rhs(pl, mc, d_fp, d_fm);
             // s lands in amp[0, N): the sources of the final convolution
-            it = moving_gmres(pl, mc, mc->b.as<double>(), amp, tol, m);
-            if (it < 0) {
-                // the reference geometry no longer preconditions well: refactor here
-                moving_reference(pl, mc);
-                const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), amp, tol, m);
-                if (it2 < 0) fail(ILH_ERR_SINGULAR, "GMRES did not converge with the current geometry's factor");
-                it = m + it2;
-            }
-            ++mc->ref_age;
-            // f = L^-1 (R s + D_s d)
-            conv_sources(pl, GK_PRIMAL, mc->src1, amp, d_f);
-            ILH_HIP(hipMemcpyAsync(d_s, amp, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            it = moving_gmres(pl, mc, mc->b.as<double>(), mc->amp.as<double>(), tol, m);
+            if (it < 0) it = moving_refactor_solve(pl, mc, tol, m);
+            moving_final(pl, mc, d_f, d_s);
         }
         if (iters) *iters = it;
         io.back(f, d_f, ng);
         io.back(s, d_s, N);
         io.done();
+    });
+}
+
+// Batched moving-body solves: many plans of one context (independent bodies
+// or trajectories, each after its own ilh_plan_update_points) solved side by
+// side.  Reference factorisations (first calls) run plan by plan on the
+// context stream; then each plan's geometry (sources, S), right-hand side,
+// GMRES steps and final convolution run on the side streams, the GMRES steps of all plans advancing in lockstep chunks
+// with one wait per chunk for the whole batch.  Device pointers only.
+extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
+                                                  const double* const* fb_minus, double* const* f, double* const* s,
+                                                  double tol, int max_iter, int* iters)
+{
+    return guarded([&] {
+        check_moving_args(tol, max_iter);
+        if (!fb_plus || !f || !s) fail(ILH_ERR_ARG, "null array of pointers");
+        ilh_ctx* ctx = batch_ctx(nprob, plans);
+        const int m = max_iter;
+        std::vector<MovingCache*> mc(nprob, nullptr);
+        for (int i = 0; i < nprob; ++i)
+            if (plans[i]->N > 0) mc[i] = moving_setup(plans[i], m);
+        const hipStream_t main = ctx->stream;
+        struct Restore {
+            ilh_ctx* c;
+            hipStream_t m;
+            ~Restore() { c->stream = m; }
+        } restore{ctx, main};
+        ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
+        for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamWaitEvent(ctx->pool[k], ctx->ev[ilh_ctx::POOL], 0));
+        auto on = [&](int i) { ctx->stream = ctx->pool[i % ilh_ctx::POOL]; };
+        std::vector<int> j(nprob, 0), chunk(nprob, 1), state(nprob, 0), it(nprob, 0);
+        for (int i = 0; i < nprob; ++i) {
+            on(i);
+            ilh_plan* pl = plans[i];
+            if (!mc[i]) {
+                fill_zero(ctx->stream, f[i], pl->g.n(GK_PRIMAL));
+                state[i] = 1;
+                continue;
+            }
+            moving_geometry(pl, mc[i]);
+            moving_rhs(pl, mc[i], fb_plus[i], fb_minus ? fb_minus[i] : nullptr);
+            gm_begin(pl, mc[i], mc[i]->b.as<double>(), mc[i]->amp.as<double>(), tol, m);
+            chunk[i] = std::max(1, std::min(m, mc[i]->last_iters));
+        }
+        for (;;) {
+            bool any = false;
+            for (int i = 0; i < nprob; ++i) {
+                if (state[i] || !mc[i]) continue;
+                on(i);
+                const int j1 = std::min(m, j[i] + chunk[i]);
+                gm_issue(plans[i], mc[i], j[i], j1, m);
+                j[i] = j1;
+                any = true;
+            }
+            if (!any) break;
+            for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamSynchronize(ctx->pool[k]));
+            for (int i = 0; i < nprob; ++i) {
+                if (state[i] || !mc[i]) continue;
+                const int* hf = (const int*)mc[i]->pinned(1);
+                state[i] = hf[0];
+                if (state[i] == 1) it[i] = hf[1];
+                else if (!state[i] && j[i] >= m) state[i] = 2;
+                chunk[i] = 2;
+            }
+        }
+        for (int i = 0; i < nprob; ++i) {
+            if (!mc[i] || state[i] != 1) continue;
+            on(i);
+            gm_end(plans[i], mc[i], mc[i]->amp.as<double>(), m, it[i]);
+            moving_final(plans[i], mc[i], f[i], s[i]);
+        }
+        // join the side streams; plans whose reference no longer preconditions
+        // are refactored and solved one by one on the context stream
+        ctx->stream = main;
+        for (int k = 0; k < ilh_ctx::POOL; ++k) {
+            ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
+            ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
+        }
+        for (int i = 0; i < nprob; ++i) {
+            if (!mc[i] || state[i] == 1) continue;
+            it[i] = moving_refactor_solve(plans[i], mc[i], tol, m);
+            moving_final(plans[i], mc[i], f[i], s[i]);
+        }
+        if (iters)
+            for (int i = 0; i < nprob; ++i) iters[i] = it[i];
     });
 }
 

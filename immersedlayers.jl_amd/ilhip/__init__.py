@@ -17,7 +17,8 @@ from .core import (Context, DeviceArray, Nodes, Edges, ScalarData, VectorData, S
                    DirichletPoisson, NeumannPoisson, SchurFactor, default_context,
                    dirichlet_solve_batched, dirichlet_solve_batched_device, DirichletBatch,
                    DirichletHeatConduction, mask, complementary_mask, mask_, complementary_mask_,
-                   create_surface_filter, filter_surface)
+                   create_surface_filter, filter_surface,
+                   dirichlet_solve_moving_batched)
 from .geometry import GridSpec as PhysicalGrid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -548,6 +548,18 @@ def dirichlet_solve_batched_device(caches, fp_devs, f_devs, s_devs):
     L.call("ilh_dirichlet_solve_batched", n, plans, ptrs(fp_devs), None, ptrs(f_devs), ptrs(s_devs), L.MEM_DEVICE)
 
 
+def dirichlet_solve_moving_batched(caches, fp_devs, f_devs, s_devs, fm_devs=None, tol=1e-12, max_iter=30):
+    """Moving-body solves of many caches side by side (each after its own
+    update_points), device buffers only; returns the GMRES iterations per cache."""
+    n = len(caches)
+    ptrs = lambda xs: (C.c_void_p * n)(*[None if x is None else x.ptr.value for x in xs])
+    plans = (C.c_void_p * n)(*[c.h.value for c in caches])
+    its = (C.c_int * n)()
+    L.call("ilh_dirichlet_solve_moving_batched", n, plans, ptrs(fp_devs), None if fm_devs is None else ptrs(fm_devs),
+           ptrs(f_devs), ptrs(s_devs), C.c_double(tol), int(max_iter), its)
+    return list(its)
+
+
 class DirichletBatch:
     """A batch of independent Dirichlet problems on fixed device buffers,
     captured as one HIP graph and replayed with `launch()`."""

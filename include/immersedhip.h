@@ -194,6 +194,17 @@ int ilh_batch_destroy(ilh_batch* batch);
 int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
                                double* s, int mem, double tol, int max_iter, int* iters);
 
+/* Batched form of ilh_dirichlet_solve_moving over nprob plans of one context
+   (independent trajectories, each moved by its own ilh_plan_update_points),
+   device pointers only, array k of each pointer array belonging to plans[k];
+   fb_minus may be NULL or hold NULL entries; iters (may be NULL) receives
+   each plan's GMRES iterations.  The plans' solves run side by side on the
+   context's side streams and their GMRES steps advance in lockstep; each
+   plan's results equal its single-plan solve bit for bit. */
+int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
+                                       const double* const* fb_minus, double* const* f, double* const* s, double tol,
+                                       int max_iter, int* iters);
+
 /* ---- masks (src/surface_operators.jl:728-871) ----
  * mask(cache) = -L^-1 D_s 1 on Nodes{Primal} (1 inside the surfaces, on the side
  * opposite the normals; 1 everywhere without points); on Nodes{Dual} and
