@@ -292,6 +292,26 @@ def main():
     # two HIP events on the plan's stream (outside the timed region)
     stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
 
+    # repeat solves with many boundary data at once (8 right-hand sides per
+    # call, seeded scalings of the same data): solves/s of the batched form
+    nrhs = 8
+    fps = np.stack([fb * (1.0 + 0.05 * k) for k in range(nrhs)], axis=1)
+    fp_m = ilhip.DeviceArray.from_host(ctx, fps)
+    fd_m = ilhip.DeviceArray(ctx, ng * nrhs)
+    sd_m = ilhip.DeviceArray(ctx, s.N * nrhs)
+    for _ in range(5):
+        prob.solve_multi_device(nrhs, fp_m, None, fd_m, sd_m)
+    ctx.synchronize()
+    reps = max(20, args.steps // (4 * nrhs))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prob.solve_multi_device(nrhs, fp_m, None, fd_m, sd_m)
+    ctx.synchronize()
+    multi_sps = reps * nrhs / (time.perf_counter() - t0)
+    f_multi0 = fd_m.download()[:ng]
+    assert np.array_equal(f_multi0, f_gpu.flat())   # column 0 is the headline solve, bit for bit
+    del fp_m, fd_m, sd_m
+
     # the same solve with host arrays in and out (PCIe-inclusive, not the metric)
     t0 = time.perf_counter()
     for _ in range(20):
@@ -395,6 +415,8 @@ def main():
         "prep_s": t_prep,
         "host_io_solves_per_s": host_sps,
         "new_geometry_solve_ms": float(np.median(new_geo_ms)),
+        "multi_rhs_solves_per_s": multi_sps,
+        "multi_rhs_per_call": nrhs,
         "moving_step_ms": float(np.mean(mv_ms)),
         "moving_gmres_iterations": mv_it,
         "moving_batched_ms_per_solve": float(np.mean(bat_ms)),

@@ -59,6 +59,7 @@ struct SolveCache {
     DevBuf terms1, terms2;    // merged source terms
     SourceSet src1, src2;
     DevBuf amp1, amp2;
+    DevBuf fin, fmin, ampm;   // batches of right-hand sides: inputs at an aligned stride, amplitudes
     ~SolveCache()
     {
         free_sources(src1);
@@ -787,6 +788,52 @@ extern "C" int ilh_dirichlet_solve(ilh_plan* pl, const double* fp, const double*
         }
         io.back(f, d_f, ng);
         io.back(s, d_s, N);
+        io.done();
+    });
+}
+
+
+// nrhs Dirichlet problems on one geometry (the boundary data of each a column
+// of fb_plus / fb_minus, N x nrhs column-major; f (n_g x nrhs), s (N x nrhs)):
+// the solve matrices are read once per group of 8 right-hand sides and each
+// FFT pass runs once over all of them.  Each column equals ilh_dirichlet_solve
+// on that column bit for bit.
+extern "C" int ilh_dirichlet_solve_multi(ilh_plan* pl, int nrhs, const double* fp, const double* fm, double* f,
+                                         double* s, int mem)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        if (nrhs < 1 || nrhs > 256) fail(ILH_ERR_ARG, "nrhs must be in [1, 256]");
+        Io io(pl, mem);
+        SolveCache* sc = prepare_dirichlet(pl);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        hipStream_t st = pl->ctx->stream;
+        double* d_f = io.out(f, ng * nrhs, pl->io_b);
+        double* d_s = io.out(s, (size_t)N * nrhs, pl->io_d);
+        if (N == 0) {
+            fill_zero(st, d_f, ng * nrhs);
+        } else {
+            // inputs at the aligned stride ldf (the surface step's 16-byte loads)
+            const int ldf = solve_ld(N);
+            const hipMemcpyKind kind = mem == ILH_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+            sc->fin.ensure(sizeof(double) * (size_t)ldf * nrhs);
+            ILH_HIP(hipMemcpy2DAsync(sc->fin.p, sizeof(double) * ldf, fp, sizeof(double) * N, sizeof(double) * N,
+                                     nrhs, kind, st));
+            const double* d_fm = nullptr;
+            if (fm) {
+                sc->fmin.ensure(sizeof(double) * (size_t)ldf * nrhs);
+                ILH_HIP(hipMemcpy2DAsync(sc->fmin.p, sizeof(double) * ldf, fm, sizeof(double) * N,
+                                         sizeof(double) * N, nrhs, kind, st));
+                d_fm = sc->fmin.as<double>();
+            }
+            sc->ampm.ensure(sizeof(double) * 2 * (size_t)N * nrhs);
+            dirichlet_amp_multi(st, sc->M[0].as<double>(), sc->M[1].as<double>(), solve_ld(N), sc->fin.as<double>(),
+                                d_fm, ldf, nrhs, sc->ampm.as<double>(), d_s, N);
+            conv_sources_multi(pl, GK_PRIMAL, sc->src1, sc->ampm.as<double>(), 2 * (size_t)N, nrhs, d_f);
+        }
+        io.back(f, d_f, ng * nrhs);
+        io.back(s, d_s, (size_t)N * nrhs);
         io.done();
     });
 }

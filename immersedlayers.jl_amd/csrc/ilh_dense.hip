@@ -978,6 +978,103 @@ void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, int ld, c
     ILH_HIP(hipGetLastError());
 }
 
+// The surface step for nb right-hand sides (columns fp + k ldf, ldf even and
+// fp 16-byte aligned; fm likewise or null): the matrix rows are read once
+// for up to 8 right-hand sides, and every column's sum has wave_dot's order
+// (each result equals the single-right-hand-side kernel's bit for bit).
+// amp_k = amp + 2 n k, s_k = s_out + n k.
+constexpr int AMP_GROUP = 8;
+
+// the dots of one matrix row with NG columns x + ldf g (NG a compile-time
+// count, so that the partial sums stay in registers)
+template <int NG>
+__device__ __forceinline__ void wave_dot_group(const double* __restrict__ row, const double* __restrict__ x, int ldf,
+                                               int m, int lane, double* out)
+{
+    const double2* r2 = (const double2*)row;
+    const int m2 = m >> 1;
+    double s[NG][4];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) s[g][0] = s[g][1] = s[g][2] = s[g][3] = 0.0;
+    int k = lane;
+    for (; k + 192 < m2; k += 256) {
+        const double2 a0 = r2[k], a1 = r2[k + 64], a2 = r2[k + 128], a3 = r2[k + 192];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const double2* x2 = (const double2*)(x + (size_t)ldf * g);
+            const double2 b0 = x2[k], b1 = x2[k + 64], b2 = x2[k + 128], b3 = x2[k + 192];
+            s[g][0] += a0.x * b0.x + a0.y * b0.y;
+            s[g][1] += a1.x * b1.x + a1.y * b1.y;
+            s[g][2] += a2.x * b2.x + a2.y * b2.y;
+            s[g][3] += a3.x * b3.x + a3.y * b3.y;
+        }
+    }
+    for (; k < m2; k += 64) {
+        const double2 a0 = r2[k];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const double2 b0 = ((const double2*)(x + (size_t)ldf * g))[k];
+            s[g][0] += a0.x * b0.x + a0.y * b0.y;
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        if ((m & 1) && lane == 0) s[g][1] += row[m - 1] * x[(size_t)ldf * g + m - 1];
+        double t = (s[g][0] + s[g][1]) + (s[g][2] + s[g][3]);
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        out[g] = t;
+    }
+}
+
+template <int NG>
+__device__ __forceinline__ void dir_amp_group(const double* __restrict__ Mp, const double* __restrict__ Mm, int ld,
+                                              const double* __restrict__ fp, const double* __restrict__ fm, int ldf,
+                                              int g0, double* __restrict__ amp, double* __restrict__ s_out, int n,
+                                              int i, int lane)
+{
+    double sp[NG], sm[NG];
+    wave_dot_group<NG>(Mp + (size_t)ld * i, fp + (size_t)ldf * g0, ldf, n, lane, sp);
+    if (fm) wave_dot_group<NG>(Mm + (size_t)ld * i, fm + (size_t)ldf * g0, ldf, n, lane, sm);
+    if (lane == 0)
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int k = g0 + g;
+            const double sv = fm ? sp[g] + sm[g] : sp[g];
+            amp[(size_t)2 * n * k + i] = sv;
+            amp[(size_t)2 * n * k + n + i] = fp[(size_t)ldf * k + i] - (fm ? fm[(size_t)ldf * k + i] : 0.0);
+            if (s_out) s_out[(size_t)n * k + i] = sv;
+        }
+}
+
+__global__ void __launch_bounds__(256) k_dir_amp_multi(const double* __restrict__ Mp, const double* __restrict__ Mm, int ld,
+                                const double* __restrict__ fp, const double* __restrict__ fm, int ldf, int nb,
+                                double* __restrict__ amp, double* __restrict__ s_out, int n)
+{
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    int g0 = 0;
+    for (; g0 + AMP_GROUP <= nb; g0 += AMP_GROUP)
+        dir_amp_group<AMP_GROUP>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane);
+    switch (nb - g0) {
+    case 1: dir_amp_group<1>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    case 2: dir_amp_group<2>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    case 3: dir_amp_group<3>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    case 4: dir_amp_group<4>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    case 5: dir_amp_group<5>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    case 6: dir_amp_group<6>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    case 7: dir_amp_group<7>(Mp, Mm, ld, fp, fm, ldf, g0, amp, s_out, n, i, lane); break;
+    default: break;
+    }
+}
+
+void dirichlet_amp_multi(hipStream_t st, const double* Mp, const double* Mm, int ld, const double* fp,
+                         const double* fm, int ldf, int nb, double* amp, double* s_out, int n)
+{
+    if (n <= 0 || nb <= 0) return;
+    k_dir_amp_multi<<<(n + 3) / 4, 256, 0, st>>>(Mp, Mm, ld, fp, fm, ldf, nb, amp, s_out, n);
+    ILH_HIP(hipGetLastError());
+}
+
 // Neumann surface step: df = A+ v+ + A- v-, ds = B+ v+ + B- v-;
 // primal amplitudes [df; v+ - v-], dual amplitudes [df; -ds]
 __global__ void k_neu_amp(const double* __restrict__ Ap, const double* __restrict__ Am, const double* __restrict__ Bp,

@@ -377,9 +377,13 @@ __device__ __forceinline__ size_t pidx(int row, int c, int H)
 // tiled Tout (H = P/2 columns, column 0 = (DC, Nyquist)).
 template <int P>
 __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2* __restrict__ tw, int npairs,
-                                                        int nrows_limit, double2* __restrict__ Tout)
+                                                        int nrows_limit, double2* __restrict__ Tout,
+                                                        size_t amp_stride, size_t out_stride)
 {
     extern __shared__ double2 lds[];
+    // right-hand side blockIdx.y of a batch: its amplitudes and output spectra
+    S.amp += amp_stride * blockIdx.y;
+    Tout += out_stride * blockIdx.y;
     constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
     constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
@@ -491,9 +495,12 @@ template <int P, int MODE>
 __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(4))) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
                                                     const double2* __restrict__ tw, int H, int ny, int nrows_in,
                                                     const double* __restrict__ spec, double scale,
-                                                    double2* __restrict__ T2, double* __restrict__ spec_out)
+                                                    double2* __restrict__ T2, double* __restrict__ spec_out,
+                                                    size_t t1_stride, size_t t2_stride)
 {
     extern __shared__ double2 lds[];
+    T1 += t1_stride * blockIdx.y;   // right-hand side blockIdx.y of a batch
+    T2 += t2_stride * blockIdx.y;
     constexpr int T = P / V, SH = P / 2 + 1, NB = tpb_of(P) / T;
     constexpr int R0 = R_first<P>(false), RL = R_last<P>(false), RO = R_last<P>(true);
     static_assert(R_first<P>(true) == RL, "the inverse starts from the forward's registers");
@@ -577,9 +584,12 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
 // Transform c of block b makes output rows 2p, 2p+1 (p = b NB + c).
 template <int P>
 __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restrict__ T2, const double2* __restrict__ tw,
-                                                        int nx, int ny, double* __restrict__ out)
+                                                        int nx, int ny, double* __restrict__ out, size_t t2_stride,
+                                                        size_t out_stride)
 {
     extern __shared__ double2 lds[];
+    T2 += t2_stride * blockIdx.y;   // right-hand side blockIdx.y of a batch
+    out += out_stride * blockIdx.y;
     constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
     constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
     const int c = threadIdx.x / T, t = threadIdx.x - c * T;
@@ -643,58 +653,69 @@ static void set_lds(const void* fn, size_t bytes)
     default: fail(ILH_ERR_ARG, "FFT length must be a power of two in [16, 8192]");                       \
     }
 
+// Batched launches (nb right-hand sides, grid.y): operand k starts `stride`
+// elements after operand k-1.
+struct Batch {
+    int nb = 1;
+    size_t in = 0, out = 0;
+};
+
 template <int P>
-static void launch_rows_fwd_p(ilh_ctx* ctx, RowSrc S, int npairs, int nrows_limit, double2* Tout)
+static void launch_rows_fwd_p(ilh_ctx* ctx, RowSrc S, int npairs, int nrows_limit, double2* Tout, Batch bt)
 {
     constexpr int NB = tpb_of(P) / (P / V);
     S.amp_lds = S.mode == ROW_TERMS && S.amp_len <= amp_lds_cap(P);
     const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P) + (S.amp_lds ? sizeof(double) * S.amp_len : 0);
     set_lds((const void*)k_rows_fwd<P>, lds);
-    k_rows_fwd<P><<<(npairs + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(S, get_twiddles(ctx, P), npairs,
-                                                                          nrows_limit, Tout);
+    k_rows_fwd<P><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+        S, get_twiddles(ctx, P), npairs, nrows_limit, Tout, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
-static void launch_rows_fwd(ilh_ctx* ctx, const RowSrc& S, int P, int npairs, int nrows_limit, double2* Tout)
+static void launch_rows_fwd(ilh_ctx* ctx, const RowSrc& S, int P, int npairs, int nrows_limit, double2* Tout,
+                            Batch bt = Batch())
 {
     if (npairs <= 0) return;
-    ILH_FOR_P(P, launch_rows_fwd_p<PP>(ctx, S, npairs, nrows_limit, Tout));
+    ILH_FOR_P(P, launch_rows_fwd_p<PP>(ctx, S, npairs, nrows_limit, Tout, bt));
 }
 
 template <int P, int MODE>
 static void launch_cols_p(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int H, int ny, int nrows_in,
-                          const double* spec, double scale, double2* T2, double* spec_out)
+                          const double* spec, double scale, double2* T2, double* spec_out, Batch bt)
 {
     constexpr int NB = tpb_of(P) / (P / V);
     const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
     set_lds((const void*)k_cols<P, MODE>, lds);
-    k_cols<P, MODE><<<(H + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(T1, slot_of_row, get_twiddles(ctx, P), H, ny,
-                                                                        nrows_in, spec, scale, T2, spec_out);
+    k_cols<P, MODE><<<dim3((H + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+        T1, slot_of_row, get_twiddles(ctx, P), H, ny, nrows_in, spec, scale, T2, spec_out, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
 template <int MODE>
 static void launch_cols(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int Px, int Py, int ny,
-                        int nrows_in, const double* spec, double scale, double2* T2, double* spec_out)
+                        int nrows_in, const double* spec, double scale, double2* T2, double* spec_out,
+                        Batch bt = Batch())
 {
     const int H = Px / 2;
-    ILH_FOR_P(Py, (launch_cols_p<PP, MODE>(ctx, T1, slot_of_row, H, ny, nrows_in, spec, scale, T2, spec_out)));
+    ILH_FOR_P(Py, (launch_cols_p<PP, MODE>(ctx, T1, slot_of_row, H, ny, nrows_in, spec, scale, T2, spec_out, bt)));
 }
 
 template <int P>
-static void launch_rows_inv_p(ilh_ctx* ctx, const double2* T2, int nx, int ny, double* out)
+static void launch_rows_inv_p(ilh_ctx* ctx, const double2* T2, int nx, int ny, double* out, Batch bt)
 {
     constexpr int NB = tpb_of(P) / (P / V);
     const int npairs = (ny + 1) / 2;
     const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
     set_lds((const void*)k_rows_inv<P>, lds);
-    k_rows_inv<P><<<(npairs + NB - 1) / NB, tpb_of(P), lds, ctx->stream>>>(T2, get_twiddles(ctx, P), nx, ny, out);
+    k_rows_inv<P><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+        T2, get_twiddles(ctx, P), nx, ny, out, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
-static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, double* out)
+static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, double* out,
+                            Batch bt = Batch())
 {
-    ILH_FOR_P(Px, launch_rows_inv_p<PP>(ctx, T2, nx, ny, out));
+    ILH_FOR_P(Px, launch_rows_inv_p<PP>(ctx, T2, nx, ny, out, bt));
 }
 
 // -------------------------------------------------------------- LGF table
@@ -976,6 +997,23 @@ void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, d
     } else {
         launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
     }
+}
+
+// nb right-hand sides at once: amplitudes amp + k amp_stride, outputs
+// out + k n(gk); the spectra in T1m / T2m (nb copies of T1 / T2, allocated here)
+void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, size_t amp_stride, int nb,
+                        double* out)
+{
+    ConvPlan& C = get_conv(pl, gk);
+    const size_t s1 = C.T1.bytes / sizeof(double2), s2 = C.T2.bytes / sizeof(double2);
+    C.T1m.ensure(C.T1.bytes * (size_t)nb);
+    C.T2m.ensure(C.T2.bytes * (size_t)nb);
+    launch_rows_fwd(pl->ctx, terms_source(C, src, amp), C.Px, src.npairs, C.ny, C.T1m.as<double2>(),
+                    Batch{nb, amp_stride, s1});
+    launch_cols<0>(pl->ctx, C.T1m.as<double2>(), src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0,
+                   C.T2m.as<double2>(), nullptr, Batch{nb, s1, s2});
+    launch_rows_inv(pl->ctx, C.T2m.as<double2>(), C.Px, C.nx, C.ny, out,
+                    Batch{nb, s2, (size_t)C.nx * C.ny});
 }
 
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, hipEvent_t* ev)

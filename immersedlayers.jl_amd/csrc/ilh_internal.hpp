@@ -121,6 +121,7 @@ struct ConvPlan {
     int H = 0;                // Px/2: half-spectrum columns (column 0 packs DC and Nyquist)
     DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], kx <= H, ky <= Py/2 (even in ky)
     DevBuf T1, T2;            // intermediate spectra, rows x H complex, 8-row tiles
+    DevBuf T1m, T2m;          // the same for batches of right-hand sides (copies back to back)
 };
 
 }  // namespace ilh
@@ -218,6 +219,8 @@ void build_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);
 void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage);
+void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, size_t amp_stride, int nb,
+                        double* out);
 
 // operators on the plan's geometry (ilh_api.hip): scatter R/D_s-type ops to the
 // grid, gather E-type ops to the surface; inverse of a dense N x N matrix
@@ -277,6 +280,8 @@ void matvec_rows(hipStream_t st, const double* Mrow, int ld, const double* x, do
                  double alpha, int accumulate);
 void dirichlet_amp(hipStream_t st, const double* Mp, const double* Mm, int ld, const double* fp,
                    const double* fm, double* amp, double* s_out, int n);
+void dirichlet_amp_multi(hipStream_t st, const double* Mp, const double* Mm, int ld, const double* fp,
+                         const double* fm, int ldf, int nb, double* amp, double* s_out, int n);
 void neumann_amp(hipStream_t st, const double* Mdf_p, const double* Mdf_m, const double* Mds_p,
                  const double* Mds_m, int ld, const double* vp, const double* vm, double* amp_f, double* amp_s,
                  double* df, double* dsv, int n);

@@ -487,6 +487,25 @@ class DirichletPoisson:
             s = filter_surface(s, c, filter_power)
         return f, s
 
+    def solve_multi(self, fbplus, fbminus=None):
+        """Many boundary data on this geometry at once: fbplus (and fbminus)
+        N x K; returns f (K Nodes{Primal}) and s (N x K array)."""
+        c = self.cache
+        fp = np.asfortranarray(np.asarray(fbplus, dtype=np.float64).reshape(c.N, -1))
+        k = fp.shape[1]
+        fm = None if fbminus is None else np.asfortranarray(np.asarray(fbminus, dtype=np.float64).reshape(c.N, k))
+        nx, ny = c.g.primal_shape()
+        f = np.zeros((nx * ny, k), order="F")
+        s = np.zeros((c.N, k), order="F")
+        L.call("ilh_dirichlet_solve_multi", c.h, k, _ptr(fp), None if fm is None else _ptr(fm), _ptr(f), _ptr(s),
+               L.MEM_HOST)
+        return [Nodes(Primal, c.g, f[:, i].reshape((nx, ny), order="F")) for i in range(k)], s
+
+    def solve_multi_device(self, nrhs, fp_dev, fm_dev, f_dev, s_dev):
+        """Device-buffer form of solve_multi (asynchronous)."""
+        L.call("ilh_dirichlet_solve_multi", self.cache.h, int(nrhs), fp_dev.ptr,
+               None if fm_dev is None else fm_dev.ptr, f_dev.ptr, s_dev.ptr, L.MEM_DEVICE)
+
     def solve_moving(self, fbplus, fbminus=None, tol=1e-12, max_iter=30):
         """The same problem after cache.update_points (moving bodies): GMRES on
         the current Schur system preconditioned by a reference geometry's

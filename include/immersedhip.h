@@ -159,6 +159,14 @@ int ilh_dirichlet_solve(ilh_plan* plan, const double* fb_plus, const double* fb_
 int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_minus,
                       double* f, double* df, double* s, double* ds, int mem);
 
+/* nrhs Dirichlet problems on one plan's geometry at once (repeat solves with
+   many boundary data): fb_plus, fb_minus (may be NULL) N x nrhs column-major,
+   f (NX-1)(NY-1) x nrhs, s N x nrhs.  The solve matrices are read once per
+   group of 8 right-hand sides and each FFT pass covers all of them; column k
+   equals ilh_dirichlet_solve on column k bit for bit.  nrhs <= 256. */
+int ilh_dirichlet_solve_multi(ilh_plan* plan, int nrhs, const double* fb_plus, const double* fb_minus, double* f,
+                              double* s, int mem);
+
 /* ---- batched forms over arrays of plans ---- */
 /* Batched form of ilh_dirichlet_solve: nprob independent plans of one context
    (any mix of grid sizes and point counts), array k of each pointer array

@@ -310,3 +310,25 @@ def test_moving_batched_matches_single(ctx):
             sk = _moved(s0, *motions[k]) if step else s0
             fr, sr = O.dirichlet_solve(og, O.Geometry(sk.X, sk.Y, sk.nx, sk.ny, sk.ds), fbs[k])
             assert rel_err(f1.flat(), fr) < 1e-10 and rel_err(s1.flat(), sr) < 1e-10
+
+
+def test_multi_rhs_matches_single(ctx):
+    """ilh_dirichlet_solve_multi: eleven boundary data on one geometry (two
+    groups of the surface step, with and without f-) equal eleven single
+    solves bit for bit."""
+    import ilhip
+    from ilhip import geometry as G
+    g, s, fb = G.config_plain()
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    prob = ilhip.DirichletPoisson(cache)
+    rng = np.random.default_rng(41)
+    K = 11
+    fps = np.stack([fb * (1 + 0.1 * k) + 0.1 * rng.standard_normal(s.N) for k in range(K)], axis=1)
+    fms = 0.2 * rng.standard_normal((s.N, K))
+    for fm in (None, fms):
+        print("multi rhs", K, "with f-" if fm is not None else "", flush=True)
+        fs, ss = prob.solve_multi(fps, fm)
+        for k in range(K):
+            f1, s1 = prob.solve(fps[:, k], None if fm is None else fm[:, k])
+            assert np.array_equal(fs[k].flat(), f1.flat()), k
+            assert np.array_equal(ss[:, k], s1.flat()), k
