@@ -876,7 +876,9 @@ namespace {
 // Nodes{Primal} -> other grid kinds by averaging the neighbouring primal cells
 // (CartesianGrids grid_interpolate!): dual node (i,j) from cells (i-1..i,
 // j-1..j), u-edge (i,j) from cells (i-1..i, j), v-edge (i,j) from (i, j-1..j);
-// values on the outer ring (missing cells) are 0.
+// values on the outer ring (missing cells) are 0.  comp: the primal values
+// are 1 - m (the complementary mask is averaged, as the reference averages
+// its primal complementary mask).
 __global__ void k_mask_to(const double* __restrict__ m, int NX, int NY, int kind, int comp, double* __restrict__ out,
                           int multiply)
 {
@@ -888,19 +890,21 @@ __global__ void k_mask_to(const double* __restrict__ m, int NX, int NY, int kind
     else { nx = NX - 1; ny = NY; }                    // v edges
     const int i = blockIdx.x * 64 + threadIdx.x % 64, j = blockIdx.y * 4 + threadIdx.x / 64;
     if (i >= nx || j >= ny) return;
+    auto mp = [&](int a, int b) {
+        const double x = m[a + (size_t)np * b];
+        return comp ? 1.0 - x : x;
+    };
     double v = 0.0;
     if (kind == ILH_NODES_PRIMAL) {
-        v = m[i + (size_t)np * j];
+        v = mp(i, j);
     } else if (kind == ILH_NODES_DUAL) {
         if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2)
-            v = 0.25 * (m[i - 1 + (size_t)np * (j - 1)] + m[i + (size_t)np * (j - 1)] + m[i - 1 + (size_t)np * j] +
-                        m[i + (size_t)np * j]);
+            v = 0.25 * (mp(i - 1, j - 1) + mp(i, j - 1) + mp(i - 1, j) + mp(i, j));
     } else if (kind == 2) {
-        if (i >= 1 && i <= NX - 2) v = 0.5 * (m[i - 1 + (size_t)np * j] + m[i + (size_t)np * j]);
+        if (i >= 1 && i <= NX - 2) v = 0.5 * (mp(i - 1, j) + mp(i, j));
     } else {
-        if (j >= 1 && j <= NY - 2) v = 0.5 * (m[i + (size_t)np * (j - 1)] + m[i + (size_t)np * j]);
+        if (j >= 1 && j <= NY - 2) v = 0.5 * (mp(i, j - 1) + mp(i, j));
     }
-    if (comp) v = 1.0 - v;
     double* o = out + i + (size_t)nx * j;
     *o = multiply ? *o * v : v;
 }

@@ -491,9 +491,16 @@ class DirichletPoisson:
         """Many boundary data on this geometry at once: fbplus (and fbminus)
         N x K; returns f (K Nodes{Primal}) and s (N x K array)."""
         c = self.cache
-        fp = np.asfortranarray(np.asarray(fbplus, dtype=np.float64).reshape(c.N, -1))
+        fp = np.asarray(fbplus, dtype=np.float64)
+        if fp.ndim == 1:
+            fp = fp.reshape(-1, 1)
+        if fp.ndim != 2 or fp.shape[0] != c.N:
+            raise L.DimensionMismatch("boundary data of shape %s, expected (%d, K)" % (fp.shape, c.N))
+        fp = np.asfortranarray(fp)
         k = fp.shape[1]
-        fm = None if fbminus is None else np.asfortranarray(np.asarray(fbminus, dtype=np.float64).reshape(c.N, k))
+        fm = None
+        if fbminus is not None:
+            fm = np.asfortranarray(np.asarray(fbminus, dtype=np.float64).reshape(fp.shape))
         nx, ny = c.g.primal_shape()
         f = np.zeros((nx * ny, k), order="F")
         s = np.zeros((c.N, k), order="F")

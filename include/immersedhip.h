@@ -216,7 +216,8 @@ int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* plans, const 
 /* ---- masks (src/surface_operators.jl:728-871) ----
  * mask(cache) = -L^-1 D_s 1 on Nodes{Primal} (1 inside the surfaces, on the side
  * opposite the normals; 1 everywhere without points); on Nodes{Dual} and
- * Edges{Primal} the primal mask averaged onto them (grid_interpolate!).
+ * Edges{Primal} the primal mask (or its complement) averaged onto them
+ * (grid_interpolate!), 0 on the outer ring those averages do not reach.
  * ilh_mask writes the mask (complementary != 0: 1 - mask, i.e.
  * complementary_mask) of grid_kind into out; ilh_mask_multiply multiplies w in
  * place (mask! / complementary_mask!). */

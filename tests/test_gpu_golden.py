@@ -127,6 +127,40 @@ def test_empty_body_list(ctx):
     assert np.all(out.data == 0.0)
     f, s = ilhip.DirichletPoisson(cache).solve(np.zeros(0))
     assert np.all(f.data == 0.0) and s.flat().size == 0
+    # masks revert to ones (complement zeros) without points (src/surface_operators.jl:870-873)
+    assert np.all(ilhip.mask(cache).data == 1.0)
+    assert np.all(ilhip.complementary_mask(cache, ilhip.Nodes(ilhip.Dual, g)).data == 0.0)
+    fs, ss = ilhip.DirichletPoisson(cache).solve_multi(np.zeros((0, 3)))
+    assert all(np.all(x.data == 0.0) for x in fs) and ss.size == 0
+    f, s, it = ilhip.DirichletPoisson(cache).solve_moving(np.zeros(0))
+    assert np.all(f.data == 0.0) and it == 0
+    assert ilhip.create_surface_filter(cache).size == 0
+
+
+def test_error_paths_of_the_extended_entry_points(ctx):
+    import ctypes as C
+    import ilhip
+    from ilhip import _lib as L
+    from ilhip import geometry as G
+    g, s, fb = G.config_plain()
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    buf = np.zeros(g.NX * g.NY * 4)
+    p = buf.ctypes.data_as(C.c_void_p)
+    with pytest.raises(ilhip.ArgumentError):
+        L.call("ilh_mask", cache.h, 7, 0, p, L.MEM_HOST)                  # no such grid kind
+    with pytest.raises(ilhip.ArgumentError):
+        L.call("ilh_surface_filter_apply", cache.h, -1, p, p, L.MEM_HOST)
+    with pytest.raises(ilhip.ArgumentError):
+        L.call("ilh_dirichlet_solve_multi", cache.h, 0, p, None, p, p, L.MEM_HOST)
+    prob = ilhip.DirichletPoisson(cache)
+    for tol, it in ((0.0, 10), (1e-10, 0), (1e-10, 201)):
+        with pytest.raises(ilhip.ArgumentError):
+            prob.solve_moving(fb, tol=tol, max_iter=it)
+    with pytest.raises(ilhip.ArgumentError):
+        ilhip.DirichletHeatConduction(cache, -1.0, 1e-4, np.ones(s.N))
+    # the plan still works after the refused calls
+    f, sv = prob.solve(fb)
+    assert np.isfinite(f.data).all()
 
 
 def _moved(s, angle, dx, dy):

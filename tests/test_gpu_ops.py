@@ -216,12 +216,14 @@ def test_masks(setup):
     il, cache, gs, og, geo, rng = setup
     m = O.mask(og, geo)
     likes = [(None, O.PRIMAL), (il.Nodes(il.Dual, gs), O.DUAL), (il.Edges(gs), None)]
+    to = lambda v, okind: (np.concatenate([O.primal_to(og, v, O.EDGE_U), O.primal_to(og, v, O.EDGE_V)])
+                           if okind is None else O.primal_to(og, v, okind))
     for like, okind in likes:
-        ref = (np.concatenate([O.primal_to(og, m, O.EDGE_U), O.primal_to(og, m, O.EDGE_V)]) if okind is None
-               else O.primal_to(og, m, okind))
+        ref = to(m, okind)
+        cref = to(1.0 - m, okind)     # the primal complement, averaged
         got = il.mask(cache, like)
         assert max_rel(got.flat(), ref) < 1e-12, okind
-        assert max_rel(il.complementary_mask(cache, like).flat(), 1.0 - ref) < 1e-12, okind
+        assert max_rel(il.complementary_mask(cache, like).flat(), cref) < 1e-12, okind
         w = rng.standard_normal(ref.size)
         x = il.mask(cache, like)
         x.flat()[:] = w
@@ -229,7 +231,7 @@ def test_masks(setup):
         assert max_rel(x.flat(), w * ref) < 1e-12, okind
         x.flat()[:] = w
         il.complementary_mask_(x, cache)
-        assert max_rel(x.flat(), w * (1.0 - ref)) < 1e-12, okind
+        assert max_rel(x.flat(), w * cref) < 1e-12, okind
 
 
 def test_surface_filter(setup):
