@@ -660,6 +660,7 @@ SolveCache* prepare_dirichlet(ilh_plan* pl)
                 transpose(st, B.as<double>() + (size_t)k * N * N, sc->M[k].as<double>(), N, N, ld);
             }
             merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, sc->terms1, sc->src1);
+            pin_source_pairs(pl, sc->src1);
             ILH_HIP(hipStreamSynchronize(st));
         }
         sc->amp1.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));
@@ -701,6 +702,8 @@ SolveCache* prepare_neumann(ilh_plan* pl)
             ILH_HIP(hipGetLastError());
             merge_sources(pl, OP_DS, OP_R_PRIMAL, GK_PRIMAL, sc->terms1, sc->src1);
             merge_sources(pl, OP_CST, OP_CXT, GK_DUAL, sc->terms2, sc->src2);
+            pin_source_pairs(pl, sc->src1);
+            pin_source_pairs(pl, sc->src2);
             ILH_HIP(hipStreamSynchronize(st));
         }
         sc->amp1.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));

@@ -404,6 +404,10 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
         // LDS: zeros, then the rows' nonzero cells
         double* a_lds = (double*)(lds + (size_t)NB * lds_len(P));
         const int4 pi = live ? S.pinfo[p] : make_int4(0, 0, 0, 0);
+        // the first round of cells, in flight across the amplitude staging
+        const int ci0 = pi.y + t;
+        const int4 ca0 = ci0 < pi.w ? S.cells[ci0] : make_int4(0, 0, 0, 0);
+        const int4 cb0 = ci0 + T < pi.w ? S.cells[ci0 + T] : make_int4(0, 0, 0, 0);
         if (S.amp_lds) {
             // 16 loads in flight per thread, then the LDS stores
             for (int base = 0; base < S.amp_len; base += 16 * (int)blockDim.x) {
@@ -422,9 +426,9 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
         }
         for (int i = t; i < P; i += T) x[lx(i)] = make_double2(0.0, 0.0);
         __syncthreads();
-        for (int ci = pi.y + t; ci < pi.w; ci += 2 * T) {
-            const int4 ca = S.cells[ci];
-            const int4 cb = ci + T < pi.w ? S.cells[ci + T] : make_int4(0, 0, 0, 0);
+        for (int ci = ci0; ci < pi.w; ci += 2 * T) {
+            const int4 ca = ci == ci0 ? ca0 : S.cells[ci];
+            const int4 cb = ci == ci0 ? cb0 : (ci + T < pi.w ? S.cells[ci + T] : make_int4(0, 0, 0, 0));
             double va, vb;
             if (S.amp_lds) cell_values<true>(S, a_lds, ca, cb, va, vb);
             else cell_values<false>(S, a_lds, ca, cb, va, vb);
@@ -967,6 +971,18 @@ void build_source_pairs(ilh_plan* pl, SourceSet& s)
     k_source_pairs<<<1, 1024, 0, pl->ctx->stream>>>(s.bins.row_ptr, s.bins.cell_ptr, ny, s.pinfo, s.slot_of_row,
                                                     s.npairs_dev);
     ILH_HIP(hipGetLastError());
+}
+
+// The exact live pair count on the host (one read-back; for caches built
+// once and used by many solves): the row pass then launches exactly.
+void pin_source_pairs(ilh_plan* pl, SourceSet& s)
+{
+    if (!s.npairs_dev) return;
+    int n = 0;
+    ILH_HIP(hipMemcpyAsync(&n, s.npairs_dev, sizeof(int), hipMemcpyDeviceToHost, pl->ctx->stream));
+    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+    s.npairs = n;
+    s.npairs_dev = nullptr;
 }
 
 static RowSrc terms_source(const ConvPlan& C, const SourceSet& src, const double* amp)

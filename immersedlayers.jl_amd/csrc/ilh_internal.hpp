@@ -216,6 +216,7 @@ struct SourceSet {
 };
 void free_sources(SourceSet& s);   // releases the bins and pair tables
 void build_source_pairs(ilh_plan* pl, SourceSet& s);
+void pin_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);
 void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage);
