@@ -67,6 +67,15 @@ struct SolveCache {
     }
 };
 
+// Stokes flow (vector cache): inverses of create_CL2invCT and
+// create_CLinvCT_scalar, and the solve's work buffers
+struct StokesCache {
+    DevBuf S2inv, Ssinv;          // row-major inverses, leading dimensions solve_ld(2N), solve_ld(N)
+    DevBuf S2;                    // create_CL2invCT row-major (the residual of the refinement step)
+    DevBuf sv;                    // surface work: dv, vb, e_phi, w, vprime, sigma (2N each); dvn, dsv, ds (N each)
+    DevBuf gE, gE2, gD, gP;       // grid work: edges (2), dual, primal
+};
+
 // Moving bodies: the inverse Schur matrix of a reference geometry (row-major,
 // leading dimension solve_ld(N)) preconditions GMRES on the current one; the
 // current geometry's merged sources [R, D_s] are rebuilt after every
@@ -117,7 +126,9 @@ static void drop_caches(ilh_plan* pl, bool release = false)
     if (pl->mov) pl->mov->drop_sources();
     delete pl->dir;
     delete pl->neu;
+    delete pl->sto;
     pl->dir = pl->neu = nullptr;
+    pl->sto = nullptr;
     for (auto& f : pl->fac) {
         delete f;
         f = nullptr;
@@ -125,10 +136,8 @@ static void drop_caches(ilh_plan* pl, bool release = false)
     for (int op = 0; op < OP_COUNT; ++op) {
         pl->terms_ok[op] = false;
         pl->bins_ok[op] = false;
-        if (release) {
-            free_bins(pl->bins[op][0]);
-            free_bins(pl->bins[op][1]);
-        }
+        if (release)
+            for (auto& b : pl->bins[op]) free_bins(b);
     }
 }
 
@@ -343,20 +352,22 @@ void need_margin(ilh_plan* pl, const char* what)
         fail(ILH_ERR_RANGE, std::string(what) + ": surface points must lie at least 5 cells inside the grid");
 }
 
-// component grids touched by a scatter op
+// component grids touched by a scatter op, in storage order of its output
 int op_components(int op, int* gks)
 {
     switch (op) {
     case OP_R_PRIMAL: case OP_DS: gks[0] = GK_PRIMAL; return 1;
-    case OP_R_DUAL: case OP_CST: case OP_CXT: gks[0] = GK_DUAL; return 1;
+    case OP_R_DUAL: case OP_CST: case OP_CXT: case OP_CVT: gks[0] = GK_DUAL; return 1;
+    case OP_RNSYM: gks[0] = GK_T_XX; gks[1] = GK_T_XY; gks[2] = GK_T_YX; gks[3] = GK_T_YY; return 4;
     default: gks[0] = GK_EDGE_U; gks[1] = GK_EDGE_V; return 2;
     }
 }
 
-void scatter_op(ilh_plan* pl, int op, const double* amp, double* out)
+void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride, int nb, double* out,
+                      size_t out_stride)
 {
     const Term* t = get_terms(pl, op);
-    int gks[2];
+    int gks[4];
     const int nc = op_components(op, gks);
     if (!pl->bins_ok[op]) {
         for (int c = 0; c < nc; ++c) build_bins(pl, t, pl->N * op_terms_per_point(op), gks[c], pl->bins[op][c]);
@@ -364,17 +375,25 @@ void scatter_op(ilh_plan* pl, int op, const double* amp, double* out)
     }
     size_t off = 0;
     for (int c = 0; c < nc; ++c) {
-        scatter(pl, t, pl->bins[op][c], amp, out + off);
+        scatter(pl, pl->bins[op][c], amp, amp_stride, nb, out + off, out_stride);
         off += pl->g.n(gks[c]);
     }
 }
 
+void scatter_op(ilh_plan* pl, int op, const double* amp, double* out)
+{
+    scatter_op_multi(pl, op, amp, 0, 1, out, 0);
+}
+
+void gather_op_multi(ilh_plan* pl, int op, const double* field, size_t field_stride, int nb, double* out,
+                     size_t out_stride)
+{
+    gather(pl, get_terms(pl, op), op_terms_per_point(op), field, field_stride, nb, out, out_stride);
+}
+
 void gather_op(ilh_plan* pl, int op, const double* field, double* out)
 {
-    const Term* t = get_terms(pl, op);
-    const double* fu = field;
-    const double* fv = field + pl->g.n(GK_EDGE_U);
-    gather(pl, t, op_terms_per_point(op), field, fu, fv, out);
+    gather_op_multi(pl, op, field, 0, 1, out, 0);
 }
 
 }  // namespace ilh
@@ -504,6 +523,42 @@ extern "C" int ilh_surface_grad_cross(ilh_plan* pl, double* vn, const double* ph
     return grid_to_surf(pl, OP_GXS, pl->g.n(GK_PRIMAL), pl->N, vn, phi, mem, true, "surface_grad_cross!");
 }
 
+// vector cache (SurfaceVectorCache, src/cache.jl:184-201): VectorData is
+// [u; v] (2N), EdgeGradient [dudx; dudy; dvdx; dvdy]
+extern "C" int ilh_surface_curl_vector(ilh_plan* pl, double* w, const double* v, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_CVT, 2 * (size_t)pl->N, pl->g.n(GK_DUAL), w, v, mem, true, "surface_curl!");
+}
+extern "C" int ilh_surface_curl_vector_adj(ilh_plan* pl, double* v, const double* s, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_CVS, pl->g.n(GK_DUAL), 2 * (size_t)pl->N, v, s, mem, true, "surface_curl!");
+}
+extern "C" int ilh_surface_divergence_symm(ilh_plan* pl, double* q, const double* dv, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_DSYM, 2 * (size_t)pl->N, pl->g.n_edges(), q, dv, mem, true,
+                        "surface_divergence_symm!");
+}
+extern "C" int ilh_surface_grad_symm(ilh_plan* pl, double* tau, const double* q, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_GSYM, pl->g.n_edges(), 2 * (size_t)pl->N, tau, q, mem, true, "surface_grad_symm!");
+}
+extern "C" int ilh_regularize_normal_symm(ilh_plan* pl, double* A, const double* v, int mem)
+{
+    PL_CHECK
+    return surf_to_grid(pl, OP_RNSYM, 2 * (size_t)pl->N, pl->g.n_tensor(), A, v, mem, false,
+                        "regularize_normal_symm!");
+}
+extern "C" int ilh_normal_interpolate_symm(ilh_plan* pl, double* tau, const double* A, int mem)
+{
+    PL_CHECK
+    return grid_to_surf(pl, OP_NISYM, pl->g.n_tensor(), 2 * (size_t)pl->N, tau, A, mem, false,
+                        "normal_interpolate_symm!");
+}
+
 // -------------------------------------------------------------- stencils
 template <class K>
 static int grid_op(ilh_plan* pl, size_t nin, size_t nout, double* out, const double* in, int mem, K&& kern)
@@ -596,10 +651,54 @@ __global__ void k_term_shift(const Term* src, int n, int ai_off, Term* dst)
     dst[k] = t;
 }
 
+// create_CL2invCT (src/matrix_operators.jl:95-125): -C_s L^-2 C_s^T on
+// VectorData (2N x 2N, column-major), column by column as the reference
+// builds it (the twice-applied inverse Laplacian of the grid is not a
+// translation-invariant kernel, so no table assembly): batches of unit
+// columns, C_s^T by one scatter, L^-1 twice by batched FFT convolutions on
+// Nodes{Dual}, C_s by one gather.
+__global__ void k_unit_columns(double* __restrict__ a, int n, int c0, int m)
+{
+    const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= (size_t)n * m) return;
+    const int col = (int)(k / n), row = (int)(k - (size_t)col * n);
+    a[k] = row == c0 + col ? 1.0 : 0.0;
+}
+
+void build_cl2invct(ilh_plan* pl, double scale, double* K)
+{
+    const int n2 = 2 * pl->N;
+    const size_t nd = pl->g.n(GK_DUAL);
+    hipStream_t st = pl->ctx->stream;
+    // batch width: at most 32 columns and ~1 GB of fields and spectra
+    const size_t per_col = 2 * nd * sizeof(double) + 4 * nd * sizeof(double2);
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)n2, ((size_t)1 << 30) / per_col}));
+    DevBuf amp, f1, f2;
+    amp.ensure(sizeof(double) * (size_t)n2 * nb);
+    f1.ensure(sizeof(double) * nd * nb);
+    f2.ensure(sizeof(double) * nd * nb);
+    for (int c0 = 0; c0 < n2; c0 += nb) {
+        const int m = std::min(nb, n2 - c0);
+        const size_t na = (size_t)n2 * m;
+        k_unit_columns<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(amp.as<double>(), n2, c0, m);
+        ILH_HIP(hipGetLastError());
+        scatter_op_multi(pl, OP_CVT, amp.as<double>(), n2, m, f1.as<double>(), nd);
+        conv_dense_multi(pl, GK_DUAL, f1.as<double>(), m, f2.as<double>());
+        conv_dense_multi(pl, GK_DUAL, f2.as<double>(), m, f1.as<double>());
+        gather_op_multi(pl, OP_CVS, f1.as<double>(), nd, m, K + (size_t)n2 * c0, n2);
+    }
+    axpby(st, K, K, -scale, 0.0, (size_t)n2 * n2);
+    ILH_HIP(hipStreamSynchronize(st));   // the batch buffers are released on return
+}
+
+// order of the matrix a Schur kind builds
+int schur_dim(const ilh_plan* pl, int kind) { return kind == ILH_SCHUR_CL2INVCT ? 2 * pl->N : pl->N; }
+
 void build_schur(ilh_plan* pl, int kind, double scale, double* K)
 {
     const double dx2 = pl->g.dx * pl->g.dx;
     switch (kind) {
+    case ILH_SCHUR_CL2INVCT: build_cl2invct(pl, scale, K); break;
     case ILH_SCHUR_RTLINVR: assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -scale * dx2, false, K); break;
     case ILH_SCHUR_CLINVCT: assemble(pl, OP_CS, OP_CST, GK_DUAL, -scale * dx2, false, K); break;
     case ILH_SCHUR_GLINVD: assemble(pl, OP_GS, OP_DS, GK_PRIMAL, -scale * dx2, false, K); break;
@@ -610,7 +709,7 @@ void build_schur(ilh_plan* pl, int kind, double scale, double* K)
 
 void factor_into(ilh_plan* pl, SolveCache* sc, int kind)
 {
-    const int N = pl->N;
+    const int N = schur_dim(pl, kind);
     sc->LU.ensure(sizeof(double) * (size_t)N * N);
     sc->piv.ensure(sizeof(int) * (size_t)std::max(1, N));
     sc->info.ensure(sizeof(int));
@@ -721,8 +820,9 @@ extern "C" int ilh_schur_build(ilh_plan* pl, int kind, double scale, double* out
 {
     return guarded([&] {
         Io io(pl, mem);
+        if (kind < 0 || kind > ILH_SCHUR_CL2INVCT) fail(ILH_ERR_ARG, "unknown Schur kind");
         if (kind != ILH_SCHUR_NRTRN) need_margin(pl, "Schur build");
-        const int N = pl->N;
+        const int N = schur_dim(pl, kind);
         const size_t NN = (size_t)N * N;
         double* d_out = io.out(out, NN, pl->io_b);
         if (N > 0) build_schur(pl, kind, scale, d_out);
@@ -735,7 +835,7 @@ extern "C" int ilh_schur_factor(ilh_plan* pl, int kind)
 {
     return guarded([&] {
         Io io(pl, ILH_MEM_DEVICE);
-        if (kind < 0 || kind > 3) fail(ILH_ERR_ARG, "unknown Schur kind");
+        if (kind < 0 || kind > ILH_SCHUR_CL2INVCT) fail(ILH_ERR_ARG, "unknown Schur kind");
         if (kind != ILH_SCHUR_NRTRN) need_margin(pl, "Schur factor");
         delete pl->fac[kind];
         pl->fac[kind] = nullptr;
@@ -754,8 +854,9 @@ extern "C" int ilh_schur_solve(ilh_plan* pl, int kind, double* x, const double* 
 {
     return guarded([&] {
         Io io(pl, mem);
-        if (kind < 0 || kind > 3 || !pl->fac[kind]) fail(ILH_ERR_STATE, "ilh_schur_factor must be called first");
-        const int N = pl->N;
+        if (kind < 0 || kind > ILH_SCHUR_CL2INVCT || !pl->fac[kind])
+            fail(ILH_ERR_STATE, "ilh_schur_factor must be called first");
+        const int N = schur_dim(pl, kind);
         const double* d_b = io.in(b, N, pl->io_a);
         double* d_x = io.out(x, N, pl->io_b);
         if (N > 0) {
@@ -1007,16 +1108,18 @@ namespace {
 // C = Et R (create_surface_filter, src/matrix_operators.jl:254-267): Et is the
 // interpolation with each grid value divided by (R 1) there, so that C keeps
 // constants and the surface integral (sum_p ds_p (C s)_p = sum_q ds_q s_q).
-// One thread per point p: its 4 x 4 support cells in (row, x) order, each
-// cell's contributions (points q with R weight cw) in the bins' order.
-// C column-major N x N, zeroed by the caller.
-__global__ void k_filter_build(const Term* __restrict__ TE, int N, const int* __restrict__ cell_ptr,
-                               const int4* __restrict__ cells, const int* __restrict__ cai,
-                               const double* __restrict__ cw, int ny, double* __restrict__ C)
+// One thread per point p and its interpolation term TE[p tstride + toff] on
+// the component grid of the bins: its 4 x 4 support cells in (row, x) order,
+// each cell's contributions (amplitude slots q with R weight cw) in the bins'
+// order, into row t.ai of C (column-major n x n, zeroed by the caller).
+__global__ void k_filter_build(const Term* __restrict__ TE, int tstride, int toff, int N,
+                               const int* __restrict__ cell_ptr, const int4* __restrict__ cells,
+                               const int* __restrict__ cai, const double* __restrict__ cw, int ny, int n,
+                               double* __restrict__ C)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
-    const Term& t = TE[p];
+    const Term& t = TE[(size_t)p * tstride + toff];
     for (int b = 0; b < 5; ++b) {
         const int j = t.by + b;
         if (t.wy[b] == 0.0 || j < 0 || j >= ny) continue;
@@ -1035,7 +1138,7 @@ __global__ void k_filter_build(const Term* __restrict__ TE, int N, const int* __
             for (int e = cl.y; e < cl.z; ++e) r1 += cw[e];
             if (r1 == 0.0) continue;
             const double phi = t.coef * t.wx[a] * t.wy[b] / r1;
-            for (int e = cl.y; e < cl.z; ++e) C[p + (size_t)N * cai[e]] += phi * cw[e];
+            for (int e = cl.y; e < cl.z; ++e) C[t.ai + (size_t)n * cai[e]] += phi * cw[e];
         }
     }
 }
@@ -1081,7 +1184,8 @@ extern "C" int ilh_surface_filter_build(ilh_plan* pl, double* C, int mem)
                 pl->bins_ok[OP_R_PRIMAL] = true;
             }
             const RowBins& b = pl->bins[OP_R_PRIMAL][0];
-            k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, d_C);
+            k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 1, 0, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, N,
+                                                            d_C);
             ILH_HIP(hipGetLastError());
         }
         io.back(C, d_C, nn);
@@ -1116,6 +1220,239 @@ extern "C" int ilh_surface_filter_apply(ilh_plan* pl, int power, const double* s
             ILH_HIP(hipStreamSynchronize(st));
         }
         io.back(s_out, d_out, N);
+        io.done();
+    });
+}
+
+// ------------------------------------------- vector cache: surface filter
+namespace {
+// R 1 on Edges{Primal} (both components), cached per geometry
+const double* filter_r1e(ilh_plan* pl, DevBuf& ones)
+{
+    const size_t ne = pl->g.n_edges();
+    if (pl->filt_gen_e != pl->gen) {
+        const int n2 = std::max(1, 2 * pl->N);
+        pl->filt_r1e.ensure(sizeof(double) * ne);
+        ones.ensure(sizeof(double) * n2);
+        k_fill_value<<<(n2 + 255) / 256, 256, 0, pl->ctx->stream>>>(ones.as<double>(), (size_t)2 * pl->N, 1.0);
+        scatter_op(pl, OP_R_EDGES, ones.as<double>(), pl->filt_r1e.as<double>());
+        pl->filt_gen_e = pl->gen;
+    }
+    return pl->filt_r1e.as<double>();
+}
+
+// v <- C^power v on VectorData (device, 2N), C applied matrix-free: scatter
+// onto the edges, divide by R 1, interpolate back; g: edge-sized scratch
+void filter_vector(ilh_plan* pl, int power, double* v, double* g)
+{
+    DevBuf ones;
+    const double* r1 = filter_r1e(pl, ones);
+    const size_t ne = pl->g.n_edges();
+    const unsigned gb = (unsigned)std::min<size_t>(4096, (ne + 255) / 256);
+    for (int k = 0; k < power; ++k) {
+        scatter_op(pl, OP_R_EDGES, v, g);
+        k_filter_weight<<<gb, 256, 0, pl->ctx->stream>>>(g, r1, ne);
+        gather_op(pl, OP_E_EDGES, g, v);
+    }
+    ILH_HIP(hipGetLastError());
+    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));   // `ones` is released on return
+}
+}  // namespace
+
+// create_surface_filter of the vector cache: block-diagonal 2N x 2N, the
+// u block from the u edges, the v block from the v edges
+extern "C" int ilh_surface_filter_build_vector(ilh_plan* pl, double* C, int mem)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        Io io(pl, mem);
+        const int N = pl->N, n2 = 2 * N;
+        const size_t nn = (size_t)n2 * n2;
+        double* d_C = io.out(C, nn, pl->io_b);
+        hipStream_t st = pl->ctx->stream;
+        if (N > 0) {
+            fill_zero(st, d_C, nn);
+            const Term* te = get_terms(pl, OP_E_EDGES);
+            const Term* tr = get_terms(pl, OP_R_EDGES);
+            if (!pl->bins_ok[OP_R_EDGES]) {
+                build_bins(pl, tr, 2 * N, GK_EDGE_U, pl->bins[OP_R_EDGES][0]);
+                build_bins(pl, tr, 2 * N, GK_EDGE_V, pl->bins[OP_R_EDGES][1]);
+                pl->bins_ok[OP_R_EDGES] = true;
+            }
+            for (int c = 0; c < 2; ++c) {
+                const RowBins& b = pl->bins[OP_R_EDGES][c];
+                k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 2, c, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny,
+                                                                n2, d_C);
+            }
+            ILH_HIP(hipGetLastError());
+        }
+        io.back(C, d_C, nn);
+        io.done();
+    });
+}
+
+extern "C" int ilh_surface_filter_apply_vector(ilh_plan* pl, int power, const double* v_in, double* v_out, int mem)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        if (power < 0 || power > 1000) fail(ILH_ERR_ARG, "power must be in [0, 1000]");
+        Io io(pl, mem);
+        const int n2 = 2 * pl->N;
+        hipStream_t st = pl->ctx->stream;
+        const double* d_in = io.in(v_in, n2, pl->io_a);
+        double* d_out = io.out(v_out, n2, pl->io_d);
+        if (n2 > 0) {
+            if (d_out != d_in) ILH_HIP(hipMemcpyAsync(d_out, d_in, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
+            pl->scratch_a.ensure(sizeof(double) * pl->g.n_edges());
+            filter_vector(pl, power, d_out, pl->scratch_a.as<double>());
+        }
+        io.back(v_out, d_out, n2);
+        io.done();
+    });
+}
+
+// ------------------------------------------------------------ Stokes flow
+// test/literate/stokes.jl:101-166 on the vector cache.  Per geometry: S =
+// create_CL2invCT (2N x 2N) and its inverse, and the inverse of Ss =
+// create_CLinvCT_scalar (N x N), row-major.  Per solve, with a = L^-2 C D_s^symm [v] (so the
+// reference's psi* is -a), phi = L^-1 R (n . [v]), v_phi = G phi:
+//   sigma = S^-1 (v_bar - E v_phi + E C a),   b = L^-2 C_s^T sigma,
+//   v = v_phi - C (a + b),   s = -(a + b + L^-1 C^x_s^T Ss^-1 G^x_s phi),
+// the reference's sequence with its negations folded into the sums (each
+// folded negation is exact, so the rounding is the reference order's).
+namespace {
+__global__ void k_stokes_rhs(int N, const double* __restrict__ vp, const double* __restrict__ vm,
+                             const double* __restrict__ nx, const double* __restrict__ ny, double* __restrict__ dv,
+                             double* __restrict__ vb, double* __restrict__ dvn)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    double d[2];
+    for (int c = 0; c < 2; ++c) {
+        const int k = p + c * N;
+        const double a = vp[k], b = vm ? vm[k] : 0.0;
+        d[c] = a - b;                 // prescribed_surface_jump!
+        dv[k] = d[c];
+        vb[k] = 0.5 * a + 0.5 * b;    // prescribed_surface_average!
+    }
+    dvn[p] = nx[p] * d[0] + ny[p] * d[1];   // pointwise_dot!(dvn, nrm, dv)
+}
+
+// vprime = (vb - e_phi) + w
+__global__ void k_stokes_vprime(int n, const double* __restrict__ vb, const double* __restrict__ ephi,
+                                const double* __restrict__ w, double* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = (vb[k] - ephi[k]) + w[k];
+}
+
+StokesCache* prepare_stokes(ilh_plan* pl)
+{
+    if (pl->sto) return pl->sto;
+    need_margin(pl, "Stokes solve");
+    StokesCache* sc = new StokesCache();
+    try {
+        const int N = pl->N, n2 = 2 * N;
+        const Grid& g = pl->g;
+        sc->sv.ensure(sizeof(double) * (size_t)std::max(1, 6 * n2 + 3 * N));
+        sc->gE.ensure(sizeof(double) * g.n_edges());
+        sc->gE2.ensure(sizeof(double) * g.n_edges());
+        sc->gD.ensure(sizeof(double) * g.n(GK_DUAL));
+        sc->gP.ensure(sizeof(double) * g.n(GK_PRIMAL));
+        if (N > 0) {
+            DevBuf K;
+            K.ensure(sizeof(double) * (size_t)n2 * n2);
+            build_cl2invct(pl, 1.0, K.as<double>());
+            sc->S2.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
+            transpose(pl->ctx->stream, K.as<double>(), sc->S2.as<double>(), n2, n2, solve_ld(n2));
+            sc->S2inv.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
+            inverse_rowmajor(pl, K.as<double>(), sc->S2inv.as<double>(), n2);
+            build_schur(pl, ILH_SCHUR_CLINVCT, 1.0, K.as<double>());
+            sc->Ssinv.ensure(sizeof(double) * (size_t)N * solve_ld(N));
+            inverse_rowmajor(pl, K.as<double>(), sc->Ssinv.as<double>(), N);
+        }
+    } catch (...) {
+        delete sc;
+        throw;
+    }
+    pl->sto = sc;
+    return sc;
+}
+}  // namespace
+
+extern "C" int ilh_stokes_solve(ilh_plan* pl, const double* v_plus, const double* v_minus, double* v, double* s,
+                                double* sigma, int filter_power, int mem)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        if (filter_power < 0 || filter_power > 1000) fail(ILH_ERR_ARG, "filter_power must be in [0, 1000]");
+        Io io(pl, mem);
+        StokesCache* sc = prepare_stokes(pl);
+        const int N = pl->N, n2 = 2 * N;
+        const Grid& g = pl->g;
+        const size_t ne = g.n_edges(), nd = g.n(GK_DUAL);
+        hipStream_t st = pl->ctx->stream;
+        const double* d_vp = io.in(v_plus, n2, pl->io_a);
+        const double* d_vm = v_minus ? io.in(v_minus, n2, pl->io_c) : nullptr;
+        double* d_v = io.out(v, ne, pl->io_b);
+        double* d_s = io.out(s, nd, pl->io_d);
+        if (n2 > 0 && !sigma) fail(ILH_ERR_ARG, "null output array");
+        // sigma is staged in the surface work area when it lives on the host
+        double* sv = sc->sv.as<double>();
+        double *dv = sv, *vb = sv + n2, *ephi = sv + 2 * n2, *w = sv + 3 * n2, *vpr = sv + 4 * n2;
+        double* sig = mem == ILH_MEM_DEVICE ? sigma : sv + 5 * n2;
+        double *dvn = sv + 6 * n2, *dsv = dvn + N, *dsn = dsv + N;
+        double *gE = sc->gE.as<double>(), *gE2 = sc->gE2.as<double>(), *gD = sc->gD.as<double>();
+        double* gP = sc->gP.as<double>();
+        if (N == 0) {
+            fill_zero(st, d_v, ne);
+            fill_zero(st, d_s, nd);
+        } else {
+            const unsigned bN = (unsigned)((N + 127) / 128), b2 = (unsigned)((n2 + 127) / 128);
+            k_stokes_rhs<<<bN, 128, 0, st>>>(N, d_vp, d_vm, pl->nx.as<double>(), pl->ny.as<double>(), dv, vb, dvn);
+            // a = L^-2 C D_s^symm [v]
+            scatter_op(pl, OP_DSYM, dv, gE);
+            k_curl_e2n(pl, gD, gE);
+            conv_dense(pl, GK_DUAL, gD, gD);
+            conv_dense(pl, GK_DUAL, gD, gD);
+            // phi = L^-1 R (n . [v]),  v_phi = G phi
+            scatter_op(pl, OP_R_PRIMAL, dvn, gP);
+            conv_dense(pl, GK_PRIMAL, gP, gP);
+            k_grad(pl, gE2, gP);
+            gather_op(pl, OP_E_EDGES, gE2, ephi);
+            // w = E C a (the reference's -E C psi*)
+            k_curl_n2e(pl, gE, gD);
+            gather_op(pl, OP_E_EDGES, gE, w);
+            k_stokes_vprime<<<b2, 128, 0, st>>>(n2, vb, ephi, w, vpr);
+            ILH_HIP(hipGetLastError());
+            // sigma = S^-1 vprime with one refinement step, so that the residual
+            // (the constraint E v = v_bar) is at roundoff as with the
+            // reference's LU solve, not at cond(S) eps
+            const int ld2 = solve_ld(n2);
+            matvec_rows(st, sc->S2inv.as<double>(), ld2, vpr, sig, n2, n2, 1.0, 0);
+            axpby(st, ephi, vpr, 1.0, 0.0, n2);
+            matvec_rows(st, sc->S2.as<double>(), ld2, sig, ephi, n2, n2, -1.0, 1);
+            matvec_rows(st, sc->S2inv.as<double>(), ld2, ephi, sig, n2, n2, 1.0, 1);
+            // b = L^-2 C_s^T sigma;  gD <- a + b
+            scatter_op(pl, OP_CVT, sig, d_s);
+            conv_dense(pl, GK_DUAL, d_s, d_s);
+            conv_dense(pl, GK_DUAL, d_s, d_s);
+            axpby(st, gD, d_s, 1.0, 1.0, nd);
+            // v = v_phi - C (a + b)
+            k_curl_n2e(pl, d_v, gD);
+            axpby(st, d_v, gE2, 1.0, -1.0, ne);
+            // streamfunction of the potential part: L^-1 C^x_s^T Ss^-1 G^x_s phi
+            gather_op(pl, OP_GXS, gP, dsv);
+            matvec_rows(st, sc->Ssinv.as<double>(), solve_ld(N), dsv, dsn, N, N, 1.0, 0);
+            scatter_op(pl, OP_CXT, dsn, d_s);
+            conv_dense(pl, GK_DUAL, d_s, d_s);
+            axpby(st, d_s, gD, -1.0, -1.0, nd);   // s = -(a + b) - c
+            if (filter_power > 0) filter_vector(pl, filter_power, sig, gE);
+        }
+        io.back(v, d_v, ne);
+        io.back(s, d_s, nd);
+        if (mem != ILH_MEM_DEVICE && n2 > 0)
+            ILH_HIP(hipMemcpyAsync(sigma, sig, sizeof(double) * n2, hipMemcpyDeviceToHost, st));
         io.done();
     });
 }
@@ -1192,12 +1529,6 @@ void batch_issue(ilh_ctx* ctx, int nprob, ilh_plan* const* plans, SolveCache* co
 
 }  // namespace
 
-// ------------------------------------------------------ batched solves
-// Repeat Dirichlet solves of many independent plans of one context: the
-// plans' caches are prepared first (in order, on the context stream), then
-// their solves are issued round-robin on POOL side streams that wait for the
-// context stream's prior work, and the context stream waits for all of them.
-// Kernels of different problems overlap on the GPU.
 extern "C" int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
                                            const double* const* fb_minus, double* const* f, double* const* s,
                                            int mem)
@@ -1518,11 +1849,12 @@ static void schur_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* 
     matvec_rows(pl->ctx->stream, mc->S.as<double>(), solve_ld(pl->N), v, out, pl->N, pl->N, 1.0, 0);
 }
 
-// K (N x N column-major, overwritten by its LU factors) -> Minv = K^-1,
-// row-major with leading dimension solve_ld(N) (allocated by the caller)
-void ilh::inverse_rowmajor(ilh_plan* pl, double* K, double* Minv)
+// K (N x N column-major, overwritten by its LU factors; N = n, or the point
+// count when n = 0) -> Minv = K^-1, row-major with leading dimension
+// solve_ld(N) (allocated by the caller)
+void ilh::inverse_rowmajor(ilh_plan* pl, double* K, double* Minv, int n)
 {
-    const int N = pl->N;
+    const int N = n > 0 ? n : pl->N;
     hipStream_t st = pl->ctx->stream;
     DevBuf piv, info, Sinv;
     piv.ensure(sizeof(int) * (size_t)N);

@@ -381,8 +381,10 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
                                                         size_t amp_stride, size_t out_stride)
 {
     extern __shared__ double2 lds[];
-    // right-hand side blockIdx.y of a batch: its amplitudes and output spectra
-    S.amp += amp_stride * blockIdx.y;
+    // right-hand side blockIdx.y of a batch: its amplitudes (or dense field)
+    // and output spectra
+    if (S.mode == ROW_DENSE) S.field += amp_stride * blockIdx.y;
+    else if (S.mode == ROW_TERMS) S.amp += amp_stride * blockIdx.y;
     Tout += out_stride * blockIdx.y;
     constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
     constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
@@ -898,6 +900,25 @@ void conv_dense(ilh_plan* pl, int gk, const double* in, double* out)
     launch_cols<0>(pl->ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.ny, C.ny, C.spec.as<double>(), 1.0,
                    C.T2.as<double2>(), nullptr);
     launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
+}
+
+void conv_dense_multi(ilh_plan* pl, int gk, const double* in, int nb, double* out)
+{
+    if (nb <= 0) return;
+    ConvPlan& C = get_conv(pl, gk);
+    const size_t n = (size_t)C.nx * C.ny;
+    const size_t s1 = C.T1.bytes / sizeof(double2), s2 = C.T2.bytes / sizeof(double2);
+    C.T1m.ensure(C.T1.bytes * (size_t)nb);
+    C.T2m.ensure(C.T2.bytes * (size_t)nb);
+    RowSrc S{};
+    S.mode = ROW_DENSE;
+    S.field = in;
+    S.nx = C.nx;
+    S.ny = C.ny;
+    launch_rows_fwd(pl->ctx, S, C.Px, (C.ny + 1) / 2, C.ny, C.T1m.as<double2>(), Batch{nb, n, s1});
+    launch_cols<0>(pl->ctx, C.T1m.as<double2>(), nullptr, C.Px, C.Py, C.ny, C.ny, C.spec.as<double>(), 1.0,
+                   C.T2m.as<double2>(), nullptr, Batch{nb, s1, s2});
+    launch_rows_inv(pl->ctx, C.T2m.as<double2>(), C.Px, C.nx, C.ny, out, Batch{nb, s2, n});
 }
 
 // Row pairs (2p, 2p+1) with at least one entry, compacted in order: pinfo[k]

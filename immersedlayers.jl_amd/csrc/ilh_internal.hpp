@@ -23,16 +23,34 @@ void check_hip(hipError_t e, const char* what);
 #define ILH_HIP(call) ::ilh::check_hip((call), #call)
 
 // -------------------------------------------------------------- grid kinds
-// Component grids: Nodes{Primal}, Nodes{Dual}, Edges{Primal}.u, Edges{Primal}.v
-enum GK { GK_PRIMAL = 0, GK_DUAL = 1, GK_EDGE_U = 2, GK_EDGE_V = 3 };
+// Component grids: Nodes{Primal}, Nodes{Dual}, Edges{Primal}.u, Edges{Primal}.v,
+// and the four components of EdgeGradient{Primal,Dual} in storage order
+// [dudx; dudy; dvdx; dvdy]: the diagonal ones on Nodes{Primal}, the
+// off-diagonal ones on Nodes{Dual} (where the MAC differences of the edge
+// components land).
+enum GK { GK_PRIMAL = 0, GK_DUAL = 1, GK_EDGE_U = 2, GK_EDGE_V = 3, GK_T_XX = 4, GK_T_XY = 5, GK_T_YX = 6,
+          GK_T_YY = 7, GK_COUNT = 8 };
 
 struct Grid {
     int NX, NY, I0x, I0y;
     double dx;
-    int nx(int gk) const { return (gk == GK_PRIMAL || gk == GK_EDGE_V) ? NX - 1 : NX; }
-    int ny(int gk) const { return (gk == GK_PRIMAL || gk == GK_EDGE_U) ? NY - 1 : NY; }
+    static constexpr bool primal_shaped(int gk) { return gk == GK_PRIMAL || gk == GK_T_XX || gk == GK_T_YY; }
+    int nx(int gk) const { return (primal_shaped(gk) || gk == GK_EDGE_V) ? NX - 1 : NX; }
+    int ny(int gk) const { return (primal_shaped(gk) || gk == GK_EDGE_U) ? NY - 1 : NY; }
     size_t n(int gk) const { return (size_t)nx(gk) * ny(gk); }
     size_t n_edges() const { return n(GK_EDGE_U) + n(GK_EDGE_V); }
+    size_t n_tensor() const { return 2 * (n(GK_PRIMAL) + n(GK_DUAL)); }
+    // offset of a component grid inside its field (Edges or EdgeGradient)
+    size_t offset(int gk) const
+    {
+        switch (gk) {
+        case GK_EDGE_V: return n(GK_EDGE_U);
+        case GK_T_XY: return n(GK_PRIMAL);
+        case GK_T_YX: return n(GK_PRIMAL) + n(GK_DUAL);
+        case GK_T_YY: return n(GK_PRIMAL) + 2 * n(GK_DUAL);
+        default: return 0;
+        }
+    }
 };
 
 // -------------------------------------------------------------------- terms
@@ -69,6 +87,13 @@ enum OpKind {
     OP_CXT,            // surface_curl_cross! ScalarData -> Nodes{Dual}
     OP_CXS,            // surface_curl_cross! Nodes{Dual} -> ScalarData
     OP_GXS,            // surface_grad_cross! Nodes{Primal} -> ScalarData
+    // vector cache (VectorData on Edges{Primal}, tensors on EdgeGradient)
+    OP_CVT,            // surface_curl!       VectorData -> Nodes{Dual}
+    OP_CVS,            // surface_curl!       Nodes{Dual} -> VectorData
+    OP_DSYM,           // surface_divergence_symm! VectorData -> Edges{Primal}
+    OP_GSYM,           // surface_grad_symm!       Edges{Primal} -> VectorData
+    OP_RNSYM,          // regularize_normal_symm!  VectorData -> EdgeGradient
+    OP_NISYM,          // normal_interpolate_symm! EdgeGradient -> VectorData
     OP_COUNT
 };
 int op_terms_per_point(int op);
@@ -142,6 +167,7 @@ struct ilh_ctx {
 struct SolveCache;   // Schur factor and solve matrices of one composite
 
 struct MovingCache;
+struct StokesCache;
 struct ilh_plan {
     ilh_ctx* ctx = nullptr;
     ilh::Grid g{};
@@ -151,7 +177,7 @@ struct ilh_plan {
     ilh::DevBuf X, Y, nx, ny, ds;            // device geometry
     ilh::DevBuf terms[ilh::OP_COUNT];        // per-op term arrays (built lazily)
     bool terms_ok[ilh::OP_COUNT] = {};
-    ilh::RowBins bins[ilh::OP_COUNT][2];     // row bins of scatter ops per component grid
+    ilh::RowBins bins[ilh::OP_COUNT][4];     // row bins of scatter ops per component grid
     bool bins_ok[ilh::OP_COUNT] = {};
     ilh::ConvPlan conv[2];                    // primal, dual
     bool conv_ok[2] = {};
@@ -162,13 +188,16 @@ struct ilh_plan {
     ilh::DevBuf lu_tinv, lu_perm, lu_tmp, lu_pmap;   // LU scratch (block inverses, row permutations)
     SolveCache* dir = nullptr;                // Dirichlet
     SolveCache* neu = nullptr;                // Neumann
-    SolveCache* fac[4] = {};                  // ilh_schur_factor caches per kind
+    SolveCache* fac[5] = {};                  // ilh_schur_factor caches per kind
+    StokesCache* sto = nullptr;               // Stokes flow (vector cache) solve matrices
     MovingCache* mov = nullptr;               // moving-body Dirichlet solves (kept across update_points)
     unsigned gen = 0;                         // incremented whenever the geometry caches are dropped
     ilh::DevBuf mask;                         // mask(cache) on Nodes{Primal} (built lazily)
     unsigned mask_gen = ~0u;                  // generation the mask belongs to
     ilh::DevBuf filt_r1;                      // R 1 on Nodes{Primal} for the surface filter
     unsigned filt_gen = ~0u;
+    ilh::DevBuf filt_r1e;                     // R 1 on Edges{Primal} for the vector surface filter
+    unsigned filt_gen_e = ~0u;
     std::atomic<int> refs{1};                 // the caller's handle, batches and heat integrators using it
 };
 
@@ -185,9 +214,14 @@ void build_terms(ilh_plan* pl, int op);
 const Term* get_terms(ilh_plan* pl, int op);
 void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& bins);
 void free_bins(RowBins& b);
-void scatter(ilh_plan* pl, const Term* terms, const RowBins& bins, const double* amp, double* out);
-void gather(ilh_plan* pl, const Term* terms, int nterms_per_point, const double* field_primal_or_dual,
-            const double* field_u, const double* field_v, double* out);
+// batched over nb operands (grid.y): operand k reads amp + k amp_stride /
+// field + k field_stride and writes out + k out_stride
+void scatter(ilh_plan* pl, const RowBins& bins, const double* amp, size_t amp_stride, int nb, double* out,
+             size_t out_stride);
+// field: the operator's whole input (Nodes, Edges or EdgeGradient); each term
+// reads its component grid at Grid::offset(gk)
+void gather(ilh_plan* pl, const Term* terms, int nterms_per_point, const double* field, size_t field_stride, int nb,
+            double* out, size_t out_stride);
 
 // stencils (ilh_stencil.hip)
 void k_divergence(ilh_plan* pl, double* p, const double* q);
@@ -203,6 +237,8 @@ void ensure_lgf(ilh_plan* pl);
 ConvPlan& get_conv(ilh_plan* pl, int gk);
 // field -> field:  out = dx^2 (G * in)
 void conv_dense(ilh_plan* pl, int gk, const double* in, double* out);
+// nb fields at once: field k at in + k n(gk), out + k n(gk) (in == out allowed)
+void conv_dense_multi(ilh_plan* pl, int gk, const double* in, int nb, double* out);
 // point sources -> field: out = dx^2 (G * sum_terms amp*term), rows listed in pairs
 struct SourceSet {
     const Term* terms = nullptr;
@@ -228,7 +264,12 @@ void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double
 // (overwritten) into row-major Minv with leading dimension solve_ld(N)
 void scatter_op(ilh_plan* pl, int op, const double* amp, double* out);
 void gather_op(ilh_plan* pl, int op, const double* field, double* out);
-void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv);
+// nb operands at once: amplitudes / fields k at a stride, outputs k at a stride
+void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride, int nb, double* out,
+                      size_t out_stride);
+void gather_op_multi(ilh_plan* pl, int op, const double* field, size_t field_stride, int nb, double* out,
+                     size_t out_stride);
+void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv, int n = 0);
 
 // dense linear algebra (ilh_dense.hip)
 // K = scale * (opA . kernel . opB): column-major N x N with leading dimension

@@ -4,7 +4,8 @@
 //
 // Every surface operator of the scalar cache (src/surface_operators.jl) is a
 // sum over points of at most two separable 5x5 windows ("terms") on one of the
-// four component grids.  The DDF weights and the MAC stencil of the composed
+// four component grids; the symmetric tensor operators of the vector cache
+// take eight (two amplitudes on each of four component grids).  The DDF weights and the MAC stencil of the composed
 // operator are folded into the window weights once per geometry, so e.g.
 // surface_divergence! = D Rf (n o f) / dx becomes a single scatter pass.
 //
@@ -164,6 +165,67 @@ __global__ void k_build_terms(int op, int N, int ddf, Grid g, const double* __re
         put(a, p, GK_DUAL, diff_win(Px, 0), Dy, n_x * i1);
         put(b, p, GK_DUAL, Dx, diff_win(Py, 0), n_y * i1);
         break;
+    // vector cache: VectorData f = (f_u at p, f_v at N + p)
+    case OP_CVT:   // C^T Rf f / dx,  w = (v_i - v_{i-1}) - (u_j - u_{j-1})
+        put(a, N + p, GK_DUAL, diff_win(Px, 0), Dy, r3);
+        put(b, p, GK_DUAL, Dx, diff_win(Py, 0), -r3);
+        break;
+    case OP_CVS:   // Rf^T C s / dx,  u = s_{j+1} - s_j, v = s_i - s_{i+1}
+        put(a, p, GK_DUAL, Dx, diff_win(Py, 0), -i1);
+        put(b, N + p, GK_DUAL, diff_win(Px, 0), Dy, i1);
+        break;
+    case OP_DSYM: case OP_GSYM: case OP_RNSYM: case OP_NISYM: {
+        // The symmetric surface tensor T = n o f + f o n - (n . f) I has
+        // T_xx = -T_yy = n_x f_u - n_y f_v and T_xy = T_yx = n_y f_u + n_x f_v.
+        // Differences: out_i = a_i - a_{i-1} maps nodes to the staggered
+        // positions between them (diff_win(w, 0)), out_i = a_{i+1} - a_i the
+        // staggered positions back to nodes (diff_win(w, -1)).
+        Term* o = out + 8 * (size_t)p;
+        const Win dPx = diff_win(Px, 0), dPy = diff_win(Py, 0), dDx = diff_win(Dx, -1), dDy = diff_win(Dy, -1);
+        if (op == OP_DSYM) {
+            // u edges: d/dx T_xx (primal -> x faces) + d/dy T_xy (dual -> x faces)
+            put(o[0], p, GK_EDGE_U, dPx, Py, n_x * r3);
+            put(o[1], N + p, GK_EDGE_U, dPx, Py, -n_y * r3);
+            put(o[2], p, GK_EDGE_U, Dx, dDy, n_y * r3);
+            put(o[3], N + p, GK_EDGE_U, Dx, dDy, n_x * r3);
+            // v edges: d/dx T_yx (dual -> y faces) + d/dy T_yy (primal -> y faces)
+            put(o[4], p, GK_EDGE_V, dDx, Dy, n_y * r3);
+            put(o[5], N + p, GK_EDGE_V, dDx, Dy, n_x * r3);
+            put(o[6], p, GK_EDGE_V, Px, dPy, -n_x * r3);
+            put(o[7], N + p, GK_EDGE_V, Px, dPy, n_y * r3);
+        } else if (op == OP_GSYM) {
+            // A = G v: A_xx = (u_{i+1} - u_i), A_yy = (v_{j+1} - v_j) on primal
+            // nodes, A_xy = (u_j - u_{j-1}), A_yx = (v_i - v_{i-1}) on dual
+            // nodes; tau = n . (A + A^T - tr(A) I), the negative adjoint of DSYM
+            put(o[0], p, GK_EDGE_U, dPx, Py, -n_x * i1);
+            put(o[1], N + p, GK_EDGE_U, dPx, Py, n_y * i1);
+            put(o[2], p, GK_EDGE_U, Dx, dDy, -n_y * i1);
+            put(o[3], N + p, GK_EDGE_U, Dx, dDy, -n_x * i1);
+            put(o[4], p, GK_EDGE_V, dDx, Dy, -n_y * i1);
+            put(o[5], N + p, GK_EDGE_V, dDx, Dy, -n_x * i1);
+            put(o[6], p, GK_EDGE_V, Px, dPy, n_x * i1);
+            put(o[7], N + p, GK_EDGE_V, Px, dPy, -n_y * i1);
+        } else if (op == OP_RNSYM) {   // R_t T
+            put(o[0], p, GK_T_XX, Px, Py, n_x * r2);
+            put(o[1], N + p, GK_T_XX, Px, Py, -n_y * r2);
+            put(o[2], p, GK_T_XY, Dx, Dy, n_y * r2);
+            put(o[3], N + p, GK_T_XY, Dx, Dy, n_x * r2);
+            put(o[4], p, GK_T_YX, Dx, Dy, n_y * r2);
+            put(o[5], N + p, GK_T_YX, Dx, Dy, n_x * r2);
+            put(o[6], p, GK_T_YY, Px, Py, -n_x * r2);
+            put(o[7], N + p, GK_T_YY, Px, Py, n_y * r2);
+        } else {   // tau = n . R_t^T (A + A^T - tr(A) I)
+            put(o[0], p, GK_T_XX, Px, Py, n_x);
+            put(o[1], N + p, GK_T_XX, Px, Py, -n_y);
+            put(o[2], p, GK_T_XY, Dx, Dy, n_y);
+            put(o[3], N + p, GK_T_XY, Dx, Dy, n_x);
+            put(o[4], p, GK_T_YX, Dx, Dy, n_y);
+            put(o[5], N + p, GK_T_YX, Dx, Dy, n_x);
+            put(o[6], p, GK_T_YY, Px, Py, -n_x);
+            put(o[7], N + p, GK_T_YY, Px, Py, n_y);
+        }
+        return;
+    }
     default: return;
     }
     out[2 * p] = a;
@@ -174,6 +236,7 @@ int op_terms_per_point(int op)
 {
     switch (op) {
     case OP_R_PRIMAL: case OP_R_DUAL: case OP_E_PRIMAL: case OP_E_DUAL: return 1;
+    case OP_DSYM: case OP_GSYM: case OP_RNSYM: case OP_NISYM: return 8;
     default: return 2;
     }
 }
@@ -182,7 +245,7 @@ bool op_is_scatter(int op)
 {
     switch (op) {
     case OP_R_PRIMAL: case OP_R_DUAL: case OP_R_EDGES: case OP_RN: case OP_RNX:
-    case OP_DS: case OP_CST: case OP_CXT: return true;
+    case OP_DS: case OP_CST: case OP_CXT: case OP_CVT: case OP_DSYM: case OP_RNSYM: return true;
     default: return false;
     }
 }
@@ -593,10 +656,12 @@ void free_sources(SourceSet& s)
 // sweep, then one thread per cell.
 __global__ void k_scatter(const int* __restrict__ cell_ptr, const int4* __restrict__ cells,
                           const int* __restrict__ cai, const double* __restrict__ cw, int nx,
-                          const double* __restrict__ amp, double* __restrict__ out)
+                          const double* __restrict__ amp, size_t amp_stride, double* __restrict__ out,
+                          size_t out_stride)
 {
     const int j = blockIdx.x;
-    double* row = out + (size_t)nx * j;
+    amp += amp_stride * blockIdx.y;
+    double* row = out + out_stride * blockIdx.y + (size_t)nx * j;
     for (int x = threadIdx.x; x < nx; x += blockDim.x) row[x] = 0.0;
     const int c0 = cell_ptr[j], c1 = cell_ptr[j + 1];
     if (c0 == c1) return;   // uniform over the work-group
@@ -609,30 +674,33 @@ __global__ void k_scatter(const int* __restrict__ cell_ptr, const int4* __restri
     }
 }
 
-void scatter(ilh_plan* pl, const Term* terms, const RowBins& bins, const double* amp, double* out)
+void scatter(ilh_plan* pl, const RowBins& bins, const double* amp, size_t amp_stride, int nb, double* out,
+             size_t out_stride)
 {
-    (void)terms;
     const int nx = pl->g.nx(bins.gk);
-    if (bins.ny <= 0) return;
-    k_scatter<<<bins.ny, 256, 0, pl->ctx->stream>>>(bins.cell_ptr, bins.cells, bins.cai, bins.cw, nx, amp, out);
+    if (bins.ny <= 0 || nb <= 0) return;
+    k_scatter<<<dim3(bins.ny, nb), 256, 0, pl->ctx->stream>>>(bins.cell_ptr, bins.cells, bins.cai, bins.cw, nx, amp,
+                                                              amp_stride, out, out_stride);
     ILH_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- gather
 struct GatherFields {
-    const double* f[4];   // by component grid
-    int nx[4], ny[4];
+    const double* f[GK_COUNT];   // by component grid
+    int nx[GK_COUNT], ny[GK_COUNT];
 };
 
-__global__ void k_gather(const Term* __restrict__ t, int T, int N, GatherFields F, double* __restrict__ out)
+__global__ void k_gather(const Term* __restrict__ t, int T, int N, GatherFields F, size_t field_stride,
+                         double* __restrict__ out, size_t out_stride)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
+    out += out_stride * blockIdx.y;
     double acc0 = 0.0, acc1 = 0.0;
     int ai0 = -1, ai1 = -1;
     for (int k = 0; k < T; ++k) {
         const Term& tt = t[p * T + k];
-        const double* fld = F.f[tt.gk];
+        const double* fld = F.f[tt.gk] + field_stride * blockIdx.y;
         const int nx = F.nx[tt.gk], ny = F.ny[tt.gk];
         double s = 0.0;
         for (int b = 0; b < 5; ++b) {
@@ -653,15 +721,19 @@ __global__ void k_gather(const Term* __restrict__ t, int T, int N, GatherFields 
     if (ai1 >= 0) out[ai1] = acc1;
 }
 
-void gather(ilh_plan* pl, const Term* terms, int T, const double* fpd, const double* fu, const double* fv,
-            double* out)
+void gather(ilh_plan* pl, const Term* terms, int T, const double* field, size_t field_stride, int nb, double* out,
+            size_t out_stride)
 {
-    if (pl->N == 0) return;
+    if (pl->N == 0 || nb <= 0) return;
     GatherFields F;
     const Grid& g = pl->g;
-    F.f[GK_PRIMAL] = fpd; F.f[GK_DUAL] = fpd; F.f[GK_EDGE_U] = fu; F.f[GK_EDGE_V] = fv;
-    for (int k = 0; k < 4; ++k) { F.nx[k] = g.nx(k); F.ny[k] = g.ny(k); }
-    k_gather<<<(pl->N + 127) / 128, 128, 0, pl->ctx->stream>>>(terms, T, pl->N, F, out);
+    for (int k = 0; k < GK_COUNT; ++k) {
+        F.f[k] = field + g.offset(k);
+        F.nx[k] = g.nx(k);
+        F.ny[k] = g.ny(k);
+    }
+    k_gather<<<dim3((pl->N + 127) / 128, nb), 128, 0, pl->ctx->stream>>>(terms, T, pl->N, F, field_stride, out,
+                                                                         out_stride);
     ILH_HIP(hipGetLastError());
 }
 

@@ -18,7 +18,10 @@ from .core import (Context, DeviceArray, Nodes, Edges, ScalarData, VectorData, S
                    dirichlet_solve_batched, dirichlet_solve_batched_device, DirichletBatch,
                    DirichletHeatConduction, mask, complementary_mask, mask_, complementary_mask_,
                    create_surface_filter, filter_surface,
-                   dirichlet_solve_moving_batched)
+                   dirichlet_solve_moving_batched,
+                   EdgeGradient, SurfaceVectorCache, surface_divergence_symm, surface_grad_symm,
+                   regularize_normal_symm, normal_interpolate_symm, create_CL2invCT, create_CLinvCT_scalar,
+                   StokesFlow)
 from .geometry import GridSpec as PhysicalGrid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -14,7 +14,8 @@ OK = 0
 ERR_ARG, ERR_DIM, ERR_NONFINITE, ERR_SINGULAR, ERR_HIP, ERR_OOM, ERR_RANGE, ERR_STATE = -1, -2, -3, -4, -5, -6, -7, -8
 MEM_HOST, MEM_DEVICE = 0, 1
 NODES_PRIMAL, NODES_DUAL, EDGES_PRIMAL = 0, 1, 2
-SCHUR_RTLINVR, SCHUR_CLINVCT, SCHUR_GLINVD, SCHUR_NRTRN = 0, 1, 2, 3
+SCHUR_RTLINVR, SCHUR_CLINVCT, SCHUR_GLINVD, SCHUR_NRTRN, SCHUR_CL2INVCT = 0, 1, 2, 3, 4
+SCHUR_CLINVCT_SCALAR = SCHUR_CLINVCT
 DDF_YANG3, DDF_ROMA, DDF_M4PRIME, DDF_WITCHHAT = 0, 1, 2, 3
 
 
@@ -89,6 +90,12 @@ SIGNATURES = {
     "ilh_surface_curl_cross": ([_vp, _vp, _vp, _i], _i),
     "ilh_surface_curl_cross_adj": ([_vp, _vp, _vp, _i], _i),
     "ilh_surface_grad_cross": ([_vp, _vp, _vp, _i], _i),
+    "ilh_surface_curl_vector": ([_vp, _vp, _vp, _i], _i),
+    "ilh_surface_curl_vector_adj": ([_vp, _vp, _vp, _i], _i),
+    "ilh_surface_divergence_symm": ([_vp, _vp, _vp, _i], _i),
+    "ilh_surface_grad_symm": ([_vp, _vp, _vp, _i], _i),
+    "ilh_regularize_normal_symm": ([_vp, _vp, _vp, _i], _i),
+    "ilh_normal_interpolate_symm": ([_vp, _vp, _vp, _i], _i),
     "ilh_divergence": ([_vp, _vp, _vp, _i], _i),
     "ilh_grad": ([_vp, _vp, _vp, _i], _i),
     "ilh_curl_nodes_to_edges": ([_vp, _vp, _vp, _i], _i),
@@ -111,6 +118,9 @@ SIGNATURES = {
     "ilh_mask": ([_vp, _i, _i, _vp, _i], _i),
     "ilh_surface_filter_build": ([_vp, _vp, _i], _i),
     "ilh_surface_filter_apply": ([_vp, _i, _vp, _vp, _i], _i),
+    "ilh_surface_filter_build_vector": ([_vp, _vp, _i], _i),
+    "ilh_surface_filter_apply_vector": ([_vp, _i, _vp, _vp, _i], _i),
+    "ilh_stokes_solve": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _i], _i),
     "ilh_mask_multiply": ([_vp, _i, _i, _vp, _i], _i),
     "ilh_heat_create": ([_vp, C.c_double, C.c_double, _vp], _i),
     "ilh_heat_set_boundary": ([_vp, _vp, _vp, _i], _i),

@@ -183,6 +183,25 @@ class VectorData:
         return self.buf
 
 
+class EdgeGradient:
+    """EdgeGradient{Primal,Dual} tensor grid data: one buffer [dudx; dudy; dvdx; dvdy],
+    dudx and dvdy on Nodes{Primal} ((NX-1, NY-1)), dudy and dvdx on Nodes{Dual} ((NX, NY))."""
+
+    def __init__(self, g, data=None):
+        npr, ndu = (g.NX - 1) * (g.NY - 1), g.NX * g.NY
+        n = 2 * (npr + ndu)
+        self.buf = np.zeros(n) if data is None else np.ascontiguousarray(data, dtype=np.float64)
+        if self.buf.size != n:
+            raise L.DimensionMismatch("EdgeGradient of length %d, expected %d" % (self.buf.size, n))
+        cuts = np.cumsum([0, npr, ndu, ndu, npr])
+        shapes = [(g.NX - 1, g.NY - 1), (g.NX, g.NY), (g.NX, g.NY), (g.NX - 1, g.NY - 1)]
+        self.dudx, self.dudy, self.dvdx, self.dvdy = (
+            self.buf[cuts[k]:cuts[k + 1]].reshape(shapes[k], order="F") for k in range(4))
+
+    def flat(self):
+        return self.buf
+
+
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -260,6 +279,24 @@ class SurfaceScalarCache:
             pass
 
 
+class SurfaceVectorCache(SurfaceScalarCache):
+    """The reference's SurfaceVectorCache (src/cache.jl:184-201): the same plan,
+    with VectorData as its surface data, Edges{Primal} as its grid data and
+    EdgeGradient as its tensor grid data."""
+
+    def zeros_grid(self):
+        return Edges(self.g)
+
+    def zeros_gridgrad(self):
+        return EdgeGradient(self.g)
+
+    def zeros_surface(self):
+        return VectorData(self.N)
+
+    def zeros_surfacescalar(self):
+        return ScalarData(self.N)
+
+
 # -------------------------------------------------------------- dispatch
 def _check_len(a, n, what):
     if a.flat().size != n:
@@ -328,12 +365,44 @@ def surface_grad(vn, phi, cache):
 
 
 def surface_curl(out, inp, cache):
-    """w = C_s^T f (Nodes{Dual} from ScalarData) or vn = C_s s (ScalarData from Nodes{Dual})."""
+    """w = C_s^T f (Nodes{Dual} from ScalarData or VectorData) or C_s s (ScalarData
+    or VectorData from Nodes{Dual}) (src/surface_operators.jl:357-455)."""
+    if isinstance(out, Nodes) and out.kind == Dual and isinstance(inp, VectorData):
+        _check_len(inp, 2 * cache.N, "VectorData")
+        return _run("ilh_surface_curl_vector", cache, out, inp)
+    if isinstance(out, VectorData) and isinstance(inp, Nodes) and inp.kind == Dual:
+        _check_len(out, 2 * cache.N, "VectorData")
+        return _run("ilh_surface_curl_vector_adj", cache, out, inp)
     if isinstance(out, Nodes) and out.kind == Dual:
         return _run("ilh_surface_curl", cache, out, inp)
     if isinstance(out, ScalarData) and isinstance(inp, Nodes) and inp.kind == Dual:
         return _run("ilh_surface_curl_adj", cache, out, inp)
     raise TypeError("surface_curl: no method for (%s, %s)" % (type(out).__name__, type(inp).__name__))
+
+
+def surface_divergence_symm(q, dv, cache):
+    """q = D R_t (n o dv + dv o n - (n . dv) I) on Edges{Primal} (src/surface_operators.jl:574-580)."""
+    _check_len(dv, 2 * cache.N, "VectorData")
+    _check_len(q, cache.g.NX * (cache.g.NY - 1) + (cache.g.NX - 1) * cache.g.NY, "Edges{Primal}")
+    return _run("ilh_surface_divergence_symm", cache, q, dv)
+
+
+def surface_grad_symm(tau, v, cache):
+    """tau = n . R_t^T (G v + (G v)^T - tr(G v) I) (src/surface_operators.jl:641-644)."""
+    _check_len(tau, 2 * cache.N, "VectorData")
+    return _run("ilh_surface_grad_symm", cache, tau, v)
+
+
+def regularize_normal_symm(A, v, cache):
+    """A = R_t (n o v + v o n - (n . v) I) on EdgeGradient (src/surface_operators.jl:121)."""
+    _check_len(v, 2 * cache.N, "VectorData")
+    return _run("ilh_regularize_normal_symm", cache, A, v)
+
+
+def normal_interpolate_symm(tau, A, cache):
+    """tau = n . R_t^T (A + A^T - tr(A) I) (src/surface_operators.jl:245)."""
+    _check_len(tau, 2 * cache.N, "VectorData")
+    return _run("ilh_normal_interpolate_symm", cache, tau, A)
 
 
 def surface_curl_cross(out, inp, cache):
@@ -419,16 +488,25 @@ def complementary_mask_(w, cache):
 
 
 def create_surface_filter(cache):
-    """C = Et R (src/matrix_operators.jl:254-267) as an N x N array: the
-    interpolation divides the regularized field by R 1 first, so C keeps
-    constants and the surface integral."""
-    Cm = np.zeros(cache.N * cache.N)
-    L.call("ilh_surface_filter_build", cache.h, _ptr(Cm), L.MEM_HOST)
-    return Cm.reshape((cache.N, cache.N), order="F")
+    """C = Et R (src/matrix_operators.jl:254-267) as an N x N array (2N x 2N,
+    block diagonal, for a SurfaceVectorCache): the interpolation divides the
+    regularized field by R 1 first, so C keeps constants and the surface integral."""
+    n = 2 * cache.N if isinstance(cache, SurfaceVectorCache) else cache.N
+    Cm = np.zeros(n * n)
+    name = "ilh_surface_filter_build_vector" if isinstance(cache, SurfaceVectorCache) else "ilh_surface_filter_build"
+    L.call(name, cache.h, _ptr(Cm), L.MEM_HOST)
+    return Cm.reshape((n, n), order="F")
 
 
 def filter_surface(s, cache, power=1):
-    """C^power s without forming C (the C^5 s / C^6 s of test/literate/dirichlet.jl:122-124)."""
+    """C^power s without forming C (the C^5 s / C^6 s of test/literate/dirichlet.jl:122-124,
+    the C^2 sigma of test/literate/stokes.jl:162 for VectorData)."""
+    if isinstance(s, VectorData):
+        x = np.ascontiguousarray(s.flat(), dtype=np.float64)
+        _check_len(s, 2 * cache.N, "VectorData")
+        out = VectorData(cache.N)
+        L.call("ilh_surface_filter_apply_vector", cache.h, int(power), _ptr(x), _ptr(out.flat()), L.MEM_HOST)
+        return out
     x = np.ascontiguousarray(s.flat() if isinstance(s, ScalarData) else s, dtype=np.float64)
     out = ScalarData(cache.N)
     L.call("ilh_surface_filter_apply", cache.h, int(power), _ptr(x), _ptr(out.flat()), L.MEM_HOST)
@@ -436,9 +514,10 @@ def filter_surface(s, cache, power=1):
 
 
 def _schur(cache, kind, scale):
-    A = np.zeros(cache.N * cache.N)
+    n = 2 * cache.N if kind == L.SCHUR_CL2INVCT else cache.N
+    A = np.zeros(n * n)
     L.call("ilh_schur_build", cache.h, kind, C.c_double(scale), _ptr(A), L.MEM_HOST)
-    return A.reshape((cache.N, cache.N), order="F")
+    return A.reshape((n, n), order="F")
 
 
 def create_RTLinvR(cache, scale=1.0):
@@ -455,6 +534,16 @@ def create_GLinvD(cache, scale=1.0):
 
 def create_nRTRn(cache, scale=1.0):
     return _schur(cache, L.SCHUR_NRTRN, scale)
+
+
+def create_CL2invCT(cache, scale=1.0):
+    """-C_s L^-2 C_s^T on VectorData (src/matrix_operators.jl:95-125), 2N x 2N."""
+    return _schur(cache, L.SCHUR_CL2INVCT, scale)
+
+
+def create_CLinvCT_scalar(cache, scale=1.0):
+    """-C_s L^-1 C_s^T on the vector cache's ScalarData (src/matrix_operators.jl:71-91)."""
+    return _schur(cache, L.SCHUR_CLINVCT_SCALAR, scale)
 
 
 # -------------------------------------------------------------- problems
@@ -681,7 +770,29 @@ class SchurFactor:
         L.call("ilh_schur_factor", cache.h, kind)
 
     def solve(self, b):
-        b = _vec(b, self.cache.N)
+        b = _vec(b, 2 * self.cache.N if self.kind == L.SCHUR_CL2INVCT else self.cache.N)
         x = np.zeros_like(b)
         L.call("ilh_schur_solve", self.cache.h, self.kind, _ptr(x), _ptr(b), L.MEM_HOST)
         return x
+
+
+class StokesFlow:
+    """The Stokes flow problem of test/literate/stokes.jl:44-166 on a
+    SurfaceVectorCache: create_CL2invCT and create_CLinvCT_scalar are built and
+    inverted once per geometry (prob_cache); ``solve`` returns the velocity
+    (Edges{Primal}), the streamfunction (Nodes{Dual}) and the surface traction
+    (VectorData) filtered ``filter_power`` times (2, as the reference)."""
+
+    def __init__(self, cache):
+        if not isinstance(cache, SurfaceVectorCache):
+            raise TypeError("StokesFlow needs a SurfaceVectorCache")
+        self.cache = cache
+
+    def solve(self, vplus, vminus=None, filter_power=2):
+        c = self.cache
+        vp = _vec(vplus.flat() if isinstance(vplus, VectorData) else vplus, 2 * c.N)
+        vm = None if vminus is None else _vec(vminus.flat() if isinstance(vminus, VectorData) else vminus, 2 * c.N)
+        v, s, sigma = Edges(c.g), Nodes(Dual, c.g), VectorData(c.N)
+        L.call("ilh_stokes_solve", c.h, _ptr(vp), None if vm is None else _ptr(vm), _ptr(v.flat()), _ptr(s.flat()),
+               _ptr(sigma.flat()), int(filter_power), L.MEM_HOST)
+        return v, s, sigma

@@ -71,6 +71,29 @@ def circle(a, ds_target, cx=0.0, cy=0.0):
     return Surface(cx + a * np.cos(th), cy + a * np.sin(th), np.cos(th), np.sin(th), ds, np.zeros(N, int))
 
 
+def rectangle(a, b, ds_target, cx=0.0, cy=0.0):
+    """Rectangle of half-lengths a (x) and b (y), as RigidBodyTools'
+    ``Rectangle(a, b, Δs)`` polygon: each side split into round(length/Δs)
+    equal panels, one point at each panel midpoint with the side's outward
+    normal (214 points for Rectangle(0.5, 0.25, 1.4·0.01), the count printed
+    in examples/stokes.ipynb)."""
+    corners = [(a, -b), (a, b), (-a, b), (-a, -b)]
+    normals = [(1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0)]
+    X, Y, NX, NY, DS = [], [], [], [], []
+    for k in range(4):
+        (x0, y0), (x1, y1) = corners[k], corners[(k + 1) % 4]
+        length = np.hypot(x1 - x0, y1 - y0)
+        n = max(1, int(round(length / ds_target)))
+        t = (np.arange(n) + 0.5) / n
+        X.append(cx + x0 + t * (x1 - x0))
+        Y.append(cy + y0 + t * (y1 - y0))
+        NX.append(np.full(n, normals[k][0]))
+        NY.append(np.full(n, normals[k][1]))
+        DS.append(np.full(n, length / n))
+    cat = np.concatenate
+    return Surface(cat(X), cat(Y), cat(NX), cat(NY), cat(DS), np.zeros(sum(len(x) for x in X), int))
+
+
 def _sample_curve(rfun, drfun, N, cx, cy, fine=200000):
     """Equal-arclength sampling of a closed curve given as r(t) = (x(t), y(t))."""
     t = np.linspace(0, 2 * np.pi, fine + 1)
@@ -229,6 +252,19 @@ def config_headline(seed=5, spacing=1.2):
     s = concat([e, st])
     fb, _, _ = fourier_data(s.theta(), rng)
     return g, s, fb
+
+
+def config_stokes():
+    """The Stokes flow example of test/literate/stokes.jl:168-185: Δx = 0.01 on
+    [-2,2]² (NX = NY = 406 dual nodes, origin 203, the x_grid of -2.02..2.02
+    the reference's notebooks print for this grid), Rectangle(0.5, 0.25) at
+    Δs = 1.4Δx (214 points), translated to the right at unit velocity:
+    exterior surface velocity (1, 0), interior (0, 0)."""
+    dx = 0.01
+    g = GridSpec(406, 406, 203, 203, dx)
+    s = rectangle(0.5, 0.25, 1.4 * dx)
+    vplus = np.concatenate([np.ones(s.N), np.zeros(s.N)])
+    return g, s, vplus
 
 
 def square_grid(n_primal, L=4.0, margin=0):

@@ -58,7 +58,12 @@ enum { ILH_SCALING_GRID = 0 };
 /* grid data kinds */
 enum { ILH_NODES_PRIMAL = 0, ILH_NODES_DUAL = 1, ILH_EDGES_PRIMAL = 2 };
 /* Schur-complement kinds (src/matrix_operators.jl) */
-enum { ILH_SCHUR_RTLINVR = 0, ILH_SCHUR_CLINVCT = 1, ILH_SCHUR_GLINVD = 2, ILH_SCHUR_NRTRN = 3 };
+enum { ILH_SCHUR_RTLINVR = 0, ILH_SCHUR_CLINVCT = 1, ILH_SCHUR_GLINVD = 2, ILH_SCHUR_NRTRN = 3,
+       ILH_SCHUR_CL2INVCT = 4,
+       /* create_CLinvCT_scalar of the vector cache (src/matrix_operators.jl:71-91) is
+          the same matrix as create_CLinvCT of the scalar cache: its scalar curl
+          regularizes n f onto the edges exactly as the scalar cache does */
+       ILH_SCHUR_CLINVCT_SCALAR = ILH_SCHUR_CLINVCT };
 
 const char* ilh_last_error(void);
 const char* ilh_version(void);
@@ -121,6 +126,28 @@ int ilh_surface_curl_cross_adj(ilh_plan* plan, double* vn, const double* s, int 
 /* surface_grad_cross!(γ, ϕ::Nodes{Primal})  (src/surface_operators.jl:708-725) */
 int ilh_surface_grad_cross(ilh_plan* plan, double* vn, const double* phi, int mem);
 
+/* ---- vector cache (SurfaceVectorCache, src/cache.jl:184-201) ----
+ * The same plan serves the vector cache: VectorData v = [u; v] (2N), its
+ * grid data Edges{Primal}, its tensor data EdgeGradient{Primal,Dual} stored
+ * [dudx; dudy; dvdx; dvdy] with dudx, dvdy on Nodes{Primal} and dudy, dvdx
+ * on Nodes{Dual} ((NX-1)(NY-1), NX NY, NX NY, (NX-1)(NY-1) values).  The
+ * scalar-data operators of the vector cache (surface_curl! of ScalarData,
+ * surface_grad_cross!, ...) are the scalar cache's entry points above. */
+/* surface_curl!(w::Nodes{Dual}, v::VectorData) = C^T R v  (src/surface_operators.jl:388-400) */
+int ilh_surface_curl_vector(ilh_plan* plan, double* w, const double* v, int mem);
+/* surface_curl!(v::VectorData, s::Nodes{Dual}) = R^T C s  (src/surface_operators.jl:443-455) */
+int ilh_surface_curl_vector_adj(ilh_plan* plan, double* v, const double* s, int mem);
+/* surface_divergence_symm!(q::Edges{Primal}, dv::VectorData)
+   = D R_t (n o dv + dv o n - (n . dv) I)  (src/surface_operators.jl:565-591) */
+int ilh_surface_divergence_symm(ilh_plan* plan, double* q_edges, const double* dv, int mem);
+/* surface_grad_symm!(tau::VectorData, v::Edges{Primal})
+   = n . R_t^T (G v + (G v)^T - tr(G v) I)  (src/surface_operators.jl:632-664) */
+int ilh_surface_grad_symm(ilh_plan* plan, double* tau, const double* v_edges, int mem);
+/* regularize_normal_symm!(A::EdgeGradient, v::VectorData)  (src/surface_operators.jl:113-138) */
+int ilh_regularize_normal_symm(ilh_plan* plan, double* A, const double* v, int mem);
+/* normal_interpolate_symm!(tau::VectorData, A::EdgeGradient)  (src/surface_operators.jl:236-267) */
+int ilh_normal_interpolate_symm(ilh_plan* plan, double* tau, const double* A, int mem);
+
 /* ---- grid stencils (A8), GridScaling ---- */
 /* divergence!(p::Nodes{Primal}, q::Edges{Primal})  (src/grid_operators.jl:25-31) */
 int ilh_divergence(ilh_plan* plan, double* p, const double* q_edges, int mem);
@@ -138,7 +165,9 @@ int ilh_inverse_laplacian(ilh_plan* plan, int grid_kind, double* out, const doub
 /* ---- Schur complements (A10-A12) ---- */
 /* create_RTLinvR / create_CLinvCT / create_GLinvD / create_nRTRn with `scale`
    (src/matrix_operators.jl:9-30,40-61,135-155,225-244).  `out` is N x N
-   column-major. */
+   column-major.  ILH_SCHUR_CL2INVCT: create_CL2invCT of the vector cache,
+   -C_s L^-2 C_s^T on VectorData (src/matrix_operators.jl:95-125), 2N x 2N;
+   factor and solve take 2N-vectors for it. */
 int ilh_schur_build(ilh_plan* plan, int kind, double scale, double* out, int mem);
 /* Factor the plan's own S of `kind` (LU with partial pivoting, kept on the device).
    The reference re-factorises inside every `S\s` (test/literate/problems.jl:291). */
@@ -234,6 +263,23 @@ int ilh_mask_multiply(ilh_plan* plan, int grid_kind, int complementary, double* 
  * test/literate/dirichlet.jl:122-124, problems.jl:298) without forming C. */
 int ilh_surface_filter_build(ilh_plan* plan, double* C, int mem);
 int ilh_surface_filter_apply(ilh_plan* plan, int power, const double* s_in, double* s_out, int mem);
+/* create_surface_filter of the vector cache: the same construction per
+ * component on its edges (block diagonal, 2N x 2N column-major), and its
+ * application C^power to VectorData. */
+int ilh_surface_filter_build_vector(ilh_plan* plan, double* C, int mem);
+int ilh_surface_filter_apply_vector(ilh_plan* plan, int power, const double* v_in, double* v_out, int mem);
+
+/* ---- Stokes flow (vector cache) ----
+ * The Stokes problem of test/literate/stokes.jl:101-166 (saddle system
+ * :29-35): v_plus / v_minus = exterior / interior surface velocities
+ * (VectorData, 2N; v_minus may be NULL for zero), outputs the velocity v
+ * (Edges{Primal}), the streamfunction s (Nodes{Dual}) and the surface
+ * traction sigma (VectorData, 2N) filtered filter_power times (the
+ * reference returns C^2 sigma: filter_power = 2).  The first call on a
+ * geometry builds and inverts create_CL2invCT and create_CLinvCT_scalar
+ * (prob_cache, :76-96). */
+int ilh_stokes_solve(ilh_plan* plan, const double* v_plus, const double* v_minus, double* v, double* s,
+                     double* sigma, int filter_power, int mem);
 
 /* ---- unbounded heat conduction with Dirichlet bodies (config: IF time marching) ----
  * The constrained ODE system of test/literate/heatconduction.jl:88-180,

@@ -24,6 +24,10 @@
  *   Schur builders       src/matrix_operators.jl:9-61,135-155,225-244 (column loops).
  *   composites           test/literate/problems.jl:259-298 (Dirichlet) and
  *                        test/literate/neumann.jl:101-142 (Neumann).
+ *   vector cache         src/surface_operators.jl:113-138,236-267,388-455,
+ *                        565-591,632-664; create_CL2invCT
+ *                        src/matrix_operators.jl:95-125; Stokes flow
+ *                        test/literate/stokes.jl:101-166.
  */
 #include "ilh_oracle.h"
 #include <math.h>
@@ -918,6 +922,252 @@ int orc_neumann_solve(const orc_grid* g, int ddf, int N, const double* X, const 
     return info;
 }
 
+
+/* ------------------------------------------- vector cache (Stokes flow) */
+/* EdgeGradient{Primal,Dual} = [dudx; dudy; dvdx; dvdy]: dudx and dvdy on
+   Nodes{Primal}, dudy and dvdx on Nodes{Dual} (where the MAC differences of
+   the edge components land: the x difference of u and the y difference of v
+   at the primal nodes, the cross differences at the dual nodes). */
+static size_t prim_size(const orc_grid* g) { return (size_t)(g->NX - 1) * (g->NY - 1); }
+static size_t dual_size(const orc_grid* g) { return (size_t)g->NX * g->NY; }
+size_t orc_tensor_size(const orc_grid* g) { return 2 * (prim_size(g) + dual_size(g)); }
+
+static void tensor_parts(const orc_grid* g, double* A, double** xx, double** xy, double** yx, double** yy)
+{
+    *xx = A;
+    *xy = A + prim_size(g);
+    *yx = *xy + dual_size(g);
+    *yy = *yx + dual_size(g);
+}
+
+/* grad!(A::EdgeGradient, q::Edges{Primal}), unscaled; the ring of the dual
+   components that has no edge pair around it stays 0 */
+static void grad_tensor_unscaled(const orc_grid* g, const double* q, double* A)
+{
+    const int NX = g->NX, NY = g->NY;
+    const double* u = q; const double* v = q + edge_u_size(g);
+    double *xx, *xy, *yx, *yy;
+    tensor_parts(g, A, &xx, &xy, &yx, &yy);
+    memset(A, 0, sizeof(double) * orc_tensor_size(g));
+    for (int j = 0; j < NY - 1; ++j)
+        for (int i = 0; i < NX - 1; ++i) {
+            xx[i + (size_t)(NX - 1) * j] = u[i + 1 + (size_t)NX * j] - u[i + (size_t)NX * j];
+            yy[i + (size_t)(NX - 1) * j] = v[i + (size_t)(NX - 1) * (j + 1)] - v[i + (size_t)(NX - 1) * j];
+        }
+    for (int j = 1; j < NY - 1; ++j)
+        for (int i = 0; i < NX; ++i)
+            xy[i + (size_t)NX * j] = u[i + (size_t)NX * j] - u[i + (size_t)NX * (j - 1)];
+    for (int j = 0; j < NY; ++j)
+        for (int i = 1; i < NX - 1; ++i)
+            yx[i + (size_t)NX * j] = v[i + (size_t)(NX - 1) * j] - v[i - 1 + (size_t)(NX - 1) * j];
+}
+
+/* divergence!(q::Edges{Primal}, A::EdgeGradient), unscaled, by rows:
+   q_u = d/dx A_xx + d/dy A_xy, q_v = d/dx A_yx + d/dy A_yy.  Applied here
+   only to symmetric tensors, for which rows and columns give the same. */
+static void div_tensor_unscaled(const orc_grid* g, double* A, double* q)
+{
+    const int NX = g->NX, NY = g->NY;
+    double* u = q; double* v = q + edge_u_size(g);
+    double *xx, *xy, *yx, *yy;
+    tensor_parts(g, A, &xx, &xy, &yx, &yy);
+    memset(q, 0, sizeof(double) * edge_size(g));
+    for (int j = 0; j < NY - 1; ++j)
+        for (int i = 1; i < NX - 1; ++i)
+            u[i + (size_t)NX * j] = (xx[i + (size_t)(NX - 1) * j] - xx[i - 1 + (size_t)(NX - 1) * j])
+                                  + (xy[i + (size_t)NX * (j + 1)] - xy[i + (size_t)NX * j]);
+    for (int j = 1; j < NY - 1; ++j)
+        for (int i = 0; i < NX - 1; ++i)
+            v[i + (size_t)(NX - 1) * j] = (yx[i + 1 + (size_t)NX * j] - yx[i + (size_t)NX * j])
+                                        + (yy[i + (size_t)(NX - 1) * j] - yy[i + (size_t)(NX - 1) * (j - 1)]);
+}
+
+/* A = R_t (n o f + f o n - (n . f) I)   (surface_operators.jl:121-138), in the
+   reference's order: tensor product, plus its transpose, minus n . f on the
+   diagonal, then one regularization per component */
+void orc_regularize_normal_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                const double* nx, const double* ny, const double* ds, const double* f, double* A)
+{
+    double* t = dalloc(4 * (size_t)N);
+    double *txx = t, *txy = t + N, *tyx = t + 2 * N, *tyy = t + 3 * N;
+    for (int p = 0; p < N; ++p) {
+        const double fu = f[p], fv = f[N + p];
+        const double nf = nx[p] * fu + ny[p] * fv;
+        txx[p] = (nx[p] * fu + nx[p] * fu) - nf;
+        txy[p] = nx[p] * fv + ny[p] * fu;
+        tyx[p] = ny[p] * fu + nx[p] * fv;
+        tyy[p] = (ny[p] * fv + ny[p] * fv) - nf;
+    }
+    double *xx, *xy, *yx, *yy;
+    tensor_parts(g, A, &xx, &xy, &yx, &yy);
+    orc_regularize(g, ORC_PRIMAL, ddf, N, X, Y, ds, txx, xx);
+    orc_regularize(g, ORC_DUAL, ddf, N, X, Y, ds, txy, xy);
+    orc_regularize(g, ORC_DUAL, ddf, N, X, Y, ds, tyx, yx);
+    orc_regularize(g, ORC_PRIMAL, ddf, N, X, Y, ds, tyy, yy);
+    free(t);
+}
+
+/* tau = n . R_t^T (A + A^T - tr(A) I)   (surface_operators.jl:245-267) */
+void orc_normal_interpolate_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                 const double* nx, const double* ny, const double* A, double* tau)
+{
+    const size_t np = prim_size(g), nd = dual_size(g);
+    double* B = dalloc(orc_tensor_size(g));
+    double *bxx, *bxy, *byx, *byy;
+    tensor_parts(g, B, &bxx, &bxy, &byx, &byy);
+    const double* axx = A; const double* axy = A + np; const double* ayx = axy + nd; const double* ayy = ayx + nd;
+    for (size_t k = 0; k < np; ++k) {
+        const double tr = axx[k] + ayy[k];
+        bxx[k] = (axx[k] + axx[k]) - tr;
+        byy[k] = (ayy[k] + ayy[k]) - tr;
+    }
+    for (size_t k = 0; k < nd; ++k) {
+        bxy[k] = ayx[k] + axy[k];
+        byx[k] = axy[k] + ayx[k];
+    }
+    double* t = dalloc(4 * (size_t)N);
+    orc_interpolate(g, ORC_PRIMAL, ddf, N, X, Y, bxx, t);
+    orc_interpolate(g, ORC_DUAL, ddf, N, X, Y, bxy, t + N);
+    orc_interpolate(g, ORC_DUAL, ddf, N, X, Y, byx, t + 2 * N);
+    orc_interpolate(g, ORC_PRIMAL, ddf, N, X, Y, byy, t + 3 * N);
+    for (int p = 0; p < N; ++p) {
+        tau[p] = nx[p] * t[p] + ny[p] * t[2 * N + p];
+        tau[N + p] = nx[p] * t[N + p] + ny[p] * t[3 * N + p];
+    }
+    free(t); free(B);
+}
+
+/* q = D R_t (n o f + f o n - (n . f) I) / dx   (surface_operators.jl:574-591) */
+void orc_surface_divergence_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                 const double* nx, const double* ny, const double* ds, const double* f, double* q)
+{
+    double* A = dalloc(orc_tensor_size(g));
+    orc_regularize_normal_symm(g, ddf, N, X, Y, nx, ny, ds, f, A);
+    div_tensor_unscaled(g, A, q);
+    div_arr(q, edge_size(g), g->dx);
+    free(A);
+}
+
+/* tau = n . R_t^T (G v + (G v)^T - tr(G v) I) / dx   (surface_operators.jl:632-664) */
+void orc_surface_grad_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                           const double* nx, const double* ny, const double* q, double* tau)
+{
+    double* A = dalloc(orc_tensor_size(g));
+    grad_tensor_unscaled(g, q, A);
+    orc_normal_interpolate_symm(g, ddf, N, X, Y, nx, ny, A, tau);
+    div_arr(tau, 2 * (size_t)N, g->dx);
+    free(A);
+}
+
+/* w = C^T R f / dx for VectorData f   (surface_operators.jl:388-400) */
+void orc_surface_curl_vector(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                             const double* ds, const double* f, double* w)
+{
+    double* q = dalloc(edge_size(g));
+    reg_edges(g, ddf, N, X, Y, ds, f, f + N, q);
+    curl_e2n_unscaled(g, q, w);
+    div_arr(w, dual_size(g), g->dx);
+    free(q);
+}
+
+/* v = R^T C s / dx   (surface_operators.jl:443-455) */
+void orc_surface_curl_vector_adj(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                 const double* s, double* v)
+{
+    double* q = dalloc(edge_size(g));
+    curl_n2e_unscaled(g, s, q);
+    interp_edges(g, ddf, N, X, Y, q, v, v + N);
+    div_arr(v, 2 * (size_t)N, g->dx);
+    free(q);
+}
+
+/* create_CL2invCT (matrix_operators.jl:95-125): 2N x 2N column-major, one
+   unit vector per column, the inverse Laplacian applied twice */
+void orc_cl2invct(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
+                  double scale, double* A)
+{
+    const int n2 = 2 * N;
+    orc_lap* L = orc_lap_create(g, ORC_DUAL);
+    double* e = dalloc(n2);
+    double* col = dalloc(n2);
+    double* w = dalloc(dual_size(g));
+    for (int c = 0; c < n2; ++c) {
+        memset(e, 0, sizeof(double) * n2);
+        e[c] = 1.0;
+        orc_surface_curl_vector(g, ddf, N, X, Y, ds, e, w);
+        orc_inverse_laplacian(L, w, w);
+        orc_inverse_laplacian(L, w, w);
+        orc_surface_curl_vector_adj(g, ddf, N, X, Y, w, col);
+        for (int r = 0; r < n2; ++r) A[r + (size_t)n2 * c] = -scale * col[r];
+    }
+    free(e); free(col); free(w);
+    orc_lap_destroy(L);
+}
+
+/* test/literate/stokes.jl:101-166 in the reference's order, traction
+   unfiltered (the reference's last step, sigma = C^2 sigma, is left to the
+   caller).  S2 (2N x 2N) and Ss (N x N) column-major, or NULL to build them
+   here; each solve re-factorises them, as S \ vprime does. */
+int orc_stokes_solve(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                     const double* nx, const double* ny, const double* ds, const double* S2, const double* Ss,
+                     const double* vplus, const double* vminus, double* v, double* s, double* sigma)
+{
+    const int n2 = 2 * N;
+    const size_t ne = edge_size(g), nd = dual_size(g), np = prim_size(g);
+    double *S2own = NULL, *Ssown = NULL;
+    if (!S2) {
+        S2own = dalloc((size_t)n2 * n2);
+        orc_cl2invct(g, ddf, N, X, Y, ds, 1.0, S2own);
+        S2 = S2own;
+    }
+    if (!Ss) {
+        Ssown = dalloc((size_t)N * N);
+        orc_schur_columns(g, ddf, N, X, Y, nx, ny, ds, ORC_S_CLINVCT, 1.0, Ssown);
+        Ss = Ssown;
+    }
+    orc_lap* Lp = cached_lap(g, ORC_PRIMAL);
+    orc_lap* Ld = cached_lap(g, ORC_DUAL);
+    double* dv = dalloc(n2); double* vb = dalloc(n2); double* vprime = dalloc(n2); double* dvn = dalloc(N);
+    double* sstar = dalloc(nd); double* phi = dalloc(np); double* vphi = dalloc(ne); double* dsv = dalloc(N);
+    for (int k = 0; k < n2; ++k) {                       /* prescribed_surface_jump! / average! */
+        const double vm = vminus ? vminus[k] : 0.0;
+        dv[k] = vplus[k] - vm;
+        vb[k] = 0.5 * vplus[k];
+        vb[k] += 0.5 * vm;
+    }
+    orc_surface_divergence_symm(g, ddf, N, X, Y, nx, ny, ds, dv, v);      /* psi* */
+    orc_curl_e2n(g, v, sstar);
+    for (size_t k = 0; k < nd; ++k) sstar[k] *= -1.0;
+    orc_inverse_laplacian(Ld, sstar, sstar);
+    orc_inverse_laplacian(Ld, sstar, sstar);
+    for (int p = 0; p < N; ++p) dvn[p] = nx[p] * dv[p] + ny[p] * dv[N + p];   /* normal velocity jump */
+    orc_regularize(g, ORC_PRIMAL, ddf, N, X, Y, ds, dvn, phi);
+    orc_inverse_laplacian(Lp, phi, phi);
+    orc_grad(g, phi, vphi);
+    interp_edges(g, ddf, N, X, Y, vphi, vprime, vprime + N);
+    for (int k = 0; k < n2; ++k) vprime[k] = vb[k] - vprime[k];
+    orc_curl_n2e(g, sstar, v);
+    interp_edges(g, ddf, N, X, Y, v, vb, vb + N);
+    for (int k = 0; k < n2; ++k) vprime[k] -= vb[k];                       /* spurious slip */
+    memcpy(sigma, vprime, sizeof(double) * n2);
+    int info = dense_solve(n2, S2, sigma);
+    orc_surface_curl_vector(g, ddf, N, X, Y, ds, sigma, s);                /* correction streamfunction */
+    for (size_t k = 0; k < nd; ++k) s[k] *= -1.0;
+    orc_inverse_laplacian(Ld, s, s);
+    orc_inverse_laplacian(Ld, s, s);
+    for (size_t k = 0; k < nd; ++k) s[k] += sstar[k];
+    orc_curl_n2e(g, s, v);
+    for (size_t k = 0; k < ne; ++k) v[k] += vphi[k];
+    orc_surface_grad_cross(g, ddf, N, X, Y, nx, ny, phi, dsv);            /* potential's streamfunction */
+    info |= dense_solve(N, Ss, dsv);
+    orc_surface_curl_cross(g, ddf, N, X, Y, nx, ny, ds, dsv, sstar);
+    for (size_t k = 0; k < nd; ++k) sstar[k] *= -1.0;
+    orc_inverse_laplacian(Ld, sstar, sstar);
+    for (size_t k = 0; k < nd; ++k) s[k] += sstar[k];
+    free(dv); free(vb); free(vprime); free(dvn); free(sstar); free(phi); free(vphi); free(dsv);
+    free(S2own); free(Ssown);
+    return info;
+}
 
 /* ------------------------------------------------------------------ heat */
 /* K(p) = e^{-x} I_p(x), x = 2a, from the power series

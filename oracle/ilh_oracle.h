@@ -19,6 +19,7 @@
  */
 #ifndef ILH_ORACLE_H
 #define ILH_ORACLE_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -87,6 +88,27 @@ void orc_schur_columns(const orc_grid* g, int ddf, int N, const double* X, const
                        const double* nx, const double* ny, const double* ds, int kind, double scale, double* A);
 void orc_schur_direct(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
                       const double* nx, const double* ny, const double* ds, int kind, double scale, double* A);
+
+/* vector cache: VectorData [u; v] (2N); EdgeGradient [dudx; dudy; dvdx; dvdy]
+   with dudx, dvdy on Nodes{Primal} and dudy, dvdx on Nodes{Dual} */
+size_t orc_tensor_size(const orc_grid* g);
+void orc_regularize_normal_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                const double* nx, const double* ny, const double* ds, const double* f, double* A);
+void orc_normal_interpolate_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                 const double* nx, const double* ny, const double* A, double* tau);
+void orc_surface_divergence_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                 const double* nx, const double* ny, const double* ds, const double* f, double* q);
+void orc_surface_grad_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                           const double* nx, const double* ny, const double* q, double* tau);
+void orc_surface_curl_vector(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                             const double* ds, const double* f, double* w);
+void orc_surface_curl_vector_adj(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                                 const double* s, double* v);
+void orc_cl2invct(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
+                  double scale, double* A);
+int orc_stokes_solve(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
+                     const double* nx, const double* ny, const double* ds, const double* S2, const double* Ss,
+                     const double* vplus, const double* vminus, double* v, double* s, double* sigma);
 
 /* dense LU with partial pivoting (A12) */
 /* unbounded heat conduction with Dirichlet bodies, IF-HERK (the constrained

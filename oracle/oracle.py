@@ -388,3 +388,95 @@ def surface_filter(g, geo, ddf_kind=YANG3):
     w[r1 != 0] = 1.0 / r1[r1 != 0]
     E = (R * (g.dx ** 2 / geo.ds)[None, :]).T
     return E @ (w[:, None] * R)
+
+
+# ------------------------------------------------------------ vector cache
+def tensor_len(g):
+    """EdgeGradient{Primal,Dual} length: [dudx; dudy; dvdx; dvdy]."""
+    return 2 * (nel(g, PRIMAL) + nel(g, DUAL))
+
+
+def regularize_normal_symm(g, geo, f, ddf_kind=YANG3):
+    out = np.zeros(tensor_len(g))
+    lib().orc_regularize_normal_symm(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny), _p(geo.ds),
+                                     _p(np.ascontiguousarray(f, float)), _p(out))
+    return out
+
+
+def normal_interpolate_symm(g, geo, A, ddf_kind=YANG3):
+    out = np.zeros(2 * geo.N)
+    lib().orc_normal_interpolate_symm(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny),
+                                      _p(np.ascontiguousarray(A, float)), _p(out))
+    return out
+
+
+def surface_divergence_symm(g, geo, f, ddf_kind=YANG3):
+    out = np.zeros(edge_len(g))
+    lib().orc_surface_divergence_symm(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny), _p(geo.ds),
+                                      _p(np.ascontiguousarray(f, float)), _p(out))
+    return out
+
+
+def surface_grad_symm(g, geo, q, ddf_kind=YANG3):
+    out = np.zeros(2 * geo.N)
+    lib().orc_surface_grad_symm(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny),
+                                _p(np.ascontiguousarray(q, float)), _p(out))
+    return out
+
+
+def surface_curl_vector(g, geo, f, ddf_kind=YANG3):
+    out = np.zeros(nel(g, DUAL))
+    lib().orc_surface_curl_vector(C.byref(g), ddf_kind, *geo.args(), _p(geo.ds),
+                                  _p(np.ascontiguousarray(f, float)), _p(out))
+    return out
+
+
+def surface_curl_vector_adj(g, geo, s, ddf_kind=YANG3):
+    out = np.zeros(2 * geo.N)
+    lib().orc_surface_curl_vector_adj(C.byref(g), ddf_kind, *geo.args(), _p(np.ascontiguousarray(s, float)),
+                                      _p(out))
+    return out
+
+
+def cl2invct(g, geo, scale=1.0, ddf_kind=YANG3):
+    """create_CL2invCT (src/matrix_operators.jl:95-125): 2N x 2N [row, col]."""
+    n = 2 * geo.N
+    A = np.zeros(n * n)
+    lib().orc_cl2invct(C.byref(g), ddf_kind, *geo.args(), _p(geo.ds), C.c_double(scale), _p(A))
+    return A.reshape(n, n).T.copy()
+
+
+def stokes_solve(g, geo, vplus, vminus=None, S2=None, Ss=None, ddf_kind=YANG3):
+    """(v, s, sigma) of test/literate/stokes.jl:101-166 with sigma unfiltered."""
+    v = np.zeros(edge_len(g))
+    s = np.zeros(nel(g, DUAL))
+    sig = np.zeros(2 * geo.N)
+    S2f = None if S2 is None else np.asfortranarray(S2).ravel(order="F").copy()
+    Ssf = None if Ss is None else np.asfortranarray(Ss).ravel(order="F").copy()
+    vm = None if vminus is None else np.ascontiguousarray(vminus, float)
+    info = lib().orc_stokes_solve(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny), _p(geo.ds),
+                                  _p(S2f), _p(Ssf), _p(np.ascontiguousarray(vplus, float)), _p(vm),
+                                  _p(v), _p(s), _p(sig))
+    if info:
+        raise np.linalg.LinAlgError("singular Stokes Schur complement (info=%d)" % info)
+    return v, s, sig
+
+
+def surface_filter_vector(g, geo, ddf_kind=YANG3):
+    """The vector cache's surface filter: surface_filter's construction on each
+    edge component (u from the u edges, v from the v edges), block diagonal
+    2N x 2N [row, col]."""
+    N = geo.N
+    Cm = np.zeros((2 * N, 2 * N))
+    for c, kind in enumerate((EDGE_U, EDGE_V)):
+        R = np.zeros((nel(g, kind), N))
+        for q in range(N):
+            e = np.zeros(N)
+            e[q] = 1.0
+            R[:, q] = regularize(g, kind, geo, e, ddf_kind)
+        r1 = R.sum(axis=1)
+        w = np.zeros_like(r1)
+        w[r1 != 0] = 1.0 / r1[r1 != 0]
+        E = (R * (g.dx ** 2 / geo.ds)[None, :]).T
+        Cm[c * N:(c + 1) * N, c * N:(c + 1) * N] = E @ (w[:, None] * R)
+    return Cm
