@@ -1236,6 +1236,7 @@ const double* filter_r1e(ilh_plan* pl, DevBuf& ones)
         ones.ensure(sizeof(double) * n2);
         k_fill_value<<<(n2 + 255) / 256, 256, 0, pl->ctx->stream>>>(ones.as<double>(), (size_t)2 * pl->N, 1.0);
         scatter_op(pl, OP_R_EDGES, ones.as<double>(), pl->filt_r1e.as<double>());
+        ILH_HIP(hipStreamSynchronize(pl->ctx->stream));   // `ones` is released by the caller
         pl->filt_gen_e = pl->gen;
     }
     return pl->filt_r1e.as<double>();
@@ -1255,7 +1256,6 @@ void filter_vector(ilh_plan* pl, int power, double* v, double* g)
         gather_op(pl, OP_E_EDGES, g, v);
     }
     ILH_HIP(hipGetLastError());
-    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));   // `ones` is released on return
 }
 }  // namespace
 

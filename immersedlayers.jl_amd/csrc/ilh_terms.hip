@@ -690,35 +690,47 @@ struct GatherFields {
     int nx[GK_COUNT], ny[GK_COUNT];
 };
 
-__global__ void k_gather(const Term* __restrict__ t, int T, int N, GatherFields F, size_t field_stride,
-                         double* __restrict__ out, size_t out_stride)
+// One wave per point: lane l takes the (term, row, column) triples l, l + 64,
+// ... of the point's T terms (T x 5 x 5), each the field value times coef wy
+// wx, into the partial sum of its term's amplitude slot; the wave then sums
+// the two slots' partials in a fixed butterfly order (bitwise reproducible).
+__global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int T, int N, GatherFields F,
+                                                size_t field_stride, double* __restrict__ out, size_t out_stride)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N) return;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= N) return;   // uniform over the wave
     out += out_stride * blockIdx.y;
-    double acc0 = 0.0, acc1 = 0.0;
-    int ai0 = -1, ai1 = -1;
-    for (int k = 0; k < T; ++k) {
-        const Term& tt = t[p * T + k];
-        const double* fld = F.f[tt.gk] + field_stride * blockIdx.y;
-        const int nx = F.nx[tt.gk], ny = F.ny[tt.gk];
-        double s = 0.0;
-        for (int b = 0; b < 5; ++b) {
-            const int row = tt.by + b;
-            if (row < 0 || row >= ny || tt.wy[b] == 0.0) continue;
-            double r = 0.0;
-            for (int a = 0; a < 5; ++a) {
-                const int col = tt.bx + a;
-                if (col >= 0 && col < nx) r += tt.wx[a] * fld[col + (size_t)nx * row];
-            }
-            s += tt.wy[b] * r;
+    const size_t foff = field_stride * blockIdx.y;
+    const Term* tp = t + (size_t)p * T;
+    const int ai0 = tp[0].ai;
+    int ai1 = -1;
+    for (int k = 1; k < T; ++k)
+        if (tp[k].ai != ai0) {
+            ai1 = tp[k].ai;
+            break;
         }
-        s *= tt.coef;
-        if (ai0 < 0 || tt.ai == ai0) { ai0 = tt.ai; acc0 += s; }
-        else { ai1 = tt.ai; acc1 += s; }
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int e = lane; e < T * 25; e += 64) {
+        const int k = e / 25, r = e - 25 * k, b = r / 5, a = r - 5 * b;
+        const Term& tt = tp[k];
+        const int gk = tt.gk, row = tt.by + b, col = tt.bx + a;
+        const int nx = F.nx[gk], ny = F.ny[gk];
+        double v = 0.0;
+        const double w = tt.wy[b] * tt.wx[a];
+        if (w != 0.0 && row >= 0 && row < ny && col >= 0 && col < nx)
+            v = tt.coef * w * F.f[gk][foff + col + (size_t)nx * row];
+        if (tt.ai == ai0) acc0 += v;
+        else acc1 += v;
     }
-    out[ai0] = acc0;
-    if (ai1 >= 0) out[ai1] = acc1;
+    for (int off = 32; off > 0; off >>= 1) {
+        acc0 += __shfl_xor(acc0, off, 64);
+        acc1 += __shfl_xor(acc1, off, 64);
+    }
+    if (lane == 0) {
+        out[ai0] = acc0;
+        if (ai1 >= 0) out[ai1] = acc1;
+    }
 }
 
 void gather(ilh_plan* pl, const Term* terms, int T, const double* field, size_t field_stride, int nb, double* out,
@@ -732,8 +744,8 @@ void gather(ilh_plan* pl, const Term* terms, int T, const double* field, size_t 
         F.nx[k] = g.nx(k);
         F.ny[k] = g.ny(k);
     }
-    k_gather<<<dim3((pl->N + 127) / 128, nb), 128, 0, pl->ctx->stream>>>(terms, T, pl->N, F, field_stride, out,
-                                                                         out_stride);
+    k_gather<<<dim3((pl->N + 3) / 4, nb), 256, 0, pl->ctx->stream>>>(terms, T, pl->N, F, field_stride, out,
+                                                                     out_stride);
     ILH_HIP(hipGetLastError());
 }
 
