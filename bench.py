@@ -94,44 +94,54 @@ def bench_ops(args):
     ctx = ilhip.Context(0)
     cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
     rng = np.random.default_rng(1)
-    ng, nd, ne, N, b = (g.NX - 1) * (g.NY - 1), g.NX * g.NY, g.NX * (g.NY - 1) + (g.NX - 1) * g.NY, s.N, 8
-    dev = {k: ilhip.DeviceArray.from_host(ctx, rng.standard_normal(n))
-           for k, n in (("p", ng), ("p2", ng), ("d", nd), ("d2", nd), ("e", ne), ("e2", ne), ("f", N))}
+    ng, nd, ne, N = (g.NX - 1) * (g.NY - 1), g.NX * g.NY, g.NX * (g.NY - 1) + (g.NX - 1) * g.NY, s.N
     P = 4096
-    ops = [
-        ("laplacian (Nodes{Primal})", "ilh_laplacian", (L.NODES_PRIMAL, "p2", "p"), 2 * b * ng),
-        ("divergence (Edges -> Nodes{Primal})", "ilh_divergence", ("p", "e"), b * (ne + ng)),
-        ("grad (Nodes{Primal} -> Edges)", "ilh_grad", ("e2", "p"), b * (ne + ng)),
-        ("curl (Nodes{Dual} -> Edges)", "ilh_curl_nodes_to_edges", ("e2", "d"), b * (ne + nd)),
-        ("curl (Edges -> Nodes{Dual})", "ilh_curl_edges_to_nodes", ("d2", "e"), b * (ne + nd)),
-        ("regularize Rc (ScalarData -> Nodes{Primal})", "ilh_regularize", (L.NODES_PRIMAL, "p2", "f"), b * ng + 32 * N),
-        ("interpolate Rc^T (Nodes{Primal} -> ScalarData)", "ilh_interpolate", (L.NODES_PRIMAL, "f", "p"),
-         N * (16 * b + b + 16)),
-        ("inverse_laplacian (Nodes{Primal}, LGF period 4096^2)", "ilh_inverse_laplacian", (L.NODES_PRIMAL, "p2", "p"),
-         2 * b * ng + b * P * (P // 2 + 1)),
-    ]
-    res = {}
-    for name, fn, a, nbytes in ops:
-        ptrs = [dev[x].ptr if isinstance(x, str) else x for x in a]
-        for _ in range(max(1, args.warmup // 10)):
-            L.call(fn, cache.h, *ptrs, L.MEM_DEVICE)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            L.call(fn, cache.h, *ptrs, L.MEM_DEVICE)
-        ctx.synchronize()
-        us = 1e6 * (time.perf_counter() - t0) / args.steps
-        res[name] = {"us_per_call": us, "algorithmic_bytes": nbytes, "GB_s": nbytes / us * 1e-3,
-                     "frac_hbm": nbytes / us * 1e-3 / HBM_PEAK_GBS}
-        if fn == "ilh_interpolate":
-            res[name]["points_per_s"] = N / (us * 1e-6)
-        log("%-55s %9.2f us  %8.1f GB/s" % (name, us, nbytes / us * 1e-3))
+    results = {}
+    # FP64 (the reference's default) and FP32 (a Float32 cache: b = 4 bytes per value; the
+    # LGF spectrum stays FP64 and the weights of the 16-point stencils FP64)
+    for dt, b, sfx in ((np.float64, 8, ""), (np.float32, 4, "_f32")):
+        dev = {k: ilhip.DeviceArray.from_host(ctx, rng.standard_normal(n), dtype=dt)
+               for k, n in (("p", ng), ("p2", ng), ("d", nd), ("d2", nd), ("e", ne), ("e2", ne), ("f", N))}
+        ops = [
+            ("laplacian (Nodes{Primal})", "ilh_laplacian", (L.NODES_PRIMAL, "p2", "p"), 2 * b * ng),
+            ("divergence (Edges -> Nodes{Primal})", "ilh_divergence", ("p", "e"), b * (ne + ng)),
+            ("grad (Nodes{Primal} -> Edges)", "ilh_grad", ("e2", "p"), b * (ne + ng)),
+            ("curl (Nodes{Dual} -> Edges)", "ilh_curl_nodes_to_edges", ("e2", "d"), b * (ne + nd)),
+            ("curl (Edges -> Nodes{Dual})", "ilh_curl_edges_to_nodes", ("d2", "e"), b * (ne + nd)),
+            ("regularize Rc (ScalarData -> Nodes{Primal})", "ilh_regularize", (L.NODES_PRIMAL, "p2", "f"),
+             b * ng + 32 * N),
+            ("interpolate Rc^T (Nodes{Primal} -> ScalarData)", "ilh_interpolate", (L.NODES_PRIMAL, "f", "p"),
+             N * (16 * b + b + 16)),
+            ("inverse_laplacian (Nodes{Primal}, LGF period 4096^2)", "ilh_inverse_laplacian",
+             (L.NODES_PRIMAL, "p2", "p"), 2 * b * ng + 8 * P * (P // 2 + 1)),
+        ]
+        res = {}
+        for name, fn, a, nbytes in ops:
+            fn += sfx
+            ptrs = [dev[x].ptr if isinstance(x, str) else x for x in a]
+            for _ in range(max(1, args.warmup // 10)):
+                L.call(fn, cache.h, *ptrs, L.MEM_DEVICE)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                L.call(fn, cache.h, *ptrs, L.MEM_DEVICE)
+            ctx.synchronize()
+            us = 1e6 * (time.perf_counter() - t0) / args.steps
+            res[name] = {"us_per_call": us, "algorithmic_bytes": nbytes, "GB_s": nbytes / us * 1e-3,
+                         "frac_hbm": nbytes / us * 1e-3 / HBM_PEAK_GBS}
+            if fn.startswith("ilh_interpolate"):
+                res[name]["points_per_s"] = N / (us * 1e-6)
+            log("%-55s %-4s %9.2f us  %8.1f GB/s" % (name, "f32" if sfx else "f64", us, nbytes / us * 1e-3))
+        results["f32" if sfx else "f64"] = res
+        for d in dev.values():
+            d.free()
+    res = results["f64"]
     best = max(res, key=lambda k: res[k]["GB_s"])
     out = {"metric": "grid/surface operator sweeps on 2048^2, HBM GB/s", "value": res[best]["GB_s"], "unit": "GB/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (i.i.d. N(0,1) fields, 4 seeded star curves)",
            "config": {"workload": "operator microbenchmark", "grid_primal": [g.NX - 1, g.NY - 1], "npts": N},
-           "ops": res}
+           "ops": res, "ops_f32": results["f32"]}
     print(json.dumps(out), flush=True)
 
 

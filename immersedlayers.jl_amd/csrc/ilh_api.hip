@@ -363,8 +363,8 @@ int op_components(int op, int* gks)
     }
 }
 
-void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride, int nb, double* out,
-                      size_t out_stride)
+template <class T>
+static void scatter_op_t(ilh_plan* pl, int op, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride)
 {
     const Term* t = get_terms(pl, op);
     int gks[4];
@@ -380,10 +380,14 @@ void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride
     }
 }
 
-void scatter_op(ilh_plan* pl, int op, const double* amp, double* out)
+void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride, int nb, double* out,
+                      size_t out_stride)
 {
-    scatter_op_multi(pl, op, amp, 0, 1, out, 0);
+    scatter_op_t(pl, op, amp, amp_stride, nb, out, out_stride);
 }
+
+void scatter_op(ilh_plan* pl, int op, const double* amp, double* out) { scatter_op_t(pl, op, amp, 0, 1, out, 0); }
+void scatter_op(ilh_plan* pl, int op, const float* amp, float* out) { scatter_op_t(pl, op, amp, 0, 1, out, 0); }
 
 void gather_op_multi(ilh_plan* pl, int op, const double* field, size_t field_stride, int nb, double* out,
                      size_t out_stride)
@@ -394,6 +398,11 @@ void gather_op_multi(ilh_plan* pl, int op, const double* field, size_t field_str
 void gather_op(ilh_plan* pl, int op, const double* field, double* out)
 {
     gather_op_multi(pl, op, field, 0, 1, out, 0);
+}
+
+void gather_op(ilh_plan* pl, int op, const float* field, float* out)
+{
+    gather(pl, get_terms(pl, op), op_terms_per_point(op), field, 0, 1, out, 0);
 }
 
 }  // namespace ilh
@@ -407,15 +416,17 @@ int kind_to_gk(int kind)
     fail(ILH_ERR_ARG, "grid kind must be Nodes{Primal} or Nodes{Dual}");
 }
 
-// generic surface -> grid and grid -> surface entry points
-int surf_to_grid(ilh_plan* pl, int op, size_t nin, size_t nout, double* out, const double* f, int mem,
-                 bool derivative, const char* name)
+// generic surface -> grid and grid -> surface entry points (FP64, or FP32 data
+// of a Float32 cache)
+template <class T>
+int surf_to_grid(ilh_plan* pl, int op, size_t nin, size_t nout, T* out, const T* f, int mem, bool derivative,
+                 const char* name)
 {
     return guarded([&] {
         Io io(pl, mem);
         if (derivative) need_margin(pl, name);
-        const double* d_in = io.in(f, nin, pl->io_a);
-        double* d_out = io.out(out, nout, pl->io_b);
+        const T* d_in = io.in(f, nin, pl->io_a);
+        T* d_out = io.out(out, nout, pl->io_b);
         if (pl->N == 0) fill_zero(pl->ctx->stream, d_out, nout);
         else scatter_op(pl, op, d_in, d_out);
         io.back(out, d_out, nout);
@@ -423,14 +434,15 @@ int surf_to_grid(ilh_plan* pl, int op, size_t nin, size_t nout, double* out, con
     });
 }
 
-int grid_to_surf(ilh_plan* pl, int op, size_t nin, size_t nout, double* out, const double* gfield, int mem,
-                 bool derivative, const char* name)
+template <class T>
+int grid_to_surf(ilh_plan* pl, int op, size_t nin, size_t nout, T* out, const T* gfield, int mem, bool derivative,
+                 const char* name)
 {
     return guarded([&] {
         Io io(pl, mem);
         if (derivative) need_margin(pl, name);
-        const double* d_in = io.in(gfield, nin, pl->io_a);
-        double* d_out = io.out(out, nout, pl->io_b);
+        const T* d_in = io.in(gfield, nin, pl->io_a);
+        T* d_out = io.out(out, nout, pl->io_b);
         if (pl->N > 0) gather_op(pl, op, d_in, d_out);
         io.back(out, d_out, nout);
         io.done();
@@ -560,13 +572,13 @@ extern "C" int ilh_normal_interpolate_symm(ilh_plan* pl, double* tau, const doub
 }
 
 // -------------------------------------------------------------- stencils
-template <class K>
-static int grid_op(ilh_plan* pl, size_t nin, size_t nout, double* out, const double* in, int mem, K&& kern)
+template <class T, class K>
+static int grid_op(ilh_plan* pl, size_t nin, size_t nout, T* out, const T* in, int mem, K&& kern)
 {
     return guarded([&] {
         Io io(pl, mem);
-        const double* d_in = io.in(in, nin, pl->io_a);
-        double* d_out = io.out(out, nout, pl->io_b);
+        const T* d_in = io.in(in, nin, pl->io_a);
+        T* d_out = io.out(out, nout, pl->io_b);
         if (mem == ILH_MEM_DEVICE && (const void*)d_in == (const void*)d_out)
             fail(ILH_ERR_ARG, "stencil output must not alias its input");
         kern(d_out, d_in);
@@ -617,6 +629,85 @@ extern "C" int ilh_inverse_laplacian(ilh_plan* pl, int kind, double* out, const 
         const size_t n = pl->g.n(gk);
         const double* d_in = io.in(in, n, pl->io_a);
         double* d_out = io.out(out, n, pl->io_b);
+        conv_dense(pl, gk, d_in, d_out);
+        io.back(out, d_out, n);
+        io.done();
+    });
+}
+
+// ------------------------------------------------- FP32 (Float32 caches)
+// The reference caches take dtype = Float32 (src/cache.jl:164-201,
+// `dtype` keyword; the Laplacian plan of src/cache.jl:321-324 follows it):
+// the same operators on Float32 grid and surface data.  Stencils compute in
+// FP32; regularize/interpolate accumulate the FP64 DDF weights and round once;
+// inverse_laplacian! reads and writes FP32 fields around FP64 transforms.
+extern "C" int ilh_regularize_f32(ilh_plan* pl, int kind, float* out, const float* f, int mem)
+{
+    if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+    if (kind == ILH_NODES_PRIMAL)
+        return surf_to_grid(pl, OP_R_PRIMAL, pl->N, pl->g.n(GK_PRIMAL), out, f, mem, false, "regularize!");
+    if (kind == ILH_NODES_DUAL)
+        return surf_to_grid(pl, OP_R_DUAL, pl->N, pl->g.n(GK_DUAL), out, f, mem, false, "regularize!");
+    if (kind == ILH_EDGES_PRIMAL)
+        return surf_to_grid(pl, OP_R_EDGES, 2 * (size_t)pl->N, pl->g.n_edges(), out, f, mem, false, "regularize!");
+    g_err = "unknown grid kind";
+    return ILH_ERR_ARG;
+}
+
+extern "C" int ilh_interpolate_f32(ilh_plan* pl, int kind, float* out, const float* gf, int mem)
+{
+    if (!pl) { g_err = "null plan"; return ILH_ERR_ARG; }
+    if (kind == ILH_NODES_PRIMAL)
+        return grid_to_surf(pl, OP_E_PRIMAL, pl->g.n(GK_PRIMAL), pl->N, out, gf, mem, false, "interpolate!");
+    if (kind == ILH_NODES_DUAL)
+        return grid_to_surf(pl, OP_E_DUAL, pl->g.n(GK_DUAL), pl->N, out, gf, mem, false, "interpolate!");
+    if (kind == ILH_EDGES_PRIMAL)
+        return grid_to_surf(pl, OP_E_EDGES, pl->g.n_edges(), 2 * (size_t)pl->N, out, gf, mem, false, "interpolate!");
+    g_err = "unknown grid kind";
+    return ILH_ERR_ARG;
+}
+
+extern "C" int ilh_divergence_f32(ilh_plan* pl, float* p, const float* q, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n_edges(), pl->g.n(GK_PRIMAL), p, q, mem,
+                   [&](float* o, const float* i) { k_divergence(pl, o, i); });
+}
+extern "C" int ilh_grad_f32(ilh_plan* pl, float* q, const float* p, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n(GK_PRIMAL), pl->g.n_edges(), q, p, mem,
+                   [&](float* o, const float* i) { k_grad(pl, o, i); });
+}
+extern "C" int ilh_curl_nodes_to_edges_f32(ilh_plan* pl, float* q, const float* s, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n(GK_DUAL), pl->g.n_edges(), q, s, mem,
+                   [&](float* o, const float* i) { k_curl_n2e(pl, o, i); });
+}
+extern "C" int ilh_curl_edges_to_nodes_f32(ilh_plan* pl, float* w, const float* q, int mem)
+{
+    PL_CHECK
+    return grid_op(pl, pl->g.n_edges(), pl->g.n(GK_DUAL), w, q, mem,
+                   [&](float* o, const float* i) { k_curl_e2n(pl, o, i); });
+}
+extern "C" int ilh_laplacian_f32(ilh_plan* pl, int kind, float* out, const float* in, int mem)
+{
+    PL_CHECK
+    int gk;
+    if (int rc = guarded([&] { gk = kind_to_gk(kind); })) return rc;
+    return grid_op(pl, pl->g.n(gk), pl->g.n(gk), out, in, mem,
+                   [&](float* o, const float* i) { k_laplacian(pl, gk, o, i); });
+}
+extern "C" int ilh_inverse_laplacian_f32(ilh_plan* pl, int kind, float* out, const float* in, int mem)
+{
+    PL_CHECK
+    return guarded([&] {
+        const int gk = kind_to_gk(kind);
+        Io io(pl, mem);
+        const size_t n = pl->g.n(gk);
+        const float* d_in = io.in(in, n, pl->io_a);
+        float* d_out = io.out(out, n, pl->io_b);
         conv_dense(pl, gk, d_in, d_out);
         io.back(out, d_out, n);
         io.done();

@@ -303,6 +303,7 @@ enum RowMode { ROW_DENSE = 0, ROW_LGF = 1, ROW_TERMS = 2 };
 struct RowSrc {
     int mode;
     const double* field;   // dense field (nx x ny)
+    const float* field32;  // the same in FP32 (a Float32 cache), read instead of field when set
     int nx, ny;
     const double* tab;     // LGF table (circular rows of the Px x Py kernel)
     int tmx, Px, Py;
@@ -383,8 +384,12 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
     extern __shared__ double2 lds[];
     // right-hand side blockIdx.y of a batch: its amplitudes (or dense field)
     // and output spectra
-    if (S.mode == ROW_DENSE) S.field += amp_stride * blockIdx.y;
-    else if (S.mode == ROW_TERMS) S.amp += amp_stride * blockIdx.y;
+    if (S.mode == ROW_DENSE) {
+        if (S.field32) S.field32 += amp_stride * blockIdx.y;
+        else S.field += amp_stride * blockIdx.y;
+    } else if (S.mode == ROW_TERMS) {
+        S.amp += amp_stride * blockIdx.y;
+    }
     Tout += out_stride * blockIdx.y;
     constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
     constexpr int R0 = R_first<P>(false), RL = R_last<P>(false);
@@ -452,7 +457,10 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
             const int xpos = io_index(R0, q / R0, q % R0, t, T, P);
             double a = 0.0, b = 0.0;
             if (live && xpos < nx_src) {
-                if (S.mode == ROW_DENSE) {
+                if (S.mode == ROW_DENSE && S.field32) {
+                    a = S.field32[xpos + (size_t)S.nx * j0];
+                    if (j0 + 1 < nrows_limit) b = S.field32[xpos + (size_t)S.nx * (j0 + 1)];
+                } else if (S.mode == ROW_DENSE) {
                     a = S.field[xpos + (size_t)S.nx * j0];
                     if (j0 + 1 < nrows_limit) b = S.field[xpos + (size_t)S.nx * (j0 + 1)];
                 } else {
@@ -587,10 +595,11 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
 }
 
 // ------------------------------------------------------------- pass 3
-// Transform c of block b makes output rows 2p, 2p+1 (p = b NB + c).
-template <int P>
+// Transform c of block b makes output rows 2p, 2p+1 (p = b NB + c), FP64 or
+// (a Float32 cache) rounded to FP32.
+template <int P, class TO>
 __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restrict__ T2, const double2* __restrict__ tw,
-                                                        int nx, int ny, double* __restrict__ out, size_t t2_stride,
+                                                        int nx, int ny, TO* __restrict__ out, size_t t2_stride,
                                                         size_t out_stride)
 {
     extern __shared__ double2 lds[];
@@ -628,8 +637,8 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restric
     for (int q = 0; q < V; ++q) {
         const int xo = io_index(RL, q / RL, q % RL, t, T, P);
         if (xo < nx) {
-            out[xo + (size_t)nx * j0] = v[q].x;
-            if (two) out[xo + (size_t)nx * (j0 + 1)] = v[q].y;
+            out[xo + (size_t)nx * j0] = (TO)v[q].x;
+            if (two) out[xo + (size_t)nx * (j0 + 1)] = (TO)v[q].y;
         }
     }
 }
@@ -706,20 +715,20 @@ static void launch_cols(ilh_ctx* ctx, const double2* T1, const int* slot_of_row,
     ILH_FOR_P(Py, (launch_cols_p<PP, MODE>(ctx, T1, slot_of_row, H, ny, nrows_in, spec, scale, T2, spec_out, bt)));
 }
 
-template <int P>
-static void launch_rows_inv_p(ilh_ctx* ctx, const double2* T2, int nx, int ny, double* out, Batch bt)
+template <int P, class TO>
+static void launch_rows_inv_p(ilh_ctx* ctx, const double2* T2, int nx, int ny, TO* out, Batch bt)
 {
     constexpr int NB = tpb_of(P) / (P / V);
     const int npairs = (ny + 1) / 2;
     const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
-    set_lds((const void*)k_rows_inv<P>, lds);
-    k_rows_inv<P><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+    set_lds((const void*)k_rows_inv<P, TO>, lds);
+    k_rows_inv<P, TO><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
         T2, get_twiddles(ctx, P), nx, ny, out, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
-static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, double* out,
-                            Batch bt = Batch())
+template <class TO>
+static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, TO* out, Batch bt = Batch())
 {
     ILH_FOR_P(Px, launch_rows_inv_p<PP>(ctx, T2, nx, ny, out, bt));
 }
@@ -894,6 +903,22 @@ void conv_dense(ilh_plan* pl, int gk, const double* in, double* out)
     RowSrc S{};
     S.mode = ROW_DENSE;
     S.field = in;
+    S.nx = C.nx;
+    S.ny = C.ny;
+    launch_rows_fwd(pl->ctx, S, C.Px, (C.ny + 1) / 2, C.ny, C.T1.as<double2>());
+    launch_cols<0>(pl->ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.ny, C.ny, C.spec.as<double>(), 1.0,
+                   C.T2.as<double2>(), nullptr);
+    launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
+}
+
+// FP32 field -> FP32 field (a Float32 cache): the FFT passes run in FP64, the
+// field is read and written in FP32
+void conv_dense(ilh_plan* pl, int gk, const float* in, float* out)
+{
+    ConvPlan& C = get_conv(pl, gk);
+    RowSrc S{};
+    S.mode = ROW_DENSE;
+    S.field32 = in;
     S.nx = C.nx;
     S.ny = C.ny;
     launch_rows_fwd(pl->ctx, S, C.Px, (C.ny + 1) / 2, C.ny, C.T1.as<double2>());

@@ -216,12 +216,13 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& bi
 void free_bins(RowBins& b);
 // batched over nb operands (grid.y): operand k reads amp + k amp_stride /
 // field + k field_stride and writes out + k out_stride
-void scatter(ilh_plan* pl, const RowBins& bins, const double* amp, size_t amp_stride, int nb, double* out,
-             size_t out_stride);
+template <class T>
+void scatter(ilh_plan* pl, const RowBins& bins, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride);
 // field: the operator's whole input (Nodes, Edges or EdgeGradient); each term
 // reads its component grid at Grid::offset(gk)
-void gather(ilh_plan* pl, const Term* terms, int nterms_per_point, const double* field, size_t field_stride, int nb,
-            double* out, size_t out_stride);
+template <class T>
+void gather(ilh_plan* pl, const Term* terms, int nterms_per_point, const T* field, size_t field_stride, int nb,
+            T* out, size_t out_stride);
 
 // stencils (ilh_stencil.hip)
 void k_divergence(ilh_plan* pl, double* p, const double* q);
@@ -229,7 +230,14 @@ void k_grad(ilh_plan* pl, double* q, const double* p);
 void k_curl_n2e(ilh_plan* pl, double* q, const double* s);
 void k_curl_e2n(ilh_plan* pl, double* w, const double* q);
 void k_laplacian(ilh_plan* pl, int gk, double* w, const double* s);
+// FP32 grid data (a Float32 cache): the same stencils
+void k_divergence(ilh_plan* pl, float* p, const float* q);
+void k_grad(ilh_plan* pl, float* q, const float* p);
+void k_curl_n2e(ilh_plan* pl, float* q, const float* s);
+void k_curl_e2n(ilh_plan* pl, float* w, const float* q);
+void k_laplacian(ilh_plan* pl, int gk, float* w, const float* s);
 void fill_zero(hipStream_t st, double* a, size_t n);
+void fill_zero(hipStream_t st, float* a, size_t n);
 void axpby(hipStream_t st, double* y, const double* x, double a, double b, size_t n);  // y = a x + b y
 
 // FFT convolution (ilh_fft.hip)
@@ -237,6 +245,7 @@ void ensure_lgf(ilh_plan* pl);
 ConvPlan& get_conv(ilh_plan* pl, int gk);
 // field -> field:  out = dx^2 (G * in)
 void conv_dense(ilh_plan* pl, int gk, const double* in, double* out);
+void conv_dense(ilh_plan* pl, int gk, const float* in, float* out);   // FP32 fields, FP64 transforms
 // nb fields at once: field k at in + k n(gk), out + k n(gk) (in == out allowed)
 void conv_dense_multi(ilh_plan* pl, int gk, const double* in, int nb, double* out);
 // point sources -> field: out = dx^2 (G * sum_terms amp*term), rows listed in pairs
@@ -269,6 +278,9 @@ void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride
                       size_t out_stride);
 void gather_op_multi(ilh_plan* pl, int op, const double* field, size_t field_stride, int nb, double* out,
                      size_t out_stride);
+// FP32 data of a Float32 cache
+void scatter_op(ilh_plan* pl, int op, const float* amp, float* out);
+void gather_op(ilh_plan* pl, int op, const float* field, float* out);
 void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv, int n = 0);
 
 // dense linear algebra (ilh_dense.hip)
@@ -367,25 +379,25 @@ struct Io {
         if (m != ILH_MEM_HOST && m != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
         ILH_HIP(hipSetDevice(p->ctx->device));
     }
-    const double* in(const double* h, size_t n, DevBuf& buf)
+    template <class T> const T* in(const T* h, size_t n, DevBuf& buf)
     {
         if (n > 0 && !h) fail(ILH_ERR_ARG, "null input array");
         if (mem == ILH_MEM_DEVICE) return h;
-        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
-        if (n) ILH_HIP(hipMemcpyAsync(buf.p, h, sizeof(double) * n, hipMemcpyHostToDevice, pl->ctx->stream));
-        return buf.as<double>();
+        buf.ensure(sizeof(T) * std::max<size_t>(n, 1));
+        if (n) ILH_HIP(hipMemcpyAsync(buf.p, h, sizeof(T) * n, hipMemcpyHostToDevice, pl->ctx->stream));
+        return buf.as<T>();
     }
-    double* out(double* h, size_t n, DevBuf& buf)
+    template <class T> T* out(T* h, size_t n, DevBuf& buf)
     {
         if (n > 0 && !h) fail(ILH_ERR_ARG, "null output array");
         if (mem == ILH_MEM_DEVICE) return h;
-        buf.ensure(sizeof(double) * std::max<size_t>(n, 1));
-        return buf.as<double>();
+        buf.ensure(sizeof(T) * std::max<size_t>(n, 1));
+        return buf.as<T>();
     }
-    void back(double* h, const double* d, size_t n)
+    template <class T> void back(T* h, const T* d, size_t n)
     {
         if (mem == ILH_MEM_DEVICE || n == 0) return;
-        ILH_HIP(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, pl->ctx->stream));
+        ILH_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, pl->ctx->stream));
     }
     void done()
     {

@@ -654,39 +654,44 @@ void free_sources(SourceSet& s)
 // out(x, j): zero, except at the row's cells, whose value is the sum of their
 // contributions in (bx, term) order.  A work-group per row: a coalesced zero
 // sweep, then one thread per cell.
+// FP64 (T = double) or FP32 (T = float) data; FP32 values are accumulated
+// in FP64 with the FP64 weights and rounded once.
+template <class T>
 __global__ void k_scatter(const int* __restrict__ cell_ptr, const int4* __restrict__ cells,
                           const int* __restrict__ cai, const double* __restrict__ cw, int nx,
-                          const double* __restrict__ amp, size_t amp_stride, double* __restrict__ out,
-                          size_t out_stride)
+                          const T* __restrict__ amp, size_t amp_stride, T* __restrict__ out, size_t out_stride)
 {
     const int j = blockIdx.x;
     amp += amp_stride * blockIdx.y;
-    double* row = out + out_stride * blockIdx.y + (size_t)nx * j;
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) row[x] = 0.0;
+    T* row = out + out_stride * blockIdx.y + (size_t)nx * j;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) row[x] = T(0);
     const int c0 = cell_ptr[j], c1 = cell_ptr[j + 1];
     if (c0 == c1) return;   // uniform over the work-group
     __syncthreads();        // the zeros are ordered before the cell values
     for (int ci = c0 + threadIdx.x; ci < c1; ci += blockDim.x) {
         const int4 cr = cells[ci];
         double acc = 0.0;
-        for (int e = cr.y; e < cr.z; ++e) acc += amp[cai[e]] * cw[e];
-        row[cr.x] = acc;
+        for (int e = cr.y; e < cr.z; ++e) acc += (double)amp[cai[e]] * cw[e];
+        row[cr.x] = (T)acc;
     }
 }
 
-void scatter(ilh_plan* pl, const RowBins& bins, const double* amp, size_t amp_stride, int nb, double* out,
-             size_t out_stride)
+template <class T>
+void scatter(ilh_plan* pl, const RowBins& bins, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride)
 {
     const int nx = pl->g.nx(bins.gk);
     if (bins.ny <= 0 || nb <= 0) return;
-    k_scatter<<<dim3(bins.ny, nb), 256, 0, pl->ctx->stream>>>(bins.cell_ptr, bins.cells, bins.cai, bins.cw, nx, amp,
-                                                              amp_stride, out, out_stride);
+    k_scatter<T><<<dim3(bins.ny, nb), 256, 0, pl->ctx->stream>>>(bins.cell_ptr, bins.cells, bins.cai, bins.cw, nx,
+                                                                 amp, amp_stride, out, out_stride);
     ILH_HIP(hipGetLastError());
 }
+template void scatter<double>(ilh_plan*, const RowBins&, const double*, size_t, int, double*, size_t);
+template void scatter<float>(ilh_plan*, const RowBins&, const float*, size_t, int, float*, size_t);
 
 // ---------------------------------------------------------------- gather
+template <class T>
 struct GatherFields {
-    const double* f[GK_COUNT];   // by component grid
+    const T* f[GK_COUNT];   // by component grid
     int nx[GK_COUNT], ny[GK_COUNT];
 };
 
@@ -694,8 +699,9 @@ struct GatherFields {
 // ... of the point's T terms (T x 5 x 5), each the field value times coef wy
 // wx, into the partial sum of its term's amplitude slot; the wave then sums
 // the two slots' partials in a fixed butterfly order (bitwise reproducible).
-__global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int T, int N, GatherFields F,
-                                                size_t field_stride, double* __restrict__ out, size_t out_stride)
+template <class D>
+__global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int T, int N, GatherFields<D> F,
+                                                size_t field_stride, D* __restrict__ out, size_t out_stride)
 {
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -719,7 +725,7 @@ __global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int 
         double v = 0.0;
         const double w = tt.wy[b] * tt.wx[a];
         if (w != 0.0 && row >= 0 && row < ny && col >= 0 && col < nx)
-            v = tt.coef * w * F.f[gk][foff + col + (size_t)nx * row];
+            v = tt.coef * w * (double)F.f[gk][foff + col + (size_t)nx * row];
         if (tt.ai == ai0) acc0 += v;
         else acc1 += v;
     }
@@ -728,25 +734,28 @@ __global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int 
         acc1 += __shfl_xor(acc1, off, 64);
     }
     if (lane == 0) {
-        out[ai0] = acc0;
-        if (ai1 >= 0) out[ai1] = acc1;
+        out[ai0] = (D)acc0;
+        if (ai1 >= 0) out[ai1] = (D)acc1;
     }
 }
 
-void gather(ilh_plan* pl, const Term* terms, int T, const double* field, size_t field_stride, int nb, double* out,
+template <class D>
+void gather(ilh_plan* pl, const Term* terms, int T, const D* field, size_t field_stride, int nb, D* out,
             size_t out_stride)
 {
     if (pl->N == 0 || nb <= 0) return;
-    GatherFields F;
+    GatherFields<D> F;
     const Grid& g = pl->g;
     for (int k = 0; k < GK_COUNT; ++k) {
         F.f[k] = field + g.offset(k);
         F.nx[k] = g.nx(k);
         F.ny[k] = g.ny(k);
     }
-    k_gather<<<dim3((pl->N + 3) / 4, nb), 256, 0, pl->ctx->stream>>>(terms, T, pl->N, F, field_stride, out,
-                                                                     out_stride);
+    k_gather<D><<<dim3((pl->N + 3) / 4, nb), 256, 0, pl->ctx->stream>>>(terms, T, pl->N, F, field_stride, out,
+                                                                        out_stride);
     ILH_HIP(hipGetLastError());
 }
+template void gather<double>(ilh_plan*, const Term*, int, const double*, size_t, int, double*, size_t);
+template void gather<float>(ilh_plan*, const Term*, int, const float*, size_t, int, float*, size_t);
 
 }  // namespace ilh

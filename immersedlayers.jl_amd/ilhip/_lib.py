@@ -90,6 +90,14 @@ SIGNATURES = {
     "ilh_surface_curl_cross": ([_vp, _vp, _vp, _i], _i),
     "ilh_surface_curl_cross_adj": ([_vp, _vp, _vp, _i], _i),
     "ilh_surface_grad_cross": ([_vp, _vp, _vp, _i], _i),
+    "ilh_regularize_f32": ([_vp, _i, _vp, _vp, _i], _i),
+    "ilh_interpolate_f32": ([_vp, _i, _vp, _vp, _i], _i),
+    "ilh_divergence_f32": ([_vp, _vp, _vp, _i], _i),
+    "ilh_grad_f32": ([_vp, _vp, _vp, _i], _i),
+    "ilh_curl_nodes_to_edges_f32": ([_vp, _vp, _vp, _i], _i),
+    "ilh_curl_edges_to_nodes_f32": ([_vp, _vp, _vp, _i], _i),
+    "ilh_laplacian_f32": ([_vp, _i, _vp, _vp, _i], _i),
+    "ilh_inverse_laplacian_f32": ([_vp, _i, _vp, _vp, _i], _i),
     "ilh_surface_curl_vector": ([_vp, _vp, _vp, _i], _i),
     "ilh_surface_curl_vector_adj": ([_vp, _vp, _vp, _i], _i),
     "ilh_surface_divergence_symm": ([_vp, _vp, _vp, _i], _i),

@@ -88,31 +88,32 @@ def default_context():
 
 
 class DeviceArray:
-    """FP64 device buffer owned by a context (``ilh_malloc``)."""
+    """FP64 (or FP32) device buffer owned by a context (``ilh_malloc``)."""
 
-    def __init__(self, ctx, n):
-        self.ctx, self.n = ctx, int(n)
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.dtype.itemsize * self.n
         p = C.c_void_p()
-        L.call("ilh_malloc", ctx.h, C.c_size_t(8 * max(self.n, 1)), C.byref(p))
+        L.call("ilh_malloc", ctx.h, C.c_size_t(max(self.nbytes, 8)), C.byref(p))
         self.ptr = p
         _live["buffer"].add(self)
 
     @classmethod
-    def from_host(cls, ctx, a):
-        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
-        d = cls(ctx, a.size)
+    def from_host(cls, ctx, a, dtype=np.float64):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+        d = cls(ctx, a.size, dtype)
         d.upload(a)
         return d
 
     def upload(self, a):
-        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype).ravel(order="F"))
         if a.size != self.n:
             raise L.DimensionMismatch("upload of %d values into %d" % (a.size, self.n))
-        L.call("ilh_memcpy_h2d", self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(8 * self.n))
+        L.call("ilh_memcpy_h2d", self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes))
 
     def download(self):
-        out = np.empty(self.n)
-        L.call("ilh_memcpy_d2h", self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(8 * self.n))
+        out = np.empty(self.n, dtype=self.dtype)
+        L.call("ilh_memcpy_d2h", self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes))
         return out
 
     def free(self):
@@ -128,13 +129,24 @@ class DeviceArray:
 
 
 # ------------------------------------------------------------ grid data
+def _dt(data, dtype):
+    """Element type of grid/surface data: FP64, or FP32 (a Float32 cache) when
+    asked for or when the given data are float32."""
+    if dtype is not None:
+        return np.dtype(dtype)
+    if data is not None and np.asarray(data).dtype == np.float32:
+        return np.dtype(np.float32)
+    return np.dtype(np.float64)
+
+
 class Nodes:
     """Nodes{Primal|Dual} grid data: ``data`` is an (nx, ny) Fortran-ordered array."""
 
-    def __init__(self, kind, g, data=None):
+    def __init__(self, kind, g, data=None, dtype=None):
         self.kind = kind
         shape = g.primal_shape() if kind == Primal else g.dual_shape()
-        self.data = np.zeros(shape, order="F") if data is None else np.asfortranarray(data, dtype=np.float64)
+        dt = _dt(data, dtype)
+        self.data = np.zeros(shape, dtype=dt, order="F") if data is None else np.asfortranarray(data, dtype=dt)
         if self.data.shape != shape:
             raise L.DimensionMismatch("Nodes{%s} of size %s, expected %s" % (kind, self.data.shape, shape))
 
@@ -149,9 +161,10 @@ class Nodes:
 class Edges:
     """Edges{Primal} grid data: one buffer [u; v], u (NX, NY-1), v (NX-1, NY)."""
 
-    def __init__(self, g, data=None):
+    def __init__(self, g, data=None, dtype=None):
         nu, nv = g.NX * (g.NY - 1), (g.NX - 1) * g.NY
-        self.buf = np.zeros(nu + nv) if data is None else np.ascontiguousarray(data, dtype=np.float64)
+        dt = _dt(data, dtype)
+        self.buf = np.zeros(nu + nv, dtype=dt) if data is None else np.ascontiguousarray(data, dtype=dt)
         if self.buf.size != nu + nv:
             raise L.DimensionMismatch("Edges{Primal} of length %d, expected %d" % (self.buf.size, nu + nv))
         self.u = self.buf[:nu].reshape((g.NX, g.NY - 1), order="F")
@@ -162,8 +175,9 @@ class Edges:
 
 
 class ScalarData:
-    def __init__(self, n, data=None):
-        self.data = np.zeros(n) if data is None else np.ascontiguousarray(data, dtype=np.float64).copy()
+    def __init__(self, n, data=None, dtype=None):
+        dt = _dt(data, dtype)
+        self.data = np.zeros(n, dtype=dt) if data is None else np.ascontiguousarray(data, dtype=dt).copy()
         if self.data.size != n:
             raise L.DimensionMismatch("ScalarData of length %d, expected %d" % (self.data.size, n))
 
@@ -172,8 +186,9 @@ class ScalarData:
 
 
 class VectorData:
-    def __init__(self, n, data=None):
-        self.buf = np.zeros(2 * n) if data is None else np.ascontiguousarray(data, dtype=np.float64).copy()
+    def __init__(self, n, data=None, dtype=None):
+        dt = _dt(data, dtype)
+        self.buf = np.zeros(2 * n, dtype=dt) if data is None else np.ascontiguousarray(data, dtype=dt).copy()
         if self.buf.size != 2 * n:
             raise L.DimensionMismatch("VectorData of length %d, expected %d" % (self.buf.size, 2 * n))
         self.u = self.buf[:n]
@@ -211,9 +226,12 @@ class SurfaceScalarCache:
     """The reference's SurfaceScalarCache (src/cache.jl:164-182) backed by an
     ``ilh_plan``: grid, surface points, normals, areas, DDF and scaling."""
 
-    def __init__(self, surface, g, ddftype=Yang3, scaling=GridScaling, ctx=None):
+    def __init__(self, surface, g, ddftype=Yang3, scaling=GridScaling, ctx=None, dtype=np.float64):
         if scaling != GridScaling:
             raise L.ArgumentError("only GridScaling is supported")
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise L.ArgumentError("dtype must be float64 or float32")
         self.ctx = ctx or default_context()
         self.g = g if isinstance(g, GridSpec) else GridSpec(*g)
         self.ddftype = ddftype
@@ -256,16 +274,16 @@ class SurfaceScalarCache:
         return ScalarData(self.N, self._ds)
 
     def zeros_grid(self):
-        return Nodes(Primal, self.g)
+        return Nodes(Primal, self.g, dtype=self.dtype)
 
     def zeros_gridcurl(self):
-        return Nodes(Dual, self.g)
+        return Nodes(Dual, self.g, dtype=self.dtype)
 
     def zeros_gridgrad(self):
-        return Edges(self.g)
+        return Edges(self.g, dtype=self.dtype)
 
     def zeros_surface(self):
-        return ScalarData(self.N)
+        return ScalarData(self.N, dtype=self.dtype)
 
     def close(self):
         if getattr(self, "h", None):
@@ -285,16 +303,16 @@ class SurfaceVectorCache(SurfaceScalarCache):
     EdgeGradient as its tensor grid data."""
 
     def zeros_grid(self):
-        return Edges(self.g)
+        return Edges(self.g, dtype=self.dtype)
 
     def zeros_gridgrad(self):
         return EdgeGradient(self.g)
 
     def zeros_surface(self):
-        return VectorData(self.N)
+        return VectorData(self.N, dtype=self.dtype)
 
     def zeros_surfacescalar(self):
-        return ScalarData(self.N)
+        return ScalarData(self.N, dtype=self.dtype)
 
 
 # -------------------------------------------------------------- dispatch
@@ -303,10 +321,23 @@ def _check_len(a, n, what):
         raise L.DimensionMismatch("%s has %d values, expected %d" % (what, a.flat().size, n))
 
 
+# operators with an FP32 entry point (Float32 caches)
+_F32 = {"ilh_regularize", "ilh_interpolate", "ilh_divergence", "ilh_grad", "ilh_curl_nodes_to_edges",
+        "ilh_curl_edges_to_nodes", "ilh_laplacian", "ilh_inverse_laplacian"}
+
+
 def _run(name, cache, out, inp, *lead):
     o, i = out.flat(), inp.flat()
     if not o.flags.c_contiguous:
         raise L.ArgumentError("output must be contiguous")
+    if o.dtype != i.dtype:
+        raise L.ArgumentError("input and output element types differ (%s, %s)" % (i.dtype, o.dtype))
+    if o.dtype == np.float32:
+        if name not in _F32:
+            raise L.ArgumentError("%s has no FP32 form" % name)
+        name += "_f32"
+    elif o.dtype != np.float64:
+        raise L.ArgumentError("grid and surface data must be float64 or float32")
     i = np.ascontiguousarray(i)
     L.call(name, cache.h, *lead, _ptr(o), _ptr(i), L.MEM_HOST)
     return out

@@ -162,6 +162,20 @@ int ilh_laplacian(ilh_plan* plan, int grid_kind, double* out, const double* in, 
 /* inverse_laplacian!(sol, rhs) / in place when out == in  (src/grid_operators.jl:144-179) */
 int ilh_inverse_laplacian(ilh_plan* plan, int grid_kind, double* out, const double* in, int mem);
 
+/* ---- FP32 data (caches built with dtype = Float32, src/cache.jl:164-201) ----
+ * The same operators on float buffers: stencils in FP32 (the difference in
+ * FP32, the 1/dx scaling rounded once); regularize!/interpolate! accumulate
+ * the FP64 DDF weights and round once; inverse_laplacian! reads and writes
+ * FP32 fields around FP64 transforms. */
+int ilh_regularize_f32(ilh_plan* plan, int grid_kind, float* out, const float* f, int mem);
+int ilh_interpolate_f32(ilh_plan* plan, int grid_kind, float* out, const float* grid, int mem);
+int ilh_divergence_f32(ilh_plan* plan, float* p, const float* q_edges, int mem);
+int ilh_grad_f32(ilh_plan* plan, float* q_edges, const float* p, int mem);
+int ilh_curl_nodes_to_edges_f32(ilh_plan* plan, float* q_edges, const float* s, int mem);
+int ilh_curl_edges_to_nodes_f32(ilh_plan* plan, float* w, const float* q_edges, int mem);
+int ilh_laplacian_f32(ilh_plan* plan, int grid_kind, float* out, const float* in, int mem);
+int ilh_inverse_laplacian_f32(ilh_plan* plan, int grid_kind, float* out, const float* in, int mem);
+
 /* ---- Schur complements (A10-A12) ---- */
 /* create_RTLinvR / create_CLinvCT / create_GLinvD / create_nRTRn with `scale`
    (src/matrix_operators.jl:9-30,40-61,135-155,225-244).  `out` is N x N
