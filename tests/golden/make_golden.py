@@ -4,7 +4,10 @@ The reference stores no arrays, so these fixtures record the oracle's results
 on seeded synthetic inputs (self-consistency of oracle and HIP path); the
 Neumann case is the reference's own published example (Circle(0.25) inside
 Square(1), dx = 0.01 on [-2,2]^2, examples/neumann.ipynb cell 37), whose
-added mass M_x = 0.22649277527914002 the oracle reproduces.
+added mass M_x = 0.22649277527914002 the oracle reproduces; the Stokes case
+is the reference's Stokes example (examples/stokes.ipynb: 214 points), whose
+outputs are published only as plots (traction, velocity and streamfunction
+probes are recorded).
 
     python tests/golden/make_golden.py
 """
@@ -60,6 +63,21 @@ def main():
                         f_norm=np.linalg.norm(f), s_norm=np.linalg.norm(st), probe=probe, f_probe=f[probe],
                         s_probe=st[probe])
     print("M =", M)
+    # the Stokes example of test/literate/stokes.jl (Rectangle(0.5, 0.25) at
+    # 1.4 dx, 214 points on the 406^2 grid, translated at unit speed)
+    g, s, vp = G.config_stokes()
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+    v, st, sig = O.stokes_solve(og, geo, vp)
+    Cf = O.surface_filter_vector(og, geo)
+    sig2 = Cf @ (Cf @ sig)
+    pv = np.arange(0, v.size, 1009)
+    ps = np.arange(0, st.size, 503)
+    np.savez_compressed(os.path.join(HERE, "stokes_rectangle.npz"), grid=np.array([g.NX, g.NY, g.I0x, g.I0y]),
+                        dx=g.dx, X=s.X, Y=s.Y, nx=s.nx, ny=s.ny, ds=s.ds, vp=vp, sigma=sig, sigma2=sig2,
+                        v_norm=np.linalg.norm(v), probe_v=pv, v_probe=v[pv], probe_s=ps, s_probe=st[ps],
+                        force=np.array([np.sum(sig2[:s.N] * s.ds), np.sum(sig2[s.N:] * s.ds)]))
+    print("Stokes force (C^2-filtered traction) =", np.sum(sig2[:s.N] * s.ds), np.sum(sig2[s.N:] * s.ds))
 
 
 if __name__ == "__main__":

@@ -57,6 +57,24 @@ def test_neumann_added_mass(ctx):
     assert np.abs(f.flat()[d["probe"]] - d["f_probe"]).max() < 1e-10 * np.abs(d["f_probe"]).max()
 
 
+def test_stokes_golden(ctx):
+    """The reference's Stokes example (test/literate/stokes.jl, 214 points on
+    406^2): traction (C^2-filtered), velocity and streamfunction probes (the
+    streamfunction up to its additive constant) and the total force."""
+    import ilhip
+    d, g, s = _load("stokes_rectangle.npz")
+    print("stokes golden N", s.N, flush=True)
+    cache = ilhip.SurfaceVectorCache(s, g, ctx=ctx)
+    v, st, sig = ilhip.StokesFlow(cache).solve(d["vp"])
+    assert rel_err(sig.flat(), d["sigma2"]) < 1e-9
+    assert np.abs(v.flat()[d["probe_v"]] - d["v_probe"]).max() < 1e-10 * np.abs(d["v_probe"]).max()
+    assert abs(np.linalg.norm(v.flat()) - d["v_norm"]) < 1e-10 * d["v_norm"]
+    ds = st.flat()[d["probe_s"]] - d["s_probe"]
+    assert np.abs(ds - ds.mean()).max() < 1e-10 * np.abs(d["s_probe"]).max()
+    F = np.array([np.sum(sig.flat()[:s.N] * s.ds), np.sum(sig.flat()[s.N:] * s.ds)])
+    assert abs(F[0] - d["force"][0]) < 1e-9 * abs(d["force"][0]) and abs(F[1]) < 1e-9 * abs(F[0])
+
+
 def test_headline_parity_and_determinism(ctx):
     import ilhip
     from ilhip import geometry as G

@@ -94,6 +94,24 @@ def test_golden_dirichlet_reproduced_by_oracle():
     assert np.abs(s - d["s"]).max() < 1e-10 * np.abs(d["s"]).max()
 
 
+def test_golden_stokes_reproduced_by_oracle():
+    """The reference's Stokes example (214 points on the 406^2 grid): the
+    oracle reproduces the fixture; the rectangle's symmetry about the x axis
+    leaves no lift (F_y = 0) and the constraint E v = v_bar holds."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "stokes_rectangle.npz"))
+    NX, NY, I0x, I0y = (int(v) for v in d["grid"])
+    g = O.grid(NX, NY, I0x, I0y, float(d["dx"]))
+    geo = O.Geometry(d["X"], d["Y"], d["nx"], d["ny"], d["ds"])
+    assert geo.N == 214
+    v, st, sig = O.stokes_solve(g, geo, d["vp"])
+    assert np.abs(sig - d["sigma"]).max() < 1e-9 * np.abs(d["sigma"]).max()
+    assert np.abs(v[d["probe_v"]] - d["v_probe"]).max() < 1e-11 * np.abs(d["v_probe"]).max()
+    assert abs(d["force"][1]) < 1e-10 * abs(d["force"][0])
+    nu = NX * (NY - 1)
+    Ev = np.concatenate([O.interpolate(g, O.EDGE_U, geo, v[:nu]), O.interpolate(g, O.EDGE_V, geo, v[nu:])])
+    assert np.abs(Ev - 0.5 * d["vp"]).max() < 1e-12
+
+
 def test_golden_neumann_reproduced_by_oracle():
     d = np.load(os.path.join(ROOT, "tests", "golden", "neumann_square_circle.npz"))
     NX, NY, I0x, I0y = (int(v) for v in d["grid"])
