@@ -54,10 +54,27 @@ def cpu_baseline(g, s, fb, budget_s):
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 20:
             break
+    # the same on one core (a shorter sample: at least one solve, about a third of the budget)
+    O.set_threads(1)
+    n1, t1 = 0, time.perf_counter()
+    while True:
+        O.dirichlet_solve(og, geo, fb, None, S)
+        n1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 >= budget_s / 3 or n1 >= 10:
+            break
+    O.set_threads(threads)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": n / el, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": "%d oracle Dirichlet solves (LU inside each S\\s, 2 FFT convolutions of 2048^2) "
-                      "on the same 1024^2 grid and %d points; S built once in %.1f s (not timed)"
-                      % (n, s.N, t_build)}, f_ref, s_ref
+            "value_1core": n1 / el1, "cpu_model": model,
+            "sample": "%d oracle Dirichlet solves on %d threads and %d on one (LU inside each S\\s, 2 FFT "
+                      "convolutions of 2048^2) on the same 1024^2 grid and %d points; S built once in %.1f s "
+                      "(not timed)" % (n, threads, n1, s.N, t_build)}, f_ref, s_ref
 
 
 def moved(s, angle, heave):
