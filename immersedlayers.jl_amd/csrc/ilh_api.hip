@@ -72,8 +72,10 @@ struct SolveCache {
 struct StokesCache {
     DevBuf S2inv, Ssinv;          // row-major inverses, leading dimensions solve_ld(2N), solve_ld(N)
     DevBuf S2;                    // create_CL2invCT row-major (the residual of the refinement step)
+    DevBuf Ka;                    // E C L^-2 C^T D_s^symm (2N x 2N, row-major): the slip of psi*
     DevBuf sv;                    // surface work: dv, vb, e_phi, w, vprime, sigma (2N each); dvn, dsv, ds (N each)
-    DevBuf gE, gE2, gD, gP;       // grid work: edges (2), dual, primal
+    DevBuf gE, gE2, gP;           // grid work: edges (2), primal
+    DevBuf gDD;                   // two Nodes{Dual} fields back to back (one batched convolution)
 };
 
 // Moving bodies: the inverse Schur matrix of a reference geometry (row-major,
@@ -782,6 +784,38 @@ void build_cl2invct(ilh_plan* pl, double scale, double* K)
     ILH_HIP(hipStreamSynchronize(st));   // the batch buffers are released on return
 }
 
+// Ka = E C L^-2 C^T D_s^symm (2N x 2N, column-major): the surface velocity
+// of the streamfunction L^-2 C^T D_s^symm [v] that the Stokes solve's psi*
+// is (with the sign), column by column like create_CL2invCT (the two curls
+// are grid stencils, applied per column).
+void build_stokes_ka(ilh_plan* pl, double* K)
+{
+    const int n2 = 2 * pl->N;
+    const Grid& g = pl->g;
+    const size_t nd = g.n(GK_DUAL), ne = g.n_edges();
+    hipStream_t st = pl->ctx->stream;
+    const size_t per_col = (2 * nd + ne) * sizeof(double) + 4 * nd * sizeof(double2);
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)n2, ((size_t)1 << 30) / per_col}));
+    DevBuf amp, fe, f1, f2;
+    amp.ensure(sizeof(double) * (size_t)n2 * nb);
+    fe.ensure(sizeof(double) * ne * nb);
+    f1.ensure(sizeof(double) * nd * nb);
+    f2.ensure(sizeof(double) * nd * nb);
+    for (int c0 = 0; c0 < n2; c0 += nb) {
+        const int m = std::min(nb, n2 - c0);
+        const size_t na = (size_t)n2 * m;
+        k_unit_columns<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(amp.as<double>(), n2, c0, m);
+        ILH_HIP(hipGetLastError());
+        scatter_op_multi(pl, OP_DSYM, amp.as<double>(), n2, m, fe.as<double>(), ne);
+        for (int k = 0; k < m; ++k) k_curl_e2n(pl, f1.as<double>() + nd * k, fe.as<double>() + ne * k);
+        conv_dense_multi(pl, GK_DUAL, f1.as<double>(), m, f2.as<double>());
+        conv_dense_multi(pl, GK_DUAL, f2.as<double>(), m, f1.as<double>());
+        for (int k = 0; k < m; ++k) k_curl_n2e(pl, fe.as<double>() + ne * k, f1.as<double>() + nd * k);
+        gather_op_multi(pl, OP_E_EDGES, fe.as<double>(), ne, m, K + (size_t)n2 * c0, n2);
+    }
+    ILH_HIP(hipStreamSynchronize(st));   // the batch buffers are released on return
+}
+
 // order of the matrix a Schur kind builds
 int schur_dim(const ilh_plan* pl, int kind) { return kind == ILH_SCHUR_CL2INVCT ? 2 * pl->N : pl->N; }
 
@@ -1404,13 +1438,16 @@ extern "C" int ilh_surface_filter_apply_vector(ilh_plan* pl, int power, const do
 
 // ------------------------------------------------------------ Stokes flow
 // test/literate/stokes.jl:101-166 on the vector cache.  Per geometry: S =
-// create_CL2invCT (2N x 2N) and its inverse, and the inverse of Ss =
-// create_CLinvCT_scalar (N x N), row-major.  Per solve, with a = L^-2 C D_s^symm [v] (so the
-// reference's psi* is -a), phi = L^-1 R (n . [v]), v_phi = G phi:
-//   sigma = S^-1 (v_bar - E v_phi + E C a),   b = L^-2 C_s^T sigma,
-//   v = v_phi - C (a + b),   s = -(a + b + L^-1 C^x_s^T Ss^-1 G^x_s phi),
-// the reference's sequence with its negations folded into the sums (each
-// folded negation is exact, so the rounding is the reference order's).
+// create_CL2invCT (2N x 2N) and its inverse, the inverse of Ss =
+// create_CLinvCT_scalar (N x N), and Ka = E C L^-2 C^T D_s^symm (2N x 2N),
+// row-major.  Per solve, with a = L^-2 C^T D_s^symm [v] (the reference's psi*
+// is -a), phi = L^-1 R (n . [v]), v_phi = G phi:
+//   sigma = S^-1 (v_bar - E v_phi + Ka [v]),   b = L^-2 C_s^T sigma,
+//   t = a + b = L^-1 (L^-1 (C^T D_s^symm [v] + C_s^T sigma)),
+//   v = v_phi - C t,   s = -(t + L^-1 C^x_s^T Ss^-1 G^x_s phi).
+// The reference's six convolutions per solve become three passes: phi on
+// Nodes{Primal}, L^-1 of the summed sources on Nodes{Dual}, then L^-1 of
+// that field and of the potential's sources together as one batched pass.
 namespace {
 __global__ void k_stokes_rhs(int N, const double* __restrict__ vp, const double* __restrict__ vm,
                              const double* __restrict__ nx, const double* __restrict__ ny, double* __restrict__ dv,
@@ -1448,7 +1485,7 @@ StokesCache* prepare_stokes(ilh_plan* pl)
         sc->sv.ensure(sizeof(double) * (size_t)std::max(1, 6 * n2 + 3 * N));
         sc->gE.ensure(sizeof(double) * g.n_edges());
         sc->gE2.ensure(sizeof(double) * g.n_edges());
-        sc->gD.ensure(sizeof(double) * g.n(GK_DUAL));
+        sc->gDD.ensure(2 * sizeof(double) * g.n(GK_DUAL));
         sc->gP.ensure(sizeof(double) * g.n(GK_PRIMAL));
         if (N > 0) {
             DevBuf K;
@@ -1461,6 +1498,10 @@ StokesCache* prepare_stokes(ilh_plan* pl)
             build_schur(pl, ILH_SCHUR_CLINVCT, 1.0, K.as<double>());
             sc->Ssinv.ensure(sizeof(double) * (size_t)N * solve_ld(N));
             inverse_rowmajor(pl, K.as<double>(), sc->Ssinv.as<double>(), N);
+            build_stokes_ka(pl, K.as<double>());
+            sc->Ka.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
+            transpose(pl->ctx->stream, K.as<double>(), sc->Ka.as<double>(), n2, n2, solve_ld(n2));
+            ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
         }
     } catch (...) {
         delete sc;
@@ -1493,51 +1534,48 @@ extern "C" int ilh_stokes_solve(ilh_plan* pl, const double* v_plus, const double
         double *dv = sv, *vb = sv + n2, *ephi = sv + 2 * n2, *w = sv + 3 * n2, *vpr = sv + 4 * n2;
         double* sig = mem == ILH_MEM_DEVICE ? sigma : sv + 5 * n2;
         double *dvn = sv + 6 * n2, *dsv = dvn + N, *dsn = dsv + N;
-        double *gE = sc->gE.as<double>(), *gE2 = sc->gE2.as<double>(), *gD = sc->gD.as<double>();
-        double* gP = sc->gP.as<double>();
+        double *gE = sc->gE.as<double>(), *gE2 = sc->gE2.as<double>(), *gP = sc->gP.as<double>();
+        double *gD = sc->gDD.as<double>(), *gD2 = gD + nd;
         if (N == 0) {
             fill_zero(st, d_v, ne);
             fill_zero(st, d_s, nd);
         } else {
             const unsigned bN = (unsigned)((N + 127) / 128), b2 = (unsigned)((n2 + 127) / 128);
+            const int ld2 = solve_ld(n2);
             k_stokes_rhs<<<bN, 128, 0, st>>>(N, d_vp, d_vm, pl->nx.as<double>(), pl->ny.as<double>(), dv, vb, dvn);
-            // a = L^-2 C D_s^symm [v]
-            scatter_op(pl, OP_DSYM, dv, gE);
-            k_curl_e2n(pl, gD, gE);
-            conv_dense(pl, GK_DUAL, gD, gD);
-            conv_dense(pl, GK_DUAL, gD, gD);
             // phi = L^-1 R (n . [v]),  v_phi = G phi
             scatter_op(pl, OP_R_PRIMAL, dvn, gP);
             conv_dense(pl, GK_PRIMAL, gP, gP);
             k_grad(pl, gE2, gP);
             gather_op(pl, OP_E_EDGES, gE2, ephi);
-            // w = E C a (the reference's -E C psi*)
-            k_curl_n2e(pl, gE, gD);
-            gather_op(pl, OP_E_EDGES, gE, w);
+            // w = Ka [v] = E C a (the reference's -E C psi*)
+            matvec_rows(st, sc->Ka.as<double>(), ld2, dv, w, n2, n2, 1.0, 0);
             k_stokes_vprime<<<b2, 128, 0, st>>>(n2, vb, ephi, w, vpr);
             ILH_HIP(hipGetLastError());
             // sigma = S^-1 vprime with one refinement step, so that the residual
             // (the constraint E v = v_bar) is at roundoff as with the
             // reference's LU solve, not at cond(S) eps
-            const int ld2 = solve_ld(n2);
             matvec_rows(st, sc->S2inv.as<double>(), ld2, vpr, sig, n2, n2, 1.0, 0);
             axpby(st, ephi, vpr, 1.0, 0.0, n2);
             matvec_rows(st, sc->S2.as<double>(), ld2, sig, ephi, n2, n2, -1.0, 1);
             matvec_rows(st, sc->S2inv.as<double>(), ld2, ephi, sig, n2, n2, 1.0, 1);
-            // b = L^-2 C_s^T sigma;  gD <- a + b
-            scatter_op(pl, OP_CVT, sig, d_s);
-            conv_dense(pl, GK_DUAL, d_s, d_s);
-            conv_dense(pl, GK_DUAL, d_s, d_s);
-            axpby(st, gD, d_s, 1.0, 1.0, nd);
-            // v = v_phi - C (a + b)
-            k_curl_n2e(pl, d_v, gD);
-            axpby(st, d_v, gE2, 1.0, -1.0, ne);
-            // streamfunction of the potential part: L^-1 C^x_s^T Ss^-1 G^x_s phi
+            // y = L^-1 (C^T D_s^symm [v] + C_s^T sigma)
+            scatter_op(pl, OP_DSYM, dv, gE);
+            k_curl_e2n(pl, gD, gE);
+            scatter_op(pl, OP_CVT, sig, gD2);
+            axpby(st, gD, gD2, 1.0, 1.0, nd);
+            conv_dense(pl, GK_DUAL, gD, gD);
+            // the potential's streamfunction sources C^x_s^T Ss^-1 G^x_s phi
             gather_op(pl, OP_GXS, gP, dsv);
             matvec_rows(st, sc->Ssinv.as<double>(), solve_ld(N), dsv, dsn, N, N, 1.0, 0);
-            scatter_op(pl, OP_CXT, dsn, d_s);
-            conv_dense(pl, GK_DUAL, d_s, d_s);
-            axpby(st, d_s, gD, -1.0, -1.0, nd);   // s = -(a + b) - c
+            scatter_op(pl, OP_CXT, dsn, gD2);
+            // [t, c] = L^-1 [y, C^x_s^T ds] in one batched pass
+            conv_dense_multi(pl, GK_DUAL, gD, 2, gD);
+            // v = v_phi - C t,  s = -(t + c)
+            k_curl_n2e(pl, d_v, gD);
+            axpby(st, d_v, gE2, 1.0, -1.0, ne);
+            axpby(st, gD2, gD, -1.0, -1.0, nd);
+            ILH_HIP(hipMemcpyAsync(d_s, gD2, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
             if (filter_power > 0) filter_vector(pl, filter_power, sig, gE);
         }
         io.back(v, d_v, ne);

@@ -73,8 +73,14 @@ def main():
     sig2 = Cf @ (Cf @ sig)
     pv = np.arange(0, v.size, 1009)
     ps = np.arange(0, st.size, 503)
+    # create_CL2invCT's smallest singular direction (the traction of a uniform
+    # streamfunction; condition number 1.7e7 here), after the C^2 filter: the
+    # direction in which two FP64 solves may differ by cond * eps
+    _, _, Vt = np.linalg.svd(O.cl2invct(og, geo))
+    null2 = Cf @ (Cf @ Vt[-1])
     np.savez_compressed(os.path.join(HERE, "stokes_rectangle.npz"), grid=np.array([g.NX, g.NY, g.I0x, g.I0y]),
                         dx=g.dx, X=s.X, Y=s.Y, nx=s.nx, ny=s.ny, ds=s.ds, vp=vp, sigma=sig, sigma2=sig2,
+                        null2=null2 / np.linalg.norm(null2),
                         v_norm=np.linalg.norm(v), probe_v=pv, v_probe=v[pv], probe_s=ps, s_probe=st[ps],
                         force=np.array([np.sum(sig2[:s.N] * s.ds), np.sum(sig2[s.N:] * s.ds)]))
     print("Stokes force (C^2-filtered traction) =", np.sum(sig2[:s.N] * s.ds), np.sum(sig2[s.N:] * s.ds))
