@@ -137,6 +137,7 @@ static void drop_caches(ilh_plan* pl, bool release = false)
     }
     for (int op = 0; op < OP_COUNT; ++op) {
         pl->terms_ok[op] = false;
+        pl->wins_ok = false;
         pl->bins_ok[op] = false;
         if (release)
             for (auto& b : pl->bins[op]) free_bins(b);
@@ -1848,14 +1849,15 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
         rc[tid] = st[L.cs() + tid];
         rs[tid] = st[L.sn() + tid];
     }
-    double a = 0.0;
-    for (int k = tid; k < n; k += 1024) a += w[k] * w[k];
-    double wn = sqrt(block_sum_1024(a, red));   // |w| before the pass
+    __shared__ double wn0;                // |w|^2 before the first pass
+    double wn = 0.0;
     for (int i = tid; i < nv; i += 1024) h[1][i] = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
-        // dots: wave i % 16 owns basis vector i
-        for (int i = wave; i < nv; i += 16) {
-            const double* v = V + (size_t)ldv * i;
+        // dots: wave i % 16 owns basis vector i; in the first pass the wave
+        // after the last vector also takes w . w (the norm before the pass)
+        const int nd = pass == 0 ? nv + 1 : nv;
+        for (int i = wave; i < nd; i += 16) {
+            const double* v = i < nv ? V + (size_t)ldv * i : w;
             double a0 = 0.0, a1 = 0.0;
             int k = lane;
             for (; k + 64 < n; k += 128) {
@@ -1865,9 +1867,13 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
             if (k < n) a0 += v[k] * w[k];
             double acc = a0 + a1;
             for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-            if (lane == 0) h[pass][i] = acc;
+            if (lane == 0) {
+                if (i < nv) h[pass][i] = acc;
+                else wn0 = acc;
+            }
         }
         __syncthreads();
+        if (pass == 0) wn = sqrt(wn0);
         // w -= V h, and |w| after the pass
         double q = 0.0;
         for (int k = tid; k < n; k += 1024) {

@@ -177,6 +177,8 @@ struct ilh_plan {
     ilh::DevBuf X, Y, nx, ny, ds;            // device geometry
     ilh::DevBuf terms[ilh::OP_COUNT];        // per-op term arrays (built lazily)
     bool terms_ok[ilh::OP_COUNT] = {};
+    ilh::DevBuf wins;                         // DDF windows of each point (primal x, y, dual x, y)
+    bool wins_ok = false;
     ilh::RowBins bins[ilh::OP_COUNT][4];     // row bins of scatter ops per component grid
     bool bins_ok[ilh::OP_COUNT] = {};
     ilh::ConvPlan conv[2];                    // primal, dual

@@ -51,20 +51,12 @@ __device__ __forceinline__ double ddf_eval(int kind, double r)
     return a <= 1.0 ? 1.0 - a : 0.0;   // witch hat
 }
 
-// 4-tap window of a DDF centred at fractional index xi: base floor(xi)-1.
+// 4-tap window of a DDF centred at fractional index xi: base floor(xi)-1,
+// weights w[e] = phi(base + e - xi) (w[4] = 0; the difference windows use it).
 struct Win {
     int b;
     double w[5];
 };
-
-__device__ __forceinline__ Win ddf_win(int kind, double xi)
-{
-    Win r;
-    r.b = (int)floor(xi) - 1;
-    for (int e = 0; e < 4; ++e) r.w[e] = ddf_eval(kind, (double)(r.b + e) - xi);
-    r.w[4] = 0.0;
-    return r;
-}
 
 // first difference w'[e] = w[e] - w[e-1] (5 taps); the caller picks the base:
 // same base for curl (out_i = a_i - a_{i-1}), base-1 for divergence
@@ -92,19 +84,38 @@ __device__ __forceinline__ void put(Term& t, int ai, int gk, const Win& x, const
     }
 }
 
-__global__ void k_build_terms(int op, int N, int ddf, Grid g, const double* __restrict__ X,
-                              const double* __restrict__ Y, const double* __restrict__ NXv,
-                              const double* __restrict__ NYv, const double* __restrict__ DS,
-                              Term* __restrict__ out)
+// The four DDF windows of every point (primal x, primal y, dual x, dual y),
+// one weight per thread: 16 threads per point, so the FP64 DDF evaluations
+// (square roots and arcsines) run in parallel rather than in a chain; every
+// operator's terms are then built from these.
+__global__ void k_ddf_windows(int N, int ddf, Grid g, const double* __restrict__ X, const double* __restrict__ Y,
+                              Win* __restrict__ wins)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 16 * N) return;
+    const int p = k >> 4, w = (k >> 2) & 3, e = k & 3;
+    // fractional indices of the point on the primal (x_i = (i+1-I0) dx) and
+    // dual (x_i = (i+1/2-I0) dx) axes, 0-based
+    const double dx = g.dx;
+    const double xp = X[p] / dx - (1.0 - g.I0x), yp = Y[p] / dx - (1.0 - g.I0y);
+    const double xi = (w & 1 ? yp : xp) + (w & 2 ? 0.5 : 0.0);
+    const int b = (int)floor(xi) - 1;
+    Win& out = wins[4 * (size_t)p + w];
+    out.w[e] = ddf_eval(ddf, (double)(b + e) - xi);
+    if (e == 0) {
+        out.b = b;
+        out.w[4] = 0.0;
+    }
+}
+
+__global__ void k_build_terms(int op, int N, Grid g, const Win* __restrict__ wins, const double* __restrict__ NXv,
+                              const double* __restrict__ NYv, const double* __restrict__ DS, Term* __restrict__ out)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const double dx = g.dx;
-    // fractional indices of the point on the primal (x_i = (i+1-I0) dx) and
-    // dual (x_i = (i+1/2-I0) dx) axes, 0-based
-    const double xp = X[p] / dx - (1.0 - g.I0x), yp = Y[p] / dx - (1.0 - g.I0y);
-    const Win Px = ddf_win(ddf, xp), Py = ddf_win(ddf, yp);
-    const Win Dx = ddf_win(ddf, xp + 0.5), Dy = ddf_win(ddf, yp + 0.5);
+    const Win Px = wins[4 * (size_t)p], Py = wins[4 * (size_t)p + 1];
+    const Win Dx = wins[4 * (size_t)p + 2], Dy = wins[4 * (size_t)p + 3];
     const double n_x = NXv[p], n_y = NYv[p], ds = DS[p];
     const double r2 = ds / (dx * dx), r3 = ds / (dx * dx * dx), i1 = 1.0 / dx;
     Term a, b;
@@ -255,9 +266,16 @@ void build_terms(ilh_plan* pl, int op)
     const int T = op_terms_per_point(op);
     pl->terms[op].ensure(sizeof(Term) * (size_t)std::max(1, pl->N * T));
     if (pl->N > 0) {
+        if (!pl->wins_ok) {
+            pl->wins.ensure(sizeof(Win) * 4 * (size_t)pl->N);
+            k_ddf_windows<<<(16 * pl->N + 255) / 256, 256, 0, pl->ctx->stream>>>(
+                pl->N, pl->ddf, pl->g, pl->X.as<double>(), pl->Y.as<double>(), pl->wins.as<Win>());
+            ILH_HIP(hipGetLastError());
+            pl->wins_ok = true;
+        }
         k_build_terms<<<(pl->N + 127) / 128, 128, 0, pl->ctx->stream>>>(
-            op, pl->N, pl->ddf, pl->g, pl->X.as<double>(), pl->Y.as<double>(), pl->nx.as<double>(),
-            pl->ny.as<double>(), pl->ds.as<double>(), pl->terms[op].as<Term>());
+            op, pl->N, pl->g, pl->wins.as<Win>(), pl->nx.as<double>(), pl->ny.as<double>(), pl->ds.as<double>(),
+            pl->terms[op].as<Term>());
         ILH_HIP(hipGetLastError());
     }
     pl->terms_ok[op] = true;
