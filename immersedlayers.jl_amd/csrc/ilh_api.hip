@@ -83,7 +83,7 @@ struct StokesCache {
 // current geometry's merged sources [R, D_s] are rebuilt after every
 // update_points.
 struct MovingCache {
-    DevBuf Pinv;
+    DevBuf P32;                // the reference inverse S_ref^-1 in FP32, row-major (leading dimension solve_ld(N))
     bool have_ref = false;
     int ref_age = 0;           // solves since the reference was factored
     DevBuf terms1, amp, V, w, z, b, fbar, grid;
@@ -1827,6 +1827,59 @@ __global__ void k_gm_matvec(const double* __restrict__ M, int ld, const double* 
     if (lane == 0) y[i] = s;
 }
 
+// dot(row, x) by one wave with the row in FP32 (the preconditioner), widened
+// to FP64: 16-byte row loads (4 values) when the row and x allow them, two in
+// flight per lane, fixed order
+__device__ __forceinline__ double wave_dot_f32(const float* __restrict__ row, const double* __restrict__ x, int m,
+                                               int lane)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k0 = 0;
+    if ((((uintptr_t)row | (uintptr_t)x) & 15) == 0) {
+        const float4* r4 = (const float4*)row;
+        const double2* x2 = (const double2*)x;
+        const int m4 = m >> 2;
+        int k = lane;
+        for (; k + 64 < m4; k += 128) {
+            const float4 a = r4[k], c = r4[k + 64];
+            const double2 b0 = x2[2 * k], b1 = x2[2 * k + 1], d0 = x2[2 * k + 128], d1 = x2[2 * k + 129];
+            s0 += (double)a.x * b0.x + (double)a.y * b0.y;
+            s1 += (double)a.z * b1.x + (double)a.w * b1.y;
+            s2 += (double)c.x * d0.x + (double)c.y * d0.y;
+            s3 += (double)c.z * d1.x + (double)c.w * d1.y;
+        }
+        if (k < m4) {
+            const float4 a = r4[k];
+            const double2 b0 = x2[2 * k], b1 = x2[2 * k + 1];
+            s0 += (double)a.x * b0.x + (double)a.y * b0.y;
+            s1 += (double)a.z * b1.x + (double)a.w * b1.y;
+        }
+        k0 = 4 * m4;
+    }
+    for (int k = k0 + lane; k < m; k += 64) s0 += (double)row[k] * x[k];
+    double t = (s0 + s1) + (s2 + s3);
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    return t;
+}
+
+// y = M x (M in FP32, row-major), or y += M x when accumulate; stops once the
+// iteration has (flags non-null and set)
+__global__ void k_gm_matvec_f32(const float* __restrict__ M, int ld, const double* __restrict__ x,
+                                double* __restrict__ y, int n, const int* __restrict__ flags, int accumulate)
+{
+    if (flags && flags[0]) return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const double s = wave_dot_f32(M + (size_t)ld * i, x, n, lane);
+    if (lane == 0) y[i] = s + (accumulate ? y[i] : 0.0);
+}
+
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        b[k] = (float)a[k];
+}
+
 // Arnoldi step j (one work-group, unless the iteration has stopped): w is
 // orthogonalised against v_0..v_j (columns of V, stride ldv) by classical
 // Gram-Schmidt, repeated once when the first pass removed more than
@@ -2004,21 +2057,34 @@ void ilh::inverse_rowmajor(ilh_plan* pl, double* K, double* Minv, int n)
     ILH_HIP(hipStreamSynchronize(st));
 }
 
-// reference factorisation at the current geometry: Pinv = S^-1, row-major
+// reference factorisation at the current geometry: P = S^-1, row-major, kept
+// in FP32.  The Krylov method (vectors, Gram-Schmidt, the residual test and S
+// itself) stays FP64, so the FP32 preconditioner changes the path but not the
+// accuracy of the solution: S P differs from S S_ref^-1 by about
+// cond(S) 2^-24 (~0.02 at the benchmark's cond 3e5), far less than the
+// geometry change it already absorbs, and it halves the bytes of one of the
+// two dense matvecs per GMRES step.
 static void moving_reference(ilh_plan* pl, MovingCache* mc)
 {
     const int N = pl->N;
-    DevBuf LU;
+    const size_t nl = (size_t)N * solve_ld(N);
+    DevBuf LU, Pd;
     LU.ensure(sizeof(double) * (size_t)N * N);
     build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
-    mc->Pinv.ensure(sizeof(double) * (size_t)N * solve_ld(N));
-    inverse_rowmajor(pl, LU.as<double>(), mc->Pinv.as<double>());
+    Pd.ensure(sizeof(double) * nl);
+    inverse_rowmajor(pl, LU.as<double>(), Pd.as<double>());
+    mc->P32.ensure(sizeof(float) * nl);
+    k_to_f32<<<(unsigned)std::min<size_t>(4096, (nl + 255) / 256), 256, 0, pl->ctx->stream>>>(
+        Pd.as<double>(), mc->P32.as<float>(), nl);
+    ILH_HIP(hipGetLastError());
+    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));   // Pd is released on return
     mc->have_ref = true;
     mc->ref_age = 0;
 }
 
-// GMRES(m) on S x = b, right-preconditioned by Pinv, classical Gram-Schmidt
-// with a conditional re-orthogonalisation; x0 = Pinv b.  The Arnoldi steps,
+// GMRES(m) on S x = b, right-preconditioned by P (the FP32 reference
+// inverse), classical Gram-Schmidt with a conditional re-orthogonalisation;
+// x0 = P b.  The Arnoldi steps,
 // Givens rotations and the residual test run on the device; the host enqueues
 // steps in chunks (the previous solve's count first) and reads back two
 // flags after each chunk, steps after convergence returning at once.  The
@@ -2034,7 +2100,8 @@ static void gm_begin(ilh_plan* pl, MovingCache* mc, const double* b, double* x, 
     mc->gm.ensure(sizeof(double) * L.size());
     mc->gmf.ensure(sizeof(int) * 2);
     double* w = mc->w.as<double>();
-    matvec_rows(st, mc->Pinv.as<double>(), ld, b, x, N, N, 1.0, 0);
+    const unsigned gR = (unsigned)((N + 3) / 4);
+    k_gm_matvec_f32<<<gR, 256, 0, st>>>(mc->P32.as<float>(), ld, b, x, N, nullptr, 0);
     schur_apply(pl, mc, x, w);
     axpby(st, w, b, 1.0, -1.0, N);
     k_gm_init<<<1, 1024, 0, st>>>(w, b, N, mc->V.as<double>(), mc->gm.as<double>(), m, tol, mc->gmf.as<int>());
@@ -2050,7 +2117,8 @@ static void gm_issue(ilh_plan* pl, MovingCache* mc, int j0, int j1, int m)
     double* V = mc->V.as<double>();
     int* flags = mc->gmf.as<int>();
     for (int j = j0; j < j1; ++j) {
-        k_gm_matvec<<<gR, 256, 0, st>>>(mc->Pinv.as<double>(), ld, V + (size_t)ld * j, mc->z.as<double>(), N, flags);
+        k_gm_matvec_f32<<<gR, 256, 0, st>>>(mc->P32.as<float>(), ld, V + (size_t)ld * j, mc->z.as<double>(), N, flags,
+                                            0);
         k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, mc->z.as<double>(), mc->w.as<double>(), N, flags);
         k_gm_arnoldi<<<1, 1024, 0, st>>>(V, ld, j, mc->w.as<double>(), N, mc->gm.as<double>(), m, flags);
     }
@@ -2066,7 +2134,8 @@ static void gm_end(ilh_plan* pl, MovingCache* mc, double* x, int m, int steps)
     if (steps <= 0) return;
     mc->last_iters = steps;
     k_gm_finish<<<1, 1024, 0, st>>>(mc->gm.as<double>(), m, steps, mc->V.as<double>(), ld, N, mc->w.as<double>());
-    matvec_rows(st, mc->Pinv.as<double>(), ld, mc->w.as<double>(), x, N, N, 1.0, 1);
+    k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 256, 0, st>>>(mc->P32.as<float>(), ld, mc->w.as<double>(), x, N,
+                                                             nullptr, 1);
     ILH_HIP(hipGetLastError());
 }
 
@@ -2094,6 +2163,8 @@ static int moving_gmres(ilh_plan* pl, MovingCache* mc, const double* b, double* 
     return steps;
 }
 
+static constexpr int REFACTOR_STEPS = 8;   // step budget floor of a re-solve after refactoring
+
 // Buffers and, on the first call, the reference inverse (which waits for the
 // device); nothing else of the current geometry.
 static MovingCache* moving_setup(ilh_plan* pl, int m)
@@ -2104,7 +2175,9 @@ static MovingCache* moving_setup(ilh_plan* pl, int m)
     if (!pl->mov) pl->mov = new MovingCache();
     MovingCache* mc = pl->mov;
     mc->amp.ensure(sizeof(double) * 2 * (size_t)N);
-    mc->V.ensure(sizeof(double) * (size_t)solve_ld(N) * (m + 1));   // basis columns, stride solve_ld(N)
+    // basis columns, stride solve_ld(N): m + 1, and at least the 9 a re-solve
+    // after a refactorisation may take (moving_refactor_solve)
+    mc->V.ensure(sizeof(double) * (size_t)solve_ld(N) * (std::max(m, REFACTOR_STEPS) + 1));
     for (DevBuf* bb : {&mc->w, &mc->z, &mc->b, &mc->fbar}) bb->ensure(sizeof(double) * (size_t)N);
     mc->grid.ensure(sizeof(double) * ng);
     mc->S.ensure(sizeof(double) * (size_t)N * solve_ld(N));
@@ -2152,10 +2225,17 @@ static void moving_final(ilh_plan* pl, MovingCache* mc, double* d_f, double* d_s
 }
 
 // the solve after a failed GMRES: refactor at the current geometry, solve again
+// The re-solve after refactoring at the current geometry may take up to
+// max(m, REFACTOR_STEPS) steps (the basis is sized for that in
+// moving_setup): its preconditioner is this geometry's inverse rounded to
+// FP32, with which GMRES converges in two or three.
 static int moving_refactor_solve(ilh_plan* pl, MovingCache* mc, double tol, int m)
 {
     moving_reference(pl, mc);
-    const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), mc->amp.as<double>(), tol, m);
+    const int m2 = std::max(m, REFACTOR_STEPS);
+    if (mc->V.bytes < sizeof(double) * (size_t)solve_ld(pl->N) * (m2 + 1))
+        fail(ILH_ERR_STATE, "GMRES basis smaller than the re-solve needs");
+    const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), mc->amp.as<double>(), tol, m2);
     if (it2 < 0) fail(ILH_ERR_SINGULAR, "GMRES did not converge with the current geometry's factor");
     return m + it2;
 }

@@ -217,7 +217,9 @@ def test_moving_body_gmres(ctx):
         print("moving step", k, "iterations", it, rel_err(f.flat(), fr), rel_err(sv.flat(), sr), flush=True)
         assert rel_err(f.flat(), fr) < 1e-10
         assert rel_err(sv.flat(), sr) < 1e-10
-    assert its[0] <= 1                   # the first call factors this geometry: exact preconditioner
+    # the first call factors this geometry: its inverse, rounded to FP32 for
+    # the preconditioner, leaves a perturbation of ~cond(S) 2^-24 for GMRES
+    assert its[0] <= 2
     # a large jump with a one-iteration budget falls back to a refactorisation
     sk = _moved(s0, 0.4, 6 * g.dx, -4 * g.dx)
     cache.update_points(sk)
