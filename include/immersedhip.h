@@ -234,14 +234,16 @@ int ilh_batch_destroy(ilh_batch* batch);
 
 /* Dirichlet solve for moving bodies (problems.jl:259-298 after every
    update_system!, src/system.jl:26-50): same outputs as ilh_dirichlet_solve,
-   but the Schur system of the current geometry is solved by GMRES (at most
-   max_iter iterations, relative residual tol), right-preconditioned by the
-   inverse Schur matrix of a reference geometry that the plan keeps across
-   ilh_plan_update_points; the current geometry's S (create_RTLinvR,
-   src/matrix_operators.jl:9-30) is assembled directly and applied as a dense
-   matvec.  The first call, and any call whose GMRES does not converge,
-   factors the current geometry and makes it the reference.  *iters (may be
-   NULL) receives the GMRES iterations (max_iter + k after a refactorisation). */
+   but the Schur system of the current geometry is solved by FP64 GMRES (at
+   most max_iter iterations, relative residual tol), right-preconditioned by
+   the inverse Schur matrix of a reference geometry that the plan keeps
+   across ilh_plan_update_points, stored in FP32; the current geometry's S
+   (create_RTLinvR, src/matrix_operators.jl:9-30) is assembled directly and
+   applied as a dense matvec.  The first call, and any call whose GMRES does
+   not converge, factors the current geometry and makes it the reference
+   (the re-solve then takes up to max(max_iter, 8) iterations).  *iters (may
+   be NULL) receives the GMRES iterations (max_iter + k after a
+   refactorisation). */
 int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
                                double* s, int mem, double tol, int max_iter, int* iters);
 
