@@ -323,7 +323,10 @@ struct RowSrc {
 };
 
 // values of two cells (a second with c0 == c1 is empty): each cell's
-// contributions in order, loaded 8 per cell at a time
+// contributions in order, loaded 8 per cell at a time.  Every amplitude of a
+// round is read before the first is used (an empty slot reads amplitude 0,
+// which exists whenever a cell does), so the reads are in flight together
+// instead of one memory round trip per contribution.
 template <bool LDS_AMP>
 __device__ __forceinline__ void cell_values(const RowSrc& S, const double* a_lds, int4 ca, int4 cb, double& va,
                                             double& vb)
@@ -333,7 +336,7 @@ __device__ __forceinline__ void cell_values(const RowSrc& S, const double* a_lds
     const int end[2] = {ca.z, cb.z};
     while (e[0] < end[0] || e[1] < end[1]) {
         int a[2][8];
-        double w[2][8];
+        double w[2][8], av[2][8];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -343,10 +346,14 @@ __device__ __forceinline__ void cell_values(const RowSrc& S, const double* a_lds
                 w[h][u] = ok ? S.cw[e[h] + u] : 0.0;
             }
 #pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) av[h][u] = LDS_AMP ? a_lds[a[h][u]] : S.amp[a[h][u]];
+#pragma unroll
         for (int h = 0; h < 2; ++h) {
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                if (e[h] + u < end[h]) acc[h] += w[h][u] * (LDS_AMP ? a_lds[a[h][u]] : S.amp[a[h][u]]);
+                if (e[h] + u < end[h]) acc[h] += w[h][u] * av[h][u];
             e[h] += 8;
         }
     }
@@ -528,14 +535,19 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
     const bool live = c < H;
     double2* x = lds + (size_t)cl * lds_len(P);
     double2 v[V];
+    // all slot lookups first, then all row loads: loads return in order, so
+    // a lookup issued after a row load would wait for that row load too
+    int slot[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int row = io_index(R0, q / R0, q % R0, t, T, P);
+        slot[q] = -1;
+        if (live && row < ny) slot[q] = slot_of_row ? slot_of_row[row] : (row < nrows_in ? row : -1);
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
         double2 val = make_double2(0.0, 0.0);
-        if (live && row < ny) {
-            const int slot = slot_of_row ? slot_of_row[row] : (row < nrows_in ? row : -1);
-            if (slot >= 0) val = T1[tidx(slot, c, H)];
-        }
+        if (slot[q] >= 0) val = T1[tidx(slot[q], c, H)];
         v[q] = val;
     }
     // MODE 0: the spectrum values this thread multiplies by, loaded before the

@@ -97,6 +97,31 @@ def test_headline_parity_and_determinism(ctx):
     assert rel_err(s1.flat(), sr) < 1e-9
 
 
+def test_dirichlet_many_points_reads_amplitudes_from_memory(ctx):
+    """About 5400 surface points (eight circles of radius 0.5 at the headline's
+    1.2 dx spacing) on the 1024^2 grid: more amplitudes than the forward row
+    pass stages in LDS (4096), so the sources read them from memory.  f, which
+    that path produces, is held to 1e-10.  s is held to 2e-8: at this size a
+    random one-ulp perturbation of the oracle's Schur matrix moves the oracle's
+    own s by 5.5e-10 (and f by 1.3e-13), and the GPU assembles and factors S in
+    another summation order (observed 4.7e-9)."""
+    import ilhip
+    from ilhip import geometry as G
+    dx = 4.0 / 1024
+    g = G.GridSpec(1025, 1025, 513, 513, dx)
+    centres = [(cx, cy) for cx in (-1.3, 0.0, 1.3) for cy in (-1.3, 0.0, 1.3) if cx or cy]
+    s = G.concat([G.circle(0.5, 1.2 * dx, cx=cx, cy=cy) for cx, cy in centres])
+    fb, _, _ = G.fourier_data(s.theta(), np.random.default_rng(11))
+    print("many points N", s.N, flush=True)
+    assert s.N > 4096
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    f1, s1 = ilhip.DirichletPoisson(cache).solve(fb)
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    fr, sr = O.dirichlet_solve(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), fb)
+    assert rel_err(f1.flat(), fr) < 1e-10
+    assert rel_err(s1.flat(), sr) < 2e-8
+
+
 def test_moving_geometry_update(ctx):
     import ilhip
     from ilhip import geometry as G
