@@ -461,28 +461,52 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
 #pragma unroll
         for (int q = 0; q < V; ++q) v[q] = x[lx(io_index(R0, q / R0, q % R0, t, T, P))];
     } else {
+        // one loop per source kind (the kind is uniform over the launch), so
+        // that all of a thread's loads are in flight together
         const int j0 = 2 * p;
-        const int nx_src = S.mode == ROW_LGF ? P : S.nx;
+        const bool two = j0 + 1 < nrows_limit;
+        if (S.mode == ROW_DENSE && S.field32) {
+            // FP32 values loaded first, widened once all are in flight
+            const float* ra = S.field32 + (size_t)S.nx * j0;
+            float fa[V], fb[V];
 #pragma unroll
-        for (int q = 0; q < V; ++q) {
-            const int xpos = io_index(R0, q / R0, q % R0, t, T, P);
-            double a = 0.0, b = 0.0;
-            if (live && xpos < nx_src) {
-                if (S.mode == ROW_DENSE && S.field32) {
-                    a = S.field32[xpos + (size_t)S.nx * j0];
-                    if (j0 + 1 < nrows_limit) b = S.field32[xpos + (size_t)S.nx * (j0 + 1)];
-                } else if (S.mode == ROW_DENSE) {
-                    a = S.field[xpos + (size_t)S.nx * j0];
-                    if (j0 + 1 < nrows_limit) b = S.field[xpos + (size_t)S.nx * (j0 + 1)];
-                } else {
-                    const int ia = xpos <= S.Px / 2 ? xpos : S.Px - xpos;
-                    const int ja = j0 <= S.Py / 2 ? j0 : S.Py - j0;
-                    const int jb = (j0 + 1) <= S.Py / 2 ? j0 + 1 : S.Py - (j0 + 1);
-                    a = S.tab[ia + (size_t)S.tmx * ja];
-                    if (j0 + 1 < nrows_limit) b = S.tab[ia + (size_t)S.tmx * jb];
+            for (int q = 0; q < V; ++q) {
+                const int xpos = io_index(R0, q / R0, q % R0, t, T, P);
+                fa[q] = 0.0f;
+                fb[q] = 0.0f;
+                if (live && xpos < S.nx) {
+                    fa[q] = ra[xpos];
+                    if (two) fb[q] = ra[xpos + S.nx];
                 }
             }
-            v[q] = make_double2(a, b);
+#pragma unroll
+            for (int q = 0; q < V; ++q) v[q] = make_double2((double)fa[q], (double)fb[q]);
+        } else if (S.mode == ROW_DENSE) {
+            const double* ra = S.field + (size_t)S.nx * j0;
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const int xpos = io_index(R0, q / R0, q % R0, t, T, P);
+                double a = 0.0, b = 0.0;
+                if (live && xpos < S.nx) {
+                    a = ra[xpos];
+                    if (two) b = ra[xpos + S.nx];
+                }
+                v[q] = make_double2(a, b);
+            }
+        } else {
+            const int ja = j0 <= S.Py / 2 ? j0 : S.Py - j0;
+            const int jb = (j0 + 1) <= S.Py / 2 ? j0 + 1 : S.Py - (j0 + 1);
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const int xpos = io_index(R0, q / R0, q % R0, t, T, P);
+                double a = 0.0, b = 0.0;
+                if (live) {
+                    const int ia = xpos <= S.Px / 2 ? xpos : S.Px - xpos;
+                    a = S.tab[ia + (size_t)S.tmx * ja];
+                    if (two) b = S.tab[ia + (size_t)S.tmx * jb];
+                }
+                v[q] = make_double2(a, b);
+            }
         }
     }
     fft<P, -1, false>(v, x, t, tw);
@@ -630,6 +654,14 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restric
     const bool two = j0 + 1 < ny;
     double2* x = lds + (size_t)c * lds_len(P);
     double2 v[V];
+    // all loads first (in flight together), then the unpacking
+    double4 abq[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+        const int m = io_index(R0, q / R0, q % R0, t, T, P);
+        const int k = m <= P / 2 ? m : P - m;
+        abq[q] = live ? *(const double4*)(T2 + pidx(j0, k == H ? 0 : k, H)) : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int m = io_index(R0, q / R0, q % R0, t, T, P);
@@ -638,7 +670,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restric
             // Z(m) = A(m) + i B(m),  Z(P-m) = conj(A(m)) + i conj(B(m)); column 0
             // holds the real DC (x) and Nyquist (y) values of rows j0, j0+1
             const int k = m <= P / 2 ? m : P - m;
-            const double4 ab = *(const double4*)(T2 + pidx(j0, k == H ? 0 : k, H));
+            const double4 ab = abq[q];
             double Ax = ab.x, Ay = ab.y, Bx = ab.z, By = ab.w;
             if (k == 0) { Ay = 0.0; By = 0.0; }
             if (k == H) { Ax = ab.y; Bx = ab.w; Ay = 0.0; By = 0.0; }
