@@ -727,13 +727,12 @@ __global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int 
     out += out_stride * blockIdx.y;
     const size_t foff = field_stride * blockIdx.y;
     const Term* tp = t + (size_t)p * T;
-    const int ai0 = tp[0].ai;
-    int ai1 = -1;
-    for (int k = 1; k < T; ++k)
-        if (tp[k].ai != ai0) {
-            ai1 = tp[k].ai;
-            break;
-        }
+    // the point's amplitude indices, term k on lane k (T <= 8): ai0 of term 0
+    // and the first that differs from it, in one load round trip
+    const int aik = lane < T ? tp[lane].ai : 0;
+    const int ai0 = __shfl(aik, 0, 64);
+    const unsigned long long dm = __ballot(lane < T && aik != ai0);
+    const int ai1 = dm ? __shfl(aik, __ffsll((long long)dm) - 1, 64) : -1;
     double acc0 = 0.0, acc1 = 0.0;
     for (int e = lane; e < T * 25; e += 64) {
         const int k = e / 25, r = e - 25 * k, b = r / 5, a = r - 5 * b;
@@ -762,6 +761,7 @@ void gather(ilh_plan* pl, const Term* terms, int T, const D* field, size_t field
             size_t out_stride)
 {
     if (pl->N == 0 || nb <= 0) return;
+    if (T < 1 || T > 64) fail(ILH_ERR_ARG, "gather: terms per point must fit one wave");
     GatherFields<D> F;
     const Grid& g = pl->g;
     for (int k = 0; k < GK_COUNT; ++k) {
