@@ -727,8 +727,9 @@ __global__ void __launch_bounds__(256) k_gather(const Term* __restrict__ t, int 
     out += out_stride * blockIdx.y;
     const size_t foff = field_stride * blockIdx.y;
     const Term* tp = t + (size_t)p * T;
-    // the point's amplitude indices, term k on lane k (T <= 8): ai0 of term 0
-    // and the first that differs from it, in one load round trip
+    // the point's amplitude indices, term k on lane k (T <= 64, checked by the
+    // launcher): ai0 of term 0 and the first that differs from it, in one load
+    // round trip
     const int aik = lane < T ? tp[lane].ai : 0;
     const int ai0 = __shfl(aik, 0, 64);
     const unsigned long long dm = __ballot(lane < T && aik != ai0);
