@@ -191,9 +191,11 @@ def test_neumann_solve(setup):
         assert rel_err(s.flat(), sr) < TOL_SOLVE
 
 
-@pytest.mark.parametrize("NX,NY", [(12, 9), (9, 40), (40, 200), (300, 70), (1100, 20), (2100, 12)])
+@pytest.mark.parametrize("NX,NY", [(12, 9), (9, 40), (40, 200), (300, 70), (1100, 20), (2100, 12), (20, 1100),
+                                   (12, 2100)])
 def test_inverse_laplacian_periods(ctx, NX, NY):
-    """Every FFT length 16..8192 and unequal row/column periods (no surface points)."""
+    """Every FFT length 16..8192 for the row passes and for the column pass, and
+    unequal row/column periods (no surface points)."""
     import ilhip as il
     from ilhip.geometry import GridSpec, Surface
     gs = GridSpec(NX, NY, NX // 2, NY // 2, 0.05)
