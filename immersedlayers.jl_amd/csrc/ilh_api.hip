@@ -73,6 +73,7 @@ struct StokesCache {
     DevBuf S2inv, Ssinv;          // row-major inverses, leading dimensions solve_ld(2N), solve_ld(N)
     DevBuf S2;                    // create_CL2invCT row-major (the residual of the refinement step)
     DevBuf Ka;                    // E C L^-2 C^T D_s^symm (2N x 2N, row-major): the slip of psi*
+    DevBuf Cf;                    // the vector surface filter C (2N x 2N, row-major)
     DevBuf sv;                    // surface work: dv, vb, e_phi, w, vprime, sigma (2N each); dvn, dsv, ds (N each)
     DevBuf gE, gE2, gP;           // grid work: edges (2), primal
     DevBuf gDD;                   // two Nodes{Dual} fields back to back (one batched convolution)
@@ -1385,8 +1386,31 @@ void filter_vector(ilh_plan* pl, int power, double* v, double* g)
 }
 }  // namespace
 
-// create_surface_filter of the vector cache: block-diagonal 2N x 2N, the
-// u block from the u edges, the v block from the v edges
+namespace {
+// create_surface_filter of the vector cache into d_C (2N x 2N, column-major,
+// N > 0): block-diagonal, the u block from the u edges, the v block from the
+// v edges
+void build_filter_vector(ilh_plan* pl, double* d_C)
+{
+    const int N = pl->N, n2 = 2 * N;
+    hipStream_t st = pl->ctx->stream;
+    fill_zero(st, d_C, (size_t)n2 * n2);
+    const Term* te = get_terms(pl, OP_E_EDGES);
+    const Term* tr = get_terms(pl, OP_R_EDGES);
+    if (!pl->bins_ok[OP_R_EDGES]) {
+        build_bins(pl, tr, 2 * N, GK_EDGE_U, pl->bins[OP_R_EDGES][0]);
+        build_bins(pl, tr, 2 * N, GK_EDGE_V, pl->bins[OP_R_EDGES][1]);
+        pl->bins_ok[OP_R_EDGES] = true;
+    }
+    for (int c = 0; c < 2; ++c) {
+        const RowBins& b = pl->bins[OP_R_EDGES][c];
+        k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 2, c, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, n2,
+                                                        d_C);
+    }
+    ILH_HIP(hipGetLastError());
+}
+}  // namespace
+
 extern "C" int ilh_surface_filter_build_vector(ilh_plan* pl, double* C, int mem)
 {
     return guarded([&] {
@@ -1395,23 +1419,7 @@ extern "C" int ilh_surface_filter_build_vector(ilh_plan* pl, double* C, int mem)
         const int N = pl->N, n2 = 2 * N;
         const size_t nn = (size_t)n2 * n2;
         double* d_C = io.out(C, nn, pl->io_b);
-        hipStream_t st = pl->ctx->stream;
-        if (N > 0) {
-            fill_zero(st, d_C, nn);
-            const Term* te = get_terms(pl, OP_E_EDGES);
-            const Term* tr = get_terms(pl, OP_R_EDGES);
-            if (!pl->bins_ok[OP_R_EDGES]) {
-                build_bins(pl, tr, 2 * N, GK_EDGE_U, pl->bins[OP_R_EDGES][0]);
-                build_bins(pl, tr, 2 * N, GK_EDGE_V, pl->bins[OP_R_EDGES][1]);
-                pl->bins_ok[OP_R_EDGES] = true;
-            }
-            for (int c = 0; c < 2; ++c) {
-                const RowBins& b = pl->bins[OP_R_EDGES][c];
-                k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 2, c, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny,
-                                                                n2, d_C);
-            }
-            ILH_HIP(hipGetLastError());
-        }
+        if (N > 0) build_filter_vector(pl, d_C);
         io.back(C, d_C, nn);
         io.done();
     });
@@ -1484,7 +1492,7 @@ StokesCache* prepare_stokes(ilh_plan* pl)
         const int N = pl->N, n2 = 2 * N;
         const Grid& g = pl->g;
         sc->sv.ensure(sizeof(double) * (size_t)std::max(1, 6 * n2 + 3 * N));
-        sc->gE.ensure(sizeof(double) * g.n_edges());
+        sc->gE.ensure(sizeof(double) * std::max(g.n_edges(), (size_t)n2));   // also the filter's second iterate
         sc->gE2.ensure(sizeof(double) * g.n_edges());
         sc->gDD.ensure(2 * sizeof(double) * g.n(GK_DUAL));
         sc->gP.ensure(sizeof(double) * g.n(GK_PRIMAL));
@@ -1502,6 +1510,9 @@ StokesCache* prepare_stokes(ilh_plan* pl)
             build_stokes_ka(pl, K.as<double>());
             sc->Ka.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
             transpose(pl->ctx->stream, K.as<double>(), sc->Ka.as<double>(), n2, n2, solve_ld(n2));
+            build_filter_vector(pl, K.as<double>());
+            sc->Cf.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
+            transpose(pl->ctx->stream, K.as<double>(), sc->Cf.as<double>(), n2, n2, solve_ld(n2));
             ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
         }
     } catch (...) {
@@ -1577,7 +1588,14 @@ extern "C" int ilh_stokes_solve(ilh_plan* pl, const double* v_plus, const double
             axpby(st, d_v, gE2, 1.0, -1.0, ne);
             axpby(st, gD2, gD, -1.0, -1.0, nd);
             ILH_HIP(hipMemcpyAsync(d_s, gD2, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
-            if (filter_power > 0) filter_vector(pl, filter_power, sig, gE);
+            // sigma <- C^k sigma with the stored filter matrix (gE holds the
+            // other iterate)
+            double *fa = sig, *fb = gE;
+            for (int k = 0; k < filter_power; ++k) {
+                matvec_rows(st, sc->Cf.as<double>(), ld2, fa, fb, n2, n2, 1.0, 0);
+                std::swap(fa, fb);
+            }
+            if (fa != sig) ILH_HIP(hipMemcpyAsync(sig, fa, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));
         }
         io.back(v, d_v, ne);
         io.back(s, d_s, nd);
