@@ -368,7 +368,8 @@ int op_components(int op, int* gks)
 }
 
 template <class T>
-static void scatter_op_t(ilh_plan* pl, int op, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride)
+static void scatter_op_t(ilh_plan* pl, int op, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride,
+                         const T* base = nullptr, double cb = 1.0)
 {
     const Term* t = get_terms(pl, op);
     int gks[4];
@@ -379,7 +380,7 @@ static void scatter_op_t(ilh_plan* pl, int op, const T* amp, size_t amp_stride, 
     }
     size_t off = 0;
     for (int c = 0; c < nc; ++c) {
-        scatter(pl, pl->bins[op][c], amp, amp_stride, nb, out + off, out_stride);
+        scatter(pl, pl->bins[op][c], amp, amp_stride, nb, out + off, out_stride, base ? base + off : nullptr, cb);
         off += pl->g.n(gks[c]);
     }
 }
@@ -391,6 +392,10 @@ void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride
 }
 
 void scatter_op(ilh_plan* pl, int op, const double* amp, double* out) { scatter_op_t(pl, op, amp, 0, 1, out, 0); }
+void scatter_op_add(ilh_plan* pl, int op, const double* amp, const double* base, double c, double* out)
+{
+    scatter_op_t(pl, op, amp, 0, 1, out, 0, base, c);
+}
 void scatter_op(ilh_plan* pl, int op, const float* amp, float* out) { scatter_op_t(pl, op, amp, 0, 1, out, 0); }
 
 void gather_op_multi(ilh_plan* pl, int op, const double* field, size_t field_stride, int nb, double* out,

@@ -34,7 +34,7 @@ struct ilh_heat {
     int M13 = 0, M23 = 0;                 // filter half-widths
     ilh::DevBuf k13, k23;                 // K(0..M) of exp(dt/3 kL), exp(2dt/3 kL)
     ilh::DevBuf S13, S23, S0;             // row-major inverse Schur matrices
-    ilh::DevBuf F, A, Bt, Y, W1, W2, Rs;  // grid fields
+    ilh::DevBuf F, A, Bt, Y, W1, W2;  // grid fields
     ilh::DevBuf e, sg, r2;                // surface vectors
     bool have_bc = false;
 };
@@ -192,14 +192,6 @@ __global__ void k_heat_y_long(const double* __restrict__ in, int nx, int ny, con
     out[o] = add ? acc + c * add[o] : acc;
 }
 
-// out = x + c y
-__global__ void k_lin(double* __restrict__ out, const double* __restrict__ x, double c, const double* __restrict__ y,
-                      size_t n)
-{
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
-        out[k] = x[k] + c * y[k];
-}
-
 // d = Tb+ - Tb- (scaled by -kappa), r2 = (Tb+ + Tb-)/2
 __global__ void k_heat_bc(const double* __restrict__ tp, const double* __restrict__ tm, int n, double kappa,
                           double* __restrict__ dneg, double* __restrict__ r2)
@@ -212,13 +204,6 @@ __global__ void k_heat_bc(const double* __restrict__ tp, const double* __restric
 }
 
 }  // namespace
-
-static void lin(hipStream_t st, double* out, const double* x, double c, const double* y, size_t n)
-{
-    size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
-    k_lin<<<(unsigned)blocks, 256, 0, st>>>(out, x, c, y, n);
-    ILH_HIP(hipGetLastError());
-}
 
 // out = exp(tau kappa L) (a + ca b) + cp add   (b, add may be null; tmp: one
 // grid field of scratch; out may alias a, b or add)
@@ -297,7 +282,7 @@ extern "C" int ilh_heat_create(ilh_plan* pl, double kappa, double dt, ilh_heat**
             ILH_HIP(hipMemcpyAsync(h->k13.p, K13.data(), sizeof(double) * K13.size(), hipMemcpyHostToDevice, st));
             ILH_HIP(hipMemcpyAsync(h->k23.p, K23.data(), sizeof(double) * K23.size(), hipMemcpyHostToDevice, st));
             const size_t ng = pl->g.n(GK_PRIMAL);
-            for (DevBuf* b : {&h->F, &h->A, &h->Bt, &h->Y, &h->W1, &h->W2, &h->Rs}) b->ensure(sizeof(double) * ng);
+            for (DevBuf* b : {&h->F, &h->A, &h->Bt, &h->Y, &h->W1, &h->W2}) b->ensure(sizeof(double) * ng);
             const int N = pl->N;
             for (DevBuf* b : {&h->e, &h->sg, &h->r2}) b->ensure(sizeof(double) * std::max(1, N));
             if (N > 0) {
@@ -357,17 +342,17 @@ extern "C" int ilh_heat_advance(ilh_heat* h, double* T, int nsteps, int mem)
         const double dt = h->dt;
         const int ld = solve_ld(std::max(1, N));
         double *F = h->F.as<double>(), *A = h->A.as<double>(), *Bt = h->Bt.as<double>(), *Y = h->Y.as<double>();
-        double *W1 = h->W1.as<double>(), *W2 = h->W2.as<double>(), *Rs = h->Rs.as<double>();
+        double *W1 = h->W1.as<double>(), *W2 = h->W2.as<double>();
         double *e = h->e.as<double>(), *sg = h->sg.as<double>(), *r2 = h->r2.as<double>();
         const double* K13 = h->k13.as<double>();
         const double* K23 = h->k23.as<double>();
-        // sigma = -(1/(dt c)) Sinv (E Y - r2);  W = F + R sigma
-        auto multiplier = [&](const double* Yh, const double* Sinv, double c, double* W) {
+        // sigma = -(1/(dt c)) Sinv (E Y - r2);  W = base + cb R sigma (one scatter pass)
+        auto multiplier = [&](const double* Yh, const double* Sinv, double c, const double* base, double cb,
+                              double* W) {
             gather_op(pl, OP_E_PRIMAL, Yh, e);
             axpby(st, e, r2, -1.0, 1.0, N);                                  // e = E Y - r2
             matvec_rows(st, Sinv, ld, e, sg, N, N, -1.0 / (dt * c), 0);
-            scatter_op(pl, OP_R_PRIMAL, sg, Rs);
-            if (W) lin(st, W, F, 1.0, Rs, ng);
+            scatter_op_add(pl, OP_R_PRIMAL, sg, base, cb, W);
         };
         for (int n = 0; n < nsteps; ++n) {
             if (N == 0) {
@@ -378,16 +363,15 @@ extern "C" int ilh_heat_advance(ilh_heat* h, double* T, int nsteps, int mem)
             }
             // stage 2: Yhat2 = H13 (T + dt a21 F)
             heat_apply(pl, K13, h->M13, d_T, dt * A21, F, Bt, Y);
-            multiplier(Y, h->S13.as<double>(), A21, W1);
+            multiplier(Y, h->S13.as<double>(), A21, F, 1.0, W1);
             // stage 3: Yhat3 = H23 [H13 (T + dt a31 W1) + dt a32 F]
             heat_apply(pl, K13, h->M13, d_T, dt * A31, W1, Bt, A, dt * A32, F);
             heat_apply(pl, K23, h->M23, A, 0.0, nullptr, Bt, Y);
-            multiplier(Y, h->S23.as<double>(), A32, W2);
+            multiplier(Y, h->S23.as<double>(), A32, F, 1.0, W2);
             // final: yhat = H23 [H13 T + dt b2 W2] + dt b3 F;  T = yhat + dt b3 R sigma3
             heat_apply(pl, K13, h->M13, d_T, 0.0, nullptr, Bt, A, dt * B2, W2);
             heat_apply(pl, K23, h->M23, A, 0.0, nullptr, Bt, Y, dt * B3, F);
-            multiplier(Y, h->S0.as<double>(), B3, nullptr);
-            lin(st, d_T, Y, dt * B3, Rs, ng);
+            multiplier(Y, h->S0.as<double>(), B3, Y, dt * B3, d_T);
         }
         io.back(T, d_T, ng);
         io.done();

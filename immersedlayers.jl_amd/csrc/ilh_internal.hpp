@@ -219,7 +219,8 @@ void free_bins(RowBins& b);
 // batched over nb operands (grid.y): operand k reads amp + k amp_stride /
 // field + k field_stride and writes out + k out_stride
 template <class T>
-void scatter(ilh_plan* pl, const RowBins& bins, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride);
+void scatter(ilh_plan* pl, const RowBins& bins, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride,
+             const T* base = nullptr, double c = 1.0);
 // field: the operator's whole input (Nodes, Edges or EdgeGradient); each term
 // reads its component grid at Grid::offset(gk)
 template <class T>
@@ -274,6 +275,8 @@ void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double
 // grid, gather E-type ops to the surface; inverse of a dense N x N matrix
 // (overwritten) into row-major Minv with leading dimension solve_ld(N)
 void scatter_op(ilh_plan* pl, int op, const double* amp, double* out);
+// out = base + c (op amp), one pass (base and out grid-sized, distinct)
+void scatter_op_add(ilh_plan* pl, int op, const double* amp, const double* base, double c, double* out);
 void gather_op(ilh_plan* pl, int op, const double* field, double* out);
 // nb operands at once: amplitudes / fields k at a stride, outputs k at a stride
 void scatter_op_multi(ilh_plan* pl, int op, const double* amp, size_t amp_stride, int nb, double* out,

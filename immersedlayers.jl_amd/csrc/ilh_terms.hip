@@ -670,19 +670,22 @@ void free_sources(SourceSet& s)
 
 // --------------------------------------------------------------- scatter
 // out(x, j): zero, except at the row's cells, whose value is the sum of their
-// contributions in (bx, term) order.  A work-group per row: a coalesced zero
-// sweep, then one thread per cell.
+// contributions in (bx, term) order; with a base field, base(x, j) plus c
+// times that.  A work-group per row: a coalesced zero (or base) sweep, then
+// one thread per cell.
 // FP64 (T = double) or FP32 (T = float) data; FP32 values are accumulated
 // in FP64 with the FP64 weights and rounded once.
 template <class T>
 __global__ void k_scatter(const int* __restrict__ cell_ptr, const int4* __restrict__ cells,
                           const int* __restrict__ cai, const double* __restrict__ cw, int nx,
-                          const T* __restrict__ amp, size_t amp_stride, T* __restrict__ out, size_t out_stride)
+                          const T* __restrict__ amp, size_t amp_stride, T* __restrict__ out, size_t out_stride,
+                          const T* __restrict__ base, double c)
 {
     const int j = blockIdx.x;
     amp += amp_stride * blockIdx.y;
     T* row = out + out_stride * blockIdx.y + (size_t)nx * j;
-    for (int x = threadIdx.x; x < nx; x += blockDim.x) row[x] = T(0);
+    const T* brow = base ? base + out_stride * blockIdx.y + (size_t)nx * j : nullptr;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) row[x] = brow ? brow[x] : T(0);
     const int c0 = cell_ptr[j], c1 = cell_ptr[j + 1];
     if (c0 == c1) return;   // uniform over the work-group
     __syncthreads();        // the zeros are ordered before the cell values
@@ -690,21 +693,24 @@ __global__ void k_scatter(const int* __restrict__ cell_ptr, const int4* __restri
         const int4 cr = cells[ci];
         double acc = 0.0;
         for (int e = cr.y; e < cr.z; ++e) acc += (double)amp[cai[e]] * cw[e];
-        row[cr.x] = (T)acc;
+        row[cr.x] = brow ? (T)((double)brow[cr.x] + c * acc) : (T)acc;
     }
 }
 
 template <class T>
-void scatter(ilh_plan* pl, const RowBins& bins, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride)
+void scatter(ilh_plan* pl, const RowBins& bins, const T* amp, size_t amp_stride, int nb, T* out, size_t out_stride,
+             const T* base, double c)
 {
     const int nx = pl->g.nx(bins.gk);
     if (bins.ny <= 0 || nb <= 0) return;
     k_scatter<T><<<dim3(bins.ny, nb), 256, 0, pl->ctx->stream>>>(bins.cell_ptr, bins.cells, bins.cai, bins.cw, nx,
-                                                                 amp, amp_stride, out, out_stride);
+                                                                 amp, amp_stride, out, out_stride, base, c);
     ILH_HIP(hipGetLastError());
 }
-template void scatter<double>(ilh_plan*, const RowBins&, const double*, size_t, int, double*, size_t);
-template void scatter<float>(ilh_plan*, const RowBins&, const float*, size_t, int, float*, size_t);
+template void scatter<double>(ilh_plan*, const RowBins&, const double*, size_t, int, double*, size_t, const double*,
+                              double);
+template void scatter<float>(ilh_plan*, const RowBins&, const float*, size_t, int, float*, size_t, const float*,
+                             double);
 
 // ---------------------------------------------------------------- gather
 template <class T>
