@@ -6,9 +6,44 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 using namespace ilh;
+
+// ------------------------------------------------ allocation-free scratch
+// Device scratch of ilh_dirichlet_solve_noalloc (the surface amplitudes and
+// the two convolution spectra).  Every GPU holds its own copy of the array
+// until the process ends, at its own address, and every call on that GPU
+// shares it.  `scratch` named in host code is a host-side stand-in, not a
+// device address: host code only ever uses the per-device address that
+// hipGetSymbolAddress returned (scratch_ptr), looked up once per device when
+// the device's first context is created.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static constexpr int MAX_DEVICES = 64;
+static void* g_scratch_ptr[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// with `device` current
+static void init_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES) fail(ILH_ERR_ARG, "device index beyond the scratch table");
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    ILH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(ILH_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = p;
+}
+
+static void* scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    void* p = device >= 0 && device < MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
+    if (!p) fail(ILH_ERR_STATE, "no context was created on this device");
+    return p;
+}
 
 // --------------------------------------------------------------- errors
 static thread_local std::string g_err;
@@ -154,6 +189,7 @@ extern "C" int ilh_ctx_create(int device, ilh_ctx** out)
         ILH_HIP(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) fail(ILH_ERR_ARG, "no such HIP device");
         ILH_HIP(hipSetDevice(device));
+        init_scratch(device);
         ilh_ctx* c = new ilh_ctx();
         c->device = device;
         ILH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1028,6 +1064,42 @@ extern "C" int ilh_dirichlet_solve(ilh_plan* pl, const double* fp, const double*
     });
 }
 
+// ilh_dirichlet_solve on device pointers with every work buffer in the
+// device's static scratch array: no allocation of any kind.  The plan's solve
+// data (solve matrices, source tables, LGF spectrum, twiddles) must exist,
+// built by an earlier ilh_dirichlet_solve on this geometry.
+extern "C" int ilh_dirichlet_solve_noalloc(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        ILH_HIP(hipSetDevice(pl->ctx->device));
+        SolveCache* sc = pl->dir;
+        if (!sc) fail(ILH_ERR_STATE, "the plan's Dirichlet solve data are not built: call ilh_dirichlet_solve once first");
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        if (!f || (N > 0 && (!fp || !s))) fail(ILH_ERR_ARG, "null array argument");
+        hipStream_t st = pl->ctx->stream;
+        if (N == 0) {
+            fill_zero(st, f, ng);
+            return;
+        }
+        unsigned char* base = (unsigned char*)scratch_ptr(pl->ctx->device);
+        if (!pl->conv_ok[GK_PRIMAL]) fail(ILH_ERR_STATE, "the plan's LGF spectrum is not built yet");
+        const ConvPlan& C = pl->conv[GK_PRIMAL];
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t b_amp = up(sizeof(double) * 2 * (size_t)N);
+        const size_t b_t1 = up(sizeof(double2) * (size_t)conv_ext_rows1(sc->src1) * C.H);
+        const size_t b_t2 = up(sizeof(double2) * (size_t)conv_ext_rows2(C) * C.H);
+        if (b_amp + b_t1 + b_t2 > NOALLOC_SCRATCH_BYTES)
+            fail(ILH_ERR_ARG, "the allocation-free solve needs " + std::to_string(b_amp + b_t1 + b_t2) +
+                                  " bytes of scratch; it holds at most " + std::to_string(NOALLOC_SCRATCH_BYTES));
+        double* amp = (double*)base;
+        double2* T1 = (double2*)(base + b_amp);
+        double2* T2 = (double2*)(base + b_amp + b_t1);
+        dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), solve_ld(N), fp, fm, amp, s, N);
+        conv_sources_ext(pl, GK_PRIMAL, sc->src1, amp, f, T1, T2);
+    });
+}
 
 // nrhs Dirichlet problems on one geometry (the boundary data of each a column
 // of fb_plus / fb_minus, N x nrhs column-major; f (n_g x nrhs), s (N x nrhs)):

@@ -99,13 +99,24 @@ __global__ void k_stage_tw(double2* tw, int off, int Ns, int R)
     tw[off + e] = make_double2(c, -s);
 }
 
-static const double2* get_twiddles(ilh_ctx* ctx, int P)
+// Set while an allocation-free caller launches: tables are then looked up
+// only, and a missing one fails the call instead of being built.
+static thread_local bool t_lookup_only = false;
+struct LookupOnly {
+    bool prev;
+    LookupOnly() : prev(t_lookup_only) { t_lookup_only = true; }
+    ~LookupOnly() { t_lookup_only = prev; }
+};
+
+static const double2* get_twiddles(ilh_ctx* ctx, int P, bool build = true)
 {
+    build = build && !t_lookup_only;
     // one table per (device, P); immutable once built
     static std::map<std::pair<int, int>, DevBuf*> cache;
     auto key = std::make_pair(ctx->device, P);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second->as<double2>();
+    if (!build) fail(ILH_ERR_STATE, "FFT twiddle table of this period is not built yet");
     if (P < 16 || P > MAXP || (P & (P - 1))) fail(ILH_ERR_ARG, "FFT length must be a power of two in [16, 8192]");
     DevBuf* b = new DevBuf();
     b->ensure(sizeof(double2) * (size_t)(tw_count(P, false) + tw_count(P, true)));
@@ -1124,6 +1135,25 @@ void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double
                    C.T2m.as<double2>(), nullptr, Batch{nb, s1, s2});
     launch_rows_inv(pl->ctx, C.T2m.as<double2>(), C.Px, C.nx, C.ny, out,
                     Batch{nb, s2, (size_t)C.nx * C.ny});
+}
+
+// conv_sources with the two spectra in caller-supplied device buffers (T1:
+// at least conv_ext_rows1(src) rows, T2: conv_ext_rows2 rows, H complex
+// values per row, 8-row tiles) and no allocation: the plan's spectrum, its
+// pinned source pairs and the twiddle table must exist (ILH_ERR_STATE).
+int conv_ext_rows1(const SourceSet& src) { return (2 * src.npairs + 7) / 8 * 8; }
+int conv_ext_rows2(const ConvPlan& C) { return (C.ny + 1 + 7) / 8 * 8; }
+
+void conv_sources_ext(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, double2* T1,
+                      double2* T2)
+{
+    if (!pl->conv_ok[gk]) fail(ILH_ERR_STATE, "the plan's LGF spectrum is not built yet");
+    if (src.npairs_dev) fail(ILH_ERR_STATE, "the source pairs are not pinned");
+    LookupOnly guard;
+    const ConvPlan& C = pl->conv[gk];
+    launch_rows_fwd(pl->ctx, terms_source(C, src, amp), C.Px, src.npairs, C.ny, T1);
+    launch_cols<0>(pl->ctx, T1, src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0, T2, nullptr);
+    launch_rows_inv(pl->ctx, (const double2*)T2, C.Px, C.nx, C.ny, out);
 }
 
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, hipEvent_t* ev)

@@ -268,6 +268,11 @@ void pin_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);
 void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage);
+// conv_sources into caller-supplied spectra buffers, allocating nothing
+int conv_ext_rows1(const SourceSet& src);
+int conv_ext_rows2(const ConvPlan& C);
+void conv_sources_ext(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, double2* T1,
+                      double2* T2);
 void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, size_t amp_stride, int nb,
                         double* out);
 

@@ -210,6 +210,20 @@ int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_mi
 int ilh_dirichlet_solve_multi(ilh_plan* plan, int nrhs, const double* fb_plus, const double* fb_minus, double* f,
                               double* s, int mem);
 
+/* Allocation-free form of ilh_dirichlet_solve (no counterpart in the
+   reference): for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Device pointers only, asynchronous on the context stream; fb_minus may be
+   NULL.  The plan's solve data must already exist (one ilh_dirichlet_solve
+   call on this geometry builds them; ILH_ERR_STATE otherwise).  The work
+   buffers (surface amplitudes and the two convolution spectra) live in one
+   static device array of 32 MiB per GPU, enough for grids up to about
+   1024 x 1024 (larger problems fail with ILH_ERR_ARG).  Every call on a GPU
+   shares that array: calls that overlap on one GPU (other streams or
+   contexts) race.  Results equal ilh_dirichlet_solve's bit for bit. */
+int ilh_dirichlet_solve_noalloc(ilh_plan* plan, const double* fb_plus, const double* fb_minus, double* f,
+                                double* s);
+
 /* ---- batched forms over arrays of plans ---- */
 /* Batched form of ilh_dirichlet_solve: nprob independent plans of one context
    (any mix of grid sizes and point counts), array k of each pointer array
