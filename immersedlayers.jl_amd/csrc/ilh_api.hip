@@ -90,6 +90,8 @@ extern "C" const char* ilh_version(void) { return "immersedhip 0.1 (gfx950, FP64
 // -------------------------------------------------------- solve caches
 struct SolveCache {
     DevBuf LU, piv, info;
+    SymFactor sym;            // create_RTLinvR: the deflated Cholesky of M (S = M diag(ds)) instead of LU
+    bool use_sym = false;
     DevBuf M[4];              // row-major N x N solve matrices
     DevBuf terms1, terms2;    // merged source terms
     SourceSet src1, src2;
@@ -214,7 +216,6 @@ void ilh::ctx_release(ilh_ctx* c)
         }
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    release_blas(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -875,9 +876,26 @@ void build_schur(ilh_plan* pl, int kind, double scale, double* K)
     }
 }
 
+// create_RTLinvR = M diag(ds) with M = -E L^-1 E^T (the (E, E) assembly of
+// the LGF, exactly symmetric): M factored by the deflated Cholesky; false
+// when it meets a non-positive pivot
+bool sym_rtlinvr(ilh_plan* pl, SymFactor& F)
+{
+    const int N = pl->N;
+    F.A.ensure(sizeof(double) * (size_t)N * N);
+    assemble(pl, OP_E_PRIMAL, OP_E_PRIMAL, GK_PRIMAL, -1.0, false, F.A.as<double>());
+    return sym_factor(pl, F, N);
+}
+
 void factor_into(ilh_plan* pl, SolveCache* sc, int kind)
 {
     const int N = schur_dim(pl, kind);
+    if (kind == ILH_SCHUR_RTLINVR && sym_rtlinvr(pl, sc->sym)) {
+        sc->use_sym = true;
+        return;
+    }
+    sc->sym = SymFactor();
+    sc->use_sym = false;
     sc->LU.ensure(sizeof(double) * (size_t)N * N);
     sc->piv.ensure(sizeof(int) * (size_t)std::max(1, N));
     sc->info.ensure(sizeof(int));
@@ -913,14 +931,22 @@ SolveCache* prepare_dirichlet(ilh_plan* pl)
         hipStream_t st = pl->ctx->stream;
         const double dx2 = pl->g.dx * pl->g.dx;
         if (N > 0) {
-            factor_into(pl, sc, ILH_SCHUR_RTLINVR);
             // RHS [K_D - I/2, -K_D - I/2] with K_D = E L^-1 D_s
             DevBuf B;
             B.ensure(sizeof(double) * (size_t)N * 2 * N);
             assemble(pl, OP_E_PRIMAL, OP_DS, GK_PRIMAL, dx2, false, B.as<double>());
             k_rhs_dir<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, st>>>(B.as<double>(), N);
             ILH_HIP(hipGetLastError());
-            lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 2 * N);
+            factor_into(pl, sc, ILH_SCHUR_RTLINVR);
+            if (sc->use_sym) {
+                // S^-1 B = diag(1 / ds) M^-1 B; the factor is not kept
+                sym_solve(pl, sc->sym, B.as<double>(), 2 * N);
+                scale_rows_inv(st, B.as<double>(), N, 2 * N, pl->ds.as<double>(), 1.0);
+                ILH_HIP(hipStreamSynchronize(st));
+                sc->sym = SymFactor();
+            } else {
+                lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 2 * N);
+            }
             const int ld = solve_ld(N);
             for (int k = 0; k < 2; ++k) {
                 sc->M[k].ensure(sizeof(double) * (size_t)N * ld);
@@ -1031,7 +1057,12 @@ extern "C" int ilh_schur_solve(ilh_plan* pl, int kind, double* x, const double* 
             if (d_x != d_b)
                 ILH_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * N, hipMemcpyDeviceToDevice, pl->ctx->stream));
             SolveCache* sc = pl->fac[kind];
-            lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, d_x, 1);
+            if (sc->use_sym) {
+                sym_solve(pl, sc->sym, d_x, 1);
+                scale_rows_inv(pl->ctx->stream, d_x, N, 1, pl->ds.as<double>(), 1.0);
+            } else {
+                lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, d_x, 1);
+            }
         }
         io.back(x, d_x, N);
         io.done();
@@ -2163,11 +2194,19 @@ static void moving_reference(ilh_plan* pl, MovingCache* mc)
 {
     const int N = pl->N;
     const size_t nl = (size_t)N * solve_ld(N);
-    DevBuf LU, Pd;
-    LU.ensure(sizeof(double) * (size_t)N * N);
-    build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
+    DevBuf Pd;
     Pd.ensure(sizeof(double) * nl);
-    inverse_rowmajor(pl, LU.as<double>(), Pd.as<double>());
+    {
+        SymFactor F;
+        if (sym_rtlinvr(pl, F)) {
+            sym_inverse_rowmajor(pl, F, pl->ds.as<double>(), 1.0, Pd.as<double>());
+        } else {
+            DevBuf LU;
+            LU.ensure(sizeof(double) * (size_t)N * N);
+            build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
+            inverse_rowmajor(pl, LU.as<double>(), Pd.as<double>());
+        }
+    }
     mc->P32.ensure(sizeof(float) * nl);
     k_to_f32<<<(unsigned)std::min<size_t>(4096, (nl + 255) / 256), 256, 0, pl->ctx->stream>>>(
         Pd.as<double>(), mc->P32.as<float>(), nl);

@@ -10,10 +10,10 @@
 //
 // LU with partial pivoting (what the reference's `S\s` runs, LAPACK getrf) is
 // blocked right-looking; the trailing updates and the blocked triangular
-// solves run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// solves run on the FP64 matrix cores (dgemm, ilh_chol.hip).  It serves the
+// Schur kinds that are not symmetrised (the Neumann and Stokes matrices) and
+// the fallback of the Cholesky path.
 #include "ilh_internal.hpp"
-
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdlib>
@@ -194,7 +194,9 @@ static void launch_assemble(ilh_plan* pl, int opA, int opB, const double* tab, i
     const int ta = op_terms_per_point(opA), tb = op_terms_per_point(opB);
     if (ta > 2 || tb > 2) fail(ILH_ERR_ARG, "assembly supports at most two terms per point");
     // E and R of one grid kind share their windows (one term per point)
-    const bool sym = !delta && ((opA == OP_E_PRIMAL && opB == OP_R_PRIMAL) || (opA == OP_E_DUAL && opB == OP_R_DUAL));
+    // (E and E of one grid kind: the symmetrised Schur matrices, any kernel)
+    const bool sym = (!delta && ((opA == OP_E_PRIMAL && opB == OP_R_PRIMAL) || (opA == OP_E_DUAL && opB == OP_R_DUAL))) ||
+                     (opA == OP_E_PRIMAL && opB == OP_E_PRIMAL) || (opA == OP_E_DUAL && opB == OP_E_DUAL);
     const int dl = delta ? 1 : 0;
     hipStream_t st = pl->ctx->stream;
     if (rowmajor && sym)
@@ -225,128 +227,13 @@ void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, d
     launch_assemble(pl, opA, opB, tab, Tn, factor, false, K, pl->N, false);
 }
 
-// ----------------------------------------------------------------- GEMM
-// C[M x Nc] -= A[M x K] * B[K x Nc], column-major with leading dims.
-// 64x64 tile per work-group of 4 waves; each wave owns a 32x32 quadrant as
-// 2x2 MFMA 16x16 blocks; K staged through LDS in chunks of 16.
-static constexpr int GT = 64, GK = 16;
-
-__global__ void __launch_bounds__(256) k_gemm_sub(int M, int Nc, int K, const double* __restrict__ A, int lda,
-                                                  const double* __restrict__ B, int ldb, double* __restrict__ C,
-                                                  int ldc)
-{
-    __shared__ double As[GK][GT + 1];   // As[k][m]
-    __shared__ double Bs[GK][GT + 1];   // Bs[k][n]
-    const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    double4_t acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        // A tile: 64 rows x 16 k, column-major source: consecutive threads read consecutive rows
-        for (int e = tid; e < GT * GK; e += 256) {
-            const int m = e % GT, k = e / GT;
-            const int gm = m0 + m, gk = k0 + k;
-            As[k][m] = (gm < M && gk < K) ? A[gm + (size_t)lda * gk] : 0.0;
-        }
-        for (int e = tid; e < GT * GK; e += 256) {
-            const int k = e % GK, n = e / GK;
-            const int gk = k0 + k, gn = n0 + n;
-            Bs[k][n] = (gk < K && gn < Nc) ? B[gk + (size_t)ldb * gn] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GK; kk += 4) {
-            const int kr = kk + (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const double a = As[kr][wm + i * 16 + (lane & 15)];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const double b = Bs[kr][wn + j * 16 + (lane & 15)];
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i][j], 0, 0, 0);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    // D[r][c] of a 16x16 block held by v_mfma_f64_16x16x4f64:
-    // c = lane % 16, r = lane / 16 + 4 * item
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int it = 0; it < 4; ++it) {
-                const int r = m0 + wm + i * 16 + (lane >> 4) + 4 * it;
-                const int c = n0 + wn + j * 16 + (lane & 15);
-                if (r < M && c < Nc) C[r + (size_t)ldc * c] -= acc[i][j][it];
-            }
-}
-
-// Plain FP64 FMA path for the same update (selected by ILH_GEMM=plain).
-__global__ void k_gemm_sub_plain(int M, int Nc, int K, const double* __restrict__ A, int lda,
-                                 const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc)
-{
-    const int r = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int c = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (r >= M || c >= Nc) return;
-    double s = 0.0;
-    for (int k = 0; k < K; ++k) s += A[r + (size_t)lda * k] * B[k + (size_t)ldb * c];
-    C[r + (size_t)ldc * c] -= s;
-}
-
-// GEMM path: 0 = rocBLAS dgemm (default), 1 = the MFMA kernel above,
-// 2 = plain FMA (ILH_GEMM=hip / ILH_GEMM=plain select the hand-written ones)
-static int gemm_path()
-{
-    static int v = -1;
-    if (v < 0) {
-        const char* e = getenv("ILH_GEMM");
-        v = (e && std::string(e) == "hip") ? 1 : (e && std::string(e) == "plain") ? 2 : 0;
-    }
-    return v;
-}
-
-static rocblas_handle blas_handle(ilh_ctx* ctx)
-{
-    if (!ctx->blas) {
-        rocblas_handle h = nullptr;
-        if (rocblas_create_handle(&h) != rocblas_status_success) fail(ILH_ERR_HIP, "rocblas_create_handle failed");
-        ctx->blas = h;
-    }
-    return (rocblas_handle)ctx->blas;
-}
-
-void release_blas(ilh_ctx* ctx)
-{
-    if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
-    ctx->blas = nullptr;
-}
-
 // C[M x Nc] -= A[M x K] B[K x Nc] on stream st (the trailing updates of the
-// blocked LU and of its multi-right-hand-side solves: plain GEMMs)
-static void gemm_sub(ilh_ctx* ctx, hipStream_t st, int M, int Nc, int K, const double* A, int lda, const double* B,
-                     int ldb, double* C, int ldc)
+// blocked LU and of its multi-right-hand-side solves), on the FP64 matrix cores
+static void gemm_sub(hipStream_t st, int M, int Nc, int K, const double* A, int lda, const double* B, int ldb,
+                     double* C, int ldc)
 {
     if (M <= 0 || Nc <= 0 || K <= 0) return;
-    const int path = gemm_path();
-    if (path == 0) {
-        rocblas_handle h = blas_handle(ctx);
-        if (rocblas_set_stream(h, st) != rocblas_status_success) fail(ILH_ERR_HIP, "rocblas_set_stream failed");
-        const double alpha = -1.0, beta = 1.0;
-        if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, M, Nc, K, &alpha, A, lda, B, ldb, &beta,
-                          C, ldc) != rocblas_status_success)
-            fail(ILH_ERR_HIP, "rocblas_dgemm failed");
-    } else if (path == 2) {
-        k_gemm_sub_plain<<<dim3((M + 63) / 64, (Nc + 3) / 4), 256, 0, st>>>(M, Nc, K, A, lda, B, ldb, C, ldc);
-    } else {
-        k_gemm_sub<<<dim3((M + GT - 1) / GT, (Nc + GT - 1) / GT), 256, 0, st>>>(M, Nc, K, A, lda, B, ldb, C, ldc);
-    }
-    ILH_HIP(hipGetLastError());
+    dgemm(st, 0, 0, M, Nc, K, -1.0, A, lda, B, ldb, 1.0, C, ldc);
 }
 
 // ------------------------------------------------------------------- LU
@@ -834,7 +721,7 @@ int lu_factor(ilh_plan* pl, double* A, int n, int* piv, int* info)
         if (rest > 0) {
             k_tri_inv<0><<<1, NB, 0, st>>>(A, n, k0, nb, Tinv);
             k_tri_apply<<<(rest + 63) / 64, 256, 0, st>>>(Tinv, k0, nb, A, n, rest, k1);
-            gemm_sub(pl->ctx, st, rest, rest, nb, A + k1 + (size_t)n * k0, n, A + k0 + (size_t)n * k1, n,
+            gemm_sub(st, rest, rest, nb, A + k1 + (size_t)n * k0, n, A + k0 + (size_t)n * k1, n,
                      A + k1 + (size_t)n * k1, n);
         }
         ILH_HIP(hipGetLastError());
@@ -900,13 +787,13 @@ void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, doubl
         const int nb = std::min(NB, n - k0);
         k_tri_apply<<<(nrhs + 63) / 64, 256, 0, st>>>(Linv + (size_t)NB * NB * (k0 / NB), k0, nb, B, n, nrhs, 0);
         const int k1 = k0 + nb;
-        gemm_sub(pl->ctx, st, n - k1, nrhs, nb, LU + k1 + (size_t)n * k0, n, B + k0, n, B + k1, n);
+        gemm_sub(st, n - k1, nrhs, nb, LU + k1 + (size_t)n * k0, n, B + k0, n, B + k1, n);
     }
     const int last = ((n - 1) / NB) * NB;
     for (int k0 = last; k0 >= 0; k0 -= NB) {
         const int nb = std::min(NB, n - k0);
         k_tri_apply<<<(nrhs + 63) / 64, 256, 0, st>>>(Uinv + (size_t)NB * NB * (k0 / NB), k0, nb, B, n, nrhs, 0);
-        gemm_sub(pl->ctx, st, k0, nrhs, nb, LU + (size_t)n * k0, n, B + k0, n, B, n);
+        gemm_sub(st, k0, nrhs, nb, LU + (size_t)n * k0, n, B + k0, n, B, n);
     }
     ILH_HIP(hipGetLastError());
 }

@@ -227,26 +227,39 @@ static void heat_apply(ilh_plan* pl, const double* K, int M, const double* a, do
 }
 
 // 2-D kernel table K(m)K(n), (M+1)^2, for the Schur assembly
+// The stage Schur matrix E H R = M diag(ds / dx^2) with M = E H E^T symmetric
+// positive definite (H the heat kernel, or the identity): inverted through the
+// deflated Cholesky of M, LU of E H R if that meets a non-positive pivot.
 static void schur_inverse(ilh_plan* pl, const std::vector<double>& K, bool identity, DevBuf& inv)
 {
     const int N = pl->N;
     hipStream_t st = pl->ctx->stream;
-    DevBuf S;
-    S.ensure(sizeof(double) * (size_t)N * N);
-    if (identity) {
-        assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, 1.0, true, S.as<double>());
-    } else {
-        const int T = (int)K.size();
+    DevBuf tabd;
+    int T = 1;
+    if (!identity) {
+        T = (int)K.size();
         std::vector<double> tab((size_t)T * T);
         for (int n = 0; n < T; ++n)
             for (int m = 0; m < T; ++m) tab[m + (size_t)T * n] = K[m] * K[n];
-        DevBuf d;
-        d.ensure(sizeof(double) * tab.size());
-        ILH_HIP(hipMemcpyAsync(d.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st));
-        assemble_table(pl, OP_E_PRIMAL, OP_R_PRIMAL, d.as<double>(), T, 1.0, S.as<double>());
+        tabd.ensure(sizeof(double) * tab.size());
+        ILH_HIP(hipMemcpyAsync(tabd.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st));
         ILH_HIP(hipStreamSynchronize(st));
     }
+    auto build = [&](int opB, double* out) {
+        if (identity) assemble(pl, OP_E_PRIMAL, opB, GK_PRIMAL, 1.0, true, out);
+        else assemble_table(pl, OP_E_PRIMAL, opB, tabd.as<double>(), T, 1.0, out);
+    };
     inv.ensure(sizeof(double) * (size_t)N * solve_ld(N));
+    SymFactor F;
+    F.A.ensure(sizeof(double) * (size_t)N * N);
+    build(OP_E_PRIMAL, F.A.as<double>());
+    if (sym_factor(pl, F, N)) {
+        sym_inverse_rowmajor(pl, F, pl->ds.as<double>(), 1.0 / (pl->g.dx * pl->g.dx), inv.as<double>());
+        return;
+    }
+    DevBuf S;
+    S.ensure(sizeof(double) * (size_t)N * N);
+    build(OP_R_PRIMAL, S.as<double>());
     inverse_rowmajor(pl, S.as<double>(), inv.as<double>());
 }
 

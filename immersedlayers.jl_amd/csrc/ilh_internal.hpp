@@ -155,7 +155,6 @@ struct ilh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cus = 0;
-    void* blas = nullptr;                     // rocBLAS handle (plain GEMMs of the dense factorisations)
     static constexpr int POOL = 4;            // side streams of batched calls (one per hardware queue)
     hipStream_t pool[POOL] = {};
     hipEvent_t ev[POOL + 1] = {};
@@ -298,10 +297,34 @@ void inverse_rowmajor(ilh_plan* pl, double* K, double* Minv, int n = 0);
 // ldk (0: N), or row-major (K[q + ldk p]) when rowmajor
 void assemble(ilh_plan* pl, int opA, int opB, int gk, double scale, bool delta_kernel, double* K, int ldk = 0,
               bool rowmajor = false);
-void release_blas(ilh_ctx* ctx);
 void assemble_table(ilh_plan* pl, int opA, int opB, const double* tab, int Tn, double factor,
                     double* K /* N x N col-major */);
 int  lu_factor(ilh_plan* pl, double* A, int n, int* piv_dev, int* info_dev);
+// C (M x Nc) = beta C + alpha opA(A) opB(B) on the FP64 matrix cores (ilh_chol.hip),
+// column-major, ta/tb = 1 for a transposed operand, beta 0 or 1; lower: only
+// the lower triangle of a square C is needed (ta = 0, tb = 1, beta = 1)
+void dgemm(hipStream_t st, int ta, int tb, int M, int Nc, int K, double alpha, const double* A, int lda,
+           const double* B, int ldb, double beta, double* C, int ldc, bool lower = false);
+// Blocked Cholesky (ilh_chol.hip): A = L L^T in A's lower triangle, the inverses
+// of L's 64 x 64 diagonal blocks in Linv; 0 or the first non-positive pivot
+int  chol_factor(ilh_plan* pl, double* A, int n, int lda, double* Linv, int* info_dev);
+void chol_solve(ilh_plan* pl, const double* L, int n, int lda, const double* Linv, double* X, int ldx, int nrhs);
+// A symmetric matrix M that is positive definite on the vectors orthogonal to
+// the constant one (the symmetrised Schur matrices E K E^T), factored as a
+// Householder-deflated Cholesky (ilh_chol.hip).  The caller fills A (n x n,
+// both triangles) and calls sym_factor; false: a non-positive pivot (the
+// caller falls back to LU).  sym_solve: X <- M^-1 X (n x nrhs, leading dim n).
+struct SymFactor {
+    int n = 0;
+    bool ok = false;
+    DevBuf A, Linv, w, wt, vec, sig, r, info;
+};
+bool sym_factor(ilh_plan* pl, SymFactor& F, int n);
+void sym_solve(ilh_plan* pl, SymFactor& F, double* X, int nrhs);
+// with S = M diag(c d): Minv = S^-1 row-major (leading dimension solve_ld(n)),
+// and X <- diag(1 / (c d)) X (n x ncols)
+void sym_inverse_rowmajor(ilh_plan* pl, SymFactor& F, const double* d, double c, double* Minv);
+void scale_rows_inv(hipStream_t st, double* X, int n, int ncols, const double* d, double c);
 void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, double* B, int nrhs);
 // row-major solve matrices use a leading dimension padded to a multiple of 32
 inline int solve_ld(int n) { return (n + 31) / 32 * 32; }
