@@ -92,6 +92,7 @@ struct SolveCache {
     DevBuf LU, piv, info;
     SymFactor sym;            // create_RTLinvR: the deflated Cholesky of M (S = M diag(ds)) instead of LU
     bool use_sym = false;
+    bool minus_ok = false;    // Dirichlet: M[1] (the interior-data matrix) built
     DevBuf M[4];              // row-major N x N solve matrices
     DevBuf terms1, terms2;    // merged source terms
     SourceSet src1, src2;
@@ -762,6 +763,14 @@ extern "C" int ilh_inverse_laplacian_f32(ilh_plan* pl, int kind, float* out, con
 
 // --------------------------------------------------------------- Schur
 namespace {
+__global__ void k_rhs_sign(double* B, int n, double sign)   // B = sign K - I/2, K in B (n x n)
+{
+    const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= (size_t)n * n) return;
+    const int r = (int)(k % n), c = (int)(k / n);
+    B[k] = sign * B[k] - (r == c ? 0.5 : 0.0);
+}
+
 __global__ void k_rhs_dir(double* B, int n)   // B = [K - I/2, -K - I/2], K in B[:, 0:n]
 {
     const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -921,47 +930,60 @@ void merge_sources(ilh_plan* pl, int op0, int op1, int gk, DevBuf& tbuf, SourceS
     build_source_pairs(pl, src);
 }
 
-SolveCache* prepare_dirichlet(ilh_plan* pl)
+// M[k] = S^-1 (sign K_D - I/2) row-major, K_D = E L^-1 D_s (sign +1 for k = 0,
+// the exterior data; -1 for k = 1, the interior data), with the kept factor
+static void dirichlet_matrix(ilh_plan* pl, SolveCache* sc, int k)
 {
-    if (pl->dir) return pl->dir;
-    need_margin(pl, "Dirichlet solve");
     const int N = pl->N;
-    SolveCache* sc = new SolveCache();
-    try {
-        hipStream_t st = pl->ctx->stream;
-        const double dx2 = pl->g.dx * pl->g.dx;
-        if (N > 0) {
-            // RHS [K_D - I/2, -K_D - I/2] with K_D = E L^-1 D_s
-            DevBuf B;
-            B.ensure(sizeof(double) * (size_t)N * 2 * N);
-            assemble(pl, OP_E_PRIMAL, OP_DS, GK_PRIMAL, dx2, false, B.as<double>());
-            k_rhs_dir<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, st>>>(B.as<double>(), N);
-            ILH_HIP(hipGetLastError());
-            factor_into(pl, sc, ILH_SCHUR_RTLINVR);
-            if (sc->use_sym) {
-                // S^-1 B = diag(1 / ds) M^-1 B; the factor is not kept
-                sym_solve(pl, sc->sym, B.as<double>(), 2 * N);
-                scale_rows_inv(st, B.as<double>(), N, 2 * N, pl->ds.as<double>(), 1.0);
-                ILH_HIP(hipStreamSynchronize(st));
-                sc->sym = SymFactor();
-            } else {
-                lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), 2 * N);
-            }
-            const int ld = solve_ld(N);
-            for (int k = 0; k < 2; ++k) {
-                sc->M[k].ensure(sizeof(double) * (size_t)N * ld);
-                transpose(st, B.as<double>() + (size_t)k * N * N, sc->M[k].as<double>(), N, N, ld);
-            }
-            merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, sc->terms1, sc->src1);
-            pin_source_pairs(pl, sc->src1);
-            ILH_HIP(hipStreamSynchronize(st));
-        }
-        sc->amp1.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));
-    } catch (...) {
-        delete sc;
-        throw;
+    hipStream_t st = pl->ctx->stream;
+    const double dx2 = pl->g.dx * pl->g.dx;
+    DevBuf B;
+    B.ensure(sizeof(double) * (size_t)N * N);
+    assemble(pl, OP_E_PRIMAL, OP_DS, GK_PRIMAL, dx2, false, B.as<double>());
+    k_rhs_sign<<<(unsigned)(((size_t)N * N + 255) / 256), 256, 0, st>>>(B.as<double>(), N, k ? -1.0 : 1.0);
+    ILH_HIP(hipGetLastError());
+    if (sc->use_sym) {
+        // S^-1 B = diag(1 / ds) M^-1 B
+        sym_solve(pl, sc->sym, B.as<double>(), N);
+        scale_rows_inv(st, B.as<double>(), N, N, pl->ds.as<double>(), 1.0);
+    } else {
+        lu_solve_multi(pl, sc->LU.as<double>(), sc->piv.as<int>(), N, B.as<double>(), N);
     }
-    pl->dir = sc;
+    const int ld = solve_ld(N);
+    sc->M[k].ensure(sizeof(double) * (size_t)N * ld);
+    transpose(st, B.as<double>(), sc->M[k].as<double>(), N, N, ld);
+    ILH_HIP(hipStreamSynchronize(st));   // B is released on return
+}
+
+// The plan's Dirichlet solve data: the factor of S, the exterior-data matrix
+// M+ and the merged sources; the interior-data matrix M- is built on the first
+// call that has interior data (need_minus), from the kept factor.
+SolveCache* prepare_dirichlet(ilh_plan* pl, bool need_minus = true)
+{
+    SolveCache* sc = pl->dir;
+    if (!sc) {
+        need_margin(pl, "Dirichlet solve");
+        const int N = pl->N;
+        sc = new SolveCache();
+        try {
+            if (N > 0) {
+                factor_into(pl, sc, ILH_SCHUR_RTLINVR);
+                dirichlet_matrix(pl, sc, 0);
+                merge_sources(pl, OP_R_PRIMAL, OP_DS, GK_PRIMAL, sc->terms1, sc->src1);
+                pin_source_pairs(pl, sc->src1);
+                ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+            }
+            sc->amp1.ensure(sizeof(double) * (size_t)std::max(1, 2 * N));
+        } catch (...) {
+            delete sc;
+            throw;
+        }
+        pl->dir = sc;
+    }
+    if (need_minus && !sc->minus_ok) {
+        if (pl->N > 0) dirichlet_matrix(pl, sc, 1);
+        sc->minus_ok = true;
+    }
     return sc;
 }
 
@@ -1074,7 +1096,7 @@ extern "C" int ilh_dirichlet_solve(ilh_plan* pl, const double* fp, const double*
 {
     return guarded([&] {
         Io io(pl, mem);
-        SolveCache* sc = prepare_dirichlet(pl);
+        SolveCache* sc = prepare_dirichlet(pl, fm != nullptr);
         const int N = pl->N;
         const size_t ng = pl->g.n(GK_PRIMAL);
         hipStream_t st = pl->ctx->stream;
@@ -1106,6 +1128,8 @@ extern "C" int ilh_dirichlet_solve_noalloc(ilh_plan* pl, const double* fp, const
         ILH_HIP(hipSetDevice(pl->ctx->device));
         SolveCache* sc = pl->dir;
         if (!sc) fail(ILH_ERR_STATE, "the plan's Dirichlet solve data are not built: call ilh_dirichlet_solve once first");
+        if (fm && !sc->minus_ok)
+            fail(ILH_ERR_STATE, "the interior-data solve matrix is not built: call ilh_dirichlet_solve once with fb_minus");
         const int N = pl->N;
         const size_t ng = pl->g.n(GK_PRIMAL);
         if (!f || (N > 0 && (!fp || !s))) fail(ILH_ERR_ARG, "null array argument");
@@ -1144,7 +1168,7 @@ extern "C" int ilh_dirichlet_solve_multi(ilh_plan* pl, int nrhs, const double* f
         if (!pl) fail(ILH_ERR_ARG, "null plan");
         if (nrhs < 1 || nrhs > 256) fail(ILH_ERR_ARG, "nrhs must be in [1, 256]");
         Io io(pl, mem);
-        SolveCache* sc = prepare_dirichlet(pl);
+        SolveCache* sc = prepare_dirichlet(pl, fm != nullptr);
         const int N = pl->N;
         const size_t ng = pl->g.n(GK_PRIMAL);
         hipStream_t st = pl->ctx->stream;
@@ -1796,7 +1820,7 @@ extern "C" int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, co
         if (mem != ILH_MEM_HOST && mem != ILH_MEM_DEVICE) fail(ILH_ERR_ARG, "mem must be ILH_MEM_HOST or ILH_MEM_DEVICE");
         ilh_ctx* ctx = batch_ctx(nprob, plans);
         std::vector<SolveCache*> sc(nprob);
-        for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i]);
+        for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i], fb_minus && fb_minus[i]);
         batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, mem);
         if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(ctx->stream));
     });
@@ -1820,7 +1844,7 @@ extern "C" int ilh_batch_create(int nprob, ilh_plan* const* plans, const double*
         if (!fb_plus || !f || !s) fail(ILH_ERR_ARG, "null array of pointers");
         ilh_ctx* ctx = batch_ctx(nprob, plans);
         std::vector<SolveCache*> sc(nprob);
-        for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i]);
+        for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i], fb_minus && fb_minus[i]);
         // one uncaptured pass: every lazily built table and attribute exists
         // before capture (capture must record kernels only)
         batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, ILH_MEM_DEVICE);
@@ -2504,7 +2528,7 @@ extern "C" int ilh_profile_dirichlet(ilh_plan* pl, const double* fp, const doubl
     return guarded([&] {
         Io io(pl, ILH_MEM_DEVICE);
         if (reps < 1 || reps > 1000000) fail(ILH_ERR_ARG, "reps must be in [1, 1000000]");
-        SolveCache* sc = prepare_dirichlet(pl);
+        SolveCache* sc = prepare_dirichlet(pl, fm != nullptr);
         const int N = pl->N;
         if (N == 0) fail(ILH_ERR_ARG, "no surface points");
         hipStream_t st = pl->ctx->stream;

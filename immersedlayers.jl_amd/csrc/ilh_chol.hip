@@ -162,58 +162,145 @@ void dgemm(hipStream_t st, int ta, int tb, int M, int Nc, int K, double alpha, c
 // ------------------------------------------------------------- Cholesky
 static constexpr int CB = 64;   // block size of the factorisation and of the solves
 
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Cholesky of the nb x nb diagonal block at (k0, k0) of A (lower triangle
 // used), written back into A's lower triangle, and the inverse of the factor
-// (CB x CB, column-major, padded with the identity) into Linv.  A pivot that
-// is not positive records its 1-based column in *info (the first one only)
-// and is replaced by 1 so that the kernel still runs to its end.
+// (CB x CB, column-major, padded with the identity) into Linv.
+// Work-group barriers, not arithmetic, set the time of a small factorisation,
+// so the block is processed in 16-wide sub-blocks: one wave factors a 16 x 16
+// diagonal sub-block on its own (lane i holds row i; the pivot and the column
+// of each step are broadcast with readlane, no barrier), then the work-group
+// solves the sub-panel below it and updates the trailing lower triangle.  The
+// inverse is built the same way: the four 16 x 16 diagonal inverses in
+// parallel (one wave each), then the off-diagonal blocks by their distance
+// from the diagonal.  A pivot that is not positive records its 1-based column
+// in *info (the first one only) and is replaced by 1, so that the kernel still
+// runs to its end.
+static constexpr int SB = 16;
+
 __global__ void __launch_bounds__(256) k_chol_diag(double* __restrict__ A, int lda, int k0, int nb,
                                                    double* __restrict__ Linv, int* __restrict__ info)
 {
     __shared__ double T[CB][CB + 1];
-    const int tid = threadIdx.x;
+    __shared__ double X[CB][CB + 1];
+    __shared__ double Y[3][SB][SB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int e = tid; e < CB * CB; e += 256) {
         const int r = e % CB, c = e / CB;
         T[r][c] = (r < nb && c < nb) ? (r >= c ? A[(k0 + r) + (size_t)lda * (k0 + c)] : 0.0) : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        double d = T[j][j];
-        __syncthreads();
-        if (!(d > 0.0)) {
-            if (tid == 0) atomicCAS(info, 0, k0 + j + 1);
-            d = 1.0;
+    for (int o = 0; o < CB; o += SB) {
+        // (a) the 16 x 16 diagonal sub-block, wave 0
+        if (wave == 0) {
+            double d[SB];
+            const int ri = lane < SB ? lane : 0;
+#pragma unroll
+            for (int c = 0; c < SB; ++c) d[c] = T[o + ri][o + c];
+#pragma unroll
+            for (int j = 0; j < SB; ++j) {
+                double piv = readlane_d(d[j], j);
+                if (!(piv > 0.0)) {
+                    if (lane == 0) atomicCAS(info, 0, k0 + o + j + 1);
+                    piv = 1.0;
+                }
+                const double ljj = sqrt(piv), rinv = 1.0 / ljj;
+                const double lij = d[j] * rinv;
+#pragma unroll
+                for (int l = j + 1; l < SB; ++l) d[l] -= lij * readlane_d(lij, l);
+                d[j] = lane > j ? lij : (lane == j ? ljj : 0.0);
+            }
+            if (lane < SB)
+#pragma unroll
+                for (int c = 0; c < SB; ++c) T[o + lane][o + c] = c <= lane ? d[c] : 0.0;
         }
-        const double ljj = sqrt(d), inv = 1.0 / ljj;
-        for (int i = j + 1 + tid; i < nb; i += 256) T[i][j] *= inv;
-        if (tid == 0) T[j][j] = ljj;
         __syncthreads();
-        // trailing lower triangle: rows i, columns l with j < l <= i
-        const int m = nb - j - 1;
-        for (int e = tid; e < m * m; e += 256) {
-            const int i = j + 1 + e % m, l = j + 1 + e / m;
-            if (l <= i) T[i][l] -= T[i][j] * T[l][j];
+        const int rest = CB - o - SB;
+        if (rest > 0) {
+            // (b) sub-panel: rows o+16.. solve p L11^T = a (one thread per row)
+            if (tid < rest) {
+                const int r = o + SB + tid;
+                double p[SB];
+#pragma unroll
+                for (int c = 0; c < SB; ++c) {
+                    double v = T[r][o + c];
+#pragma unroll
+                    for (int k = 0; k < c; ++k) v -= p[k] * T[o + c][o + k];
+                    p[c] = v / T[o + c][o + c];
+                }
+#pragma unroll
+                for (int c = 0; c < SB; ++c) T[r][o + c] = p[c];
+            }
+            __syncthreads();
+            // (c) trailing lower triangle: T[r][s] -= sum_k P[r][k] P[s][k]
+            for (int e = tid; e < rest * rest; e += 256) {
+                const int ri = e % rest, si = e / rest;
+                if (si > ri) continue;
+                const int r = o + SB + ri, sc = o + SB + si;
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < SB; ++k) acc += T[r][o + k] * T[sc][o + k];
+                T[r][sc] -= acc;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     for (int e = tid; e < nb * nb; e += 256) {
         const int r = e % nb, c = e / nb;
         if (r >= c) A[(k0 + r) + (size_t)lda * (k0 + c)] = T[r][c];
     }
-    // inverse of the lower-triangular factor: thread c forms column c
-    if (tid < CB) {
-        const int c = tid;
-        double x[CB];
+    // inverse: the diagonal 16 x 16 inverses, wave b forms block b (lane c < 16: column c)
+    {
+        const int o = wave * SB, c = lane;
+        if (c < SB) {
+            double x[SB];
 #pragma unroll
-        for (int r = 0; r < CB; ++r) {
-            double v = r == c ? 1.0 : 0.0;
+            for (int r = 0; r < SB; ++r) {
+                double v = r == c ? 1.0 : 0.0;
 #pragma unroll
-            for (int k = 0; k < r; ++k) v -= T[r][k] * x[k];
-            x[r] = v / T[r][r];
+                for (int k = 0; k < r; ++k) v -= T[o + r][o + k] * x[k];
+                x[r] = v / T[o + r][o + r];
+            }
+#pragma unroll
+            for (int r = 0; r < SB; ++r) X[o + r][o + c] = x[r];
         }
-#pragma unroll
-        for (int r = 0; r < CB; ++r) Linv[r + CB * c] = x[r];
+        for (int e = tid; e < CB * CB; e += 256) {   // zero the strict upper blocks
+            const int r = e % CB, cc = e / CB;
+            if (r / SB < cc / SB) X[r][cc] = 0.0;
+        }
     }
+    __syncthreads();
+    // off-diagonal blocks (bi, bj), bi - bj = dist: X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj
+    for (int dist = 1; dist < CB / SB; ++dist) {
+        const int nblk = CB / SB - dist;
+        for (int e = tid; e < nblk * SB * SB; e += 256) {
+            const int b = e / (SB * SB), rr = e % SB, cc = (e / SB) % SB;
+            const int bj = b, bi = b + dist;
+            double acc = 0.0;
+            for (int kb = bj; kb < bi; ++kb)
+#pragma unroll
+                for (int k = 0; k < SB; ++k) acc += T[bi * SB + rr][kb * SB + k] * X[kb * SB + k][bj * SB + cc];
+            Y[b][rr][cc] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < nblk * SB * SB; e += 256) {
+            const int b = e / (SB * SB), rr = e % SB, cc = (e / SB) % SB;
+            const int bj = b, bi = b + dist;
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < SB; ++k) acc += X[bi * SB + rr][bi * SB + k] * Y[b][k][cc];
+            X[bi * SB + rr][bj * SB + cc] = -acc;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < CB * CB; e += 256) Linv[e] = X[e % CB][e / CB];
 }
 
 // Blocked right-looking Cholesky of the n x n matrix A (lower triangle,
@@ -243,26 +330,50 @@ int chol_factor(ilh_plan* pl, double* A, int n, int lda, double* Linv, int* info
     return h;
 }
 
+// Triangular solves with the factor of chol_factor, recursive: the halves
+// are split at multiples of CB, so that the leaves are the diagonal blocks
+// (applied as their explicit inverses, in place: the GEMM's single row of
+// tiles reads every entry it overwrites) and the updates between halves are
+// GEMMs with K up to n / 2 (a few passes over X instead of one per block).
+// koff: offset of this diagonal block in L (the leaf's inverse is block koff / CB).
+static void trsm_lower(hipStream_t st, const double* L, int lda, const double* Linv, int koff, int n, double* X,
+                       int ldx, int nrhs)
+{
+    if (n <= CB) {
+        dgemm(st, 0, 0, n, nrhs, n, 1.0, Linv + (size_t)CB * CB * (koff / CB), CB, X, ldx, 0.0, X, ldx, false);
+        return;
+    }
+    const int n1 = ((n + 1) / 2 + CB - 1) / CB * CB;
+    trsm_lower(st, L, lda, Linv, koff, n1, X, ldx, nrhs);
+    // X2 -= L21 X1
+    dgemm(st, 0, 0, n - n1, nrhs, n1, -1.0, L + (koff + n1) + (size_t)lda * koff, lda, X, ldx, 1.0, X + n1, ldx,
+          false);
+    trsm_lower(st, L, lda, Linv, koff + n1, n - n1, X + n1, ldx, nrhs);
+}
+
+static void trsm_lower_t(hipStream_t st, const double* L, int lda, const double* Linv, int koff, int n, double* X,
+                         int ldx, int nrhs)
+{
+    if (n <= CB) {
+        dgemm(st, 1, 0, n, nrhs, n, 1.0, Linv + (size_t)CB * CB * (koff / CB), CB, X, ldx, 0.0, X, ldx, false);
+        return;
+    }
+    const int n1 = ((n + 1) / 2 + CB - 1) / CB * CB;
+    trsm_lower_t(st, L, lda, Linv, koff + n1, n - n1, X + n1, ldx, nrhs);
+    // X1 -= L21^T X2
+    dgemm(st, 1, 0, n1, nrhs, n - n1, -1.0, L + (koff + n1) + (size_t)lda * koff, lda, X + n1, ldx, 1.0, X, ldx,
+          false);
+    trsm_lower_t(st, L, lda, Linv, koff, n1, X, ldx, nrhs);
+}
+
 // X <- (L L^T)^-1 X for the factor of chol_factor; X is n x nrhs with leading
-// dimension ldx.  Diagonal blocks apply their explicit inverses in place
-// (each work-group of the GEMM owns whole columns of the block row).
+// dimension ldx
 void chol_solve(ilh_plan* pl, const double* L, int n, int lda, const double* Linv, double* X, int ldx, int nrhs)
 {
     hipStream_t st = pl->ctx->stream;
-    const int nblk = (n + CB - 1) / CB;
-    for (int b = 0; b < nblk; ++b) {   // L y = x
-        const int k0 = b * CB, nb = std::min(CB, n - k0), k1 = k0 + nb;
-        dgemm(st, 0, 0, nb, nrhs, nb, 1.0, Linv + (size_t)CB * CB * b, CB, X + k0, ldx, 0.0, X + k0, ldx, false);
-        if (n - k1 > 0)
-            dgemm(st, 0, 0, n - k1, nrhs, nb, -1.0, L + k1 + (size_t)lda * k0, lda, X + k0, ldx, 1.0, X + k1, ldx,
-                  false);
-    }
-    for (int b = nblk - 1; b >= 0; --b) {   // L^T x = y
-        const int k0 = b * CB, nb = std::min(CB, n - k0);
-        dgemm(st, 1, 0, nb, nrhs, nb, 1.0, Linv + (size_t)CB * CB * b, CB, X + k0, ldx, 0.0, X + k0, ldx, false);
-        if (k0 > 0)
-            dgemm(st, 1, 0, k0, nrhs, nb, -1.0, L + k0, lda, X + k0, ldx, 1.0, X, ldx, false);
-    }
+    if (n <= 0 || nrhs <= 0) return;
+    trsm_lower(st, L, lda, Linv, 0, n, X, ldx, nrhs);
+    trsm_lower_t(st, L, lda, Linv, 0, n, X, ldx, nrhs);
 }
 
 // --------------------------------------------------- deflated symmetric solve

@@ -194,7 +194,8 @@ int ilh_schur_solve(ilh_plan* plan, int kind, double* x, const double* b, int me
    test/literate/dirichlet.jl:28-44): fb_plus/fb_minus = exterior/interior data
    (prescribed_surface_jump!/average!, src/bc.jl:20-24,47-51); fb_minus may be
    NULL (zero).  Outputs f (Nodes{Primal}) and the unfiltered strength s.
-   The first call on a geometry builds and factors S. */
+   The first call on a geometry builds and factors S and the exterior-data
+   solve matrix; the first call with fb_minus also builds the interior-data one. */
 int ilh_dirichlet_solve(ilh_plan* plan, const double* fb_plus, const double* fb_minus,
                         double* f, double* s, int mem);
 /* Neumann solve of test/literate/neumann.jl:101-142: outputs f (Nodes{Primal}),
@@ -215,7 +216,8 @@ int ilh_dirichlet_solve_multi(ilh_plan* plan, int nrhs, const double* fb_plus, c
    graph capture, tight serving loops, harnesses that forbid hipMalloc).
    Device pointers only, asynchronous on the context stream; fb_minus may be
    NULL.  The plan's solve data must already exist (one ilh_dirichlet_solve
-   call on this geometry builds them; ILH_ERR_STATE otherwise).  The work
+   call on this geometry builds them, one with fb_minus those that fb_minus
+   needs; ILH_ERR_STATE otherwise).  The work
    buffers (surface amplitudes and the two convolution spectra) live in one
    static device array of 32 MiB per GPU, enough for grids up to about
    1024 x 1024 (larger problems fail with ILH_ERR_ARG).  Every call on a GPU

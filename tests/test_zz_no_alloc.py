@@ -74,6 +74,11 @@ def test_noalloc_errors(ctx):
     # before the plan's solve data exist
     with pytest.raises(L.StateError):
         L.call("ilh_dirichlet_solve_noalloc", cache.h, d.ptr, None, f.ptr, sv.ptr)
+    # prepared without interior data: fb_minus needs a preparing call with it
+    L.call("ilh_dirichlet_solve", cache.h, d.ptr, None, f.ptr, sv.ptr, L.MEM_DEVICE)
+    with pytest.raises(L.StateError):
+        L.call("ilh_dirichlet_solve_noalloc", cache.h, d.ptr, d.ptr, f.ptr, sv.ptr)
+    L.call("ilh_dirichlet_solve_noalloc", cache.h, d.ptr, None, f.ptr, sv.ptr)
     with pytest.raises(L.ArgumentError):
         L.call("ilh_dirichlet_solve_noalloc", None, d.ptr, None, f.ptr, sv.ptr)
     # a grid whose spectra exceed the 32 MiB scratch
