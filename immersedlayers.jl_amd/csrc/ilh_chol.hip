@@ -35,7 +35,10 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // (BETA0: C is not read).  opA(A)[m][k] = A[m + lda k] (TA = 0) or
 // A[k + lda m] (TA = 1); opB(B)[k][n] = B[k + ldb n] (TB = 0) or B[n + ldb k]
 // (TB = 1).  LOWER: tiles lying wholly above the diagonal of a square C are
-// skipped (their entries are never read).
+// skipped (their entries are never read).  tri restricts the K range of each
+// tile to where a triangular operand is nonzero (its other triangle must hold
+// zeros): TRI_A_LOWER opA(A)[m][k] = 0 for k > m, TRI_A_UPPER for k < m,
+// TRI_B_LOWER opB(B)[k][n] = 0 for k < n, TRI_B_UPPER for k > n.
 //
 // 64 x 64 tile per work-group of four waves, each wave a 32 x 32 quadrant of
 // 2 x 2 blocks of 16 x 16.  K runs in chunks of 16 through two LDS buffers:
@@ -53,12 +56,18 @@ template <int TA, int TB, bool LOWER, bool BETA0>
 // overwrites: the in-place panel and diagonal-block products below; hence no
 // __restrict__.)
 __global__ void __launch_bounds__(256) k_dgemm(int M, int Nc, int K, double alpha, const double* A, int lda,
-                                               const double* B, int ldb, double* C, int ldc)
+                                               const double* B, int ldb, double* C, int ldc, int tri)
 {
     __shared__ double As[2][GKC][GT + 2];   // As[k][m]
     __shared__ double Bs[2][GKC][GT + 2];   // Bs[k][n]
     const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
     if (LOWER && m0 + GT - 1 < n0) return;
+    int klo = 0, khi = K;
+    if (tri == TRI_A_LOWER) khi = min(K, m0 + GT);
+    else if (tri == TRI_A_UPPER) klo = m0;
+    else if (tri == TRI_B_LOWER) klo = n0;
+    else if (tri == TRI_B_UPPER) khi = min(K, n0 + GT);
+    const int kbase = klo / GKC * GKC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     double ra[4], rb[4];
@@ -69,12 +78,14 @@ __global__ void __launch_bounds__(256) k_dgemm(int M, int Nc, int K, double alph
             if (TA == 0) { m = tid & 63; k = (tid >> 6) + 4 * j; }
             else { k = tid & 15; m = (tid >> 4) + 16 * j; }
             const int gm = m0 + m, gk = k0 + k;
-            ra[j] = (gm < M && gk < K) ? (TA == 0 ? A[gm + (size_t)lda * gk] : A[gk + (size_t)lda * gm]) : 0.0;
+            ra[j] = (gm < M && gk >= klo && gk < khi) ? (TA == 0 ? A[gm + (size_t)lda * gk] : A[gk + (size_t)lda * gm])
+                                                      : 0.0;
             int n, kb;
             if (TB == 0) { kb = tid & 15; n = (tid >> 4) + 16 * j; }
             else { n = tid & 63; kb = (tid >> 6) + 4 * j; }
             const int gn = n0 + n, gkb = k0 + kb;
-            rb[j] = (gn < Nc && gkb < K) ? (TB == 0 ? B[gkb + (size_t)ldb * gn] : B[gn + (size_t)ldb * gkb]) : 0.0;
+            rb[j] = (gn < Nc && gkb >= klo && gkb < khi) ? (TB == 0 ? B[gkb + (size_t)ldb * gn] : B[gn + (size_t)ldb * gkb])
+                                                          : 0.0;
         }
     };
     auto store = [&](int buf) {
@@ -91,13 +102,13 @@ __global__ void __launch_bounds__(256) k_dgemm(int M, int Nc, int K, double alph
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    const int nch = (K + GKC - 1) / GKC;
-    load(0);
+    const int nch = khi > kbase ? (khi - kbase + GKC - 1) / GKC : 0;
+    load(kbase);
     store(0);
     __syncthreads();
     for (int c = 0; c < nch; ++c) {
         const int buf = c & 1;
-        if (c + 1 < nch) load((c + 1) * GKC);
+        if (c + 1 < nch) load(kbase + (c + 1) * GKC);
 #pragma unroll
         for (int kk = 0; kk < GKC; kk += 4) {
             const int kr = kk + (lane >> 4);
@@ -131,14 +142,14 @@ __global__ void __launch_bounds__(256) k_dgemm(int M, int Nc, int K, double alph
 }
 
 void dgemm(hipStream_t st, int ta, int tb, int M, int Nc, int K, double alpha, const double* A, int lda,
-           const double* B, int ldb, double beta, double* C, int ldc, bool lower)
+           const double* B, int ldb, double beta, double* C, int ldc, bool lower, int tri)
 {
     if (M <= 0 || Nc <= 0) return;
     if (beta != 0.0 && beta != 1.0) fail(ILH_ERR_ARG, "dgemm: beta must be 0 or 1");
     const dim3 grid((M + GT - 1) / GT, (Nc + GT - 1) / GT);
     const bool b0 = beta == 0.0;
 #define ILH_DG(TA_, TB_, LO_, B0_) \
-    k_dgemm<TA_, TB_, LO_, B0_><<<grid, 256, 0, st>>>(M, Nc, K, alpha, A, lda, B, ldb, C, ldc)
+    k_dgemm<TA_, TB_, LO_, B0_><<<grid, 256, 0, st>>>(M, Nc, K, alpha, A, lda, B, ldb, C, ldc, tri)
     if (K <= 0) {
         if (!b0) return;
         alpha = 0.0;   // C = 0
@@ -172,7 +183,7 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
 
 // Cholesky of the nb x nb diagonal block at (k0, k0) of A (lower triangle
 // used), written back into A's lower triangle, and the inverse of the factor
-// (CB x CB, column-major, padded with the identity) into Linv.
+// (nb x nb with the zeros of its upper triangle) into Winv (leading dimension ldw).
 // Work-group barriers, not arithmetic, set the time of a small factorisation,
 // so the block is processed in 16-wide sub-blocks: one wave factors a 16 x 16
 // diagonal sub-block on its own (lane i holds row i; the pivot and the column
@@ -186,7 +197,7 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
 static constexpr int SB = 16;
 
 __global__ void __launch_bounds__(256) k_chol_diag(double* __restrict__ A, int lda, int k0, int nb,
-                                                   double* __restrict__ Linv, int* __restrict__ info)
+                                                   double* __restrict__ Winv, int ldw, int* __restrict__ info)
 {
     __shared__ double T[CB][CB + 1];
     __shared__ double X[CB][CB + 1];
@@ -300,80 +311,73 @@ __global__ void __launch_bounds__(256) k_chol_diag(double* __restrict__ A, int l
         }
         __syncthreads();
     }
-    for (int e = tid; e < CB * CB; e += 256) Linv[e] = X[e % CB][e / CB];
+    for (int e = tid; e < nb * nb; e += 256) {
+        const int r = e % nb, c = e / nb;
+        Winv[r + (size_t)ldw * c] = X[r][c];
+    }
 }
 
-// Blocked right-looking Cholesky of the n x n matrix A (lower triangle,
-// leading dimension lda): on return A's lower triangle holds L (A = L L^T)
-// and Linv the inverses of its diagonal blocks (ceil(n / CB) blocks of
-// CB x CB).  Returns 0, or the 1-based column of the first non-positive pivot.
-int chol_factor(ilh_plan* pl, double* A, int n, int lda, double* Linv, int* info_dev)
+// dst[i + ldd j] = src[i + lds j], i < rows, j < cols
+__global__ void k_copy2d(const double* __restrict__ src, int lds, double* __restrict__ dst, int ldd, int rows,
+                         int cols)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < rows && j < cols) dst[i + (size_t)ldd * j] = src[i + (size_t)lds * j];
+}
+
+// Recursive Cholesky of the n x n block at (koff, koff) of A (lower triangle),
+// which also forms W = L^-1 of that block (W's strict upper triangle must hold
+// zeros on entry).  With L = [L11 0; L21 L22] and W = L^-1 = [W11 0; W21 W22]:
+//   L21 = A21 W11^T,  A22 -= L21 L21^T,  W21 = -W22 (L21 W11),
+// halves split at multiples of CB, the leaves the diagonal-block kernel: the
+// work between leaves is large GEMMs on the matrix cores (the triangular
+// factors' zero halves skipped).  T: scratch of n/2 x n/2 at least.
+static void chol_rec(hipStream_t st, double* A, int lda, double* W, int ldw, int koff, int n, double* T, int* info)
+{
+    if (n <= CB) {
+        k_chol_diag<<<1, 256, 0, st>>>(A, lda, koff, n, W + koff + (size_t)ldw * koff, ldw, info);
+        ILH_HIP(hipGetLastError());
+        return;
+    }
+    const int n1 = ((n + 1) / 2 + CB - 1) / CB * CB, n2 = n - n1;
+    chol_rec(st, A, lda, W, ldw, koff, n1, T, info);
+    double* A21 = A + (koff + n1) + (size_t)lda * koff;
+    double* A22 = A + (koff + n1) + (size_t)lda * (koff + n1);
+    const double* W11 = W + koff + (size_t)ldw * koff;
+    double* W21 = W + (koff + n1) + (size_t)ldw * koff;
+    const double* W22 = W + (koff + n1) + (size_t)ldw * (koff + n1);
+    dgemm(st, 0, 1, n2, n1, n1, 1.0, A21, lda, W11, ldw, 0.0, T, n2, false, TRI_B_UPPER);   // L21
+    k_copy2d<<<dim3((n2 + 255) / 256, n1), 256, 0, st>>>(T, n2, A21, lda, n2, n1);
+    ILH_HIP(hipGetLastError());
+    dgemm(st, 0, 1, n2, n2, n1, -1.0, A21, lda, A21, lda, 1.0, A22, lda, true);
+    chol_rec(st, A, lda, W, ldw, koff + n1, n2, T, info);
+    dgemm(st, 0, 0, n2, n1, n1, 1.0, A21, lda, W11, ldw, 0.0, T, n2, false, TRI_B_LOWER);   // L21 W11
+    dgemm(st, 0, 0, n2, n1, n2, -1.0, W22, ldw, T, n2, 0.0, W21, ldw, false, TRI_A_LOWER);
+}
+
+// Cholesky of the n x n matrix A (lower triangle, leading dimension lda):
+// W (n x n, leading dimension n) receives L^-1; A's diagonal blocks hold L's,
+// the rest of A is scratch afterwards.  Returns 0, or the 1-based column of
+// the first non-positive pivot.
+int chol_factor(ilh_plan* pl, double* A, int n, int lda, double* W, double* T, int* info_dev)
 {
     hipStream_t st = pl->ctx->stream;
     ILH_HIP(hipMemsetAsync(info_dev, 0, sizeof(int), st));
-    for (int k0 = 0, b = 0; k0 < n; k0 += CB, ++b) {
-        const int nb = std::min(CB, n - k0), k1 = k0 + nb, rest = n - k1;
-        double* Lb = Linv + (size_t)CB * CB * b;
-        k_chol_diag<<<1, 256, 0, st>>>(A, lda, k0, nb, Lb, info_dev);
-        ILH_HIP(hipGetLastError());
-        if (rest > 0) {
-            // panel: L21 = A21 L11^-T, in place (each work-group owns whole rows)
-            double* A21 = A + k1 + (size_t)lda * k0;
-            dgemm(st, 0, 1, rest, nb, nb, 1.0, A21, lda, Lb, CB, 0.0, A21, lda, false);
-            // trailing lower triangle: A22 -= L21 L21^T
-            dgemm(st, 0, 1, rest, rest, nb, -1.0, A21, lda, A21, lda, 1.0, A + k1 + (size_t)lda * k1, lda, true);
-        }
-    }
+    ILH_HIP(hipMemsetAsync(W, 0, sizeof(double) * (size_t)n * n, st));
+    chol_rec(st, A, lda, W, n, 0, n, T, info_dev);
     int h = 0;
     ILH_HIP(hipMemcpyAsync(&h, info_dev, sizeof(int), hipMemcpyDeviceToHost, st));
     ILH_HIP(hipStreamSynchronize(st));
     return h;
 }
 
-// Triangular solves with the factor of chol_factor, recursive: the halves
-// are split at multiples of CB, so that the leaves are the diagonal blocks
-// (applied as their explicit inverses, in place: the GEMM's single row of
-// tiles reads every entry it overwrites) and the updates between halves are
-// GEMMs with K up to n / 2 (a few passes over X instead of one per block).
-// koff: offset of this diagonal block in L (the leaf's inverse is block koff / CB).
-static void trsm_lower(hipStream_t st, const double* L, int lda, const double* Linv, int koff, int n, double* X,
-                       int ldx, int nrhs)
-{
-    if (n <= CB) {
-        dgemm(st, 0, 0, n, nrhs, n, 1.0, Linv + (size_t)CB * CB * (koff / CB), CB, X, ldx, 0.0, X, ldx, false);
-        return;
-    }
-    const int n1 = ((n + 1) / 2 + CB - 1) / CB * CB;
-    trsm_lower(st, L, lda, Linv, koff, n1, X, ldx, nrhs);
-    // X2 -= L21 X1
-    dgemm(st, 0, 0, n - n1, nrhs, n1, -1.0, L + (koff + n1) + (size_t)lda * koff, lda, X, ldx, 1.0, X + n1, ldx,
-          false);
-    trsm_lower(st, L, lda, Linv, koff + n1, n - n1, X + n1, ldx, nrhs);
-}
-
-static void trsm_lower_t(hipStream_t st, const double* L, int lda, const double* Linv, int koff, int n, double* X,
-                         int ldx, int nrhs)
-{
-    if (n <= CB) {
-        dgemm(st, 1, 0, n, nrhs, n, 1.0, Linv + (size_t)CB * CB * (koff / CB), CB, X, ldx, 0.0, X, ldx, false);
-        return;
-    }
-    const int n1 = ((n + 1) / 2 + CB - 1) / CB * CB;
-    trsm_lower_t(st, L, lda, Linv, koff + n1, n - n1, X + n1, ldx, nrhs);
-    // X1 -= L21^T X2
-    dgemm(st, 1, 0, n1, nrhs, n - n1, -1.0, L + (koff + n1) + (size_t)lda * koff, lda, X + n1, ldx, 1.0, X, ldx,
-          false);
-    trsm_lower_t(st, L, lda, Linv, koff, n1, X, ldx, nrhs);
-}
-
-// X <- (L L^T)^-1 X for the factor of chol_factor; X is n x nrhs with leading
-// dimension ldx
-void chol_solve(ilh_plan* pl, const double* L, int n, int lda, const double* Linv, double* X, int ldx, int nrhs)
+// X <- (L L^T)^-1 X = W^T (W X) (X n x nrhs, leading dimension ldx; T n x nrhs scratch)
+void chol_solve(ilh_plan* pl, const double* W, int n, double* X, int ldx, int nrhs, double* T)
 {
     hipStream_t st = pl->ctx->stream;
     if (n <= 0 || nrhs <= 0) return;
-    trsm_lower(st, L, lda, Linv, 0, n, X, ldx, nrhs);
-    trsm_lower_t(st, L, lda, Linv, 0, n, X, ldx, nrhs);
+    dgemm(st, 0, 0, n, nrhs, n, 1.0, W, n, X, ldx, 0.0, T, n, false, TRI_A_LOWER);
+    dgemm(st, 1, 0, n, nrhs, n, 1.0, W, n, T, n, 0.0, X, ldx, false, TRI_A_UPPER);
 }
 
 // --------------------------------------------------- deflated symmetric solve
@@ -490,10 +494,11 @@ bool sym_factor(ilh_plan* pl, SymFactor& F, int n)
 {
     hipStream_t st = pl->ctx->stream;
     F.n = n;
-    const int m = n - 1, nblk = (std::max(m, 1) + CB - 1) / CB;
+    const int m = n - 1;
     F.w.ensure(sizeof(double) * (size_t)n);
     F.vec.ensure(sizeof(double) * (size_t)2 * n + 8);
-    F.Linv.ensure(sizeof(double) * (size_t)CB * CB * nblk);
+    F.W.ensure(sizeof(double) * (size_t)std::max(m, 1) * std::max(m, 1));
+    F.T.ensure(sizeof(double) * ((size_t)std::max(m, 2) / 2 + CB) * ((size_t)std::max(m, 2) / 2 + CB));
     F.info.ensure(sizeof(int));
     double* A = F.A.as<double>();
     double* w = F.w.as<double>();
@@ -505,14 +510,14 @@ bool sym_factor(ilh_plan* pl, SymFactor& F, int n)
     k_rank2<<<dim3((n + 255) / 256, n), 256, 0, st>>>(A, n, n, w, p);
     ILH_HIP(hipGetLastError());
     if (m > 0) {
-        const int bad = chol_factor(pl, A + 1 + (size_t)n, m, n, F.Linv.as<double>(), F.info.as<int>());
+        const int bad = chol_factor(pl, A + 1 + (size_t)n, m, n, F.W.as<double>(), F.T.as<double>(), F.info.as<int>());
         if (bad) return false;
     }
     // wt = Mt^-1 beta, sigma = alpha - beta . wt
     F.wt.ensure(sizeof(double) * (size_t)std::max(m, 1));
     if (m > 0) {
         ILH_HIP(hipMemcpyAsync(F.wt.p, A + 1, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-        chol_solve(pl, A + 1 + (size_t)n, m, n, F.Linv.as<double>(), F.wt.as<double>(), m, 1);
+        chol_solve(pl, F.W.as<double>(), m, F.wt.as<double>(), m, 1, F.T.as<double>());
     }
     F.sig.ensure(sizeof(double));
     k_sigma<<<1, 1024, 0, st>>>(A, F.wt.as<double>(), m, F.sig.as<double>());
@@ -590,7 +595,10 @@ void sym_solve(ilh_plan* pl, SymFactor& F, double* X, int nrhs)
     ILH_HIP(hipGetLastError());
     const double* A = F.A.as<double>();
     // rows 1.. : Mt^-1 X'[1:, :]
-    if (m > 0) chol_solve(pl, A + 1 + (size_t)n, m, n, F.Linv.as<double>(), X + 1, n, nrhs);
+    if (m > 0) {
+        F.T2.ensure(sizeof(double) * (size_t)m * nrhs);
+        chol_solve(pl, F.W.as<double>(), m, X + 1, n, nrhs, F.T2.as<double>());
+    }
     // first row: y1 = (X'[0, :] - beta^T Mt^-1 X'[1:, :]) / sigma
     k_coldot<<<(nrhs + 3) / 4, 256, 0, st>>>(A + 1, X + 1, m, n, nrhs, r);
     k_first_row<<<(nrhs + 255) / 256, 256, 0, st>>>(X, n, nrhs, r, F.sig.as<double>());

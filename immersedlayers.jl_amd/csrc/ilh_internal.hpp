@@ -303,12 +303,14 @@ int  lu_factor(ilh_plan* pl, double* A, int n, int* piv_dev, int* info_dev);
 // C (M x Nc) = beta C + alpha opA(A) opB(B) on the FP64 matrix cores (ilh_chol.hip),
 // column-major, ta/tb = 1 for a transposed operand, beta 0 or 1; lower: only
 // the lower triangle of a square C is needed (ta = 0, tb = 1, beta = 1)
+// tri: the K range of each tile restricted to a triangular operand's nonzero half
+enum { TRI_NONE = 0, TRI_A_LOWER = 1, TRI_A_UPPER = 2, TRI_B_LOWER = 3, TRI_B_UPPER = 4 };
 void dgemm(hipStream_t st, int ta, int tb, int M, int Nc, int K, double alpha, const double* A, int lda,
-           const double* B, int ldb, double beta, double* C, int ldc, bool lower = false);
-// Blocked Cholesky (ilh_chol.hip): A = L L^T in A's lower triangle, the inverses
-// of L's 64 x 64 diagonal blocks in Linv; 0 or the first non-positive pivot
-int  chol_factor(ilh_plan* pl, double* A, int n, int lda, double* Linv, int* info_dev);
-void chol_solve(ilh_plan* pl, const double* L, int n, int lda, const double* Linv, double* X, int ldx, int nrhs);
+           const double* B, int ldb, double beta, double* C, int ldc, bool lower = false, int tri = TRI_NONE);
+// Recursive Cholesky (ilh_chol.hip): W (n x n) = L^-1 for A = L L^T; 0 or the
+// first non-positive pivot.  chol_solve: X <- W^T W X.
+int  chol_factor(ilh_plan* pl, double* A, int n, int lda, double* W, double* T, int* info_dev);
+void chol_solve(ilh_plan* pl, const double* W, int n, double* X, int ldx, int nrhs, double* T);
 // A symmetric matrix M that is positive definite on the vectors orthogonal to
 // the constant one (the symmetrised Schur matrices E K E^T), factored as a
 // Householder-deflated Cholesky (ilh_chol.hip).  The caller fills A (n x n,
@@ -317,7 +319,7 @@ void chol_solve(ilh_plan* pl, const double* L, int n, int lda, const double* Lin
 struct SymFactor {
     int n = 0;
     bool ok = false;
-    DevBuf A, Linv, w, wt, vec, sig, r, info;
+    DevBuf A, W, T, T2, w, wt, vec, sig, r, info;
 };
 bool sym_factor(ilh_plan* pl, SymFactor& F, int n);
 void sym_solve(ilh_plan* pl, SymFactor& F, double* X, int nrhs);
