@@ -1117,6 +1117,34 @@ extern "C" int ilh_dirichlet_solve(ilh_plan* pl, const double* fp, const double*
     });
 }
 
+// Mixed-precision form: the surface step (solve matrices, matvec) in FP64,
+// the convolution that forms f in FP32
+extern "C" int ilh_dirichlet_solve_mixed(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s,
+                                         int mem)
+{
+    return guarded([&] {
+        Io io(pl, mem);
+        SolveCache* sc = prepare_dirichlet(pl, fm != nullptr);
+        const int N = pl->N;
+        const size_t ng = pl->g.n(GK_PRIMAL);
+        hipStream_t st = pl->ctx->stream;
+        const double* d_fp = io.in(fp, N, pl->io_a);
+        const double* d_fm = fm ? io.in(fm, N, pl->io_c) : nullptr;
+        double* d_f = io.out(f, ng, pl->io_b);
+        double* d_s = io.out(s, N, pl->io_d);
+        if (N == 0) {
+            fill_zero(st, d_f, ng);
+        } else {
+            dirichlet_amp(st, sc->M[0].as<double>(), sc->M[1].as<double>(), solve_ld(N), d_fp, d_fm,
+                          sc->amp1.as<double>(), d_s, N);
+            conv_sources_f32(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), d_f);
+        }
+        io.back(f, d_f, ng);
+        io.back(s, d_s, N);
+        io.done();
+    });
+}
+
 // ilh_dirichlet_solve on device pointers with every work buffer in the
 // device's static scratch array: no allocation of any kind.  The plan's solve
 // data (solve matrices, source tables, LGF spectrum, twiddles) must exist,

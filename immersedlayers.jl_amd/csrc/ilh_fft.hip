@@ -132,46 +132,94 @@ static const double2* get_twiddles(ilh_ctx* ctx, int P, bool build = true)
     return b->as<double2>();
 }
 
-// ---------------------------------------------------------- butterflies
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+// The FP32 transforms' twiddles: the FP64 table rounded once.
+__global__ void k_tw_to_f32(const double2* __restrict__ a, float2* __restrict__ b, int n)
 {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = make_float2((float)a[e].x, (float)a[e].y);
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-template <int S>   // multiply by S*i
-__device__ __forceinline__ double2 mul_si(double2 a) { return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x); }
 
-template <int S>
-__device__ __forceinline__ void dft2(double2& a, double2& b)
+static const float2* get_twiddles_f32(ilh_ctx* ctx, int P, bool build = true)
 {
-    const double2 t = a;
+    build = build && !t_lookup_only;
+    static std::map<std::pair<int, int>, DevBuf*> cache;
+    auto key = std::make_pair(ctx->device, P);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second->as<float2>();
+    if (!build) fail(ILH_ERR_STATE, "FFT twiddle table of this period is not built yet");
+    const double2* d = get_twiddles(ctx, P);
+    const int n = tw_count(P, false) + tw_count(P, true);
+    DevBuf* b = new DevBuf();
+    b->ensure(sizeof(float2) * (size_t)n);
+    k_tw_to_f32<<<(n + 255) / 256, 256, 0, ctx->stream>>>(d, b->as<float2>(), n);
+    ILH_HIP(hipGetLastError());
+    ILH_HIP(hipStreamSynchronize(ctx->stream));
+    cache[key] = b;
+    return b->as<float2>();
+}
+
+template <class R> struct Cplx;
+template <> struct Cplx<double> {
+    using C2 = double2;
+    using Q4 = double4;
+};
+template <> struct Cplx<float> {
+    using C2 = float2;
+    using Q4 = float4;
+};
+template <class R> static const typename Cplx<R>::C2* twiddles(ilh_ctx* ctx, int P);
+template <> const double2* twiddles<double>(ilh_ctx* ctx, int P) { return get_twiddles(ctx, P); }
+template <> const float2* twiddles<float>(ilh_ctx* ctx, int P) { return get_twiddles_f32(ctx, P); }
+
+template <class C2> __device__ __forceinline__ C2 mk2(decltype(C2().x) a, decltype(C2().x) b)
+{
+    C2 r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+
+// ---------------------------------------------------------- butterflies
+template <class C2> __device__ __forceinline__ C2 cmul(C2 a, C2 b)
+{
+    return mk2<C2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class C2> __device__ __forceinline__ C2 cadd(C2 a, C2 b) { return mk2<C2>(a.x + b.x, a.y + b.y); }
+template <class C2> __device__ __forceinline__ C2 csub(C2 a, C2 b) { return mk2<C2>(a.x - b.x, a.y - b.y); }
+template <int S, class C2>   // multiply by S*i
+__device__ __forceinline__ C2 mul_si(C2 a) { return S > 0 ? mk2<C2>(-a.y, a.x) : mk2<C2>(a.y, -a.x); }
+
+template <int S, class C2>
+__device__ __forceinline__ void dft2(C2& a, C2& b)
+{
+    const C2 t = a;
     a = cadd(t, b);
     b = csub(t, b);
 }
 
-template <int S>
-__device__ __forceinline__ void dft4(double2& a0, double2& a1, double2& a2, double2& a3)
+template <int S, class C2>
+__device__ __forceinline__ void dft4(C2& a0, C2& a1, C2& a2, C2& a3)
 {
-    const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
-    const double2 t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+    const C2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    const C2 t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
     a0 = cadd(t0, t2);
     a2 = csub(t0, t2);
     a1 = cadd(t1, t3);
     a3 = csub(t1, t3);
 }
 
-template <int S>
-__device__ __forceinline__ void dft8(double2* a)
+template <int S, class C2>
+__device__ __forceinline__ void dft8(C2* a)
 {
-    const double c = 0.70710678118654752440084436210485;
-    double2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
-    double2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    using R = decltype(C2().x);
+    const R c = (R)0.70710678118654752440084436210485;
+    C2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    C2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
     dft4<S>(e0, e1, e2, e3);
     dft4<S>(o0, o1, o2, o3);
-    o1 = cmul(o1, make_double2(c, S * c));
+    o1 = cmul(o1, mk2<C2>(c, S * c));
     o2 = mul_si<S>(o2);
-    o3 = cmul(o3, make_double2(-c, S * c));
+    o3 = cmul(o3, mk2<C2>(-c, S * c));
     a[0] = cadd(e0, o0);
     a[4] = csub(e0, o0);
     a[1] = cadd(e1, o1);
@@ -182,8 +230,8 @@ __device__ __forceinline__ void dft8(double2* a)
     a[7] = csub(e3, o3);
 }
 
-template <int R, int S>
-__device__ __forceinline__ void dft(double2* a)
+template <int R, int S, class C2>
+__device__ __forceinline__ void dft(C2* a)
 {
     if constexpr (R == 8) dft8<S>(a);
     else if constexpr (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
@@ -208,12 +256,12 @@ __device__ __forceinline__ int io_index(int R, int q, int r, int t, int T, int P
     return t + q * T + r * (P / R);
 }
 
-template <int P, bool REV, int s>
-__device__ __forceinline__ void tw_load(double2 (&w)[V - 1], int t, const double2* __restrict__ tw)
+template <int P, bool REV, int s, class C2>
+__device__ __forceinline__ void tw_load(C2 (&w)[V - 1], int t, const C2* __restrict__ tw)
 {
     constexpr int R = radix(P, REV, s), Ns = nsub(P, REV, s), T = P / V;
     if constexpr (Ns > 1) {
-        const double2* base = tw + tw_off(P, REV, s);
+        const C2* base = tw + tw_off(P, REV, s);
 #pragma unroll
         for (int q = 0; q < V / R; ++q) {
             const int k = (t + q * T) & (Ns - 1);
@@ -223,8 +271,8 @@ __device__ __forceinline__ void tw_load(double2 (&w)[V - 1], int t, const double
     }
 }
 
-template <int P, int S, bool REV, int s>
-__device__ __forceinline__ void st_compute(double2 (&v)[V], const double2 (&w)[V - 1])
+template <int P, int S, bool REV, int s, class C2>
+__device__ __forceinline__ void st_compute(C2 (&v)[V], const C2 (&w)[V - 1])
 {
     constexpr int R = radix(P, REV, s), Ns = nsub(P, REV, s);
 #pragma unroll
@@ -232,7 +280,7 @@ __device__ __forceinline__ void st_compute(double2 (&v)[V], const double2 (&w)[V
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                double2 ww = w[q * (R - 1) + r - 1];
+                C2 ww = w[q * (R - 1) + r - 1];
                 if (S > 0) ww.y = -ww.y;
                 v[q * R + r] = cmul(v[q * R + r], ww);
             }
@@ -241,8 +289,8 @@ __device__ __forceinline__ void st_compute(double2 (&v)[V], const double2 (&w)[V
     }
 }
 
-template <int P, bool REV, int s>
-__device__ __forceinline__ void st_store(const double2 (&v)[V], double2* x, int t)
+template <int P, bool REV, int s, class C2>
+__device__ __forceinline__ void st_store(const C2 (&v)[V], C2* x, int t)
 {
     constexpr int R = radix(P, REV, s), Ns = nsub(P, REV, s), T = P / V;
 #pragma unroll
@@ -254,8 +302,8 @@ __device__ __forceinline__ void st_store(const double2 (&v)[V], double2* x, int 
     }
 }
 
-template <int P, int R>
-__device__ __forceinline__ void st_load(double2 (&v)[V], const double2* x, int t)
+template <int P, int R, class C2>
+__device__ __forceinline__ void st_load(C2 (&v)[V], const C2* x, int t)
 {
     constexpr int T = P / V;
 #pragma unroll
@@ -268,9 +316,9 @@ struct NoHook {
     __device__ void operator()() const {}
 };
 
-template <int P, int S, bool REV, int s, class Pre>
-__device__ __forceinline__ void fft_from(double2 (&v)[V], double2 (&w)[V - 1], double2* x, int t,
-                                         const double2* __restrict__ tw, Pre& pre)
+template <int P, int S, bool REV, int s, class Pre, class C2>
+__device__ __forceinline__ void fft_from(C2 (&v)[V], C2 (&w)[V - 1], C2* x, int t, const C2* __restrict__ tw,
+                                         Pre& pre)
 {
     st_compute<P, S, REV, s>(v, w);
     if constexpr (s + 1 < n_stages(P)) {
@@ -290,11 +338,10 @@ __device__ __forceinline__ void fft_from(double2 (&v)[V], double2 (&w)[V - 1], d
 // `x` is the transform's LDS exchange buffer; the caller synchronises the
 // work-group before the first exchange if it used the buffer.  `pre` runs
 // before the last stage (loads issued there overlap its butterflies).
-template <int P, int S, bool REV, class Pre = NoHook>
-__device__ __forceinline__ void fft(double2 (&v)[V], double2* x, int t, const double2* __restrict__ tw,
-                                    Pre pre = Pre())
+template <int P, int S, bool REV, class Pre = NoHook, class C2>
+__device__ __forceinline__ void fft(C2 (&v)[V], C2* x, int t, const C2* __restrict__ tw, Pre pre = Pre())
 {
-    double2 w[V - 1];
+    C2 w[V - 1];
     fft_from<P, S, REV, 0>(v, w, x, t, tw, pre);
 }
 
@@ -398,12 +445,16 @@ __device__ __forceinline__ size_t pidx(int row, int c, int H)
 // Transform c of block b handles row pair p = b NB + c (field rows 2p, 2p+1,
 // or pinfo[p].x, +1) and writes the two half spectra to rows 2p, 2p+1 of the
 // tiled Tout (H = P/2 columns, column 0 = (DC, Nyquist)).
-template <int P>
-__global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2* __restrict__ tw, int npairs,
-                                                        int nrows_limit, double2* __restrict__ Tout,
-                                                        size_t amp_stride, size_t out_stride)
+template <int P, class R>
+__global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const typename Cplx<R>::C2* __restrict__ tw,
+                                                        int npairs, int nrows_limit,
+                                                        typename Cplx<R>::C2* __restrict__ Tout, size_t amp_stride,
+                                                        size_t out_stride)
 {
-    extern __shared__ double2 lds[];
+    using C2 = typename Cplx<R>::C2;
+    using Q4 = typename Cplx<R>::Q4;
+    extern __shared__ double2 lds_raw[];
+    C2* lds = (C2*)lds_raw;
     // right-hand side blockIdx.y of a batch: its amplitudes (or dense field)
     // and output spectra
     if (S.mode == ROW_DENSE) {
@@ -426,12 +477,12 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
     }
     const int p = xcd_block_of(blockIdx.x, nblk) * NB + c;
     const bool live = p < npairs;
-    double2* x = lds + (size_t)c * lds_len(P);
-    double2 v[V];
+    C2* x = lds + (size_t)c * lds_len(P);
+    C2 v[V];
     if (S.mode == ROW_TERMS) {
         // the packed pair (row j0 -> real part, row j0+1 -> imaginary part) in
         // LDS: zeros, then the rows' nonzero cells
-        double* a_lds = (double*)(lds + (size_t)NB * lds_len(P));
+        double* a_lds = (double*)(lds + (size_t)NB * lds_len(P));   // 16-byte aligned (NB * P slots)
         const int4 pi = live ? S.pinfo[p] : make_int4(0, 0, 0, 0);
         // the first round of cells, in flight across the amplitude staging
         const int ci0 = pi.y + t;
@@ -453,7 +504,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
                 }
             }
         }
-        for (int i = t; i < P; i += T) x[lx(i)] = make_double2(0.0, 0.0);
+        for (int i = t; i < P; i += T) x[lx(i)] = mk2<C2>(0, 0);
         __syncthreads();
         for (int ci = ci0; ci < pi.w; ci += 2 * T) {
             const int4 ca = ci == ci0 ? ca0 : S.cells[ci];
@@ -461,11 +512,11 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
             double va, vb;
             if (S.amp_lds) cell_values<true>(S, a_lds, ca, cb, va, vb);
             else cell_values<false>(S, a_lds, ca, cb, va, vb);
-            if (ci < pi.z) x[lx(ca.x)].x = va;
-            else x[lx(ca.x)].y = va;
+            if (ci < pi.z) x[lx(ca.x)].x = (R)va;
+            else x[lx(ca.x)].y = (R)va;
             if (ci + T < pi.w) {
-                if (ci + T < pi.z) x[lx(cb.x)].x = vb;
-                else x[lx(cb.x)].y = vb;
+                if (ci + T < pi.z) x[lx(cb.x)].x = (R)vb;
+                else x[lx(cb.x)].y = (R)vb;
             }
         }
         __syncthreads();
@@ -491,7 +542,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
                 }
             }
 #pragma unroll
-            for (int q = 0; q < V; ++q) v[q] = make_double2((double)fa[q], (double)fb[q]);
+            for (int q = 0; q < V; ++q) v[q] = mk2<C2>((R)fa[q], (R)fb[q]);
         } else if (S.mode == ROW_DENSE) {
             const double* ra = S.field + (size_t)S.nx * j0;
 #pragma unroll
@@ -502,7 +553,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
                     a = ra[xpos];
                     if (two) b = ra[xpos + S.nx];
                 }
-                v[q] = make_double2(a, b);
+                v[q] = mk2<C2>((R)a, (R)b);
             }
         } else {
             const int ja = j0 <= S.Py / 2 ? j0 : S.Py - j0;
@@ -516,7 +567,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
                     a = S.tab[ia + (size_t)S.tmx * ja];
                     if (two) b = S.tab[ia + (size_t)S.tmx * jb];
                 }
-                v[q] = make_double2(a, b);
+                v[q] = mk2<C2>((R)a, (R)b);
             }
         }
     }
@@ -530,16 +581,23 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
 #pragma unroll
     for (int i = 0; i < H / T; ++i) {
         const int k = t + i * T;
-        const double2 z = x[lx(k)];
-        double4 ab;
+        const C2 z = x[lx(k)];
+        Q4 ab;
         if (k == 0) {
-            const double2 zn = x[lx(P / 2)];
-            ab = make_double4(z.x, zn.x, z.y, zn.y);
+            const C2 zn = x[lx(P / 2)];
+            ab.x = z.x;
+            ab.y = zn.x;
+            ab.z = z.y;
+            ab.w = zn.y;
         } else {
-            const double2 zc = x[lx(P - k)];
-            ab = make_double4(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y), 0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+            const C2 zc = x[lx(P - k)];
+            const R h = (R)0.5;
+            ab.x = h * (z.x + zc.x);
+            ab.y = h * (z.y - zc.y);
+            ab.z = h * (z.y + zc.y);
+            ab.w = -h * (z.x - zc.x);
         }
-        *(double4*)(Tout + tidx(2 * p, k, H)) = ab;
+        *(Q4*)(Tout + tidx(2 * p, k, H)) = ab;
     }
 }
 
@@ -551,14 +609,20 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const double2*
 // kx = c < H and kx = H (the Nyquist column, at c = H).
 // MODE 0: convolve, write rows 0..ny-1 of T2 (2-row tiles).  MODE 1: forward only, write
 // the spectrum * scale (the LGF spectrum build).
-template <int P, int MODE>
-__global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(4))) k_cols(const double2* __restrict__ T1, const int* __restrict__ slot_of_row,
-                                                    const double2* __restrict__ tw, int H, int ny, int nrows_in,
-                                                    const double* __restrict__ spec, double scale,
-                                                    double2* __restrict__ T2, double* __restrict__ spec_out,
-                                                    size_t t1_stride, size_t t2_stride)
+// R: the forward transform and the spectrum product; RI: the inverse
+// transform and T2 (RI = float after R = double: the split-precision
+// convolution of the mixed-precision solve).
+template <int P, int MODE, class R, class RI>
+__global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(4)))
+k_cols(const typename Cplx<R>::C2* __restrict__ T1, const int* __restrict__ slot_of_row,
+       const typename Cplx<R>::C2* __restrict__ tw, const typename Cplx<RI>::C2* __restrict__ tw_inv, int H, int ny,
+       int nrows_in, const R* __restrict__ spec, double scale, typename Cplx<RI>::C2* __restrict__ T2,
+       double* __restrict__ spec_out, size_t t1_stride, size_t t2_stride)
 {
-    extern __shared__ double2 lds[];
+    using C2 = typename Cplx<R>::C2;
+    using C2I = typename Cplx<RI>::C2;
+    extern __shared__ double2 lds_raw[];
+    C2* lds = (C2*)lds_raw;
     T1 += t1_stride * blockIdx.y;   // right-hand side blockIdx.y of a batch
     T2 += t2_stride * blockIdx.y;
     constexpr int T = P / V, SH = P / 2 + 1, NB = tpb_of(P) / T;
@@ -568,8 +632,8 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
     const int c0 = xcd_block() * NB;
     const int c = c0 + cl;
     const bool live = c < H;
-    double2* x = lds + (size_t)cl * lds_len(P);
-    double2 v[V];
+    C2* x = lds + (size_t)cl * lds_len(P);
+    C2 v[V];
     // all slot lookups first, then all row loads: loads return in order, so
     // a lookup issued after a row load would wait for that row load too
     int slot[V];
@@ -581,13 +645,13 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) {
-        double2 val = make_double2(0.0, 0.0);
+        C2 val = mk2<C2>(0, 0);
         if (slot[q] >= 0) val = T1[tidx(slot[q], c, H)];
         v[q] = val;
     }
     // MODE 0: the spectrum values this thread multiplies by, loaded before the
     // forward's last stage
-    double g[V];
+    R g[V];
     auto pre = [&]() {
         if (MODE == 0 && live) {
 #pragma unroll
@@ -625,22 +689,25 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
             for (int q = 0; q < V; ++q) {
                 if (c == 0) {
                     const int ky = io_index(RL, q / RL, q % RL, t, T, P);
-                    const double gn = spec[(size_t)H * SH + (ky <= P / 2 ? ky : P - ky)];
-                    const double2 zc = x[lx((P - ky) & (P - 1))];
-                    const double a = 0.5 * (g[q] + gn), b = 0.5 * (g[q] - gn);
-                    v[q] = make_double2(a * v[q].x + b * zc.x, a * v[q].y - b * zc.y);
+                    const R gn = spec[(size_t)H * SH + (ky <= P / 2 ? ky : P - ky)];
+                    const C2 zc = x[lx((P - ky) & (P - 1))];
+                    const R a = (R)0.5 * (g[q] + gn), b = (R)0.5 * (g[q] - gn);
+                    v[q] = mk2<C2>(a * v[q].x + b * zc.x, a * v[q].y - b * zc.y);
                 } else {
                     v[q].x *= g[q];
                     v[q].y *= g[q];
                 }
             }
         }
-        fft<P, 1, true>(v, x, t, tw);
+        C2I vi[V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) vi[q] = mk2<C2I>((RI)v[q].x, (RI)v[q].y);
+        fft<P, 1, true>(vi, (C2I*)x, t, tw_inv);
         if (!live) return;
 #pragma unroll
         for (int q = 0; q < V; ++q) {
             const int row = io_index(RO, q / RO, q % RO, t, T, P);
-            if (row < ny) T2[pidx(row, c, H)] = v[q];
+            if (row < ny) T2[pidx(row, c, H)] = vi[q];
         }
     }
 }
@@ -648,12 +715,15 @@ __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(
 // ------------------------------------------------------------- pass 3
 // Transform c of block b makes output rows 2p, 2p+1 (p = b NB + c), FP64 or
 // (a Float32 cache) rounded to FP32.
-template <int P, class TO>
-__global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restrict__ T2, const double2* __restrict__ tw,
-                                                        int nx, int ny, TO* __restrict__ out, size_t t2_stride,
-                                                        size_t out_stride)
+template <int P, class TO, class R>
+__global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const typename Cplx<R>::C2* __restrict__ T2,
+                                                        const typename Cplx<R>::C2* __restrict__ tw, int nx, int ny,
+                                                        TO* __restrict__ out, size_t t2_stride, size_t out_stride)
 {
-    extern __shared__ double2 lds[];
+    using C2 = typename Cplx<R>::C2;
+    using Q4 = typename Cplx<R>::Q4;
+    extern __shared__ double2 lds_raw[];
+    C2* lds = (C2*)lds_raw;
     T2 += t2_stride * blockIdx.y;   // right-hand side blockIdx.y of a batch
     out += out_stride * blockIdx.y;
     constexpr int T = P / V, H = P / 2, NB = tpb_of(P) / T;
@@ -663,30 +733,31 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const double2* __restric
     const int j0 = 2 * p;
     const bool live = j0 < ny;
     const bool two = j0 + 1 < ny;
-    double2* x = lds + (size_t)c * lds_len(P);
-    double2 v[V];
+    C2* x = lds + (size_t)c * lds_len(P);
+    C2 v[V];
     // all loads first (in flight together), then the unpacking
-    double4 abq[V];
+    Q4 abq[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int m = io_index(R0, q / R0, q % R0, t, T, P);
         const int k = m <= P / 2 ? m : P - m;
-        abq[q] = live ? *(const double4*)(T2 + pidx(j0, k == H ? 0 : k, H)) : make_double4(0.0, 0.0, 0.0, 0.0);
+        if (live) abq[q] = *(const Q4*)(T2 + pidx(j0, k == H ? 0 : k, H));
+        else abq[q].x = abq[q].y = abq[q].z = abq[q].w = 0;
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int m = io_index(R0, q / R0, q % R0, t, T, P);
-        double2 z = make_double2(0.0, 0.0);
+        C2 z = mk2<C2>(0, 0);
         if (live) {
             // Z(m) = A(m) + i B(m),  Z(P-m) = conj(A(m)) + i conj(B(m)); column 0
             // holds the real DC (x) and Nyquist (y) values of rows j0, j0+1
             const int k = m <= P / 2 ? m : P - m;
-            const double4 ab = abq[q];
-            double Ax = ab.x, Ay = ab.y, Bx = ab.z, By = ab.w;
-            if (k == 0) { Ay = 0.0; By = 0.0; }
-            if (k == H) { Ax = ab.y; Bx = ab.w; Ay = 0.0; By = 0.0; }
-            if (!two) { Bx = 0.0; By = 0.0; }
-            z = m <= P / 2 ? make_double2(Ax - By, Ay + Bx) : make_double2(Ax + By, Bx - Ay);
+            const Q4 ab = abq[q];
+            R Ax = ab.x, Ay = ab.y, Bx = ab.z, By = ab.w;
+            if (k == 0) { Ay = 0; By = 0; }
+            if (k == H) { Ax = ab.y; Bx = ab.w; Ay = 0; By = 0; }
+            if (!two) { Bx = 0; By = 0; }
+            z = m <= P / 2 ? mk2<C2>(Ax - By, Ay + Bx) : mk2<C2>(Ax + By, Bx - Ay);
         }
         v[q] = z;
     }
@@ -734,62 +805,69 @@ struct Batch {
     size_t in = 0, out = 0;
 };
 
-template <int P>
-static void launch_rows_fwd_p(ilh_ctx* ctx, RowSrc S, int npairs, int nrows_limit, double2* Tout, Batch bt)
+template <int P, class C2>
+static void launch_rows_fwd_p(ilh_ctx* ctx, RowSrc S, int npairs, int nrows_limit, C2* Tout, Batch bt)
 {
+    using R = decltype(C2().x);
     constexpr int NB = tpb_of(P) / (P / V);
     S.amp_lds = S.mode == ROW_TERMS && S.amp_len <= amp_lds_cap(P);
-    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P) + (S.amp_lds ? sizeof(double) * S.amp_len : 0);
-    set_lds((const void*)k_rows_fwd<P>, lds);
-    k_rows_fwd<P><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
-        S, get_twiddles(ctx, P), npairs, nrows_limit, Tout, bt.in, bt.out);
+    const size_t lds = sizeof(C2) * (size_t)NB * lds_len(P) + (S.amp_lds ? sizeof(double) * S.amp_len : 0);
+    set_lds((const void*)k_rows_fwd<P, R>, lds);
+    k_rows_fwd<P, R><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+        S, twiddles<R>(ctx, P), npairs, nrows_limit, Tout, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
-static void launch_rows_fwd(ilh_ctx* ctx, const RowSrc& S, int P, int npairs, int nrows_limit, double2* Tout,
+template <class C2>
+static void launch_rows_fwd(ilh_ctx* ctx, const RowSrc& S, int P, int npairs, int nrows_limit, C2* Tout,
                             Batch bt = Batch())
 {
     if (npairs <= 0) return;
-    ILH_FOR_P(P, launch_rows_fwd_p<PP>(ctx, S, npairs, nrows_limit, Tout, bt));
+    ILH_FOR_P(P, (launch_rows_fwd_p<PP, C2>(ctx, S, npairs, nrows_limit, Tout, bt)));
 }
 
-template <int P, int MODE>
-static void launch_cols_p(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int H, int ny, int nrows_in,
-                          const double* spec, double scale, double2* T2, double* spec_out, Batch bt)
+template <int P, int MODE, class C2, class C2I>
+static void launch_cols_p(ilh_ctx* ctx, const C2* T1, const int* slot_of_row, int H, int ny, int nrows_in,
+                          const decltype(C2().x)* spec, double scale, C2I* T2, double* spec_out, Batch bt)
 {
+    using R = decltype(C2().x);
+    using RI = decltype(C2I().x);
     constexpr int NB = tpb_of(P) / (P / V);
-    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
-    set_lds((const void*)k_cols<P, MODE>, lds);
-    k_cols<P, MODE><<<dim3((H + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
-        T1, slot_of_row, get_twiddles(ctx, P), H, ny, nrows_in, spec, scale, T2, spec_out, bt.in, bt.out);
+    const size_t lds = sizeof(C2) * (size_t)NB * lds_len(P);
+    set_lds((const void*)k_cols<P, MODE, R, RI>, lds);
+    k_cols<P, MODE, R, RI><<<dim3((H + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+        T1, slot_of_row, twiddles<R>(ctx, P), twiddles<RI>(ctx, P), H, ny, nrows_in, spec, scale, T2, spec_out,
+        bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
-template <int MODE>
-static void launch_cols(ilh_ctx* ctx, const double2* T1, const int* slot_of_row, int Px, int Py, int ny,
-                        int nrows_in, const double* spec, double scale, double2* T2, double* spec_out,
-                        Batch bt = Batch())
+// T2 may hold a narrower type than T1 (C2I = float2 after C2 = double2)
+template <int MODE, class C2, class C2I>
+static void launch_cols(ilh_ctx* ctx, const C2* T1, const int* slot_of_row, int Px, int Py, int ny, int nrows_in,
+                        const decltype(C2().x)* spec, double scale, C2I* T2, double* spec_out, Batch bt = Batch())
 {
     const int H = Px / 2;
-    ILH_FOR_P(Py, (launch_cols_p<PP, MODE>(ctx, T1, slot_of_row, H, ny, nrows_in, spec, scale, T2, spec_out, bt)));
+    ILH_FOR_P(Py, (launch_cols_p<PP, MODE, C2, C2I>(ctx, T1, slot_of_row, H, ny, nrows_in, spec, scale, T2,
+                                                     spec_out, bt)));
 }
 
-template <int P, class TO>
-static void launch_rows_inv_p(ilh_ctx* ctx, const double2* T2, int nx, int ny, TO* out, Batch bt)
+template <int P, class TO, class C2>
+static void launch_rows_inv_p(ilh_ctx* ctx, const C2* T2, int nx, int ny, TO* out, Batch bt)
 {
+    using R = decltype(C2().x);
     constexpr int NB = tpb_of(P) / (P / V);
     const int npairs = (ny + 1) / 2;
-    const size_t lds = sizeof(double2) * (size_t)NB * lds_len(P);
-    set_lds((const void*)k_rows_inv<P, TO>, lds);
-    k_rows_inv<P, TO><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
-        T2, get_twiddles(ctx, P), nx, ny, out, bt.in, bt.out);
+    const size_t lds = sizeof(C2) * (size_t)NB * lds_len(P);
+    set_lds((const void*)k_rows_inv<P, TO, R>, lds);
+    k_rows_inv<P, TO, R><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+        T2, twiddles<R>(ctx, P), nx, ny, out, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }
 
-template <class TO>
-static void launch_rows_inv(ilh_ctx* ctx, const double2* T2, int Px, int nx, int ny, TO* out, Batch bt = Batch())
+template <class TO, class C2>
+static void launch_rows_inv(ilh_ctx* ctx, const C2* T2, int Px, int nx, int ny, TO* out, Batch bt = Batch())
 {
-    ILH_FOR_P(Px, launch_rows_inv_p<PP>(ctx, T2, nx, ny, out, bt));
+    ILH_FOR_P(Px, (launch_rows_inv_p<PP, TO, C2>(ctx, T2, nx, ny, out, bt)));
 }
 
 // -------------------------------------------------------------- LGF table
@@ -950,8 +1028,8 @@ ConvPlan& get_conv(ilh_plan* pl, int gk)
     S.Py = C.Py;
     launch_rows_fwd(ctx, S, C.Px, C.Py / 2, C.Py, C.T1.as<double2>());
     const double scale = g.dx * g.dx / ((double)C.Px * C.Py);
-    launch_cols<1>(ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.Py, C.Py, nullptr, scale, nullptr,
-                   C.spec.as<double>());
+    launch_cols<1, double2, double2>(ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.Py, C.Py, nullptr, scale,
+                                     (double2*)nullptr, C.spec.as<double>());
     pl->conv_ok[gk] = true;
     return C;
 }
@@ -970,8 +1048,28 @@ void conv_dense(ilh_plan* pl, int gk, const double* in, double* out)
     launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
 }
 
-// FP32 field -> FP32 field (a Float32 cache): the FFT passes run in FP64, the
-// field is read and written in FP32
+__global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e < n) b[e] = (float)a[e];
+}
+
+// the LGF spectrum rounded to FP32 once (the FP32 transforms multiply by it)
+static const float* spec_f32(ilh_plan* pl, ConvPlan& C)
+{
+    if (!C.spec32_ok) {
+        const size_t n = (size_t)(C.H + 1) * (C.Py / 2 + 1);
+        C.spec32.ensure(sizeof(float) * n);
+        k_d2f<<<(unsigned)((n + 255) / 256), 256, 0, pl->ctx->stream>>>(C.spec.as<double>(), C.spec32.as<float>(), n);
+        ILH_HIP(hipGetLastError());
+        C.spec32_ok = true;
+    }
+    return C.spec32.as<float>();
+}
+
+// FP32 field -> FP32 field (a Float32 cache): the three passes in FP32
+// (FP32 spectra intermediates, spectrum, twiddles and LDS exchanges), as an
+// FP32 FFT plan of the same convolution
 void conv_dense(ilh_plan* pl, int gk, const float* in, float* out)
 {
     ConvPlan& C = get_conv(pl, gk);
@@ -980,10 +1078,10 @@ void conv_dense(ilh_plan* pl, int gk, const float* in, float* out)
     S.field32 = in;
     S.nx = C.nx;
     S.ny = C.ny;
-    launch_rows_fwd(pl->ctx, S, C.Px, (C.ny + 1) / 2, C.ny, C.T1.as<double2>());
-    launch_cols<0>(pl->ctx, C.T1.as<double2>(), nullptr, C.Px, C.Py, C.ny, C.ny, C.spec.as<double>(), 1.0,
-                   C.T2.as<double2>(), nullptr);
-    launch_rows_inv(pl->ctx, C.T2.as<double2>(), C.Px, C.nx, C.ny, out);
+    const float* sp = spec_f32(pl, C);
+    launch_rows_fwd(pl->ctx, S, C.Px, (C.ny + 1) / 2, C.ny, C.T1.as<float2>());
+    launch_cols<0>(pl->ctx, C.T1.as<float2>(), nullptr, C.Px, C.Py, C.ny, C.ny, sp, 1.0, C.T2.as<float2>(), nullptr);
+    launch_rows_inv(pl->ctx, (const float2*)C.T2.as<float2>(), C.Px, C.nx, C.ny, out);
 }
 
 void conv_dense_multi(ilh_plan* pl, int gk, const double* in, int nb, double* out)
@@ -1154,6 +1252,21 @@ void conv_sources_ext(ilh_plan* pl, int gk, const SourceSet& src, const double* 
     launch_rows_fwd(pl->ctx, terms_source(C, src, amp), C.Px, src.npairs, C.ny, T1);
     launch_cols<0>(pl->ctx, T1, src.slot_of_row, C.Px, C.Py, C.ny, 0, C.spec.as<double>(), 1.0, T2, nullptr);
     launch_rows_inv(pl->ctx, (const double2*)T2, C.Px, C.nx, C.ny, out);
+}
+
+// conv_sources in split precision: the forward transforms and the product
+// with the spectrum in FP64, the inverse column transform, the spectrum it
+// writes and the inverse row transform in FP32.  (The surface sources have
+// large high-wavenumber content -- the double layer's differences -- which an
+// FP32 forward transform smears into the small low-wavenumber coefficients;
+// after the product the spectrum is smooth and FP32 keeps ~1e-7.)
+void conv_sources_f32(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out)
+{
+    ConvPlan& C = get_conv(pl, gk);
+    launch_rows_fwd(pl->ctx, terms_source(C, src, amp), C.Px, src.npairs, C.ny, C.T1.as<double2>());
+    launch_cols<0>(pl->ctx, (const double2*)C.T1.as<double2>(), src.slot_of_row, C.Px, C.Py, C.ny, 0,
+                   C.spec.as<double>(), 1.0, C.T2.as<float2>(), nullptr);
+    launch_rows_inv(pl->ctx, (const float2*)C.T2.as<float2>(), C.Px, C.nx, C.ny, out);
 }
 
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, hipEvent_t* ev)

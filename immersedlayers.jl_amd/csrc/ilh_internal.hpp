@@ -145,6 +145,8 @@ struct ConvPlan {
     int Px = 0, Py = 0;       // padded periods (powers of two, >= 2n-2)
     int H = 0;                // Px/2: half-spectrum columns (column 0 packs DC and Nyquist)
     DevBuf spec;              // real spectrum * dx^2/(Px*Py): [kx][ky], kx <= H, ky <= Py/2 (even in ky)
+    DevBuf spec32;            // the same rounded to FP32 (the FP32 passes), built on first use
+    bool spec32_ok = false;
     DevBuf T1, T2;            // intermediate spectra, rows x H complex, 8-row tiles
     DevBuf T1m, T2m;          // the same for batches of right-hand sides (copies back to back)
 };
@@ -247,7 +249,7 @@ void ensure_lgf(ilh_plan* pl);
 ConvPlan& get_conv(ilh_plan* pl, int gk);
 // field -> field:  out = dx^2 (G * in)
 void conv_dense(ilh_plan* pl, int gk, const double* in, double* out);
-void conv_dense(ilh_plan* pl, int gk, const float* in, float* out);   // FP32 fields, FP64 transforms
+void conv_dense(ilh_plan* pl, int gk, const float* in, float* out);   // FP32 fields, FP32 transforms
 // nb fields at once: field k at in + k n(gk), out + k n(gk) (in == out allowed)
 void conv_dense_multi(ilh_plan* pl, int gk, const double* in, int nb, double* out);
 // point sources -> field: out = dx^2 (G * sum_terms amp*term), rows listed in pairs
@@ -267,6 +269,9 @@ void pin_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);
 void conv_stage(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out, int stage);
+// the same convolution in split precision: forward transforms and the spectrum
+// product in FP64, the inverse transforms in FP32 (FP64 output)
+void conv_sources_f32(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out);
 // conv_sources into caller-supplied spectra buffers, allocating nothing
 int conv_ext_rows1(const SourceSet& src);
 int conv_ext_rows2(const ConvPlan& C);

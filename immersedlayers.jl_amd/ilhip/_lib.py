@@ -119,6 +119,7 @@ SIGNATURES = {
     "ilh_dirichlet_solve_moving": ([_vp, _vp, _vp, _vp, _vp, _i, C.c_double, _i, _vp], _i),
     "ilh_dirichlet_solve_multi": ([_vp, _i, _vp, _vp, _vp, _vp, _i], _i),
     "ilh_dirichlet_solve_noalloc": ([_vp, _vp, _vp, _vp, _vp], _i),
+    "ilh_dirichlet_solve_mixed": ([_vp, _vp, _vp, _vp, _vp, _i], _i),
     "ilh_dirichlet_solve_moving_batched": ([_i, _vp, _vp, _vp, _vp, _vp, C.c_double, _i, _vp], _i),
     "ilh_dirichlet_solve_batched": ([_i, _vp, _vp, _vp, _vp, _vp, _i], _i),
     "ilh_batch_create": ([_i, _vp, _vp, _vp, _vp, _vp, _vp], _i),

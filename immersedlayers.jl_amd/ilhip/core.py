@@ -607,6 +607,23 @@ class DirichletPoisson:
             s = filter_surface(s, c, filter_power)
         return f, s
 
+    def solve_mixed(self, fbplus, fbminus=None):
+        """(f, s) with the convolution that forms f in FP32 and s in FP64
+        (ilh_dirichlet_solve_mixed): f to the FP32 rounding of the convolution."""
+        c = self.cache
+        fp = _vec(fbplus, c.N)
+        fm = None if fbminus is None else _vec(fbminus, c.N)
+        f = Nodes(Primal, c.g)
+        s = ScalarData(c.N)
+        L.call("ilh_dirichlet_solve_mixed", c.h, _ptr(fp), None if fm is None else _ptr(fm), _ptr(f.flat()),
+               _ptr(s.flat()), L.MEM_HOST)
+        return f, s
+
+    def solve_mixed_device(self, fp_dev, fm_dev, f_dev, s_dev):
+        """Device-resident form of solve_mixed (asynchronous)."""
+        L.call("ilh_dirichlet_solve_mixed", self.cache.h, fp_dev.ptr, None if fm_dev is None else fm_dev.ptr,
+               f_dev.ptr, s_dev.ptr, L.MEM_DEVICE)
+
     def solve_multi(self, fbplus, fbminus=None):
         """Many boundary data on this geometry at once: fbplus (and fbminus)
         N x K; returns f (K Nodes{Primal}) and s (N x K array)."""

@@ -165,8 +165,10 @@ int ilh_inverse_laplacian(ilh_plan* plan, int grid_kind, double* out, const doub
 /* ---- FP32 data (caches built with dtype = Float32, src/cache.jl:164-201) ----
  * The same operators on float buffers: stencils in FP32 (the difference in
  * FP32, the 1/dx scaling rounded once); regularize!/interpolate! accumulate
- * the FP64 DDF weights and round once; inverse_laplacian! reads and writes
- * FP32 fields around FP64 transforms. */
+ * the FP64 DDF weights and round once; inverse_laplacian! is the FFT
+ * convolution in FP32 (FP32 transforms, spectra, LGF spectrum rounded once
+ * and LDS exchanges), as an FP32 FFT plan computes it: within 2e-6 of the
+ * FP64 convolution (max-abs, relative to the output's largest entry). */
 int ilh_regularize_f32(ilh_plan* plan, int grid_kind, float* out, const float* f, int mem);
 int ilh_interpolate_f32(ilh_plan* plan, int grid_kind, float* out, const float* grid, int mem);
 int ilh_divergence_f32(ilh_plan* plan, float* p, const float* q_edges, int mem);
@@ -198,6 +200,15 @@ int ilh_schur_solve(ilh_plan* plan, int kind, double* x, const double* b, int me
    solve matrix; the first call with fb_minus also builds the interior-data one. */
 int ilh_dirichlet_solve(ilh_plan* plan, const double* fb_plus, const double* fb_minus,
                         double* f, double* s, int mem);
+/* Mixed-precision form of ilh_dirichlet_solve (same arguments and outputs):
+   the surface strength s is the FP64 solve (solve matrices and matvec in
+   FP64, so it equals ilh_dirichlet_solve's s bit for bit), and f is the
+   convolution in split precision: the forward transforms and the product
+   with the LGF spectrum in FP64, the inverse column transform, the spectrum
+   it writes and the inverse row transform in FP32.  f agrees with the FP64
+   solve to relative L2 ~1e-7 (bound 1e-6 on the 1024^2 workload). */
+int ilh_dirichlet_solve_mixed(ilh_plan* plan, const double* fb_plus, const double* fb_minus,
+                              double* f, double* s, int mem);
 /* Neumann solve of test/literate/neumann.jl:101-142: outputs f (Nodes{Primal}),
    df = [ϕ], s (Nodes{Dual} streamfunction) and ds = [ψ]. vn_minus may be NULL. */
 int ilh_neumann_solve(ilh_plan* plan, const double* vn_plus, const double* vn_minus,

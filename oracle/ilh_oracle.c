@@ -790,26 +790,50 @@ void orc_schur_direct(const orc_grid* g, int ddf, int N, const double* X, const 
 }
 
 /* ------------------------------------------------------------------ LU */
-/* Right-looking LU with partial pivoting (what LAPACK getrf computes). */
+/* LU with partial pivoting (LAPACK getrf, what Julia's `\` on a Matrix runs),
+   blocked right-looking: each 64-column panel is factored column by column
+   with the row interchanges applied to whole rows, then the block row U12 =
+   L11^-1 A12 and the trailing update A22 -= L21 U12 (columns in parallel).
+   The pivots are those of the unblocked algorithm; the blocking only changes
+   the summation order of the updates. */
 int orc_lu_factor(int n, double* A, int* piv)
 {
     int info = 0;
-    for (int k = 0; k < n; ++k) {
-        int pr = k; double mx = fabs(A[k + (size_t)n * k]);
-        for (int i = k + 1; i < n; ++i) { double v = fabs(A[i + (size_t)n * k]); if (v > mx) { mx = v; pr = i; } }
-        piv[k] = pr;
-        if (mx == 0.0) { if (!info) info = k + 1; continue; }
-        if (pr != k)
-            for (int j = 0; j < n; ++j) { double t = A[k + (size_t)n * j]; A[k + (size_t)n * j] = A[pr + (size_t)n * j]; A[pr + (size_t)n * j] = t; }
-        double inv = 1.0 / A[k + (size_t)n * k];
-        for (int i = k + 1; i < n; ++i) A[i + (size_t)n * k] *= inv;
-        #pragma omp parallel for schedule(static) if (n - k > 256)
-        for (int j = k + 1; j < n; ++j) {
-            double akj = A[k + (size_t)n * j];
-            if (akj == 0.0) continue;
+    const int NB = 64;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int nb = n - k0 < NB ? n - k0 : NB, k1 = k0 + nb;
+        for (int k = k0; k < k1; ++k) {
+            int pr = k; double mx = fabs(A[k + (size_t)n * k]);
+            for (int i = k + 1; i < n; ++i) { double v = fabs(A[i + (size_t)n * k]); if (v > mx) { mx = v; pr = i; } }
+            piv[k] = pr;
+            if (mx == 0.0) { if (!info) info = k + 1; continue; }
+            if (pr != k)
+                for (int j = 0; j < n; ++j) { double t = A[k + (size_t)n * j]; A[k + (size_t)n * j] = A[pr + (size_t)n * j]; A[pr + (size_t)n * j] = t; }
+            const double inv = 1.0 / A[k + (size_t)n * k];
+            double* ck = A + (size_t)n * k;
+            for (int i = k + 1; i < n; ++i) ck[i] *= inv;
+            for (int j = k + 1; j < k1; ++j) {
+                double* cj = A + (size_t)n * j;
+                const double akj = cj[k];
+                if (akj == 0.0) continue;
+                for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * akj;
+            }
+        }
+        #pragma omp parallel for schedule(static) if (n - k1 > 128)
+        for (int j = k1; j < n; ++j) {
             double* cj = A + (size_t)n * j;
-            const double* ck = A + (size_t)n * k;
-            for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * akj;
+            for (int k = k0; k < k1; ++k) {           /* U12: unit-lower solve */
+                const double akj = cj[k];
+                if (akj == 0.0) continue;
+                const double* ck = A + (size_t)n * k;
+                for (int i = k + 1; i < k1; ++i) cj[i] -= ck[i] * akj;
+            }
+            for (int k = k0; k < k1; ++k) {           /* A22 -= L21 U12 */
+                const double akj = cj[k];
+                if (akj == 0.0) continue;
+                const double* ck = A + (size_t)n * k;
+                for (int i = k1; i < n; ++i) cj[i] -= ck[i] * akj;
+            }
         }
     }
     return info;
@@ -1241,6 +1265,73 @@ static void heat_schur(const orc_grid* g, int ddf, int N, const double* X, const
     free(h);
 }
 
+/* The same stage matrix assembled directly: entry (p, q) is the interpolation
+   functional of p applied to the heat kernel K(|m|) K(|n|) (|m|, |n| <= M)
+   convolved with the regularized unit strength of q, summed over the two
+   points' local patches; pairs whose patches lie more than M cells apart are
+   zero.  Equal to the column loop above whenever every patch and its kernel
+   support lie inside the grid (the 5-cell margin of the solves). */
+static void heat_schur_direct(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
+                              const double* K, int M, double* S)
+{
+    static const double one = 1.0;
+    if (!K) { K = &one; M = 0; }
+    patch* cols = (patch*)malloc(sizeof(patch) * (size_t)(N > 0 ? N : 1));
+    patch* rows = (patch*)malloc(sizeof(patch) * (size_t)(N > 0 ? N : 1));
+    #pragma omp parallel for
+    for (int k = 0; k < N; ++k) {
+        col_patch(g, ddf, X, Y, NULL, NULL, ds, k, ORC_S_RTLINVR, &cols[k]);
+        row_patch(g, ddf, X, Y, NULL, NULL, k, ORC_S_RTLINVR, &rows[k]);
+    }
+    #pragma omp parallel
+    {
+        int ia[LOC * LOC], ja[LOC * LOC], ib[LOC * LOC], jb[LOC * LOC];
+        double va[LOC * LOC], vb[LOC * LOC];
+        #pragma omp for schedule(dynamic, 8)
+        for (int q = 0; q < N; ++q) {
+            int nb = 0;
+            for (int j = 0; j < LOC; ++j)
+                for (int i = 0; i < LOC; ++i) {
+                    double v = cols[q].v[i + LOC * j];
+                    if (v != 0.0) { ib[nb] = i + cols[q].off_x; jb[nb] = j + cols[q].off_y; vb[nb] = v; ++nb; }
+                }
+            for (int p = 0; p < N; ++p) {
+                if (abs(rows[p].off_x - cols[q].off_x) > M + LOC || abs(rows[p].off_y - cols[q].off_y) > M + LOC) {
+                    S[p + (size_t)N * q] = 0.0;
+                    continue;
+                }
+                int na = 0;
+                for (int j = 0; j < LOC; ++j)
+                    for (int i = 0; i < LOC; ++i) {
+                        double v = rows[p].v[i + LOC * j];
+                        if (v != 0.0) { ia[na] = i + rows[p].off_x; ja[na] = j + rows[p].off_y; va[na] = v; ++na; }
+                    }
+                double acc = 0.0;
+                for (int a = 0; a < na; ++a) {
+                    double sum = 0.0;
+                    for (int b = 0; b < nb; ++b) {
+                        const int di = abs(ia[a] - ib[b]), dj = abs(ja[a] - jb[b]);
+                        if (di <= M && dj <= M) sum += K[di] * K[dj] * vb[b];
+                    }
+                    acc += va[a] * sum;
+                }
+                S[p + (size_t)N * q] = acc;
+            }
+        }
+    }
+    free(cols);
+    free(rows);
+}
+
+/* the stage matrix E H R of the kernel K (NULL: the identity) by the column
+   loop (method 1) or the direct assembly (method 0), N x N column-major */
+void orc_heat_schur(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
+                    const double* K, int M, int method, double* S)
+{
+    if (method == 1) heat_schur(g, ddf, N, X, Y, ds, K, M, S);
+    else heat_schur_direct(g, ddf, N, X, Y, ds, K, M, S);
+}
+
 int orc_heat_advance(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* nx_,
                      const double* ny_, const double* ds, double kappa, double dt, int nsteps, double* T,
                      const double* tb_plus, const double* tb_minus)
@@ -1263,9 +1354,9 @@ int orc_heat_advance(const orc_grid* g, int ddf, int N, const double* X, const d
         p13 = (int*)malloc(sizeof(int) * N);
         p23 = (int*)malloc(sizeof(int) * N);
         p0 = (int*)malloc(sizeof(int) * N);
-        heat_schur(g, ddf, N, X, Y, ds, K13, M13, S13);
-        heat_schur(g, ddf, N, X, Y, ds, K23, M23, S23);
-        heat_schur(g, ddf, N, X, Y, ds, NULL, 0, S0);
+        heat_schur_direct(g, ddf, N, X, Y, ds, K13, M13, S13);
+        heat_schur_direct(g, ddf, N, X, Y, ds, K23, M23, S23);
+        heat_schur_direct(g, ddf, N, X, Y, ds, NULL, 0, S0);
         info = orc_lu_factor(N, S13, p13) | orc_lu_factor(N, S23, p23) | orc_lu_factor(N, S0, p0);
     }
     /* r1 = surface_divergence(-kappa [T]), r2 = surface average */

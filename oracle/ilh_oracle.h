@@ -118,6 +118,11 @@ int orc_stokes_solve(const orc_grid* g, int ddf, int N, const double* X, const d
    node field (zero outside), and nsteps steps of T (Nodes{Primal}, in place). */
 int  orc_heat_kernel(double a, double* K, int maxlen);
 void orc_heat_apply(int nx, int ny, const double* K, int M, const double* in, double* out);
+/* the stage matrix E H R of kernel K (K(|m|) K(|n|), |m|,|n| <= M; NULL: the
+   identity), method 1 the column loop of the reference's create_RTLinvR shape,
+   0 the direct assembly */
+void orc_heat_schur(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
+                    const double* K, int M, int method, double* S);
 int  orc_heat_advance(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* nx,
                       const double* ny, const double* ds, double kappa, double dt, int nsteps, double* T,
                       const double* tb_plus, const double* tb_minus);

@@ -326,6 +326,15 @@ def heat_apply(nx, ny, K, field):
     return out
 
 
+def heat_schur(g, geo, K, method="direct", ddf_kind=YANG3):
+    """Stage matrix E H R (numpy [row, col]) of the separable kernel K (None: identity)."""
+    A = np.zeros(geo.N * geo.N)
+    Kp = None if K is None else np.ascontiguousarray(K, float)
+    lib().orc_heat_schur(C.byref(g), ddf_kind, *geo.args(), _p(geo.ds), _p(Kp), 0 if K is None else len(K) - 1,
+                         1 if method == "columns" else 0, _p(A))
+    return A.reshape(geo.N, geo.N).T.copy()
+
+
 def heat_advance(g, geo, kappa, dt, nsteps, T0, tb_plus, tb_minus=None, ddf_kind=YANG3):
     """nsteps IF-HERK steps of the Dirichlet heat conduction problem (Nodes{Primal} T)."""
     T = np.ascontiguousarray(np.asarray(T0, float).ravel(order="F")).copy()

@@ -1,0 +1,252 @@
+"""The benchmark configurations at their stated workloads, on the GPU.
+
+C2  operator sweeps on 2048^2 Nodes{Primal} (fields i.i.d. N(0,1)) with four
+    star curves of 2048 points each at ~0.5 dx: every stencil, regularize! /
+    interpolate! and inverse_laplacian! against the oracle, FP64 and FP32
+    (src/grid_operators.jl:25-179, src/surface_operators.jl:13-88).
+C3  the 64 batched Dirichlet problems of config_batched() (grids 256^2 ..
+    1024^2, 200-3000 points): each batched result equals its single solve bit
+    for bit; a seeded subset that includes the largest problem matches the
+    oracle (test/literate/problems.jl:259-298).
+C4  heat conduction at 2048^2 with three star bodies (5465 points), 200
+    IF-HERK steps at Fo = 0.25: E T = (T_b+ + T_b-)/2 after every step, and
+    the first two steps against the oracle (test/literate/heatconduction.jl).
+C5  the 1024^2 ellipse + star geometry (1885 points) under 32 seeded
+    pitch/heave trajectories: the batched moving-body solve equals each
+    trajectory's single solve bit for bit, and a subset of steps matches the
+    oracle's direct solve (src/system.jl:26-50).
+
+Bars: operators 1e-12 (max-abs relative), inverse_laplacian! 1e-13 (relative
+L2), f of every FP64 solve 1e-10 (relative L2).  The surface strength s
+solves the Schur system, whose condition number at these point spacings is
+1e5-1e7, so two FP64 computations of s agree to ~cond(S) eps: its bar is 1e-9.
+FP32 operators: regularize!/interpolate! equal the FP64 path rounded to FP32
+bit for bit; stencils within 5e-7 of the oracle; inverse_laplacian! (FP32
+transforms) within 2e-6 of the oracle (max-abs relative).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import max_rel, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP, TOL_F, TOL_S = 1e-12, 1e-10, 1e-9
+
+
+# ------------------------------------------------------------------ C2
+@pytest.fixture(scope="module")
+def c2(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    g, s = G.config_microbench()
+    print("C2: grid", g.NX - 1, "N", s.N, flush=True)
+    c64 = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+    return ilhip, g, s, c64, og, geo
+
+
+def test_c2_stencils_fp64(c2):
+    il, g, s, c, og, geo = c2
+    rng = np.random.default_rng(202)
+    q = rng.standard_normal(O.edge_len(og))
+    p = rng.standard_normal(g.primal_shape())
+    d = rng.standard_normal(g.dual_shape())
+    out = il.Nodes(il.Primal, g)
+    il.divergence(out, il.Edges(g, q), c)
+    assert max_rel(out.flat(), O.divergence(og, q)) < TOL_OP
+    e = il.Edges(g)
+    il.grad(e, il.Nodes(il.Primal, g, p), c)
+    assert max_rel(e.flat(), O.grad(og, p.ravel(order="F"))) < TOL_OP
+    il.curl(e, il.Nodes(il.Dual, g, d), c)
+    assert max_rel(e.flat(), O.curl_n2e(og, d.ravel(order="F"))) < TOL_OP
+    w = il.Nodes(il.Dual, g)
+    il.curl(w, il.Edges(g, q), c)
+    assert max_rel(w.flat(), O.curl_e2n(og, q)) < TOL_OP
+    il.laplacian(out, il.Nodes(il.Primal, g, p), c)
+    assert max_rel(out.flat(), O.laplacian(og, O.PRIMAL, p.ravel(order="F"))) < TOL_OP
+
+
+def test_c2_regularize_interpolate_fp64(c2):
+    il, g, s, c, og, geo = c2
+    rng = np.random.default_rng(203)
+    f = rng.standard_normal(s.N)
+    for kind, okind in ((il.Primal, O.PRIMAL), (il.Dual, O.DUAL)):
+        out = il.Nodes(kind, g)
+        il.regularize(out, il.ScalarData(s.N, f), c)
+        assert max_rel(out.flat(), O.regularize(og, okind, geo, f)) < TOL_OP, kind
+        fld = rng.standard_normal(out.data.shape)
+        sv = il.ScalarData(s.N)
+        il.interpolate(sv, il.Nodes(kind, g, fld), c)
+        assert max_rel(sv.flat(), O.interpolate(og, okind, geo, fld.ravel(order="F"))) < TOL_OP, kind
+
+
+def test_c2_inverse_laplacian_fp64(c2):
+    il, g, s, c, og, geo = c2
+    rng = np.random.default_rng(204)
+    w = rng.standard_normal(g.primal_shape())
+    out = il.Nodes(il.Primal, g)
+    il.inverse_laplacian(out, c, rhs=il.Nodes(il.Primal, g, w))
+    ref = O.InverseLaplacian(og, O.PRIMAL)(w.ravel(order="F"))
+    assert rel_err(out.flat(), ref) < 1e-13
+
+
+def test_c2_fp32(c2, ctx):
+    il, g, s, c64, og, geo = c2
+    c32 = il.SurfaceScalarCache(s, g, ctx=ctx, dtype=np.float32)
+    rng = np.random.default_rng(205)
+    N = s.N
+    f = rng.standard_normal(N).astype(np.float32)
+    o32, o64 = il.Nodes(il.Primal, g, dtype=np.float32), il.Nodes(il.Primal, g)
+    il.regularize(o32, il.ScalarData(N, f), c32)
+    il.regularize(o64, il.ScalarData(N, f.astype(np.float64)), c64)
+    assert np.array_equal(o32.flat(), o64.flat().astype(np.float32))
+    fld = rng.standard_normal(g.primal_shape()).astype(np.float32)
+    s32, s64 = il.ScalarData(N, dtype=np.float32), il.ScalarData(N)
+    il.interpolate(s32, il.Nodes(il.Primal, g, fld), c32)
+    il.interpolate(s64, il.Nodes(il.Primal, g, fld.astype(np.float64)), c64)
+    assert np.array_equal(s32.flat(), s64.flat().astype(np.float32))
+    # inverse Laplacian: the FFT convolution in FP32 arithmetic
+    w32 = il.Nodes(il.Primal, g, dtype=np.float32)
+    il.inverse_laplacian(w32, c32, rhs=il.Nodes(il.Primal, g, fld))
+    ref = O.InverseLaplacian(og, O.PRIMAL)(fld.ravel(order="F").astype(np.float64))
+    err = max_rel(w32.flat(), ref)
+    print("C2 FP32 inverse_laplacian max rel", err, flush=True)
+    assert err < 2e-6
+    # stencils in FP32
+    q = rng.standard_normal(O.edge_len(og)).astype(np.float32)
+    out = il.Nodes(il.Primal, g, dtype=np.float32)
+    il.divergence(out, il.Edges(g, q), c32)
+    assert max_rel(out.flat(), O.divergence(og, q.astype(np.float64))) < 5e-7
+    il.laplacian(out, il.Nodes(il.Primal, g, fld), c32)
+    assert max_rel(out.flat(), O.laplacian(og, O.PRIMAL, fld.ravel(order="F").astype(np.float64))) < 5e-7
+    c32.close()
+
+
+# ------------------------------------------------------------------ C3
+def test_c3_batched_64_problems(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    probs = G.config_batched()
+    assert len(probs) == 64
+    sizes = [g.NX - 1 for g, _, _ in probs]
+    Ns = [s.N for _, s, _ in probs]
+    print("C3: 64 problems, n in", sorted(set(sizes)), "N", min(Ns), "-", max(Ns), flush=True)
+    caches = [ilhip.SurfaceScalarCache(s, g, ctx=ctx) for g, s, _ in probs]
+    out = ilhip.dirichlet_solve_batched(caches, [fb for _, _, fb in probs])
+    singles = []
+    for k, ((g, s, fb), c) in enumerate(zip(probs, caches)):
+        f1, s1 = ilhip.DirichletPoisson(c).solve(fb)
+        assert np.array_equal(out[k][0].flat(), f1.flat()), k
+        assert np.array_equal(out[k][1].flat(), s1.flat()), k
+        singles.append((f1, s1))
+    # oracle on a seeded subset that includes the largest problem
+    rng = np.random.default_rng(33)
+    big = int(np.argmax([n * n * 1e4 + N for n, N in zip(sizes, Ns)]))
+    subset = sorted({big, *rng.choice(64, 2, replace=False).tolist()})
+    for k in subset:
+        g, s, fb = probs[k]
+        og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+        fr, sr = O.dirichlet_solve(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), fb)
+        ef, es = rel_err(singles[k][0].flat(), fr), rel_err(singles[k][1].flat(), sr)
+        print("C3 problem", k, "n", g.NX - 1, "N", s.N, "f", ef, "s", es, flush=True)
+        assert ef < TOL_F and es < TOL_S
+    for c in caches:
+        c.close()
+
+
+# ------------------------------------------------------------------ C4
+def test_c4_heat_2048(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    g, s, T0 = G.config_heat()
+    print("C4: grid", g.NX - 1, "N", s.N, flush=True)
+    kappa, dt = 1.0, 0.25 * g.dx * g.dx
+    tb = np.ones(s.N)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    heat = ilhip.DirichletHeatConduction(cache, kappa, dt, tb)
+    T = ilhip.Nodes(ilhip.Primal, g, T0)
+    Eb = ilhip.ScalarData(s.N)
+    worst = 0.0
+    snaps = {}
+    for step in range(1, 201):
+        T = heat.advance(T, 1)
+        ilhip.interpolate(Eb, T, cache)
+        worst = max(worst, np.abs(Eb.flat() - 0.5).max())
+        if step <= 2:
+            snaps[step] = T.flat().copy()
+    print("C4: max |E T - T_b| over 200 steps", worst, flush=True)
+    assert worst < 1e-10
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+    Tr = O.heat_advance(og, geo, kappa, dt, 2, T0.ravel(order="F"), tb)
+    e2 = rel_err(snaps[2], Tr)
+    print("C4: two steps vs oracle", e2, flush=True)
+    assert e2 < 1e-10
+    assert np.isfinite(T.flat()).all()
+    heat.close()
+    cache.close()
+
+
+# ------------------------------------------------------------------ C5
+def _pitch_heave(s, t, freq, amp=0.2):
+    """Pitch about the origin (amplitude `amp` rad) and heave in y (amplitude
+    `amp`) at frequency `freq`, time t: same points, panel lengths and count."""
+    from ilhip.geometry import Surface
+    ang = amp * np.sin(2 * np.pi * freq * t)
+    dy = amp * np.sin(2 * np.pi * freq * t + 0.5 * np.pi) - amp
+    c, n = np.cos(ang), np.sin(ang)
+    return Surface(c * s.X - n * s.Y, n * s.X + c * s.Y + dy, c * s.nx - n * s.ny, n * s.nx + c * s.ny, s.ds,
+                   s.body)
+
+
+def test_c5_moving_32_trajectories(ctx):
+    import ilhip
+    from ilhip import geometry as G
+    g, s0, fb = G.config_headline()
+    ntraj, nsteps, dt = 32, 3, 0.004
+    rng = np.random.default_rng(55)
+    freqs = rng.uniform(0.5, 2.0, ntraj)
+    print("C5: grid", g.NX - 1, "N", s0.N, "trajectories", ntraj, flush=True)
+    ng = (g.NX - 1) * (g.NY - 1)
+    cb = [ilhip.SurfaceScalarCache(s0, g, ctx=ctx) for _ in range(ntraj)]
+    fbs = [fb * (1.0 + 0.05 * k) for k in range(ntraj)]
+    fp = [ilhip.DeviceArray.from_host(ctx, x) for x in fbs]
+    fd = [ilhip.DeviceArray(ctx, ng) for _ in range(ntraj)]
+    sd = [ilhip.DeviceArray(ctx, s0.N) for _ in range(ntraj)]
+    check = [0, 17]   # trajectories also checked against the oracle
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    results = []
+    for step in range(nsteps):
+        surfs = [_pitch_heave(s0, step * dt, freqs[k]) for k in range(ntraj)]
+        if step:
+            for k in range(ntraj):
+                cb[k].update_points(surfs[k])
+        its = ilhip.dirichlet_solve_moving_batched(cb, fp, fd, sd, tol=1e-12, max_iter=40)
+        ctx.synchronize()
+        results.append(([x.download() for x in fd], [x.download() for x in sd], list(its), surfs))
+        print("C5 step", step, "GMRES iterations", min(its), "-", max(its), flush=True)
+    for c in cb:
+        c.close()
+    del fd, sd
+    # each trajectory's single-plan solves, replayed: bit for bit
+    for k in range(ntraj):
+        c1 = ilhip.SurfaceScalarCache(s0, g, ctx=ctx)
+        prob = ilhip.DirichletPoisson(c1)
+        for step in range(nsteps):
+            fs, ss, its, surfs = results[step]
+            if step:
+                c1.update_points(surfs[k])
+            f1, s1, it1 = prob.solve_moving(fbs[k], tol=1e-12, max_iter=40)
+            assert it1 == its[k], (k, step)
+            assert np.array_equal(fs[k], f1.flat()), (k, step)
+            assert np.array_equal(ss[k], s1.flat()), (k, step)
+            if k in check and step in (0, nsteps - 1):
+                sk = surfs[k]
+                fr, sr = O.dirichlet_solve(og, O.Geometry(sk.X, sk.Y, sk.nx, sk.ny, sk.ds), fbs[k])
+                ef, es = rel_err(f1.flat(), fr), rel_err(s1.flat(), sr)
+                print("C5 trajectory", k, "step", step, "f", ef, "s", es, flush=True)
+                assert ef < TOL_F and es < TOL_S
+        c1.close()

@@ -1,12 +1,15 @@
 """FP32 operators (caches with dtype = Float32, src/cache.jl:164-201) against
 the FP64 oracle and the FP64 HIP path.
 
-regularize!/interpolate! accumulate the FP64 DDF weights and round once, and
-inverse_laplacian! rounds the FP64 convolution of the FP32 field once, so on
-the same (FP32) inputs their FP32 outputs equal the FP64 outputs rounded to
-FP32 bit for bit.  The stencils compute in FP32 (the difference in FP32, the
-1/dx scaling rounded once): against the oracle to 4 FP32 ulps of the output's
-magnitude (max-abs relative 5e-7).
+regularize!/interpolate! accumulate the FP64 DDF weights and round once, so
+on the same (FP32) inputs their FP32 outputs equal the FP64 outputs rounded
+to FP32 bit for bit.  The stencils compute in FP32 (the difference in FP32,
+the 1/dx scaling rounded once): against the oracle to 4 FP32 ulps of the
+output's magnitude (max-abs relative 5e-7).  inverse_laplacian! is the FFT
+convolution in FP32 arithmetic: against the oracle within TOL_CONV32 = 2e-6
+(max-abs relative), the accumulated rounding of FP32 transforms of these
+lengths (an FP32 FFT of the same zero-padded convolution lands at 3e-7 to
+8e-7).
 """
 import numpy as np
 import pytest
@@ -17,6 +20,7 @@ from conftest import max_rel
 pytestmark = pytest.mark.gpu
 
 TOL_F32 = 5e-7
+TOL_CONV32 = 2e-6
 
 
 @pytest.fixture(scope="module")
@@ -91,10 +95,12 @@ def test_inverse_laplacian_f32(setup, kind):
     o64 = il.Nodes(k, gs)
     il.inverse_laplacian(o32, c32, rhs=il.Nodes(k, gs, a))
     il.inverse_laplacian(o64, c64, rhs=il.Nodes(k, gs, a.astype(np.float64)))
-    assert np.array_equal(o32.flat(), o64.flat().astype(np.float32))
     okind = O.PRIMAL if kind == "Primal" else O.DUAL
     ref = O.InverseLaplacian(og, okind)(a.ravel(order="F").astype(np.float64))
-    assert max_rel(o32.flat(), ref) < TOL_F32
+    err = max_rel(o32.flat(), ref)
+    print("FP32 inverse_laplacian", kind, "max rel", err, flush=True)
+    assert err < TOL_CONV32
+    assert max_rel(o64.flat(), ref) < 1e-13
 
 
 def test_f32_type_checks(setup):
@@ -112,3 +118,25 @@ def test_f32_type_checks(setup):
 def G_star(gs):
     from ilhip import geometry as G
     return G.circle(0.5, 1.4 * gs.dx)
+
+
+def test_mixed_precision_dirichlet_solve(ctx):
+    """ilh_dirichlet_solve_mixed on the 1024^2 workload: s equals the FP64
+    solve bit for bit (the surface step is FP64); f, formed by the FP32
+    convolution, is within 1e-6 of the oracle (relative L2)."""
+    import ilhip
+    from ilhip import geometry as G
+    from conftest import rel_err
+    g, s, fb = G.config_headline()
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    prob = ilhip.DirichletPoisson(cache)
+    f64, s64 = prob.solve(fb)
+    f32, s32 = prob.solve_mixed(fb)
+    assert np.array_equal(s32.flat(), s64.flat())
+    og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
+    fr, sr = O.dirichlet_solve(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), fb)
+    ef = rel_err(f32.flat(), fr)
+    print("mixed solve: f rel L2 vs oracle", ef, "(FP64 path", rel_err(f64.flat(), fr), ")", flush=True)
+    assert ef < 1e-6
+    assert rel_err(s32.flat(), sr) < 1e-9
+    cache.close()

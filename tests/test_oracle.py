@@ -367,6 +367,22 @@ def _heat_setup(n=64, dx=0.04, width=4.0):
     return g, s, T0
 
 
+def test_heat_stage_matrix_direct_matches_column_loop():
+    """The direct assembly of the stage matrices E H R equals the column loop
+    (regularize a unit strength, apply the heat kernel on the whole grid,
+    interpolate) for the kernels of both stages and for the identity."""
+    g, s, _ = _heat_setup()
+    geo = _geo(s)
+    for a in (0.25 / 3.0, 0.5 / 3.0, 3.0):
+        K = O.heat_kernel(a)
+        A = O.heat_schur(g, geo, K, "direct")
+        B = O.heat_schur(g, geo, K, "columns")
+        assert np.abs(A - B).max() < 1e-13 * np.abs(B).max(), a
+    A = O.heat_schur(g, geo, None, "direct")
+    B = O.heat_schur(g, geo, None, "columns")
+    assert np.abs(A - B).max() < 1e-13 * np.abs(B).max()
+
+
 def test_heat_no_bodies_is_the_exact_semigroup():
     """Without bodies IF-HERK integrates diffusion exactly: k steps of dt equal
     one application of exp(k dt kappa L)."""
