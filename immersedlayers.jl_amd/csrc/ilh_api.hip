@@ -145,12 +145,17 @@ struct MovingCache {
     }
     SourceSet src1;
     bool src_ok = false;
+    // learned preconditioner: the band predicted for the current geometry
+    DevBuf band;
+    bool band_ok = false;
+    bool use_band = false;     // this solve's preconditioner: the band (else the FP32 reference inverse)
     // the current geometry's matrix and sources are stale (their buffers are
     // kept for the next geometry)
     void drop_sources()
     {
         S_ok = false;
         src_ok = false;
+        band_ok = false;
     }
     ~MovingCache()
     {
@@ -1031,6 +1036,22 @@ SolveCache* prepare_neumann(ilh_plan* pl)
     return sc;
 }
 }  // namespace
+
+// S^-1 of create_RTLinvR at the plan's geometry (the deflated Cholesky of
+// M, or LU when that meets a non-positive pivot), row-major
+void ilh::exact_inverse_rowmajor(ilh_plan* pl, double* Pd)
+{
+    const int N = pl->N;
+    SymFactor F;
+    if (sym_rtlinvr(pl, F)) {
+        sym_inverse_rowmajor(pl, F, pl->ds.as<double>(), 1.0, Pd);
+        return;
+    }
+    DevBuf LU;
+    LU.ensure(sizeof(double) * (size_t)N * N);
+    build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
+    inverse_rowmajor(pl, LU.as<double>(), Pd);
+}
 
 extern "C" int ilh_schur_build(ilh_plan* pl, int kind, double scale, double* out, int mem)
 {
@@ -2248,17 +2269,7 @@ static void moving_reference(ilh_plan* pl, MovingCache* mc)
     const size_t nl = (size_t)N * solve_ld(N);
     DevBuf Pd;
     Pd.ensure(sizeof(double) * nl);
-    {
-        SymFactor F;
-        if (sym_rtlinvr(pl, F)) {
-            sym_inverse_rowmajor(pl, F, pl->ds.as<double>(), 1.0, Pd.as<double>());
-        } else {
-            DevBuf LU;
-            LU.ensure(sizeof(double) * (size_t)N * N);
-            build_schur(pl, ILH_SCHUR_RTLINVR, 1.0, LU.as<double>());
-            inverse_rowmajor(pl, LU.as<double>(), Pd.as<double>());
-        }
-    }
+    exact_inverse_rowmajor(pl, Pd.as<double>());
     mc->P32.ensure(sizeof(float) * nl);
     k_to_f32<<<(unsigned)std::min<size_t>(4096, (nl + 255) / 256), 256, 0, pl->ctx->stream>>>(
         Pd.as<double>(), mc->P32.as<float>(), nl);
@@ -2277,6 +2288,21 @@ static void moving_reference(ilh_plan* pl, MovingCache* mc)
 // phases below issue work on the context's current stream without waiting,
 // so that the batched form can run many plans' solves side by side.
 
+// z (+)= P v with this solve's preconditioner: the learned band or the FP32
+// reference inverse; skipped once flags[0] is set (flags may be null)
+static void prec_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* z, const int* flags, int accumulate)
+{
+    const int N = pl->N;
+    hipStream_t st = pl->ctx->stream;
+    if (mc->use_band) {
+        band_apply(st, mc->band.as<double>(), pl->nbr.as<int>(), lpc_width(pl->lpc), v, z, N, flags, accumulate);
+    } else {
+        k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 256, 0, st>>>(mc->P32.as<float>(), solve_ld(N), v, z, N, flags,
+                                                                 accumulate);
+        ILH_HIP(hipGetLastError());
+    }
+}
+
 // x0 = P b, r0 = b - S x0, v_0, g
 static void gm_begin(ilh_plan* pl, MovingCache* mc, const double* b, double* x, double tol, int m)
 {
@@ -2286,8 +2312,8 @@ static void gm_begin(ilh_plan* pl, MovingCache* mc, const double* b, double* x, 
     mc->gm.ensure(sizeof(double) * L.size());
     mc->gmf.ensure(sizeof(int) * 2);
     double* w = mc->w.as<double>();
-    const unsigned gR = (unsigned)((N + 3) / 4);
-    k_gm_matvec_f32<<<gR, 256, 0, st>>>(mc->P32.as<float>(), ld, b, x, N, nullptr, 0);
+    (void)ld;
+    prec_apply(pl, mc, b, x, nullptr, 0);
     schur_apply(pl, mc, x, w);
     axpby(st, w, b, 1.0, -1.0, N);
     k_gm_init<<<1, 1024, 0, st>>>(w, b, N, mc->V.as<double>(), mc->gm.as<double>(), m, tol, mc->gmf.as<int>());
@@ -2303,8 +2329,7 @@ static void gm_issue(ilh_plan* pl, MovingCache* mc, int j0, int j1, int m)
     double* V = mc->V.as<double>();
     int* flags = mc->gmf.as<int>();
     for (int j = j0; j < j1; ++j) {
-        k_gm_matvec_f32<<<gR, 256, 0, st>>>(mc->P32.as<float>(), ld, V + (size_t)ld * j, mc->z.as<double>(), N, flags,
-                                            0);
+        prec_apply(pl, mc, V + (size_t)ld * j, mc->z.as<double>(), flags, 0);
         k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, mc->z.as<double>(), mc->w.as<double>(), N, flags);
         k_gm_arnoldi<<<1, 1024, 0, st>>>(V, ld, j, mc->w.as<double>(), N, mc->gm.as<double>(), m, flags);
     }
@@ -2320,9 +2345,8 @@ static void gm_end(ilh_plan* pl, MovingCache* mc, double* x, int m, int steps)
     if (steps <= 0) return;
     mc->last_iters = steps;
     k_gm_finish<<<1, 1024, 0, st>>>(mc->gm.as<double>(), m, steps, mc->V.as<double>(), ld, N, mc->w.as<double>());
-    k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 256, 0, st>>>(mc->P32.as<float>(), ld, mc->w.as<double>(), x, N,
-                                                             nullptr, 1);
     ILH_HIP(hipGetLastError());
+    prec_apply(pl, mc, mc->w.as<double>(), x, nullptr, 1);
 }
 
 // the whole solve on one plan: the steps taken, or -1 without convergence
@@ -2368,7 +2392,14 @@ static MovingCache* moving_setup(ilh_plan* pl, int m)
     mc->grid.ensure(sizeof(double) * ng);
     mc->S.ensure(sizeof(double) * (size_t)N * solve_ld(N));
     (void)mc->pinned(1);
-    if (!mc->have_ref) moving_reference(pl, mc);
+    // with a learned preconditioner attached no factorisation is needed
+    mc->use_band = pl->lpc != nullptr;
+    if (mc->use_band) {
+        plan_neighbours(pl, lpc_half_band(pl->lpc));
+        mc->band.ensure(sizeof(double) * (size_t)N * lpc_width(pl->lpc));
+    } else if (!mc->have_ref) {
+        moving_reference(pl, mc);
+    }
     return mc;
 }
 
@@ -2385,6 +2416,11 @@ static void moving_geometry(ilh_plan* pl, MovingCache* mc)
         assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -pl->g.dx * pl->g.dx, false, mc->S.as<double>(),
                  solve_ld(pl->N), true);
         mc->S_ok = true;
+    }
+    if (mc->use_band && !mc->band_ok) {
+        // the network's band for this geometry (features, BF16 forward pass)
+        lpc_band(pl->lpc, pl, mc->band.as<double>());
+        mc->band_ok = true;
     }
 }
 
@@ -2417,11 +2453,16 @@ static void moving_final(ilh_plan* pl, MovingCache* mc, double* d_f, double* d_s
 // FP32, with which GMRES converges in two or three.
 static int moving_refactor_solve(ilh_plan* pl, MovingCache* mc, double tol, int m)
 {
+    // the exact fallback: this geometry's own factorisation (also when the
+    // learned band was the preconditioner; the next solve uses the band again)
     moving_reference(pl, mc);
+    const bool band = mc->use_band;
+    mc->use_band = false;
     const int m2 = std::max(m, REFACTOR_STEPS);
     if (mc->V.bytes < sizeof(double) * (size_t)solve_ld(pl->N) * (m2 + 1))
         fail(ILH_ERR_STATE, "GMRES basis smaller than the re-solve needs");
     const int it2 = moving_gmres(pl, mc, mc->b.as<double>(), mc->amp.as<double>(), tol, m2);
+    mc->use_band = band;
     if (it2 < 0) fail(ILH_ERR_SINGULAR, "GMRES did not converge with the current geometry's factor");
     return m + it2;
 }
@@ -2430,6 +2471,20 @@ static void check_moving_args(double tol, int max_iter)
 {
     if (!(tol > 0.0) || tol >= 1.0) fail(ILH_ERR_ARG, "tol must be in (0, 1)");
     if (max_iter < 1 || max_iter > 200) fail(ILH_ERR_ARG, "max_iter must be in [1, 200]");
+}
+
+// The moving-body solves of this plan use the learned band of `model` as
+// their GMRES preconditioner (null: the FP32 reference inverse again).
+extern "C" int ilh_plan_set_preconditioner(ilh_plan* pl, ilh_lpc* model)
+{
+    return guarded([&] {
+        if (!pl) fail(ILH_ERR_ARG, "null plan");
+        pl->lpc = model;
+        if (pl->mov) {
+            pl->mov->band_ok = false;
+            pl->mov->use_band = false;
+        }
+    });
 }
 
 extern "C" int ilh_dirichlet_solve_moving(ilh_plan* pl, const double* fp, const double* fm, double* f, double* s,
@@ -2483,6 +2538,13 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
         std::vector<MovingCache*> mc(nprob, nullptr);
         for (int i = 0; i < nprob; ++i)
             if (plans[i]->N > 0) mc[i] = moving_setup(plans[i], m);
+        // learned bands first, plan by plan on the context stream (the model's
+        // work buffers are shared by its plans)
+        for (int i = 0; i < nprob; ++i)
+            if (mc[i] && mc[i]->use_band && !mc[i]->band_ok) {
+                lpc_band(plans[i]->lpc, plans[i], mc[i]->band.as<double>());
+                mc[i]->band_ok = true;
+            }
         const hipStream_t main = ctx->stream;
         struct Restore {
             ilh_ctx* c;

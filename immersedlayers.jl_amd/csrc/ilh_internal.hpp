@@ -166,6 +166,7 @@ struct ilh_ctx {
 };
 
 struct SolveCache;   // Schur factor and solve matrices of one composite
+struct ilh_lpc;      // learned preconditioner model (ilh_learned.hip)
 
 struct MovingCache;
 struct StokesCache;
@@ -202,6 +203,13 @@ struct ilh_plan {
     ilh::DevBuf filt_r1e;                     // R 1 on Edges{Primal} for the vector surface filter
     unsigned filt_gen_e = ~0u;
     std::atomic<int> refs{1};                 // the caller's handle, batches and heat integrators using it
+    // bodies: point offsets of each body's contiguous points (empty: one body)
+    std::vector<int> body_off;
+    unsigned body_gen = 0;
+    ilh::DevBuf nbr;                          // N x (2K+1) neighbours along each body's curve (learned preconditioner)
+    int nbr_K = -1;
+    unsigned nbr_gen = ~0u;
+    ilh_lpc* lpc = nullptr;                   // learned preconditioner of the moving-body solves (not owned)
 };
 
 namespace ilh {
@@ -380,6 +388,18 @@ void neumann_amp(hipStream_t st, const double* Mdf_p, const double* Mdf_m, const
                  const double* Mds_m, int ld, const double* vp, const double* vm, double* amp_f, double* amp_s,
                  double* df, double* dsv, int n);
 void check_finite(hipStream_t st, const double* a, size_t n, int* flag_dev);
+
+// learned preconditioner (ilh_learned.hip): neighbour table of the plan,
+// predicted band coefficients (N x (2K+1)) of its current geometry, and the
+// band's product z (+)= B v (skipped once flags[0] is set)
+void plan_neighbours(ilh_plan* pl, int K);
+void lpc_band(ilh_lpc* m, ilh_plan* pl, double* coef);
+int lpc_width(const ilh_lpc* m);
+int lpc_half_band(const ilh_lpc* m);
+void band_apply(hipStream_t st, const double* coef, const int* nbr, int W, const double* v, double* z, int n,
+                const int* flags, int accumulate);
+// S^-1 of the plan's create_RTLinvR, row-major with leading dimension solve_ld(N) (ilh_api.hip)
+void exact_inverse_rowmajor(ilh_plan* pl, double* Minv);
 
 }  // namespace ilh
 

@@ -125,6 +125,18 @@ SIGNATURES = {
     "ilh_batch_create": ([_i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "ilh_batch_launch": ([_vp], _i),
     "ilh_batch_destroy": ([_vp], _i),
+    "ilh_plan_set_bodies": ([_vp, _i, _vp], _i),
+    "ilh_lpc_create": ([_vp, _i, C.c_ulonglong, C.POINTER(_vp)], _i),
+    "ilh_lpc_destroy": ([_vp], _i),
+    "ilh_lpc_add_geometry": ([_vp, _vp], _i),
+    "ilh_lpc_rows": ([_vp], _i),
+    "ilh_lpc_train": ([_vp, _i, _i, _d, C.c_ulonglong, _dp], _i),
+    "ilh_lpc_dims": ([_vp, _ip], _i),
+    "ilh_lpc_params": ([_vp, _vp, _i], _i),
+    "ilh_lpc_loss_grad": ([_vp, _i, _vp, _vp, _dp, _vp], _i),
+    "ilh_lpc_predict": ([_vp, _vp, _vp, _vp], _i),
+    "ilh_lpc_standardization": ([_vp, _vp], _i),
+    "ilh_plan_set_preconditioner": ([_vp, _vp], _i),
     "ilh_mask": ([_vp, _i, _i, _vp, _i], _i),
     "ilh_surface_filter_build": ([_vp, _vp, _i], _i),
     "ilh_surface_filter_apply": ([_vp, _i, _vp, _vp, _i], _i),

@@ -242,6 +242,16 @@ class SurfaceScalarCache:
                _ptr(self._nx), _ptr(self._ny), _ptr(self._ds), ddftype, 0, C.byref(h))
         self.h = h
         _live["cache"].add(self)
+        self.body_offsets = _body_offsets(getattr(surface, "body", None), self.N)
+        if len(self.body_offsets) > 2:
+            off = np.ascontiguousarray(self.body_offsets, dtype=np.int32)
+            L.call("ilh_plan_set_bodies", self.h, len(off) - 1, off.ctypes.data_as(C.c_void_p))
+
+    def set_preconditioner(self, model):
+        """Moving-body solves use the learned band of `model` (None: the FP32
+        reference inverse)."""
+        self._lpc = model
+        L.call("ilh_plan_set_preconditioner", self.h, None if model is None else model.h)
 
     def _set_points(self, s):
         self._X = np.ascontiguousarray(s.X, dtype=np.float64)
@@ -288,6 +298,97 @@ class SurfaceScalarCache:
     def close(self):
         if getattr(self, "h", None):
             L.load().ilh_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _body_offsets(body, N):
+    """Point offsets of the bodies from the per-point body labels (each body's
+    points contiguous), [0, N] for one body."""
+    if body is None or N == 0:
+        return [0, N]
+    b = np.asarray(body)
+    starts = [0] + [i for i in range(1, N) if b[i] != b[i - 1]]
+    if len(set(b[starts].tolist())) != len(starts):
+        raise L.ArgumentError("each body's points must be contiguous")
+    return starts + [N]
+
+
+class LearnedPreconditioner:
+    """The learned Schur preconditioner for moving bodies (ilh_lpc): a network
+    predicting the band of S^-1 along each body's curve from the local
+    geometry, trained on the GPU on exact inverses the plain method computes."""
+
+    def __init__(self, ctx=None, half_band=16, seed=0):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        L.call("ilh_lpc_create", self.ctx.h, int(half_band), C.c_ulonglong(seed), C.byref(h))
+        self.h = h
+        d = (C.c_int * 7)()
+        L.call("ilh_lpc_dims", self.h, d)
+        self.dims = tuple(d)
+        self.K, self.W, self.F, self.Fp, self.H, self.Op, self.nparam = self.dims
+        _live["heat"].add(self)
+
+    def add_geometry(self, cache):
+        L.call("ilh_lpc_add_geometry", self.h, cache.h)
+
+    @property
+    def rows(self):
+        return L.load().ilh_lpc_rows(self.h)
+
+    def train(self, steps, batch=2048, lr=1e-3, seed=0):
+        loss = C.c_double()
+        L.call("ilh_lpc_train", self.h, int(steps), int(batch), float(lr), C.c_ulonglong(seed), C.byref(loss))
+        return loss.value
+
+    def params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        L.call("ilh_lpc_params", self.h, p.ctypes.data_as(C.c_void_p), 1)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparam:
+            raise L.DimensionMismatch("expected %d parameters" % self.nparam)
+        L.call("ilh_lpc_params", self.h, p.ctypes.data_as(C.c_void_p), 0)
+
+    def loss_grad(self, X_bits, T):
+        """Loss and gradients of one standardized batch (X: rows x Fp BF16 bit
+        patterns as uint16, T: rows x Op float32)."""
+        X_bits = np.ascontiguousarray(X_bits, dtype=np.uint16)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        rows = X_bits.shape[0]
+        if X_bits.shape != (rows, self.Fp) or T.shape != (rows, self.Op):
+            raise L.DimensionMismatch("batch shapes")
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_double()
+        L.call("ilh_lpc_loss_grad", self.h, rows, X_bits.ctypes.data_as(C.c_void_p), T.ctypes.data_as(C.c_void_p),
+               C.byref(loss), g.ctypes.data_as(C.c_void_p))
+        return loss.value, g
+
+    def predict(self, cache, band=True):
+        """(features N x F, band N x (2K+1) or None) of the cache's geometry."""
+        F = np.zeros((cache.N, self.F), dtype=np.float32)
+        B = np.zeros((cache.N, self.W)) if band else None
+        L.call("ilh_lpc_predict", self.h, cache.h, F.ctypes.data_as(C.c_void_p),
+               None if B is None else B.ctypes.data_as(C.c_void_p))
+        return F, B
+
+    def standardization(self):
+        out = np.zeros(2 * self.F + 2 * self.W, dtype=np.float32)
+        L.call("ilh_lpc_standardization", self.h, out.ctypes.data_as(C.c_void_p))
+        F, W = self.F, self.W
+        return out[:F], out[F:2 * F], out[2 * F:2 * F + W], out[2 * F + W:]
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().ilh_lpc_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -285,6 +285,48 @@ int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* plans, const 
                                        const double* const* fb_minus, double* const* f, double* const* s, double tol,
                                        int max_iter, int* iters);
 
+/* ---- learned preconditioner for moving bodies (no reference counterpart:
+ * a replacement for the per-step LU of the Schur matrix that update_system!
+ * forces, src/system.jl:26-50, src/problem.jl:95-105) ----
+ * A network (features of the local geometry -> 256 -> 256 -> 256 -> band,
+ * ReLU, BF16 weights and activations, FP32 accumulation on the matrix cores)
+ * predicts, for each surface point, the 2K+1 entries of S^-1 around it along
+ * its body's curve (S = create_RTLinvR).  It is trained on the GPU (Adam,
+ * FP32 master weights) on the bands of exact inverses that the plain method
+ * computes, and used as the GMRES preconditioner of the moving-body solves
+ * (ilh_dirichlet_solve_moving and its batched form), which fall back to the
+ * exact factorisation when GMRES does not converge within max_iter. */
+typedef struct ilh_lpc ilh_lpc;
+/* Point offsets of the bodies (offsets[0] = 0 < ... < offsets[nbodies] = N):
+   each body's points are contiguous and ordered along its closed curve
+   (RigidBodyTools' BodyList order).  Without it the points form one body. */
+int ilh_plan_set_bodies(ilh_plan* plan, int nbodies, const int* offsets);
+/* A model with half band K (band 2K+1, 1 <= K <= 31) and seeded initial weights. */
+int ilh_lpc_create(ilh_ctx* ctx, int half_band, unsigned long long seed, ilh_lpc** out);
+int ilh_lpc_destroy(ilh_lpc* model);
+/* Training rows of the plan's current geometry: its features and the band of
+   its exact inverse (one plain-method factorisation, N unit solves). */
+int ilh_lpc_add_geometry(ilh_lpc* model, ilh_plan* plan);
+int ilh_lpc_rows(const ilh_lpc* model);
+/* steps Adam steps (learning rate lr) on seeded minibatches of `batch` rows;
+   *loss = the steps' mean loss (standardized targets).  The standardization is
+   fixed by the first call. */
+int ilh_lpc_train(ilh_lpc* model, int steps, int batch, double lr, unsigned long long seed, double* loss);
+/* dims = [K, 2K+1, F (features), Fp (padded), 256 (width), Op (padded outputs), parameter count] */
+int ilh_lpc_dims(const ilh_lpc* model, int* dims);
+/* FP32 master parameters (W1 Fp x 256 row-major, b1, W2, b2, W3, b3, W4 256 x Op, b4) out (get) or in */
+int ilh_lpc_params(ilh_lpc* model, float* params, int get);
+/* loss and all parameter gradients of one standardized batch (X: rows x Fp
+   BF16 bit patterns, T: rows x Op FP32): the training step without the update */
+int ilh_lpc_loss_grad(ilh_lpc* model, int rows, const unsigned short* X, const float* T, double* loss, float* grads);
+/* features (N x F) and the predicted band (N x (2K+1), may be NULL) of the plan's geometry */
+int ilh_lpc_predict(ilh_lpc* model, ilh_plan* plan, float* features, double* band);
+/* [mu_x (F), sd_x (F), mu_y (2K+1), sd_y (2K+1)] */
+int ilh_lpc_standardization(ilh_lpc* model, float* out);
+/* moving-body solves of the plan use the model's band (NULL: the FP32
+   reference inverse); the model must outlive its use by the plan */
+int ilh_plan_set_preconditioner(ilh_plan* plan, ilh_lpc* model);
+
 /* ---- masks (src/surface_operators.jl:728-871) ----
  * mask(cache) = -L^-1 D_s 1 on Nodes{Primal} (1 inside the surfaces, on the side
  * opposite the normals; 1 everywhere without points); on Nodes{Dual} and

@@ -486,3 +486,52 @@ def test_surface_filter_pins():
     noisy = smooth + 0.3 * rng.standard_normal(geo.N)
     f6 = np.linalg.matrix_power(C, 6) @ noisy
     assert np.linalg.norm(f6 - smooth) < 0.5 * np.linalg.norm(noisy - smooth)
+
+
+# ------------------------------------------------------- learned network
+def test_learned_network_backward_matches_finite_differences():
+    """The FP64 restatement's backward pass (no BF16 rounding) against central
+    differences of its loss, on a small network of the same structure."""
+    import learned as LN
+    rng = np.random.default_rng(12)
+    K, H, Op = 2, 16, 16
+    W = 2 * K + 1
+    F = 6 * W + 3
+    Fp = 64
+    shapes = [(Fp, H), (H,), (H, H), (H,), (H, H), (H,), (H, Op), (Op,)]
+    n = sum(int(np.prod(s)) for s in shapes)
+    dims = (K, W, F, Fp, H, Op, n)
+    p = rng.standard_normal(n) * 0.3
+    X = rng.standard_normal((23, Fp))
+    T = rng.standard_normal((23, Op))
+    loss, g = LN.loss_grad(p, dims, X, T, bf16_round=False)
+    h = 1e-6
+    for i in rng.choice(n, 40, replace=False):
+        e = np.zeros(n)
+        e[i] = h
+        fd = (LN.loss_grad(p + e, dims, X, T, False)[0] - LN.loss_grad(p - e, dims, X, T, False)[0]) / (2 * h)
+        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_learned_features_geometry():
+    """Features of a circle: tangent/normal offsets of the neighbours follow
+    the chord geometry (the j-th neighbour at angle j dtheta sits at
+    (r sin(j dtheta), r (cos(j dtheta) - 1)) / dx in the point's frame, the
+    normal pointing outward), and the periodic index wraps within each body."""
+    import learned as LN
+    from ilhip import geometry as G
+    dx = 0.02
+    c1 = G.circle(0.5, 1.4 * dx)
+    c2 = G.circle(0.3, 1.4 * dx, 1.0, 0.0)
+    s = G.concat([c1, c2])
+    off = [0, c1.N, c1.N + c2.N]
+    K = 3
+    Fm = LN.features(s.X, s.Y, s.nx, s.ny, s.ds, dx, 103, 103, off, K)
+    W = 2 * K + 1
+    dth = 2 * np.pi / c1.N
+    j = np.arange(-K, K + 1)
+    # tangent (-ny, nx) is counter-clockwise for the outward normal
+    assert np.allclose(Fm[0, :W], 0.5 * np.sin(j * dth) / dx, atol=1e-9)
+    assert np.allclose(Fm[0, W:2 * W], 0.5 * (np.cos(j * dth) - 1) / dx, atol=1e-9)
+    nb = LN.neighbours(off, s.N, K)
+    assert nb[c1.N].min() >= c1.N and nb[0].max() < c1.N
