@@ -164,10 +164,12 @@ def bench_ops(args):
 
 def bench_batched(args):
     """64 independent Dirichlet problems (grids 256^2..1024^2, 1-3 bodies,
-    dx = 0.01) solved per step by one batched call; repeat solves with each
-    problem's Schur factor cached.  Also times the same step captured once as a
-    HIP graph and replayed, and the 64 solves issued one after another on one
-    stream."""
+    dx = 0.01) solved per step; repeat solves with each problem's Schur factor
+    cached.  The step is four launches over the tiles of every problem
+    (surface step, row, column and inverse-row passes).  value: the step
+    captured once as a HIP graph (after the autotuner picked each pass's
+    schedule and work-group shape) and replayed; also the eager batched call
+    and the 64 solves issued one after another on one stream."""
     import ilhip
     from ilhip import geometry as G
     probs = G.config_batched()
@@ -180,6 +182,20 @@ def bench_batched(args):
     fds = [ilhip.DeviceArray(ctx, (g.NX - 1) * (g.NY - 1)) for g, _, _ in probs]
     sds = [ilhip.DeviceArray(ctx, s.N) for _, s, _ in probs]
     nprob = len(probs)
+    t0 = time.perf_counter()
+    batch = ilhip.DirichletBatch(caches, fps, fds, sds)     # autotunes, then captures the step as one graph
+    ctx.synchronize()
+    t_tune = time.perf_counter() - t0
+    tuning = batch.tuning()
+    for _ in range(args.warmup):
+        batch.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.launch()
+    ctx.synchronize()
+    graphed = nprob * args.steps / (time.perf_counter() - t0)
+    batch.close()
     for _ in range(args.warmup):
         ilhip.dirichlet_solve_batched_device(caches, fps, fds, sds)
     ctx.synchronize()
@@ -188,14 +204,6 @@ def bench_batched(args):
         ilhip.dirichlet_solve_batched_device(caches, fps, fds, sds)
     ctx.synchronize()
     batched = nprob * args.steps / (time.perf_counter() - t0)
-    batch = ilhip.DirichletBatch(caches, fps, fds, sds)     # the same step captured as one graph
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.launch()
-    ctx.synchronize()
-    graphed = nprob * args.steps / (time.perf_counter() - t0)
-    batch.close()
     probs_dp = [ilhip.DirichletPoisson(c) for c in caches]
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -204,6 +212,16 @@ def bench_batched(args):
     ctx.synchronize()
     sequential = nprob * args.steps / (time.perf_counter() - t0)
     sizes = [g.NX - 1 for g, _, _ in probs]
+    # the whole step with the headline's byte model per problem (two
+    # convolutions' fields and spectra, the N x N solve matrix)
+    step_bytes = 0
+    for g, s, _ in probs:
+        nx, ny = g.primal_shape()
+        P = 16
+        while P < 2 * max(nx, ny) - 2:
+            P *= 2
+        step_bytes += 9 * 8 * nx * ny + 2 * 8 * P * (P // 2 + 1) + 8 * s.N * s.N
+    step_gbs = step_bytes * graphed / nprob / 1e9
     # value: the captured batch replayed (one graph launch per step of 64 solves)
     out = {"metric": "batched Dirichlet solves/s (64 mixed problems)", "value": graphed, "unit": "solves/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * nprob / graphed,
@@ -212,6 +230,10 @@ def bench_batched(args):
            "config": {"workload": "64 independent problems per step, Schur factors cached",
                       "grid_primal_sizes": sorted(set(sizes)), "npts_total": int(sum(s.N for _, s, _ in probs)),
                       "dx": 0.01},
+           "roofline": {"bound": "hbm", "kernel": "whole batched step (4 launches)",
+                        "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                        "traffic": None, "algorithmic_bytes_per_step": step_bytes},
+           "tuning": tuning, "tune_and_capture_s": t_tune,
            "eager_batched_solves_per_s": batched, "sequential_solves_per_s": sequential, "prep_s": t_prep}
     print(json.dumps(out), flush=True)
 
@@ -382,6 +404,9 @@ def main():
     fp = ilhip.DeviceArray.from_host(ctx, fb)
     fd = ilhip.DeviceArray(ctx, ng)
     sd = ilhip.DeviceArray(ctx, s.N)
+    # per-kernel launch time, before the timed region: each stage launched
+    # alone, back to back, between two HIP events on the plan's stream
+    stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
     for _ in range(args.warmup):
         prob.solve_device(fp, None, fd, sd)
     ctx.synchronize()
@@ -394,9 +419,6 @@ def main():
     sps = args.steps / wall
     f_dev = fd.download()
     assert np.allclose(f_dev, f_gpu.flat(), rtol=0, atol=0)
-    # per-kernel launch time: each stage launched alone, back to back, between
-    # two HIP events on the plan's stream (outside the timed region)
-    stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
 
     # repeat solves with many boundary data at once (8 right-hand sides per
     # call, seeded scalings of the same data): solves/s of the batched form

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 
@@ -222,6 +223,9 @@ void ilh::ctx_release(ilh_ctx* c)
         }
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->tev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->fused) fused_destroy(c->fused);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -676,8 +680,18 @@ extern "C" int ilh_inverse_laplacian(ilh_plan* pl, int kind, double* out, const 
 {
     PL_CHECK
     return guarded([&] {
-        const int gk = kind_to_gk(kind);
         Io io(pl, mem);
+        if (kind == ILH_EDGES_PRIMAL) {
+            const size_t nu = pl->g.n(GK_EDGE_U), n = pl->g.n_edges();
+            const double* d_in = io.in(in, n, pl->io_a);
+            double* d_out = io.out(out, n, pl->io_b);
+            conv_dense(pl, GK_EDGE_U, d_in, d_out);
+            conv_dense(pl, GK_EDGE_V, d_in + nu, d_out + nu);
+            io.back(out, d_out, n);
+            io.done();
+            return;
+        }
+        const int gk = kind_to_gk(kind);
         const size_t n = pl->g.n(gk);
         const double* d_in = io.in(in, n, pl->io_a);
         double* d_out = io.out(out, n, pl->io_b);
@@ -874,14 +888,57 @@ void build_stokes_ka(ilh_plan* pl, double* K)
     ILH_HIP(hipStreamSynchronize(st));   // the batch buffers are released on return
 }
 
+// create_GLinvD_symm = -G_s^symm L^-1 D_s^symm (2N x 2N, column-major), column
+// batches like create_CL2invCT: unit VectorData columns, D_s^symm onto
+// Edges{Primal}, the inverse Laplacian of each edge component on its own grid
+// (the components are gathered into contiguous batches for the batched
+// convolution), G_s^symm back to VectorData.
+void build_glinvd_symm(ilh_plan* pl, double scale, double* K)
+{
+    const int n2 = 2 * pl->N;
+    const Grid& g = pl->g;
+    const size_t nu = g.n(GK_EDGE_U), nv = g.n(GK_EDGE_V), ne = g.n_edges();
+    hipStream_t st = pl->ctx->stream;
+    const size_t per_col = 3 * ne * sizeof(double) + 4 * std::max(nu, nv) * sizeof(double2);
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>({(size_t)32, (size_t)n2, ((size_t)1 << 30) / per_col}));
+    DevBuf amp, fe, c1, c2;
+    amp.ensure(sizeof(double) * (size_t)n2 * nb);
+    fe.ensure(sizeof(double) * ne * nb);
+    c1.ensure(sizeof(double) * std::max(nu, nv) * nb);
+    c2.ensure(sizeof(double) * std::max(nu, nv) * nb);
+    for (int c0 = 0; c0 < n2; c0 += nb) {
+        const int m = std::min(nb, n2 - c0);
+        const size_t na = (size_t)n2 * m;
+        k_unit_columns<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(amp.as<double>(), n2, c0, m);
+        ILH_HIP(hipGetLastError());
+        scatter_op_multi(pl, OP_DSYM, amp.as<double>(), n2, m, fe.as<double>(), ne);
+        for (int comp = 0; comp < 2; ++comp) {
+            const size_t off = comp ? nu : 0, n = comp ? nv : nu;
+            const int gk = comp ? GK_EDGE_V : GK_EDGE_U;
+            ILH_HIP(hipMemcpy2DAsync(c1.as<double>(), sizeof(double) * n, fe.as<double>() + off, sizeof(double) * ne,
+                                     sizeof(double) * n, m, hipMemcpyDeviceToDevice, st));
+            conv_dense_multi(pl, gk, c1.as<double>(), m, c2.as<double>());
+            ILH_HIP(hipMemcpy2DAsync(fe.as<double>() + off, sizeof(double) * ne, c2.as<double>(), sizeof(double) * n,
+                                     sizeof(double) * n, m, hipMemcpyDeviceToDevice, st));
+        }
+        gather_op_multi(pl, OP_GSYM, fe.as<double>(), ne, m, K + (size_t)n2 * c0, n2);
+    }
+    axpby(st, K, K, -scale, 0.0, (size_t)n2 * n2);
+    ILH_HIP(hipStreamSynchronize(st));   // the batch buffers are released on return
+}
+
 // order of the matrix a Schur kind builds
-int schur_dim(const ilh_plan* pl, int kind) { return kind == ILH_SCHUR_CL2INVCT ? 2 * pl->N : pl->N; }
+int schur_dim(const ilh_plan* pl, int kind)
+{
+    return kind == ILH_SCHUR_CL2INVCT || kind == ILH_SCHUR_GLINVD_SYMM ? 2 * pl->N : pl->N;
+}
 
 void build_schur(ilh_plan* pl, int kind, double scale, double* K)
 {
     const double dx2 = pl->g.dx * pl->g.dx;
     switch (kind) {
     case ILH_SCHUR_CL2INVCT: build_cl2invct(pl, scale, K); break;
+    case ILH_SCHUR_GLINVD_SYMM: build_glinvd_symm(pl, scale, K); break;
     case ILH_SCHUR_RTLINVR: assemble(pl, OP_E_PRIMAL, OP_R_PRIMAL, GK_PRIMAL, -scale * dx2, false, K); break;
     case ILH_SCHUR_CLINVCT: assemble(pl, OP_CS, OP_CST, GK_DUAL, -scale * dx2, false, K); break;
     case ILH_SCHUR_GLINVD: assemble(pl, OP_GS, OP_DS, GK_PRIMAL, -scale * dx2, false, K); break;
@@ -1057,7 +1114,7 @@ extern "C" int ilh_schur_build(ilh_plan* pl, int kind, double scale, double* out
 {
     return guarded([&] {
         Io io(pl, mem);
-        if (kind < 0 || kind > ILH_SCHUR_CL2INVCT) fail(ILH_ERR_ARG, "unknown Schur kind");
+        if (kind < 0 || kind > ILH_SCHUR_GLINVD_SYMM) fail(ILH_ERR_ARG, "unknown Schur kind");
         if (kind != ILH_SCHUR_NRTRN) need_margin(pl, "Schur build");
         const int N = schur_dim(pl, kind);
         const size_t NN = (size_t)N * N;
@@ -1072,7 +1129,7 @@ extern "C" int ilh_schur_factor(ilh_plan* pl, int kind)
 {
     return guarded([&] {
         Io io(pl, ILH_MEM_DEVICE);
-        if (kind < 0 || kind > ILH_SCHUR_CL2INVCT) fail(ILH_ERR_ARG, "unknown Schur kind");
+        if (kind < 0 || kind > ILH_SCHUR_GLINVD_SYMM) fail(ILH_ERR_ARG, "unknown Schur kind");
         if (kind != ILH_SCHUR_NRTRN) need_margin(pl, "Schur factor");
         delete pl->fac[kind];
         pl->fac[kind] = nullptr;
@@ -1091,7 +1148,7 @@ extern "C" int ilh_schur_solve(ilh_plan* pl, int kind, double* x, const double* 
 {
     return guarded([&] {
         Io io(pl, mem);
-        if (kind < 0 || kind > ILH_SCHUR_CL2INVCT || !pl->fac[kind])
+        if (kind < 0 || kind > ILH_SCHUR_GLINVD_SYMM || !pl->fac[kind])
             fail(ILH_ERR_STATE, "ilh_schur_factor must be called first");
         const int N = schur_dim(pl, kind);
         const double* d_b = io.in(b, N, pl->io_a);
@@ -1856,6 +1913,72 @@ void batch_issue(ilh_ctx* ctx, int nprob, ilh_plan* const* plans, SolveCache* co
     }
 }
 
+// The fused form (one launch per pass over every problem, ilh_batch.hip)
+// serves a batch whose problems' FFT periods it supports; problems without
+// surface points only get their field zeroed.
+bool batch_fusable(int nprob, ilh_plan* const* plans)
+{
+    for (int i = 0; i < nprob; ++i) {
+        if (plans[i]->N == 0) continue;
+        const ConvPlan& C = get_conv(plans[i], GK_PRIMAL);
+        if (!fused_supported(C.Px, C.Py)) return false;
+    }
+    return true;
+}
+
+// the problems' operands (device pointers) for the fused passes; zero: the
+// fields of problems without surface points
+void batch_probs(ilh_ctx* ctx, int nprob, ilh_plan* const* plans, SolveCache* const* sc,
+                 const double* const* fp, const double* const* fm, double* const* f, double* const* s,
+                 std::vector<BatchProb>& probs, std::vector<std::pair<double*, size_t>>& zero)
+{
+    probs.clear();
+    zero.clear();
+    for (int i = 0; i < nprob; ++i) {
+        ilh_plan* pl = plans[i];
+        const int N = pl->N;
+        if (N == 0) {
+            zero.emplace_back(f[i], pl->g.n(GK_PRIMAL));
+            continue;
+        }
+        const SourceSet& src = sc[i]->src1;
+        if (src.npairs_dev) fail(ILH_ERR_STATE, "the source pairs are not pinned");
+        ConvPlan& C = get_conv(pl, GK_PRIMAL);
+        BatchProb b;
+        std::memset(&b, 0, sizeof(b));   // padding too: batches are compared bytewise
+        b.Mp = sc[i]->M[0].as<double>();
+        b.Mm = fm[i] ? sc[i]->M[1].as<double>() : nullptr;
+        b.ld = solve_ld(N);
+        b.fp = fp[i];
+        b.fm = fm[i];
+        b.amp = sc[i]->amp1.as<double>();
+        b.s = s[i];
+        b.n = N;
+        b.pinfo = src.pinfo;
+        b.cells = src.bins.cells;
+        b.cai = src.bins.cai;
+        b.cw = src.bins.cw;
+        b.npairs = src.npairs;
+        b.nx = C.nx;
+        b.ny = C.ny;
+        b.Px = C.Px;
+        b.Py = C.Py;
+        b.T1 = C.T1.as<double2>();
+        b.slot = src.slot_of_row;
+        b.spec = C.spec.as<double>();
+        b.T2 = C.T2.as<double2>();
+        b.out = f[i];
+        b.twx = fft_twiddles(ctx, C.Px);
+        b.twy = fft_twiddles(ctx, C.Py);
+        probs.push_back(b);
+    }
+}
+
+bool same_probs(const std::vector<BatchProb>& a, const std::vector<BatchProb>& b)
+{
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), sizeof(BatchProb) * a.size()) == 0;
+}
+
 }  // namespace
 
 extern "C" int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
@@ -1870,8 +1993,47 @@ extern "C" int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, co
         ilh_ctx* ctx = batch_ctx(nprob, plans);
         std::vector<SolveCache*> sc(nprob);
         for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i], fb_minus && fb_minus[i]);
-        batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, mem);
-        if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(ctx->stream));
+        if (!batch_fusable(nprob, plans)) {
+            batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, mem);
+            if (mem == ILH_MEM_HOST) ILH_HIP(hipStreamSynchronize(ctx->stream));
+            return;
+        }
+        // device operands: the caller's, or each plan's staging buffers
+        std::vector<const double*> dp(nprob), dm(nprob);
+        std::vector<double*> df(nprob), ds(nprob);
+        for (int i = 0; i < nprob; ++i) {
+            ilh_plan* pl = plans[i];
+            Io io(pl, mem);
+            const int N = pl->N;
+            dp[i] = io.in(fb_plus[i], N, pl->io_a);
+            const double* fm_i = fb_minus ? fb_minus[i] : nullptr;
+            dm[i] = fm_i ? io.in(fm_i, N, pl->io_c) : nullptr;
+            df[i] = io.out(f[i], pl->g.n(GK_PRIMAL), pl->io_b);
+            ds[i] = io.out(s[i], N, pl->io_d);
+        }
+        std::vector<BatchProb> probs;
+        std::vector<std::pair<double*, size_t>> zero;
+        batch_probs(ctx, nprob, plans, sc.data(), dp.data(), dm.data(), df.data(), ds.data(), probs, zero);
+        if (!probs.empty()) {
+            if (!ctx->fused || !same_probs(fused_probs(ctx->fused), probs)) {
+                if (ctx->fused) {
+                    ILH_HIP(hipStreamSynchronize(ctx->stream));
+                    fused_destroy(ctx->fused);
+                    ctx->fused = nullptr;
+                }
+                ctx->fused = fused_create(ctx, probs);
+            }
+            fused_run(ctx->fused);
+        }
+        for (auto& z : zero) fill_zero(ctx->stream, z.first, z.second);
+        if (mem == ILH_MEM_HOST) {
+            for (int i = 0; i < nprob; ++i) {
+                Io io(plans[i], mem);
+                io.back(f[i], df[i], plans[i]->g.n(GK_PRIMAL));
+                io.back(s[i], ds[i], plans[i]->N);
+            }
+            ILH_HIP(hipStreamSynchronize(ctx->stream));
+        }
     });
 }
 
@@ -1883,6 +2045,11 @@ struct ilh_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<ilh_plan*> plans;
     std::vector<unsigned> gens;   // plan generations the graph's buffers belong to
+    FusedBatch* fused = nullptr;  // the fused passes (autotuned), when the batch's periods allow them
+    ~ilh_batch()
+    {
+        if (fused) fused_destroy(fused);
+    }
 };
 
 extern "C" int ilh_batch_create(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
@@ -1894,38 +2061,62 @@ extern "C" int ilh_batch_create(int nprob, ilh_plan* const* plans, const double*
         ilh_ctx* ctx = batch_ctx(nprob, plans);
         std::vector<SolveCache*> sc(nprob);
         for (int i = 0; i < nprob; ++i) sc[i] = prepare_dirichlet(plans[i], fb_minus && fb_minus[i]);
-        // one uncaptured pass: every lazily built table and attribute exists
-        // before capture (capture must record kernels only)
-        batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, ILH_MEM_DEVICE);
-        ILH_HIP(hipStreamSynchronize(ctx->stream));
-        ilh_batch* b = new ilh_batch();
+        std::unique_ptr<ilh_batch> own(new ilh_batch());
+        ilh_batch* b = own.get();
         b->ctx = ctx;
         b->plans.assign(plans, plans + nprob);
         for (int i = 0; i < nprob; ++i) b->gens.push_back(plans[i]->gen);
-        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) {
-            delete b;
-            fail(ILH_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+        std::vector<std::pair<double*, size_t>> zero;
+        if (batch_fusable(nprob, plans)) {
+            std::vector<const double*> fm(nprob, nullptr);
+            if (fb_minus) fm.assign(fb_minus, fb_minus + nprob);
+            std::vector<BatchProb> probs;
+            batch_probs(ctx, nprob, plans, sc.data(), fb_plus, fm.data(), f, s, probs, zero);
+            if (!probs.empty()) {
+                b->fused = fused_create(ctx, probs);
+                fused_tune(b->fused);   // also the one uncaptured pass below needs
+            }
         }
+        auto issue = [&] {
+            if (b->fused || !zero.empty()) {
+                if (b->fused) fused_run(b->fused);
+                for (auto& z : zero) fill_zero(ctx->stream, z.first, z.second);
+            } else {
+                batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, ILH_MEM_DEVICE);
+            }
+        };
+        // one uncaptured pass: every lazily built table and attribute exists
+        // before capture (capture must record kernels only)
+        issue();
+        ILH_HIP(hipStreamSynchronize(ctx->stream));
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) fail(ILH_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
         try {
-            batch_issue(ctx, nprob, plans, sc.data(), fb_plus, fb_minus, f, s, ILH_MEM_DEVICE);
+            issue();
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(ctx->stream, &g);
             if (g) (void)hipGraphDestroy(g);
-            delete b;
             throw;
         }
         e = hipStreamEndCapture(ctx->stream, &b->graph);
         if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
         if (e != hipSuccess) {
             if (b->graph) (void)hipGraphDestroy(b->graph);
-            delete b;
             fail(ILH_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
         }
         ctx_retain(ctx);
         for (ilh_plan* p : b->plans) plan_retain(p);
-        *out = b;
+        *out = own.release();
+    });
+}
+
+extern "C" int ilh_batch_tuning(const ilh_batch* b, int* config, float* pass_ms)
+{
+    return guarded([&] {
+        if (!b) fail(ILH_ERR_ARG, "null batch");
+        if (!b->fused) fail(ILH_ERR_STATE, "the batch runs per-problem kernels (an FFT period the fused passes lack)");
+        fused_config(b->fused, config, pass_ms);
     });
 }
 
@@ -1949,6 +2140,8 @@ extern "C" int ilh_batch_destroy(ilh_batch* b)
         (void)hipStreamSynchronize(b->ctx->stream);
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->fused) fused_destroy(b->fused);
+        b->fused = nullptr;
         for (ilh_plan* p : b->plans) plan_release(p);
         ctx_release(b->ctx);
         delete b;
@@ -2622,9 +2815,9 @@ extern "C" int ilh_profile_dirichlet(ilh_plan* pl, const double* fp, const doubl
         const int N = pl->N;
         if (N == 0) fail(ILH_ERR_ARG, "no surface points");
         hipStream_t st = pl->ctx->stream;
-        hipEvent_t e0, e1;
-        ILH_HIP(hipEventCreate(&e0));
-        ILH_HIP(hipEventCreate(&e1));
+        for (hipEvent_t& e : pl->ctx->tev)
+            if (!e) ILH_HIP(hipEventCreate(&e));
+        const hipEvent_t e0 = pl->ctx->tev[0], e1 = pl->ctx->tev[1];
         float ms = 0.f;
         auto timed = [&](auto&& body) {
             ILH_HIP(hipEventRecord(e0, st));
@@ -2648,8 +2841,6 @@ extern "C" int ilh_profile_dirichlet(ilh_plan* pl, const double* fp, const doubl
             for (int k = 1; k < 4; ++k)
                 stage_ms[k] = timed([&] { conv_stage(pl, GK_PRIMAL, sc->src1, sc->amp1.as<double>(), f, k - 1); });
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         if (total_ms) *total_ms = tot;
     });
 }

@@ -151,6 +151,8 @@ struct ConvPlan {
     DevBuf T1m, T2m;          // the same for batches of right-hand sides (copies back to back)
 };
 
+struct FusedBatch;            // fused batched solves (ilh_batch.hip)
+
 }  // namespace ilh
 
 struct ilh_ctx {
@@ -160,6 +162,8 @@ struct ilh_ctx {
     static constexpr int POOL = 4;            // side streams of batched calls (one per hardware queue)
     hipStream_t pool[POOL] = {};
     hipEvent_t ev[POOL + 1] = {};
+    hipEvent_t tev[2] = {};                   // timing events of the profiling entry points (made on first use)
+    ilh::FusedBatch* fused = nullptr;         // the last eager fused batch (reused while its problems are unchanged)
     // owners: the caller's handle, every plan and batch of the context; the
     // context is torn down when the last one lets go (any destruction order)
     std::atomic<int> refs{1};
@@ -183,8 +187,8 @@ struct ilh_plan {
     bool wins_ok = false;
     ilh::RowBins bins[ilh::OP_COUNT][4];     // row bins of scatter ops per component grid
     bool bins_ok[ilh::OP_COUNT] = {};
-    ilh::ConvPlan conv[2];                    // primal, dual
-    bool conv_ok[2] = {};
+    ilh::ConvPlan conv[4];                    // primal, dual, u edges, v edges
+    bool conv_ok[4] = {};
     ilh::DevBuf lgf;                          // LGF table tmx x tmy (covers both kinds)
     int lgf_mx = 0, lgf_my = 0;
     ilh::DevBuf io_a, io_b, io_c, io_d;       // staging for host-memory calls
@@ -192,7 +196,7 @@ struct ilh_plan {
     ilh::DevBuf lu_tinv, lu_perm, lu_tmp, lu_pmap;   // LU scratch (block inverses, row permutations)
     SolveCache* dir = nullptr;                // Dirichlet
     SolveCache* neu = nullptr;                // Neumann
-    SolveCache* fac[5] = {};                  // ilh_schur_factor caches per kind
+    SolveCache* fac[6] = {};                  // ilh_schur_factor caches per kind
     StokesCache* sto = nullptr;               // Stokes flow (vector cache) solve matrices
     MovingCache* mov = nullptr;               // moving-body Dirichlet solves (kept across update_points)
     unsigned gen = 0;                         // incremented whenever the geometry caches are dropped
@@ -287,6 +291,50 @@ void conv_sources_ext(ilh_plan* pl, int gk, const SourceSet& src, const double* 
                       double2* T2);
 void conv_sources_multi(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, size_t amp_stride, int nb,
                         double* out);
+// the FP64 twiddle table of period P (ilh_fft.hip), built on first use
+const double2* fft_twiddles(ilh_ctx* ctx, int P);
+
+// ---------------------------------------- fused batched solves (ilh_batch.hip)
+// One problem of a fused batch of Dirichlet repeat solves: device operands.
+struct BatchProb {
+    const double* Mp;          // exterior-data solve matrix (row-major, leading dimension ld)
+    const double* Mm;          // interior-data one, or null
+    const double* fp;          // boundary data f+ (n), f- (n or null)
+    const double* fm;
+    double* amp;               // amplitudes [s; f+ - f-] (2n)
+    double* s;                 // surface strength out (n)
+    int n, ld;
+    const int4* pinfo;         // merged source pairs (pinned count npairs)
+    const int4* cells;
+    const int* cai;
+    const double* cw;
+    int npairs;
+    int nx, ny, Px, Py;        // primal grid and periods
+    double2* T1;               // the problem's spectra buffers
+    const int* slot;
+    const double* spec;
+    double2* T2;
+    double* out;               // grid field out (nx ny)
+    const double2* twx;        // twiddles of Px and Py
+    const double2* twy;
+};
+// Per pass (surface step, rows, columns, inverse rows): schedule (0 one
+// work-group per tile, 1 persistent queues), work-groups per CU of the
+// queue schedule, threads per work-group; the row pass's amplitude LDS cap.
+struct FusedCfg {
+    int sched[4], mult[4], tpb[4];
+    int amp_cap;
+};
+struct FusedBatch;
+bool fused_supported(int Px, int Py);
+FusedBatch* fused_create(ilh_ctx* ctx, const std::vector<BatchProb>& probs);
+void fused_destroy(FusedBatch* fb);
+void fused_run(FusedBatch* fb);
+void fused_tune(FusedBatch* fb);
+// cfg: 13 ints (sched, mult, threads of each pass; the amplitude cap), ms:
+// the tuned time of each pass (0 before tuning)
+void fused_config(const FusedBatch* fb, int* cfg, float* ms);
+const std::vector<BatchProb>& fused_probs(const FusedBatch* fb);
 
 // operators on the plan's geometry (ilh_api.hip): scatter R/D_s-type ops to the
 // grid, gather E-type ops to the surface; inverse of a dense N x N matrix

@@ -21,6 +21,7 @@ from .core import (Context, DeviceArray, Nodes, Edges, ScalarData, VectorData, S
                    dirichlet_solve_moving_batched,
                    EdgeGradient, SurfaceVectorCache, surface_divergence_symm, surface_grad_symm,
                    regularize_normal_symm, normal_interpolate_symm, create_CL2invCT, create_CLinvCT_scalar,
+                   create_GLinvD_symm,
                    StokesFlow, LearnedPreconditioner)
 from .geometry import GridSpec as PhysicalGrid
 

@@ -14,7 +14,7 @@ OK = 0
 ERR_ARG, ERR_DIM, ERR_NONFINITE, ERR_SINGULAR, ERR_HIP, ERR_OOM, ERR_RANGE, ERR_STATE = -1, -2, -3, -4, -5, -6, -7, -8
 MEM_HOST, MEM_DEVICE = 0, 1
 NODES_PRIMAL, NODES_DUAL, EDGES_PRIMAL = 0, 1, 2
-SCHUR_RTLINVR, SCHUR_CLINVCT, SCHUR_GLINVD, SCHUR_NRTRN, SCHUR_CL2INVCT = 0, 1, 2, 3, 4
+SCHUR_RTLINVR, SCHUR_CLINVCT, SCHUR_GLINVD, SCHUR_NRTRN, SCHUR_CL2INVCT, SCHUR_GLINVD_SYMM = 0, 1, 2, 3, 4, 5
 SCHUR_CLINVCT_SCALAR = SCHUR_CLINVCT
 DDF_YANG3, DDF_ROMA, DDF_M4PRIME, DDF_WITCHHAT = 0, 1, 2, 3
 
@@ -124,6 +124,7 @@ SIGNATURES = {
     "ilh_dirichlet_solve_batched": ([_i, _vp, _vp, _vp, _vp, _vp, _i], _i),
     "ilh_batch_create": ([_i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "ilh_batch_launch": ([_vp], _i),
+    "ilh_batch_tuning": ([_vp, _vp, _vp], _i),
     "ilh_batch_destroy": ([_vp], _i),
     "ilh_plan_set_bodies": ([_vp, _i, _vp], _i),
     "ilh_lpc_create": ([_vp, _i, C.c_ulonglong, C.POINTER(_vp)], _i),

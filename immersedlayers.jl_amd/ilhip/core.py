@@ -160,6 +160,7 @@ class Nodes:
 
 class Edges:
     """Edges{Primal} grid data: one buffer [u; v], u (NX, NY-1), v (NX-1, NY)."""
+    ilh_kind = L.EDGES_PRIMAL
 
     def __init__(self, g, data=None, dtype=None):
         nu, nv = g.NX * (g.NY - 1), (g.NX - 1) * g.NY
@@ -645,8 +646,12 @@ def filter_surface(s, cache, power=1):
     return out
 
 
+def _schur_dim(cache, kind):
+    return 2 * cache.N if kind in (L.SCHUR_CL2INVCT, L.SCHUR_GLINVD_SYMM) else cache.N
+
+
 def _schur(cache, kind, scale):
-    n = 2 * cache.N if kind == L.SCHUR_CL2INVCT else cache.N
+    n = _schur_dim(cache, kind)
     A = np.zeros(n * n)
     L.call("ilh_schur_build", cache.h, kind, C.c_double(scale), _ptr(A), L.MEM_HOST)
     return A.reshape((n, n), order="F")
@@ -671,6 +676,11 @@ def create_nRTRn(cache, scale=1.0):
 def create_CL2invCT(cache, scale=1.0):
     """-C_s L^-2 C_s^T on VectorData (src/matrix_operators.jl:95-125), 2N x 2N."""
     return _schur(cache, L.SCHUR_CL2INVCT, scale)
+
+
+def create_GLinvD_symm(cache, scale=1.0):
+    """-G_s^symm L^-1 D_s^symm on VectorData (src/matrix_operators.jl:165-185), 2N x 2N."""
+    return _schur(cache, L.SCHUR_GLINVD_SYMM, scale)
 
 
 def create_CLinvCT_scalar(cache, scale=1.0):
@@ -841,6 +851,21 @@ class DirichletBatch:
     def launch(self):
         L.call("ilh_batch_launch", self.h)
 
+    PASSES = ("surface", "rows", "columns", "inverse_rows")
+
+    def tuning(self):
+        """The autotuner's choice per pass ({schedule, workgroups_per_cu,
+        threads, ms}) and the row pass's amplitude LDS cap."""
+        cfg = (C.c_int * 13)()
+        ms = (C.c_float * 4)()
+        L.call("ilh_batch_tuning", self.h, cfg, ms)
+        out = {}
+        for k, name in enumerate(self.PASSES):
+            out[name] = {"schedule": ("dispatch", "queue")[cfg[3 * k]], "workgroups_per_cu": cfg[3 * k + 1],
+                         "threads": cfg[3 * k + 2], "ms": float(ms[k])}
+        out["amp_lds_cap"] = cfg[12]
+        return out
+
     def close(self):
         if self.h:
             L.load().ilh_batch_destroy(self.h)
@@ -919,7 +944,7 @@ class SchurFactor:
         L.call("ilh_schur_factor", cache.h, kind)
 
     def solve(self, b):
-        b = _vec(b, 2 * self.cache.N if self.kind == L.SCHUR_CL2INVCT else self.cache.N)
+        b = _vec(b, _schur_dim(self.cache, self.kind))
         x = np.zeros_like(b)
         L.call("ilh_schur_solve", self.cache.h, self.kind, _ptr(x), _ptr(b), L.MEM_HOST)
         return x

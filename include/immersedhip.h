@@ -60,6 +60,9 @@ enum { ILH_NODES_PRIMAL = 0, ILH_NODES_DUAL = 1, ILH_EDGES_PRIMAL = 2 };
 /* Schur-complement kinds (src/matrix_operators.jl) */
 enum { ILH_SCHUR_RTLINVR = 0, ILH_SCHUR_CLINVCT = 1, ILH_SCHUR_GLINVD = 2, ILH_SCHUR_NRTRN = 3,
        ILH_SCHUR_CL2INVCT = 4,
+       /* create_GLinvD_symm of the vector cache (src/matrix_operators.jl:165-185):
+          -G_s^symm L^-1 D_s^symm on VectorData, 2N x 2N */
+       ILH_SCHUR_GLINVD_SYMM = 5,
        /* create_CLinvCT_scalar of the vector cache (src/matrix_operators.jl:71-91) is
           the same matrix as create_CLinvCT of the scalar cache: its scalar curl
           regularizes n f onto the edges exactly as the scalar cache does */
@@ -159,7 +162,8 @@ int ilh_curl_nodes_to_edges(ilh_plan* plan, double* q_edges, const double* s, in
 int ilh_curl_edges_to_nodes(ilh_plan* plan, double* w, const double* q_edges, int mem);
 /* laplacian!(w, s) on Nodes{Primal|Dual}  (src/grid_operators.jl:114-134) */
 int ilh_laplacian(ilh_plan* plan, int grid_kind, double* out, const double* in, int mem);
-/* inverse_laplacian!(sol, rhs) / in place when out == in  (src/grid_operators.jl:144-179) */
+/* inverse_laplacian!(sol, rhs) / in place when out == in  (src/grid_operators.jl:144-179);
+   grid_kind ILH_EDGES_PRIMAL convolves each component on its own grid */
 int ilh_inverse_laplacian(ilh_plan* plan, int grid_kind, double* out, const double* in, int mem);
 
 /* ---- FP32 data (caches built with dtype = Float32, src/cache.jl:164-201) ----
@@ -183,7 +187,10 @@ int ilh_inverse_laplacian_f32(ilh_plan* plan, int grid_kind, float* out, const f
    (src/matrix_operators.jl:9-30,40-61,135-155,225-244).  `out` is N x N
    column-major.  ILH_SCHUR_CL2INVCT: create_CL2invCT of the vector cache,
    -C_s L^-2 C_s^T on VectorData (src/matrix_operators.jl:95-125), 2N x 2N;
-   factor and solve take 2N-vectors for it. */
+   ILH_SCHUR_GLINVD_SYMM: create_GLinvD_symm of the vector cache,
+   -G_s^symm L^-1 D_s^symm with the inverse Laplacian of each Edges{Primal}
+   component (src/matrix_operators.jl:165-185), 2N x 2N; factor and solve take
+   2N-vectors for both. */
 int ilh_schur_build(ilh_plan* plan, int kind, double scale, double* out, int mem);
 /* Factor the plan's own S of `kind` (LU with partial pivoting, kept on the device).
    The reference re-factorises inside every `S\s` (test/literate/problems.jl:291). */
@@ -240,24 +247,39 @@ int ilh_dirichlet_solve_noalloc(ilh_plan* plan, const double* fb_plus, const dou
 /* ---- batched forms over arrays of plans ---- */
 /* Batched form of ilh_dirichlet_solve: nprob independent plans of one context
    (any mix of grid sizes and point counts), array k of each pointer array
-   belonging to plans[k]; fb_minus may be NULL, or hold NULL entries.  The
-   problems' solves run concurrently on the context's side streams; with
-   ILH_MEM_DEVICE the call is asynchronous on the context stream like the
-   single-problem form.  (The reference solves problems one at a time; this
-   is the batched form of test/literate/problems.jl:259-298.) */
+   belonging to plans[k]; fb_minus may be NULL, or hold NULL entries.  When
+   every problem's FFT periods lie in [128, 2048] the batch runs as four
+   launches in all -- the surface step, the row, column and inverse-row
+   passes, each over the tiles of every problem -- else each problem's
+   kernels run on the context's side streams.  Results equal the single
+   solves bit for bit.  With ILH_MEM_DEVICE the call is asynchronous on the
+   context stream like the single-problem form.  (The reference solves
+   problems one at a time; this is the batched form of
+   test/literate/problems.jl:259-298.) */
 int ilh_dirichlet_solve_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
                                 const double* const* fb_minus, double* const* f, double* const* s, int mem);
 
 /* Persistent batch: the batched solve above on fixed device buffers, captured
-   once as a HIP graph (ilh_batch_create prepares every plan, runs the batch
-   once and captures it) and replayed by ilh_batch_launch, asynchronously on
-   the context stream, one launch per step.  Inputs are read at each launch.
-   A plan whose geometry changes invalidates the batch (ILH_ERR_STATE). */
+   once as a HIP graph and replayed by ilh_batch_launch, asynchronously on the
+   context stream, one launch per step.  ilh_batch_create prepares every plan,
+   autotunes the four passes on the batch itself (per pass: one work-group per
+   tile or a persistent grid pulling tiles from per-XCD queues, its size,
+   256 or 512 threads per work-group, amplitudes staged in LDS or not; each
+   candidate timed, the fastest kept), runs the batch once and captures it.
+   Inputs are read at each launch.  A plan whose geometry changes invalidates
+   the batch (ILH_ERR_STATE). */
 typedef struct ilh_batch ilh_batch;
 int ilh_batch_create(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
                      const double* const* fb_minus, double* const* f, double* const* s, ilh_batch** out);
 int ilh_batch_launch(ilh_batch* batch);
 int ilh_batch_destroy(ilh_batch* batch);
+/* The autotuner's choice: config[13] = (schedule 0 one work-group per tile /
+   1 persistent queues, work-groups per CU, threads per work-group) of the
+   surface step, row, column and inverse-row passes, then the row pass's
+   amplitude LDS cap (0: none); pass_ms[4] the tuned time of each pass.
+   ILH_ERR_STATE for a batch that runs per-problem kernels.  Either pointer may
+   be NULL.  (No counterpart in the reference.) */
+int ilh_batch_tuning(const ilh_batch* batch, int* config, float* pass_ms);
 
 /* Dirichlet solve for moving bodies (problems.jl:259-298 after every
    update_system!, src/system.jl:26-50): same outputs as ilh_dirichlet_solve,

@@ -1128,6 +1128,33 @@ void orc_cl2invct(const orc_grid* g, int ddf, int N, const double* X, const doub
     orc_lap_destroy(L);
 }
 
+/* create_GLinvD_symm (src/matrix_operators.jl:165-185), 2N x 2N column-major:
+   column c = -scale G_s^symm L^-1 D_s^symm e_c, the inverse Laplacian of
+   Edges{Primal} taken per component on its own grid */
+void orc_glinvd_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* nx,
+                     const double* ny, const double* ds, double scale, double* A)
+{
+    const int n2 = 2 * N;
+    const size_t nu = edge_u_size(g);
+    orc_lap* Lu = orc_lap_create(g, ORC_EDGE_U);
+    orc_lap* Lv = orc_lap_create(g, ORC_EDGE_V);
+    double* e = dalloc(n2);
+    double* col = dalloc(n2);
+    double* q = dalloc(edge_size(g));
+    for (int c = 0; c < n2; ++c) {
+        memset(e, 0, sizeof(double) * n2);
+        e[c] = 1.0;
+        orc_surface_divergence_symm(g, ddf, N, X, Y, nx, ny, ds, e, q);
+        orc_inverse_laplacian(Lu, q, q);
+        orc_inverse_laplacian(Lv, q + nu, q + nu);
+        orc_surface_grad_symm(g, ddf, N, X, Y, nx, ny, q, col);
+        for (int r = 0; r < n2; ++r) A[r + (size_t)n2 * c] = -scale * col[r];
+    }
+    free(e); free(col); free(q);
+    orc_lap_destroy(Lu);
+    orc_lap_destroy(Lv);
+}
+
 /* test/literate/stokes.jl:101-166 in the reference's order, traction
    unfiltered (the reference's last step, sigma = C^2 sigma, is left to the
    caller).  S2 (2N x 2N) and Ss (N x N) column-major, or NULL to build them

@@ -106,6 +106,8 @@ void orc_surface_curl_vector_adj(const orc_grid* g, int ddf, int N, const double
                                  const double* s, double* v);
 void orc_cl2invct(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* ds,
                   double scale, double* A);
+void orc_glinvd_symm(const orc_grid* g, int ddf, int N, const double* X, const double* Y, const double* nx,
+                     const double* ny, const double* ds, double scale, double* A);
 int orc_stokes_solve(const orc_grid* g, int ddf, int N, const double* X, const double* Y,
                      const double* nx, const double* ny, const double* ds, const double* S2, const double* Ss,
                      const double* vplus, const double* vminus, double* v, double* s, double* sigma);

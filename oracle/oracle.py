@@ -455,6 +455,15 @@ def cl2invct(g, geo, scale=1.0, ddf_kind=YANG3):
     return A.reshape(n, n).T.copy()
 
 
+def glinvd_symm(g, geo, scale=1.0, ddf_kind=YANG3):
+    """create_GLinvD_symm (src/matrix_operators.jl:165-185): 2N x 2N [row, col]."""
+    n = 2 * geo.N
+    A = np.zeros(n * n)
+    lib().orc_glinvd_symm(C.byref(g), ddf_kind, *geo.args(), _p(geo.nx), _p(geo.ny), _p(geo.ds),
+                          C.c_double(scale), _p(A))
+    return A.reshape(n, n).T.copy()
+
+
 def stokes_solve(g, geo, vplus, vminus=None, S2=None, Ss=None, ddf_kind=YANG3):
     """(v, s, sigma) of test/literate/stokes.jl:101-166 with sigma unfiltered."""
     v = np.zeros(edge_len(g))

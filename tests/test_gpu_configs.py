@@ -142,6 +142,34 @@ def test_c3_batched_64_problems(ctx):
         assert np.array_equal(out[k][0].flat(), f1.flat()), k
         assert np.array_equal(out[k][1].flat(), s1.flat()), k
         singles.append((f1, s1))
+    # the autotuned batch captured as a graph: replays with new data, eager
+    # solves in between, each replay equal to the single solves bit for bit
+    rng = np.random.default_rng(21)
+    fps = [ilhip.DeviceArray.from_host(ctx, fb) for _, _, fb in probs]
+    fds = [ilhip.DeviceArray(ctx, (g.NX - 1) * (g.NY - 1)) for g, _, _ in probs]
+    sds = [ilhip.DeviceArray(ctx, s.N) for _, s, _ in probs]
+    batch = ilhip.DirichletBatch(caches, fps, fds, sds)
+    print("C3 tuning", batch.tuning(), flush=True)
+    for rep in range(2):
+        pick = sorted(rng.choice(64, 6, replace=False).tolist())
+        scale = [1.0 + 0.5 * rng.standard_normal() for _ in range(64)]
+        for k in range(64):
+            fps[k].upload(probs[k][2] * scale[k])
+        batch.launch()
+        ctx.synchronize()
+        for k in pick:
+            f1, s1 = ilhip.DirichletPoisson(caches[k]).solve(probs[k][2] * scale[k])
+            assert np.array_equal(fds[k].download(), f1.flat()), (rep, k)
+            assert np.array_equal(sds[k].download(), s1.flat()), (rep, k)
+        print("C3 replay", rep, "checked", pick, flush=True)
+    import time
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        batch.launch()
+    ctx.synchronize()
+    print("C3 graph replay: %.0f solves/s" % (64 * 20 / (time.perf_counter() - t0)), flush=True)
+    batch.close()
     # oracle on a seeded subset that includes the largest problem
     rng = np.random.default_rng(33)
     big = int(np.argmax([n * n * 1e4 + N for n, N in zip(sizes, Ns)]))

@@ -90,6 +90,22 @@ def test_cl2invct(setup):
     assert rel_err(ref @ x, b) < 1e-9
 
 
+def test_glinvd_symm(setup):
+    il, cache, gs, og, geo, rng = setup
+    A = il.create_GLinvD_symm(cache)
+    ref = O.glinvd_symm(og, geo)
+    assert A.shape == (2 * geo.N, 2 * geo.N)
+    assert max_rel(A, ref) < TOL_OP
+    assert max_rel(il.create_GLinvD_symm(cache, scale=2.0), 2.0 * ref) < TOL_OP
+    # the inverse Laplacian of Edges{Primal}, each component on its own grid
+    q = rng.standard_normal(O.edge_len(og))
+    out = il.Edges(gs)
+    il.inverse_laplacian(out, cache, il.Edges(gs, q))
+    nu = O.nel(og, O.EDGE_U)
+    ref_q = np.concatenate([O.InverseLaplacian(og, O.EDGE_U)(q[:nu]), O.InverseLaplacian(og, O.EDGE_V)(q[nu:])])
+    assert max_rel(out.flat(), ref_q) < TOL_OP
+
+
 def test_vector_surface_filter(setup):
     il, cache, gs, og, geo, rng = setup
     Cr = O.surface_filter_vector(og, geo)

@@ -10,6 +10,10 @@ function is pinned by what the mathematics fixes:
   * surface -> grid operators are the (weighted) adjoints of those;
   * create_CL2invCT, the inverse Laplacian applied twice, equals the Gram
     matrix of the single inverse Laplacian (the grid convolution is symmetric);
+  * create_GLinvD_symm equals dx^2 W^-1 D^T L^-1 D (D = D_s^symm, W = diag(ds, ds))
+    by the adjoint identity G_s^symm = -dx^2 W^-1 D^T pinned above, its W-weighted
+    form is symmetric negative semidefinite (L^-1 is), and each edge component
+    is inverted on its own grid (a swapped component grid fails the first);
   * the Stokes solve satisfies its constraint E v = v_bar and div v = R(n.[v])
     algebraically, and converges to the exact rotating-cylinder solution
     (v = ω a²/r outside, rigid inside, traction jump -2ω) at first order.
@@ -130,6 +134,31 @@ def test_cl2invct_is_the_gram_matrix_of_one_inverse_laplacian():
     DS = D[:, None] * S2
     assert max_rel(DS, DS.T) < 1e-11
     assert np.linalg.eigvalsh(0.5 * (DS + DS.T)).max() < 1e-10 * np.abs(DS).max()
+
+
+def test_glinvd_symm_is_the_weighted_gram_matrix_of_the_inverse_laplacian():
+    g, geo = _setup(n=40)
+    N = geo.N
+    A = O.glinvd_symm(g, geo, scale=1.0)
+    assert A.shape == (2 * N, 2 * N)
+    nu = O.nel(g, O.EDGE_U)
+    Lu, Lv = O.InverseLaplacian(g, O.EDGE_U), O.InverseLaplacian(g, O.EDGE_V)
+    U = np.zeros((O.edge_len(g), 2 * N))
+    V = np.zeros_like(U)
+    for k in range(2 * N):
+        e = np.zeros(2 * N)
+        e[k] = 1.0
+        U[:, k] = O.surface_divergence_symm(g, geo, e)
+        V[:nu, k] = Lu(U[:nu, k])
+        V[nu:, k] = Lv(U[nu:, k])
+    W = np.concatenate([geo.ds, geo.ds])
+    ref = g.dx ** 2 * (U.T @ V) / W[:, None]
+    assert max_rel(A, ref) < 1e-11
+    assert max_rel(O.glinvd_symm(g, geo, scale=-0.5), -0.5 * A) < 1e-14
+    WA = W[:, None] * A
+    assert max_rel(WA, WA.T) < 1e-11
+    assert np.linalg.eigvalsh(0.5 * (WA + WA.T)).max() < 1e-10 * np.abs(WA).max()
+    assert np.abs(A).max() > 0
 
 
 def _edges_split(g, v):
