@@ -112,7 +112,9 @@ struct StokesCache {
     DevBuf S2inv, Ssinv;          // row-major inverses, leading dimensions solve_ld(2N), solve_ld(N)
     DevBuf S2;                    // create_CL2invCT row-major (the residual of the refinement step)
     DevBuf Ka;                    // E C L^-2 C^T D_s^symm (2N x 2N, row-major): the slip of psi*
-    DevBuf Cf;                    // the vector surface filter C (2N x 2N, row-major)
+    DevBuf Cf;                    // the vector surface filter's two diagonal blocks (N x N each, row-major,
+                                  // leading dimension solve_ld(N)), built on the first filtered solve
+    bool cf_ok = false;
     DevBuf sv;                    // surface work: dv, vb, e_phi, w, vprime, sigma (2N each); dvn, dsv, ds (N each)
     DevBuf gE, gE2, gP;           // grid work: edges (2), primal
     DevBuf gDD;                   // two Nodes{Dual} fields back to back (one batched convolution)
@@ -1476,10 +1478,11 @@ namespace {
 // One thread per point p and its interpolation term TE[p tstride + toff] on
 // the component grid of the bins: its 4 x 4 support cells in (row, x) order,
 // each cell's contributions (amplitude slots q with R weight cw) in the bins'
-// order, into row t.ai of C (column-major n x n, zeroed by the caller).
+// order, into row t.ai - off of C (column-major n x n, zeroed by the caller;
+// slots are taken relative to off too: one diagonal block of a larger C).
 __global__ void k_filter_build(const Term* __restrict__ TE, int tstride, int toff, int N,
                                const int* __restrict__ cell_ptr, const int4* __restrict__ cells,
-                               const int* __restrict__ cai, const double* __restrict__ cw, int ny, int n,
+                               const int* __restrict__ cai, const double* __restrict__ cw, int ny, int n, int off,
                                double* __restrict__ C)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1503,7 +1506,7 @@ __global__ void k_filter_build(const Term* __restrict__ TE, int tstride, int tof
             for (int e = cl.y; e < cl.z; ++e) r1 += cw[e];
             if (r1 == 0.0) continue;
             const double phi = t.coef * t.wx[a] * t.wy[b] / r1;
-            for (int e = cl.y; e < cl.z; ++e) C[t.ai + (size_t)n * cai[e]] += phi * cw[e];
+            for (int e = cl.y; e < cl.z; ++e) C[(t.ai - off) + (size_t)n * (cai[e] - off)] += phi * cw[e];
         }
     }
 }
@@ -1549,7 +1552,7 @@ extern "C" int ilh_surface_filter_build(ilh_plan* pl, double* C, int mem)
                 pl->bins_ok[OP_R_PRIMAL] = true;
             }
             const RowBins& b = pl->bins[OP_R_PRIMAL][0];
-            k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 1, 0, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, N,
+            k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 1, 0, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, N, 0,
                                                             d_C);
             ILH_HIP(hipGetLastError());
         }
@@ -1627,12 +1630,13 @@ void filter_vector(ilh_plan* pl, int power, double* v, double* g)
 namespace {
 // create_surface_filter of the vector cache into d_C (2N x 2N, column-major,
 // N > 0): block-diagonal, the u block from the u edges, the v block from the
-// v edges
-void build_filter_vector(ilh_plan* pl, double* d_C)
+// v edges.  block = 0 or 1: that diagonal block alone (N x N, column-major).
+void build_filter_vector(ilh_plan* pl, double* d_C, int block = -1)
 {
     const int N = pl->N, n2 = 2 * N;
     hipStream_t st = pl->ctx->stream;
-    fill_zero(st, d_C, (size_t)n2 * n2);
+    const int n = block < 0 ? n2 : N;
+    fill_zero(st, d_C, (size_t)n * n);
     const Term* te = get_terms(pl, OP_E_EDGES);
     const Term* tr = get_terms(pl, OP_R_EDGES);
     if (!pl->bins_ok[OP_R_EDGES]) {
@@ -1641,9 +1645,10 @@ void build_filter_vector(ilh_plan* pl, double* d_C)
         pl->bins_ok[OP_R_EDGES] = true;
     }
     for (int c = 0; c < 2; ++c) {
+        if (block >= 0 && c != block) continue;
         const RowBins& b = pl->bins[OP_R_EDGES][c];
-        k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 2, c, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, n2,
-                                                        d_C);
+        k_filter_build<<<(N + 127) / 128, 128, 0, st>>>(te, 2, c, N, b.cell_ptr, b.cells, b.cai, b.cw, b.ny, n,
+                                                        block < 0 ? 0 : c * N, d_C);
     }
     ILH_HIP(hipGetLastError());
 }
@@ -1721,6 +1726,23 @@ __global__ void k_stokes_vprime(int n, const double* __restrict__ vb, const doub
     if (k < n) out[k] = (vb[k] - ephi[k]) + w[k];
 }
 
+// the vector filter's diagonal blocks (C is block-diagonal: u and v
+// components filter independently)
+void stokes_filter(ilh_plan* pl, StokesCache* sc)
+{
+    if (sc->cf_ok || pl->N == 0) return;
+    const int N = pl->N, ld = solve_ld(N);
+    DevBuf K;
+    K.ensure(sizeof(double) * (size_t)N * N);
+    sc->Cf.ensure(sizeof(double) * 2 * (size_t)N * ld);
+    for (int c = 0; c < 2; ++c) {
+        build_filter_vector(pl, K.as<double>(), c);
+        transpose(pl->ctx->stream, K.as<double>(), sc->Cf.as<double>() + (size_t)c * N * ld, N, N, ld);
+    }
+    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));   // K is released on return
+    sc->cf_ok = true;
+}
+
 StokesCache* prepare_stokes(ilh_plan* pl)
 {
     if (pl->sto) return pl->sto;
@@ -1748,9 +1770,6 @@ StokesCache* prepare_stokes(ilh_plan* pl)
             build_stokes_ka(pl, K.as<double>());
             sc->Ka.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
             transpose(pl->ctx->stream, K.as<double>(), sc->Ka.as<double>(), n2, n2, solve_ld(n2));
-            build_filter_vector(pl, K.as<double>());
-            sc->Cf.ensure(sizeof(double) * (size_t)n2 * solve_ld(n2));
-            transpose(pl->ctx->stream, K.as<double>(), sc->Cf.as<double>(), n2, n2, solve_ld(n2));
             ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
         }
     } catch (...) {
@@ -1826,11 +1845,14 @@ extern "C" int ilh_stokes_solve(ilh_plan* pl, const double* v_plus, const double
             axpby(st, d_v, gE2, 1.0, -1.0, ne);
             axpby(st, gD2, gD, -1.0, -1.0, nd);
             ILH_HIP(hipMemcpyAsync(d_s, gD2, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
-            // sigma <- C^k sigma with the stored filter matrix (gE holds the
-            // other iterate)
+            // sigma <- C^k sigma with the stored diagonal blocks of the filter
+            // (gE holds the other iterate)
+            if (filter_power > 0) stokes_filter(pl, sc);
             double *fa = sig, *fb = gE;
+            const int ld1 = solve_ld(N);
             for (int k = 0; k < filter_power; ++k) {
-                matvec_rows(st, sc->Cf.as<double>(), ld2, fa, fb, n2, n2, 1.0, 0);
+                matvec_rows(st, sc->Cf.as<double>(), ld1, fa, fb, N, N, 1.0, 0);
+                matvec_rows(st, sc->Cf.as<double>() + (size_t)N * ld1, ld1, fa + N, fb + N, N, N, 1.0, 0);
                 std::swap(fa, fb);
             }
             if (fa != sig) ILH_HIP(hipMemcpyAsync(sig, fa, sizeof(double) * n2, hipMemcpyDeviceToDevice, st));

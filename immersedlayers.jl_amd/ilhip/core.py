@@ -699,14 +699,17 @@ def _vec(a, n):
 class DirichletPoisson:
     """The Dirichlet Poisson problem of test/literate/problems.jl:243-298:
     S = create_RTLinvR is built and factored once per geometry; ``solve``
-    returns the grid field f and the (unfiltered) surface strength s."""
+    returns the grid field f and the surface strength s, filtered as C^6 s
+    by default like the reference's solve."""
 
     def __init__(self, cache):
         self.cache = cache
 
-    def solve(self, fbplus, fbminus=None, filter_power=0):
-        """(f, s); with filter_power = k > 0, s is returned filtered as C^k s
-        (the reference's solve returns C^6 s, test/literate/problems.jl:298)."""
+    def solve(self, fbplus, fbminus=None, filter_power=6):
+        """(f, s) with s filtered as C^k s, k = filter_power (6, as the
+        reference's solve returns C^6 s, test/literate/problems.jl:298;
+        0: the unfiltered strength, which the device-buffer, batched and
+        moving-body forms return)."""
         c = self.cache
         fp = _vec(fbplus, c.N)
         fm = None if fbminus is None else _vec(fbminus, c.N)

@@ -382,10 +382,17 @@ def primal_to(g, m, kind):
 
 def surface_filter(g, geo, ddf_kind=YANG3):
     """C = Et R (create_surface_filter, src/matrix_operators.jl:254-267), dense
-    [row, col]: Et interpolates the regularized field divided by R 1 (zero where
-    R 1 = 0), so that C keeps constants and sum_p ds_p (C s)_p = sum_q ds_q s_q
-    (the property test/literate/matrices.jl:49-56 states).  E is formed from R
-    by adjointness (E = dx^2 diag(1/ds) R^T)."""
+    [row, col].  The reference builds Et from CartesianGrids' regularization
+    with filter=true (a dependency not in the reference tree; Project.toml
+    pins CartesianGrids 0.1.29 / 0.2), the filtered interpolation of Goza et
+    al. (2016, J. Comput. Phys. 321, 860-873) that test/literate/matrices.jl:55,73
+    cites: the interpolation weight of point p at grid node j divided by the
+    regularized unit strength there,
+        Et[p, j] = E[p, j] / (R 1)_j    (0 where (R 1)_j = 0),
+    so that C keeps constants and the surface integral, sum_p ds_p (C s)_p =
+    sum_q ds_q s_q (the property matrices.jl:49-56 states), and ds C is
+    symmetric positive semidefinite.  E is formed from R by adjointness
+    (E = dx^2 diag(1/ds) R^T)."""
     N = geo.N
     R = np.zeros((nel(g, PRIMAL), N))
     for q in range(N):

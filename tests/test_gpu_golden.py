@@ -33,9 +33,14 @@ def test_dirichlet_golden(ctx):
     d, g, s = _load("dirichlet_plain.npz")
     print("dirichlet golden N", s.N, flush=True)
     cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
-    f, sv = ilhip.DirichletPoisson(cache).solve(d["fb"])
+    f, sv = ilhip.DirichletPoisson(cache).solve(d["fb"], filter_power=0)
     assert rel_err(f.flat(), d["f"]) < 1e-10
     assert rel_err(sv.flat(), d["s"]) < 1e-10
+    # the default, like the reference's solve (problems.jl:298): C^6 s
+    f6, s6 = ilhip.DirichletPoisson(cache).solve(d["fb"])
+    assert np.array_equal(f6.flat(), f.flat())
+    Cm = O.surface_filter(O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx), O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds))
+    assert rel_err(s6.flat(), np.linalg.matrix_power(Cm, 6) @ d["s"]) < 1e-10
 
 
 def test_neumann_added_mass(ctx):
@@ -87,8 +92,8 @@ def test_headline_parity_and_determinism(ctx):
     print("headline parity N", s.N, flush=True)
     cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
     prob = ilhip.DirichletPoisson(cache)
-    f1, s1 = prob.solve(fb)
-    f2, s2 = prob.solve(fb)
+    f1, s1 = prob.solve(fb, filter_power=0)
+    f2, s2 = prob.solve(fb, filter_power=0)
     assert np.array_equal(f1.flat(), f2.flat()) and np.array_equal(s1.flat(), s2.flat())   # bitwise
     og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
     geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
@@ -115,7 +120,7 @@ def test_dirichlet_many_points_reads_amplitudes_from_memory(ctx):
     print("many points N", s.N, flush=True)
     assert s.N > 4096
     cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
-    f1, s1 = ilhip.DirichletPoisson(cache).solve(fb)
+    f1, s1 = ilhip.DirichletPoisson(cache).solve(fb, filter_power=0)
     og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
     fr, sr = O.dirichlet_solve(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), fb)
     assert rel_err(f1.flat(), fr) < 1e-10
@@ -128,11 +133,11 @@ def test_moving_geometry_update(ctx):
     g, s, fb = G.config_plain()
     cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
     prob = ilhip.DirichletPoisson(cache)
-    prob.solve(fb)
+    prob.solve(fb, filter_power=0)
     s2 = G.circle(0.5, 1.4 * g.dx, cx=0.13, cy=-0.07)
     assert s2.N == s.N
     cache.update_points(s2)
-    f, sv = prob.solve(fb)
+    f, sv = prob.solve(fb, filter_power=0)
     og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
     fr, sr = O.dirichlet_solve(og, O.Geometry(s2.X, s2.Y, s2.nx, s2.ny, s2.ds), fb)
     assert rel_err(f.flat(), fr) < 1e-10 and rel_err(sv.flat(), sr) < 1e-10
@@ -173,7 +178,7 @@ def test_empty_body_list(ctx):
     out = ilhip.Nodes(ilhip.Primal, g, np.ones(g.primal_shape()))
     ilhip.regularize(out, ilhip.ScalarData(0), cache)
     assert np.all(out.data == 0.0)
-    f, s = ilhip.DirichletPoisson(cache).solve(np.zeros(0))
+    f, s = ilhip.DirichletPoisson(cache).solve(np.zeros(0), filter_power=0)
     assert np.all(f.data == 0.0) and s.flat().size == 0
     # masks revert to ones (complement zeros) without points (src/surface_operators.jl:870-873)
     assert np.all(ilhip.mask(cache).data == 1.0)
@@ -207,7 +212,7 @@ def test_error_paths_of_the_extended_entry_points(ctx):
     with pytest.raises(ilhip.ArgumentError):
         ilhip.DirichletHeatConduction(cache, -1.0, 1e-4, np.ones(s.N))
     # the plan still works after the refused calls
-    f, sv = prob.solve(fb)
+    f, sv = prob.solve(fb, filter_power=0)
     assert np.isfinite(f.data).all()
 
 
@@ -265,7 +270,7 @@ def test_batched_solves(ctx):
     fms = [None if k % 2 else 0.2 * rng.standard_normal(c.N) for k, c in enumerate(caches)]
     out = ilhip.dirichlet_solve_batched(caches, [fb for _, _, fb in probs], fms)
     for k, ((g, s, fb), c) in enumerate(zip(probs, caches)):
-        f1, s1 = ilhip.DirichletPoisson(c).solve(fb, fms[k])
+        f1, s1 = ilhip.DirichletPoisson(c).solve(fb, fms[k], filter_power=0)
         assert np.array_equal(out[k][0].flat(), f1.flat()) and np.array_equal(out[k][1].flat(), s1.flat())
         og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
         fr, sr = O.dirichlet_solve(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), fb, fms[k])
@@ -295,7 +300,7 @@ def test_batch_graph_replays(ctx):
         ctx.synchronize()
         print("batch replay", rep, flush=True)
         for k, c in enumerate(caches):
-            f1, s1 = ilhip.DirichletPoisson(c).solve(fbs[k])      # eager work between replays
+            f1, s1 = ilhip.DirichletPoisson(c).solve(fbs[k], filter_power=0)      # eager work between replays
             assert np.array_equal(fds[k].download(), f1.flat())
             assert np.array_equal(sds[k].download(), s1.flat())
     g, s, fb = probs[0]
@@ -305,9 +310,18 @@ def test_batch_graph_replays(ctx):
     batch.close()
 
 
-def test_heat_if_herk_matches_oracle(ctx):
+@pytest.mark.parametrize("fo,steps,tol_body", [(0.25, 12, 1e-10), (50.0, 3, 1e-3)])
+def test_heat_if_herk_matches_oracle(ctx, fo, steps, tol_body):
     """IF-HERK heat conduction (test/literate/heatconduction.jl): the GPU march
-    equals the oracle's over 12 steps at Fo = 0.25, with and without a body."""
+    equals the oracle's, with and without a body, at Fo = 0.25 (12 steps) and
+    Fo = 50 (3 steps: the stage filters' half-widths, 54 and 75 points, take
+    the long x and y passes; the bodiless march pins them to 1e-10).  At
+    Fo = 50 the stage Schur matrices E exp(c dt kappa L) R have condition
+    numbers 4e11, 9e13 and 4e15 (c = 1/3, 2/3, 1: the heat kernel is ~10 cells
+    wide against 1.4-cell point spacing), so the body case agrees with the
+    oracle's LU solves only to ~kappa eps (1e-3 stated, 5e-5 observed), and the
+    march overshoots (|T| up to ~7, the oracle's alike): the scheme's accuracy
+    limit, not a kernel error."""
     import ilhip
     from ilhip import geometry as G
     n, dx = 96, 0.03
@@ -316,19 +330,21 @@ def test_heat_if_herk_matches_oracle(ctx):
     Xg, Yg = np.meshgrid(xs, xs, indexing="ij")
     T0 = np.exp(-((Xg - 0.5) ** 2 + (Yg + 0.2) ** 2) / (2 * (5 * dx) ** 2))
     og = O.grid(g.NX, g.NY, g.I0x, g.I0y, g.dx)
-    kappa, dt = 1.0, 0.25 * dx * dx
+    kappa, dt = 1.0, fo * dx * dx
     for s in (G.circle(0.4, 1.4 * dx, -0.2, 0.1), None):
         if s is None:
             e = np.zeros(0)
             s = G.Surface(e, e, e, e, e, np.zeros(0, int))
         tb = np.ones(s.N)
-        print("heat N", s.N, flush=True)
+        print("heat Fo", fo, "N", s.N, flush=True)
         cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
         heat = ilhip.DirichletHeatConduction(cache, kappa, dt, tb)
-        T = heat.advance(ilhip.Nodes(ilhip.Primal, g, T0), 12)
-        Tr = O.heat_advance(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), kappa, dt, 12, T0.ravel(order="F"), tb)
-        assert rel_err(T.flat(), Tr) < 1e-10
-        if s.N:
+        T = heat.advance(ilhip.Nodes(ilhip.Primal, g, T0), steps)
+        Tr = O.heat_advance(og, O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds), kappa, dt, steps, T0.ravel(order="F"), tb)
+        err = rel_err(T.flat(), Tr)
+        print("  rel err", err, flush=True)
+        assert err < (tol_body if s.N else 1e-10)
+        if s.N and fo < 1:
             Eb = ilhip.ScalarData(s.N)
             ilhip.interpolate(Eb, T, cache)
             assert np.abs(Eb.flat() - 0.5).max() < 1e-10      # E T = (T_b+ + T_b-)/2, T_b- = 0
@@ -346,14 +362,14 @@ def test_handles_outlive_in_any_order():
     cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
     heat = ilhip.DirichletHeatConduction(cache, 1.0, 0.25 * g.dx ** 2, np.ones(s.N))
     ctx.close()                               # the plan still holds the context
-    f1, s1 = ilhip.DirichletPoisson(cache).solve(fb)
+    f1, s1 = ilhip.DirichletPoisson(cache).solve(fb, filter_power=0)
     cache.close()                             # the heat integrator still holds the plan
     T = ilhip.Nodes(ilhip.Primal, g)
     heat.advance(T, 1)
     heat.close()                              # last owner: plan and context go
     ctx2 = ilhip.Context(0)
     cache2 = ilhip.SurfaceScalarCache(s, g, ctx=ctx2)
-    f2, s2 = ilhip.DirichletPoisson(cache2).solve(fb)
+    f2, s2 = ilhip.DirichletPoisson(cache2).solve(fb, filter_power=0)
     assert np.array_equal(f1.flat(), f2.flat()) and np.array_equal(s1.flat(), s2.flat())
     assert np.isfinite(T.flat()).all() and T.flat().max() > 0
     cache2.close()
@@ -413,6 +429,6 @@ def test_multi_rhs_matches_single(ctx):
         print("multi rhs", K, "with f-" if fm is not None else "", flush=True)
         fs, ss = prob.solve_multi(fps, fm)
         for k in range(K):
-            f1, s1 = prob.solve(fps[:, k], None if fm is None else fm[:, k])
+            f1, s1 = prob.solve(fps[:, k], None if fm is None else fm[:, k], filter_power=0)
             assert np.array_equal(fs[k].flat(), f1.flat()), k
             assert np.array_equal(ss[:, k], s1.flat()), k

@@ -170,7 +170,7 @@ def test_dirichlet_solve(setup):
     prob = il.DirichletPoisson(cache)
     S = O.schur(og, geo, O.S_RTLINVR)
     for minus in (None, fm):
-        f, s = prob.solve(fp, minus)
+        f, s = prob.solve(fp, minus, filter_power=0)
         fr, sr = O.dirichlet_solve(og, geo, fp, minus, S)
         assert rel_err(f.flat(), fr) < TOL_SOLVE
         assert rel_err(s.flat(), sr) < TOL_SOLVE
@@ -248,3 +248,34 @@ def test_surface_filter(setup):
         got = il.filter_surface(il.ScalarData(geo.N, sv), cache, k).flat()
         ref = np.linalg.matrix_power(Cr, k) @ sv
         assert max_rel(got, ref) < 1e-12, k
+
+
+@pytest.mark.parametrize("ddf", ["Roma", "M4prime", "Witchhat"])
+def test_other_ddfs(ctx, ddf):
+    """The non-default DDF kinds (CartesianGrids' Roma, M4prime, Witchhat) on
+    the plain workload: regularize/interpolate to 1e-12, create_RTLinvR to
+    1e-12 and the Dirichlet solve to the FP64 bar against the oracle with the
+    same kernel."""
+    import ilhip as il
+    gs, s = CASES["plain"]
+    kind = getattr(il, ddf)
+    okind = getattr(O, ddf.upper())
+    cache = il.SurfaceScalarCache(s, gs, ddftype=kind, ctx=ctx)
+    og = O.grid(gs.NX, gs.NY, gs.I0x, gs.I0y, gs.dx)
+    geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
+    rng = np.random.default_rng(23)
+    f = rng.standard_normal(geo.N)
+    out = il.Nodes(il.Primal, gs)
+    il.regularize(out, il.ScalarData(geo.N, f), cache)
+    assert max_rel(out.flat(), O.regularize(og, O.PRIMAL, geo, f, ddf_kind=okind)) < TOL_OP
+    fld = rng.standard_normal(out.flat().size)
+    sv = il.ScalarData(geo.N)
+    il.interpolate(sv, il.Nodes(il.Primal, gs, fld.reshape(out.data.shape, order="F")), cache)
+    assert max_rel(sv.flat(), O.interpolate(og, O.PRIMAL, geo, fld, ddf_kind=okind)) < TOL_OP
+    S = O.schur(og, geo, O.S_RTLINVR, ddf_kind=okind)
+    assert max_rel(il.create_RTLinvR(cache), S) < TOL_OP
+    fp = rng.standard_normal(geo.N)
+    fg, sg = il.DirichletPoisson(cache).solve(fp, filter_power=0)
+    fr, sr = O.dirichlet_solve(og, geo, fp, None, S, ddf_kind=okind)
+    print(ddf, "f", rel_err(fg.flat(), fr), "s", rel_err(sg.flat(), sr), flush=True)
+    assert rel_err(fg.flat(), fr) < TOL_SOLVE and rel_err(sg.flat(), sr) < TOL_SOLVE

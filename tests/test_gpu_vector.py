@@ -150,6 +150,17 @@ def test_stokes_solve(setup, ctx):
     assert np.array_equal(d_v.download(), v.flat())
     assert np.array_equal(d_s.download(), s.flat())
     assert np.array_equal(d_g.download(), sig.flat())
+    # odd and even filter powers (the last iterate lands in either buffer), on
+    # host and device memory, against C^k of the unfiltered traction
+    rv, rs, rsig = O.stokes_solve(og, geo, vp, vm)
+    Cf = O.surface_filter_vector(og, geo)
+    for k in (1, 3, 4):
+        _, _, sigk = prob.solve(vp, vm, filter_power=k)
+        assert rel_err(sigk.flat(), np.linalg.matrix_power(Cf, k) @ rsig) < 1e-9, k
+        L.call("ilh_stokes_solve", cache.h, dv_p.ptr, dv_m.ptr, d_v.ptr, d_s.ptr, d_g.ptr, k, L.MEM_DEVICE)
+        ctx.synchronize()
+        assert np.array_equal(d_g.download(), sigk.flat()), k
+        assert np.array_equal(d_v.download(), v.flat())
 
 
 def test_stokes_translating_rectangle_constraint(ctx):

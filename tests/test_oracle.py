@@ -479,6 +479,14 @@ def test_surface_filter_pins():
         f = O.regularize(og, O.PRIMAL, geo, e)
         f = np.where(r1 != 0, f / np.where(r1 != 0, r1, 1.0), 0.0)
         assert np.abs(O.interpolate(og, O.PRIMAL, geo, f) - C[:, q]).max() < 1e-14
+    # the pins tell the grid-side normalization from the point-side one
+    # (diag(1 / (E R 1)) E R, which keeps constants but not the integral)
+    R = np.stack([O.regularize(og, O.PRIMAL, geo, np.eye(geo.N)[q]) for q in range(geo.N)], axis=1)
+    E = (R * (og.dx ** 2 / ds)[None, :]).T
+    ER = E @ R
+    Cp = ER / (ER @ np.ones(geo.N))[:, None]
+    assert np.abs(Cp @ np.ones(geo.N) - 1.0).max() < 1e-13
+    assert np.abs(ds @ Cp - ds).max() > 1e-6 * ds.max()
     # filtering smooths: the filtered strength of a noisy signal is closer to its smooth part
     rng = np.random.default_rng(5)
     th = s.theta()
