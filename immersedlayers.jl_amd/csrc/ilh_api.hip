@@ -228,6 +228,10 @@ void ilh::ctx_release(ilh_ctx* c)
     for (hipEvent_t e : c->tev)
         if (e) (void)hipEventDestroy(e);
     if (c->fused) fused_destroy(c->fused);
+    for (FusedBatch* fb : c->fused_mv)
+        if (fb) fused_destroy(fb);
+    if (c->mv_dev) (void)hipFree(c->mv_dev);
+    if (c->mv_host) (void)hipHostFree(c->mv_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2302,8 +2306,8 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, si
 // |w|; the previous Givens rotations are applied, a new one zeroes
 // H[j+1, j], and |g[j+1]| (the residual norm) is tested against tol |b|.
 // Every sum has a fixed order (deterministic).
-__global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int ldv, int j, double* __restrict__ w,
-                                                     int n, double* __restrict__ st, int m, int* __restrict__ flags)
+__device__ __forceinline__ void gm_arnoldi_body(double* __restrict__ V, int ldv, int j, double* __restrict__ w, int n,
+                                                double* __restrict__ st, int m, int* __restrict__ flags)
 {
     __shared__ double h[2][256];
     __shared__ double red[16];
@@ -2391,6 +2395,74 @@ __global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int
             flags[1] = m;
         }
     }
+}
+
+__global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int ldv, int j, double* __restrict__ w,
+                                                     int n, double* __restrict__ st, int m, int* __restrict__ flags)
+{
+    gm_arnoldi_body(V, ldv, j, w, n, st, m, flags);
+}
+
+// ---- the GMRES steps of many plans in lockstep, one launch per phase
+// One plan's operands of a batched GMRES step (device pointers).
+struct BGm {
+    const float* P32;          // FP32 reference inverse (row-major, leading dimension ld), or null
+    const double* band;        // the learned band (n x bw) and its neighbours, when P32 is null
+    const int* nbr;
+    int bw;
+    const double* S;           // the current geometry's matrix (row-major, ld)
+    int ld, n;
+    double *V, *w, *z, *st;    // basis (stride ld), work vectors, GMRES state
+    int* flags;
+};
+
+// z = P v_j for every plan still running; tiles (plan << 16 | 4-row block),
+// one wave per row, each row in the single solve's order (k_gm_matvec_f32,
+// k_band_apply)
+__global__ void __launch_bounds__(256) k_bgm_prec(const BGm* __restrict__ g, const unsigned* __restrict__ tiles, int j)
+{
+    const unsigned code = tiles[blockIdx.x];
+    const BGm& G = g[code >> 16];
+    if (G.flags[0]) return;
+    const int i = (int)(code & 0xFFFF) * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= G.n) return;
+    const double* v = G.V + (size_t)G.ld * j;
+    if (G.P32) {
+        const double s = wave_dot_f32(G.P32 + (size_t)G.ld * i, v, G.n, lane);
+        if (lane == 0) G.z[i] = s;
+    } else if (lane == 0) {
+        double s = 0.0;
+        for (int k = 0; k < G.bw; ++k) s += G.band[(size_t)i * G.bw + k] * v[G.nbr[(size_t)i * G.bw + k]];
+        G.z[i] = s;
+    }
+}
+
+// w = S z for every plan still running (k_gm_matvec's arithmetic)
+__global__ void __launch_bounds__(256) k_bgm_matvec(const BGm* __restrict__ g, const unsigned* __restrict__ tiles)
+{
+    const unsigned code = tiles[blockIdx.x];
+    const BGm& G = g[code >> 16];
+    if (G.flags[0]) return;
+    const int i = (int)(code & 0xFFFF) * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= G.n) return;
+    const double s = wave_dot(G.S + (size_t)G.ld * i, G.z, G.n, lane);
+    if (lane == 0) G.w[i] = s;
+}
+
+// Arnoldi step j of plan blockIdx.x
+__global__ void __launch_bounds__(1024) k_bgm_arnoldi(const BGm* __restrict__ g, int j, int m)
+{
+    const BGm& G = g[blockIdx.x];
+    gm_arnoldi_body(G.V, G.ld, j, G.w, G.n, G.st, m, G.flags);
+}
+
+// every plan's two flags into one array (for one read-back)
+__global__ void k_bgm_flags(const BGm* __restrict__ g, int nprob, int* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    out[2 * i] = g[i].flags[0];
+    out[2 * i + 1] = g[i].flags[1];
 }
 
 // y = H^-1 g (upper triangular, j x j, thread 0), then w = V y (one
@@ -2639,26 +2711,88 @@ static void moving_geometry(ilh_plan* pl, MovingCache* mc)
     }
 }
 
-// amp = [0; d], f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291)
-static void moving_rhs(ilh_plan* pl, MovingCache* mc, const double* d_fp, const double* d_fm)
+// amp = [0; d], f* = L^-1 D_s d, b = E f* - fbar  (test/literate/problems.jl:281-291);
+// the convolution left out when `conv` is false (the batched form runs all
+// plans' convolutions in one fused call between the two halves)
+static void moving_rhs_jump(ilh_plan* pl, MovingCache* mc, const double* d_fp, const double* d_fm)
 {
     const int N = pl->N;
-    hipStream_t st = pl->ctx->stream;
-    double* amp = mc->amp.as<double>();
-    k_jump_avg<<<(N + 255) / 256, 256, 0, st>>>(d_fp, d_fm, N, amp, mc->fbar.as<double>());
-    conv_sources(pl, GK_PRIMAL, mc->src1, amp, mc->grid.as<double>());
-    gather_op(pl, OP_E_PRIMAL, mc->grid.as<double>(), mc->b.as<double>());
-    axpby(st, mc->b.as<double>(), mc->fbar.as<double>(), -1.0, 1.0, N);
+    k_jump_avg<<<(N + 255) / 256, 256, 0, pl->ctx->stream>>>(d_fp, d_fm, N, mc->amp.as<double>(),
+                                                            mc->fbar.as<double>());
 }
 
-// f = L^-1 (R s + D_s d) with s in amp[0, N)
-static void moving_final(ilh_plan* pl, MovingCache* mc, double* d_f, double* d_s)
+static void moving_rhs_finish(ilh_plan* pl, MovingCache* mc)
+{
+    gather_op(pl, OP_E_PRIMAL, mc->grid.as<double>(), mc->b.as<double>());
+    axpby(pl->ctx->stream, mc->b.as<double>(), mc->fbar.as<double>(), -1.0, 1.0, pl->N);
+}
+
+static void moving_rhs(ilh_plan* pl, MovingCache* mc, const double* d_fp, const double* d_fm)
+{
+    moving_rhs_jump(pl, mc, d_fp, d_fm);
+    conv_sources(pl, GK_PRIMAL, mc->src1, mc->amp.as<double>(), mc->grid.as<double>());
+    moving_rhs_finish(pl, mc);
+}
+
+// f = L^-1 (R s + D_s d) with s in amp[0, N) (the convolution left out when
+// `conv` is false: the batched form runs it for all plans at once)
+static void moving_final(ilh_plan* pl, MovingCache* mc, double* d_f, double* d_s, bool conv = true)
 {
     const int N = pl->N;
     double* amp = mc->amp.as<double>();
-    conv_sources(pl, GK_PRIMAL, mc->src1, amp, d_f);
+    if (conv) conv_sources(pl, GK_PRIMAL, mc->src1, amp, d_f);
     ILH_HIP(hipMemcpyAsync(d_s, amp, sizeof(double) * N, hipMemcpyDeviceToDevice, pl->ctx->stream));
     ++mc->ref_age;
+}
+
+// The convolutions L^-1 (sources of amp) of many moving-body plans as one
+// fused call (four launches in all), out[i] the grid field of plan i; the
+// plans' row-pair counts stay on the device.  False when a plan's FFT periods
+// are outside the fused passes' set (the caller convolves plan by plan).
+static bool moving_conv_fused(ilh_ctx* ctx, int slot, const std::vector<ilh_plan*>& pls,
+                              const std::vector<MovingCache*>& mcs, const std::vector<double*>& out)
+{
+    std::vector<BatchProb> probs;
+    for (size_t k = 0; k < pls.size(); ++k) {
+        ilh_plan* pl = pls[k];
+        const ConvPlan& C = get_conv(pl, GK_PRIMAL);
+        if (!fused_supported(C.Px, C.Py)) return false;
+        const SourceSet& src = mcs[k]->src1;
+        BatchProb b;
+        std::memset(&b, 0, sizeof(b));
+        b.amp = mcs[k]->amp.as<double>();
+        b.n = pl->N;
+        b.pinfo = src.pinfo;
+        b.cells = src.bins.cells;
+        b.cai = src.bins.cai;
+        b.cw = src.bins.cw;
+        b.npairs = src.npairs;
+        b.npairs_dev = src.npairs_dev;
+        b.nx = C.nx;
+        b.ny = C.ny;
+        b.Px = C.Px;
+        b.Py = C.Py;
+        b.T1 = C.T1.as<double2>();
+        b.slot = src.slot_of_row;
+        b.spec = C.spec.as<double>();
+        b.T2 = C.T2.as<double2>();
+        b.out = out[k];
+        b.twx = fft_twiddles(ctx, C.Px);
+        b.twy = fft_twiddles(ctx, C.Py);
+        probs.push_back(b);
+    }
+    if (probs.empty()) return true;
+    FusedBatch*& fb = ctx->fused_mv[slot];
+    if (!fb || !same_probs(fused_probs(fb), probs)) {
+        if (fb) {
+            ILH_HIP(hipStreamSynchronize(ctx->stream));
+            fused_destroy(fb);
+            fb = nullptr;
+        }
+        fb = fused_create(ctx, probs);
+    }
+    fused_run(fb, 1);
+    return true;
 }
 
 // the solve after a failed GMRES: refactor at the current geometry, solve again
@@ -2766,10 +2900,26 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             hipStream_t m;
             ~Restore() { c->stream = m; }
         } restore{ctx, main};
-        ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
-        for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamWaitEvent(ctx->pool[k], ctx->ev[ilh_ctx::POOL], 0));
+        // fork the side streams from / join them into the context stream
+        auto fork = [&] {
+            ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
+            for (int k = 0; k < ilh_ctx::POOL; ++k)
+                ILH_HIP(hipStreamWaitEvent(ctx->pool[k], ctx->ev[ilh_ctx::POOL], 0));
+        };
+        auto join = [&] {
+            ctx->stream = main;
+            for (int k = 0; k < ilh_ctx::POOL; ++k) {
+                ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
+                ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
+            }
+        };
+        fork();
         auto on = [&](int i) { ctx->stream = ctx->pool[i % ilh_ctx::POOL]; };
-        std::vector<int> j(nprob, 0), chunk(nprob, 1), state(nprob, 0), it(nprob, 0);
+        std::vector<int> state(nprob, 0), it(nprob, 0);
+        int chunk = 1;
+        std::vector<ilh_plan*> live_pl;
+        std::vector<MovingCache*> live_mc;
+        std::vector<double*> grid_out, f_out;
         for (int i = 0; i < nprob; ++i) {
             on(i);
             ilh_plan* pl = plans[i];
@@ -2779,49 +2929,125 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 continue;
             }
             moving_geometry(pl, mc[i]);
-            moving_rhs(pl, mc[i], fb_plus[i], fb_minus ? fb_minus[i] : nullptr);
+            moving_rhs_jump(pl, mc[i], fb_plus[i], fb_minus ? fb_minus[i] : nullptr);
+            live_pl.push_back(pl);
+            live_mc.push_back(mc[i]);
+            grid_out.push_back(mc[i]->grid.as<double>());
+            f_out.push_back(f[i]);
+        }
+        join();
+        // f* = L^-1 D_s d of every plan in one fused convolution
+        const bool fused_conv = moving_conv_fused(ctx, 0, live_pl, live_mc, grid_out);
+        fork();
+        for (int i = 0; i < nprob; ++i) {
+            if (!mc[i]) continue;
+            on(i);
+            ilh_plan* pl = plans[i];
+            if (!fused_conv) conv_sources(pl, GK_PRIMAL, mc[i]->src1, mc[i]->amp.as<double>(), mc[i]->grid.as<double>());
+            moving_rhs_finish(pl, mc[i]);
             gm_begin(pl, mc[i], mc[i]->b.as<double>(), mc[i]->amp.as<double>(), tol, m);
-            chunk[i] = std::max(1, std::min(m, mc[i]->last_iters));
+            chunk = std::max(chunk, std::min(m, mc[i]->last_iters));
         }
-        for (;;) {
-            bool any = false;
-            for (int i = 0; i < nprob; ++i) {
-                if (state[i] || !mc[i]) continue;
-                on(i);
-                const int j1 = std::min(m, j[i] + chunk[i]);
-                gm_issue(plans[i], mc[i], j[i], j1, m);
-                j[i] = j1;
-                any = true;
+        join();
+        // the GMRES steps of every plan in lockstep on the context stream: per
+        // step one launch each for P v, S z and the Arnoldi step over all
+        // plans (plans that stopped skip their tiles), one flag read-back per
+        // chunk of steps
+        std::vector<BGm> hg;
+        std::vector<int> gi;   // plan of each descriptor
+        std::vector<unsigned> tiles;
+        for (int i = 0; i < nprob; ++i) {
+            if (!mc[i]) continue;
+            ilh_plan* pl = plans[i];
+            MovingCache* c = mc[i];
+            BGm G{};
+            G.P32 = c->use_band ? nullptr : c->P32.as<float>();
+            G.band = c->use_band ? c->band.as<double>() : nullptr;
+            G.nbr = c->use_band ? pl->nbr.as<int>() : nullptr;
+            G.bw = c->use_band ? lpc_width(pl->lpc) : 0;
+            G.S = c->S.as<double>();
+            G.ld = solve_ld(pl->N);
+            G.n = pl->N;
+            G.V = c->V.as<double>();
+            G.w = c->w.as<double>();
+            G.z = c->z.as<double>();
+            G.st = c->gm.as<double>();
+            G.flags = c->gmf.as<int>();
+            if ((pl->N + 3) / 4 > 65535 || hg.size() >= 65535) fail(ILH_ERR_ARG, "batched GMRES too large");
+            for (int b = 0; b < (pl->N + 3) / 4; ++b) tiles.push_back(((unsigned)hg.size() << 16) | (unsigned)b);
+            hg.push_back(G);
+            gi.push_back(i);
+        }
+        const int nd = (int)hg.size();
+        if (nd > 0) {
+            const size_t bytes_g = sizeof(BGm) * nd, off_t = (bytes_g + 255) / 256 * 256;
+            const size_t off_f = off_t + (sizeof(unsigned) * tiles.size() + 255) / 256 * 256;
+            const size_t need = off_f + sizeof(int) * 2 * nd;
+            if (need > ctx->mv_cap) {
+                ILH_HIP(hipStreamSynchronize(main));
+                if (ctx->mv_dev) (void)hipFree(ctx->mv_dev);
+                ctx->mv_dev = nullptr;
+                ctx->mv_cap = 0;
+                ILH_HIP(hipMalloc(&ctx->mv_dev, need));
+                ctx->mv_cap = need;
             }
-            if (!any) break;
-            for (int k = 0; k < ilh_ctx::POOL; ++k) ILH_HIP(hipStreamSynchronize(ctx->pool[k]));
-            for (int i = 0; i < nprob; ++i) {
-                if (state[i] || !mc[i]) continue;
-                const int* hf = (const int*)mc[i]->pinned(1);
-                state[i] = hf[0];
-                if (state[i] == 1) it[i] = hf[1];
-                else if (!state[i] && j[i] >= m) state[i] = 2;
-                chunk[i] = 2;
+            if ((size_t)(2 * nd) > ctx->mv_host_cap) {
+                if (ctx->mv_host) (void)hipHostFree(ctx->mv_host);
+                ctx->mv_host = nullptr;
+                ctx->mv_host_cap = 0;
+                ILH_HIP(hipHostMalloc((void**)&ctx->mv_host, sizeof(int) * 2 * nd, hipHostMallocDefault));
+                ctx->mv_host_cap = 2 * nd;
+            }
+            char* base = (char*)ctx->mv_dev;
+            BGm* dg = (BGm*)base;
+            unsigned* dt = (unsigned*)(base + off_t);
+            int* df = (int*)(base + off_f);
+            ILH_HIP(hipMemcpyAsync(dg, hg.data(), bytes_g, hipMemcpyHostToDevice, main));
+            ILH_HIP(hipMemcpyAsync(dt, tiles.data(), sizeof(unsigned) * tiles.size(), hipMemcpyHostToDevice, main));
+            const unsigned nt = (unsigned)tiles.size();
+            int j = 0;
+            for (;;) {
+                const int j1 = std::min(m, j + chunk);
+                for (int jj = j; jj < j1; ++jj) {
+                    k_bgm_prec<<<nt, 256, 0, main>>>(dg, dt, jj);
+                    k_bgm_matvec<<<nt, 256, 0, main>>>(dg, dt);
+                    k_bgm_arnoldi<<<(unsigned)nd, 1024, 0, main>>>(dg, jj, m);
+                }
+                k_bgm_flags<<<(unsigned)((nd + 255) / 256), 256, 0, main>>>(dg, nd, df);
+                ILH_HIP(hipGetLastError());
+                ILH_HIP(hipMemcpyAsync(ctx->mv_host, df, sizeof(int) * 2 * nd, hipMemcpyDeviceToHost, main));
+                ILH_HIP(hipStreamSynchronize(main));   // also keeps hg and tiles alive for their copies
+                j = j1;
+                bool running = false;
+                for (int d = 0; d < nd; ++d) {
+                    const int i = gi[d];
+                    if (state[i]) continue;
+                    state[i] = ctx->mv_host[2 * d];
+                    if (state[i] == 1) it[i] = ctx->mv_host[2 * d + 1];
+                    else if (!state[i] && j >= m) state[i] = 2;
+                    running = running || !state[i];
+                }
+                if (!running) break;
+                chunk = 2;
             }
         }
+        fork();
         for (int i = 0; i < nprob; ++i) {
             if (!mc[i] || state[i] != 1) continue;
             on(i);
             gm_end(plans[i], mc[i], mc[i]->amp.as<double>(), m, it[i]);
-            moving_final(plans[i], mc[i], f[i], s[i]);
         }
         // join the side streams; plans whose reference no longer preconditions
         // are refactored and solved one by one on the context stream
-        ctx->stream = main;
-        for (int k = 0; k < ilh_ctx::POOL; ++k) {
-            ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
-            ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
-        }
+        join();
         for (int i = 0; i < nprob; ++i) {
             if (!mc[i] || state[i] == 1) continue;
             it[i] = moving_refactor_solve(plans[i], mc[i], tol, m);
-            moving_final(plans[i], mc[i], f[i], s[i]);
         }
+        // f = L^-1 (R s + D_s d) of every plan in one fused convolution
+        const bool fused_final = moving_conv_fused(ctx, 1, live_pl, live_mc, f_out);
+        for (int i = 0; i < nprob; ++i)
+            if (mc[i]) moving_final(plans[i], mc[i], f[i], s[i], !fused_final);
         if (iters)
             for (int i = 0; i < nprob; ++i) iters[i] = it[i];
     });

@@ -148,8 +148,11 @@ __global__ void __launch_bounds__(TPB) k_batch_rows(const BatchProb* __restrict_
         S.amp = B.amp;
         S.amp_len = 2 * B.n;
         S.amp_lds = 2 * B.n <= amp_cap;
-        ILH_BATCH_FOR_P(B.Px, (rows_fwd_body<PP, TPB / (PP / V), double, true>(S, B.twx, B.npairs, B.ny, B.T1, blk,
-                                                                                 lds_raw, tid)));
+        // a live count on the device: tiles past it have no pairs
+        const int np = B.npairs_dev ? *B.npairs_dev : B.npairs;
+        ILH_BATCH_FOR_P(B.Px, if (blk * (TPB / (PP / V)) >= np) return;
+                        (rows_fwd_body<PP, TPB / (PP / V), double, true>(S, B.twx, np, B.ny, B.T1, blk, lds_raw,
+                                                                          tid)));
     });
 }
 
@@ -396,10 +399,11 @@ void launch_pass(FusedBatch* fb, int pass, int sched, int mult, int tpb, int amp
 
 }  // namespace
 
-void fused_run(FusedBatch* fb)
+void fused_run(FusedBatch* fb, int first_pass)
 {
     const FusedCfg& c = fb->cfg;
-    for (int pass = 0; pass < NPASS; ++pass) launch_pass(fb, pass, c.sched[pass], c.mult[pass], c.tpb[pass], c.amp_cap);
+    for (int pass = first_pass; pass < NPASS; ++pass)
+        launch_pass(fb, pass, c.sched[pass], c.mult[pass], c.tpb[pass], c.amp_cap);
 }
 
 // Times every candidate of each pass (one untimed launch, then the mean of

@@ -164,6 +164,12 @@ struct ilh_ctx {
     hipEvent_t ev[POOL + 1] = {};
     hipEvent_t tev[2] = {};                   // timing events of the profiling entry points (made on first use)
     ilh::FusedBatch* fused = nullptr;         // the last eager fused batch (reused while its problems are unchanged)
+    ilh::FusedBatch* fused_mv[2] = {};        // the batched moving solve's two convolutions (right-hand side, final)
+    // batched moving-body GMRES: device descriptors and tiles, pinned flags
+    void* mv_dev = nullptr;
+    size_t mv_cap = 0;
+    int* mv_host = nullptr;
+    size_t mv_host_cap = 0;
     // owners: the caller's handle, every plan and batch of the context; the
     // context is torn down when the last one lets go (any destruction order)
     std::atomic<int> refs{1};
@@ -317,6 +323,7 @@ struct BatchProb {
     double* out;               // grid field out (nx ny)
     const double2* twx;        // twiddles of Px and Py
     const double2* twy;
+    const int* npairs_dev;     // the live pair count on the device (npairs then an upper bound), or null
 };
 // Per pass (surface step, rows, columns, inverse rows): schedule (0 one
 // work-group per tile, 1 persistent queues), work-groups per CU of the
@@ -329,7 +336,9 @@ struct FusedBatch;
 bool fused_supported(int Px, int Py);
 FusedBatch* fused_create(ilh_ctx* ctx, const std::vector<BatchProb>& probs);
 void fused_destroy(FusedBatch* fb);
-void fused_run(FusedBatch* fb);
+// the passes from first_pass on (1: the convolution only, of amplitudes
+// already in place)
+void fused_run(FusedBatch* fb, int first_pass = 0);
 void fused_tune(FusedBatch* fb);
 // cfg: 13 ints (sched, mult, threads of each pass; the amplitude cap), ms:
 // the tuned time of each pass (0 before tuning)
