@@ -300,9 +300,11 @@ int ilh_dirichlet_solve_moving(ilh_plan* plan, const double* fb_plus, const doub
    (independent trajectories, each moved by its own ilh_plan_update_points),
    device pointers only, array k of each pointer array belonging to plans[k];
    fb_minus may be NULL or hold NULL entries; iters (may be NULL) receives
-   each plan's GMRES iterations.  The plans' solves run side by side on the
-   context's side streams and their GMRES steps advance in lockstep; each
-   plan's results equal its single-plan solve bit for bit. */
+   each plan's GMRES iterations.  The plans' geometry rebuilds run on the
+   context's side streams; both convolutions of the step run as one fused
+   batched call each, and the GMRES steps of all plans advance in lockstep,
+   one launch per phase over every plan.  Each plan's results equal its
+   single-plan solve bit for bit. */
 int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* plans, const double* const* fb_plus,
                                        const double* const* fb_minus, double* const* f, double* const* s, double tol,
                                        int max_iter, int* iters);
