@@ -440,6 +440,22 @@ def main():
     assert np.array_equal(f_multi0, f_gpu.flat())   # column 0 is the headline solve, bit for bit
     del fp_m, fd_m, sd_m
 
+    # the split-precision solve (FP64 forward transforms and s, FP32 inverse
+    # transforms): solves/s and f against the FP64 solve
+    fd32 = ilhip.DeviceArray(ctx, ng)
+    for _ in range(5):
+        prob.solve_mixed_device(fp, None, fd32, sd)
+    ctx.synchronize()
+    reps = max(50, min(args.steps, 1000))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prob.solve_mixed_device(fp, None, fd32, sd)
+    ctx.synchronize()
+    mixed_sps = reps / (time.perf_counter() - t0)
+    f32 = fd32.download()
+    mixed_err = float(np.linalg.norm(f32 - f_gpu.flat()) / np.linalg.norm(f_gpu.flat()))
+    del fd32
+
     # the same solve with host arrays in and out (PCIe-inclusive, not the metric)
     t0 = time.perf_counter()
     for _ in range(20):
@@ -471,6 +487,42 @@ def main():
         if k:   # step 0 factors the reference geometry
             mv_ms.append(1e3 * (time.perf_counter() - t0))
             mv_it.append(it)
+
+    # the learned preconditioner: trained on the exact inverses of ten
+    # geometries (the bodies pitched and heaved at seeded amplitudes, and two
+    # other star shapes), then the moving steps above solved with its band
+    # (no factorisation at any geometry)
+    t0 = time.perf_counter()
+    model = ilhip.LearnedPreconditioner(ctx, half_band=16, seed=7)
+    rng_l = np.random.default_rng(17)
+    geoms = [moved(s, rng_l.uniform(-0.3, 0.3), rng_l.uniform(-0.2, 0.2)) for _ in range(8)]
+    geoms += [G.config_headline(seed=sd_)[1] for sd_ in (6, 7)]
+    for sk in geoms:
+        ck = ilhip.SurfaceScalarCache(sk, g, ctx=ctx)
+        model.add_geometry(ck)
+        ck.close()
+    t_data = time.perf_counter() - t0
+    n_rows = model.rows
+    t0 = time.perf_counter()
+    model.train(2000, batch=2048, lr=1e-3, seed=1)
+    loss = model.train(1000, batch=2048, lr=2e-4, seed=2)
+    t_train = time.perf_counter() - t0
+    lcache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    lcache.set_preconditioner(model)
+    lprob = ilhip.DirichletPoisson(lcache)
+    ln_ms, ln_it = [], []
+    for k in range(11):
+        t = 0.004 * k
+        lcache.update_points(moved(s, 0.05 * np.sin(2 * np.pi * t), 0.2 * np.sin(2 * np.pi * t)))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        it = lprob.solve_moving_device(fp, None, fd, sd, tol=1e-12, max_iter=80)
+        ctx.synchronize()
+        if k:
+            ln_ms.append(1e3 * (time.perf_counter() - t0))
+            ln_it.append(it)
+    lcache.close()
+    model.close()
 
     # batched moving bodies: 32 trajectories of the same bodies, each heaving
     # 0.2 sin(2 pi f t) and pitching 0.05 sin(2 pi f t) with its own seeded
@@ -550,9 +602,15 @@ def main():
         "moving_batched_ms_per_solve": float(np.mean(bat_ms)),
         "moving_batched_trajectories": ntraj,
         "moving_batched_mean_iterations": bat_it,
+        "mixed_solves_per_s": mixed_sps,
+        "mixed_f_rel_err_vs_fp64": mixed_err,
+        "learned_ms_per_solve": float(np.mean(ln_ms)),
+        "learned_gmres_iterations": ln_it,
+        "learned_training": {"rows": n_rows, "data_s": t_data, "train_s": t_train, "steps": 3000, "final_loss": loss},
         "preconditioned_speedup": {
             "single": float(np.mean(new_geo_ms) / np.mean(mv_ms)),
-            "batched": float(np.mean(new_geo_ms) / np.mean(bat_ms))},
+            "batched": float(np.mean(new_geo_ms) / np.mean(bat_ms)),
+            "learned": float(np.mean(new_geo_ms) / np.mean(ln_ms))},
     }
     if not args.no_cpu:
         cb, f_ref, s_ref = cpu_baseline(g, s, fb, args.cpu_budget)

@@ -231,6 +231,7 @@ void ilh::ctx_release(ilh_ctx* c)
     for (FusedBatch* fb : c->fused_mv)
         if (fb) fused_destroy(fb);
     if (c->mv_dev) (void)hipFree(c->mv_dev);
+    if (c->geo_dev) (void)hipFree(c->geo_dev);
     if (c->mv_host) (void)hipHostFree(c->mv_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2206,9 +2207,9 @@ __device__ __forceinline__ double block_sum_1024(double a, double* red)
 
 // r0 = w: beta = |r0|, bnorm = |b|, v_0 = r0 / beta, g = (beta, 0, ...);
 // converged at once when b = 0 or beta <= tol |b|
-__global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, const double* __restrict__ b, int n,
-                                                  double* __restrict__ V, double* __restrict__ st, int m, double tol,
-                                                  int* __restrict__ flags)   // v_0 = V[0, n)
+__device__ __forceinline__ void gm_init_body(const double* __restrict__ w, const double* __restrict__ b, int n,
+                                             double* __restrict__ V, double* __restrict__ st, int m, double tol,
+                                             int* __restrict__ flags)   // v_0 = V[0, n)
 {
     __shared__ double red[16];
     const GmLayout L{m};
@@ -2232,6 +2233,13 @@ __global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, 
         const double inv = 1.0 / beta;
         for (int k = threadIdx.x; k < n; k += 1024) V[k] = w[k] * inv;
     }
+}
+
+__global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, const double* __restrict__ b, int n,
+                                                  double* __restrict__ V, double* __restrict__ st, int m, double tol,
+                                                  int* __restrict__ flags)
+{
+    gm_init_body(w, b, n, V, st, m, tol, flags);
 }
 
 // y = M x (M row-major, one wave per row) unless the iteration has stopped
@@ -2414,6 +2422,13 @@ struct BGm {
     int ld, n;
     double *V, *w, *z, *st;    // basis (stride ld), work vectors, GMRES state
     int* flags;
+    // the step around the GMRES steps: boundary data, amplitudes [s; d]
+    // (s the solution), fbar, the right-hand side b, the grid field f*, E's
+    // terms on the primal grid (nx x ny), the strength out
+    const double *fp, *fm;
+    double *amp, *fbar, *b, *grid, *s_out;
+    const Term* tE;
+    int gnx, gny;
 };
 
 // z = P v_j for every plan still running; tiles (plan << 16 | 4-row block),
@@ -2467,9 +2482,8 @@ __global__ void k_bgm_flags(const BGm* __restrict__ g, int nprob, int* __restric
 
 // y = H^-1 g (upper triangular, j x j, thread 0), then w = V y (one
 // work-group, j <= 256)
-__global__ void __launch_bounds__(1024) k_gm_finish(const double* __restrict__ st, int m, int j,
-                                                    const double* __restrict__ V, int ldv, int n,
-                                                    double* __restrict__ w)
+__device__ __forceinline__ void gm_finish_body(const double* __restrict__ st, int m, int j,
+                                               const double* __restrict__ V, int ldv, int n, double* __restrict__ w)
 {
     __shared__ double y[256];
     __shared__ double Hs[64 * 64];   // the leading j x j block of H when j <= 64
@@ -2494,6 +2508,111 @@ __global__ void __launch_bounds__(1024) k_gm_finish(const double* __restrict__ s
         for (int i = 0; i < j; ++i) acc += V[k + (size_t)ldv * i] * y[i];
         w[k] = acc;
     }
+}
+
+__global__ void __launch_bounds__(1024) k_gm_finish(const double* __restrict__ st, int m, int j,
+                                                    const double* __restrict__ V, int ldv, int n,
+                                                    double* __restrict__ w)
+{
+    gm_finish_body(st, m, j, V, ldv, n, w);
+}
+
+// ---- the rest of a moving-body step over every plan (grid.y = plan; the
+// single-plan kernels' arithmetic, element for element)
+// amp = [0; f+ - f-], fbar = (f+ + f-)/2   (k_jump_avg)
+__global__ void k_bgm_jump(const BGm* __restrict__ g)
+{
+    const BGm& G = g[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= G.n) return;
+    const double a = G.fp[k], b = G.fm ? G.fm[k] : 0.0;
+    G.amp[k] = 0.0;
+    G.amp[G.n + k] = a - b;
+    G.fbar[k] = 0.5 * (a + b);
+}
+
+// b = E f* (k_gather with one term per point on Nodes{Primal}), then
+// b = -fbar + b (axpby)
+__global__ void __launch_bounds__(256) k_bgm_rhs(const BGm* __restrict__ g)
+{
+    const BGm& G = g[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= G.n) return;
+    const Term& tt = G.tE[p];
+    double acc0 = 0.0;
+    if (lane < 25) {
+        const int b = lane / 5, a = lane - 5 * b;
+        const int row = tt.by + b, col = tt.bx + a;
+        const double w = tt.wy[b] * tt.wx[a];
+        if (w != 0.0 && row >= 0 && row < G.gny && col >= 0 && col < G.gnx)
+            acc0 += tt.coef * w * G.grid[col + (size_t)G.gnx * row];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc0 += __shfl_xor(acc0, off, 64);
+    if (lane == 0) {
+        const double bv = acc0;
+        G.b[tt.ai] = -1.0 * G.fbar[tt.ai] + 1.0 * bv;
+    }
+}
+
+// mode 0: x = P b (x0 of GMRES); mode 1: x += P w for plans that converged
+// (the update after the steps)
+__global__ void __launch_bounds__(256) k_bgm_prec_xb(const BGm* __restrict__ g, const unsigned* __restrict__ tiles,
+                                                     int mode)
+{
+    const unsigned code = tiles[blockIdx.x];
+    const BGm& G = g[code >> 16];
+    if (mode == 1 && (G.flags[0] != 1 || G.flags[1] <= 0)) return;   // no steps: x stays P b
+    const int i = (int)(code & 0xFFFF) * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= G.n) return;
+    const double* v = mode == 0 ? G.b : G.w;
+    double* x = G.amp;
+    if (G.P32) {
+        const double s = wave_dot_f32(G.P32 + (size_t)G.ld * i, v, G.n, lane);
+        if (lane == 0) x[i] = s + (mode == 1 ? x[i] : 0.0);
+    } else if (lane == 0) {
+        double s = 0.0;
+        for (int k = 0; k < G.bw; ++k) s += G.band[(size_t)i * G.bw + k] * v[G.nbr[(size_t)i * G.bw + k]];
+        x[i] = mode == 1 ? x[i] + s : s;
+    }
+}
+
+// w = S x, then w = b - w (matvec_rows and axpby of gm_begin)
+__global__ void __launch_bounds__(256) k_bgm_resid(const BGm* __restrict__ g, const unsigned* __restrict__ tiles)
+{
+    const unsigned code = tiles[blockIdx.x];
+    const BGm& G = g[code >> 16];
+    const int i = (int)(code & 0xFFFF) * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= G.n) return;
+    const double s = wave_dot(G.S + (size_t)G.ld * i, G.amp, G.n, lane);
+    if (lane == 0) {
+        const double sx = 1.0 * s + 0.0;
+        G.w[i] = 1.0 * G.b[i] + -1.0 * sx;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_bgm_init(const BGm* __restrict__ g, int m, double tol)
+{
+    const BGm& G = g[blockIdx.x];
+    gm_init_body(G.w, G.b, G.n, G.V, G.st, m, tol, G.flags);
+}
+
+// y = H^-1 g, w = V y for the plans that converged
+__global__ void __launch_bounds__(1024) k_bgm_finish(const BGm* __restrict__ g, int m)
+{
+    const BGm& G = g[blockIdx.x];
+    if (G.flags[0] != 1) return;
+    const int j = G.flags[1];
+    if (j <= 0) return;
+    gm_finish_body(G.st, m, j, G.V, G.ld, G.n, G.w);
+}
+
+// s = amp[0, n)
+__global__ void k_bgm_copy_s(const BGm* __restrict__ g)
+{
+    const BGm& G = g[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < G.n) G.s_out[k] = G.amp[k];
 }
 
 // amp = [0; f+ - f-] (2n),  fbar = (f+ + f-)/2
@@ -2913,10 +3032,16 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
             }
         };
+        // the merged sources [R, D_s] (and E's terms, for S) of every plan whose
+        // geometry changed, built together on the context stream
+        std::vector<SourceJob> sj;
+        for (int i = 0; i < nprob; ++i)
+            if (mc[i] && !mc[i]->src_ok) sj.push_back(SourceJob{plans[i], &mc[i]->terms1, &mc[i]->src1});
+        merge_sources_batched(ctx, sj, OP_R_PRIMAL, OP_DS, OP_E_PRIMAL, GK_PRIMAL);
+        for (const SourceJob& j : sj) j.pl->mov->src_ok = true;
         fork();
         auto on = [&](int i) { ctx->stream = ctx->pool[i % ilh_ctx::POOL]; };
         std::vector<int> state(nprob, 0), it(nprob, 0);
-        int chunk = 1;
         std::vector<ilh_plan*> live_pl;
         std::vector<MovingCache*> live_mc;
         std::vector<double*> grid_out, f_out;
@@ -2928,38 +3053,26 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 state[i] = 1;
                 continue;
             }
-            moving_geometry(pl, mc[i]);
-            moving_rhs_jump(pl, mc[i], fb_plus[i], fb_minus ? fb_minus[i] : nullptr);
+            moving_geometry(pl, mc[i]);   // S of the new geometry (the sources are built above)
             live_pl.push_back(pl);
             live_mc.push_back(mc[i]);
             grid_out.push_back(mc[i]->grid.as<double>());
             f_out.push_back(f[i]);
         }
         join();
-        // f* = L^-1 D_s d of every plan in one fused convolution
-        const bool fused_conv = moving_conv_fused(ctx, 0, live_pl, live_mc, grid_out);
-        fork();
-        for (int i = 0; i < nprob; ++i) {
-            if (!mc[i]) continue;
-            on(i);
-            ilh_plan* pl = plans[i];
-            if (!fused_conv) conv_sources(pl, GK_PRIMAL, mc[i]->src1, mc[i]->amp.as<double>(), mc[i]->grid.as<double>());
-            moving_rhs_finish(pl, mc[i]);
-            gm_begin(pl, mc[i], mc[i]->b.as<double>(), mc[i]->amp.as<double>(), tol, m);
-            chunk = std::max(chunk, std::min(m, mc[i]->last_iters));
-        }
-        join();
-        // the GMRES steps of every plan in lockstep on the context stream: per
-        // step one launch each for P v, S z and the Arnoldi step over all
-        // plans (plans that stopped skip their tiles), one flag read-back per
-        // chunk of steps
+        // every plan's step around the GMRES steps as batched launches over all
+        // plans on the context stream; the GMRES steps in lockstep, one launch
+        // per phase, one flag read-back per chunk of steps
         std::vector<BGm> hg;
         std::vector<int> gi;   // plan of each descriptor
         std::vector<unsigned> tiles;
+        int chunk = 1, maxn = 1;
         for (int i = 0; i < nprob; ++i) {
             if (!mc[i]) continue;
             ilh_plan* pl = plans[i];
             MovingCache* c = mc[i];
+            c->gm.ensure(sizeof(double) * GmLayout{m}.size());
+            c->gmf.ensure(sizeof(int) * 2);
             BGm G{};
             G.P32 = c->use_band ? nullptr : c->P32.as<float>();
             G.band = c->use_band ? c->band.as<double>() : nullptr;
@@ -2973,10 +3086,22 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             G.z = c->z.as<double>();
             G.st = c->gm.as<double>();
             G.flags = c->gmf.as<int>();
+            G.fp = fb_plus[i];
+            G.fm = fb_minus ? fb_minus[i] : nullptr;
+            G.amp = c->amp.as<double>();
+            G.fbar = c->fbar.as<double>();
+            G.b = c->b.as<double>();
+            G.grid = c->grid.as<double>();
+            G.s_out = s[i];
+            G.tE = get_terms(pl, OP_E_PRIMAL);
+            G.gnx = pl->g.nx(GK_PRIMAL);
+            G.gny = pl->g.ny(GK_PRIMAL);
             if ((pl->N + 3) / 4 > 65535 || hg.size() >= 65535) fail(ILH_ERR_ARG, "batched GMRES too large");
             for (int b = 0; b < (pl->N + 3) / 4; ++b) tiles.push_back(((unsigned)hg.size() << 16) | (unsigned)b);
             hg.push_back(G);
             gi.push_back(i);
+            chunk = std::max(chunk, std::min(m, c->last_iters));
+            maxn = std::max(maxn, pl->N);
         }
         const int nd = (int)hg.size();
         if (nd > 0) {
@@ -3005,6 +3130,18 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             ILH_HIP(hipMemcpyAsync(dg, hg.data(), bytes_g, hipMemcpyHostToDevice, main));
             ILH_HIP(hipMemcpyAsync(dt, tiles.data(), sizeof(unsigned) * tiles.size(), hipMemcpyHostToDevice, main));
             const unsigned nt = (unsigned)tiles.size();
+            const dim3 gel((unsigned)((maxn + 255) / 256), (unsigned)nd);
+            // amp = [0; d], fbar; f* = L^-1 D_s d of every plan (one fused call);
+            // b = E f* - fbar; x0 = P b, r0 = b - S x0, v_0
+            k_bgm_jump<<<gel, 256, 0, main>>>(dg);
+            if (!moving_conv_fused(ctx, 0, live_pl, live_mc, grid_out))
+                for (size_t k = 0; k < live_pl.size(); ++k)
+                    conv_sources(live_pl[k], GK_PRIMAL, live_mc[k]->src1, live_mc[k]->amp.as<double>(), grid_out[k]);
+            k_bgm_rhs<<<dim3((unsigned)((maxn + 3) / 4), (unsigned)nd), 256, 0, main>>>(dg);
+            k_bgm_prec_xb<<<nt, 256, 0, main>>>(dg, dt, 0);
+            k_bgm_resid<<<nt, 256, 0, main>>>(dg, dt);
+            k_bgm_init<<<(unsigned)nd, 1024, 0, main>>>(dg, m, tol);
+            ILH_HIP(hipGetLastError());
             int j = 0;
             for (;;) {
                 const int j1 = std::min(m, j + chunk);
@@ -3030,24 +3167,27 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 if (!running) break;
                 chunk = 2;
             }
+            // x += P (V y) for the plans that converged
+            k_bgm_finish<<<(unsigned)nd, 1024, 0, main>>>(dg, m);
+            k_bgm_prec_xb<<<nt, 256, 0, main>>>(dg, dt, 1);
+            ILH_HIP(hipGetLastError());
+            for (int i = 0; i < nprob; ++i)
+                if (mc[i] && state[i] == 1 && it[i] > 0) mc[i]->last_iters = it[i];
+            // plans whose reference no longer preconditions are refactored and
+            // solved one by one
+            for (int i = 0; i < nprob; ++i) {
+                if (!mc[i] || state[i] == 1) continue;
+                it[i] = moving_refactor_solve(plans[i], mc[i], tol, m);
+            }
+            // f = L^-1 (R s + D_s d) of every plan in one fused call; s out
+            if (!moving_conv_fused(ctx, 1, live_pl, live_mc, f_out))
+                for (size_t k = 0; k < live_pl.size(); ++k)
+                    conv_sources(live_pl[k], GK_PRIMAL, live_mc[k]->src1, live_mc[k]->amp.as<double>(), f_out[k]);
+            k_bgm_copy_s<<<gel, 256, 0, main>>>(dg);
+            ILH_HIP(hipGetLastError());
+            for (int i = 0; i < nprob; ++i)
+                if (mc[i]) ++mc[i]->ref_age;
         }
-        fork();
-        for (int i = 0; i < nprob; ++i) {
-            if (!mc[i] || state[i] != 1) continue;
-            on(i);
-            gm_end(plans[i], mc[i], mc[i]->amp.as<double>(), m, it[i]);
-        }
-        // join the side streams; plans whose reference no longer preconditions
-        // are refactored and solved one by one on the context stream
-        join();
-        for (int i = 0; i < nprob; ++i) {
-            if (!mc[i] || state[i] == 1) continue;
-            it[i] = moving_refactor_solve(plans[i], mc[i], tol, m);
-        }
-        // f = L^-1 (R s + D_s d) of every plan in one fused convolution
-        const bool fused_final = moving_conv_fused(ctx, 1, live_pl, live_mc, f_out);
-        for (int i = 0; i < nprob; ++i)
-            if (mc[i]) moving_final(plans[i], mc[i], f[i], s[i], !fused_final);
         if (iters)
             for (int i = 0; i < nprob; ++i) iters[i] = it[i];
     });

@@ -503,91 +503,6 @@ void conv_dense_multi(ilh_plan* pl, int gk, const double* in, int nb, double* ou
     launch_rows_inv(pl->ctx, C.T2m.as<double2>(), C.Px, C.nx, C.ny, out, Batch{nb, s2, n});
 }
 
-// Row pairs (2p, 2p+1) with at least one entry, compacted in order: pinfo[k]
-// = {2p, first cell of row 2p, first cell of row 2p+1, end of the pair's
-// cells}, slot_of_row[2p] = 2k, slot_of_row[2p+1] = 2k+1 (-1 for rows of idle
-// pairs), *npairs = live pairs.  One work-group of 1024 threads.
-__global__ void __launch_bounds__(1024) k_source_pairs(const int* __restrict__ row_ptr,
-                                                       const int* __restrict__ cell_ptr, int ny,
-                                                       int4* __restrict__ pinfo, int* __restrict__ slot_of_row,
-                                                       int* __restrict__ npairs)
-{
-    __shared__ int wsum[16];
-    __shared__ int carry;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int np = (ny + 1) / 2;
-    if (t == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < np; base += 1024) {
-        const int p = base + t;
-        const int j = 2 * p;
-        bool on = false;
-        if (p < np) {
-            on = row_ptr[j + 1] > row_ptr[j];
-            if (j + 1 < ny) on = on || row_ptr[j + 2] > row_ptr[j + 1];
-        }
-        const int f = on ? 1 : 0;
-        int inc = f;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int u = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += u;
-        }
-        if (lane == 63) wsum[wave] = inc;
-        __syncthreads();
-        int k = carry;
-        for (int w = 0; w < wave; ++w) k += wsum[w];
-        k += inc - f;
-        if (p < np) {
-            if (on) pinfo[k] = make_int4(j, cell_ptr[j], cell_ptr[j + 1], cell_ptr[min(j + 2, ny)]);
-            slot_of_row[j] = on ? 2 * k : -1;
-            if (j + 1 < ny) slot_of_row[j + 1] = on ? 2 * k + 1 : -1;
-        }
-        __syncthreads();
-        if (t == 1023) carry = k + f;
-        __syncthreads();
-    }
-    if (t == 0) *npairs = carry;
-}
-
-// The source set's pair tables, built on the device from its row bins (no
-// read-back: the row pass launches over the upper bound (ny + 1) / 2 and
-// reads the live count).
-void build_source_pairs(ilh_plan* pl, SourceSet& s)
-{
-    const int ny = s.bins.ny;
-    const size_t np = (size_t)(ny + 1) / 2;
-    // tables are kept from the previous geometry when large enough
-    if (np + 1 > s.cap_pairs) {
-        if (s.pinfo) (void)hipFree(s.pinfo);
-        s.pinfo = nullptr;
-        s.cap_pairs = np + 1;
-        ILH_HIP(hipMalloc(&s.pinfo, sizeof(int4) * s.cap_pairs));
-    }
-    if ((size_t)ny + 1 > s.cap_rows) {
-        if (s.slot_of_row) (void)hipFree(s.slot_of_row);
-        s.slot_of_row = nullptr;
-        s.cap_rows = (size_t)ny + 1;
-        ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * s.cap_rows));   // rows, then the live count
-    }
-    s.npairs = (int)np;
-    s.npairs_dev = s.slot_of_row + ny;
-    k_source_pairs<<<1, 1024, 0, pl->ctx->stream>>>(s.bins.row_ptr, s.bins.cell_ptr, ny, s.pinfo, s.slot_of_row,
-                                                    s.npairs_dev);
-    ILH_HIP(hipGetLastError());
-}
-
-// The exact live pair count on the host (one read-back; for caches built
-// once and used by many solves): the row pass then launches exactly.
-void pin_source_pairs(ilh_plan* pl, SourceSet& s)
-{
-    if (!s.npairs_dev) return;
-    int n = 0;
-    ILH_HIP(hipMemcpyAsync(&n, s.npairs_dev, sizeof(int), hipMemcpyDeviceToHost, pl->ctx->stream));
-    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
-    s.npairs = n;
-    s.npairs_dev = nullptr;
-}
-
 static RowSrc terms_source(const ConvPlan& C, const SourceSet& src, const double* amp)
 {
     RowSrc S{};

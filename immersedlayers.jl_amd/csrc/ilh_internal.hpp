@@ -170,6 +170,10 @@ struct ilh_ctx {
     size_t mv_cap = 0;
     int* mv_host = nullptr;
     size_t mv_host_cap = 0;
+    // batched source builds: device job table and its host copy
+    void* geo_dev = nullptr;
+    size_t geo_cap = 0;
+    std::vector<char> geo_host;
     // owners: the caller's handle, every plan and batch of the context; the
     // context is torn down when the last one lets go (any destruction order)
     std::atomic<int> refs{1};
@@ -283,6 +287,14 @@ struct SourceSet {
 };
 void free_sources(SourceSet& s);   // releases the bins and pair tables
 void build_source_pairs(ilh_plan* pl, SourceSet& s);
+// the merged sources [op0; op1] of many plans (and the terms of extra_op, or
+// -1), every kernel one launch over all plans, on the context stream
+struct SourceJob {
+    ilh_plan* pl;
+    DevBuf* tbuf;
+    SourceSet* src;
+};
+void merge_sources_batched(ilh_ctx* ctx, const std::vector<SourceJob>& jobs, int op0, int op1, int extra_op, int gk);
 void pin_source_pairs(ilh_plan* pl, SourceSet& s);
 void conv_sources(ilh_plan* pl, int gk, const SourceSet& src, const double* amp, double* out,
                   hipEvent_t* ev = nullptr);

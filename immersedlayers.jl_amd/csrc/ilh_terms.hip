@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace ilh {
 
@@ -88,10 +89,10 @@ __device__ __forceinline__ void put(Term& t, int ai, int gk, const Win& x, const
 // one weight per thread: 16 threads per point, so the FP64 DDF evaluations
 // (square roots and arcsines) run in parallel rather than in a chain; every
 // operator's terms are then built from these.
-__global__ void k_ddf_windows(int N, int ddf, Grid g, const double* __restrict__ X, const double* __restrict__ Y,
-                              Win* __restrict__ wins)
+__device__ __forceinline__ void k_ddf_windows_body(int N, int ddf, Grid g, const double* __restrict__ X, const double* __restrict__ Y,
+                              Win* __restrict__ wins, int bx)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = bx * blockDim.x + threadIdx.x;
     if (k >= 16 * N) return;
     const int p = k >> 4, w = (k >> 2) & 3, e = k & 3;
     // fractional indices of the point on the primal (x_i = (i+1-I0) dx) and
@@ -108,10 +109,16 @@ __global__ void k_ddf_windows(int N, int ddf, Grid g, const double* __restrict__
     }
 }
 
-__global__ void k_build_terms(int op, int N, Grid g, const Win* __restrict__ wins, const double* __restrict__ NXv,
-                              const double* __restrict__ NYv, const double* __restrict__ DS, Term* __restrict__ out)
+__global__ void k_ddf_windows(int N, int ddf, Grid g, const double* __restrict__ X, const double* __restrict__ Y,
+                              Win* __restrict__ wins)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    k_ddf_windows_body(N, ddf, g, X, Y, wins, blockIdx.x);
+}
+
+__device__ __forceinline__ void k_build_terms_body(int op, int N, Grid g, const Win* __restrict__ wins, const double* __restrict__ NXv,
+                              const double* __restrict__ NYv, const double* __restrict__ DS, Term* __restrict__ out, int bx)
+{
+    const int p = bx * blockDim.x + threadIdx.x;
     if (p >= N) return;
     const double dx = g.dx;
     const Win Px = wins[4 * (size_t)p], Py = wins[4 * (size_t)p + 1];
@@ -243,6 +250,12 @@ __global__ void k_build_terms(int op, int N, Grid g, const Win* __restrict__ win
     out[2 * p + 1] = b;
 }
 
+__global__ void k_build_terms(int op, int N, Grid g, const Win* __restrict__ wins, const double* __restrict__ NXv,
+                              const double* __restrict__ NYv, const double* __restrict__ DS, Term* __restrict__ out)
+{
+    k_build_terms_body(op, N, g, wins, NXv, NYv, DS, out, blockIdx.x);
+}
+
 int op_terms_per_point(int op)
 {
     switch (op) {
@@ -290,8 +303,8 @@ const Term* get_terms(ilh_plan* pl, int op)
 // ------------------------------------------------------------ row binning
 // entries per row (terms of grid kind gk with a nonzero weight in the row):
 // one work-group, counts in LDS (dynamic, ny ints), written to cnt[0..ny)
-__global__ void __launch_bounds__(1024) k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny,
-                                                    int* __restrict__ cnt)
+__device__ __forceinline__ void k_bin_count_body(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                    int* __restrict__ cnt, int bx)
 {
     extern __shared__ int lc[];
     for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
@@ -307,12 +320,18 @@ __global__ void __launch_bounds__(1024) k_bin_count(const Term* __restrict__ t, 
     for (int r = threadIdx.x; r < ny; r += blockDim.x) cnt[r] = lc[r];
 }
 
+__global__ void __launch_bounds__(1024) k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                    int* __restrict__ cnt)
+{
+    k_bin_count_body(t, nt, gk, ny, cnt, blockIdx.x);
+}
+
 // exclusive scan of cnt[0..n) into ptr[0..n], single work-group of 1024
 // threads, 4096 values per round (4 per thread, wave shuffles, one LDS step).
 // n is *n_dev when n_dev is not null (a count produced on the device, at most
 // the n given).
-__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int n, int* __restrict__ ptr,
-                                               const int* __restrict__ n_dev)
+__device__ __forceinline__ void k_scan_body(const int* __restrict__ cnt, int n, int* __restrict__ ptr,
+                                               const int* __restrict__ n_dev, int bx)
 {
     __shared__ int wsum[16];
     __shared__ int carry;
@@ -348,10 +367,16 @@ __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int 
     if (t == 0) ptr[n] = carry;
 }
 
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int n, int* __restrict__ ptr,
+                                               const int* __restrict__ n_dev)
+{
+    k_scan_body(cnt, n, ptr, n_dev, blockIdx.x);
+}
+
 // the entries themselves, in arbitrary order within a row (k_bin_sort orders
 // them); one work-group, row cursors in LDS (dynamic, ny ints)
-__global__ void __launch_bounds__(1024) k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny,
-                                                   const int* __restrict__ ptr, int* __restrict__ ent)
+__device__ __forceinline__ void k_bin_fill_body(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                   const int* __restrict__ ptr, int* __restrict__ ent, int bx)
 {
     extern __shared__ int lc[];
     for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
@@ -368,12 +393,18 @@ __global__ void __launch_bounds__(1024) k_bin_fill(const Term* __restrict__ t, i
     }
 }
 
+__global__ void __launch_bounds__(1024) k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                   const int* __restrict__ ptr, int* __restrict__ ent)
+{
+    k_bin_fill_body(t, nt, gk, ny, ptr, ent, blockIdx.x);
+}
+
 // Sort each row's entries by (bx, term index): the order in which a cell sums
 // its contributions.  Bitonic sort in LDS for rows up to 4096 entries, an
 // in-place insertion sort by one thread beyond that.
-__global__ void k_bin_sort(const Term* __restrict__ t, const int* __restrict__ ptr, int* __restrict__ ent)
+__device__ __forceinline__ void k_bin_sort_body(const Term* __restrict__ t, const int* __restrict__ ptr, int* __restrict__ ent, int bx)
 {
-    const int row = blockIdx.x;
+    const int row = bx;
     const int lo = ptr[row], n = ptr[row + 1] - lo;
     if (n <= 1) return;
     __shared__ unsigned long long key[4096];
@@ -418,18 +449,29 @@ __global__ void k_bin_sort(const Term* __restrict__ t, const int* __restrict__ p
     for (int i = threadIdx.x; i < n; i += blockDim.x) ent[lo + i] = (int)(key[i] & 0xffffffffu);
 }
 
-__global__ void k_bin_bx(const Term* __restrict__ t, const int* __restrict__ ent, const int* __restrict__ n_dev,
-                         int* __restrict__ ebx)
+__global__ void k_bin_sort(const Term* __restrict__ t, const int* __restrict__ ptr, int* __restrict__ ent)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    k_bin_sort_body(t, ptr, ent, blockIdx.x);
+}
+
+__device__ __forceinline__ void k_bin_bx_body(const Term* __restrict__ t, const int* __restrict__ ent, const int* __restrict__ n_dev,
+                         int* __restrict__ ebx, int bx)
+{
+    const int k = bx * blockDim.x + threadIdx.x;
     if (k < *n_dev) ebx[k] = t[ent[k]].bx;
 }
 
-// entry records of row j: amplitude index and coef * wy[j - by] * wx[e]
-__global__ void k_bin_rec(const Term* __restrict__ t, const int* __restrict__ ptr, const int* __restrict__ ent,
-                          int* __restrict__ eai, double* __restrict__ erw)
+__global__ void k_bin_bx(const Term* __restrict__ t, const int* __restrict__ ent, const int* __restrict__ n_dev,
+                         int* __restrict__ ebx)
 {
-    const int j = blockIdx.x;
+    k_bin_bx_body(t, ent, n_dev, ebx, blockIdx.x);
+}
+
+// entry records of row j: amplitude index and coef * wy[j - by] * wx[e]
+__device__ __forceinline__ void k_bin_rec_body(const Term* __restrict__ t, const int* __restrict__ ptr, const int* __restrict__ ent,
+                          int* __restrict__ eai, double* __restrict__ erw, int bx)
+{
+    const int j = bx;
     for (int k = ptr[j] + threadIdx.x; k < ptr[j + 1]; k += blockDim.x) {
         const Term& tt = t[ent[k]];
         const double cw = tt.coef * tt.wy[j - tt.by];
@@ -437,6 +479,12 @@ __global__ void k_bin_rec(const Term* __restrict__ t, const int* __restrict__ pt
 #pragma unroll
         for (int e = 0; e < 5; ++e) erw[5 * (size_t)k + e] = cw * tt.wx[e];
     }
+}
+
+__global__ void k_bin_rec(const Term* __restrict__ t, const int* __restrict__ ptr, const int* __restrict__ ent,
+                          int* __restrict__ eai, double* __restrict__ erw)
+{
+    k_bin_rec_body(t, ptr, ent, eai, erw, blockIdx.x);
 }
 
 // Cells of a row: the x positions some entry contributes to (nonzero weight).
@@ -456,12 +504,12 @@ __device__ void mark_cells(const int* __restrict__ ptr, const int* __restrict__ 
     __syncthreads();
 }
 
-__global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
-                             int nx, int* __restrict__ cnt)
+__device__ __forceinline__ void k_cell_count_body(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
+                             int nx, int* __restrict__ cnt, int bx)
 {
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int tot[256];
-    const int j = blockIdx.x;
+    const int j = bx;
     if (ptr[j] == ptr[j + 1]) {   // no entries: no cells
         if (threadIdx.x == 0) cnt[j] = 0;
         return;
@@ -478,16 +526,22 @@ __global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict_
     }
 }
 
+__global__ void k_cell_count(const int* __restrict__ ptr, const int* __restrict__ ebx, const double* __restrict__ erw,
+                             int nx, int* __restrict__ cnt)
+{
+    k_cell_count_body(ptr, ebx, erw, nx, cnt, blockIdx.x);
+}
+
 static constexpr int CELL_LDS_ENTRIES = 4096;   // row entries whose bx are staged in LDS
 
-__global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__ ebx_g, const double* __restrict__ erw,
-                            int nx, const int* __restrict__ cptr, int* __restrict__ cells3)
+__device__ __forceinline__ void k_cell_fill_body(const int* __restrict__ ptr, const int* __restrict__ ebx_g, const double* __restrict__ erw,
+                            int nx, const int* __restrict__ cptr, int* __restrict__ cells3, int bx)
 {
     __shared__ unsigned bm[CELL_WORDS];
     __shared__ int woff[CELL_WORDS];
     __shared__ int wsum[4];
     __shared__ int sbx[CELL_LDS_ENTRIES];
-    const int j = blockIdx.x;
+    const int j = bx;
     if (ptr[j] == ptr[j + 1]) return;   // no entries: no cells
     const int lo_g = ptr[j], nrow = ptr[j + 1] - lo_g;
     const bool staged = nrow <= CELL_LDS_ENTRIES;
@@ -544,11 +598,17 @@ __global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__
     }
 }
 
-// contributions of cell c (x, entries [klo, khi)): the entries with a nonzero weight at x
-__global__ void k_contrib_count(const int* __restrict__ cells3, const int* __restrict__ n_dev,
-                                const int* __restrict__ ebx, const double* __restrict__ erw, int* __restrict__ cnt)
+__global__ void k_cell_fill(const int* __restrict__ ptr, const int* __restrict__ ebx_g, const double* __restrict__ erw,
+                            int nx, const int* __restrict__ cptr, int* __restrict__ cells3)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    k_cell_fill_body(ptr, ebx_g, erw, nx, cptr, cells3, blockIdx.x);
+}
+
+// contributions of cell c (x, entries [klo, khi)): the entries with a nonzero weight at x
+__device__ __forceinline__ void k_contrib_count_body(const int* __restrict__ cells3, const int* __restrict__ n_dev,
+                                const int* __restrict__ ebx, const double* __restrict__ erw, int* __restrict__ cnt, int bx)
+{
+    const int c = bx * blockDim.x + threadIdx.x;
     if (c >= *n_dev) return;
     const int x = cells3[3 * (size_t)c], klo = cells3[3 * (size_t)c + 1], khi = cells3[3 * (size_t)c + 2];
     int m = 0;
@@ -556,12 +616,18 @@ __global__ void k_contrib_count(const int* __restrict__ cells3, const int* __res
     cnt[c] = m;
 }
 
-__global__ void k_contrib_fill(const int* __restrict__ cells3, const int* __restrict__ n_dev,
+__global__ void k_contrib_count(const int* __restrict__ cells3, const int* __restrict__ n_dev,
+                                const int* __restrict__ ebx, const double* __restrict__ erw, int* __restrict__ cnt)
+{
+    k_contrib_count_body(cells3, n_dev, ebx, erw, cnt, blockIdx.x);
+}
+
+__device__ __forceinline__ void k_contrib_fill_body(const int* __restrict__ cells3, const int* __restrict__ n_dev,
                                const int* __restrict__ ebx, const int* __restrict__ eai,
                                const double* __restrict__ erw, const int* __restrict__ off, int4* __restrict__ cells,
-                               int* __restrict__ cai, double* __restrict__ cw)
+                               int* __restrict__ cai, double* __restrict__ cw, int bx)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = bx * blockDim.x + threadIdx.x;
     if (c >= *n_dev) return;
     const int x = cells3[3 * (size_t)c], klo = cells3[3 * (size_t)c + 1], khi = cells3[3 * (size_t)c + 2];
     int e = off[c];
@@ -576,12 +642,28 @@ __global__ void k_contrib_fill(const int* __restrict__ cells3, const int* __rest
     cells[c] = make_int4(x, off[c], e, 0);
 }
 
+__global__ void k_contrib_fill(const int* __restrict__ cells3, const int* __restrict__ n_dev,
+                               const int* __restrict__ ebx, const int* __restrict__ eai,
+                               const double* __restrict__ erw, const int* __restrict__ off, int4* __restrict__ cells,
+                               int* __restrict__ cai, double* __restrict__ cw)
+{
+    k_contrib_fill_body(cells3, n_dev, ebx, eai, erw, off, cells, cai, cw, blockIdx.x);
+}
+
 // Row bins of a term list for one output grid kind.  Buffers are sized by
 // upper bounds (a term touches at most 5 rows, an entry at most 5 cells), so
 // the whole build runs asynchronously on the stream.
-void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
+namespace {
+// The bins' sizes and buffers for nterms terms on grid kind gk (host side of
+// build_bins: buffers grow, never shrink).
+struct BinDims {
+    int ny, nx;
+    long c_max;
+    size_t E, Cm;
+};
+
+BinDims bins_prepare(ilh_plan* pl, int nterms, int gk, RowBins& b)
 {
-    hipStream_t st = pl->ctx->stream;
     b.gk = gk;
     b.ny = pl->g.ny(gk);
     const int nx = pl->g.nx(gk);
@@ -618,10 +700,42 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
         grow(b.coff, Cm + 1);
         b.cap_c = Cm;
     }
-    const size_t E = b.cap_e, Cm = b.cap_c;
+    if (b.ny > 16384) fail(ILH_ERR_ARG, "grid too tall for the row bins (more than 16384 rows)");
+    return BinDims{b.ny, nx, c_max, b.cap_e, b.cap_c};
+}
+
+// the pair tables' buffers (host side of build_source_pairs)
+void pairs_prepare(SourceSet& s)
+{
+    const int ny = s.bins.ny;
+    const size_t np = (size_t)(ny + 1) / 2;
+    // tables are kept from the previous geometry when large enough
+    if (np + 1 > s.cap_pairs) {
+        if (s.pinfo) (void)hipFree(s.pinfo);
+        s.pinfo = nullptr;
+        s.cap_pairs = np + 1;
+        ILH_HIP(hipMalloc(&s.pinfo, sizeof(int4) * s.cap_pairs));
+    }
+    if ((size_t)ny + 1 > s.cap_rows) {
+        if (s.slot_of_row) (void)hipFree(s.slot_of_row);
+        s.slot_of_row = nullptr;
+        s.cap_rows = (size_t)ny + 1;
+        ILH_HIP(hipMalloc(&s.slot_of_row, sizeof(int) * s.cap_rows));   // rows, then the live count
+    }
+    s.npairs = (int)np;
+    s.npairs_dev = s.slot_of_row + ny;
+}
+}  // namespace
+
+void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
+{
+    hipStream_t st = pl->ctx->stream;
+    const BinDims d = bins_prepare(pl, nterms, gk, b);
+    const int nx = d.nx;
+    const long c_max = d.c_max;
+    const size_t E = d.E, Cm = d.Cm;
     int* cnt = b.cnt;
     int* ebx = b.entries + E;
-    if (b.ny > 16384) fail(ILH_ERR_ARG, "grid too tall for the row bins (more than 16384 rows)");
     const size_t lc_bytes = sizeof(int) * (size_t)b.ny;
     // entries per row, sorted by (bx, term)
     k_bin_count<<<1, 1024, lc_bytes, st>>>(terms, nterms, gk, b.ny, cnt);
@@ -666,6 +780,327 @@ void free_sources(SourceSet& s)
     if (s.pinfo) (void)hipFree(s.pinfo);
     if (s.slot_of_row) (void)hipFree(s.slot_of_row);
     s = SourceSet();
+}
+
+// ----------------------------------------------------------- source pairs
+// Row pairs (2p, 2p+1) with at least one entry, compacted in order: pinfo[k]
+// = {2p, first cell of row 2p, first cell of row 2p+1, end of the pair's
+// cells}, slot_of_row[2p] = 2k, slot_of_row[2p+1] = 2k+1 (-1 for rows of idle
+// pairs), *npairs = live pairs.  One work-group of 1024 threads.
+__device__ __forceinline__ void k_source_pairs_body(const int* __restrict__ row_ptr, const int* __restrict__ cell_ptr,
+                                                    int ny, int4* __restrict__ pinfo, int* __restrict__ slot_of_row,
+                                                    int* __restrict__ npairs)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int np = (ny + 1) / 2;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < np; base += 1024) {
+        const int p = base + t;
+        const int j = 2 * p;
+        bool on = false;
+        if (p < np) {
+            on = row_ptr[j + 1] > row_ptr[j];
+            if (j + 1 < ny) on = on || row_ptr[j + 2] > row_ptr[j + 1];
+        }
+        const int f = on ? 1 : 0;
+        int inc = f;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += u;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int k = carry;
+        for (int w = 0; w < wave; ++w) k += wsum[w];
+        k += inc - f;
+        if (p < np) {
+            if (on) pinfo[k] = make_int4(j, cell_ptr[j], cell_ptr[j + 1], cell_ptr[min(j + 2, ny)]);
+            slot_of_row[j] = on ? 2 * k : -1;
+            if (j + 1 < ny) slot_of_row[j + 1] = on ? 2 * k + 1 : -1;
+        }
+        __syncthreads();
+        if (t == 1023) carry = k + f;
+        __syncthreads();
+    }
+    if (t == 0) *npairs = carry;
+}
+
+__global__ void __launch_bounds__(1024) k_source_pairs(const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ cell_ptr, int ny,
+                                                       int4* __restrict__ pinfo, int* __restrict__ slot_of_row,
+                                                       int* __restrict__ npairs)
+{
+    k_source_pairs_body(row_ptr, cell_ptr, ny, pinfo, slot_of_row, npairs);
+}
+
+// The source set's pair tables, built on the device from its row bins (no
+// read-back: the row pass launches over the upper bound (ny + 1) / 2 and
+// reads the live count).
+void build_source_pairs(ilh_plan* pl, SourceSet& s)
+{
+    const int ny = s.bins.ny;
+    pairs_prepare(s);
+    k_source_pairs<<<1, 1024, 0, pl->ctx->stream>>>(s.bins.row_ptr, s.bins.cell_ptr, ny, s.pinfo, s.slot_of_row,
+                                                    s.npairs_dev);
+    ILH_HIP(hipGetLastError());
+}
+
+// The exact live pair count on the host (one read-back; for caches built
+// once and used by many solves): the row pass then launches exactly.
+void pin_source_pairs(ilh_plan* pl, SourceSet& s)
+{
+    if (!s.npairs_dev) return;
+    int n = 0;
+    ILH_HIP(hipMemcpyAsync(&n, s.npairs_dev, sizeof(int), hipMemcpyDeviceToHost, pl->ctx->stream));
+    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+    s.npairs = n;
+    s.npairs_dev = nullptr;
+}
+
+
+// --------------------------------------------------- batched source build
+// The merged sources of many plans built together (the moving-body step of
+// many trajectories): DDF windows, terms, the merge, the row bins and the
+// pair tables, each kernel ONE launch over all plans (grid.y = plan; grid.x
+// the largest plan's, blocks past a plan's own extent return at once).  The
+// kernels are the single-plan ones' bodies, so the tables are identical.
+namespace {
+struct GeoJob {
+    int N, ddf;
+    Grid g;
+    const double *X, *Y, *nxv, *nyv, *ds;
+    Win* wins;
+    Term* t[3];      // terms of op[k] (op[k] < 0: none)
+    int op[3];
+    Term* merged;    // [terms of op 0; terms of op 1, amplitude slots + N]
+    int n0, n1;
+    int nt, gk, ny, nx, c_max;
+    int *cnt, *row_ptr, *entries, *ebx, *eai;
+    double* erw;
+    int *cell_ptr, *cells3, *ccnt, *coff;
+    int4* cells;
+    int* cai;
+    double* cw;
+    unsigned gE, gC;
+    int4* pinfo;
+    int *slot_of_row, *npairs_dev;
+};
+
+__global__ void k_geo_windows(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    k_ddf_windows_body(j.N, j.ddf, j.g, j.X, j.Y, j.wins, blockIdx.x);
+}
+
+__global__ void k_geo_terms(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    const int z = blockIdx.z;
+    if (j.op[z] < 0) return;
+    k_build_terms_body(j.op[z], j.N, j.g, j.wins, j.nxv, j.nyv, j.ds, j.t[z], blockIdx.x);
+}
+
+__global__ void k_geo_merge(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < j.n0) {
+        j.merged[k] = j.t[0][k];
+    } else if (k < j.n0 + j.n1) {
+        Term t = j.t[1][k - j.n0];
+        t.ai += j.N;
+        j.merged[k] = t;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_geo_bin_count(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    k_bin_count_body(j.merged, j.nt, j.gk, j.ny, j.cnt, 0);
+}
+
+// which: 0 entries per row, 1 cells per row, 2 contributions per cell
+__global__ void __launch_bounds__(1024) k_geo_scan(const GeoJob* __restrict__ J, int which)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if (which == 0) k_scan_body(j.cnt, j.ny, j.row_ptr, nullptr, 0);
+    else if (which == 1) k_scan_body(j.cnt, j.ny, j.cell_ptr, nullptr, 0);
+    else k_scan_body(j.ccnt, j.c_max, j.coff, j.cell_ptr + j.ny, 0);
+}
+
+__global__ void __launch_bounds__(1024) k_geo_bin_fill(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    k_bin_fill_body(j.merged, j.nt, j.gk, j.ny, j.row_ptr, j.entries, 0);
+}
+
+__global__ void k_geo_bin_sort(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if ((int)blockIdx.x >= j.ny) return;
+    k_bin_sort_body(j.merged, j.row_ptr, j.entries, blockIdx.x);
+}
+
+__global__ void k_geo_bin_bx(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if (blockIdx.x >= j.gE) return;
+    k_bin_bx_body(j.merged, j.entries, j.row_ptr + j.ny, j.ebx, blockIdx.x);
+}
+
+__global__ void k_geo_bin_rec(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if ((int)blockIdx.x >= j.ny) return;
+    k_bin_rec_body(j.merged, j.row_ptr, j.entries, j.eai, j.erw, blockIdx.x);
+}
+
+__global__ void k_geo_cell_count(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if ((int)blockIdx.x >= j.ny) return;
+    k_cell_count_body(j.row_ptr, j.ebx, j.erw, j.nx, j.cnt, blockIdx.x);
+}
+
+__global__ void k_geo_cell_fill(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if ((int)blockIdx.x >= j.ny) return;
+    k_cell_fill_body(j.row_ptr, j.ebx, j.erw, j.nx, j.cell_ptr, j.cells3, blockIdx.x);
+}
+
+__global__ void k_geo_contrib_count(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if (blockIdx.x >= j.gC) return;
+    k_contrib_count_body(j.cells3, j.cell_ptr + j.ny, j.ebx, j.erw, j.ccnt, blockIdx.x);
+}
+
+__global__ void k_geo_contrib_fill(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    if (blockIdx.x >= j.gC) return;
+    k_contrib_fill_body(j.cells3, j.cell_ptr + j.ny, j.ebx, j.eai, j.erw, j.coff, j.cells, j.cai, j.cw, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(1024) k_geo_pairs(const GeoJob* __restrict__ J)
+{
+    const GeoJob& j = J[blockIdx.y];
+    k_source_pairs_body(j.row_ptr, j.cell_ptr, j.ny, j.pinfo, j.slot_of_row, j.npairs_dev);
+}
+}  // namespace
+
+void merge_sources_batched(ilh_ctx* ctx, const std::vector<SourceJob>& jobs, int op0, int op1, int extra_op, int gk)
+{
+    if (jobs.empty()) return;
+    if (jobs.size() > 65535) fail(ILH_ERR_ARG, "too many plans for one batched source build");
+    hipStream_t st = ctx->stream;
+    // the previous call's job table may still be in flight from the host copy
+    ILH_HIP(hipStreamSynchronize(st));
+    std::vector<GeoJob> hj(jobs.size());
+    unsigned gx_w = 1, gx_t = 1, gx_m = 1, gE = 1, gC = 1;
+    int max_ny = 1;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        ilh_plan* pl = jobs[i].pl;
+        const int N = pl->N;
+        if (N <= 0) fail(ILH_ERR_ARG, "batched source build of a plan without points");
+        GeoJob& j = hj[i];
+        std::memset(&j, 0, sizeof(j));
+        j.N = N;
+        j.ddf = pl->ddf;
+        j.g = pl->g;
+        j.X = pl->X.as<double>();
+        j.Y = pl->Y.as<double>();
+        j.nxv = pl->nx.as<double>();
+        j.nyv = pl->ny.as<double>();
+        j.ds = pl->ds.as<double>();
+        pl->wins.ensure(sizeof(Win) * 4 * (size_t)N);
+        j.wins = pl->wins.as<Win>();
+        const int ops[3] = {op0, op1, extra_op};
+        for (int k = 0; k < 3; ++k) {
+            j.op[k] = ops[k];
+            if (ops[k] < 0) continue;
+            pl->terms[ops[k]].ensure(sizeof(Term) * (size_t)N * op_terms_per_point(ops[k]));
+            j.t[k] = pl->terms[ops[k]].as<Term>();
+        }
+        j.n0 = N * op_terms_per_point(op0);
+        j.n1 = N * op_terms_per_point(op1);
+        jobs[i].tbuf->ensure(sizeof(Term) * (size_t)(j.n0 + j.n1));
+        j.merged = jobs[i].tbuf->as<Term>();
+        SourceSet& src = *jobs[i].src;
+        src.terms = j.merged;
+        src.amp_len = 2 * N;
+        const BinDims d = bins_prepare(pl, j.n0 + j.n1, gk, src.bins);
+        RowBins& b = src.bins;
+        j.nt = j.n0 + j.n1;
+        j.gk = gk;
+        j.ny = d.ny;
+        j.nx = d.nx;
+        j.c_max = (int)d.c_max;
+        j.cnt = b.cnt;
+        j.row_ptr = b.row_ptr;
+        j.entries = b.entries;
+        j.ebx = b.entries + d.E;
+        j.eai = b.eai;
+        j.erw = b.erw;
+        j.cell_ptr = b.cell_ptr;
+        j.cells3 = b.cells3;
+        j.ccnt = b.ccnt;
+        j.coff = b.coff;
+        j.cells = b.cells;
+        j.cai = b.cai;
+        j.cw = b.cw;
+        j.gE = (unsigned)((d.E + 255) / 256);
+        j.gC = (unsigned)((d.Cm + 255) / 256);
+        pairs_prepare(src);
+        j.pinfo = src.pinfo;
+        j.slot_of_row = src.slot_of_row;
+        j.npairs_dev = src.npairs_dev;
+        gx_w = std::max(gx_w, (unsigned)((16 * N + 255) / 256));
+        gx_t = std::max(gx_t, (unsigned)((N + 127) / 128));
+        gx_m = std::max(gx_m, (unsigned)((j.n0 + j.n1 + 255) / 256));
+        gE = std::max(gE, j.gE);
+        gC = std::max(gC, j.gC);
+        max_ny = std::max(max_ny, d.ny);
+    }
+    const size_t bytes = sizeof(GeoJob) * hj.size();
+    if (bytes > ctx->geo_cap) {
+        if (ctx->geo_dev) (void)hipFree(ctx->geo_dev);
+        ctx->geo_dev = nullptr;
+        ctx->geo_cap = 0;
+        ILH_HIP(hipMalloc(&ctx->geo_dev, bytes));
+        ctx->geo_cap = bytes;
+    }
+    ctx->geo_host.assign((const char*)hj.data(), (const char*)hj.data() + bytes);
+    const GeoJob* J = (const GeoJob*)ctx->geo_dev;
+    ILH_HIP(hipMemcpyAsync(ctx->geo_dev, ctx->geo_host.data(), bytes, hipMemcpyHostToDevice, st));
+    const unsigned np = (unsigned)hj.size();
+    const size_t lc_bytes = sizeof(int) * (size_t)max_ny;
+    k_geo_windows<<<dim3(gx_w, np), 256, 0, st>>>(J);
+    k_geo_terms<<<dim3(gx_t, np, 3), 128, 0, st>>>(J);
+    k_geo_merge<<<dim3(gx_m, np), 256, 0, st>>>(J);
+    k_geo_bin_count<<<dim3(1, np), 1024, lc_bytes, st>>>(J);
+    k_geo_scan<<<dim3(1, np), 1024, 0, st>>>(J, 0);
+    k_geo_bin_fill<<<dim3(1, np), 1024, lc_bytes, st>>>(J);
+    k_geo_bin_sort<<<dim3((unsigned)max_ny, np), 256, 0, st>>>(J);
+    k_geo_bin_bx<<<dim3(gE, np), 256, 0, st>>>(J);
+    k_geo_bin_rec<<<dim3((unsigned)max_ny, np), 64, 0, st>>>(J);
+    k_geo_cell_count<<<dim3((unsigned)max_ny, np), 256, 0, st>>>(J);
+    k_geo_scan<<<dim3(1, np), 1024, 0, st>>>(J, 1);
+    k_geo_cell_fill<<<dim3((unsigned)max_ny, np), 256, 0, st>>>(J);
+    k_geo_contrib_count<<<dim3(gC, np), 256, 0, st>>>(J);
+    k_geo_scan<<<dim3(1, np), 1024, 0, st>>>(J, 2);
+    k_geo_contrib_fill<<<dim3(gC, np), 256, 0, st>>>(J);
+    k_geo_pairs<<<dim3(1, np), 1024, 0, st>>>(J);
+    ILH_HIP(hipGetLastError());
+    for (const SourceJob& sj : jobs) {
+        sj.pl->wins_ok = true;
+        for (int op : {op0, op1, extra_op})
+            if (op >= 0) sj.pl->terms_ok[op] = true;
+    }
 }
 
 // --------------------------------------------------------------- scatter
