@@ -349,118 +349,10 @@ def bench_stokes(args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle CPU time")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dirichlet", "ops", "batched", "heat", "stokes"], default="dirichlet",
-                    help="dirichlet: the repeat-solve metric; ops: operator sweeps on the 2048^2 "
-                         "microbenchmark grid (GB/s against HBM peak)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run 5 device copies of 256 MiB (torch copy_) as the reference "
-                         "kernel for scaling the FETCH_SIZE/WRITE_SIZE counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
-              file=sys.stderr)
-        sys.exit(2)
-    if os.environ.get("RANK", "0") != "0":
-        return
-
-    if args.copy_calibration:
-        import torch
-        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 5 x %d bytes read + written" % (a.numel() * 8))
-
-    if args.workload == "ops":
-        return bench_ops(args)
-    if args.workload == "batched":
-        return bench_batched(args)
-    if args.workload == "heat":
-        return bench_heat(args)
-    if args.workload == "stokes":
-        return bench_stokes(args)
-
-    import ilhip
-    from ilhip import geometry as G
-
-    g, s, fb = G.config_headline()
-    nx, ny = g.primal_shape()
-    ng = nx * ny
-    log("workload: grid %dx%d, N=%d" % (nx, ny, s.N))
-    ctx = ilhip.Context(0)
-    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
-    prob = ilhip.DirichletPoisson(cache)
-    t0 = time.perf_counter()
-    f_gpu, s_gpu = prob.solve(fb, filter_power=0)   # builds S, its factor and the solve matrices
-    t_prep = time.perf_counter() - t0
-    fp = ilhip.DeviceArray.from_host(ctx, fb)
-    fd = ilhip.DeviceArray(ctx, ng)
-    sd = ilhip.DeviceArray(ctx, s.N)
-    # per-kernel launch time, before the timed region: each stage launched
-    # alone, back to back, between two HIP events on the plan's stream
-    stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
-    for _ in range(args.warmup):
-        prob.solve_device(fp, None, fd, sd)
-    ctx.synchronize()
-    # timed region: exactly K solves launched back to back from one host call
-    # (HIP events only around the whole loop), synchronized on both sides
-    t0 = time.perf_counter()
-    _, tot_ms = prob.profile(fp, None, fd, sd, args.steps, stages=False)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    sps = args.steps / wall
-    f_dev = fd.download()
-    assert np.allclose(f_dev, f_gpu.flat(), rtol=0, atol=0)
-
-    # repeat solves with many boundary data at once (8 right-hand sides per
-    # call, seeded scalings of the same data): solves/s of the batched form
-    nrhs = 8
-    fps = np.stack([fb * (1.0 + 0.05 * k) for k in range(nrhs)], axis=1)
-    fp_m = ilhip.DeviceArray.from_host(ctx, fps)
-    fd_m = ilhip.DeviceArray(ctx, ng * nrhs)
-    sd_m = ilhip.DeviceArray(ctx, s.N * nrhs)
-    for _ in range(5):
-        prob.solve_multi_device(nrhs, fp_m, None, fd_m, sd_m)
-    ctx.synchronize()
-    reps = max(20, args.steps // (4 * nrhs))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        prob.solve_multi_device(nrhs, fp_m, None, fd_m, sd_m)
-    ctx.synchronize()
-    multi_sps = reps * nrhs / (time.perf_counter() - t0)
-    f_multi0 = fd_m.download()[:ng]
-    assert np.array_equal(f_multi0, f_gpu.flat())   # column 0 is the headline solve, bit for bit
-    del fp_m, fd_m, sd_m
-
-    # the split-precision solve (FP64 forward transforms and s, FP32 inverse
-    # transforms): solves/s and f against the FP64 solve
-    fd32 = ilhip.DeviceArray(ctx, ng)
-    for _ in range(5):
-        prob.solve_mixed_device(fp, None, fd32, sd)
-    ctx.synchronize()
-    reps = max(50, min(args.steps, 1000))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        prob.solve_mixed_device(fp, None, fd32, sd)
-    ctx.synchronize()
-    mixed_sps = reps / (time.perf_counter() - t0)
-    f32 = fd32.download()
-    mixed_err = float(np.linalg.norm(f32 - f_gpu.flat()) / np.linalg.norm(f_gpu.flat()))
-    del fd32
-
-    # the same solve with host arrays in and out (PCIe-inclusive, not the metric)
-    t0 = time.perf_counter()
-    for _ in range(20):
-        prob.solve(fb, filter_power=0)
-    host_sps = 20 / (time.perf_counter() - t0)
+def moving_measurements(ilhip, G, ctx, cache, prob, g, s, fp, fd, sd, ng):
+    """Measurement (ii) on the headline geometry: the plain new-geometry solve,
+    moving-body steps (GMRES with the recycled FP32 inverse; with the learned
+    band), and 32 trajectories batched.  Leaves `cache` at geometry `s`."""
     # new-geometry solve (a moving-body step): update_system! with the bodies
     # pitched and heaved, then Schur build + factorisation + solve matrices + one solve
     new_geo_ms = []
@@ -552,6 +444,171 @@ def main():
         c.close()
     del tf, ts
 
+    cache.update_points(s)
+    return {
+        "new_geometry_solve_ms": float(np.median(new_geo_ms)),
+        "moving_step_ms": float(np.mean(mv_ms)),
+        "moving_gmres_iterations": mv_it,
+        "moving_batched_ms_per_solve": float(np.mean(bat_ms)),
+        "moving_batched_trajectories": ntraj,
+        "moving_batched_mean_iterations": bat_it,
+        "learned_ms_per_solve": float(np.mean(ln_ms)),
+        "learned_gmres_iterations": ln_it,
+        "learned_training": {"rows": n_rows, "data_s": t_data, "train_s": t_train, "steps": 3000,
+                             "final_loss": loss},
+        "preconditioned_speedup": {
+            "single": float(np.mean(new_geo_ms) / np.mean(mv_ms)),
+            "batched": float(np.mean(new_geo_ms) / np.mean(bat_ms)),
+            "learned": float(np.mean(new_geo_ms) / np.mean(ln_ms))},
+    }
+
+
+def bench_moving(args):
+    """Measurement (ii) alone: new-geometry, moving-body and batched
+    moving-body solves on the headline geometry (32 trajectories)."""
+    import ilhip
+    from ilhip import geometry as G
+    g, s, fb = G.config_headline()
+    ng = (g.NX - 1) * (g.NY - 1)
+    ctx = ilhip.Context(0)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    prob = ilhip.DirichletPoisson(cache)
+    prob.solve(fb, filter_power=0)
+    fp = ilhip.DeviceArray.from_host(ctx, fb)
+    fd = ilhip.DeviceArray(ctx, ng)
+    sd = ilhip.DeviceArray(ctx, s.N)
+    ex = moving_measurements(ilhip, G, ctx, cache, prob, g, s, fp, fd, sd, ng)
+    value = 1e3 / ex["moving_batched_ms_per_solve"]
+    out = {"metric": "moving-body Dirichlet solves/s (32 trajectories batched, new geometry each step)",
+           "value": value, "unit": "solves/s", "n_gpus": 1, "steps": 4, "warmup": 1,
+           "ms_per_step": ex["moving_batched_ms_per_solve"] * ex["moving_batched_trajectories"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded pitch/heave trajectories of the headline bodies)",
+           "config": {"workload": "moving bodies, 1024^2 grid, N = %d, 32 trajectories" % s.N}}
+    out.update(ex)
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle CPU time")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the measurements after the headline's (new geometry, moving bodies, learned "
+                         "preconditioner), e.g. for counter runs")
+    ap.add_argument("--workload", choices=["dirichlet", "ops", "batched", "heat", "stokes", "moving"],
+                    default="dirichlet",
+                    help="dirichlet: the repeat-solve metric; ops: operator sweeps on the 2048^2 "
+                         "microbenchmark grid (GB/s against HBM peak)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run 5 device copies of 256 MiB (torch copy_) as the reference "
+                         "kernel for scaling the FETCH_SIZE/WRITE_SIZE counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    if os.environ.get("RANK", "0") != "0":
+        return
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 5 x %d bytes read + written" % (a.numel() * 8))
+
+    if args.workload == "ops":
+        return bench_ops(args)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "heat":
+        return bench_heat(args)
+    if args.workload == "stokes":
+        return bench_stokes(args)
+    if args.workload == "moving":
+        return bench_moving(args)
+
+    import ilhip
+    from ilhip import geometry as G
+
+    g, s, fb = G.config_headline()
+    nx, ny = g.primal_shape()
+    ng = nx * ny
+    log("workload: grid %dx%d, N=%d" % (nx, ny, s.N))
+    ctx = ilhip.Context(0)
+    cache = ilhip.SurfaceScalarCache(s, g, ctx=ctx)
+    prob = ilhip.DirichletPoisson(cache)
+    t0 = time.perf_counter()
+    f_gpu, s_gpu = prob.solve(fb, filter_power=0)   # builds S, its factor and the solve matrices
+    t_prep = time.perf_counter() - t0
+    fp = ilhip.DeviceArray.from_host(ctx, fb)
+    fd = ilhip.DeviceArray(ctx, ng)
+    sd = ilhip.DeviceArray(ctx, s.N)
+    # per-kernel launch time, before the timed region: each stage launched
+    # alone, back to back, between two HIP events on the plan's stream
+    stage_ms, _ = prob.profile(fp, None, fd, sd, max(200, min(args.steps, 2000)))
+    for _ in range(args.warmup):
+        prob.solve_device(fp, None, fd, sd)
+    ctx.synchronize()
+    # timed region: exactly K solves launched back to back from one host call
+    # (HIP events only around the whole loop), synchronized on both sides
+    t0 = time.perf_counter()
+    _, tot_ms = prob.profile(fp, None, fd, sd, args.steps, stages=False)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    sps = args.steps / wall
+    f_dev = fd.download()
+    assert np.allclose(f_dev, f_gpu.flat(), rtol=0, atol=0)
+
+    # repeat solves with many boundary data at once (8 right-hand sides per
+    # call, seeded scalings of the same data): solves/s of the batched form
+    nrhs = 8
+    fps = np.stack([fb * (1.0 + 0.05 * k) for k in range(nrhs)], axis=1)
+    fp_m = ilhip.DeviceArray.from_host(ctx, fps)
+    fd_m = ilhip.DeviceArray(ctx, ng * nrhs)
+    sd_m = ilhip.DeviceArray(ctx, s.N * nrhs)
+    for _ in range(5):
+        prob.solve_multi_device(nrhs, fp_m, None, fd_m, sd_m)
+    ctx.synchronize()
+    reps = max(20, args.steps // (4 * nrhs))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prob.solve_multi_device(nrhs, fp_m, None, fd_m, sd_m)
+    ctx.synchronize()
+    multi_sps = reps * nrhs / (time.perf_counter() - t0)
+    f_multi0 = fd_m.download()[:ng]
+    assert np.array_equal(f_multi0, f_gpu.flat())   # column 0 is the headline solve, bit for bit
+    del fp_m, fd_m, sd_m
+
+    # the split-precision solve (FP64 forward transforms and s, FP32 inverse
+    # transforms): solves/s and f against the FP64 solve
+    fd32 = ilhip.DeviceArray(ctx, ng)
+    for _ in range(5):
+        prob.solve_mixed_device(fp, None, fd32, sd)
+    ctx.synchronize()
+    reps = max(50, min(args.steps, 1000))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prob.solve_mixed_device(fp, None, fd32, sd)
+    ctx.synchronize()
+    mixed_sps = reps / (time.perf_counter() - t0)
+    f32 = fd32.download()
+    mixed_err = float(np.linalg.norm(f32 - f_gpu.flat()) / np.linalg.norm(f_gpu.flat()))
+    del fd32
+
+    # the same solve with host arrays in and out (PCIe-inclusive, not the metric)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        prob.solve(fb, filter_power=0)
+    host_sps = 20 / (time.perf_counter() - t0)
+    extra = moving_measurements(ilhip, G, ctx, cache, prob, g, s, fp, fd, sd, ng) if not args.no_extras else {}
+
     # algorithmic bytes per launch (FP64) of each kernel of the solve
     N, b = s.N, 8
     Px = Py = 2048
@@ -594,24 +651,12 @@ def main():
         "solve_model_frac": solve_gbs / HBM_PEAK_GBS,
         "prep_s": t_prep,
         "host_io_solves_per_s": host_sps,
-        "new_geometry_solve_ms": float(np.median(new_geo_ms)),
         "multi_rhs_solves_per_s": multi_sps,
         "multi_rhs_per_call": nrhs,
-        "moving_step_ms": float(np.mean(mv_ms)),
-        "moving_gmres_iterations": mv_it,
-        "moving_batched_ms_per_solve": float(np.mean(bat_ms)),
-        "moving_batched_trajectories": ntraj,
-        "moving_batched_mean_iterations": bat_it,
         "mixed_solves_per_s": mixed_sps,
         "mixed_f_rel_err_vs_fp64": mixed_err,
-        "learned_ms_per_solve": float(np.mean(ln_ms)),
-        "learned_gmres_iterations": ln_it,
-        "learned_training": {"rows": n_rows, "data_s": t_data, "train_s": t_train, "steps": 3000, "final_loss": loss},
-        "preconditioned_speedup": {
-            "single": float(np.mean(new_geo_ms) / np.mean(mv_ms)),
-            "batched": float(np.mean(new_geo_ms) / np.mean(bat_ms)),
-            "learned": float(np.mean(new_geo_ms) / np.mean(ln_ms))},
     }
+    out.update(extra)
     if not args.no_cpu:
         cb, f_ref, s_ref = cpu_baseline(g, s, fb, args.cpu_budget)
         out["cpu_baseline"] = cb
