@@ -154,13 +154,26 @@ __global__ void k_sum_splits(const float* __restrict__ part, int nz, size_t stri
     out[e] = s;
 }
 
-__global__ void k_colsum_bf(const uint16_t* __restrict__ X, int rows, int cols, int ld, float* __restrict__ out)
+// column sums of a BF16 matrix (the bias gradients): a work-group of 1024
+// threads per 32 columns, thread (g, c) summing rows g, g + 32, ... of column
+// c, then the 32 partial sums of each column added in a fixed order
+// (deterministic)
+__global__ void __launch_bounds__(1024) k_colsum_bf(const uint16_t* __restrict__ X, int rows, int cols, int ld,
+                                                    float* __restrict__ out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float part[32][33];
+    const int cl = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int c = blockIdx.x * 32 + cl;
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) s += bf2f(X[(size_t)r * ld + c]);
-    out[c] = s;
+    if (c < cols)
+        for (int r = g; r < rows; r += 32) s += bf2f(X[(size_t)r * ld + c]);
+    part[g][cl] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) {
+        float t = 0.f;
+        for (int k = 0; k < 32; ++k) t += part[k][cl];
+        out[c] = t;
+    }
 }
 
 // dZ = dA * (A > 0), BF16 (A the ReLU output of the layer)
@@ -465,7 +478,7 @@ double lpc_backward(ilh_lpc* m, int rows)
               m->splitbuf.as<float>(), nullptr, out, splits, nW);
         k_sum_splits<<<(unsigned)((nW + 255) / 256), 256, 0, st>>>(m->splitbuf.as<float>(), splits, nW, nW,
                                                                     Gf + m->off_w[l]);
-        k_colsum_bf<<<(out + 255) / 256, 256, 0, st>>>(m->dZ.as<uint16_t>(), rows, out, out, Gf + m->off_b[l]);
+        k_colsum_bf<<<(out + 31) / 32, 1024, 0, st>>>(m->dZ.as<uint16_t>(), rows, out, out, Gf + m->off_b[l]);
         ILH_HIP(hipGetLastError());
         if (l > 0) {
             // dA = dZ W^T (rows x in), then dZ = dA * (act[l] > 0)
