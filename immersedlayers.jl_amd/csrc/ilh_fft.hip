@@ -45,7 +45,7 @@ __global__ void k_stage_tw(double2* tw, int off, int Ns, int R)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= Ns * (R - 1)) return;
-    const int k = e / (R - 1), r = e % (R - 1) + 1;
+    const int r = e / Ns + 1, k = e % Ns;   // entry (r - 1) Ns + k: a stage's loads of one r are contiguous over k
     double s, c;
     sincospi(2.0 * (double)((long long)r * k) / (double)(Ns * R), &s, &c);
     tw[off + e] = make_double2(c, -s);

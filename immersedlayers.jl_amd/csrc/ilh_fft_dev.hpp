@@ -45,7 +45,9 @@ __host__ __device__ constexpr int nsub(int P, bool rev, int s)
     return n;
 }
 // twiddle table: fwd stages then rev stages; stage s holds
-// W_{Ns R}^{r k} = exp(-2 pi i r k / (Ns R)) at [k (R-1) + r - 1], k < Ns, 1 <= r < R
+// W_{Ns R}^{r k} = exp(-2 pi i r k / (Ns R)) at [(r-1) Ns + k], k < Ns, 1 <= r < R,
+// so that the 64 lanes of a wave loading one r read consecutive k: 1 KB in
+// 8 cache lines per load instead of 64 lanes 112 bytes apart
 __host__ __device__ constexpr int tw_count(int P, bool rev)
 {
     int c = 0;
@@ -166,7 +168,7 @@ __device__ __forceinline__ void tw_load(C2 (&w)[V - 1], int t, const C2* __restr
         for (int q = 0; q < V / R; ++q) {
             const int k = (t + q * T) & (Ns - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) w[q * (R - 1) + r - 1] = base[k * (R - 1) + r - 1];
+            for (int r = 1; r < R; ++r) w[q * (R - 1) + r - 1] = base[(r - 1) * Ns + k];
         }
     }
 }
