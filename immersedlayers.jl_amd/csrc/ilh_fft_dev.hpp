@@ -642,12 +642,15 @@ __device__ __forceinline__ void rows_inv_body(const typename Cplx<R>::C2* __rest
     }
     fft<P, 1, false>(v, x, t, tw);
     if (!live) return;
+    // the field is the pass's result, read by no later pass of the
+    // convolution: streaming stores (whole lines per wave) leave it out of the
+    // L2 write-back at the kernel's end
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int xo = io_index(RL, q / RL, q % RL, t, T, P);
         if (xo < nx) {
-            out[xo + (size_t)nx * j0] = (TO)v[q].x;
-            if (two) out[xo + (size_t)nx * (j0 + 1)] = (TO)v[q].y;
+            __builtin_nontemporal_store((TO)v[q].x, &out[xo + (size_t)nx * j0]);
+            if (two) __builtin_nontemporal_store((TO)v[q].y, &out[xo + (size_t)nx * (j0 + 1)]);
         }
     }
 }
