@@ -2243,7 +2243,7 @@ __global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, 
 }
 
 // y = M x (M row-major, one wave per row) unless the iteration has stopped
-__global__ void k_gm_matvec(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
+__global__ void __launch_bounds__(256) k_gm_matvec(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
                             int n, const int* __restrict__ flags)
 {
     if (flags[0]) return;

@@ -824,7 +824,7 @@ void transpose(hipStream_t st, const double* A, double* At, int rows, int cols, 
 
 // One wave per row: y[i] = alpha * dot(M[i, :], x) (+ y[i]).  M row-major n x m
 // with leading dimension ld (a multiple of 2, rows 16-byte aligned).
-__global__ void k_matvec_rows(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
+__global__ void __launch_bounds__(256) k_matvec_rows(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
                               int n, int m, double alpha, int accumulate)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -842,7 +842,7 @@ void matvec_rows(hipStream_t st, const double* M, int ld, const double* x, doubl
 }
 
 // Dirichlet surface step: s = M+ f+ (+ M- f-),  amp = [s; f+ - f-], s_out = s
-__global__ void k_dir_amp(const double* __restrict__ Mp, const double* __restrict__ Mm, int ld,
+__global__ void __launch_bounds__(256) k_dir_amp(const double* __restrict__ Mp, const double* __restrict__ Mm, int ld,
                           const double* __restrict__ fp, const double* __restrict__ fm, double* __restrict__ amp,
                           double* __restrict__ s_out, int n)
 {
@@ -890,10 +890,10 @@ __device__ __forceinline__ void wave_dot_group(const double* __restrict__ row, c
         for (int g = 0; g < NG; ++g) {
             const double2* x2 = (const double2*)(x + (size_t)ldf * g);
             const double2 b0 = x2[k], b1 = x2[k + 64], b2 = x2[k + 128], b3 = x2[k + 192];
-            s[g][0] += a0.x * b0.x + a0.y * b0.y;
-            s[g][1] += a1.x * b1.x + a1.y * b1.y;
-            s[g][2] += a2.x * b2.x + a2.y * b2.y;
-            s[g][3] += a3.x * b3.x + a3.y * b3.y;
+            s[g][0] += dot2(a0, b0);
+            s[g][1] += dot2(a1, b1);
+            s[g][2] += dot2(a2, b2);
+            s[g][3] += dot2(a3, b3);
         }
     }
     for (; k < m2; k += 64) {
@@ -901,7 +901,7 @@ __device__ __forceinline__ void wave_dot_group(const double* __restrict__ row, c
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const double2 b0 = ((const double2*)(x + (size_t)ldf * g))[k];
-            s[g][0] += a0.x * b0.x + a0.y * b0.y;
+            s[g][0] += dot2(a0, b0);
         }
     }
 #pragma unroll
@@ -964,7 +964,7 @@ void dirichlet_amp_multi(hipStream_t st, const double* Mp, const double* Mm, int
 
 // Neumann surface step: df = A+ v+ + A- v-, ds = B+ v+ + B- v-;
 // primal amplitudes [df; v+ - v-], dual amplitudes [df; -ds]
-__global__ void k_neu_amp(const double* __restrict__ Ap, const double* __restrict__ Am, const double* __restrict__ Bp,
+__global__ void __launch_bounds__(256) k_neu_amp(const double* __restrict__ Ap, const double* __restrict__ Am, const double* __restrict__ Bp,
                           const double* __restrict__ Bm, int ld, const double* __restrict__ vp,
                           const double* __restrict__ vm, double* __restrict__ af, double* __restrict__ as,
                           double* __restrict__ df, double* __restrict__ dsv, int n)

@@ -413,6 +413,10 @@ void lu_solve_multi(ilh_plan* pl, const double* LU, const int* piv, int n, doubl
 // row-major solve matrices use a leading dimension padded to a multiple of 32
 inline int solve_ld(int n) { return (n + 31) / 32 * 32; }
 void transpose(hipStream_t st, const double* A, double* At, int rows, int cols, int ldt);
+// a.x b.x + a.y b.y with one explicit fused multiply-add, so that the rounding
+// is the same wherever the compiler places it
+__device__ __forceinline__ double dot2(double2 a, double2 b) { return __builtin_fma(a.x, b.x, a.y * b.y); }
+
 // dot(row, x) over m entries by one wave of 64 lanes (16-byte loads when x is
 // 16-byte aligned), the same fixed order on every call
 __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const double* __restrict__ x, int m, int lane)
@@ -423,23 +427,54 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
         return s;
     }
-    // 16-byte loads, four independent partial sums per lane
+    // 16-byte loads, four independent partial sums per lane: lane l adds
+    // entries l + 256 j + 64 u into s_u while l + 256 j + 192 < m/2, then the
+    // remaining l + 256 J + 64 u into s0.  Loads go out four of those rounds
+    // (32 loads) at a time, and the remainder's with the last of them, all
+    // before the first is used: a row of a few thousand entries is then one
+    // or two memory round trips instead of one per round.
     const double2* r2 = (const double2*)row;
     const double2* x2 = (const double2*)x;
     const int m2 = m >> 1;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int k = lane;
-    for (; k + 192 < m2; k += 256) {
-        const double2 a0 = r2[k], a1 = r2[k + 64], a2 = r2[k + 128], a3 = r2[k + 192];
-        const double2 b0 = x2[k], b1 = x2[k + 64], b2 = x2[k + 128], b3 = x2[k + 192];
-        s0 += a0.x * b0.x + a0.y * b0.y;
-        s1 += a1.x * b1.x + a1.y * b1.y;
-        s2 += a2.x * b2.x + a2.y * b2.y;
-        s3 += a3.x * b3.x + a3.y * b3.y;
-    }
-    for (; k < m2; k += 64) {
-        const double2 a0 = r2[k], b0 = x2[k];
-        s0 += a0.x * b0.x + a0.y * b0.y;
+    for (;;) {
+        double2 a[16], b[16], ra[3], rb[3];
+        // rounds j < 4 of this chunk, and where the remainder starts if the
+        // chunk holds the last round
+        int nr = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nr += k + 256 * j + 192 < m2 ? 1 : 0;
+        const int kr = k + 256 * nr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = j < nr;
+                a[4 * j + u] = ok ? r2[k + 256 * j + 64 * u] : make_double2(0.0, 0.0);
+                b[4 * j + u] = ok ? x2[k + 256 * j + 64 * u] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const bool ok = nr < 4 && kr + 64 * u < m2;
+            ra[u] = ok ? r2[kr + 64 * u] : make_double2(0.0, 0.0);
+            rb[u] = ok ? x2[kr + 64 * u] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j < nr) {
+                s0 += dot2(a[4 * j], b[4 * j]);
+                s1 += dot2(a[4 * j + 1], b[4 * j + 1]);
+                s2 += dot2(a[4 * j + 2], b[4 * j + 2]);
+                s3 += dot2(a[4 * j + 3], b[4 * j + 3]);
+            }
+        k = kr;
+        if (nr < 4) {
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+                if (kr + 64 * u < m2) s0 += dot2(ra[u], rb[u]);
+            break;
+        }
     }
     if ((m & 1) && lane == 0) s1 += row[m - 1] * x[m - 1];
     double s = (s0 + s1) + (s2 + s3);
