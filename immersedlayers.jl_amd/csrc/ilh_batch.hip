@@ -107,8 +107,8 @@ __global__ void __launch_bounds__(256) k_batch_amp(const BatchProb* __restrict__
         const int n = B.n;
         const int i = (int)(code & 0xFFFF) * 4 + (tid >> 6), lane = tid & 63;
         if (i >= n) return;
-        double s = wave_dot(B.Mp + (size_t)B.ld * i, B.fp, n, lane);
-        if (B.fm) s += wave_dot(B.Mm + (size_t)B.ld * i, B.fm, n, lane);
+        double s = wave_dot<1>(B.Mp + (size_t)B.ld * i, B.fp, n, lane);
+        if (B.fm) s += wave_dot<1>(B.Mm + (size_t)B.ld * i, B.fm, n, lane);
         if (lane == 0) {
             B.amp[i] = s;
             B.amp[n + i] = B.fp[i] - (B.fm ? B.fm[i] : 0.0);

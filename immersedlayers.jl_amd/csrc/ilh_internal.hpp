@@ -419,6 +419,7 @@ __device__ __forceinline__ double dot2(double2 a, double2 b) { return __builtin_
 
 // dot(row, x) over m entries by one wave of 64 lanes (16-byte loads when x is
 // 16-byte aligned), the same fixed order on every call
+template <int CH = 4>
 __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const double* __restrict__ x, int m, int lane)
 {
     if (((uintptr_t)x & 15) != 0) {   // caller's vector not 16-byte aligned: plain loads
@@ -429,25 +430,27 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const
     }
     // 16-byte loads, four independent partial sums per lane: lane l adds
     // entries l + 256 j + 64 u into s_u while l + 256 j + 192 < m/2, then the
-    // remaining l + 256 J + 64 u into s0.  Loads go out four of those rounds
-    // (32 loads) at a time, and the remainder's with the last of them, all
-    // before the first is used: a row of a few thousand entries is then one
-    // or two memory round trips instead of one per round.
+    // remaining l + 256 J + 64 u into s0.  Loads go out CH of those rounds
+    // (8 CH loads) at a time, and the remainder's with the last of them, all
+    // before the first is used: with CH = 4 a row of a few thousand entries is
+    // one or two memory round trips.  CH only sets how many loads are in
+    // flight (and the registers they hold: kernels with many waves per SIMD
+    // use CH = 1); every CH sums in the same order.
     const double2* r2 = (const double2*)row;
     const double2* x2 = (const double2*)x;
     const int m2 = m >> 1;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int k = lane;
     for (;;) {
-        double2 a[16], b[16], ra[3], rb[3];
+        double2 a[4 * CH], b[4 * CH], ra[3], rb[3];
         // rounds j < 4 of this chunk, and where the remainder starts if the
         // chunk holds the last round
         int nr = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nr += k + 256 * j + 192 < m2 ? 1 : 0;
+        for (int j = 0; j < CH; ++j) nr += k + 256 * j + 192 < m2 ? 1 : 0;
         const int kr = k + 256 * nr;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CH; ++j)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const bool ok = j < nr;
@@ -456,12 +459,12 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const
             }
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
-            const bool ok = nr < 4 && kr + 64 * u < m2;
+            const bool ok = nr < CH && kr + 64 * u < m2;
             ra[u] = ok ? r2[kr + 64 * u] : make_double2(0.0, 0.0);
             rb[u] = ok ? x2[kr + 64 * u] : make_double2(0.0, 0.0);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CH; ++j)
             if (j < nr) {
                 s0 += dot2(a[4 * j], b[4 * j]);
                 s1 += dot2(a[4 * j + 1], b[4 * j + 1]);
@@ -469,7 +472,7 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const
                 s3 += dot2(a[4 * j + 3], b[4 * j + 3]);
             }
         k = kr;
-        if (nr < 4) {
+        if (nr < CH) {
 #pragma unroll
             for (int u = 0; u < 3; ++u)
                 if (kr + 64 * u < m2) s0 += dot2(ra[u], rb[u]);
