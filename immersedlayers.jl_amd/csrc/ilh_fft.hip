@@ -516,6 +516,11 @@ static RowSrc terms_source(const ConvPlan& C, const SourceSet& src, const double
     S.amp = amp;
     S.amp_len = src.amp_len;
     S.npairs_dev = src.npairs_dev;
+    if (src.inl_stride > 0 && !src.npairs_dev) {
+        S.inl_ai = src.inl_ai;
+        S.inl_w = src.inl_w;
+        S.inl_stride = src.inl_stride;
+    }
     return S;
 }
 

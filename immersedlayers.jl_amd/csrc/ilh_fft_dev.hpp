@@ -280,6 +280,9 @@ struct RowSrc {
     int amp_lds;           // 1: stage amp[0..amp_len) in LDS after the transform buffers
     const int* npairs_dev; // surface sources: the live pair count on the device (the
                            // launch covers an upper bound), or null
+    const int* inl_ai;     // surface sources: first contributions of each cell by slot
+    const double* inl_w;   // (SourceSet::inl_ai), or null
+    int inl_stride;
 };
 
 // values of two cells (a second with c0 == c1 is empty): each cell's
@@ -313,7 +316,55 @@ __device__ __forceinline__ void cell_values(const RowSrc& S, const double* a_lds
         for (int h = 0; h < 2; ++h) {
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                if (e[h] + u < end[h]) acc[h] += w[h][u] * av[h][u];
+                if (e[h] + u < end[h]) acc[h] = __builtin_fma(w[h][u], av[h][u], acc[h]);
+            e[h] += 8;
+        }
+    }
+    va = acc[0];
+    vb = acc[1];
+}
+
+// cell_values with each cell's first 8 contributions already loaded (ia, wa:
+// the inline copies, 0 past a cell's count): the same sums in the same order
+template <bool LDS_AMP>
+__device__ __forceinline__ void cell_values_inl(const RowSrc& S, const double* a_lds, int4 ca, int4 cb,
+                                                const int (&ia)[2][8], const double (&wa)[2][8], double& va,
+                                                double& vb)
+{
+    double acc[2] = {0.0, 0.0};
+    const int c0[2] = {ca.y, cb.y};
+    const int end[2] = {ca.z, cb.z};
+    double av[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) av[h][u] = LDS_AMP ? a_lds[ia[h][u]] : S.amp[ia[h][u]];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0[h] + u < end[h]) acc[h] = __builtin_fma(wa[h][u], av[h][u], acc[h]);
+    int e[2] = {c0[0] + 8, c0[1] + 8};
+    while (e[0] < end[0] || e[1] < end[1]) {
+        int a[2][8];
+        double w[2][8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool ok = e[h] + u < end[h];
+                a[h][u] = ok ? S.cai[e[h] + u] : 0;
+                w[h][u] = ok ? S.cw[e[h] + u] : 0.0;
+            }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) av[h][u] = LDS_AMP ? a_lds[a[h][u]] : S.amp[a[h][u]];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e[h] + u < end[h]) acc[h] = __builtin_fma(w[h][u], av[h][u], acc[h]);
             e[h] += 8;
         }
     }
@@ -372,10 +423,25 @@ __device__ __forceinline__ void rows_fwd_body(RowSrc S, const typename Cplx<R>::
         // LDS: zeros, then the rows' nonzero cells
         double* a_lds = (double*)(lds + (size_t)NB * lds_len(P));   // 16-byte aligned (NB * P slots)
         const int4 pi = live ? S.pinfo[p] : make_int4(0, 0, 0, 0);
-        // the first round of cells, in flight across the amplitude staging
+        // the first round of cells (and, with the inline copies, their first
+        // contributions), in flight across the amplitude staging
         const int ci0 = pi.y + t;
         const int4 ca0 = ci0 < pi.w ? S.cells[ci0] : make_int4(0, 0, 0, 0);
         const int4 cb0 = ci0 + T < pi.w ? S.cells[ci0 + T] : make_int4(0, 0, 0, 0);
+        int ia0[2][8];
+        double wa0[2][8];
+        if (S.inl_ai) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int ci = ci0 + h * T;
+                const bool ok = ci < pi.w;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    ia0[h][u] = ok ? S.inl_ai[(size_t)u * S.inl_stride + ci] : 0;
+                    wa0[h][u] = ok ? S.inl_w[(size_t)u * S.inl_stride + ci] : 0.0;
+                }
+            }
+        }
         if (S.amp_lds) {
             // 16 loads in flight per thread, then the LDS stores
             for (int base = 0; base < S.amp_len; base += 16 * (int)blockDim.x) {
@@ -398,8 +464,14 @@ __device__ __forceinline__ void rows_fwd_body(RowSrc S, const typename Cplx<R>::
             const int4 ca = ci == ci0 ? ca0 : S.cells[ci];
             const int4 cb = ci == ci0 ? cb0 : (ci + T < pi.w ? S.cells[ci + T] : make_int4(0, 0, 0, 0));
             double va, vb;
-            if (S.amp_lds) cell_values<true>(S, a_lds, ca, cb, va, vb);
-            else cell_values<false>(S, a_lds, ca, cb, va, vb);
+            if (S.inl_ai && ci == ci0) {
+                if (S.amp_lds) cell_values_inl<true>(S, a_lds, ca, cb, ia0, wa0, va, vb);
+                else cell_values_inl<false>(S, a_lds, ca, cb, ia0, wa0, va, vb);
+            } else if (S.amp_lds) {
+                cell_values<true>(S, a_lds, ca, cb, va, vb);
+            } else {
+                cell_values<false>(S, a_lds, ca, cb, va, vb);
+            }
             if (ci < pi.z) x[lx(ca.x)].x = (R)va;
             else x[lx(ca.x)].y = (R)va;
             if (ci + T < pi.w) {

@@ -284,7 +284,17 @@ struct SourceSet {
     int npairs = 0;            // upper bound of the live row pairs ((ny + 1) / 2)
     int* npairs_dev = nullptr; // device: live row pairs (after slot_of_row's rows)
     size_t cap_pairs = 0, cap_rows = 0;   // allocated sizes of pinfo and slot_of_row
+    // The first INL_K contributions of every cell, copied by slot (built by
+    // pin_source_pairs; null otherwise): inl_ai[u inl_stride + c] and
+    // inl_w[u inl_stride + c] are bins.cai / bins.cw[c0 + u] of cell c (0 past
+    // the cell's count).  The row pass loads them with the cell itself instead
+    // of after it.
+    int* inl_ai = nullptr;
+    double* inl_w = nullptr;
+    int inl_stride = 0;
+    size_t inl_cap = 0;        // cells the inline copies have room for
 };
+constexpr int INL_K = 8;
 void free_sources(SourceSet& s);   // releases the bins and pair tables
 void build_source_pairs(ilh_plan* pl, SourceSet& s);
 // the merged sources [op0; op1] of many plans (and the terms of extra_op, or

@@ -724,6 +724,7 @@ void pairs_prepare(SourceSet& s)
     }
     s.npairs = (int)np;
     s.npairs_dev = s.slot_of_row + ny;
+    s.inl_stride = 0;   // the inline copies describe pinned cells only
 }
 }  // namespace
 
@@ -779,6 +780,7 @@ void free_sources(SourceSet& s)
     free_bins(s.bins);
     if (s.pinfo) (void)hipFree(s.pinfo);
     if (s.slot_of_row) (void)hipFree(s.slot_of_row);
+    if (s.inl_ai) (void)hipFree(s.inl_ai);
     s = SourceSet();
 }
 
@@ -850,14 +852,52 @@ void build_source_pairs(ilh_plan* pl, SourceSet& s)
 
 // The exact live pair count on the host (one read-back; for caches built
 // once and used by many solves): the row pass then launches exactly.
+// the first INL_K contributions of cells [0, ncells), by slot (layout of
+// SourceSet::inl_ai)
+__global__ void k_inline_contribs(const int4* __restrict__ cells, const int* __restrict__ cai,
+                                  const double* __restrict__ cw, int ncells, int stride, int* __restrict__ ia,
+                                  double* __restrict__ wa)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int4 cl = cells[c];
+#pragma unroll
+    for (int u = 0; u < INL_K; ++u) {
+        const bool ok = cl.y + u < cl.z;
+        ia[(size_t)u * stride + c] = ok ? cai[cl.y + u] : 0;
+        wa[(size_t)u * stride + c] = ok ? cw[cl.y + u] : 0.0;
+    }
+}
+
 void pin_source_pairs(ilh_plan* pl, SourceSet& s)
 {
     if (!s.npairs_dev) return;
-    int n = 0;
-    ILH_HIP(hipMemcpyAsync(&n, s.npairs_dev, sizeof(int), hipMemcpyDeviceToHost, pl->ctx->stream));
-    ILH_HIP(hipStreamSynchronize(pl->ctx->stream));
+    hipStream_t st = pl->ctx->stream;
+    int n = 0, ncells = 0;
+    ILH_HIP(hipMemcpyAsync(&n, s.npairs_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+    ILH_HIP(hipMemcpyAsync(&ncells, s.bins.cell_ptr + s.bins.ny, sizeof(int), hipMemcpyDeviceToHost, st));
+    ILH_HIP(hipStreamSynchronize(st));
     s.npairs = n;
     s.npairs_dev = nullptr;
+    // the cells' first contributions copied by slot, for the row pass
+    s.inl_stride = 0;
+    if (ncells <= 0) return;
+    const size_t cap = ((size_t)ncells + 255) / 256 * 256;
+    if (cap > s.inl_cap) {
+        if (s.inl_ai) (void)hipFree(s.inl_ai);
+        s.inl_ai = nullptr;
+        s.inl_w = nullptr;
+        s.inl_cap = 0;
+        void* p = nullptr;
+        ILH_HIP(hipMalloc(&p, (sizeof(int) + sizeof(double)) * INL_K * cap));
+        s.inl_ai = (int*)p;
+        s.inl_w = (double*)((char*)p + sizeof(int) * INL_K * cap);
+        s.inl_cap = cap;
+    }
+    k_inline_contribs<<<(ncells + 255) / 256, 256, 0, st>>>(s.bins.cells, s.bins.cai, s.bins.cw, ncells,
+                                                            (int)s.inl_cap, s.inl_ai, s.inl_w);
+    ILH_HIP(hipGetLastError());
+    s.inl_stride = (int)s.inl_cap;
 }
 
 
