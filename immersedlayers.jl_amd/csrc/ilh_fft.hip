@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_fwd(RowSrc S, const typename
                             threadIdx.x);
 }
 
-template <int P, int MODE, class R, class RI>
+template <int P, int MODE, class R, class RI, int TR>
 __global__ void __launch_bounds__(tpb_of(P)) __attribute__((amdgpu_waves_per_eu(4)))
 k_cols(const typename Cplx<R>::C2* __restrict__ T1, const int* __restrict__ slot_of_row,
        const typename Cplx<R>::C2* __restrict__ tw, const typename Cplx<RI>::C2* __restrict__ tw_inv, int H, int ny,
@@ -157,11 +157,11 @@ k_cols(const typename Cplx<R>::C2* __restrict__ T1, const int* __restrict__ slot
     extern __shared__ double2 lds_raw[];
     T1 += t1_stride * blockIdx.y;
     T2 += t2_stride * blockIdx.y;
-    cols_body<P, tpb_of(P) / (P / V), MODE, R, RI>(T1, slot_of_row, tw, tw_inv, H, ny, nrows_in, spec, scale, T2,
+    cols_body<P, tpb_of(P) / (P / V), MODE, R, RI, TR>(T1, slot_of_row, tw, tw_inv, H, ny, nrows_in, spec, scale, T2,
                                                     spec_out, xcd_block(), (C2*)lds_raw, threadIdx.x);
 }
 
-template <int P, class TO, class R>
+template <int P, class TO, class R, int TR>
 __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const typename Cplx<R>::C2* __restrict__ T2,
                                                         const typename Cplx<R>::C2* __restrict__ tw, int nx, int ny,
                                                         TO* __restrict__ out, size_t t2_stride, size_t out_stride)
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(tpb_of(P)) k_rows_inv(const typename Cplx<R>::
     extern __shared__ double2 lds_raw[];
     T2 += t2_stride * blockIdx.y;
     out += out_stride * blockIdx.y;
-    rows_inv_body<P, tpb_of(P) / (P / V), TO, R>(T2, tw, nx, ny, out, xcd_block(), (C2*)lds_raw, threadIdx.x);
+    rows_inv_body<P, tpb_of(P) / (P / V), TO, R, TR>(T2, tw, nx, ny, out, xcd_block(), (C2*)lds_raw, threadIdx.x);
 }
 
 // ------------------------------------------------------------- launches
@@ -234,8 +234,11 @@ static void launch_cols_p(ilh_ctx* ctx, const C2* T1, const int* slot_of_row, in
     using RI = decltype(C2I().x);
     constexpr int NB = tpb_of(P) / (P / V);
     const size_t lds = sizeof(C2) * (size_t)NB * lds_len(P);
-    set_lds((const void*)k_cols<P, MODE, R, RI>, lds);
-    k_cols<P, MODE, R, RI><<<dim3((H + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+    // one data set: 4-row T2 tiles with streaming stores; several: 2-row tiles
+    // (the inverse row pass reads the same layout: launch_rows_inv)
+    auto kern = bt.nb == 1 ? k_cols<P, MODE, R, RI, 4> : k_cols<P, MODE, R, RI, 2>;
+    set_lds((const void*)kern, lds);
+    kern<<<dim3((H + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
         T1, slot_of_row, twiddles<R>(ctx, P), twiddles<RI>(ctx, P), H, ny, nrows_in, spec, scale, T2, spec_out,
         bt.in, bt.out);
     ILH_HIP(hipGetLastError());
@@ -258,8 +261,9 @@ static void launch_rows_inv_p(ilh_ctx* ctx, const C2* T2, int nx, int ny, TO* ou
     constexpr int NB = tpb_of(P) / (P / V);
     const int npairs = (ny + 1) / 2;
     const size_t lds = sizeof(C2) * (size_t)NB * lds_len(P);
-    set_lds((const void*)k_rows_inv<P, TO, R>, lds);
-    k_rows_inv<P, TO, R><<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
+    auto kern = bt.nb == 1 ? k_rows_inv<P, TO, R, 4> : k_rows_inv<P, TO, R, 2>;   // launch_cols' T2 layout
+    set_lds((const void*)kern, lds);
+    kern<<<dim3((npairs + NB - 1) / NB, bt.nb), tpb_of(P), lds, ctx->stream>>>(
         T2, twiddles<R>(ctx, P), nx, ny, out, bt.in, bt.out);
     ILH_HIP(hipGetLastError());
 }

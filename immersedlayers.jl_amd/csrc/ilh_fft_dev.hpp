@@ -386,16 +386,19 @@ __device__ __forceinline__ size_t tidx(int row, int c, int H)
 {
     return (size_t)(row >> 3) * 8 * H + (size_t)c * 8 + (row & 7);
 }
-// The column pass output uses 4-row tiles instead: a row pair's value at c is
-// 32 contiguous bytes (the inverse row pass's 32-byte loads, two columns per
-// 128-byte line), and a column's run of 4 rows is 64 contiguous bytes (the
-// column pass writes half lines, with streaming stores).  Measured on the
-// 1024^2 convolution against 2-row tiles (whole lines for the inverse row
-// pass, 32-byte pieces for the column pass) and 8-row tiles (the reverse):
-// 37.9 us per solve against 38.7 and 40.4.
-__device__ __forceinline__ size_t pidx(int row, int c, int H)
+// The column pass output uses TR-row tiles instead (TR = 2 or 4): a row
+// pair's value at c is 32 contiguous bytes (the inverse row pass's loads).
+// TR = 2: consecutive c adjacent, so the inverse row pass reads whole lines
+// and the column pass writes 32-byte pieces.  TR = 4: a column's run of 4
+// rows is 64 bytes, written with streaming stores, and each line holds two
+// columns.  One convolution at a time (1024^2) runs 37.9 us with TR = 4
+// against 38.7 (TR = 2) and 40.4 (8-row tiles); launches over many data sets
+// or problems, whose inverse row pass dominates, keep TR = 2 (8 data sets:
+// 41k against 38.6k solves/s; the 64-problem batch: 77.5k against 75.4k).
+template <int TR> __device__ __forceinline__ size_t pidx(int row, int c, int H)
 {
-    return (size_t)(row >> 2) * 4 * H + (size_t)c * 4 + (row & 3);
+    static_assert(TR == 2 || TR == 4, "T2 tile rows");
+    return (size_t)(row / TR) * TR * H + (size_t)c * TR + (row % TR);
 }
 
 // ------------------------------------------------------------- pass 1
@@ -567,13 +570,13 @@ __device__ __forceinline__ void rows_fwd_body(RowSrc S, const typename Cplx<R>::
 // or slot = r when slot_of_row is null and r < nrows_in.  The LGF spectrum is
 // stored for ky <= P/2 (even in ky) as spec[c][ky], (P/2+1) per column, for
 // kx = c < H and kx = H (the Nyquist column, at c = H).
-// MODE 0: convolve, write rows 0..ny-1 of T2 (4-row tiles).  MODE 1: forward only, write
+// MODE 0: convolve, write rows 0..ny-1 of T2 (TR-row tiles).  MODE 1: forward only, write
 // the spectrum * scale (the LGF spectrum build).
 // R: the forward transform and the spectrum product; RI: the inverse
 // transform and T2 (RI = float after R = double: the split-precision
 // convolution of the mixed-precision solve).
 // Logical block blk: columns c0 = blk NB .. c0 + NB - 1.
-template <int P, int NB, int MODE, class R, class RI>
+template <int P, int NB, int MODE, class R, class RI, int TR = 2>
 __device__ __forceinline__ void cols_body(const typename Cplx<R>::C2* __restrict__ T1,
                                           const int* __restrict__ slot_of_row,
                                           const typename Cplx<R>::C2* __restrict__ tw,
@@ -663,15 +666,19 @@ __device__ __forceinline__ void cols_body(const typename Cplx<R>::C2* __restrict
         for (int q = 0; q < V; ++q) vi[q] = mk2<C2I>((RI)v[q].x, (RI)v[q].y);
         fft<P, 1, true>(vi, (C2I*)x, t, tw_inv);
         if (!live) return;
-        // T2 is read once, by the next pass: streaming stores keep it out of
-        // the L2 write-back at the kernel's end
+        // TR = 4: T2 is read once, by the next pass: streaming stores keep it
+        // out of the L2 write-back at the kernel's end
         typedef RI v2 __attribute__((ext_vector_type(2)));
 #pragma unroll
         for (int q = 0; q < V; ++q) {
             const int row = io_index(RO, q / RO, q % RO, t, T, P);
             if (row < ny) {
-                const v2 tv = {vi[q].x, vi[q].y};
-                __builtin_nontemporal_store(tv, (v2*)&T2[pidx(row, c, H)]);
+                if constexpr (TR == 4) {
+                    const v2 tv = {vi[q].x, vi[q].y};
+                    __builtin_nontemporal_store(tv, (v2*)&T2[pidx<TR>(row, c, H)]);
+                } else {
+                    T2[pidx<TR>(row, c, H)] = vi[q];
+                }
             }
         }
     }
@@ -680,7 +687,7 @@ __device__ __forceinline__ void cols_body(const typename Cplx<R>::C2* __restrict
 // ------------------------------------------------------------- pass 3
 // Transform c of logical block blk makes output rows 2p, 2p+1 (p = blk NB +
 // c), FP64 or (a Float32 cache) rounded to FP32.
-template <int P, int NB, class TO, class R>
+template <int P, int NB, class TO, class R, int TR = 2>
 __device__ __forceinline__ void rows_inv_body(const typename Cplx<R>::C2* __restrict__ T2,
                                               const typename Cplx<R>::C2* __restrict__ tw, int nx, int ny,
                                               TO* __restrict__ out, int blk, typename Cplx<R>::C2* lds, int tid)
@@ -702,7 +709,7 @@ __device__ __forceinline__ void rows_inv_body(const typename Cplx<R>::C2* __rest
     for (int q = 0; q < V; ++q) {
         const int m = io_index(R0, q / R0, q % R0, t, T, P);
         const int k = m <= P / 2 ? m : P - m;
-        if (live) abq[q] = *(const Q4*)(T2 + pidx(j0, k == H ? 0 : k, H));
+        if (live) abq[q] = *(const Q4*)(T2 + pidx<TR>(j0, k == H ? 0 : k, H));
         else abq[q].x = abq[q].y = abq[q].z = abq[q].w = 0;
     }
 #pragma unroll
