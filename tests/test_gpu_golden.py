@@ -1,10 +1,11 @@
 """GPU tests against the golden fixtures, the reference's published Neumann
 added mass, headline-size parity, determinism and error paths.
 
-Tolerances: FP64 solves agree with the oracle to relative L2 <= 1e-10.  The
-surface strength s of the 1024^2 workload is the solution of a Schur system
-with condition number ~3e5, so two independent FP64 computations (different
-summation orders) agree to ~kappa*eps; its bar is 1e-9.
+Tolerances: FP64 solves agree with the oracle to relative L2 <= 1e-10,
+including the surface strength s of the 1024^2 headline (a Schur system of
+condition number ~3e5: two FP64 computations in different summation orders
+agree to ~kappa*eps; observed 6.6e-12).  Two larger or worse-conditioned
+systems below state their own bars and why.
 """
 import os
 
@@ -99,7 +100,7 @@ def test_headline_parity_and_determinism(ctx):
     geo = O.Geometry(s.X, s.Y, s.nx, s.ny, s.ds)
     fr, sr = O.dirichlet_solve(og, geo, fb)
     assert rel_err(f1.flat(), fr) < 1e-10
-    assert rel_err(s1.flat(), sr) < 1e-9
+    assert rel_err(s1.flat(), sr) < 1e-10
 
 
 def test_dirichlet_many_points_reads_amplitudes_from_memory(ctx):
