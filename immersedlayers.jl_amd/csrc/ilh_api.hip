@@ -3019,11 +3019,23 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 mc[i]->band_ok = true;
             }
         const hipStream_t main = ctx->stream;
+        bool joined = false;
+        // on any exit: the context stream back, and (if the step ended before
+        // its join) the side streams' work ordered before later work on it
         struct Restore {
             ilh_ctx* c;
             hipStream_t m;
-            ~Restore() { c->stream = m; }
-        } restore{ctx, main};
+            const bool* joined;
+            ~Restore()
+            {
+                c->stream = m;
+                if (!*joined)
+                    for (int k = 0; k < ilh_ctx::POOL; ++k) {
+                        (void)hipEventRecord(c->ev[k], c->pool[k]);
+                        (void)hipStreamWaitEvent(m, c->ev[k], 0);
+                    }
+            }
+        } restore{ctx, main, &joined};
         // fork the side streams from / join them into the context stream
         auto fork = [&] {
             ILH_HIP(hipEventRecord(ctx->ev[ilh_ctx::POOL], main));
@@ -3036,6 +3048,7 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 ILH_HIP(hipEventRecord(ctx->ev[k], ctx->pool[k]));
                 ILH_HIP(hipStreamWaitEvent(main, ctx->ev[k], 0));
             }
+            joined = true;
         };
         // the merged sources [R, D_s] (and E's terms, for S) of every plan whose
         // geometry changed, built together on the context stream
@@ -3044,6 +3057,11 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             if (mc[i] && !mc[i]->src_ok) sj.push_back(SourceJob{plans[i], &mc[i]->terms1, &mc[i]->src1});
         merge_sources_batched(ctx, sj, OP_R_PRIMAL, OP_DS, OP_E_PRIMAL, GK_PRIMAL);
         for (const SourceJob& j : sj) j.pl->mov->src_ok = true;
+        // every plan's LGF table and spectrum exist before the fork: the side
+        // streams' assemblies read the table, and the first convolution runs
+        // on the context stream beside them
+        for (int i = 0; i < nprob; ++i)
+            if (mc[i]) (void)get_conv(plans[i], GK_PRIMAL);
         fork();
         auto on = [&](int i) { ctx->stream = ctx->pool[i % ilh_ctx::POOL]; };
         std::vector<int> state(nprob, 0), it(nprob, 0);
@@ -3064,7 +3082,10 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             grid_out.push_back(mc[i]->grid.as<double>());
             f_out.push_back(f[i]);
         }
-        join();
+        // the side streams (S and the learned band of every plan) are joined
+        // where the step first needs them, before the preconditioner: the
+        // first convolution and the right-hand side run beside them
+        ctx->stream = main;
         // every plan's step around the GMRES steps as batched launches over all
         // plans on the context stream; the GMRES steps in lockstep, one launch
         // per phase, one flag read-back per chunk of steps
@@ -3143,6 +3164,7 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 for (size_t k = 0; k < live_pl.size(); ++k)
                     conv_sources(live_pl[k], GK_PRIMAL, live_mc[k]->src1, live_mc[k]->amp.as<double>(), grid_out[k]);
             k_bgm_rhs<<<dim3((unsigned)((maxn + 3) / 4), (unsigned)nd), 256, 0, main>>>(dg);
+            join();
             k_bgm_prec_xb<<<nt, 256, 0, main>>>(dg, dt, 0);
             k_bgm_resid<<<nt, 256, 0, main>>>(dg, dt);
             k_bgm_init<<<(unsigned)nd, 1024, 0, main>>>(dg, m, tol);
@@ -3193,6 +3215,7 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             for (int i = 0; i < nprob; ++i)
                 if (mc[i]) ++mc[i]->ref_age;
         }
+        if (!joined) join();
         if (iters)
             for (int i = 0; i < nprob; ++i) iters[i] = it[i];
     });
