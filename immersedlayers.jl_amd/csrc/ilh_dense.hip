@@ -123,13 +123,17 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
         for (int ib = 0; ib < TBn; ++ib) {
             const Term& b = sb[qi * TBn + ib];
             if (a.gk != b.gk) continue;
+            // SYM kinds pair plain DDF windows, whose fifth tap is zero: the
+            // correlations' end entries are then exact zeros and are left out
+            // (the sums below skip only zero products, in the same order)
+            constexpr int NW = SYM ? 4 : 5, U0 = 5 - NW, U1 = 4 + NW;
             double cx[9], cy[9];
 #pragma unroll
             for (int u = 0; u < 9; ++u) { cx[u] = 0.0; cy[u] = 0.0; }
 #pragma unroll
-            for (int i = 0; i < 5; ++i)
+            for (int i = 0; i < NW; ++i)
 #pragma unroll
-                for (int k = 0; k < 5; ++k) {
+                for (int k = 0; k < NW; ++k) {
                     cx[i - k + 4] += a.wx[i] * b.wx[k];
                     cy[i - k + 4] += a.wy[i] * b.wy[k];
                 }
@@ -140,7 +144,7 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
                 const int u = -dbx + 4, v = -dby + 4;
                 if (u >= 0 && u < 9 && v >= 0 && v < 9) s = cx[u] * cy[v];
             } else {
-                for (int v = 0; v < 9; ++v) {
+                for (int v = U0; v < U1; ++v) {
                     const int jj = abs(dby + v - 4);
                     if (cy[v] == 0.0 || jj >= Tn) continue;   // kernel zero beyond the table
                     double r = 0.0;
@@ -151,14 +155,14 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
                         // land outside the LDS aperture.
                         const int row = bni * (jj - bj0) - bi0;
 #pragma unroll
-                        for (int u = 0; u < 9; ++u) {
+                        for (int u = U0; u < U1; ++u) {
                             const int ii = abs(dbx + u - 4);
                             if (ii < Tn) r += cx[u] * sG[row + ii];
                         }
                     } else {
                         const double* Gr = G + (size_t)Tn * jj;
 #pragma unroll
-                        for (int u = 0; u < 9; ++u) {
+                        for (int u = U0; u < U1; ++u) {
                             const int ii = abs(dbx + u - 4);
                             if (ii < Tn) r += cx[u] * Gr[ii];
                         }
