@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
 {
     __shared__ Term sa[AT * 2], sb[AT * 2];
     __shared__ double sG[GBOX];
-    __shared__ int sbox[4];
+    __shared__ int sbox[5];
     __shared__ double cAq[AT], cBp[AT];   // SYM: coefficients of the mirrored entries
     if (SYM && blockIdx.y < blockIdx.x) return;
     const int p0 = blockIdx.x * AT, q0 = blockIdx.y * AT;
@@ -84,32 +84,27 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
             by0 = min(by0, __shfl_xor(by0, off, 64)); by1 = max(by1, __shfl_xor(by1, off, 64));
         }
         if (lane == 0) {
-            // |d| over d in [lo, hi] with d = dbx + u - 4
-            auto absrange = [](int lo, int hi, int& r0, int& r1) {
-                if (lo >= 0) { r0 = lo; r1 = hi; }
-                else if (hi <= 0) { r0 = -hi; r1 = -lo; }
-                else { r0 = 0; r1 = max(-lo, hi); }
-            };
-            int i0, i1, j0, j1;
-            absrange(ax0 - bx1 - 4, ax1 - bx0 + 4, i0, i1);
-            absrange(ay0 - by1 - 4, ay1 - by0 + 4, j0, j1);
-            i1 = min(i1, Tn - 1);
-            j1 = min(j1, Tn - 1);
-            const int ni = i1 - i0 + 1, nj = j1 - j0 + 1;
-            const bool fits = ni > 0 && nj > 0 && (long)ni * nj <= GBOX;
-            sbox[0] = fits ? i0 : -1;
-            sbox[1] = j0;
-            sbox[2] = ni;
-            sbox[3] = nj;
+            // signed offsets d = dbx + u - 4 (and the same in y) the tile pair
+            // needs: the box [x0, x1] x [y0, y1], staged unfolded, with the
+            // (even) table mirrored into it and zero beyond its edge
+            const int x0 = ax0 - bx1 - 4, x1 = ax1 - bx0 + 4;
+            const int y0 = ay0 - by1 - 4, y1 = ay1 - by0 + 4;
+            const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+            const bool fits = bw > 0 && bh > 0 && (long)bw * bh <= GBOX;
+            sbox[0] = fits ? 1 : -1;
+            sbox[1] = x0;
+            sbox[2] = y0;
+            sbox[3] = bw;
+            sbox[4] = bh;
         }
     }
     __syncthreads();
-    const int bi0 = delta ? -1 : sbox[0];
-    const int bj0 = sbox[1], bni = sbox[2], bnj = sbox[3];
-    if (bi0 >= 0) {
-        for (int e = tid; e < bni * bnj; e += AT * AT) {
-            const int jj = bj0 + e / bni, ii = bi0 + e % bni;
-            sG[e] = G[(size_t)Tn * jj + ii];
+    const bool boxed = !delta && sbox[0] > 0;
+    const int bx0 = sbox[1], by0 = sbox[2], bw = sbox[3], bh = sbox[4];
+    if (boxed) {
+        for (int e = tid; e < bw * bh; e += AT * AT) {
+            const int ix = abs(bx0 + e % bw), iy = abs(by0 + e / bw);
+            sG[e] = (ix < Tn && iy < Tn) ? G[(size_t)Tn * iy + ix] : 0.0;
         }
     }
     __syncthreads();
@@ -148,17 +143,16 @@ __global__ void __launch_bounds__(AT * AT) k_assemble(const Term* __restrict__ T
                     const int jj = abs(dby + v - 4);
                     if (cy[v] == 0.0 || jj >= Tn) continue;   // kernel zero beyond the table
                     double r = 0.0;
-                    if (bi0 >= 0) {
-                        // staged box: every |dbx + u - 4| < Tn lies in [bi0, bi0 + bni), so
-                        // the index below is >= 0.  Only the integer index is offset:
-                        // a pointer before an LDS array, used as a generic pointer, can
-                        // land outside the LDS aperture.
-                        const int row = bni * (jj - bj0) - bi0;
+                    if (boxed) {
+                        // staged box: the row of offset dby + v - 4, from offset
+                        // dbx - 4 on, holds the table values the u loop needs
+                        // in order (zeros beyond the table add nothing).  Only
+                        // the integer index is offset: a pointer before an LDS
+                        // array, used as a generic pointer, can land outside
+                        // the LDS aperture.
+                        const int row = bw * (dby + v - 4 - by0) + (dbx - 4 - bx0);
 #pragma unroll
-                        for (int u = U0; u < U1; ++u) {
-                            const int ii = abs(dbx + u - 4);
-                            if (ii < Tn) r += cx[u] * sG[row + ii];
-                        }
+                        for (int u = U0; u < U1; ++u) r += cx[u] * sG[row + u];
                     } else {
                         const double* Gr = G + (size_t)Tn * jj;
 #pragma unroll
