@@ -132,6 +132,7 @@ struct MovingCache {
     DevBuf S;                  // create_RTLinvR at the current geometry, row-major
     bool S_ok = false;
     DevBuf gm, gmf;            // GMRES state and flags on the device
+    DevBuf arn;                // partial sums of an Arnoldi step
     int last_iters = 4;        // steps of the previous solve: the first chunk enqueued
     double* hpin = nullptr;    // pinned host buffer for the GMRES flags
     size_t hpin_n = 0;
@@ -2248,8 +2249,8 @@ __global__ void __launch_bounds__(1024) k_gm_init(const double* __restrict__ w, 
 }
 
 // y = M x (M row-major, one wave per row) unless the iteration has stopped
-__global__ void __launch_bounds__(256) k_gm_matvec(const double* __restrict__ M, int ld, const double* __restrict__ x, double* __restrict__ y,
-                            int n, const int* __restrict__ flags)
+__global__ void __launch_bounds__(256) k_gm_matvec(const double* __restrict__ M, int ld, const double* __restrict__ x,
+                                                   double* __restrict__ y, int n, const int* __restrict__ flags)
 {
     if (flags[0]) return;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -2261,159 +2262,136 @@ __global__ void __launch_bounds__(256) k_gm_matvec(const double* __restrict__ M,
 // dot(row, x) by one wave with the row in FP32 (the preconditioner), widened
 // to FP64: 16-byte row loads (4 values) when the row and x allow them, two in
 // flight per lane, fixed order
+// (double)a0 b.x + (double)a1 b.y with one explicit fused multiply-add, so
+// that the rounding is the same wherever the compiler places it
+__device__ __forceinline__ double dot2f(float a0, float a1, double2 b)
+{
+    return __builtin_fma((double)a0, b.x, (double)a1 * b.y);
+}
+
 __device__ __forceinline__ double wave_dot_f32(const float* __restrict__ row, const double* __restrict__ x, int m,
                                                int lane)
 {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int k0 = 0;
     if ((((uintptr_t)row | (uintptr_t)x) & 15) == 0) {
+        // lane l takes the float4 entries l + 64 r, round r even into s0, s1
+        // and odd into s2, s3, in increasing r; the loads of four rounds go
+        // out before the first is used (a row of a few thousand entries is
+        // two memory round trips)
         const float4* r4 = (const float4*)row;
         const double2* x2 = (const double2*)x;
         const int m4 = m >> 2;
-        int k = lane;
-        for (; k + 64 < m4; k += 128) {
-            const float4 a = r4[k], c = r4[k + 64];
-            const double2 b0 = x2[2 * k], b1 = x2[2 * k + 1], d0 = x2[2 * k + 128], d1 = x2[2 * k + 129];
-            s0 += (double)a.x * b0.x + (double)a.y * b0.y;
-            s1 += (double)a.z * b1.x + (double)a.w * b1.y;
-            s2 += (double)c.x * d0.x + (double)c.y * d0.y;
-            s3 += (double)c.z * d1.x + (double)c.w * d1.y;
-        }
-        if (k < m4) {
-            const float4 a = r4[k];
-            const double2 b0 = x2[2 * k], b1 = x2[2 * k + 1];
-            s0 += (double)a.x * b0.x + (double)a.y * b0.y;
-            s1 += (double)a.z * b1.x + (double)a.w * b1.y;
+        for (int k = lane; k < m4; k += 64 * 4) {
+            float4 a[4];
+            double2 b0[4], b1[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int e = k + 64 * r;
+                const bool ok = e < m4;
+                a[r] = ok ? r4[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                b0[r] = ok ? x2[2 * e] : make_double2(0.0, 0.0);
+                b1[r] = ok ? x2[2 * e + 1] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                if (k + 64 * r < m4) {
+                    s0 += dot2f(a[r].x, a[r].y, b0[r]);
+                    s1 += dot2f(a[r].z, a[r].w, b1[r]);
+                }
+                if (k + 64 * (r + 1) < m4) {
+                    s2 += dot2f(a[r + 1].x, a[r + 1].y, b0[r + 1]);
+                    s3 += dot2f(a[r + 1].z, a[r + 1].w, b1[r + 1]);
+                }
+            }
         }
         k0 = 4 * m4;
     }
-    for (int k = k0 + lane; k < m; k += 64) s0 += (double)row[k] * x[k];
+    for (int k = k0 + lane; k < m; k += 64) s0 = __builtin_fma((double)row[k], x[k], s0);
     double t = (s0 + s1) + (s2 + s3);
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     return t;
 }
 
-// y = M x (M in FP32, row-major), or y += M x when accumulate; stops once the
-// iteration has (flags non-null and set)
-__global__ void k_gm_matvec_f32(const float* __restrict__ M, int ld, const double* __restrict__ x,
-                                double* __restrict__ y, int n, const int* __restrict__ flags, int accumulate)
+// wave_dot_f32 of two FP32 rows with one x by one wave: x is loaded once for
+// both, and each row's sum is wave_dot_f32's, bit for bit (rows whose
+// alignment differs take wave_dot_f32 one by one)
+__device__ __forceinline__ double2 wave_dot_f32_rows2(const float* __restrict__ ra, const float* __restrict__ rb,
+                                                      const double* __restrict__ x, int m, int lane)
+{
+    const bool va = (((uintptr_t)ra | (uintptr_t)x) & 15) == 0, vb = (((uintptr_t)rb | (uintptr_t)x) & 15) == 0;
+    if (va != vb) return make_double2(wave_dot_f32(ra, x, m, lane), wave_dot_f32(rb, x, m, lane));
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    int k0 = 0;
+    if (va) {
+        const float4* a4 = (const float4*)ra;
+        const float4* c4 = (const float4*)rb;
+        const double2* x2 = (const double2*)x;
+        const int m4 = m >> 2;
+        for (int k = lane; k < m4; k += 64 * 4) {
+            float4 a[4], c[4];
+            double2 b0[4], b1[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int e = k + 64 * r;
+                const bool ok = e < m4;
+                a[r] = ok ? a4[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                c[r] = ok ? c4[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+                b0[r] = ok ? x2[2 * e] : make_double2(0.0, 0.0);
+                b1[r] = ok ? x2[2 * e + 1] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                if (k + 64 * r < m4) {
+                    s0 += dot2f(a[r].x, a[r].y, b0[r]);
+                    s1 += dot2f(a[r].z, a[r].w, b1[r]);
+                    t0 += dot2f(c[r].x, c[r].y, b0[r]);
+                    t1 += dot2f(c[r].z, c[r].w, b1[r]);
+                }
+                if (k + 64 * (r + 1) < m4) {
+                    s2 += dot2f(a[r + 1].x, a[r + 1].y, b0[r + 1]);
+                    s3 += dot2f(a[r + 1].z, a[r + 1].w, b1[r + 1]);
+                    t2 += dot2f(c[r + 1].x, c[r + 1].y, b0[r + 1]);
+                    t3 += dot2f(c[r + 1].z, c[r + 1].w, b1[r + 1]);
+                }
+            }
+        }
+        k0 = 4 * m4;
+    }
+    for (int k = k0 + lane; k < m; k += 64) {
+        s0 = __builtin_fma((double)ra[k], x[k], s0);
+        t0 = __builtin_fma((double)rb[k], x[k], t0);
+    }
+    double s = (s0 + s1) + (s2 + s3), t = (t0 + t1) + (t2 + t3);
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        t += __shfl_xor(t, off, 64);
+    }
+    return make_double2(s, t);
+}
+
+// y = M x (M in FP32, row-major, two rows per wave, four per work-group of
+// 128), or y += M x when accumulate; stops once the iteration has (flags
+// non-null and set)
+__global__ void __launch_bounds__(128) k_gm_matvec_f32(const float* __restrict__ M, int ld, const double* __restrict__ x,
+                                                       double* __restrict__ y, int n, const int* __restrict__ flags,
+                                                       int accumulate)
 {
     if (flags && flags[0]) return;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + 2 * (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
-    const double s = wave_dot_f32(M + (size_t)ld * i, x, n, lane);
-    if (lane == 0) y[i] = s + (accumulate ? y[i] : 0.0);
+    const bool two = i + 1 < n;
+    const double2 s = wave_dot_f32_rows2(M + (size_t)ld * i, M + (size_t)ld * (two ? i + 1 : i), x, n, lane);
+    if (lane == 0) {
+        y[i] = s.x + (accumulate ? y[i] : 0.0);
+        if (two) y[i + 1] = s.y + (accumulate ? y[i + 1] : 0.0);
+    }
 }
 
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)
 {
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
         b[k] = (float)a[k];
-}
-
-// Arnoldi step j (one work-group, unless the iteration has stopped): w is
-// orthogonalised against v_0..v_j (columns of V, stride ldv) by classical
-// Gram-Schmidt, repeated once when the first pass removed more than
-// 1 - 1/sqrt(2) of |w| (the Daniel-Gragg-Kaufman-Stewart test), and
-// normalised into v_{j+1}; column j of H gets the summed coefficients and
-// |w|; the previous Givens rotations are applied, a new one zeroes
-// H[j+1, j], and |g[j+1]| (the residual norm) is tested against tol |b|.
-// Every sum has a fixed order (deterministic).
-__device__ __forceinline__ void gm_arnoldi_body(double* __restrict__ V, int ldv, int j, double* __restrict__ w, int n,
-                                                double* __restrict__ st, int m, int* __restrict__ flags)
-{
-    __shared__ double h[2][256];
-    __shared__ double red[16];
-    __shared__ double rc[256], rs[256];   // the previous rotations, staged for thread 0
-    if (flags[0]) return;
-    const GmLayout L{m};
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nv = j + 1;
-    if (tid < j) {
-        rc[tid] = st[L.cs() + tid];
-        rs[tid] = st[L.sn() + tid];
-    }
-    __shared__ double wn0;                // |w|^2 before the first pass
-    double wn = 0.0;
-    for (int i = tid; i < nv; i += 1024) h[1][i] = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-        // dots: wave i % 16 owns basis vector i; in the first pass the wave
-        // after the last vector also takes w . w (the norm before the pass)
-        const int nd = pass == 0 ? nv + 1 : nv;
-        for (int i = wave; i < nd; i += 16) {
-            const double* v = i < nv ? V + (size_t)ldv * i : w;
-            double a0 = 0.0, a1 = 0.0;
-            int k = lane;
-            for (; k + 64 < n; k += 128) {
-                a0 += v[k] * w[k];
-                a1 += v[k + 64] * w[k + 64];
-            }
-            if (k < n) a0 += v[k] * w[k];
-            double acc = a0 + a1;
-            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-            if (lane == 0) {
-                if (i < nv) h[pass][i] = acc;
-                else wn0 = acc;
-            }
-        }
-        __syncthreads();
-        if (pass == 0) wn = sqrt(wn0);
-        // w -= V h, and |w| after the pass
-        double q = 0.0;
-        for (int k = tid; k < n; k += 1024) {
-            double acc = w[k];
-            for (int i = 0; i < nv; ++i) acc -= V[k + (size_t)ldv * i] * h[pass][i];
-            w[k] = acc;
-            q += acc * acc;
-        }
-        const double wn1 = sqrt(block_sum_1024(q, red));   // its barriers publish w
-        const bool again = pass == 0 && wn1 < 0.70710678118654752 * wn;
-        wn = wn1;
-        if (!again) break;
-    }
-    const double hn = wn;
-    if (hn > 0.0) {
-        double* vn = V + (size_t)ldv * nv;
-        const double inv = 1.0 / hn;
-        for (int k = tid; k < n; k += 1024) vn[k] = w[k] * inv;
-    }
-    if (tid == 0) {
-        // the column in LDS (h[0]), rotations from LDS: no dependent global
-        // loads in the serial part
-        double* col = h[0];
-        for (int i = 0; i <= j; ++i) col[i] += h[1][i];
-        col[j + 1] = hn;
-        for (int i = 0; i < j; ++i) {
-            const double t = rc[i] * col[i] + rs[i] * col[i + 1];
-            col[i + 1] = -rs[i] * col[i] + rc[i] * col[i + 1];
-            col[i] = t;
-        }
-        const double r = hypot(col[j], col[j + 1]);
-        const double cj = r == 0.0 ? 1.0 : col[j] / r, sj = r == 0.0 ? 0.0 : col[j + 1] / r;
-        col[j] = r;
-        col[j + 1] = 0.0;
-        double* H = st + L.H() + (size_t)(m + 1) * j;
-        for (int i = 0; i <= j + 1; ++i) H[i] = col[i];
-        double* g = st + L.g();
-        const double gj = g[j];
-        st[L.cs() + j] = cj;
-        st[L.sn() + j] = sj;
-        g[j + 1] = -sj * gj;
-        g[j] = cj * gj;
-        if (fabs(-sj * gj) <= st[L.tol()] * st[L.bnorm()] || hn == 0.0) {
-            flags[0] = 1;
-            flags[1] = j + 1;
-        } else if (j + 1 == m) {
-            flags[0] = 2;
-            flags[1] = m;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(1024) k_gm_arnoldi(double* __restrict__ V, int ldv, int j, double* __restrict__ w,
-                                                     int n, double* __restrict__ st, int m, int* __restrict__ flags)
-{
-    gm_arnoldi_body(V, ldv, j, w, n, st, m, flags);
 }
 
 // ---- the GMRES steps of many plans in lockstep, one launch per phase
@@ -2427,6 +2405,7 @@ struct BGm {
     int ld, n;
     double *V, *w, *z, *st;    // basis (stride ld), work vectors, GMRES state
     int* flags;
+    double* part;              // the Arnoldi step's partial sums (ArnLayout)
     // the step around the GMRES steps: boundary data, amplitudes [s; d]
     // (s the solution), fbar, the right-hand side b, the grid field f*, E's
     // terms on the primal grid (nx x ny), the strength out
@@ -2437,23 +2416,30 @@ struct BGm {
 };
 
 // z = P v_j for every plan still running; tiles (plan << 16 | 4-row block),
-// one wave per row, each row in the single solve's order (k_gm_matvec_f32,
-// k_band_apply)
-__global__ void __launch_bounds__(256) k_bgm_prec(const BGm* __restrict__ g, const unsigned* __restrict__ tiles, int j)
+// two rows per wave (work-groups of 128), each row in the single solve's order
+// (k_gm_matvec_f32, k_band_apply)
+__global__ void __launch_bounds__(128) k_bgm_prec(const BGm* __restrict__ g, const unsigned* __restrict__ tiles, int j)
 {
     const unsigned code = tiles[blockIdx.x];
     const BGm& G = g[code >> 16];
     if (G.flags[0]) return;
-    const int i = (int)(code & 0xFFFF) * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int i = (int)(code & 0xFFFF) * 4 + 2 * (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= G.n) return;
+    const bool two = i + 1 < G.n;
     const double* v = G.V + (size_t)G.ld * j;
     if (G.P32) {
-        const double s = wave_dot_f32(G.P32 + (size_t)G.ld * i, v, G.n, lane);
-        if (lane == 0) G.z[i] = s;
-    } else if (lane == 0) {
-        double s = 0.0;
-        for (int k = 0; k < G.bw; ++k) s += G.band[(size_t)i * G.bw + k] * v[G.nbr[(size_t)i * G.bw + k]];
-        G.z[i] = s;
+        const double2 s =
+            wave_dot_f32_rows2(G.P32 + (size_t)G.ld * i, G.P32 + (size_t)G.ld * (two ? i + 1 : i), v, G.n, lane);
+        if (lane == 0) {
+            G.z[i] = s.x;
+            if (two) G.z[i + 1] = s.y;
+        }
+    } else {
+        for (int r = 0; r < (two ? 2 : 1); ++r) {
+            const double s =
+                band_row(G.band + (size_t)(i + r) * G.bw, G.nbr + (size_t)(i + r) * G.bw, G.bw, v, lane);
+            if (lane == 0) G.z[i + r] = s;
+        }
     }
 }
 
@@ -2470,10 +2456,329 @@ __global__ void __launch_bounds__(256) k_bgm_matvec(const BGm* __restrict__ g, c
 }
 
 // Arnoldi step j of plan blockIdx.x
-__global__ void __launch_bounds__(1024) k_bgm_arnoldi(const BGm* __restrict__ g, int j, int m)
+// ---- Arnoldi step j, unless the iteration has stopped: w is orthogonalised
+// against v_0..v_j (columns of V, stride ld) by classical Gram-Schmidt,
+// repeated once when the first pass removed more than 1 - 1/sqrt(2) of |w|
+// (the Daniel-Gragg-Kaufman-Stewart test), and normalised into v_{j+1};
+// column j of H gets the summed coefficients and |w|; the previous Givens
+// rotations are applied, a new one zeroes H[j+1, j], and |g[j+1]| (the
+// residual norm) is tested against tol |b|.
+//
+// Each phase is a launch over slices of 64 entries (one work-group each), so
+// the basis streams through every CU's loads rather than one:
+//   k_arn_dots    slice dots v_i . w and w . w
+//   k_arn_update  h = the slice dots summed, w1 = w - V h on the slice, its
+//                 slice |w1|^2, and the second pass's slice dots v_i . w1
+//                 (slice-local: they need only this slice of w1)
+//   k_arn_final   the second pass (when the test asks for it): h' = its dots
+//                 summed, w2 = w1 - V h'; |w2| = sqrt(|w1|^2 - |h'|^2)
+//                 (Pythagoras: V orthonormal, and the second pass only runs
+//                 where h' is a rounding-level correction, so the difference
+//                 loses nothing); v_{j+1} = w2 / |w2| on the slice; slice 0
+//                 finishes column j of H
+// Partial sums go to `part` (ArnLayout) and every block that needs a total
+// adds them in slice order, so that all of them see the same value; every sum
+// has a fixed order (deterministic).  The batched form runs the same kernels
+// over (slice, plan) grids: each plan's arithmetic is the single solve's.
+
+// partial sums of one Arnoldi step (doubles): the slice dots of each pass
+// [G][m + 2], the first pass's slice norms [G], its h [m + 2]
+struct ArnLayout {
+    int G, m;   // slices of 64 entries, restart length
+    __host__ __device__ size_t dots(int pass) const { return (size_t)pass * G * (m + 2); }
+    __host__ __device__ size_t norms() const { return (size_t)2 * G * (m + 2); }
+    __host__ __device__ size_t h() const { return (size_t)2 * G * (m + 2) + G; }
+    __host__ __device__ size_t size() const { return h() + (size_t)m + 2; }
+};
+__host__ __device__ inline int arn_slices(int n) { return (n + 63) / 64; }
+
+// one plan's Arnoldi operands (device pointers)
+struct ArnOp {
+    double* V;
+    int ld;
+    double* w;
+    int n;
+    double* st;
+    int* flags;
+    double* part;
+};
+
+__device__ __forceinline__ ArnOp arn_of(const BGm& G) { return ArnOp{G.V, G.ld, G.w, G.n, G.st, G.flags, G.part}; }
+
+// x[0] + x[s] + ... (cnt terms) by one wave: lane l adds terms l, l + 64, ...
+// in order, then a fixed tree over the lanes (every lane gets the sum)
+__device__ __forceinline__ double wave_sum_strided(const double* __restrict__ x, size_t s, int cnt, int lane)
 {
-    const BGm& G = g[blockIdx.x];
-    gm_arnoldi_body(G.V, G.ld, j, G.w, G.n, G.st, m, G.flags);
+    double a = 0.0;
+    for (int b = lane; b < cnt; b += 64) a += x[(size_t)b * s];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    return a;
+}
+
+// the second pass runs: |w| after the first pass < |w| before it / sqrt(2)
+// (ww = sum of the first pass's slice dots w . w, q0 = sum of its slice norms)
+__device__ __forceinline__ bool arn_again(double ww, double q0)
+{
+    return sqrt(q0) < 0.70710678118654752 * sqrt(ww);
+}
+
+// the operands of this block: the single solve's (BAT false, slice =
+// blockIdx.x) or plan blockIdx.y's of the batch
+template <bool BAT>
+__device__ __forceinline__ ArnOp arn_block(const ArnOp& one, const BGm* __restrict__ g)
+{
+    return BAT ? arn_of(g[blockIdx.y]) : one;
+}
+
+// the slice dots v_i . w, i < nd (index nv: w . w), of entry k = 64 b + lane
+// (wk = w[k], 0 past the end): wave u takes v_u, v_{u+4}, ..., eight of them
+// per batch of loads, each a fixed tree over the slice's 64 entries
+__device__ __forceinline__ void arn_slice_dots(const ArnOp& A, double* __restrict__ out, int nd, int nv, int k,
+                                               bool in, double wk, int wave, int lane)
+{
+    for (int i0 = wave; i0 < nd; i0 += 32) {
+        double p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + 4 * u;
+            const double x = in && i < nd ? (i < nv ? A.V[(size_t)A.ld * i + k] : wk) : 0.0;
+            p[u] = x * wk;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] += __shfl_xor(p[u], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (i0 + 4 * u < nd) out[i0 + 4 * u] = p[u];
+    }
+}
+
+// The kernels below issue every load that does not depend on a test (the
+// flags, the second-pass test) before the test: data written by the previous
+// launch may come from another XCD's writes, and each dependent round trip
+// costs about a microsecond.  The loads stay inside the buffers whatever the
+// flags say.
+
+// the first pass's slice dots (and w . w)
+template <bool BAT>
+__global__ void __launch_bounds__(256) k_arn_dots(ArnOp one, const BGm* __restrict__ g, int j, int m)
+{
+    const ArnOp A = arn_block<BAT>(one, g);
+    const ArnLayout L{arn_slices(A.n), m};
+    const int b = blockIdx.x;
+    if (b >= L.G) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nv = j + 1;
+    const int k = 64 * b + lane;
+    const bool in = k < A.n;
+    const double wk = in ? A.w[k] : 0.0;
+    if (A.flags[0]) return;
+    arn_slice_dots(A, A.part + L.dots(0) + (size_t)(m + 2) * b, nv + 1, nv, k, in, wk, wave, lane);
+}
+
+// the first pass's h_t = its slice dots summed in slice order, 32 slices per
+// batch of loads (slice 0 keeps h for k_arn_final); every thread t < nv
+__device__ __forceinline__ double arn_h_sum(const double* __restrict__ d, int m, int G, const double (&hp)[32])
+{
+    double a = 0.0;
+#pragma unroll
+    for (int u = 0; u < 32; ++u)
+        if (u < G) a += hp[u];
+    for (int s0 = 32; s0 < G; s0 += 8) {
+        double y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) y[u] = s0 + u < G ? d[(size_t)(m + 2) * (s0 + u)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (s0 + u < G) a += y[u];
+    }
+    return a;
+}
+
+// acc = sum_{i = wave mod 4, i < nv} x_i h_i at entry k, the first eight
+// basis entries already in x
+__device__ __forceinline__ double arn_vh(const ArnOp& A, const double* __restrict__ h, double (&x)[8], int nv, int k,
+                                        bool in, int wave)
+{
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (wave + 4 * u < nv) acc += x[u] * h[wave + 4 * u];
+    for (int i0 = wave + 32; i0 < nv; i0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + 4 * u;
+            x[u] = in && i < nv ? A.V[(size_t)A.ld * i + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 4 * u < nv) acc += x[u] * h[i0 + 4 * u];
+    }
+    return acc;
+}
+
+// this wave's first eight basis entries at entry k
+__device__ __forceinline__ void arn_load_x(const ArnOp& A, double (&x)[8], int nv, int k, bool in, int wave)
+{
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int i = wave + 4 * u;
+        x[u] = in && i < nv ? A.V[(size_t)A.ld * i + k] : 0.0;
+    }
+}
+
+// first pass: h = the slice dots summed; the slice's w1 = w - (sum over waves
+// u of sum_{i = u mod 4} h_i v_i), its |w1|^2, and the second pass's slice dots
+template <bool BAT>
+__global__ void __launch_bounds__(256) k_arn_update(ArnOp one, const BGm* __restrict__ g, int j, int m)
+{
+    __shared__ double h[256];
+    __shared__ double su[4][64];
+    __shared__ double sw[64];
+    const ArnOp A = arn_block<BAT>(one, g);
+    const ArnLayout L{arn_slices(A.n), m};
+    const int b = blockIdx.x;
+    if (b >= L.G) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nv = j + 1;
+    const int k = 64 * b + lane;
+    const bool in = k < A.n;
+    double x[8];
+    arn_load_x(A, x, nv, k, in, wave);
+    const double wk = in && wave == 0 ? A.w[k] : 0.0;
+    const double* d = A.part + L.dots(0) + t;
+    const bool ht = t < nv;
+    double hp[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) hp[u] = ht && u < L.G ? d[(size_t)(m + 2) * u] : 0.0;
+    if (A.flags[0]) return;
+    if (ht) {
+        const double a = arn_h_sum(d, m, L.G, hp);
+        h[t] = a;
+        if (b == 0) A.part[L.h() + t] = a;
+    }
+    __syncthreads();
+    su[wave][lane] = arn_vh(A, h, x, nv, k, in, wave);
+    __syncthreads();
+    if (wave == 0) {
+        double q = 0.0, r = 0.0;
+        if (in) {
+            r = wk - ((su[0][lane] + su[1][lane]) + (su[2][lane] + su[3][lane]));
+            A.w[k] = r;
+            q = r * r;
+        }
+        sw[lane] = r;
+        for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+        if (lane == 0) A.part[L.norms() + b] = q;
+    }
+    __syncthreads();
+    arn_slice_dots(A, A.part + L.dots(1) + (size_t)(m + 2) * b, nv, nv, k, in, sw[lane], wave, lane);
+}
+
+// the second pass if the test asks for it, |w|, v_{j+1} on the slice; slice 0
+// (one thread for the serial part, the column and rotations staged in LDS)
+// finishes column j of H.  Slices that start after slice 0 has set the flags
+// skip v_{j+1}, which a stopped iteration never reads.
+template <bool BAT>
+__global__ void __launch_bounds__(256) k_arn_final(ArnOp one, const BGm* __restrict__ g, int j, int m)
+{
+    __shared__ double h[256];
+    __shared__ double su[4][64];
+    __shared__ double col[256];
+    __shared__ double rc[256], rs[256];
+    const ArnOp A = arn_block<BAT>(one, g);
+    const ArnLayout L{arn_slices(A.n), m};
+    const int b = blockIdx.x;
+    if (b >= L.G) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nv = j + 1;
+    const int k = 64 * b + lane;
+    const bool in = k < A.n;
+    const GmLayout G{m};
+    double* st = A.st;
+    // everything this block reads, before the tests
+    double x[8];
+    arn_load_x(A, x, nv, k, in, wave);
+    const double wk = in && wave == 0 ? A.w[k] : 0.0;
+    const double* d = A.part + L.dots(1) + t;
+    const bool ht = t < nv;
+    double hp[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) hp[u] = ht && u < L.G ? d[(size_t)(m + 2) * u] : 0.0;
+    const double ww = wave_sum_strided(A.part + L.dots(0) + nv, (size_t)m + 2, L.G, lane);
+    const double q0 = wave_sum_strided(A.part + L.norms(), 1, L.G, lane);
+    double c0 = 0.0, gj = 0.0, tl = 0.0, bn = 0.0;
+    if (b == 0) {
+        if (ht) c0 = A.part[L.h() + t];
+        if (t < j) {
+            rc[t] = st[G.cs() + t];
+            rs[t] = st[G.sn() + t];
+        }
+        if (t == 0) {
+            gj = st[G.g() + j];
+            tl = st[G.tol()];
+            bn = st[G.bnorm()];
+        }
+    }
+    if (A.flags[0]) return;
+    const bool again = arn_again(ww, q0);
+    double hn, r = wk;
+    if (again) {
+        if (ht) h[t] = arn_h_sum(d, m, L.G, hp);
+        __syncthreads();
+        double hh = 0.0;   // |h'|^2, lanes i = lane mod 64, then a fixed tree
+        for (int i = lane; i < nv; i += 64) hh += h[i] * h[i];
+        for (int off = 32; off > 0; off >>= 1) hh += __shfl_xor(hh, off, 64);
+        hn = sqrt(fmax(q0 - hh, 0.0));
+        su[wave][lane] = arn_vh(A, h, x, nv, k, in, wave);
+        __syncthreads();
+        if (wave == 0) r = wk - ((su[0][lane] + su[1][lane]) + (su[2][lane] + su[3][lane]));
+    } else {
+        hn = sqrt(q0);
+    }
+    if (wave == 0 && hn > 0.0 && in) {
+        const double inv = 1.0 / hn;
+        A.V[(size_t)A.ld * nv + k] = r * inv;
+    }
+    if (b != 0) return;
+    if (ht) col[t] = c0 + (again ? h[t] : 0.0);
+    __syncthreads();
+    if (t == 0) {
+        col[j + 1] = hn;
+        double a = col[0];   // entry i of the column, rotated by rotations < i
+        for (int i = 0; i < j; ++i) {
+            const double c = col[i + 1];
+            col[i] = rc[i] * a + rs[i] * c;
+            a = -rs[i] * a + rc[i] * c;
+        }
+        col[j] = a;
+        const double rr = hypot(col[j], col[j + 1]);
+        const double cj = rr == 0.0 ? 1.0 : col[j] / rr, sj = rr == 0.0 ? 0.0 : col[j + 1] / rr;
+        col[j] = rr;
+        col[j + 1] = 0.0;
+        double* H = st + G.H() + (size_t)(m + 1) * j;
+        for (int i = 0; i <= j + 1; ++i) H[i] = col[i];
+        st[G.cs() + j] = cj;
+        st[G.sn() + j] = sj;
+        st[G.g() + j + 1] = -sj * gj;
+        st[G.g() + j] = cj * gj;
+        int* flags = A.flags;
+        if (fabs(-sj * gj) <= tl * bn || hn == 0.0) {
+            flags[0] = 1;
+            flags[1] = j + 1;
+        } else if (j + 1 == m) {
+            flags[0] = 2;
+            flags[1] = m;
+        }
+    }
+}
+
+// the three launches of Arnoldi step j (grid: slices, plans)
+template <bool BAT>
+static void arn_step(hipStream_t st, const ArnOp& one, const BGm* g, dim3 grid, int j, int m)
+{
+    k_arn_dots<BAT><<<grid, 256, 0, st>>>(one, g, j, m);
+    k_arn_update<BAT><<<grid, 256, 0, st>>>(one, g, j, m);
+    k_arn_final<BAT><<<grid, 256, 0, st>>>(one, g, j, m);
 }
 
 // every plan's two flags into one array (for one read-back)
@@ -2575,10 +2880,9 @@ __global__ void __launch_bounds__(256) k_bgm_prec_xb(const BGm* __restrict__ g, 
     if (G.P32) {
         const double s = wave_dot_f32(G.P32 + (size_t)G.ld * i, v, G.n, lane);
         if (lane == 0) x[i] = s + (mode == 1 ? x[i] : 0.0);
-    } else if (lane == 0) {
-        double s = 0.0;
-        for (int k = 0; k < G.bw; ++k) s += G.band[(size_t)i * G.bw + k] * v[G.nbr[(size_t)i * G.bw + k]];
-        x[i] = mode == 1 ? x[i] + s : s;
+    } else {
+        const double s = band_row(G.band + (size_t)i * G.bw, G.nbr + (size_t)i * G.bw, G.bw, v, lane);
+        if (lane == 0) x[i] = mode == 1 ? x[i] + s : s;
     }
 }
 
@@ -2708,7 +3012,7 @@ static void prec_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* z
     if (mc->use_band) {
         band_apply(st, mc->band.as<double>(), pl->nbr.as<int>(), lpc_width(pl->lpc), v, z, N, flags, accumulate);
     } else {
-        k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 256, 0, st>>>(mc->P32.as<float>(), solve_ld(N), v, z, N, flags,
+        k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 128, 0, st>>>(mc->P32.as<float>(), solve_ld(N), v, z, N, flags,
                                                                  accumulate);
         ILH_HIP(hipGetLastError());
     }
@@ -2722,6 +3026,7 @@ static void gm_begin(ilh_plan* pl, MovingCache* mc, const double* b, double* x, 
     const GmLayout L{m};
     mc->gm.ensure(sizeof(double) * L.size());
     mc->gmf.ensure(sizeof(int) * 2);
+    mc->arn.ensure(sizeof(double) * ArnLayout{arn_slices(N), m}.size());
     double* w = mc->w.as<double>();
     (void)ld;
     prec_apply(pl, mc, b, x, nullptr, 0);
@@ -2739,10 +3044,11 @@ static void gm_issue(ilh_plan* pl, MovingCache* mc, int j0, int j1, int m)
     const unsigned gR = (unsigned)((N + 3) / 4);
     double* V = mc->V.as<double>();
     int* flags = mc->gmf.as<int>();
+    const ArnOp A{V, ld, mc->w.as<double>(), N, mc->gm.as<double>(), flags, mc->arn.as<double>()};
     for (int j = j0; j < j1; ++j) {
         prec_apply(pl, mc, V + (size_t)ld * j, mc->z.as<double>(), flags, 0);
         k_gm_matvec<<<gR, 256, 0, st>>>(mc->S.as<double>(), ld, mc->z.as<double>(), mc->w.as<double>(), N, flags);
-        k_gm_arnoldi<<<1, 1024, 0, st>>>(V, ld, j, mc->w.as<double>(), N, mc->gm.as<double>(), m, flags);
+        arn_step<false>(st, A, nullptr, dim3((unsigned)arn_slices(N)), j, m);
     }
     ILH_HIP(hipGetLastError());
     ILH_HIP(hipMemcpyAsync(mc->pinned(1), flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -3099,6 +3405,7 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             MovingCache* c = mc[i];
             c->gm.ensure(sizeof(double) * GmLayout{m}.size());
             c->gmf.ensure(sizeof(int) * 2);
+            c->arn.ensure(sizeof(double) * ArnLayout{arn_slices(pl->N), m}.size());
             BGm G{};
             G.P32 = c->use_band ? nullptr : c->P32.as<float>();
             G.band = c->use_band ? c->band.as<double>() : nullptr;
@@ -3112,6 +3419,7 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             G.z = c->z.as<double>();
             G.st = c->gm.as<double>();
             G.flags = c->gmf.as<int>();
+            G.part = c->arn.as<double>();
             G.fp = fb_plus[i];
             G.fm = fb_minus ? fb_minus[i] : nullptr;
             G.amp = c->amp.as<double>();
@@ -3173,9 +3481,9 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
             for (;;) {
                 const int j1 = std::min(m, j + chunk);
                 for (int jj = j; jj < j1; ++jj) {
-                    k_bgm_prec<<<nt, 256, 0, main>>>(dg, dt, jj);
+                    k_bgm_prec<<<nt, 128, 0, main>>>(dg, dt, jj);
                     k_bgm_matvec<<<nt, 256, 0, main>>>(dg, dt);
-                    k_bgm_arnoldi<<<(unsigned)nd, 1024, 0, main>>>(dg, jj, m);
+                    arn_step<true>(main, ArnOp{}, dg, dim3((unsigned)arn_slices(maxn), (unsigned)nd), jj, m);
                 }
                 k_bgm_flags<<<(unsigned)((nd + 255) / 256), 256, 0, main>>>(dg, nd, df);
                 ILH_HIP(hipGetLastError());

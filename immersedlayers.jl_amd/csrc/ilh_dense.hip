@@ -904,7 +904,7 @@ __device__ __forceinline__ void wave_dot_group(const double* __restrict__ row, c
     }
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
-        if ((m & 1) && lane == 0) s[g][1] += row[m - 1] * x[(size_t)ldf * g + m - 1];
+        if ((m & 1) && lane == 0) s[g][1] = __builtin_fma(row[m - 1], x[(size_t)ldf * g + m - 1], s[g][1]);
         double t = (s[g][0] + s[g][1]) + (s[g][2] + s[g][3]);
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
         out[g] = t;

@@ -434,7 +434,7 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const
 {
     if (((uintptr_t)x & 15) != 0) {   // caller's vector not 16-byte aligned: plain loads
         double s = 0.0;
-        for (int k = lane; k < m; k += 64) s += row[k] * x[k];
+        for (int k = lane; k < m; k += 64) s = __builtin_fma(row[k], x[k], s);
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
         return s;
     }
@@ -489,7 +489,7 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ row, const
             break;
         }
     }
-    if ((m & 1) && lane == 0) s1 += row[m - 1] * x[m - 1];
+    if ((m & 1) && lane == 0) s1 = __builtin_fma(row[m - 1], x[m - 1], s1);
     double s = (s0 + s1) + (s2 + s3);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
@@ -515,6 +515,34 @@ int lpc_width(const ilh_lpc* m);
 int lpc_half_band(const ilh_lpc* m);
 void band_apply(hipStream_t st, const double* coef, const int* nbr, int W, const double* v, double* z, int n,
                 const int* flags, int accumulate);
+
+// sum_j coef[j] v[nbr[j]] over one band row of W entries by one wave, in the
+// order j = 0 .. W-1 (one fused multiply-add per entry, every lane gets the
+// sum): lane l loads coef[j0 + l] and nbr[j0 + l], then gathers v, 64 entries
+// per round of two dependent loads; the chain then reads the lanes' values in
+// order (v_readlane), the same arithmetic as a single thread's loop
+__device__ __forceinline__ double readlane_f64(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double band_row(const double* __restrict__ coef, const int* __restrict__ nbr, int W,
+                                           const double* __restrict__ v, int lane)
+{
+    double s = 0.0;
+    for (int j0 = 0; j0 < W; j0 += 64) {
+        const bool ok = j0 + lane < W;
+        const int ix = ok ? nbr[j0 + lane] : 0;
+        const double c = ok ? coef[j0 + lane] : 0.0;
+        const double x = ok ? v[ix] : 0.0;
+        const int cnt = W - j0 < 64 ? W - j0 : 64;
+        for (int u = 0; u < cnt; ++u) s = __builtin_fma(readlane_f64(c, u), readlane_f64(x, u), s);
+    }
+    return s;
+}
 // S^-1 of the plan's create_RTLinvR, row-major with leading dimension solve_ld(N) (ilh_api.hip)
 void exact_inverse_rowmajor(ilh_plan* pl, double* Minv);
 

@@ -340,24 +340,24 @@ __global__ void k_band_coef(const float* __restrict__ Y, int ldy, int N, int W, 
     coef[e] = ((double)Y[(size_t)p * ldy + j] * sd[j] + mu[j]) / (ds[p] * dx);
 }
 
-// z (+)= band(coef) v, skipped once the flags say the solve has converged
-__global__ void k_band_apply(const double* __restrict__ coef, const int* __restrict__ nbr, int W,
+// z (+)= band(coef) v, skipped once the flags say the solve has converged;
+// one wave per row
+__global__ void __launch_bounds__(256) k_band_apply(const double* __restrict__ coef, const int* __restrict__ nbr, int W,
                              const double* __restrict__ v, double* __restrict__ z, int n, const int* __restrict__ flags,
                              int accumulate)
 {
     if (flags && flags[0]) return;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (p >= n) return;
-    double s = 0.0;
-    for (int j = 0; j < W; ++j) s += coef[(size_t)p * W + j] * v[nbr[(size_t)p * W + j]];
-    z[p] = accumulate ? z[p] + s : s;
+    const double s = band_row(coef + (size_t)p * W, nbr + (size_t)p * W, W, v, lane);
+    if (lane == 0) z[p] = accumulate ? z[p] + s : s;
 }
 
 void band_apply(hipStream_t st, const double* coef, const int* nbr, int W, const double* v, double* z, int n,
                 const int* flags, int accumulate)
 {
     if (n <= 0) return;
-    k_band_apply<<<(n + 255) / 256, 256, 0, st>>>(coef, nbr, W, v, z, n, flags, accumulate);
+    k_band_apply<<<(n + 3) / 4, 256, 0, st>>>(coef, nbr, W, v, z, n, flags, accumulate);
     ILH_HIP(hipGetLastError());
 }
 
