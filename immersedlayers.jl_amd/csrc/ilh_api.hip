@@ -2370,22 +2370,17 @@ __device__ __forceinline__ double2 wave_dot_f32_rows2(const float* __restrict__ 
     return make_double2(s, t);
 }
 
-// y = M x (M in FP32, row-major, two rows per wave, four per work-group of
-// 128), or y += M x when accumulate; stops once the iteration has (flags
-// non-null and set)
-__global__ void __launch_bounds__(128) k_gm_matvec_f32(const float* __restrict__ M, int ld, const double* __restrict__ x,
+// y = M x (M in FP32, row-major, one wave per row), or y += M x when
+// accumulate; stops once the iteration has (flags non-null and set)
+__global__ void __launch_bounds__(256) k_gm_matvec_f32(const float* __restrict__ M, int ld, const double* __restrict__ x,
                                                        double* __restrict__ y, int n, const int* __restrict__ flags,
                                                        int accumulate)
 {
     if (flags && flags[0]) return;
-    const int i = blockIdx.x * 4 + 2 * (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
-    const bool two = i + 1 < n;
-    const double2 s = wave_dot_f32_rows2(M + (size_t)ld * i, M + (size_t)ld * (two ? i + 1 : i), x, n, lane);
-    if (lane == 0) {
-        y[i] = s.x + (accumulate ? y[i] : 0.0);
-        if (two) y[i + 1] = s.y + (accumulate ? y[i + 1] : 0.0);
-    }
+    const double s = wave_dot_f32(M + (size_t)ld * i, x, n, lane);
+    if (lane == 0) y[i] = s + (accumulate ? y[i] : 0.0);
 }
 
 __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)
@@ -2790,41 +2785,81 @@ __global__ void k_bgm_flags(const BGm* __restrict__ g, int nprob, int* __restric
     out[2 * i + 1] = g[i].flags[1];
 }
 
-// y = H^-1 g (upper triangular, j x j, thread 0), then w = V y (one
-// work-group, j <= 256)
-__device__ __forceinline__ void gm_finish_body(const double* __restrict__ st, int m, int j,
-                                               const double* __restrict__ V, int ldv, int n, double* __restrict__ w)
+// y = H^-1 g (upper triangular, j x j), then w = V y on one 64-entry slice b
+// per work-group of 256.  Every work-group solves for y itself (wave 0, the
+// column sweep: the right-hand side in registers, lane l holding rows
+// l + 64 q, y_i broadcast by v_readlane, H staged in LDS when j <= 64), the
+// same arithmetic in each; its slice's V entries for the first 32 basis
+// vectors are loaded before the sweep.  The waves split the basis vectors
+// mod 4 and their sums are added in a fixed order (deterministic).
+__device__ __forceinline__ void gm_finish_slice(const double* __restrict__ st, int m, int j,
+                                                const double* __restrict__ V, int ldv, int n,
+                                                double* __restrict__ w, int b)
 {
-    __shared__ double y[256];
     __shared__ double Hs[64 * 64];   // the leading j x j block of H when j <= 64
+    __shared__ double y[256];
+    __shared__ double su[4][64];
     const GmLayout L{m};
     const double* H = st + L.H();
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int k = 64 * b + lane;
+    const bool in = k < n;
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int i = wave + 4 * u;
+        x[u] = in && i < j ? V[k + (size_t)ldv * i] : 0.0;
+    }
     const bool staged = j <= 64;
     if (staged)
-        for (int e = threadIdx.x; e < j * j; e += 1024) Hs[e] = H[e % j + (size_t)(m + 1) * (e / j)];
-    for (int i = threadIdx.x; i < j; i += 1024) y[i] = st[L.g() + i];
+        for (int e = t; e < j * j; e += 256) Hs[e] = H[e % j + (size_t)(m + 1) * (e / j)];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wave == 0) {
+        double r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = lane + 64 * q;
+            r[q] = i < j ? st[L.g() + i] : 0.0;
+        }
         for (int i = j - 1; i >= 0; --i) {
-            double t = y[i];
-            for (int k = i + 1; k < j; ++k)
-                t -= (staged ? Hs[i + j * k] : H[i + (size_t)(m + 1) * k]) * y[k];
-            y[i] = t / (staged ? Hs[i + j * i] : H[i + (size_t)(m + 1) * i]);
+            const int q = i >> 6;
+            const double ri = readlane_f64(q == 0 ? r[0] : q == 1 ? r[1] : q == 2 ? r[2] : r[3], i & 63);
+            const double yi = ri / (staged ? Hs[i + j * i] : H[i + (size_t)(m + 1) * i]);
+            if (lane == 0) y[i] = yi;
+#pragma unroll
+            for (int q2 = 0; q2 < 4; ++q2) {
+                const int kk = lane + 64 * q2;
+                if (kk < i)
+                    r[q2] = __builtin_fma(-(staged ? Hs[kk + j * i] : H[kk + (size_t)(m + 1) * i]), yi, r[q2]);
+            }
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < n; k += 1024) {
-        double acc = 0.0;
-        for (int i = 0; i < j; ++i) acc += V[k + (size_t)ldv * i] * y[i];
-        w[k] = acc;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (wave + 4 * u < j) acc = __builtin_fma(x[u], y[wave + 4 * u], acc);
+    for (int i0 = wave + 32; i0 < j; i0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = i0 + 4 * u;
+            x[u] = in && i < j ? V[k + (size_t)ldv * i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + 4 * u < j) acc = __builtin_fma(x[u], y[i0 + 4 * u], acc);
     }
+    su[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && in) w[k] = (su[0][lane] + su[1][lane]) + (su[2][lane] + su[3][lane]);
 }
 
-__global__ void __launch_bounds__(1024) k_gm_finish(const double* __restrict__ st, int m, int j,
-                                                    const double* __restrict__ V, int ldv, int n,
-                                                    double* __restrict__ w)
+// grid: 64-entry slices of n
+__global__ void __launch_bounds__(256) k_gm_finish(const double* __restrict__ st, int m, int j,
+                                                   const double* __restrict__ V, int ldv, int n,
+                                                   double* __restrict__ w)
 {
-    gm_finish_body(st, m, j, V, ldv, n, w);
+    gm_finish_slice(st, m, j, V, ldv, n, w, blockIdx.x);
 }
 
 // ---- the rest of a moving-body step over every plan (grid.y = plan; the
@@ -2906,14 +2941,14 @@ __global__ void __launch_bounds__(1024) k_bgm_init(const BGm* __restrict__ g, in
     gm_init_body(G.w, G.b, G.n, G.V, G.st, m, tol, G.flags);
 }
 
-// y = H^-1 g, w = V y for the plans that converged
-__global__ void __launch_bounds__(1024) k_bgm_finish(const BGm* __restrict__ g, int m)
+// y = H^-1 g, w = V y for the plans that converged (grid: slices, plans)
+__global__ void __launch_bounds__(256) k_bgm_finish(const BGm* __restrict__ g, int m)
 {
-    const BGm& G = g[blockIdx.x];
+    const BGm& G = g[blockIdx.y];
     if (G.flags[0] != 1) return;
     const int j = G.flags[1];
-    if (j <= 0) return;
-    gm_finish_body(G.st, m, j, G.V, G.ld, G.n, G.w);
+    if (j <= 0 || (int)blockIdx.x >= arn_slices(G.n)) return;
+    gm_finish_slice(G.st, m, j, G.V, G.ld, G.n, G.w, blockIdx.x);
 }
 
 // s = amp[0, n)
@@ -3012,7 +3047,7 @@ static void prec_apply(ilh_plan* pl, MovingCache* mc, const double* v, double* z
     if (mc->use_band) {
         band_apply(st, mc->band.as<double>(), pl->nbr.as<int>(), lpc_width(pl->lpc), v, z, N, flags, accumulate);
     } else {
-        k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 128, 0, st>>>(mc->P32.as<float>(), solve_ld(N), v, z, N, flags,
+        k_gm_matvec_f32<<<(unsigned)((N + 3) / 4), 256, 0, st>>>(mc->P32.as<float>(), solve_ld(N), v, z, N, flags,
                                                                  accumulate);
         ILH_HIP(hipGetLastError());
     }
@@ -3061,7 +3096,8 @@ static void gm_end(ilh_plan* pl, MovingCache* mc, double* x, int m, int steps)
     hipStream_t st = pl->ctx->stream;
     if (steps <= 0) return;
     mc->last_iters = steps;
-    k_gm_finish<<<1, 1024, 0, st>>>(mc->gm.as<double>(), m, steps, mc->V.as<double>(), ld, N, mc->w.as<double>());
+    k_gm_finish<<<(unsigned)arn_slices(N), 256, 0, st>>>(mc->gm.as<double>(), m, steps, mc->V.as<double>(), ld, N,
+                                                         mc->w.as<double>());
     ILH_HIP(hipGetLastError());
     prec_apply(pl, mc, mc->w.as<double>(), x, nullptr, 1);
 }
@@ -3503,7 +3539,7 @@ extern "C" int ilh_dirichlet_solve_moving_batched(int nprob, ilh_plan* const* pl
                 chunk = 2;
             }
             // x += P (V y) for the plans that converged
-            k_bgm_finish<<<(unsigned)nd, 1024, 0, main>>>(dg, m);
+            k_bgm_finish<<<dim3((unsigned)arn_slices(maxn), (unsigned)nd), 256, 0, main>>>(dg, m);
             k_bgm_prec_xb<<<nt, 256, 0, main>>>(dg, dt, 1);
             ILH_HIP(hipGetLastError());
             for (int i = 0; i < nprob; ++i)
