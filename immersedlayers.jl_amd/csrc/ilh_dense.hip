@@ -25,7 +25,12 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // ------------------------------------------------------------ assembly
 static constexpr int AT = 16;   // points per assembly tile edge
 
-static constexpr int GBOX = 4096;   // doubles of the kernel table staged per tile (32 KB)
+// doubles of the kernel table staged per tile (12 KB).  Tiles whose box is
+// larger read the table from memory instead; on the 1024^2 headline bodies
+// the occupancy a smaller box allows outweighs those reads (per N = 1885
+// matrix under the profiler: 57.9 us at 4096, 50.2 at 2048, 48.6 at 1536,
+// 58.0 at 1024)
+static constexpr int GBOX = 1536;
 
 // One 16x16 tile of (p, q) entries per work-group.  The table entries a tile
 // touches lie in a box of |dbx + u|, |dby + v| whose sides are the two tiles'
