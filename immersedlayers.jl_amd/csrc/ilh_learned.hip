@@ -124,6 +124,92 @@ __global__ void __launch_bounds__(256) k_bgemm(int M, int N, int K, int kchunk, 
             }
 }
 
+// C = A B (+ bias, ReLU), A row-major (M x K), B row-major (K x N), K <= 256
+// (every forward layer of the network): the work-group's whole 64 x K slice
+// of A and K x 64 slice of B go out in one round of 16-byte loads (eight
+// BF16 each), B transposed into LDS on the way, and the K/32 MFMA steps then
+// run from LDS without further barriers.  The MFMA sequence and the epilogue
+// are k_bgemm<0, 0>'s, so the results are its results bit for bit; k_bgemm
+// waits for one load round per 32-wide step.  Needs lda, ldb and N multiples
+// of 8 and 16-byte aligned A and B.
+static constexpr int FK = 256, FKP = FK + 8;
+
+__global__ void __launch_bounds__(256) k_bgemm_fwd(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
+                                                   const uint16_t* __restrict__ B, int ldb,
+                                                   const float* __restrict__ bias, int relu, float* __restrict__ Cf,
+                                                   uint16_t* __restrict__ Cb, int ldc)
+{
+    __shared__ __align__(16) uint16_t As[BT][FKP];   // [m][k]
+    __shared__ __align__(16) uint16_t Bs[BT][FKP];   // [n][k]
+    const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int kv = K >> 3, K32 = (K + BKC - 1) / BKC * BKC;
+    uint4 xa[8], xb[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int v = tid + 256 * e;
+        const int m = v / kv, k8 = (v - m * kv) * 8, gm = m0 + m;
+        xa[e] = v < BT * kv && gm < M ? *(const uint4*)&A[(size_t)gm * lda + k8] : make_uint4(0u, 0u, 0u, 0u);
+        const int k = v >> 3, n8 = (v & 7) * 8, gn = n0 + n8;
+        xb[e] = v < K * 8 && gn < N ? *(const uint4*)&B[(size_t)k * ldb + gn] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int v = tid + 256 * e;
+        if (v < BT * kv) {
+            const int m = v / kv, k8 = (v - m * kv) * 8;
+            *(uint4*)&As[m][k8] = xa[e];
+        }
+        if (v < K * 8) {
+            const int k = v >> 3, n8 = (v & 7) * 8;
+            const uint16_t* h = (const uint16_t*)&xb[e];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Bs[n8 + i][k] = h[i];
+        }
+    }
+    // zeros for k in [K, K32)
+    for (int e = tid; e < BT * (K32 - K); e += 256) {
+        const int r = e / (K32 - K), k = K + e % (K32 - K);
+        As[r][k] = 0;
+        Bs[r][k] = 0;
+    }
+    __syncthreads();
+    float4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    const int kq = 8 * (lane >> 4);
+    for (int k0 = 0; k0 < K32; k0 += BKC) {
+        bf16x8_t a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8_t*)&As[wm + 16 * i + (lane & 15)][k0 + kq];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8_t*)&Bs[wn + 16 * j + (lane & 15)][k0 + kq];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int gn = n0 + wn + 16 * j + (lane & 15);
+                if (gm < M && gn < N) {
+                    float v = acc[i][j][r];
+                    if (bias) v += bias[gn];
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    if (Cf) Cf[(size_t)gm * ldc + gn] = v;
+                    if (Cb) Cb[(size_t)gm * ldc + gn] = f2bf(v);
+                }
+            }
+}
+
 static void bgemm(hipStream_t st, int ta, int tb, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
                   int ldb, const float* bias, int relu, float* Cf, uint16_t* Cb, int ldc, int splits = 1,
                   size_t split_stride = 0)
@@ -131,7 +217,11 @@ static void bgemm(hipStream_t st, int ta, int tb, int M, int N, int K, const uin
     if (M <= 0 || N <= 0) return;
     const int kchunk = ((K + splits - 1) / splits + BKC - 1) / BKC * BKC;
     const dim3 grid((M + BT - 1) / BT, (N + BT - 1) / BT, splits);
-    if (ta == 0 && tb == 0)
+    const bool fwd = ta == 0 && tb == 0 && splits == 1 && K > 0 && K <= FK && K % 8 == 0 && lda % 8 == 0 &&
+                     ldb % 8 == 0 && N % 8 == 0 && (((uintptr_t)A | (uintptr_t)B) & 15) == 0;
+    if (fwd)
+        k_bgemm_fwd<<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, bias, relu, Cf, Cb, ldc);
+    else if (ta == 0 && tb == 0)
         k_bgemm<0, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, A, lda, B, ldb, bias, relu, Cf, Cb, ldc, split_stride);
     else if (ta == 0 && tb == 1)
         k_bgemm<0, 1><<<grid, 256, 0, st>>>(M, N, K, kchunk, A, lda, B, ldb, bias, relu, Cf, Cb, ldc, split_stride);
