@@ -124,53 +124,80 @@ __global__ void __launch_bounds__(256) k_bgemm(int M, int N, int K, int kchunk, 
             }
 }
 
-// C = A B (+ bias, ReLU), A row-major (M x K), B row-major (K x N), K <= 256
-// (every forward layer of the network): the work-group's whole 64 x K slice
-// of A and K x 64 slice of B go out in one round of 16-byte loads (eight
-// BF16 each), B transposed into LDS on the way, and the K/32 MFMA steps then
-// run from LDS without further barriers.  The MFMA sequence and the epilogue
-// are k_bgemm<0, 0>'s, so the results are its results bit for bit; k_bgemm
-// waits for one load round per 32-wide step.  Needs lda, ldb and N multiples
-// of 8 and 16-byte aligned A and B.
+// k_bgemm<TA, TB> for K slices of at most 256 (every forward layer, the
+// weight gradients split over 256-row slices, the input gradients): the
+// work-group's whole 64 x K slices of both operands go out in one round of
+// 16-byte loads (eight BF16 each), an operand whose memory order is not
+// [row][k] transposed into LDS on the way, and the K/32 MFMA steps then run
+// from LDS without further barriers.  The MFMA sequence and the epilogue are
+// k_bgemm's, so the results are its results bit for bit; k_bgemm waits for
+// one load round per 32-wide step.  Needs 16-byte aligned operands, leading
+// dimensions and the 16-byte-loaded extents (K or M, N) multiples of 8.
 static constexpr int FK = 256, FKP = FK + 8;
 
-__global__ void __launch_bounds__(256) k_bgemm_fwd(int M, int N, int K, const uint16_t* __restrict__ A, int lda,
-                                                   const uint16_t* __restrict__ B, int ldb,
-                                                   const float* __restrict__ bias, int relu, float* __restrict__ Cf,
-                                                   uint16_t* __restrict__ Cb, int ldc)
+// one operand's 64 x KL slice into S[row][k] (rows r0.., k from kbeg): KT = 0
+// for memory [row][k] (16-byte loads along k), 1 for [k][row] (along rows,
+// transposed into LDS)
+template <int KT>
+__device__ __forceinline__ void wide_load(const uint16_t* __restrict__ X, int ld, int R, int r0, int kbeg, int KL,
+                                          int tid, uint4 (&x)[8])
+{
+    const int kv = KL >> 3;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int v = tid + 256 * e;
+        if (KT == 0) {
+            const int r = v / kv, k8 = (v - r * kv) * 8;
+            x[e] = v < BT * kv && r0 + r < R ? *(const uint4*)&X[(size_t)(r0 + r) * ld + kbeg + k8]
+                                             : make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            const int k = v >> 3, r8 = (v & 7) * 8;
+            x[e] = v < KL * 8 && r0 + r8 < R ? *(const uint4*)&X[(size_t)(kbeg + k) * ld + r0 + r8]
+                                             : make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+}
+
+template <int KT>
+__device__ __forceinline__ void wide_store(uint16_t (*S)[FKP], int KL, int tid, const uint4 (&x)[8])
+{
+    const int kv = KL >> 3;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int v = tid + 256 * e;
+        if (KT == 0) {
+            if (v < BT * kv) {
+                const int r = v / kv, k8 = (v - r * kv) * 8;
+                *(uint4*)&S[r][k8] = x[e];
+            }
+        } else if (v < KL * 8) {
+            const int k = v >> 3, r8 = (v & 7) * 8;
+            const uint16_t* h = (const uint16_t*)&x[e];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) S[r8 + i][k] = h[i];
+        }
+    }
+}
+
+template <int TA, int TB>
+__global__ void __launch_bounds__(256) k_bgemm_w(int M, int N, int K, int kchunk, const uint16_t* __restrict__ A,
+                                                 int lda, const uint16_t* __restrict__ B, int ldb,
+                                                 const float* __restrict__ bias, int relu, float* __restrict__ Cf,
+                                                 uint16_t* __restrict__ Cb, int ldc, size_t split_stride)
 {
     __shared__ __align__(16) uint16_t As[BT][FKP];   // [m][k]
     __shared__ __align__(16) uint16_t Bs[BT][FKP];   // [n][k]
     const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int kv = K >> 3, K32 = (K + BKC - 1) / BKC * BKC;
+    const int kbeg = blockIdx.z * kchunk, KL = min(K, kbeg + kchunk) - kbeg, K32 = (KL + BKC - 1) / BKC * BKC;
     uint4 xa[8], xb[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int v = tid + 256 * e;
-        const int m = v / kv, k8 = (v - m * kv) * 8, gm = m0 + m;
-        xa[e] = v < BT * kv && gm < M ? *(const uint4*)&A[(size_t)gm * lda + k8] : make_uint4(0u, 0u, 0u, 0u);
-        const int k = v >> 3, n8 = (v & 7) * 8, gn = n0 + n8;
-        xb[e] = v < K * 8 && gn < N ? *(const uint4*)&B[(size_t)k * ldb + gn] : make_uint4(0u, 0u, 0u, 0u);
-    }
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int v = tid + 256 * e;
-        if (v < BT * kv) {
-            const int m = v / kv, k8 = (v - m * kv) * 8;
-            *(uint4*)&As[m][k8] = xa[e];
-        }
-        if (v < K * 8) {
-            const int k = v >> 3, n8 = (v & 7) * 8;
-            const uint16_t* h = (const uint16_t*)&xb[e];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) Bs[n8 + i][k] = h[i];
-        }
-    }
-    // zeros for k in [K, K32)
-    for (int e = tid; e < BT * (K32 - K); e += 256) {
-        const int r = e / (K32 - K), k = K + e % (K32 - K);
+    wide_load<TA == 0 ? 0 : 1>(A, lda, M, m0, kbeg, KL, tid, xa);
+    wide_load<TB == 1 ? 0 : 1>(B, ldb, N, n0, kbeg, KL, tid, xb);
+    wide_store<TA == 0 ? 0 : 1>(As, KL, tid, xa);
+    wide_store<TB == 1 ? 0 : 1>(Bs, KL, tid, xb);
+    for (int e = tid; e < BT * (K32 - KL); e += 256) {   // zeros for k in [KL, K32)
+        const int r = e / (K32 - KL), k = KL + e % (K32 - KL);
         As[r][k] = 0;
         Bs[r][k] = 0;
     }
@@ -192,6 +219,7 @@ __global__ void __launch_bounds__(256) k_bgemm_fwd(int M, int N, int K, const ui
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    float* Cz = Cf ? Cf + (size_t)blockIdx.z * split_stride : nullptr;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -204,7 +232,7 @@ __global__ void __launch_bounds__(256) k_bgemm_fwd(int M, int N, int K, const ui
                     float v = acc[i][j][r];
                     if (bias) v += bias[gn];
                     if (relu) v = v > 0.f ? v : 0.f;
-                    if (Cf) Cf[(size_t)gm * ldc + gn] = v;
+                    if (Cz) Cz[(size_t)gm * ldc + gn] = v;
                     if (Cb) Cb[(size_t)gm * ldc + gn] = f2bf(v);
                 }
             }
@@ -217,10 +245,18 @@ static void bgemm(hipStream_t st, int ta, int tb, int M, int N, int K, const uin
     if (M <= 0 || N <= 0) return;
     const int kchunk = ((K + splits - 1) / splits + BKC - 1) / BKC * BKC;
     const dim3 grid((M + BT - 1) / BT, (N + BT - 1) / BT, splits);
-    const bool fwd = ta == 0 && tb == 0 && splits == 1 && K > 0 && K <= FK && K % 8 == 0 && lda % 8 == 0 &&
-                     ldb % 8 == 0 && N % 8 == 0 && (((uintptr_t)A | (uintptr_t)B) & 15) == 0;
-    if (fwd)
-        k_bgemm_fwd<<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, bias, relu, Cf, Cb, ldc);
+    // the one-round form: every K slice at most 256, the 16-byte-loaded
+    // extents (K along [row][k] operands, M or N along [k][row] ones) and the
+    // leading dimensions multiples of 8, both operands 16-byte aligned
+    const bool wide = K > 0 && std::min(K, kchunk) <= FK && lda % 8 == 0 && ldb % 8 == 0 &&
+                      (((uintptr_t)A | (uintptr_t)B) & 15) == 0 && (ta == 1 ? M % 8 == 0 : K % 8 == 0) &&
+                      (tb == 0 ? N % 8 == 0 : K % 8 == 0);
+    if (wide && ta == 0 && tb == 0)
+        k_bgemm_w<0, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, A, lda, B, ldb, bias, relu, Cf, Cb, ldc, split_stride);
+    else if (wide && ta == 0 && tb == 1)
+        k_bgemm_w<0, 1><<<grid, 256, 0, st>>>(M, N, K, kchunk, A, lda, B, ldb, bias, relu, Cf, Cb, ldc, split_stride);
+    else if (wide && ta == 1 && tb == 0)
+        k_bgemm_w<1, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, A, lda, B, ldb, bias, relu, Cf, Cb, ldc, split_stride);
     else if (ta == 0 && tb == 0)
         k_bgemm<0, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, A, lda, B, ldb, bias, relu, Cf, Cb, ldc, split_stride);
     else if (ta == 0 && tb == 1)
