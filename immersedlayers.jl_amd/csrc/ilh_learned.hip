@@ -302,6 +302,37 @@ __global__ void __launch_bounds__(1024) k_colsum_bf(const uint16_t* __restrict__
     }
 }
 
+// k_colsum_bf's sums over more of the GPU, bit for bit: thread (g, c8) of
+// row class g = blockIdx.y adds rows g, g + 32, ... of columns c8 .. c8 + 7
+// (16-byte loads; each column's additions are k_colsum_bf's thread (g, c)'s,
+// in the same order) into part[g][c]; k_colsum_fin then adds the 32 partial
+// sums of each column in g order.  Needs cols and ld multiples of 8 and X
+// 16-byte aligned.
+__global__ void __launch_bounds__(64) k_colsum_part(const uint16_t* __restrict__ X, int rows, int cols, int ld,
+                                                    float* __restrict__ part)
+{
+    const int c8 = (blockIdx.x * 64 + threadIdx.x) * 8, g = blockIdx.y;
+    if (c8 >= cols) return;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int r = g; r < rows; r += 32) {
+        const uint4 v = *(const uint4*)&X[(size_t)r * ld + c8];
+        const uint16_t* h = (const uint16_t*)&v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s[i] += bf2f(h[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) part[(size_t)g * cols + c8 + i] = s[i];
+}
+
+__global__ void k_colsum_fin(const float* __restrict__ part, int cols, float* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float t = 0.f;
+    for (int k = 0; k < 32; ++k) t += part[(size_t)k * cols + c];
+    out[c] = t;
+}
+
 // dZ = dA * (A > 0), BF16 (A the ReLU output of the layer)
 __global__ void k_relu_bwd(const float* __restrict__ dA, const uint16_t* __restrict__ Aact, size_t n,
                            uint16_t* __restrict__ dZ)
@@ -527,6 +558,7 @@ struct ilh_lpc {
     DevBuf mux, sdx, muy, sdy;        // standardization (frozen at the first training step)
     // work buffers
     DevBuf act[4], zf, dA, dZ, Yf, Tf, part, idx, splitbuf, feat, Xb, band;
+    DevBuf colpart;                   // the bias gradients' partial column sums (32 x columns)
     size_t cap_rows = 0;
 };
 
@@ -604,7 +636,15 @@ double lpc_backward(ilh_lpc* m, int rows)
               m->splitbuf.as<float>(), nullptr, out, splits, nW);
         k_sum_splits<<<(unsigned)((nW + 255) / 256), 256, 0, st>>>(m->splitbuf.as<float>(), splits, nW, nW,
                                                                     Gf + m->off_w[l]);
-        k_colsum_bf<<<(out + 31) / 32, 1024, 0, st>>>(m->dZ.as<uint16_t>(), rows, out, out, Gf + m->off_b[l]);
+        if (out % 8 == 0 && ((uintptr_t)m->dZ.p & 15) == 0) {
+            m->colpart.ensure(sizeof(float) * 32 * (size_t)out);
+            k_colsum_part<<<dim3((unsigned)((out / 8 + 63) / 64), 32), 64, 0, st>>>(m->dZ.as<uint16_t>(), rows, out,
+                                                                                    out, m->colpart.as<float>());
+            k_colsum_fin<<<(unsigned)((out + 255) / 256), 256, 0, st>>>(m->colpart.as<float>(), out,
+                                                                       Gf + m->off_b[l]);
+        } else {
+            k_colsum_bf<<<(out + 31) / 32, 1024, 0, st>>>(m->dZ.as<uint16_t>(), rows, out, out, Gf + m->off_b[l]);
+        }
         ILH_HIP(hipGetLastError());
         if (l > 0) {
             // dA = dZ W^T (rows x in), then dZ = dA * (act[l] > 0)
