@@ -301,6 +301,31 @@ const Term* get_terms(ilh_plan* pl, int op)
 }
 
 // ------------------------------------------------------------ row binning
+// The binning kernels walk the terms with every thread's next BIN_BATCH
+// terms (k0, k0 + blockDim.x, ...) loaded together: the window's first row
+// and a mask of its rows with a nonzero weight (0 for terms of another grid
+// kind or past the end), so a thread waits for one round of loads per batch
+// rather than one per term.
+constexpr int BIN_BATCH = 8;
+
+__device__ __forceinline__ void bin_rows(const Term* __restrict__ t, int nt, int gk, int k0, int (&row0)[BIN_BATCH],
+                                         unsigned (&msk)[BIN_BATCH])
+{
+#pragma unroll
+    for (int u = 0; u < BIN_BATCH; ++u) {
+        const int k = k0 + u * (int)blockDim.x;
+        unsigned m = 0;
+        int r = 0;
+        if (k < nt && t[k].gk == gk) {
+            r = t[k].by;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) m |= (t[k].wy[e] != 0.0 ? 1u : 0u) << e;
+        }
+        row0[u] = r;
+        msk[u] = m;
+    }
+}
+
 // entries per row (terms of grid kind gk with a nonzero weight in the row):
 // one work-group, counts in LDS (dynamic, ny ints), written to cnt[0..ny)
 __device__ __forceinline__ void k_bin_count_body(const Term* __restrict__ t, int nt, int gk, int ny,
@@ -309,12 +334,16 @@ __device__ __forceinline__ void k_bin_count_body(const Term* __restrict__ t, int
     extern __shared__ int lc[];
     for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
     __syncthreads();
-    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
-        if (t[k].gk != gk) continue;
-        for (int e = 0; e < 5; ++e) {
-            const int row = t[k].by + e;
-            if (row >= 0 && row < ny && t[k].wy[e] != 0.0) atomicAdd(&lc[row], 1);
-        }
+    for (int k0 = threadIdx.x; k0 < nt; k0 += blockDim.x * BIN_BATCH) {
+        int row0[BIN_BATCH];
+        unsigned msk[BIN_BATCH];
+        bin_rows(t, nt, gk, k0, row0, msk);
+#pragma unroll
+        for (int u = 0; u < BIN_BATCH; ++u)
+            for (int e = 0; e < 5; ++e) {
+                const int row = row0[u] + e;
+                if ((msk[u] >> e & 1u) && row >= 0 && row < ny) atomicAdd(&lc[row], 1);
+            }
     }
     __syncthreads();
     for (int r = threadIdx.x; r < ny; r += blockDim.x) cnt[r] = lc[r];
@@ -381,15 +410,19 @@ __device__ __forceinline__ void k_bin_fill_body(const Term* __restrict__ t, int 
     extern __shared__ int lc[];
     for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
     __syncthreads();
-    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
-        if (t[k].gk != gk) continue;
-        for (int e = 0; e < 5; ++e) {
-            const int row = t[k].by + e;
-            if (row >= 0 && row < ny && t[k].wy[e] != 0.0) {
-                const int pos = atomicAdd(&lc[row], 1);
-                ent[ptr[row] + pos] = k;
+    for (int k0 = threadIdx.x; k0 < nt; k0 += blockDim.x * BIN_BATCH) {
+        int row0[BIN_BATCH];
+        unsigned msk[BIN_BATCH];
+        bin_rows(t, nt, gk, k0, row0, msk);
+#pragma unroll
+        for (int u = 0; u < BIN_BATCH; ++u)
+            for (int e = 0; e < 5; ++e) {
+                const int row = row0[u] + e;
+                if ((msk[u] >> e & 1u) && row >= 0 && row < ny) {
+                    const int pos = atomicAdd(&lc[row], 1);
+                    ent[ptr[row] + pos] = k0 + u * (int)blockDim.x;
+                }
             }
-        }
     }
 }
 
