@@ -349,11 +349,6 @@ __device__ __forceinline__ void k_bin_count_body(const Term* __restrict__ t, int
     for (int r = threadIdx.x; r < ny; r += blockDim.x) cnt[r] = lc[r];
 }
 
-__global__ void __launch_bounds__(1024) k_bin_count(const Term* __restrict__ t, int nt, int gk, int ny,
-                                                    int* __restrict__ cnt)
-{
-    k_bin_count_body(t, nt, gk, ny, cnt, blockIdx.x);
-}
 
 // exclusive scan of cnt[0..n) into ptr[0..n], single work-group of 1024
 // threads, 4096 values per round (4 per thread, wave shuffles, one LDS step).
@@ -426,10 +421,52 @@ __device__ __forceinline__ void k_bin_fill_body(const Term* __restrict__ t, int 
     }
 }
 
-__global__ void __launch_bounds__(1024) k_bin_fill(const Term* __restrict__ t, int nt, int gk, int ny,
-                                                   const int* __restrict__ ptr, int* __restrict__ ent)
+
+// The three steps of a row binning in one work-group of 1024: the entries per
+// row counted in LDS, their exclusive scan (k_scan_body, LDS to LDS) copied to
+// ptr[0..ny], and the entries placed with LDS cursors (in arbitrary order
+// within a row; k_bin_sort orders them).  Dynamic LDS: 2 ny + 1 ints.  Two
+// launches fewer than counting, scanning and filling apart (as the batched
+// k_geo_bin_count, k_geo_scan and k_geo_bin_fill still do), with the same
+// ptr and the same entries per row.
+__global__ void __launch_bounds__(1024) k_bin_build(const Term* __restrict__ t, int nt, int gk, int ny,
+                                                    int* __restrict__ ptr, int* __restrict__ ent)
 {
-    k_bin_fill_body(t, nt, gk, ny, ptr, ent, blockIdx.x);
+    extern __shared__ int lc[];   // ny counts (then cursors), then ny + 1 offsets
+    int* lp = lc + ny;
+    for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
+    __syncthreads();
+    for (int k0 = threadIdx.x; k0 < nt; k0 += blockDim.x * BIN_BATCH) {
+        int row0[BIN_BATCH];
+        unsigned msk[BIN_BATCH];
+        bin_rows(t, nt, gk, k0, row0, msk);
+#pragma unroll
+        for (int u = 0; u < BIN_BATCH; ++u)
+            for (int e = 0; e < 5; ++e) {
+                const int row = row0[u] + e;
+                if ((msk[u] >> e & 1u) && row >= 0 && row < ny) atomicAdd(&lc[row], 1);
+            }
+    }
+    __syncthreads();
+    k_scan_body(lc, ny, lp, nullptr, 0);
+    __syncthreads();
+    for (int r = threadIdx.x; r <= ny; r += blockDim.x) ptr[r] = lp[r];
+    for (int r = threadIdx.x; r < ny; r += blockDim.x) lc[r] = 0;
+    __syncthreads();
+    for (int k0 = threadIdx.x; k0 < nt; k0 += blockDim.x * BIN_BATCH) {
+        int row0[BIN_BATCH];
+        unsigned msk[BIN_BATCH];
+        bin_rows(t, nt, gk, k0, row0, msk);
+#pragma unroll
+        for (int u = 0; u < BIN_BATCH; ++u)
+            for (int e = 0; e < 5; ++e) {
+                const int row = row0[u] + e;
+                if ((msk[u] >> e & 1u) && row >= 0 && row < ny) {
+                    const int pos = atomicAdd(&lc[row], 1);
+                    ent[lp[row] + pos] = k0 + u * (int)blockDim.x;
+                }
+            }
+    }
 }
 
 // Sort each row's entries by (bx, term index): the order in which a cell sums
@@ -772,10 +809,8 @@ void build_bins(ilh_plan* pl, const Term* terms, int nterms, int gk, RowBins& b)
     int* ebx = b.entries + E;
     const size_t lc_bytes = sizeof(int) * (size_t)b.ny;
     // entries per row, sorted by (bx, term)
-    k_bin_count<<<1, 1024, lc_bytes, st>>>(terms, nterms, gk, b.ny, cnt);
-    k_scan<<<1, 1024, 0, st>>>(cnt, b.ny, b.row_ptr, nullptr);
+    k_bin_build<<<1, 1024, 2 * lc_bytes + sizeof(int), st>>>(terms, nterms, gk, b.ny, b.row_ptr, b.entries);
     if (nterms > 0) {
-        k_bin_fill<<<1, 1024, lc_bytes, st>>>(terms, nterms, gk, b.ny, b.row_ptr, b.entries);
         k_bin_sort<<<b.ny, 256, 0, st>>>(terms, b.row_ptr, b.entries);
         k_bin_bx<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(terms, b.entries, b.row_ptr + b.ny, ebx);
         k_bin_rec<<<b.ny, 64, 0, st>>>(terms, b.row_ptr, b.entries, b.eai, b.erw);
