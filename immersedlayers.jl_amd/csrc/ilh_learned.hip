@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(256) k_bgemm_w(int M, int N, int K, int kchunk
     const int m0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int kbeg = blockIdx.z * kchunk, KL = min(K, kbeg + kchunk) - kbeg, K32 = (KL + BKC - 1) / BKC * BKC;
+    // this split's K slice (empty past the end: the partial sums are then zeros)
+    const int kbeg = blockIdx.z * kchunk, KL = max(0, min(K, kbeg + kchunk) - kbeg), K32 = (KL + BKC - 1) / BKC * BKC;
     uint4 xa[8], xb[8];
     wide_load<TA == 0 ? 0 : 1>(A, lda, M, m0, kbeg, KL, tid, xa);
     wide_load<TB == 1 ? 0 : 1>(B, ldb, N, n0, kbeg, KL, tid, xb);
