@@ -2467,9 +2467,11 @@ __global__ void __launch_bounds__(256) k_bgm_matvec(const BGm* __restrict__ g, c
 //                 (slice-local: they need only this slice of w1)
 //   k_arn_final   the second pass (when the test asks for it): h' = its dots
 //                 summed, w2 = w1 - V h'; |w2| = sqrt(|w1|^2 - |h'|^2)
-//                 (Pythagoras: V orthonormal, and the second pass only runs
-//                 where h' is a rounding-level correction, so the difference
-//                 loses nothing); v_{j+1} = w2 / |w2| on the slice; slice 0
+//                 (Pythagoras, V orthonormal: h' is of the order of the first
+//                 pass's rounding, eps |w|, so the difference is exact to
+//                 working precision unless w lies in span V to that
+//                 precision, where the clamp at 0 ends the iteration as a
+//                 breakdown); v_{j+1} = w2 / |w2| on the slice; slice 0
 //                 finishes column j of H
 // Partial sums go to `part` (ArnLayout) and every block that needs a total
 // adds them in slice order, so that all of them see the same value; every sum
