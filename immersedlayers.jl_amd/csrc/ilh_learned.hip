@@ -398,12 +398,15 @@ __global__ void k_lp_features(int N, int W, Grid g, const double* __restrict__ X
                               const double* __restrict__ ds, const int* __restrict__ nbr, float* __restrict__ F,
                               int ldf)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N) return;
+    // one thread per (point, neighbour): the neighbour's six features (four
+    // FP64 sines and cosines), and the point's own three for neighbour 0
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * W) return;
+    const int p = (int)(e / W), j = (int)(e % W);
     const double dx = g.dx, pi2 = 6.283185307179586476925286766559;
     const double xp = X[p] / dx, yp = Y[p] / dx, nx = nxv[p], ny = nyv[p];
     float* f = F + (size_t)p * ldf;
-    for (int j = 0; j < W; ++j) {
+    {
         const int q = nbr[(size_t)p * W + j];
         const double xq = X[q] / dx, yq = Y[q] / dx;
         const double ddx = xq - xp, ddy = yq - yp;
@@ -417,9 +420,11 @@ __global__ void k_lp_features(int N, int W, Grid g, const double* __restrict__ X
         f[4 * W + j] = (float)sin(pi2 * fy);
         f[5 * W + j] = (float)cos(pi2 * fy);
     }
-    f[6 * W] = (float)nx;
-    f[6 * W + 1] = (float)ny;
-    f[6 * W + 2] = (float)(ds[p] / dx);
+    if (j == 0) {
+        f[6 * W] = (float)nx;
+        f[6 * W + 1] = (float)ny;
+        f[6 * W + 2] = (float)(ds[p] / dx);
+    }
 }
 
 // column mean and standard deviation (FP64 sums, one work-group per column)
@@ -686,7 +691,7 @@ void lpc_features(ilh_lpc* m, ilh_plan* pl)
     plan_neighbours(pl, m->K);
     const int N = pl->N;
     m->feat.ensure(sizeof(float) * (size_t)std::max(N, 1) * m->F);
-    k_lp_features<<<(N + 255) / 256, 256, 0, pl->ctx->stream>>>(N, m->W, pl->g, pl->X.as<double>(), pl->Y.as<double>(),
+    k_lp_features<<<(unsigned)(((size_t)N * m->W + 255) / 256), 256, 0, pl->ctx->stream>>>(N, m->W, pl->g, pl->X.as<double>(), pl->Y.as<double>(),
                                                                 pl->nx.as<double>(), pl->ny.as<double>(),
                                                                 pl->ds.as<double>(), pl->nbr.as<int>(),
                                                                 m->feat.as<float>(), m->F);
