@@ -159,6 +159,63 @@ __global__ void __launch_bounds__(256) k_heat_y(const double* __restrict__ in, i
     }
 }
 
+// k_heat_y for filters of at most MB taps a side: the same staging, then each
+// thread takes 16 consecutive rows of its column (thread (tx, ty): rows
+// 16 ty .. 16 ty + 15) from a register window of 16 + 2 MB strip values, so
+// the strip is read 16 + 2 MB times per thread instead of 16 (2M + 1).  Every
+// output sums K[0] col[0] and then K[p] (col[-p] + col[p]) for p = 1, 2, ...
+// in k_heat_y's order; the taps past M are zero and add nothing.
+constexpr int YMB = 16;
+
+template <int MB>
+__global__ void __launch_bounds__(256) k_heat_y_w(const double* __restrict__ in, int nx, int ny,
+                                                  const double* __restrict__ K, int M, double c,
+                                                  const double* __restrict__ add, double* __restrict__ out)
+{
+    extern __shared__ double strip[];   // [YR + 2M][YC]
+    constexpr int QMAX = (YR + 2 * MB) / 4, NO = YR / 4;
+    const int tx = threadIdx.x & (YC - 1), ty = threadIdx.x / YC;   // 64 x 4
+    const int i = blockIdx.x * YC + tx, y0 = blockIdx.y * YR;
+    const int rows = YR + 2 * M;
+    double v[QMAX], ad[NO];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        const int r = ty + 4 * q, y = y0 - M + r;
+        v[q] = (r < rows && i < nx && y >= 0 && y < ny) ? in[i + (size_t)nx * y] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+        const int y = y0 + NO * ty + q;
+        ad[q] = (add && i < nx && y < ny) ? add[i + (size_t)nx * y] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        const int r = ty + 4 * q;
+        if (r < rows) strip[r * YC + tx] = v[q];
+    }
+    __syncthreads();
+    if (i >= nx) return;
+    // window entry k is strip row NO ty + k - (MB - M): output q's centre is
+    // entry q + MB
+    double w[NO + 2 * MB], kk[MB + 1];
+#pragma unroll
+    for (int k = 0; k < NO + 2 * MB; ++k) {
+        const int r = NO * ty + k - (MB - M);
+        w[k] = r >= 0 && r < rows ? strip[r * YC + tx] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p <= MB; ++p) kk[p] = p <= M ? K[p] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+        const int y = y0 + NO * ty + q;
+        if (y >= ny) break;
+        double acc = kk[0] * w[q + MB];
+#pragma unroll
+        for (int p = 1; p <= MB; ++p) acc += kk[p] * (w[q + MB - p] + w[q + MB + p]);
+        out[i + (size_t)nx * y] = add ? acc + c * ad[q] : acc;
+    }
+}
+
 // the same passes for filters longer than the tiled ones take
 __global__ void k_heat_x_long(const double* __restrict__ a, double c, const double* __restrict__ b, int nx, int ny,
                               const double* __restrict__ K, int M, double* __restrict__ out)
@@ -217,7 +274,10 @@ static void heat_apply(ilh_plan* pl, const double* K, int M, const double* a, do
     else
         k_heat_x_long<<<dim3((nx + HX - 1) / HX, (ny + HY - 1) / HY), dim3(HX, HY), 0, st>>>(a, ca, b, nx, ny, K, M,
                                                                                           tmp);
-    if (M <= MY_LDS)
+    if (M <= YMB)
+        k_heat_y_w<YMB><<<dim3((nx + YC - 1) / YC, (ny + YR - 1) / YR), 256, sizeof(double) * (YR + 2 * M) * YC,
+                           st>>>(tmp, nx, ny, K, M, cp, add, out);
+    else if (M <= MY_LDS)
         k_heat_y<<<dim3((nx + YC - 1) / YC, (ny + YR - 1) / YR), 256, sizeof(double) * (YR + 2 * M) * YC, st>>>(
             tmp, nx, ny, K, M, cp, add, out);
     else
